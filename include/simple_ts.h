@@ -1,0 +1,171 @@
+/*
+ * simple_ts.h -- C ABI of the MI355X SIMPLE-TS time-step library
+ * (libsimple_ts_hip.so, HIP/CDNA4, gfx950).
+ *
+ * The reference (kshterev/SIMPLE-TS) has no operator or FFI API: its time
+ * step is inline code in main() of SIMPLE_TS_MPI.cpp plus void(void) helpers
+ * on file-scope globals.  Each entry point below replaces one of those call
+ * sites; the citation "S:n" is the line in SIMPLE_TS_MPI.cpp.  Arguments are
+ * plain pointers and sizes; device memory is owned by the context.
+ *
+ * Layout of every field: double[Nx*Ny], node = i + j*Nx (i fastest), the
+ * reference's own layout (S:140-153, WriteReadData.h:107-134).
+ *
+ * Error convention: every function returns STS_OK (0) or a negative code and
+ * records a message readable with sts_last_error().  The reference has no
+ * error codes (I/O failures print and continue, WriteReadData.h:64-67) and
+ * silently ends loop 2 on a NaN residual (S:4433-4437); this library fails
+ * loudly instead: any non-finite residual returns STS_ERR_NONFINITE.
+ *
+ * Threading: one context per host thread; all work of a context is issued on
+ * the context's own HIP stream, in order.
+ */
+#ifndef SIMPLE_TS_H
+#define SIMPLE_TS_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define STS_OK 0
+#define STS_ERR_ARG (-1)
+#define STS_ERR_HIP (-2)
+#define STS_ERR_NONFINITE (-3)
+#define STS_ERR_UNSUPPORTED (-4)
+#define STS_ERR_IO (-5)
+
+/* Body parameters (Circles_Objects.h:38-78, Polyhedrons_Objects.h:32-100).
+   Body k (0-based) is the one whose cells hold vol_inf_fb == k+1: circles
+   first, then polyhedrons (S:6306-6466). */
+typedef struct sts_body {
+  double u_body, v_body, p_body, T_body;
+  int is_slip;            /* is_VelocitySlipBC_body */
+  double F_slip;          /* F_VelocitySlip */
+  int is_jump;            /* is_TemperatureJumpBC_body */
+  double F_jump;          /* F_TemperatureJump */
+  double dTdn_on_wall_0;
+} sts_body;
+
+/* Everything the time step consumes besides the fields: mesh, body mask,
+   solve boxes (define_area_for_solving, S:6009-6267), scalars (S:2013-2071,
+   S:4347-4361) and boundary-condition switches (EnteredData.txt:62-110). */
+typedef struct sts_grid {
+  int Nx, Ny;
+  const double *hx, *hy;            /* [Nx], [Ny] */
+  const unsigned int *vol_inf_fb;   /* [Nx*Ny]: 0 fluid, k>=1 body k-1 */
+  int nbodies;
+  const sts_body *bodies;
+  int Nx_u[2], Ny_u[2], Nx_v[2], Ny_v[2], Nx_o[2], Ny_o[2];
+  double ht, gamma1, Kn, cT, cT1, cT2, cPch, c_Fr_v;
+  int given_xb, given_xe, kn_local;
+  int dudx_0_xb, durhodx_0_xb, drhodt_xb, dudx_0_xe, durhodx_0_xe, drhodt_xe,
+      u_Min_Mout_xe, dudy_0_yb, dudy_0_ye;
+  int dvdx_0_xb, dvdx_0_xe, dvdy_0_yb, dvdy_0_ye;
+  int dpdx_0_xb, dpdx_0_xe, dpdy_0_yb, dpdy_0_ye;
+  int dTdx_0_xb, dTdx_0_xe, dTdy_0_yb, dTdy_0_ye;
+} sts_grid;
+
+/* Field identifiers, the reference's array names (S:245-1792). */
+enum sts_field {
+  STS_U = 0, STS_V, STS_P, STS_T, STS_RHO, STS_SQRT_T,
+  STS_U_PR, STS_V_PR, STS_P_PR, STS_T_PR, STS_RHO_PR,
+  STS_RHO_U, STS_RHO_V, STS_FX, STS_FY,
+  STS_GAMMA_XF, STS_GAMMA_YF, STS_D_UX, STS_D_UY, STS_D_VX, STS_D_VY, STS_DTX, STS_DTY,
+  STS_U_PSEUDO, STS_V_PSEUDO, STS_D_P_C_12, STS_D_P_C_34, STS_APX, STS_APY, STS_BPX, STS_BPY,
+  STS_SCT1, STS_TW_U, STS_TW_V,
+  STS_NFIELDS
+};
+
+/* Parts of the per-iteration boundary conditions, run in the reference's
+   order S:4460-4527 when several are requested. */
+#define STS_BC_SLIP_U 0x01        /* BoundaryConditions_Velocities_SlipBC_u, S:7098 */
+#define STS_BC_SLIP_V 0x02        /* BoundaryConditions_Velocities_SlipBC_v, S:7215 */
+#define STS_BC_VELOCITIES 0x04    /* BoundaryConditions_Velocities, S:6790 */
+#define STS_BC_JUMP 0x08          /* BoundaryConditions_Temperature_TemperatureJumpBC, S:7744 */
+#define STS_BC_TEMPERATURE 0x10   /* BoundaryConditions_Temperature, S:7650 */
+#define STS_BC_PRESSURE 0x20      /* BoundaryConditions_Pressure, S:7335 */
+#define STS_BC_ALL 0x3f
+
+typedef struct sts_ctx sts_ctx;
+
+/* Last error message of the calling thread ("" if none). */
+const char *sts_last_error(void);
+
+/* Allocates the device state for one case on the current HIP device.  All
+   fields start at zero, as the reference zero-fills them before loop 1
+   (S:2872-2930).  precision: 64 (FP64, reference arithmetic). */
+int sts_create(const sts_grid *g, int precision, sts_ctx **out);
+int sts_destroy(sts_ctx *ctx);
+
+/* Host <-> device copies of one field (double[Nx*Ny]); synchronous. */
+int sts_set_field(sts_ctx *ctx, int field, const double *host);
+int sts_get_field(sts_ctx *ctx, int field, double *host);
+
+/* Per-iteration scalar recomputation of cT and cPch, S:4347-4361. */
+int sts_set_scalars(sts_ctx *ctx, double cT, double cPch);
+
+/* Time-step prologue, S:3233-3239: p_pr, u_pr, v_pr, Temper_pr, rho_pr <- current. */
+int sts_save_previous(sts_ctx *ctx);
+
+/* Coefficient assembly: u pseudo-velocity S:3269-3344 + S:3478-3545, v
+   pseudo-velocity S:3373-3450 + S:3561-3630, energy source S:3655-3756. */
+int sts_assemble(sts_ctx *ctx);
+
+/* Loop 3, S:3763-4131: n fused T/p Jacobi sweeps.  res (may be NULL) gets
+   2*n doubles: max|p-p_old|, max|T-T_old| of each sweep. */
+int sts_pt_jacobi(sts_ctx *ctx, int n, double *res);
+
+/* Velocity correction S:4170-4305, then sqrt_T and rho S:4323-4330, then the
+   face densities S:4368 (Solve_rho_in_middle_points).  res (may be NULL)
+   gets max|du|, max|dv|. */
+int sts_correct(sts_ctx *ctx, double *res);
+
+/* sqrt_T and rho over all cells, S:4323-4330 (also S:2948-2953). */
+int sts_properties(sts_ctx *ctx);
+
+/* Solve_rho_in_middle_points, S:10382-10437. */
+int sts_rho_in_middle_points(sts_ctx *ctx);
+
+/* Boundary conditions of one outer iteration, S:4460-4527; parts: STS_BC_*.
+   p_probe (may be NULL) gets p[Nx_o[0] + j*Nx], j = int((Ny_o[1]-1.0)-Ny_o[0]),
+   the value BoundaryConditions_Pressure_begin_iteration_for_it reads into
+   p_BC_xb when the inlet velocity is given (S:7509-7514). */
+int sts_boundary_conditions(sts_ctx *ctx, int parts, double *p_probe);
+
+/* SolveFluxes S:10440-10454 and SolveDiffisionTerms S:10457-10618. */
+int sts_fluxes_diffusion(sts_ctx *ctx);
+
+/* ApplyBodiesConditionsToVariables, S:6470-6584. */
+int sts_apply_bodies(sts_ctx *ctx);
+
+/* Body-cell temperature overwrite done before every save, S:5566-5567 ->
+   S:8048-8099. */
+int sts_jump_overwrite(sts_ctx *ctx);
+
+/* Inflow velocity u[i < Nx_u[0]] = u_in on fluid cells
+   (BoundaryConditions_function_of_time, S:8102-8144). */
+int sts_set_inflow_u(sts_ctx *ctx, double u_in);
+
+/* Continuity residual S:10621-10646 (SolveContinuityEquation): writes the
+   stationary and full fields into cont_stat / cont (host, double[Nx*Ny], may
+   be NULL) and norms[3] = {max|cont|, sqrt(sum cont^2), sum|cont|} over the
+   cells the reference fills (i < Nx-1, j < Ny-1) that are fluid. */
+int sts_continuity(sts_ctx *ctx, double *cont_stat, double *cont, double *norms);
+
+/* One outer iteration of loop 2, S:3263-4608: assembly, n_pc sweeps,
+   correction, boundary conditions, fluxes and diffusion.  res[4] = max|du|,
+   max|dv| and the last sweep's max|dp|, max|dT| -- the four numbers the
+   reference's loop-2 test reads (S:4433-4437).  res_p0T0 (may be NULL) gets
+   the first sweep's max|dp|, max|dT| (max_residual_in_*_Iter_0).  p_probe as
+   in sts_boundary_conditions. */
+int sts_outer_iteration(sts_ctx *ctx, int n_pc, double *res, double *res_p0T0,
+                        double *p_probe);
+
+/* Wall time of the context's stream work since the last call, in ms, and
+   the number of kernels launched (diagnostics). */
+int sts_stream_info(sts_ctx *ctx, void **hip_stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,255 @@
+"""simple-ts_amd: MI355X-native SIMPLE-TS time step (host-side Python mirror).
+
+The product is the C ABI in include/simple_ts.h, implemented by the HIP
+library libsimple_ts_hip.so in this directory, and the host driver
+``sts_gpu`` (C++) that mirrors the reference's main() loops.  This module
+binds the same C ABI with ctypes for tests and benchmarks; it never falls
+back to a CPU path: loading fails loudly if the HIP library is missing.
+
+The reference (kshterev/SIMPLE-TS) exposes no API of its own; the methods of
+:class:`Context` carry the names of the reference's call sites in
+SIMPLE_TS_MPI.cpp that they replace (cited as S:line).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_HIP = os.path.join(HERE, "libsimple_ts_hip.so")
+LIB_HOST = os.path.join(HERE, "libsimple_ts_host.so")
+DRIVER = os.path.join(HERE, "sts_gpu")
+
+STS_OK = 0
+ERRORS = {-1: "STS_ERR_ARG", -2: "STS_ERR_HIP", -3: "STS_ERR_NONFINITE", -4: "STS_ERR_UNSUPPORTED",
+          -5: "STS_ERR_IO"}
+
+# enum sts_field (include/simple_ts.h), the reference's array names
+FIELDS = ["u", "v", "p", "Temper", "rho", "sqrt_T", "u_pr", "v_pr", "p_pr", "Temper_pr", "rho_pr",
+          "rho_u", "rho_v", "Fx", "Fy", "Gamma_xf", "Gamma_yf", "D_ux", "D_uy", "D_vx", "D_vy", "DTx", "DTy",
+          "u_pseudo", "v_pseudo", "d_p_c_12", "d_p_c_34", "apx", "apy", "bpx", "bpy", "ScT1",
+          "Temper_of_fluid_on_the_wall_u", "Temper_of_fluid_on_the_wall_v"]
+FIELD_ID = {n: k for k, n in enumerate(FIELDS)}
+
+BC_SLIP_U, BC_SLIP_V, BC_VELOCITIES, BC_JUMP, BC_TEMPERATURE, BC_PRESSURE = 1, 2, 4, 8, 16, 32
+BC_ALL = 63
+
+# edge boundary-condition switches in sts_grid order (EnteredData.txt:81-109)
+BC_FLAGS = ["dudx_0_xb", "durhodx_0_xb", "drhodt_xb", "dudx_0_xe", "durhodx_0_xe", "drhodt_xe",
+            "u_Min_Mout_xe", "dudy_0_yb", "dudy_0_ye", "dvdx_0_xb", "dvdx_0_xe", "dvdy_0_yb", "dvdy_0_ye",
+            "dpdx_0_xb", "dpdx_0_xe", "dpdy_0_yb", "dpdy_0_ye", "dTdx_0_xb", "dTdx_0_xe", "dTdy_0_yb",
+            "dTdy_0_ye"]
+
+
+class Body(ctypes.Structure):
+    _fields_ = [("u_body", ctypes.c_double), ("v_body", ctypes.c_double), ("p_body", ctypes.c_double),
+                ("T_body", ctypes.c_double), ("is_slip", ctypes.c_int), ("F_slip", ctypes.c_double),
+                ("is_jump", ctypes.c_int), ("F_jump", ctypes.c_double), ("dTdn_on_wall_0", ctypes.c_double)]
+
+
+_I2 = ctypes.c_int * 2
+_DP = ctypes.POINTER(ctypes.c_double)
+_UP = ctypes.POINTER(ctypes.c_uint)
+
+
+def grid_fields(with_mask):
+    """ctypes field list of sts_grid (and of the oracle's ora_grid, which
+    carries the boolean mask pointer after vol_inf_fb)."""
+    f = [("Nx", ctypes.c_int), ("Ny", ctypes.c_int), ("hx", _DP), ("hy", _DP), ("vol_inf_fb", _UP)]
+    if with_mask:
+        f.append(("m", ctypes.POINTER(ctypes.c_ubyte)))
+    f += [("nbodies", ctypes.c_int), ("bodies", ctypes.POINTER(Body))]
+    f += [(n, _I2) for n in ("Nx_u", "Ny_u", "Nx_v", "Ny_v", "Nx_o", "Ny_o")]
+    f += [(n, ctypes.c_double) for n in ("ht", "gamma1", "Kn", "cT", "cT1", "cT2", "cPch", "c_Fr_v")]
+    f += [(n, ctypes.c_int) for n in ("given_xb", "given_xe", "kn_local")]
+    f += [(n, ctypes.c_int) for n in BC_FLAGS]
+    return f
+
+
+class Grid(ctypes.Structure):
+    _fields_ = grid_fields(False)
+
+
+class Problem:
+    """A SIMPLE-TS case as the time step sees it: mesh, body mask and table,
+    solve boxes, scalars and boundary-condition switches.  Keeps the numpy
+    arrays alive while ctypes structures point into them."""
+
+    SCALARS = ("ht", "gamma1", "Kn", "cT", "cT1", "cT2", "cPch", "c_Fr_v")
+
+    def __init__(self, hx, hy, vol_inf_fb, bodies, boxes, scalars, given_xb=0, given_xe=0, kn_local=0,
+                 bc=None):
+        self.hx = np.ascontiguousarray(hx, dtype=np.float64)
+        self.hy = np.ascontiguousarray(hy, dtype=np.float64)
+        self.Nx, self.Ny = self.hx.size, self.hy.size
+        self.vol = np.ascontiguousarray(vol_inf_fb, dtype=np.uint32).reshape(-1)
+        if self.vol.size != self.Nx * self.Ny:
+            raise ValueError("vol_inf_fb must have Nx*Ny entries")
+        self.mask = (self.vol == 0).astype(np.uint8)
+        self.bodies = list(bodies)
+        self.boxes = {k: tuple(int(x) for x in v) for k, v in boxes.items()}
+        self.scalars = dict(scalars)
+        self.given_xb, self.given_xe, self.kn_local = int(given_xb), int(given_xe), int(kn_local)
+        self.bc = {k: 0 for k in BC_FLAGS}
+        if bc:
+            for k, v in bc.items():
+                if k not in self.bc:
+                    raise KeyError(k)
+                self.bc[k] = int(v)
+        self._body_arr = (Body * max(1, len(self.bodies)))()
+        for k, b in enumerate(self.bodies):
+            self._body_arr[k] = Body(**b) if isinstance(b, dict) else b
+
+    @property
+    def Na(self):
+        return self.Nx * self.Ny
+
+    def fill(self, st):
+        """Fill a ctypes sts_grid / ora_grid structure."""
+        st.Nx, st.Ny = self.Nx, self.Ny
+        st.hx = self.hx.ctypes.data_as(_DP)
+        st.hy = self.hy.ctypes.data_as(_DP)
+        st.vol_inf_fb = self.vol.ctypes.data_as(_UP)
+        if hasattr(st, "m"):
+            st.m = self.mask.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte))
+        st.nbodies = len(self.bodies)
+        st.bodies = ctypes.cast(self._body_arr, ctypes.POINTER(Body))
+        for k in ("Nx_u", "Ny_u", "Nx_v", "Ny_v", "Nx_o", "Ny_o"):
+            setattr(st, k, _I2(*self.boxes[k]))
+        for k in self.SCALARS:
+            setattr(st, k, float(self.scalars[k]))
+        st.given_xb, st.given_xe, st.kn_local = self.given_xb, self.given_xe, self.kn_local
+        for k, v in self.bc.items():
+            setattr(st, k, v)
+        return st
+
+
+_lib = None
+
+
+def lib():
+    """The HIP library.  Raises if it is not built: there is no CPU path."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_HIP):
+            raise RuntimeError(f"{LIB_HIP} is not built; run __graft_entry__.build() (no CPU fallback exists)")
+        L = ctypes.CDLL(LIB_HIP)
+        c_int, c_double, vp = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
+        L.sts_last_error.restype = ctypes.c_char_p
+        L.sts_create.argtypes = [ctypes.POINTER(Grid), c_int, ctypes.POINTER(vp)]
+        for name, args in [
+            ("sts_destroy", [vp]), ("sts_set_field", [vp, c_int, _DP]), ("sts_get_field", [vp, c_int, _DP]),
+            ("sts_set_scalars", [vp, c_double, c_double]), ("sts_save_previous", [vp]), ("sts_assemble", [vp]),
+            ("sts_pt_jacobi", [vp, c_int, _DP]), ("sts_correct", [vp, _DP]), ("sts_properties", [vp]),
+            ("sts_rho_in_middle_points", [vp]), ("sts_boundary_conditions", [vp, c_int, _DP]),
+            ("sts_fluxes_diffusion", [vp]), ("sts_apply_bodies", [vp]), ("sts_jump_overwrite", [vp]),
+            ("sts_set_inflow_u", [vp, c_double]), ("sts_continuity", [vp, _DP, _DP, _DP]),
+            ("sts_outer_iteration", [vp, c_int, _DP, _DP, _DP]), ("sts_stream_info", [vp, ctypes.POINTER(vp)]),
+        ]:
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = c_int
+        _lib = L
+    return _lib
+
+
+class StsError(RuntimeError):
+    pass
+
+
+def _check(rc, what):
+    if rc != STS_OK:
+        msg = lib().sts_last_error().decode(errors="replace")
+        raise StsError(f"{what}: {ERRORS.get(rc, rc)}: {msg}")
+
+
+def _dp(a):
+    return a.ctypes.data_as(_DP)
+
+
+class Context:
+    """Device state of one case (sts_ctx)."""
+
+    def __init__(self, problem, precision=64):
+        self.problem = problem
+        self._grid = problem.fill(Grid())
+        h = ctypes.c_void_p()
+        _check(lib().sts_create(ctypes.byref(self._grid), precision, ctypes.byref(h)), "sts_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().sts_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # fields -------------------------------------------------------------
+    def set(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(-1)
+        if a.size != self.problem.Na:
+            raise ValueError(f"{name}: expected {self.problem.Na} values")
+        _check(lib().sts_set_field(self.h, FIELD_ID[name], _dp(a)), f"sts_set_field({name})")
+
+    def get(self, name):
+        a = np.empty(self.problem.Na, dtype=np.float64)
+        _check(lib().sts_get_field(self.h, FIELD_ID[name], _dp(a)), f"sts_get_field({name})")
+        return a
+
+    def set_scalars(self, cT, cPch):
+        _check(lib().sts_set_scalars(self.h, cT, cPch), "sts_set_scalars")
+
+    # stages ---------------------------------------------------------------
+    def save_previous(self):  # S:3233-3239
+        _check(lib().sts_save_previous(self.h), "sts_save_previous")
+
+    def assemble(self):  # S:3269-3756
+        _check(lib().sts_assemble(self.h), "sts_assemble")
+
+    def pt_jacobi(self, n):  # S:3763-4131
+        r = np.zeros(2 * n)
+        _check(lib().sts_pt_jacobi(self.h, n, _dp(r)), "sts_pt_jacobi")
+        return r.reshape(n, 2)
+
+    def correct(self):  # S:4170-4330, S:4368
+        r = np.zeros(2)
+        _check(lib().sts_correct(self.h, _dp(r)), "sts_correct")
+        return r
+
+    def properties(self):  # S:4323-4330
+        _check(lib().sts_properties(self.h), "sts_properties")
+
+    def rho_in_middle_points(self):  # S:10382
+        _check(lib().sts_rho_in_middle_points(self.h), "sts_rho_in_middle_points")
+
+    def boundary_conditions(self, parts=BC_ALL):  # S:4460-4527
+        pr = np.zeros(1)
+        _check(lib().sts_boundary_conditions(self.h, parts, _dp(pr)), "sts_boundary_conditions")
+        return float(pr[0])
+
+    def fluxes_diffusion(self):  # S:4531-4532
+        _check(lib().sts_fluxes_diffusion(self.h), "sts_fluxes_diffusion")
+
+    def apply_bodies(self):  # S:6470
+        _check(lib().sts_apply_bodies(self.h), "sts_apply_bodies")
+
+    def jump_overwrite(self):  # S:5566-5567
+        _check(lib().sts_jump_overwrite(self.h), "sts_jump_overwrite")
+
+    def set_inflow_u(self, u_in):  # S:8102-8144
+        _check(lib().sts_set_inflow_u(self.h, u_in), "sts_set_inflow_u")
+
+    def continuity(self):  # S:10621-10646
+        n = self.problem.Na
+        cs, cf, nr = np.zeros(n), np.zeros(n), np.zeros(3)
+        _check(lib().sts_continuity(self.h, _dp(cs), _dp(cf), _dp(nr)), "sts_continuity")
+        return cs, cf, nr
+
+    def outer_iteration(self, n_pc):  # S:3263-4608
+        r, r0, pr = np.zeros(4), np.zeros(2), np.zeros(1)
+        _check(lib().sts_outer_iteration(self.h, n_pc, _dp(r), _dp(r0), _dp(pr)), "sts_outer_iteration")
+        return r, r0, float(pr[0])

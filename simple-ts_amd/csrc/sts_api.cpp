@@ -1,0 +1,424 @@
+// Host side of the C ABI declared in include/simple_ts.h: device state of one
+// case and the entry points that replace the reference's call sites in
+// main() (SIMPLE_TS_MPI.cpp).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "simple_ts.h"
+#include "sts_device.h"
+#include "sts_launch.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_OK(expr)                                                                    \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail(STS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));      \
+  } while (0)
+
+constexpr int MAX_SWEEPS = 62;
+constexpr int NRED = 2 + 2 * MAX_SWEEPS;
+
+}  // namespace
+
+struct sts_ctx {
+  int Nx = 0, Ny = 0, Na = 0;
+  hipStream_t stream = nullptr;
+  KGrid g{};
+  double *hx = nullptr, *hy = nullptr;
+  uint8_t *m = nullptr;
+  uint32_t *vol = nullptr;
+  DevBody *bodies = nullptr;
+  double *f[STS_NFIELDS] = {};
+  double *p_alt = nullptr, *T_alt = nullptr;  // loop-3 ping-pong partners of p, Temper
+  unsigned long long *red = nullptr;          // residual maxima as IEEE bit patterns
+  double *probe = nullptr;
+  unsigned long long *h_red = nullptr;        // pinned host mirror of red (+ probe)
+  double *partial = nullptr;                  // continuity norms, one triple per block
+  double *cont_stat = nullptr, *cont = nullptr;
+
+  KFields fields() const {
+    KFields k;
+    for (int q = 0; q < STS_NFIELDS; q++) k.f[q] = f[q];
+    return k;
+  }
+};
+
+extern "C" const char *sts_last_error(void) { return g_err.c_str(); }
+
+static unsigned bc_bits(const sts_grid *g) {
+  unsigned b = 0;
+  b |= g->dudx_0_xb ? BC_DUDX_XB : 0u;
+  b |= g->durhodx_0_xb ? BC_DURHODX_XB : 0u;
+  b |= g->drhodt_xb ? BC_DRHODT_XB : 0u;
+  b |= g->dudx_0_xe ? BC_DUDX_XE : 0u;
+  b |= g->durhodx_0_xe ? BC_DURHODX_XE : 0u;
+  b |= g->drhodt_xe ? BC_DRHODT_XE : 0u;
+  b |= g->u_Min_Mout_xe ? BC_UMINMOUT_XE : 0u;
+  b |= g->dudy_0_yb ? BC_DUDY_YB : 0u;
+  b |= g->dudy_0_ye ? BC_DUDY_YE : 0u;
+  b |= g->dvdx_0_xb ? BC_DVDX_XB : 0u;
+  b |= g->dvdx_0_xe ? BC_DVDX_XE : 0u;
+  b |= g->dvdy_0_yb ? BC_DVDY_YB : 0u;
+  b |= g->dvdy_0_ye ? BC_DVDY_YE : 0u;
+  b |= g->dpdx_0_xb ? BC_DPDX_XB : 0u;
+  b |= g->dpdx_0_xe ? BC_DPDX_XE : 0u;
+  b |= g->dpdy_0_yb ? BC_DPDY_YB : 0u;
+  b |= g->dpdy_0_ye ? BC_DPDY_YE : 0u;
+  b |= g->dTdx_0_xb ? BC_DTDX_XB : 0u;
+  b |= g->dTdx_0_xe ? BC_DTDX_XE : 0u;
+  b |= g->dTdy_0_yb ? BC_DTDY_YB : 0u;
+  b |= g->dTdy_0_ye ? BC_DTDY_YE : 0u;
+  return b;
+}
+
+// Solve boxes must leave the first and last column/row out, as every
+// non-periodic box of define_area_for_solving (S:6009-6267) does: the
+// stencils then stay inside the grid with the reference's clamped indices.
+static int check_box(const char *name, const int *b, int N) {
+  if (b[0] < 1 || b[1] > N - 1 || b[0] >= b[1])
+    return fail(STS_ERR_ARG, std::string("solve box ") + name + " must satisfy 1 <= lo < hi <= N-1");
+  return STS_OK;
+}
+
+extern "C" int sts_destroy(sts_ctx *c) {
+  if (!c) return STS_OK;
+  for (int q = 0; q < STS_NFIELDS; q++) (void)hipFree(c->f[q]);
+  (void)hipFree(c->p_alt);
+  (void)hipFree(c->T_alt);
+  (void)hipFree(c->hx);
+  (void)hipFree(c->hy);
+  (void)hipFree(c->m);
+  (void)hipFree(c->vol);
+  (void)hipFree(c->bodies);
+  (void)hipFree(c->red);
+  (void)hipFree(c->probe);
+  (void)hipFree(c->partial);
+  (void)hipFree(c->cont_stat);
+  (void)hipFree(c->cont);
+  if (c->h_red) (void)hipHostFree(c->h_red);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return STS_OK;
+}
+
+static int alloc_state(sts_ctx *c, const sts_grid *gi, const std::vector<uint8_t> &mask) {
+  const size_t Na = (size_t)c->Na;
+  const size_t bytes = Na * sizeof(double);
+  HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  for (int q = 0; q < STS_NFIELDS; q++) {
+    HIP_OK(hipMalloc(&c->f[q], bytes));
+    HIP_OK(hipMemsetAsync(c->f[q], 0, bytes, c->stream));
+  }
+  HIP_OK(hipMalloc(&c->p_alt, bytes));
+  HIP_OK(hipMalloc(&c->T_alt, bytes));
+  HIP_OK(hipMalloc(&c->hx, sizeof(double) * gi->Nx));
+  HIP_OK(hipMalloc(&c->hy, sizeof(double) * gi->Ny));
+  HIP_OK(hipMalloc(&c->m, Na));
+  HIP_OK(hipMalloc(&c->vol, sizeof(uint32_t) * Na));
+  const size_t nb = (size_t)(gi->nbodies > 0 ? gi->nbodies : 1);
+  HIP_OK(hipMalloc(&c->bodies, sizeof(DevBody) * nb));
+  HIP_OK(hipMalloc(&c->red, sizeof(unsigned long long) * NRED));
+  HIP_OK(hipMalloc(&c->probe, sizeof(double)));
+  HIP_OK(hipHostMalloc(&c->h_red, sizeof(unsigned long long) * (NRED + 1), hipHostMallocDefault));
+  HIP_OK(hipMemcpyAsync(c->hx, gi->hx, sizeof(double) * gi->Nx, hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipMemcpyAsync(c->hy, gi->hy, sizeof(double) * gi->Ny, hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipMemcpyAsync(c->m, mask.data(), Na, hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipMemcpyAsync(c->vol, gi->vol_inf_fb, sizeof(uint32_t) * Na, hipMemcpyHostToDevice, c->stream));
+  std::vector<DevBody> db(nb);
+  for (int k = 0; k < gi->nbodies; k++) {
+    const sts_body &b = gi->bodies[k];
+    db[k] = DevBody{b.u_body, b.v_body, b.p_body, b.T_body, b.F_slip, b.F_jump, b.dTdn_on_wall_0, b.is_slip, b.is_jump};
+  }
+  HIP_OK(hipMemcpyAsync(c->bodies, db.data(), sizeof(DevBody) * nb, hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return STS_OK;
+}
+
+extern "C" int sts_create(const sts_grid *gi, int precision, sts_ctx **out) {
+  if (!gi || !out) return fail(STS_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (precision != 64) return fail(STS_ERR_UNSUPPORTED, "only precision 64 is provided by sts_create");
+  if (gi->Nx < 3 || gi->Ny < 3 || !gi->hx || !gi->hy || !gi->vol_inf_fb)
+    return fail(STS_ERR_ARG, "grid needs Nx, Ny >= 3, hx, hy and vol_inf_fb");
+  if (gi->nbodies < 0 || (gi->nbodies > 0 && !gi->bodies)) return fail(STS_ERR_ARG, "bad body table");
+  int rc;
+  if ((rc = check_box("Nx_u", gi->Nx_u, gi->Nx)) || (rc = check_box("Ny_u", gi->Ny_u, gi->Ny)) ||
+      (rc = check_box("Nx_v", gi->Nx_v, gi->Nx)) || (rc = check_box("Ny_v", gi->Ny_v, gi->Ny)) ||
+      (rc = check_box("Nx_o", gi->Nx_o, gi->Nx)) || (rc = check_box("Ny_o", gi->Ny_o, gi->Ny)))
+    return rc;
+  const long long Na = (long long)gi->Nx * gi->Ny;
+  if (Na > (1ll << 31) - 1) return fail(STS_ERR_ARG, "grid too large for 32-bit node indices");
+  std::vector<uint8_t> mask((size_t)Na);
+  for (long long c = 0; c < Na; c++) {
+    uint32_t k = gi->vol_inf_fb[c];
+    if ((long long)k > gi->nbodies) return fail(STS_ERR_ARG, "vol_inf_fb names a body beyond nbodies");
+    mask[(size_t)c] = (k == 0u);
+  }
+  // S:6906-6912, S:6977-6983 read rho_v/v one row up; on a fluid cell of the
+  // top row that is outside the arrays in the reference, so such a case is
+  // refused rather than reproduced.
+  const size_t top = (size_t)(gi->Ny - 1) * (size_t)gi->Nx;
+  if ((gi->drhodt_xb && mask[top + (size_t)(gi->Nx_u[0] - 1)]) ||
+      (gi->drhodt_xe && mask[top + (size_t)(gi->Nx_u[1] - 1)]))
+    return fail(STS_ERR_UNSUPPORTED, "drhodt_durhodx_dvrhody edge condition on a fluid cell of the top row");
+  sts_ctx *c = new sts_ctx();
+  c->Nx = gi->Nx;
+  c->Ny = gi->Ny;
+  c->Na = (int)Na;
+  int arc = alloc_state(c, gi, mask);
+  if (arc) {
+    sts_destroy(c);
+    return arc;
+  }
+  KGrid &g = c->g;
+  g.Nx = gi->Nx;
+  g.Ny = gi->Ny;
+  g.hx = c->hx;
+  g.hy = c->hy;
+  g.m = c->m;
+  g.vol = c->vol;
+  g.bodies = c->bodies;
+  g.nbodies = gi->nbodies;
+  g.Nx_u0 = gi->Nx_u[0]; g.Nx_u1 = gi->Nx_u[1]; g.Ny_u0 = gi->Ny_u[0]; g.Ny_u1 = gi->Ny_u[1];
+  g.Nx_v0 = gi->Nx_v[0]; g.Nx_v1 = gi->Nx_v[1]; g.Ny_v0 = gi->Ny_v[0]; g.Ny_v1 = gi->Ny_v[1];
+  g.Nx_o0 = gi->Nx_o[0]; g.Nx_o1 = gi->Nx_o[1]; g.Ny_o0 = gi->Ny_o[0]; g.Ny_o1 = gi->Ny_o[1];
+  g.ht = gi->ht;
+  g.gamma1 = gi->gamma1;
+  g.Kn = gi->Kn;
+  g.cT = gi->cT;
+  g.cT1 = gi->cT1;
+  g.cT2 = gi->cT2;
+  g.cPch = gi->cPch;
+  g.c_Fr_v = gi->c_Fr_v;
+  g.given_xb = gi->given_xb;
+  g.given_xe = gi->given_xe;
+  g.kn_local = gi->kn_local;
+  g.bc = bc_bits(gi);
+  *out = c;
+  return STS_OK;
+}
+
+extern "C" int sts_set_field(sts_ctx *c, int field, const double *host) {
+  if (!c || !host || field < 0 || field >= STS_NFIELDS) return fail(STS_ERR_ARG, "bad field or pointer");
+  HIP_OK(hipMemcpyAsync(c->f[field], host, sizeof(double) * (size_t)c->Na, hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return STS_OK;
+}
+
+extern "C" int sts_get_field(sts_ctx *c, int field, double *host) {
+  if (!c || !host || field < 0 || field >= STS_NFIELDS) return fail(STS_ERR_ARG, "bad field or pointer");
+  HIP_OK(hipMemcpyAsync(host, c->f[field], sizeof(double) * (size_t)c->Na, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return STS_OK;
+}
+
+extern "C" int sts_set_scalars(sts_ctx *c, double cT, double cPch) {
+  if (!c) return fail(STS_ERR_ARG, "null context");
+  c->g.cT = cT;
+  c->g.cPch = cPch;
+  return STS_OK;
+}
+
+extern "C" int sts_stream_info(sts_ctx *c, void **s) {
+  if (!c || !s) return fail(STS_ERR_ARG, "null argument");
+  *s = (void *)c->stream;
+  return STS_OK;
+}
+
+extern "C" int sts_save_previous(sts_ctx *c) {
+  if (!c) return fail(STS_ERR_ARG, "null context");
+  const double *src[5] = {c->f[STS_P], c->f[STS_U], c->f[STS_V], c->f[STS_T], c->f[STS_RHO]};
+  double *dst[5] = {c->f[STS_P_PR], c->f[STS_U_PR], c->f[STS_V_PR], c->f[STS_T_PR], c->f[STS_RHO_PR]};
+  HIP_OK(launch_copy5(src, dst, c->Na, c->stream));
+  return STS_OK;
+}
+
+extern "C" int sts_assemble(sts_ctx *c) {
+  if (!c) return fail(STS_ERR_ARG, "null context");
+  HIP_OK(launch_assemble(c->g, c->fields(), nullptr, 0, c->stream));
+  return STS_OK;
+}
+
+static double bits_to_double(unsigned long long b) {
+  double d;
+  std::memcpy(&d, &b, sizeof d);
+  return d;
+}
+
+// n sweeps reading slot base 2+2s; pointer identity of p/T follows the parity.
+static int run_sweeps(sts_ctx *c, int n) {
+  for (int s = 0; s < n; s++) {
+    KFields kf = c->fields();
+    HIP_OK(launch_sweep(c->g, kf, c->f[STS_P], c->f[STS_T], c->p_alt, c->T_alt, c->red + 2 + 2 * s,
+                        c->red + 3 + 2 * s, c->stream));
+    std::swap(c->f[STS_P], c->p_alt);
+    std::swap(c->f[STS_T], c->T_alt);
+  }
+  return STS_OK;
+}
+
+static int read_red(sts_ctx *c, int nslots, bool with_probe) {
+  HIP_OK(hipMemcpyAsync(c->h_red, c->red, sizeof(unsigned long long) * nslots, hipMemcpyDeviceToHost, c->stream));
+  if (with_probe)
+    HIP_OK(hipMemcpyAsync(c->h_red + NRED, c->probe, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  for (int q = 0; q < nslots; q++)
+    if (!std::isfinite(bits_to_double(c->h_red[q])))
+      return fail(STS_ERR_NONFINITE, "non-finite residual (slot " + std::to_string(q) + ")");
+  return STS_OK;
+}
+
+extern "C" int sts_pt_jacobi(sts_ctx *c, int n, double *res) {
+  if (!c || n < 1 || n > MAX_SWEEPS) return fail(STS_ERR_ARG, "sweep count must be 1..62");
+  HIP_OK(hipMemsetAsync(c->red, 0, sizeof(unsigned long long) * NRED, c->stream));
+  int rc = run_sweeps(c, n);
+  if (rc) return rc;
+  if ((rc = read_red(c, 2 + 2 * n, false))) return rc;
+  if (res)
+    for (int s = 0; s < n; s++) {
+      res[2 * s] = bits_to_double(c->h_red[2 + 2 * s]);
+      res[2 * s + 1] = bits_to_double(c->h_red[3 + 2 * s]);
+    }
+  return STS_OK;
+}
+
+extern "C" int sts_correct(sts_ctx *c, double *res) {
+  if (!c) return fail(STS_ERR_ARG, "null context");
+  HIP_OK(hipMemsetAsync(c->red, 0, sizeof(unsigned long long) * 2, c->stream));
+  HIP_OK(launch_correct(c->g, c->fields(), c->red, c->red + 1, 1, c->stream));
+  int rc = read_red(c, 2, false);
+  if (rc) return rc;
+  if (res) {
+    res[0] = bits_to_double(c->h_red[0]);
+    res[1] = bits_to_double(c->h_red[1]);
+  }
+  return STS_OK;
+}
+
+extern "C" int sts_properties(sts_ctx *c) {
+  if (!c) return fail(STS_ERR_ARG, "null context");
+  HIP_OK(launch_properties(c->g, c->fields(), c->stream));
+  return STS_OK;
+}
+
+extern "C" int sts_rho_in_middle_points(sts_ctx *c) {
+  if (!c) return fail(STS_ERR_ARG, "null context");
+  HIP_OK(launch_rho_mid(c->g, c->fields(), c->stream));
+  return STS_OK;
+}
+
+static int enqueue_bcs(sts_ctx *c, int parts, bool probe) {
+  const int wall = parts & (STS_BC_SLIP_U | STS_BC_SLIP_V | STS_BC_JUMP);
+  const int edge = parts & (STS_BC_VELOCITIES | STS_BC_TEMPERATURE | STS_BC_PRESSURE);
+  // The jump condition reads only T and rho, which the velocity conditions do
+  // not touch, so it runs with the slip conditions ahead of the edge copies;
+  // the temperature copies still see the wall values' source unchanged.
+  if (wall) HIP_OK(launch_wall(c->g, c->fields(), wall, c->stream));
+  if (edge || probe) HIP_OK(launch_edges(c->g, c->fields(), edge, probe ? c->probe : nullptr, c->stream));
+  return STS_OK;
+}
+
+extern "C" int sts_boundary_conditions(sts_ctx *c, int parts, double *p_probe) {
+  if (!c || (parts & ~STS_BC_ALL)) return fail(STS_ERR_ARG, "bad boundary-condition parts");
+  int rc = enqueue_bcs(c, parts, p_probe != nullptr);
+  if (rc) return rc;
+  if (p_probe) {
+    HIP_OK(hipMemcpyAsync(p_probe, c->probe, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  }
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return STS_OK;
+}
+
+extern "C" int sts_fluxes_diffusion(sts_ctx *c) {
+  if (!c) return fail(STS_ERR_ARG, "null context");
+  HIP_OK(launch_flux_diff(c->g, c->fields(), c->stream));
+  return STS_OK;
+}
+
+extern "C" int sts_apply_bodies(sts_ctx *c) {
+  if (!c) return fail(STS_ERR_ARG, "null context");
+  HIP_OK(launch_apply_bodies(c->g, c->fields(), c->stream));
+  return STS_OK;
+}
+
+extern "C" int sts_jump_overwrite(sts_ctx *c) {
+  if (!c) return fail(STS_ERR_ARG, "null context");
+  HIP_OK(launch_wall(c->g, c->fields(), STS_BC_JUMP, c->stream));
+  HIP_OK(launch_jump_overwrite(c->g, c->fields(), c->stream));
+  return STS_OK;
+}
+
+extern "C" int sts_set_inflow_u(sts_ctx *c, double u_in) {
+  if (!c) return fail(STS_ERR_ARG, "null context");
+  HIP_OK(launch_set_inflow(c->g, c->fields(), u_in, c->stream));
+  return STS_OK;
+}
+
+extern "C" int sts_continuity(sts_ctx *c, double *cont_stat, double *cont, double *norms) {
+  if (!c) return fail(STS_ERR_ARG, "null context");
+  const int nb = continuity_blocks(c->g);
+  if (!c->partial) {
+    HIP_OK(hipMalloc(&c->partial, sizeof(double) * 3 * (size_t)nb));
+    HIP_OK(hipMalloc(&c->cont_stat, sizeof(double) * (size_t)c->Na));
+    HIP_OK(hipMalloc(&c->cont, sizeof(double) * (size_t)c->Na));
+    HIP_OK(hipMemsetAsync(c->cont_stat, 0, sizeof(double) * (size_t)c->Na, c->stream));
+    HIP_OK(hipMemsetAsync(c->cont, 0, sizeof(double) * (size_t)c->Na, c->stream));
+  }
+  HIP_OK(launch_continuity(c->g, c->fields(), c->cont_stat, c->cont, c->partial, c->stream));
+  std::vector<double> part(3 * (size_t)nb);
+  HIP_OK(hipMemcpyAsync(part.data(), c->partial, sizeof(double) * part.size(), hipMemcpyDeviceToHost, c->stream));
+  if (cont_stat)
+    HIP_OK(hipMemcpyAsync(cont_stat, c->cont_stat, sizeof(double) * (size_t)c->Na, hipMemcpyDeviceToHost, c->stream));
+  if (cont) HIP_OK(hipMemcpyAsync(cont, c->cont, sizeof(double) * (size_t)c->Na, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  if (norms) {
+    double mx = 0, s2 = 0, s1 = 0;
+    for (int b = 0; b < nb; b++) {
+      mx = std::fmax(mx, part[3 * b]);
+      s2 += part[3 * b + 1];
+      s1 += part[3 * b + 2];
+    }
+    norms[0] = mx;
+    norms[1] = std::sqrt(s2);
+    norms[2] = s1;
+  }
+  return STS_OK;
+}
+
+extern "C" int sts_outer_iteration(sts_ctx *c, int n_pc, double *res, double *res_p0T0, double *p_probe) {
+  if (!c || n_pc < 1 || n_pc > MAX_SWEEPS) return fail(STS_ERR_ARG, "N_I_p_c must be 1..62");
+  const int nslots = 2 + 2 * n_pc;
+  HIP_OK(launch_assemble(c->g, c->fields(), c->red, nslots, c->stream));
+  int rc = run_sweeps(c, n_pc);
+  if (rc) return rc;
+  HIP_OK(launch_correct(c->g, c->fields(), c->red, c->red + 1, 1, c->stream));
+  if ((rc = enqueue_bcs(c, STS_BC_ALL, true))) return rc;
+  HIP_OK(launch_flux_diff(c->g, c->fields(), c->stream));
+  if ((rc = read_red(c, nslots, true))) return rc;
+  if (res) {
+    res[0] = bits_to_double(c->h_red[0]);
+    res[1] = bits_to_double(c->h_red[1]);
+    res[2] = bits_to_double(c->h_red[2 + 2 * (n_pc - 1)]);
+    res[3] = bits_to_double(c->h_red[3 + 2 * (n_pc - 1)]);
+  }
+  if (res_p0T0) {
+    res_p0T0[0] = bits_to_double(c->h_red[2]);
+    res_p0T0[1] = bits_to_double(c->h_red[3]);
+  }
+  if (p_probe) std::memcpy(p_probe, &c->h_red[NRED], sizeof(double));
+  return STS_OK;
+}

@@ -1,0 +1,78 @@
+// Device-side data structures and per-cell helpers shared by the SIMPLE-TS
+// kernels.  Index semantics follow the reference's ij_function macro
+// (SIMPLE_TS_MPI.cpp:140-153): neighbour indices are clamped at the domain
+// edge, so a cell on the edge sees itself as its missing neighbour.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "simple_ts.h"
+
+struct DevBody {
+  double u_body, v_body, p_body, T_body, F_slip, F_jump, dTdn;
+  int is_slip, is_jump;
+};
+
+// Edge boundary-condition switches, one bit each (EnteredData.txt:81-109).
+enum : unsigned {
+  BC_DUDX_XB = 1u << 0, BC_DURHODX_XB = 1u << 1, BC_DRHODT_XB = 1u << 2,
+  BC_DUDX_XE = 1u << 3, BC_DURHODX_XE = 1u << 4, BC_DRHODT_XE = 1u << 5,
+  BC_UMINMOUT_XE = 1u << 6, BC_DUDY_YB = 1u << 7, BC_DUDY_YE = 1u << 8,
+  BC_DVDX_XB = 1u << 9, BC_DVDX_XE = 1u << 10, BC_DVDY_YB = 1u << 11, BC_DVDY_YE = 1u << 12,
+  BC_DPDX_XB = 1u << 13, BC_DPDX_XE = 1u << 14, BC_DPDY_YB = 1u << 15, BC_DPDY_YE = 1u << 16,
+  BC_DTDX_XB = 1u << 17, BC_DTDX_XE = 1u << 18, BC_DTDY_YB = 1u << 19, BC_DTDY_YE = 1u << 20,
+};
+
+struct KGrid {
+  int Nx, Ny;
+  const double *hx, *hy;
+  const uint8_t *m;       // vol_inf_fb_bool
+  const uint32_t *vol;    // vol_inf_fb
+  const DevBody *bodies;
+  int nbodies;
+  int Nx_u0, Nx_u1, Ny_u0, Ny_u1, Nx_v0, Nx_v1, Ny_v0, Ny_v1, Nx_o0, Nx_o1, Ny_o0, Ny_o1;
+  double ht, gamma1, Kn, cT, cT1, cT2, cPch, c_Fr_v;
+  int given_xb, given_xe, kn_local;
+  unsigned bc;
+};
+
+struct KFields {
+  double *f[STS_NFIELDS];
+};
+
+// Clamped neighbour indices of cell (i, j).
+struct Nbr {
+  unsigned i, j, ij, i_1, i_2, i1, i_1j, i1j, ij_1, ij1, i_1j_1, i_1j1, i1j_1, i1j1, i_2j;
+  __device__ __forceinline__ Nbr(unsigned ii, unsigned jj, unsigned Nx, unsigned Ny) {
+    i = ii;
+    j = jj;
+    ij = ii + jj * Nx;
+    const unsigned l = (0u < ii), r = (ii < Nx - 1u), d = (0u < jj), u = (jj < Ny - 1u);
+    i_1 = ii - l;
+    i_2 = ii - l - (1u < ii);
+    i1 = ii + r;
+    i_1j = ij - l;
+    i1j = ij + r;
+    ij_1 = ij - Nx * d;
+    ij1 = ij + Nx * u;
+    i_1j_1 = ij - l - Nx * d;
+    i_1j1 = ij - l + Nx * u;
+    i1j_1 = ij + r - Nx * d;
+    i1j1 = ij + r + Nx * u;
+    i_2j = i_2 + jj * Nx;
+  }
+};
+
+__device__ __forceinline__ double maxi0(double a) {  // maximum(0, a), S:121
+  return (0.0 > a) ? 0.0 : a;
+}
+__device__ __forceinline__ double upwind_rho(double r0, double r1, double v1) {  // S:130
+  return (v1 == 0) ? (0.5 * (r0 + r1)) : ((v1 > 0) ? r0 : r1);
+}
+__device__ __forceinline__ double Li(double f1, double f2, double h1, double h2) {  // S:133
+  return (f1 * h2 + f2 * h1) / (h1 + h2);
+}
+__device__ __forceinline__ double Hi(double f1, double f2, double h1, double h2) {  // S:136
+  return (h1 + h2) * f1 * f2 / (h1 * f2 + h2 * f1);
+}
+__device__ __forceinline__ double bd(unsigned b) { return (double)b; }

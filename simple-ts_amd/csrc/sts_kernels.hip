@@ -1,0 +1,842 @@
+// FP64 kernels of the SIMPLE-TS time step for CDNA4 (gfx950).
+//
+// One thread per cell (or per face), 64-wide wavefronts along i, so every
+// field access of a wavefront is one contiguous 512-byte row segment of the
+// reference's i-fastest layout.  The update sets of the reference's rule
+// vectors (SIMPLE_TS_MPI.cpp:8581-9061) are evaluated as predicates on the
+// body mask; inside a set the reference's "bc" formula is used, which is
+// bit-identical to its "nch" formula where all masks are 1.  Built with
+// -ffp-contract=off so each a*b+c rounds twice, as the reference's SSE2 build.
+#include "sts_device.h"
+
+#define FLD(name) fl.f[STS_##name]
+
+namespace {
+
+constexpr int BX = 64, BY = 4;
+
+__device__ __forceinline__ bool in_box(int i, int j, int x0, int x1, int y0, int y1) {
+  return i >= x0 && i < x1 && j >= y0 && j < y1;
+}
+
+// Non-negative doubles order like their bit patterns; a NaN (0x7ff8...)
+// compares above +inf and so survives the max and is reported.
+__device__ __forceinline__ unsigned long long absbits(double r) {
+  return (unsigned long long)__double_as_longlong(fabs(r));
+}
+
+__device__ __forceinline__ unsigned long long wave_max(unsigned long long v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    unsigned long long w = __shfl_xor(v, o, 64);
+    v = (w > v) ? w : v;
+  }
+  return v;
+}
+
+// Block-wide max of up to two values, one atomicMax per block and value.
+template <int NV>
+__device__ __forceinline__ void block_max_atomic(unsigned long long (&v)[NV], unsigned long long *dst0,
+                                                 unsigned long long *dst1) {
+  __shared__ unsigned long long sm[NV][16];
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  const int lane = tid & 63, wid = tid >> 6;
+  const int nw = (blockDim.x * blockDim.y + 63) >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    unsigned long long w = wave_max(v[k]);
+    if (lane == 0) sm[k][wid] = w;
+  }
+  __syncthreads();
+  if (tid == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+      unsigned long long b = 0;
+      for (int q = 0; q < nw; q++) b = (sm[k][q] > b) ? sm[k][q] : b;
+      unsigned long long *d = (k == 0) ? dst0 : dst1;
+      if (b) atomicMax(d, b);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// A2 + A3 + A4: coefficient assembly (S:3269-3756)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void cell_assemble_u(const KGrid &g, const KFields &fl, const Nbr &n) {
+  const uint8_t *m = g.m;
+  const double *hx = g.hx, *hy = g.hy;
+  const double *Fx = FLD(FX), *Fy = FLD(FY), *u = FLD(U), *v = FLD(V), *sqrt_T = FLD(SQRT_T);
+  const double *Gamma_yf = FLD(GAMMA_YF), *D_ux = FLD(D_UX), *D_uy = FLD(D_UY);
+  const double *rho_pr = FLD(RHO_PR), *u_pr = FLD(U_PR), *rho = FLD(RHO), *rho_u = FLD(RHO_U);
+  const unsigned i = n.i, j = n.j, ij = n.ij, i_1j = n.i_1j, i1j = n.i1j, ij_1 = n.ij_1, ij1 = n.ij1;
+  const unsigned i_1j_1 = n.i_1j_1, i_1j1 = n.i_1j1, i_2j = n.i_2j, i_1 = n.i_1;
+  const double cT = g.cT, ht = g.ht;
+  double F_xu1 = 0.5 * (Fx[i_1j] + Fx[ij]);
+  double F_xu2 = 0.5 * (Fx[ij] + Fx[i1j]);
+  double Scu = cT * (bd(m[i1j]) * u[i1j] * sqrt_T[ij] * hy[j] / (hx[i] * 3.0)
+                     + bd(m[i_2j]) * u[i_1j] * sqrt_T[i_1j] * hy[j] / (hx[i_1] * 3.0)
+                     + Gamma_yf[ij1] * (v[ij1] * bd(m[ij1]) - v[i_1j1] * bd(m[i_1j1]))
+                     - Gamma_yf[ij] * (v[ij] * bd(m[ij_1]) - v[i_1j] * bd(m[i_1j_1]))
+                     - sqrt_T[ij] * (v[ij1] * bd(m[ij1]) - v[ij] * bd(m[ij_1])) * 2.0 / 3.0
+                     + sqrt_T[i_1j] * (v[i_1j1] * bd(m[i_1j1]) - v[i_1j] * bd(m[i_1j_1])) * 2.0 / 3.0);
+  double Spu = -(D_ux[ij] + D_ux[i1j]) / 3.0;
+  double au1 = maxi0(F_xu1) + D_ux[ij];
+  double au2 = maxi0(-F_xu2) + D_ux[i1j];
+  double au3 = 0.5 * (maxi0(Fy[i_1j]) + maxi0(Fy[ij])) + D_uy[ij];
+  double au4 = 0.5 * (maxi0(-Fy[i_1j1]) + maxi0(-Fy[ij1])) + D_uy[ij1];
+  double bu3 = (rho_pr[i_1j] * hx[i_1] + rho_pr[ij] * hx[i]) * u_pr[ij] * hy[j] * 0.5 / ht + Scu;
+  double au0 = au1 + au2 + au3 + au4 + 0.5 * (-Fy[i_1j] - Fy[ij] + Fy[i_1j1] + Fy[ij1]) + F_xu2 - F_xu1 - Spu
+               + (rho[i_1j] * hx[i_1] + rho[ij] * hx[i]) * hy[j] * 0.5 / ht;
+  double ups = (au1 * bd(m[i1j]) * u[i_1j] + au2 * bd(m[i_2j]) * u[i1j] + au3 * u[ij_1] + au4 * u[ij1] + bu3) / au0;
+  double dpc = g.cPch * hy[j] / au0;
+  double h = hy[j] * rho_u[ij];
+  FLD(U_PSEUDO)[ij] = ups;
+  FLD(D_P_C_12)[ij] = dpc;
+  FLD(APX)[ij] = dpc * h;
+  FLD(BPX)[ij] = bd(m[i_1j] * m[ij]) * h * ups;
+}
+
+__device__ __forceinline__ void cell_assemble_v(const KGrid &g, const KFields &fl, const Nbr &n) {
+  const uint8_t *m = g.m;
+  const unsigned Nx = (unsigned)g.Nx;
+  const double *hx = g.hx, *hy = g.hy;
+  const double *Fx = FLD(FX), *Fy = FLD(FY), *u = FLD(U), *v = FLD(V), *sqrt_T = FLD(SQRT_T);
+  const double *Gamma_xf = FLD(GAMMA_XF), *D_vx = FLD(D_VX), *D_vy = FLD(D_VY);
+  const double *rho_pr = FLD(RHO_PR), *v_pr = FLD(V_PR), *rho = FLD(RHO), *rho_v = FLD(RHO_V);
+  const unsigned i = n.i, j = n.j, ij = n.ij, i_1j = n.i_1j, i1j = n.i1j, ij_1 = n.ij_1, ij1 = n.ij1;
+  const unsigned i_1j_1 = n.i_1j_1, i1j_1 = n.i1j_1;
+  const double cT = g.cT, ht = g.ht;
+  // S:3399 ij_1 - Nx*(0 <= (int)(j-2)); S:3400 hy[j - 1*(0 <= (int)(j-1))]
+  const unsigned jm2 = ij_1 - Nx * (0 <= (int)(j - 2));
+  const double hyjm = hy[j - 1 * (0 <= (int)(j - 1))];
+  double F_yv3 = 0.5 * (Fy[ij_1] + Fy[ij]);
+  double F_yv4 = 0.5 * (Fy[ij] + Fy[ij1]);
+  double Scv = cT * (bd(m[ij1]) * v[ij1] * sqrt_T[ij] * hx[i] / (hy[j] * 3.0)
+                     + bd(m[jm2]) * v[ij_1] * sqrt_T[ij_1] * hx[i] / (hyjm * 3.0)
+                     + Gamma_xf[i1j] * (u[i1j] * bd(m[i1j]) - u[i1j_1] * bd(m[i1j_1]))
+                     - Gamma_xf[ij] * (u[ij] * bd(m[i_1j]) - u[ij_1] * bd(m[i_1j_1]))
+                     - sqrt_T[ij] * (u[i1j] * bd(m[i1j]) - u[ij] * bd(m[i_1j])) * 2.0 / 3.0
+                     + sqrt_T[ij_1] * (u[i1j_1] * bd(m[i1j_1]) - u[ij_1] * bd(m[i_1j_1])) * 2.0 / 3.0)
+               - g.c_Fr_v * (rho[ij_1] * hx[i] * 0.5 * hyjm + rho[ij] * hx[i] * 0.5 * hy[j]);
+  double Spv = -(D_vy[ij] + D_vy[ij1]) / 3.0;
+  double av1 = 0.5 * (maxi0(Fx[ij]) + maxi0(Fx[ij_1])) + D_vx[ij];
+  double av2 = 0.5 * (maxi0(-Fx[i1j]) + maxi0(-Fx[i1j_1])) + D_vx[i1j];
+  double av3 = maxi0(F_yv3) + D_vy[ij];
+  double av4 = maxi0(-F_yv4) + D_vy[ij1];
+  double bv3 = (rho_pr[ij_1] * hy[j - 1] + rho_pr[ij] * hy[j]) * v_pr[ij] * hx[i] * 0.5 / ht + Scv;
+  double av0 = av1 + av2 + av3 + av4 + 0.5 * (Fx[i1j] - Fx[ij] + Fx[i1j_1] - Fx[ij_1]) + F_yv4 - F_yv3 - Spv
+               + (rho[ij_1] * hy[j - 1] + rho[ij] * hy[j]) * hx[i] * 0.5 / ht;
+  double vps = (av1 * v[i_1j] + av2 * v[i1j] + av3 * bd(m[jm2]) * v[ij_1] + av4 * bd(m[ij1]) * v[ij1] + bv3) / av0;
+  double dpc = g.cPch * hx[i] / av0;
+  double h = hx[i] * rho_v[ij];
+  FLD(V_PSEUDO)[ij] = vps;
+  FLD(D_P_C_34)[ij] = dpc;
+  FLD(APY)[ij] = dpc * h;
+  FLD(BPY)[ij] = bd(m[ij - Nx] * m[ij]) * h * vps;
+}
+
+__device__ __forceinline__ void cell_energy_source(const KGrid &g, const KFields &fl, const Nbr &n) {
+  const uint8_t *m = g.m;
+  const unsigned Nx = (unsigned)g.Nx;
+  const double *hx = g.hx, *hy = g.hy;
+  const double *u = FLD(U), *v = FLD(V), *sqrt_T = FLD(SQRT_T), *p_pr = FLD(P_PR);
+  const unsigned i = n.i, j = n.j, ij = n.ij, i_1j = n.i_1j, i1j = n.i1j, ij_1 = n.ij_1, ij1 = n.ij1;
+  const unsigned i_1j_1 = n.i_1j_1, i_1j1 = n.i_1j1, i1j_1 = n.i1j_1, i1j1 = n.i1j1, i_1 = n.i_1, i1 = n.i1;
+  // S:3729-3740: the upper neighbour row is guarded by j < Nx-1
+  const unsigned jm = j - 1 * (0 < j), jp = j + 1 * (j < (Nx - 1));
+  double dudx_T = (bd(m[i1j]) * u[i1j] - bd(m[i_1j]) * u[ij]) / hx[i];
+  double dvdy_T = (bd(m[ij1]) * v[ij1] - bd(m[ij_1]) * v[ij]) / hy[j];
+  double s0 = ((u[ij] - u[ij_1]) / (bd(m[i_1j_1] && m[ij_1]) * hy[jm] + hy[j]))
+              + ((v[ij] - v[i_1j]) / (bd(m[i_1j_1] && m[i_1j]) * hx[i_1] + hx[i]));
+  double s1 = ((u[i1j] - u[i1j_1]) / (bd(m[ij_1] && m[i1j_1]) * hy[jm] + hy[j]))
+              + ((v[i1j] - v[ij]) / (hx[i] + bd(m[i1j_1] && m[i1j]) * hx[i1]));
+  double s2 = ((u[ij1] - u[ij]) / (hy[j] + bd(m[i_1j1] && m[ij1]) * hy[jp]))
+              + ((v[ij1] - v[i_1j1]) / (bd(m[i_1j] && m[i_1j1]) * hx[i_1] + hx[i]));
+  double s3 = ((u[i1j1] - u[i1j]) / (hy[j] + bd(m[ij1] && m[i1j1]) * hy[jp]))
+              + ((v[i1j1] - v[ij1]) / (hx[i] + bd(m[i1j] && m[i1j1]) * hx[i1]));
+  double dsum = dudx_T + dvdy_T;
+  FLD(SCT1)[ij] = g.ht * sqrt_T[ij] * g.cT2 * hx[i] * hy[j]
+                      * (2.0 * (dudx_T * dudx_T + dvdy_T * dvdy_T) + s0 * s0 + s1 * s1 + s2 * s2 + s3 * s3
+                         - dsum * dsum * 2.0 / 3.0)
+                  + p_pr[ij] * hx[i] * hy[j];
+}
+
+__global__ void __launch_bounds__(BX *BY) k_assemble(KGrid g, KFields fl, unsigned long long *red, int nred) {
+  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  if (red && blockIdx.x == 0 && blockIdx.y == 0) {
+    const int t = threadIdx.x + threadIdx.y * BX;
+    if (t < nred) red[t] = 0ull;
+  }
+  if (i >= g.Nx || j >= g.Ny) return;
+  const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny);
+  const uint8_t *m = g.m;
+  if (in_box(i, j, g.Nx_u0, g.Nx_u1, g.Ny_u0, g.Ny_u1) && m[n.i_1j] && m[n.ij]) cell_assemble_u(g, fl, n);
+  if (in_box(i, j, g.Nx_v0, g.Nx_v1, g.Ny_v0, g.Ny_v1) && m[n.ij_1] && m[n.ij]) cell_assemble_v(g, fl, n);
+  if (in_box(i, j, g.Nx_o0, g.Nx_o1, g.Ny_o0, g.Ny_o1) && m[n.ij]) cell_energy_source(g, fl, n);
+}
+
+// ---------------------------------------------------------------------------
+// A5: one fused T/p Jacobi sweep (S:3805-4027).  Reads (Po, To), writes
+// (Pn, Tn) on every cell: the update set gets the new values, every other
+// cell is copied through, which restates the loop-3 entry copy S:3763-3764
+// and the pointer swap S:3792-3795 without a separate pass.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(BX *BY) k_sweep(KGrid g, KFields fl, const double *__restrict__ Po,
+                                                  const double *__restrict__ To, double *__restrict__ Pn,
+                                                  double *__restrict__ Tn, unsigned long long *red_p,
+                                                  unsigned long long *red_T) {
+  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  unsigned long long r[2] = {0ull, 0ull};
+  if (i < g.Nx && j < g.Ny) {
+    const unsigned ij = (unsigned)i + (unsigned)j * (unsigned)g.Nx;
+    const uint8_t *m = g.m;
+    if (in_box(i, j, g.Nx_o0, g.Nx_o1, g.Ny_o0, g.Ny_o1) && m[ij]) {
+      const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny);
+      const unsigned i_1j = n.i_1j, i1j = n.i1j, ij_1 = n.ij_1, ij1 = n.ij1;
+      const double *Fx = FLD(FX), *Fy = FLD(FY), *DTx = FLD(DTX), *DTy = FLD(DTY), *u = FLD(U), *v = FLD(V);
+      const double *apx = FLD(APX), *apy = FLD(APY), *bpx = FLD(BPX), *bpy = FLD(BPY);
+      const double ht = g.ht;
+      const double hxi = g.hx[i], hyj = g.hy[j];
+      double aT1 = maxi0(Fx[ij]) + DTx[ij];
+      double aT2 = maxi0(-Fx[i1j]) + DTx[i1j];
+      double aT3 = maxi0(Fy[ij]) + DTy[ij];
+      double aT4 = maxi0(-Fy[ij1]) + DTy[ij1];
+      // S:3838-3848: a body neighbour contributes the fluid temperature on
+      // its wall instead (m*T_old + !m*T_wall, with T_wall finite).
+      double X1 = m[i_1j] ? To[i_1j] : FLD(TW_U)[ij];
+      double X2 = m[i1j] ? To[i1j] : FLD(TW_U)[i1j];
+      double X3 = m[ij_1] ? To[ij_1] : FLD(TW_V)[ij];
+      double X4 = m[ij1] ? To[ij1] : FLD(TW_V)[ij1];
+      const double Pij = Po[ij];
+      double tn = (ht * (aT1 * X1 + aT2 * X2 + aT3 * X3 + aT4 * X4
+                         + (1.0 - g.gamma1) * Pij * ((u[i1j] - u[ij]) * hyj + (v[ij1] - v[ij]) * hxi))
+                   + FLD(SCT1)[ij])
+                  / (ht * (aT1 + aT2 + aT3 + aT4 + Fx[i1j] - Fx[ij] + Fy[ij1] - Fy[ij]) + hxi * hyj * FLD(RHO)[ij]);
+      double s = apx[ij] * Po[i_1j] + apx[i1j] * Po[i1j] + apy[ij] * Po[ij_1] + apy[ij1] * Po[ij1] + bpx[ij]
+                 - bpx[i1j] + bpy[ij] - bpy[ij1];
+      // S:3887-3888: inflow/outflow flux terms of the given-velocity columns
+      if (g.given_xb && i == g.Nx_o0) s = s + Fx[ij];
+      if (g.given_xe && i == g.Nx_o1 - 1) s = s - Fx[i1j];
+      double pn = (ht * s + FLD(P_PR)[ij] * hxi * hyj / (FLD(T_PR)[ij]))
+                  / (ht * (apx[ij] + apx[i1j] + apy[ij] + apy[ij1]) + hxi * hyj / (tn));
+      Tn[ij] = tn;
+      Pn[ij] = pn;
+      r[0] = absbits(pn - Pij);
+      r[1] = absbits(tn - To[ij]);
+    } else {
+      Pn[ij] = Po[ij];
+      Tn[ij] = To[ij];
+    }
+  }
+  block_max_atomic<2>(r, red_p, red_T);
+}
+
+// ---------------------------------------------------------------------------
+// A6 + A7 + A9: velocity correction (S:4170-4305), sqrt_T and rho
+// (S:4323-4330), face densities (S:10382-10437).  The face density of face
+// ij needs rho of the west/south cell; it is recomputed here from p and T
+// with the same operations, so no second pass is needed.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double rho_of(const uint8_t *m, const double *p, const double *T, unsigned c) {
+  return m[c] ? p[c] / T[c] : 0.0;
+}
+
+__global__ void __launch_bounds__(BX *BY) k_correct(KGrid g, KFields fl, unsigned long long *red_u,
+                                                    unsigned long long *red_v, int do_correct) {
+  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  unsigned long long r[2] = {0ull, 0ull};
+  if (i < g.Nx && j < g.Ny) {
+    const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny);
+    const uint8_t *m = g.m;
+    const unsigned ij = n.ij, i_1j = n.i_1j, ij_1 = n.ij_1;
+    const double *p = FLD(P), *T = FLD(T);
+    double uf = FLD(U)[ij], vf = FLD(V)[ij];
+    if (do_correct) {
+      if (in_box(i, j, g.Nx_u0, g.Nx_u1, g.Ny_u0, g.Ny_u1) && m[i_1j] && m[ij]) {
+        double un = FLD(U_PSEUDO)[ij] - (p[ij] - p[i_1j]) * FLD(D_P_C_12)[ij];
+        r[0] = absbits(un - uf);
+        uf = un;
+        FLD(U)[ij] = un;
+      }
+      if (in_box(i, j, g.Nx_v0, g.Nx_v1, g.Ny_v0, g.Ny_v1) && m[ij_1] && m[ij]) {
+        double vn = FLD(V_PSEUDO)[ij] - (p[ij] - p[ij - (unsigned)g.Nx]) * FLD(D_P_C_34)[ij];
+        r[1] = absbits(vn - vf);
+        vf = vn;
+        FLD(V)[ij] = vn;
+      }
+    }
+    const double rij = rho_of(m, p, T, ij);
+    FLD(SQRT_T)[ij] = sqrt(T[ij]);
+    FLD(RHO)[ij] = rij;
+    double ru, rv;
+    if (m[i_1j] && m[ij]) ru = upwind_rho(rho_of(m, p, T, i_1j), rij, uf);
+    else if (m[i_1j] != m[ij]) ru = m[i_1j] ? rho_of(m, p, T, i_1j) : rij;
+    else ru = 0.0;
+    if (m[ij_1] && m[ij]) rv = upwind_rho(rho_of(m, p, T, ij_1), rij, vf);
+    else if (m[ij_1] != m[ij]) rv = m[ij_1] ? rho_of(m, p, T, ij_1) : rij;
+    else rv = 0.0;
+    FLD(RHO_U)[ij] = ru;
+    FLD(RHO_V)[ij] = rv;
+  }
+  if (do_correct) block_max_atomic<2>(r, red_u, red_v);
+}
+
+// A7 alone (S:4323-4330)
+__global__ void __launch_bounds__(BX *BY) k_properties(KGrid g, KFields fl) {
+  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  if (i >= g.Nx || j >= g.Ny) return;
+  const unsigned c = (unsigned)i + (unsigned)j * (unsigned)g.Nx;
+  const double T = FLD(T)[c];
+  FLD(SQRT_T)[c] = sqrt(T);
+  FLD(RHO)[c] = g.m[c] ? FLD(P)[c] / T : 0.0;
+}
+
+// A9 alone (S:10382-10437), from the stored rho
+__global__ void __launch_bounds__(BX *BY) k_rho_mid(KGrid g, KFields fl) {
+  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  if (i >= g.Nx || j >= g.Ny) return;
+  const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny);
+  const uint8_t *m = g.m;
+  const double *rho = FLD(RHO);
+  const unsigned ij = n.ij, i_1j = n.i_1j, ij_1 = n.ij_1;
+  double ru, rv;
+  if (m[i_1j] && m[ij]) ru = upwind_rho(rho[i_1j], rho[ij], FLD(U)[ij]);
+  else if (m[i_1j] != m[ij]) ru = m[i_1j] ? rho[i_1j] : rho[ij];
+  else ru = 0.0;
+  if (m[ij_1] && m[ij]) rv = upwind_rho(rho[ij_1], rho[ij], FLD(V)[ij]);
+  else if (m[ij_1] != m[ij]) rv = m[ij_1] ? rho[ij_1] : rho[ij];
+  else rv = 0.0;
+  FLD(RHO_U)[ij] = ru;
+  FLD(RHO_V)[ij] = rv;
+}
+
+// ---------------------------------------------------------------------------
+// A12 wall part: velocity slip (S:7098-7331) and the fluid temperature on the
+// walls (S:7744-8045), per face.  The reference walks lists of cells in
+// increasing node order and each cell writes up to two faces; per face the
+// last writer of that walk is evaluated here.  Every value read is a face or
+// cell that no writer of this pass modifies, so faces are independent.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ const DevBody *body_of(const KGrid &g, uint32_t k) {
+  return (k >= 1u && (int)k <= g.nbodies) ? &g.bodies[k - 1] : nullptr;
+}
+
+// Slip cell for u: body cell with fluid below or above, slip on (S:10169-10176).
+__device__ __forceinline__ const DevBody *slip_cell_u(const KGrid &g, const Nbr &c) {
+  const uint32_t *vol = g.vol;
+  if (!((0u < vol[c.ij]) && ((0u == vol[c.ij_1]) || (0u == vol[c.ij1])))) return nullptr;
+  const DevBody *b = body_of(g, vol[c.ij]);
+  return (b && b->is_slip) ? b : nullptr;
+}
+__device__ __forceinline__ const DevBody *slip_cell_v(const KGrid &g, const Nbr &c) {
+  const uint32_t *vol = g.vol;
+  if (!((0u < vol[c.ij]) && ((0u == vol[c.i_1j]) || (0u == vol[c.i1j])))) return nullptr;
+  const DevBody *b = body_of(g, vol[c.ij]);
+  return (b && b->is_slip) ? b : nullptr;
+}
+
+// The two writes of slip cell c for u (S:7155-7206): which==0 -> u[c.ij],
+// which==1 -> u[c.i1j].  Returns true and the value if the write happens.
+__device__ __forceinline__ bool slip_u_write(const KGrid &g, const KFields &fl, const Nbr &c, const DevBody *b,
+                                             int which, double &val) {
+  const uint8_t *m = g.m;
+  const double *u = FLD(U), *rho_u = FLD(RHO_U);
+  const double FK = b->F_slip * g.Kn, ub = b->u_body;
+  if (0u == g.vol[c.ij_1]) {
+    const double hh = g.hy[c.j - 1 * (0 < c.j)];
+    if (which == 0) {
+      if (!m[c.i_1j_1]) return false;
+      double a = g.kn_local ? FK / (rho_u[c.ij_1] * 0.5 * hh) : FK / (0.5 * hh);
+      val = (u[c.ij_1] * a + ub) / (1.0 + a);
+    } else {
+      if (!m[c.i1j_1]) return false;
+      double a = g.kn_local ? FK / (rho_u[c.i1j_1] * 0.5 * hh) : FK / (0.5 * hh);
+      val = (u[c.i1j_1] * a + ub) / (1.0 + a);
+    }
+  } else {
+    const double hh = g.hy[c.j + 1 * (c.j < (unsigned)(g.Ny - 1))];
+    if (which == 0) {
+      if (!m[c.i_1j1]) return false;
+      double a = g.kn_local ? FK / (rho_u[c.ij1] * 0.5 * hh) : FK / (0.5 * hh);
+      val = (u[c.ij1] * a + ub) / (1.0 + a);
+    } else {
+      if (!m[c.i1j1]) return false;
+      double a = g.kn_local ? FK / (rho_u[c.i1j1] * 0.5 * hh) : FK / (0.5 * hh);
+      val = (u[c.i1j1] * a + ub) / (1.0 + a);
+    }
+  }
+  return true;
+}
+
+// S:7273-7324: which==0 -> v[c.ij], which==1 -> v[c.ij1]
+__device__ __forceinline__ bool slip_v_write(const KGrid &g, const KFields &fl, const Nbr &c, const DevBody *b,
+                                             int which, double &val) {
+  const uint8_t *m = g.m;
+  const double *v = FLD(V), *rho_v = FLD(RHO_V);
+  const double FK = b->F_slip * g.Kn, vb = b->v_body;
+  if (0u == g.vol[c.i_1j]) {
+    const double hh = g.hx[c.i_1];
+    if (which == 0) {
+      if (!m[c.i_1j_1]) return false;
+      double a = g.kn_local ? FK / (rho_v[c.i_1j] * 0.5 * hh) : FK / (0.5 * hh);
+      val = (v[c.i_1j] * a + vb) / (1.0 + a);
+    } else {
+      if (!m[c.i_1j1]) return false;
+      double a = g.kn_local ? FK / (rho_v[c.i_1j1] * 0.5 * hh) : FK / (0.5 * hh);
+      val = (v[c.i_1j1] * a + vb) / (1.0 + a);
+    }
+  } else {
+    const double hh = g.hx[c.i1];
+    if (which == 0) {
+      if (!m[c.i1j_1]) return false;
+      double a = g.kn_local ? FK / (rho_v[c.i1j] * 0.5 * hh) : FK / (0.5 * hh);
+      val = (v[c.i1j] * a + vb) / (1.0 + a);
+    } else {
+      if (!m[c.i1j1]) return false;
+      double a = g.kn_local ? FK / (rho_v[c.i1j1] * 0.5 * hh) : FK / (0.5 * hh);
+      val = (v[c.i1j1] * a + vb) / (1.0 + a);
+    }
+  }
+  return true;
+}
+
+// Wall temperature seen from a fluid cell through a face with body k (S:7762-7797).
+__device__ __forceinline__ double wall_T(const KGrid &g, uint32_t k, double T_fluid, double rho_fluid,
+                                         double half_h) {
+  const DevBody *b = body_of(g, k);
+  if (!b) return 0.0;
+  if (b->is_jump) {
+    double FK = g.kn_local ? (b->F_jump * g.Kn) / rho_fluid : b->F_jump * g.Kn;
+    double a = FK / half_h;
+    return (a * T_fluid + b->T_body) / (1.0 + a);
+  } else if (b->dTdn) {
+    return T_fluid;
+  }
+  return b->T_body;
+}
+
+__global__ void __launch_bounds__(BX *BY) k_wall(KGrid g, KFields fl, int parts) {
+  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  if (i >= g.Nx || j >= g.Ny) return;
+  const unsigned Nx = (unsigned)g.Nx, Ny = (unsigned)g.Ny;
+  const unsigned k = (unsigned)i + (unsigned)j * Nx;
+  const uint8_t *m = g.m;
+  if (parts & STS_BC_SLIP_U) {
+    double val = 0.0;
+    bool w = false;
+    if (i >= 1) {  // cell (i-1, j) writes u[i1j]
+      const Nbr c((unsigned)i - 1u, (unsigned)j, Nx, Ny);
+      const DevBody *b = slip_cell_u(g, c);
+      if (b) w |= slip_u_write(g, fl, c, b, 1, val);
+    }
+    const Nbr c((unsigned)i, (unsigned)j, Nx, Ny);
+    const DevBody *b = slip_cell_u(g, c);
+    if (b) {
+      w |= slip_u_write(g, fl, c, b, 0, val);
+      if ((unsigned)i == Nx - 1u) w |= slip_u_write(g, fl, c, b, 1, val);  // clamped i1j == ij
+    }
+    if (w) FLD(U)[k] = val;
+  }
+  if (parts & STS_BC_SLIP_V) {
+    double val = 0.0;
+    bool w = false;
+    if (j >= 1) {  // cell (i, j-1) writes v[ij1]
+      const Nbr c((unsigned)i, (unsigned)j - 1u, Nx, Ny);
+      const DevBody *b = slip_cell_v(g, c);
+      if (b) w |= slip_v_write(g, fl, c, b, 1, val);
+    }
+    const Nbr c((unsigned)i, (unsigned)j, Nx, Ny);
+    const DevBody *b = slip_cell_v(g, c);
+    if (b) {
+      w |= slip_v_write(g, fl, c, b, 0, val);
+      if ((unsigned)j == Ny - 1u) w |= slip_v_write(g, fl, c, b, 1, val);  // clamped ij1 == ij
+    }
+    if (w) FLD(V)[k] = val;
+  }
+  if (parts & STS_BC_JUMP) {
+    const double *T = FLD(T), *rho = FLD(RHO);
+    if (i >= 1) {
+      const unsigned w_ = k - 1u;
+      if (m[k] && !m[w_]) FLD(TW_U)[k] = wall_T(g, g.vol[w_], T[k], rho[k], 0.5 * g.hx[i]);
+      else if (m[w_] && !m[k]) FLD(TW_U)[k] = wall_T(g, g.vol[k], T[w_], rho[w_], 0.5 * g.hx[i - 1]);
+    }
+    if (j >= 1) {
+      const unsigned s_ = k - Nx;
+      if (m[k] && !m[s_]) FLD(TW_V)[k] = wall_T(g, g.vol[s_], T[k], rho[k], 0.5 * g.hy[j]);
+      else if (m[s_] && !m[k]) FLD(TW_V)[k] = wall_T(g, g.vol[k], T[s_], rho[s_], 0.5 * g.hy[j - 1]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// A12 edge part: Neumann copies and inflow/outflow mass conditions at the
+// solve-box edges, S:6790-7096 (velocities), S:7650-7741 (temperature),
+// S:7335-7428 (pressure).  O(Nx+Ny) work whose phases read each other's
+// results, so one workgroup runs the phases in the reference's order with a
+// barrier between them.
+// ---------------------------------------------------------------------------
+constexpr int EDGE_THREADS = 1024;
+
+__device__ __forceinline__ void edge_x(const KGrid &g, double *a, int b0, int b1, bool lo, bool hi) {
+  const int Nx = g.Nx, Ny = g.Ny;
+  const uint8_t *m = g.m;
+  if (lo) {
+    for (int t = threadIdx.x; t < Ny * b0; t += EDGE_THREADS) {
+      int j = t / b0, i = t - j * b0;
+      int node = b0 + j * Nx;
+      if (m[node]) a[i + j * Nx] = a[node];
+    }
+  }
+  if (hi) {
+    const int w = Nx - b1;
+    for (int t = threadIdx.x; t < Ny * w; t += EDGE_THREADS) {
+      int j = t / w, i = b1 + (t - j * w);
+      int node = b1 - 1 + j * Nx;
+      if (m[node]) a[i + j * Nx] = a[node];
+    }
+  }
+}
+
+__device__ __forceinline__ void edge_y(const KGrid &g, double *a, int b0, int b1, bool lo, bool hi) {
+  const int Nx = g.Nx, Ny = g.Ny;
+  const uint8_t *m = g.m;
+  if (lo) {
+    for (int t = threadIdx.x; t < Nx * b0; t += EDGE_THREADS) {
+      int j = t / Nx, i = t - j * Nx;
+      int node = i + b0 * Nx;
+      if (m[node]) a[i + j * Nx] = a[node];
+    }
+  }
+  if (hi) {
+    const int h = Ny - b1;
+    for (int t = threadIdx.x; t < Nx * h; t += EDGE_THREADS) {
+      int jj = t / Nx, i = t - jj * Nx, j = b1 + jj;
+      int node = i + (b1 - 1) * Nx;
+      if (m[node]) a[i + j * Nx] = a[node];
+    }
+  }
+}
+
+__global__ void __launch_bounds__(EDGE_THREADS) k_edges(KGrid g, KFields fl, int parts, double *probe) {
+  const int Nx = g.Nx, Ny = g.Ny;
+  const uint8_t *m = g.m;
+  const unsigned bc = g.bc;
+  double *u = FLD(U), *v = FLD(V);
+  const double *rho = FLD(RHO), *rho_pr = FLD(RHO_PR), *rho_u = FLD(RHO_U), *rho_v = FLD(RHO_V);
+  const double *hx = g.hx, *hy = g.hy;
+  if (parts & STS_BC_VELOCITIES) {
+    edge_x(g, v, g.Nx_v0, g.Nx_v1, bc & BC_DVDX_XB, false);
+    __syncthreads();
+    edge_x(g, v, g.Nx_v0, g.Nx_v1, false, bc & BC_DVDX_XE);
+    __syncthreads();
+    edge_y(g, v, g.Ny_v0, g.Ny_v1, bc & BC_DVDY_YB, false);
+    __syncthreads();
+    edge_y(g, v, g.Ny_v0, g.Ny_v1, false, bc & BC_DVDY_YE);
+    __syncthreads();
+    edge_x(g, u, g.Nx_u0, g.Nx_u1, bc & BC_DUDX_XB, false);
+    __syncthreads();
+    if (bc & BC_DURHODX_XB)
+      for (int j = threadIdx.x; j < Ny; j += EDGE_THREADS) {
+        int node = g.Nx_u0 - 1 + j * Nx;
+        if (m[node]) u[node] = (rho_u[node + 1] * u[node + 1]) / rho_u[node];
+      }
+    __syncthreads();
+    if (bc & BC_DRHODT_XB) {
+      const int i = g.Nx_u0 - 1;
+      for (int j = threadIdx.x; j < Ny; j += EDGE_THREADS) {
+        int node = i + j * Nx;
+        if (m[node] && j + 1 < Ny)
+          u[node] = ((rho[node] - rho_pr[node]) * hx[i] * hy[j] / g.ht
+                     + (rho_v[node + Nx] * v[node + Nx] - rho_v[node] * v[node]) * hx[i]
+                     + rho_u[node + 1] * u[node + 1] * hy[j])
+                    / (rho_u[node] * hy[j]);
+      }
+    }
+    __syncthreads();
+    edge_x(g, u, g.Nx_u0, g.Nx_u1, false, bc & BC_DUDX_XE);
+    __syncthreads();
+    if (bc & BC_DURHODX_XE)
+      for (int j = threadIdx.x; j < Ny; j += EDGE_THREADS) {
+        int node = g.Nx_u1 - 1 + j * Nx;
+        if (m[node]) u[node + 1] = (rho_u[node] * u[node]) / rho_u[node + 1];
+      }
+    __syncthreads();
+    if (bc & BC_DRHODT_XE) {
+      const int i = g.Nx_u1 - 1;
+      for (int j = threadIdx.x; j < Ny; j += EDGE_THREADS) {
+        int node = i + j * Nx;
+        if (m[node] && j + 1 < Ny)
+          u[node + 1] = (-(rho[node] - rho_pr[node]) * hx[i] * hy[j] / g.ht
+                         - (rho_v[node + Nx] * v[node + Nx] - rho_v[node] * v[node]) * hx[i]
+                         + rho_u[node] * u[node] * hy[j])
+                        / (rho_u[node + 1] * hy[j]);
+      }
+    }
+    __syncthreads();
+    if (bc & BC_UMINMOUT_XE) {  // S:7031-7059 (serial build): the last row's rho_u
+      const double Min = rho_u[g.Nx_u0 + (Ny - 1) * Nx];
+      const double Mout = rho_u[g.Nx_u1 + (Ny - 1) * Nx];
+      for (int j = threadIdx.x; j < Ny; j += EDGE_THREADS) {
+        int node = g.Nx_u1 - 1 + j * Nx;
+        u[node + 1] = u[node] * Min / Mout;
+      }
+    }
+    __syncthreads();
+    edge_y(g, u, g.Ny_u0, g.Ny_u1, bc & BC_DUDY_YB, false);
+    __syncthreads();
+    edge_y(g, u, g.Ny_u0, g.Ny_u1, false, bc & BC_DUDY_YE);
+    __syncthreads();
+  }
+  if (parts & STS_BC_TEMPERATURE) {
+    double *T = FLD(T);
+    edge_x(g, T, g.Nx_o0, g.Nx_o1, bc & BC_DTDX_XB, false);
+    __syncthreads();
+    edge_x(g, T, g.Nx_o0, g.Nx_o1, false, bc & BC_DTDX_XE);
+    __syncthreads();
+    edge_y(g, T, g.Ny_o0, g.Ny_o1, bc & BC_DTDY_YB, false);
+    __syncthreads();
+    edge_y(g, T, g.Ny_o0, g.Ny_o1, false, bc & BC_DTDY_YE);
+    __syncthreads();
+  }
+  if (parts & STS_BC_PRESSURE) {
+    double *p = FLD(P);
+    edge_x(g, p, g.Nx_o0, g.Nx_o1, bc & BC_DPDX_XB, false);
+    __syncthreads();
+    edge_x(g, p, g.Nx_o0, g.Nx_o1, false, bc & BC_DPDX_XE);
+    __syncthreads();
+    edge_y(g, p, g.Ny_o0, g.Ny_o1, bc & BC_DPDY_YB, false);
+    __syncthreads();
+    edge_y(g, p, g.Ny_o0, g.Ny_o1, false, bc & BC_DPDY_YE);
+    __syncthreads();
+  }
+  if (probe && threadIdx.x == 0) {
+    // S:7512-7513: the cell p_BC_xb is read from (not touched by the edge copies)
+    int jj = int((g.Ny_o1 - 1.0) - g.Ny_o0);
+    probe[0] = (jj >= 0 && jj < Ny) ? FLD(P)[g.Nx_o0 + jj * Nx] : 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// A10 + A11: fluxes (S:10440-10454) and diffusion coefficients
+// (S:10457-10618) over all cells.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(BX *BY) k_flux_diff(KGrid g, KFields fl) {
+  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  if (i >= g.Nx || j >= g.Ny) return;
+  const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny);
+  const uint8_t *m = g.m;
+  const double *hx = g.hx, *hy = g.hy, *sT = FLD(SQRT_T), *Twu = FLD(TW_U), *Twv = FLD(TW_V);
+  const unsigned ij = n.ij, i_1j = n.i_1j, ij_1 = n.ij_1, i_1j_1 = n.i_1j_1, i_1 = n.i_1;
+  const unsigned jm = (unsigned)j - 1u * (0 < j);
+  const double cT = g.cT, cT1 = g.cT1;
+  FLD(FX)[ij] = hy[j] * FLD(RHO_U)[ij] * FLD(U)[ij];
+  FLD(FY)[ij] = hx[i] * FLD(RHO_V)[ij] * FLD(V)[ij];
+  const bool m00 = m[i_1j_1], m10 = m[ij_1], m01 = m[i_1j], m11 = m[ij];
+  double Gy = 0.0, Gx = 0.0;  // all four cells body: the reference leaves its initial 0
+  if (m00 && m10 && m01 && m11) {
+    Gy = 0.5 * (Li(sT[i_1j_1], sT[i_1j], hy[jm], hy[j]) + Li(sT[ij_1], sT[ij], hy[jm], hy[j]));
+    Gx = 0.5 * (Li(sT[i_1j_1], sT[ij_1], hx[i_1], hx[i]) + Li(sT[i_1j], sT[ij], hx[i_1], hx[i]));
+  } else if (m00 || m10 || m01 || m11) {
+    Gy = 0.5 * (bd(m00 && m01) * Li(sT[i_1j_1], sT[i_1j], hy[jm], hy[j]) + bd(m00 != m01) * sqrt(Twv[i_1j])
+                + bd(m10 && m11) * Li(sT[ij_1], sT[ij], hy[jm], hy[j]) + bd(m10 != m11) * sqrt(Twv[ij])
+                + bd((!(m00 && m01)) || (!(m10 && m11))) * Li(sqrt(Twu[ij_1]), sqrt(Twu[ij]), hy[jm], hy[j]));
+    Gx = 0.5 * (bd(m00 && m10) * Li(sT[i_1j_1], sT[ij_1], hx[i_1], hx[i]) + bd(m00 != m10) * sqrt(Twu[ij_1])
+                + bd(m01 && m11) * Li(sT[i_1j], sT[ij], hx[i_1], hx[i]) + bd(m01 != m11) * sqrt(Twu[ij])
+                + bd((!(m00 && m10)) || (!(m01 && m11))) * Li(sqrt(Twv[i_1j]), sqrt(Twv[ij]), hx[i_1], hx[i]));
+  }
+  FLD(GAMMA_YF)[ij] = Gy;
+  FLD(GAMMA_XF)[ij] = Gx;
+  FLD(D_UX)[ij] = cT * sT[i_1j] * (hy[j] / hx[i_1]);
+  FLD(D_UY)[ij] = ((m11 && m01) || (m10 && m00))
+                      ? cT * Gy * ((hx[i] + hx[i_1]) / (bd(m11 * m01) * hy[j] + bd(m10 * m00) * hy[jm]))
+                      : 0.0;
+  FLD(D_VX)[ij] = ((m11 && m10) || (m01 && m00))
+                      ? cT * Gx * ((hy[j] + hy[jm]) / (bd(m11 * m10) * hx[i] + bd(m01 * m00) * hx[i_1]))
+                      : 0.0;
+  FLD(D_VY)[ij] = cT * sT[ij_1] * (hx[i] / hy[jm]);
+  FLD(DTX)[ij] = (m11 || m01) ? cT1 * (hy[j] / (0.5 * (bd(m11) * hx[i] + bd(m01) * hx[i_1])))
+                                    * (bd((!m11) || (!m01)) * sqrt(Twu[ij])
+                                       + bd(m11 * m01) * Hi(sT[i_1j], sT[ij], hx[i_1], hx[i]))
+                              : 0.0;
+  FLD(DTY)[ij] = (m11 || m10) ? cT1 * (hx[i] / (0.5 * (bd(m11) * hy[j] + bd(m10) * hy[jm])))
+                                    * (bd((!m11) || (!m10)) * sqrt(Twv[ij])
+                                       + bd(m11 * m10) * Hi(sT[ij_1], sT[ij], hy[jm], hy[j]))
+                              : 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// Setup and output helpers
+// ---------------------------------------------------------------------------
+// ApplyBodiesConditionsToVariables, S:6470-6584, per face: the reference's
+// i-major walk lets cell (i,j) overwrite what cell (i-1,j) wrote to u[ij]
+// (and cell (i,j) overwrite cell (i,j-1) for v[ij]).
+__global__ void __launch_bounds__(BX *BY) k_apply_bodies(KGrid g, KFields fl) {
+  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  if (i >= g.Nx || j >= g.Ny) return;
+  const unsigned Nx = (unsigned)g.Nx;
+  const unsigned k = (unsigned)i + (unsigned)j * Nx;
+  const DevBody *b = body_of(g, g.vol[k]);
+  const DevBody *bw = (i >= 1) ? body_of(g, g.vol[k - 1u]) : nullptr;
+  const DevBody *bs = (j >= 1) ? body_of(g, g.vol[k - Nx]) : nullptr;
+  if (b) FLD(U)[k] = b->u_body;
+  else if (bw) FLD(U)[k] = bw->u_body;
+  if (b) FLD(V)[k] = b->v_body;
+  else if (bs) FLD(V)[k] = bs->v_body;
+  if (b) {
+    FLD(P)[k] = b->p_body;
+    FLD(T)[k] = b->T_body;
+    FLD(RHO)[k] = g.m[k] ? b->p_body / b->T_body : 0.0;
+  }
+}
+
+// S:8048-8099 after the wall temperatures are refreshed: each body cell gets
+// the wall temperature of the face to its last fluid neighbour in the
+// reference's walk order (above, right, left, below).
+__global__ void __launch_bounds__(BX *BY) k_jump_overwrite(KGrid g, KFields fl) {
+  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  if (i >= g.Nx || j >= g.Ny) return;
+  const unsigned Nx = (unsigned)g.Nx;
+  const unsigned k = (unsigned)i + (unsigned)j * Nx;
+  const uint8_t *m = g.m;
+  if (m[k]) return;
+  double *T = FLD(T);
+  if (j + 1 < g.Ny && m[k + Nx]) T[k] = FLD(TW_V)[k + Nx];
+  else if (i + 1 < g.Nx && m[k + 1u]) T[k] = FLD(TW_U)[k + 1u];
+  else if (i >= 1 && m[k - 1u]) T[k] = FLD(TW_U)[k];
+  else if (j >= 1 && m[k - Nx]) T[k] = FLD(TW_V)[k];
+}
+
+__global__ void __launch_bounds__(BX *BY) k_copy5(const double *a0, const double *a1, const double *a2,
+                                                  const double *a3, const double *a4, double *b0, double *b1,
+                                                  double *b2, double *b3, double *b4, int n) {
+  const int t = (blockIdx.x + blockIdx.y * gridDim.x) * (BX * BY) + threadIdx.x + threadIdx.y * BX;
+  if (t >= n) return;
+  b0[t] = a0[t];
+  b1[t] = a1[t];
+  b2[t] = a2[t];
+  b3[t] = a3[t];
+  b4[t] = a4[t];
+}
+
+__global__ void __launch_bounds__(BX *BY) k_set_inflow(KGrid g, KFields fl, double u_in) {
+  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  if (i >= g.Nx_u0 || j >= g.Ny) return;
+  const unsigned k = (unsigned)i + (unsigned)j * (unsigned)g.Nx;
+  if (g.m[k]) FLD(U)[k] = u_in;
+}
+
+// S:10621-10646 plus its norms over fluid cells (one partial per block).
+__global__ void __launch_bounds__(BX *BY) k_continuity(KGrid g, KFields fl, double *cs, double *cf,
+                                                       double *partial) {
+  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  const int Nx = g.Nx;
+  double amax = 0.0, s2 = 0.0, s1 = 0.0;
+  if (i < g.Nx - 1 && j < g.Ny - 1) {
+    const int ij = i + j * Nx;
+    const double *rho_u = FLD(RHO_U), *rho_v = FLD(RHO_V), *u = FLD(U), *v = FLD(V);
+    double Fx0 = g.hy[j] * rho_u[ij] * u[ij], Fx1 = g.hy[j] * rho_u[ij + 1] * u[ij + 1];
+    double Fy0 = g.hx[i] * rho_v[ij] * v[ij], Fy1 = g.hx[i] * rho_v[ij + Nx] * v[ij + Nx];
+    double st = Fx1 - Fx0 + Fy1 - Fy0;
+    double c = (FLD(RHO)[ij] - FLD(RHO_PR)[ij]) * g.hx[i] * g.hy[j] / g.ht + st;
+    cs[ij] = st;
+    cf[ij] = c;
+    if (g.m[ij]) {
+      amax = fabs(c);
+      s2 = c * c;
+      s1 = fabs(c);
+    }
+  }
+  __shared__ double sm[3][BX * BY];
+  const int t = threadIdx.x + threadIdx.y * BX;
+  sm[0][t] = amax;
+  sm[1][t] = s2;
+  sm[2][t] = s1;
+  __syncthreads();
+  for (int o = (BX * BY) / 2; o > 0; o >>= 1) {
+    if (t < o) {
+      sm[0][t] = fmax(sm[0][t], sm[0][t + o]);
+      sm[1][t] += sm[1][t + o];
+      sm[2][t] += sm[2][t + o];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    const int b = blockIdx.x + blockIdx.y * gridDim.x;
+    partial[3 * b] = sm[0][0];
+    partial[3 * b + 1] = sm[1][0];
+    partial[3 * b + 2] = sm[2][0];
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Launch wrappers used by the host API (sts_api.cpp)
+// ---------------------------------------------------------------------------
+#include "sts_launch.h"
+
+static inline dim3 grid2d(const KGrid &g) { return dim3((g.Nx + BX - 1) / BX, (g.Ny + BY - 1) / BY); }
+
+hipError_t launch_assemble(const KGrid &g, const KFields &f, unsigned long long *red, int nred, hipStream_t s) {
+  hipLaunchKernelGGL(k_assemble, grid2d(g), dim3(BX, BY), 0, s, g, f, red, nred);
+  return hipGetLastError();
+}
+hipError_t launch_sweep(const KGrid &g, const KFields &f, const double *Po, const double *To, double *Pn, double *Tn,
+                        unsigned long long *rp, unsigned long long *rT, hipStream_t s) {
+  hipLaunchKernelGGL(k_sweep, grid2d(g), dim3(BX, BY), 0, s, g, f, Po, To, Pn, Tn, rp, rT);
+  return hipGetLastError();
+}
+hipError_t launch_correct(const KGrid &g, const KFields &f, unsigned long long *ru, unsigned long long *rv,
+                          int do_correct, hipStream_t s) {
+  hipLaunchKernelGGL(k_correct, grid2d(g), dim3(BX, BY), 0, s, g, f, ru, rv, do_correct);
+  return hipGetLastError();
+}
+hipError_t launch_properties(const KGrid &g, const KFields &f, hipStream_t s) {
+  hipLaunchKernelGGL(k_properties, grid2d(g), dim3(BX, BY), 0, s, g, f);
+  return hipGetLastError();
+}
+hipError_t launch_rho_mid(const KGrid &g, const KFields &f, hipStream_t s) {
+  hipLaunchKernelGGL(k_rho_mid, grid2d(g), dim3(BX, BY), 0, s, g, f);
+  return hipGetLastError();
+}
+hipError_t launch_wall(const KGrid &g, const KFields &f, int parts, hipStream_t s) {
+  hipLaunchKernelGGL(k_wall, grid2d(g), dim3(BX, BY), 0, s, g, f, parts);
+  return hipGetLastError();
+}
+hipError_t launch_edges(const KGrid &g, const KFields &f, int parts, double *probe, hipStream_t s) {
+  hipLaunchKernelGGL(k_edges, dim3(1), dim3(EDGE_THREADS), 0, s, g, f, parts, probe);
+  return hipGetLastError();
+}
+hipError_t launch_flux_diff(const KGrid &g, const KFields &f, hipStream_t s) {
+  hipLaunchKernelGGL(k_flux_diff, grid2d(g), dim3(BX, BY), 0, s, g, f);
+  return hipGetLastError();
+}
+hipError_t launch_apply_bodies(const KGrid &g, const KFields &f, hipStream_t s) {
+  hipLaunchKernelGGL(k_apply_bodies, grid2d(g), dim3(BX, BY), 0, s, g, f);
+  return hipGetLastError();
+}
+hipError_t launch_jump_overwrite(const KGrid &g, const KFields &f, hipStream_t s) {
+  hipLaunchKernelGGL(k_jump_overwrite, grid2d(g), dim3(BX, BY), 0, s, g, f);
+  return hipGetLastError();
+}
+hipError_t launch_copy5(const double *const *src, double *const *dst, int n, hipStream_t s) {
+  const int blocks = (n + BX * BY - 1) / (BX * BY);
+  dim3 gr(blocks < 65535 ? blocks : 65535, (blocks + 65534) / 65535);
+  hipLaunchKernelGGL(k_copy5, gr, dim3(BX, BY), 0, s, src[0], src[1], src[2], src[3], src[4], dst[0], dst[1], dst[2],
+                     dst[3], dst[4], n);
+  return hipGetLastError();
+}
+hipError_t launch_set_inflow(const KGrid &g, const KFields &f, double u_in, hipStream_t s) {
+  dim3 gr((g.Nx_u0 + BX - 1) / BX > 0 ? (g.Nx_u0 + BX - 1) / BX : 1, (g.Ny + BY - 1) / BY);
+  hipLaunchKernelGGL(k_set_inflow, gr, dim3(BX, BY), 0, s, g, f, u_in);
+  return hipGetLastError();
+}
+int continuity_blocks(const KGrid &g) {
+  dim3 gr = grid2d(g);
+  return (int)(gr.x * gr.y);
+}
+hipError_t launch_continuity(const KGrid &g, const KFields &f, double *cs, double *cf, double *partial,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(k_continuity, grid2d(g), dim3(BX, BY), 0, s, g, f, cs, cf, partial);
+  return hipGetLastError();
+}
