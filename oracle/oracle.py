@@ -129,6 +129,9 @@ class Oracle:
             if parts & bit:
                 self._call(name)
 
+    def fluxes(self):
+        self._call("ora_fluxes")
+
     def fluxes_diffusion(self):
         self._call("ora_fluxes")
         self._call("ora_diffusion")

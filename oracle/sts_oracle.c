@@ -241,11 +241,12 @@ void ora_pt_jacobi(const ora_grid *g, ora_fields *f, int nsweep, double *res) {
       res[2 * s + 1] = rT;
     }
   }
-  if (f->p != p0) { /* odd sweep count: hand the caller's buffers back */
-    memcpy(po0, f->p_old, Na * sizeof(double));
-    memcpy(p0, f->p, Na * sizeof(double));
-    memcpy(To0, f->T_old, Na * sizeof(double));
-    memcpy(T0, f->T, Na * sizeof(double));
+  if (f->p != p0) { /* odd sweep count: swap contents back into the caller's buffers */
+    for (unsigned c = 0; c < Na; c++) {
+      double t;
+      t = p0[c]; p0[c] = po0[c]; po0[c] = t;
+      t = T0[c]; T0[c] = To0[c]; To0[c] = t;
+    }
     f->p = p0; f->p_old = po0; f->T = T0; f->T_old = To0;
   }
 }

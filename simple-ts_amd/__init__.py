@@ -253,3 +253,86 @@ class Context:
         r, r0, pr = np.zeros(4), np.zeros(2), np.zeros(1)
         _check(lib().sts_outer_iteration(self.h, n_pc, _dp(r), _dp(r0), _dp(pr)), "sts_outer_iteration")
         return r, r0, float(pr[0])
+
+
+# ---------------------------------------------------------------------------
+# Host setup (libsimple_ts_host.so, include/simple_ts_host.h)
+# ---------------------------------------------------------------------------
+_hlib = None
+
+
+def host_lib():
+    global _hlib
+    if _hlib is None:
+        if not os.path.exists(LIB_HOST):
+            raise RuntimeError(f"{LIB_HOST} is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_HOST)
+        vp = ctypes.c_void_p
+        L.sts_host_last_error.restype = ctypes.c_char_p
+        L.sts_case_load.argtypes = [ctypes.c_char_p, ctypes.POINTER(vp)]
+        L.sts_case_load.restype = ctypes.c_int
+        L.sts_case_free.argtypes = [vp]
+        L.sts_case_free.restype = None
+        L.sts_case_dims.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        L.sts_case_grid.argtypes = [vp, ctypes.POINTER(Grid)]
+        L.sts_case_mesh.argtypes = [vp] + [_DP] * 6
+        L.sts_case_vol.argtypes = [vp, _UP]
+        L.sts_case_value.argtypes = [vp, ctypes.c_char_p]
+        L.sts_case_value.restype = ctypes.c_double
+        L.sts_case_write_config.argtypes = [vp, ctypes.c_char_p]
+        _hlib = L
+    return _hlib
+
+
+class Case:
+    """A case directory (EnteredData.txt + body files) after the reference's
+    setup: mesh, body mask, solve boxes and constants."""
+
+    def __init__(self, directory):
+        L = host_lib()
+        h = ctypes.c_void_p()
+        rc = L.sts_case_load(os.fsencode(directory), ctypes.byref(h))
+        if rc != STS_OK:
+            raise StsError(f"sts_case_load({directory}): {L.sts_host_last_error().decode()}")
+        self.h = h
+        nx, ny = ctypes.c_int(), ctypes.c_int()
+        L.sts_case_dims(h, ctypes.byref(nx), ctypes.byref(ny))
+        self.Nx, self.Ny = nx.value, ny.value
+        self.x_f, self.y_f = np.zeros(self.Nx + 1), np.zeros(self.Ny + 1)
+        self.hx, self.hy = np.zeros(self.Nx), np.zeros(self.Ny)
+        self.x_v, self.y_v = np.zeros(self.Nx), np.zeros(self.Ny)
+        L.sts_case_mesh(h, *[_dp(a) for a in (self.x_f, self.y_f, self.hx, self.hy, self.x_v, self.y_v)])
+        self.vol = np.zeros(self.Nx * self.Ny, dtype=np.uint32)
+        L.sts_case_vol(h, self.vol.ctypes.data_as(_UP))
+        g = Grid()
+        L.sts_case_grid(h, ctypes.byref(g))
+        self.bodies = [dict(u_body=g.bodies[k].u_body, v_body=g.bodies[k].v_body, p_body=g.bodies[k].p_body,
+                            T_body=g.bodies[k].T_body, is_slip=g.bodies[k].is_slip, F_slip=g.bodies[k].F_slip,
+                            is_jump=g.bodies[k].is_jump, F_jump=g.bodies[k].F_jump,
+                            dTdn_on_wall_0=g.bodies[k].dTdn_on_wall_0) for k in range(g.nbodies)]
+        self.boxes = {k: tuple(getattr(g, k)) for k in ("Nx_u", "Ny_u", "Nx_v", "Ny_v", "Nx_o", "Ny_o")}
+        self.scalars = {k: getattr(g, k) for k in Problem.SCALARS}
+        self.flags = dict(given_xb=g.given_xb, given_xe=g.given_xe, kn_local=g.kn_local)
+        self.bc = {k: getattr(g, k) for k in BC_FLAGS}
+
+    def value(self, name):
+        return host_lib().sts_case_value(self.h, name.encode())
+
+    def problem(self):
+        return Problem(self.hx, self.hy, self.vol, self.bodies, self.boxes, self.scalars, bc=self.bc, **self.flags)
+
+    def write_config(self, path):
+        rc = host_lib().sts_case_write_config(self.h, os.fsencode(path))
+        if rc != STS_OK:
+            raise StsError(host_lib().sts_host_last_error().decode())
+
+    def close(self):
+        if self.h:
+            host_lib().sts_case_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,180 @@
+"""Case directories in the reference's input format (EnteredData.txt and
+body files, Objects_for_Solving.h:306-412), and the benchmark workloads.
+
+The reference ships one input (a pressure-driven microchannel); the other
+workloads -- subsonic and supersonic flow past a confined square, and
+Rayleigh-Benard boxes -- are written here in the same format.  The shipped
+EnteredData.txt (tests/golden/shipped) is the template that provides every
+line's comment text; entries are overridden by their variable names.
+"""
+import math
+import os
+import shutil
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+SHIPPED = os.path.join(ROOT, "tests", "golden", "shipped")
+
+NAMES = None
+
+
+def entry_names():
+    """Variable names of the 126 EnteredData.txt lines, from their comments."""
+    global NAMES
+    if NAMES is None:
+        names = []
+        with open(os.path.join(SHIPPED, "EnteredData.txt")) as f:
+            for line in f:
+                com = line.split("//", 1)[1].strip()
+                names.append(com.split()[0].split("=")[0].strip())
+        NAMES = names
+    return NAMES
+
+
+def fmt(v):
+    if isinstance(v, bool):
+        return str(int(v))
+    if isinstance(v, (int, np.integer)):
+        return str(int(v))
+    return repr(float(v))
+
+
+def write_circles(dst, circles):
+    with open(os.path.join(dst, "solve_circles_b.txt"), "w") as f:
+        f.write(f"{len(circles)}\t//N_circles\n")
+        for b in circles:
+            f.write(f"{fmt(b['x_c'])}\t{fmt(b['y_c'])}\t{fmt(b['r_c'])}\t//x_c y_c r_c\n")
+            f.write(f"{fmt(b.get('u_body', 0.0))}\t{fmt(b.get('v_body', 0.0))}\t//u_body v_body\n")
+            f.write("0\t//p_body\n")
+            f.write(f"{fmt(b.get('T_body', 1.0))}\t//T_body\n")
+            f.write(f"{int(b.get('slip', 0))}\t//is_VelocitySlipBC_body\n")
+            f.write(f"{fmt(b.get('F_slip', 1.1466))}\t//F_VelocitySlip\n")
+            f.write("0\t//w_VelocitySlipBC\n")
+            f.write(f"{int(b.get('jump', 0))}\t//is_TemperatureJumpBC_body\n")
+            f.write(f"{fmt(b.get('F_jump', 2.1904))}\t//F_TemperatureJump\n")
+            f.write(f"{fmt(b.get('dTdn', 0.0))}\t//dTdn_on_wall_0\n")
+            f.write("0\t//ToSolveDragCoefficientForThisBody\n")
+
+
+def write_case(dst, overrides=None, polyhedrons=None, template=SHIPPED, circles=None):
+    """Writes dst/EnteredData.txt (template with overrides) and
+    dst/solve_polyhedrons_b.txt (template's, or the given polyhedron list:
+    dicts with vertices [(x, y), ...], inside points p1, p2 and body
+    parameters)."""
+    os.makedirs(dst, exist_ok=True)
+    names = entry_names()
+    lines = open(os.path.join(template, "EnteredData.txt")).read().splitlines()
+    overrides = dict(overrides or {})
+    out = []
+    for k, line in enumerate(lines):
+        val, rest = line.split("\t", 1)
+        if names[k] in overrides:
+            val = fmt(overrides.pop(names[k]))
+        out.append(val + "\t" + rest)
+    if overrides:
+        raise KeyError(f"unknown entries: {sorted(overrides)}")
+    with open(os.path.join(dst, "EnteredData.txt"), "w") as f:
+        f.write("\n".join(out) + "\n")
+    if polyhedrons is None:
+        shutil.copy(os.path.join(template, "solve_polyhedrons_b.txt"), dst)
+    else:
+        with open(os.path.join(dst, "solve_polyhedrons_b.txt"), "w") as f:
+            f.write(f"{len(polyhedrons)}\t//N_polyhedrons\n")
+            for b in polyhedrons:
+                v = b["vertices"]
+                f.write(f"{len(v)}\t//N_vertex_body\n")
+                f.write(f"{fmt(b['p1'][0])}\t{fmt(b['p1'][1])}\t//point1_inside_x point1_inside_y\n")
+                f.write(f"{fmt(b['p2'][0])}\t{fmt(b['p2'][1])}\t//point2_inside_x point2_inside_y\n")
+                f.write(f"{fmt(b.get('u_body', 0.0))}\t{fmt(b.get('v_body', 0.0))}\t//u_body v_body\n")
+                f.write("0\t//pressure in polyhedron - must be 0\n")
+                f.write(f"{fmt(b.get('T_body', 1.0))}\t//Temperature in polyhedron\n")
+                f.write(f"{int(b.get('slip', 0))}\t//is_VelocitySlipBC_body\n")
+                f.write(f"{fmt(b.get('F_slip', 1.1466))}\t//F_VelocitySlip\n")
+                f.write("0\t//w_VelocitySlipBC\n")
+                f.write(f"{int(b.get('jump', 0))}\t//is_TemperatureJumpBC_body\n")
+                f.write(f"{fmt(b.get('F_jump', 2.1904))}\t//F_TemperatureJump\n")
+                f.write(f"{fmt(b.get('dTdn', 0.0))}\t//dTdn_on_wall_0\n")
+                f.write("0\t//ToSolveDragCoefficientForThisBody\n")
+                for x, y in v:
+                    f.write(f"{fmt(x)}\t{fmt(y)}\n")
+    if circles:
+        write_circles(dst, circles)
+    return dst
+
+
+def u_inlet(Re, Kn):
+    """Inlet velocity of a (Re, Kn) pair, u = Re*Kn*sqrt(15*pi/128), the
+    relation the reference applies when GivenReKn is set (S:2022, S:7491)."""
+    return Re * Kn * math.sqrt(15.0 * math.pi / 128.0)
+
+
+def confined_square(dst, Nx, Ny, Re, Kn, steps, dt_over_tin=0.01, L=15.0, H=1.0, side=0.25, xc=4.0, yc=0.5,
+                    tol=1e-8, N_I=5000, N_I_p_c=2, extra=None, wall_slip=0, circles=None):
+    """Flow past a confined square in a channel of length L and height H:
+    square of side `side` centred at (xc, yc), isothermal no-slip walls (T=1),
+    uniform inlet velocity u_in = u_inlet(Re, Kn) held on the first two u
+    columns, outlet pressure fixed (the last two p columns are never updated,
+    S:6120-6122), uniform initial field, dt = dt_over_tin * H / u_in.
+
+    Nx*Ny is the reference's array shape, boundary cells included (the
+    update boxes of S:6009-6267 leave out row 0 and the last two rows), so
+    the channel holds Ny-3 fluid rows between one bottom and two top wall rows,
+    and the cell centres run from x = 0 (inlet column) to x = L."""
+    hx, hy = L / (Nx - 1), H / (Ny - 3)
+    u_in = u_inlet(Re, Kn)
+    s2 = 0.5 * side
+    big = 1e5
+    walls = [
+        dict(vertices=[(-big, -big), (big, -big), (big, 0.0), (-big, 0.0)], p1=(0.0, -5e4), p2=(1.0, -6e4),
+             slip=wall_slip),
+        dict(vertices=[(-big, H), (big, H), (big, big), (-big, big)], p1=(0.0, 5e4), p2=(1.0, 6e4), slip=wall_slip),
+    ]
+    if side > 0:
+        walls.append(dict(vertices=[(xc - s2, yc - s2), (xc + s2, yc - s2), (xc + s2, yc + s2), (xc - s2, yc + s2)],
+                          p1=(xc - 0.3 * s2, yc - 0.2 * s2), p2=(xc + 0.3 * s2, yc + 0.4 * s2)))
+    ov = dict(u_gas_nd=u_in, v_gas_nd=0.0, T_gas_nd=1.0, p_gas_nd=1.0, Kn=Kn, Re=Re, TypeOfNonDimensiolization=3,
+              Nt=steps, Nx=Nx, Ny=Ny, N_I=N_I, N_I_p_c=N_I_p_c, N_I_T=0, ht=dt_over_tin * H / u_in,
+              x_b=0.0, x_e=L, y_b=-0.5 * hy, y_e=H + 1.5 * hy, kind_of_mesh=5, hxmin=hx, hxmax=hx, hymin=hy,
+              hymax=hy, MaxError_Velocities=tol, MaxError_p_c=tol, MaxError_T=tol, MaxError_to_stop_program=1e-14,
+              max_time_for_solve=1000.0, Nt_save_solved_data=10 ** 9, Nt_save_DragCoefficient=10 ** 9,
+              Nt_save_temp_data=1, Given_velocity_on_xb=1, Given_velocity_on_xe=0, Pressure_BC=0, GivenReKn=1,
+              u_given_xb=u_in, drhodt_durhodx_dvrhody_0_BC_xb=0, drhodt_durhodx_dvrhody_0_BC_xe=0, dudx_0_BC_xe=1,
+              dvdx_0_BC_xb=0, dvdx_0_BC_xe=1, dTdx_0_BC_xe=1, dpdx_0_BC_xb=1, To_Use_Kn_local_in_wall_BC=0,
+              N_SubDomains_x=1)
+    if circles:
+        ov["ToImport_data_for_circles_from_file_b"] = 1
+    ov.update(extra or {})
+    return write_case(dst, ov, walls, circles=circles)
+
+
+def rayleigh_benard(dst, Nx, Ny, Kn=0.05, Fr=1.0, steps=10, dt=0.01, W=2.0, H=1.0, T_hot=2.0, T_cold=1.0,
+                    tol=1e-8, N_I=5000, N_I_p_c=2, jump=False, extra=None):
+    """Closed box W*H of gas at rest between a hot bottom wall (T_hot) and a
+    cold top wall (T_cold), side walls at the mean temperature, gravity along
+    -y with c_Fr_v = 2/(gamma*Fr*Ma^2) (S:2066).  As in confined_square, the
+    fluid occupies columns 1..Nx-3 and rows 1..Ny-3 of the Nx*Ny array."""
+    hx, hy = W / (Nx - 3), H / (Ny - 3)
+    Tm = 0.5 * (T_hot + T_cold)
+    big = 1e5
+    walls = [
+        dict(vertices=[(-big, -big), (big, -big), (big, 0.0), (-big, 0.0)], p1=(0.0, -5e4), p2=(1.0, -6e4),
+             T_body=T_hot, jump=int(jump)),
+        dict(vertices=[(-big, H), (big, H), (big, big), (-big, big)], p1=(0.0, 5e4), p2=(1.0, 6e4), T_body=T_cold,
+             jump=int(jump)),
+        dict(vertices=[(-big, 0.0), (0.0, 0.0), (0.0, H), (-big, H)], p1=(-5e4, 0.3 * H), p2=(-6e4, 0.6 * H),
+             T_body=Tm, jump=int(jump)),
+        dict(vertices=[(W, 0.0), (big, 0.0), (big, H), (W, H)], p1=(5e4, 0.3 * H), p2=(6e4, 0.6 * H), T_body=Tm,
+             jump=int(jump)),
+    ]
+    ov = dict(u_gas_nd=0.0, v_gas_nd=0.0, T_gas_nd=Tm, p_gas_nd=1.0, Kn=Kn, Fr=Fr, SolveGravityField=1,
+              TypeOfNonDimensiolization=3, Nt=steps, Nx=Nx, Ny=Ny, N_I=N_I, N_I_p_c=N_I_p_c, N_I_T=0, ht=dt,
+              x_b=-0.5 * hx, x_e=W + 1.5 * hx, y_b=-0.5 * hy, y_e=H + 1.5 * hy, kind_of_mesh=5, hxmin=hx, hxmax=hx,
+              hymin=hy, hymax=hy, MaxError_Velocities=tol, MaxError_p_c=tol, MaxError_T=tol,
+              MaxError_to_stop_program=1e-14, max_time_for_solve=1000.0, Nt_save_solved_data=10 ** 9,
+              Nt_save_DragCoefficient=10 ** 9, Nt_save_temp_data=1, Given_velocity_on_xb=0, Given_velocity_on_xe=0,
+              Pressure_BC=0, GivenReKn=0, drhodt_durhodx_dvrhody_0_BC_xb=0, drhodt_durhodx_dvrhody_0_BC_xe=0,
+              dvdx_0_BC_xb=0, dvdx_0_BC_xe=0, dTdx_0_BC_xe=0, To_Use_Kn_local_in_wall_BC=1, N_SubDomains_x=1)
+    ov.update(extra or {})
+    return write_case(dst, ov, walls)

@@ -1,0 +1,506 @@
+// sts_gpu: the SIMPLE-TS program on one MI355X.
+//
+// Mirrors main() of the reference (SIMPLE_TS_MPI.cpp, "S:n"): it reads the
+// same EnteredData.txt and body files from the case directory, performs the
+// same setup sequence (S:2476-3226), runs loops 1-3 (S:3230-4691) with the
+// same convergence and stop logic, and writes the same output files (mesh
+// text files, u/v/p/Temper/vol_inf_fb fields, EnteredData.txt, it_begin.txt,
+// Tmp.txt, Iterations_in_loops.txt, ContinuityEquation*).  Every field
+// operation runs on the GPU through the C ABI of libsimple_ts_hip.so; the
+// host only evaluates the loop-control scalars, as the reference's rank 0
+// does.
+//
+// Usage: sts_gpu [case_dir] [--timing file.json]
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <chrono>
+#include <sstream>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "simple_ts.h"
+#include "sts_case.h"
+
+namespace {
+
+[[noreturn]] void die(const std::string &msg) {
+  std::cerr << "sts_gpu: " << msg << std::endl;
+  std::exit(2);
+}
+
+void ck(int rc, const char *what) {
+  if (rc != STS_OK) die(std::string(what) + ": " + sts_last_error());
+}
+
+// WriteMassiveToFile_1D (WriteReadData.h:139-173): scientific, 15 digits.
+template <typename T>
+void write_text(const std::string &path, const T *a, size_t n) {
+  std::ofstream o(path);
+  if (!o) die("cannot write " + path);
+  for (size_t k = 0; k < n; k++) o << std::setiosflags(std::ios::scientific) << std::setprecision(15) << a[k] << std::endl;
+}
+
+template <typename T>
+void write_bin(const std::string &path, const T *a, size_t n) {
+  std::ofstream o(path, std::ios::out | std::ios::trunc | std::ios::binary);
+  if (!o) die("cannot write " + path);
+  o.write((const char *)a, (std::streamsize)(sizeof(T) * n));
+}
+
+template <typename T>
+void read_field(const std::string &path, T *a, size_t n, bool binary) {
+  std::ifstream in(binary ? std::ifstream(path, std::ios::in | std::ios::binary) : std::ifstream(path));
+  if (!in) die("cannot read " + path);
+  if (binary) {
+    in.read((char *)a, (std::streamsize)(sizeof(T) * n));
+    if ((size_t)in.gcount() != sizeof(T) * n) die("short file " + path);
+  } else {
+    for (size_t k = 0; k < n; k++)
+      if (!(in >> a[k])) die("cannot parse " + path);
+  }
+}
+
+std::string num(double v) {  // toString (stringConv.h): default stream format
+  std::ostringstream s;
+  s << v;
+  return s.str();
+}
+
+struct Driver {
+  sts::Case c;
+  std::string dir;
+  sts_ctx *ctx = nullptr;
+  sts_grid grid{};
+  int Nx = 0, Ny = 0;
+  size_t Na = 0;
+  std::vector<unsigned char> mask;
+
+  // loop state (S:245-1792 globals of the same names)
+  double it = 0, it_begin = 0, ht = 0;
+  unsigned it_counter = 0, Iter = 0, Iter_pr1 = 0, Iter_pr2 = 0, N_I = 1, N_I_p_c = 2, Iter_p_c = 0;
+  bool continue_iter = true, countinue_time_step = true, solve_is_finished = false;
+  std::vector<double> max_res_p, max_res_T, max_res_p0, max_res_T0;
+  std::vector<unsigned> max_Iter_p_c;
+  double max_res_u = 0, max_res_v = 0, max_res_u0 = 0, max_res_v0 = 0;
+  double u_given_xb_in_it = 0, p_BC_xb = 0;
+  bool tmp_first = true, iters_first = true;
+  time_t time_solve_start = 0;
+  double time_solve_stop = 0;
+  long long outer_total = 0;
+
+  std::string path(const std::string &f) const { return dir + "/" + f; }
+
+  void host_fields(std::vector<double> &u, std::vector<double> &v, std::vector<double> &p, std::vector<double> &T,
+                   std::vector<double> &rho) {
+    for (std::vector<double> *a : {&u, &v, &p, &T, &rho}) a->assign(Na, 0.0);
+    for (size_t k = 0; k < Na; k++) {
+      u[k] = c.get("u_gas_nd");
+      v[k] = c.get("v_gas_nd");
+      p[k] = c.get("p_gas_nd");
+      rho[k] = c.rho_gas_nd;
+      T[k] = c.get("T_gas_nd");
+    }
+  }
+
+  void get(int f, std::vector<double> &a) {
+    a.resize(Na);
+    ck(sts_get_field(ctx, f, a.data()), "sts_get_field");
+  }
+  void set(int f, const std::vector<double> &a) { ck(sts_set_field(ctx, f, a.data()), "sts_set_field"); }
+
+  // S:2547-2562, S:2908-2930: fixed p (and T, rho) in the columns outside
+  // the solve box when the pressure is given on both ends.
+  void pressure_columns(bool with_rho) {
+    std::vector<double> p, T, rho;
+    get(STS_P, p);
+    get(STS_T, T);
+    if (with_rho) get(STS_RHO, rho);
+    const double pb = c.get("p_BC_xb"), pe = c.get("p_BC_xe"), Tb = c.get("T_BC_xb"), Te = c.get("T_BC_xe");
+    for (int j = 0; j < Ny; j++) {
+      for (int i = 0; i < c.Nx_o[0]; i++) {
+        size_t n = (size_t)i + (size_t)j * Nx;
+        if (!mask[n]) continue;
+        p[n] = pb;
+        T[n] = Tb;
+        if (with_rho) rho[n] = pb / Tb;
+      }
+      for (int i = c.Nx_o[1]; i < Nx; i++) {
+        size_t n = (size_t)i + (size_t)j * Nx;
+        if (!mask[n]) continue;
+        p[n] = pe;
+        T[n] = Te;
+        if (with_rho) rho[n] = pe / Te;
+      }
+    }
+    set(STS_P, p);
+    set(STS_T, T);
+    if (with_rho) set(STS_RHO, rho);
+  }
+
+  // WriteSolvedDataToFile, S:5564-5731 (serial build).
+  void write_solved() {
+    ck(sts_jump_overwrite(ctx), "sts_jump_overwrite");  // S:5566-5567
+    std::vector<double> u, v, p, T;
+    get(STS_U, u);
+    get(STS_V, v);
+    get(STS_P, p);
+    get(STS_T, T);
+    const bool bin = c.flag("ToWriteSolvedDataToBinaryFile");
+    const std::string ext = bin ? ".bin" : ".txt";
+    auto out = [&](const std::string &name) {
+      if (bin) {
+        write_bin(path(name + ".bin"), name == "u" ? u.data() : name == "v" ? v.data() : name == "p" ? p.data() : T.data(),
+                  Na);
+      } else {
+        write_text(path(name + ".txt"), name == "u" ? u.data() : name == "v" ? v.data() : name == "p" ? p.data() : T.data(),
+                   Na);
+      }
+    };
+    for (const char *n : {"u", "v", "p", "Temper"}) out(n);
+    if (bin) write_bin(path("vol_inf_fb.bin"), c.vol.data(), Na);
+    else write_text(path("vol_inf_fb.txt"), c.vol.data(), Na);
+    if (c.flag("ToWriteSolvedDataToNewFiles")) {
+      const std::string s = num(it);
+      if (bin) {
+        write_bin(path("u." + s + ".bin"), u.data(), Na);
+        write_bin(path("v." + s + ".bin"), v.data(), Na);
+        write_bin(path("p." + s + ".bin"), p.data(), Na);
+        write_bin(path("Temper." + s + ".bin"), T.data(), Na);
+        write_bin(path("vol_inf_fb." + s + ".bin"), c.vol.data(), Na);
+      } else {
+        write_text(path("u." + s + ".txt"), u.data(), Na);
+        write_text(path("v." + s + ".txt"), v.data(), Na);
+        write_text(path("p." + s + ".txt"), p.data(), Na);
+        write_text(path("Temper." + s + ".txt"), T.data(), Na);
+        write_text(path("vol_inf_fb.txt." + s + ".txt"), c.vol.data(), Na);  // the reference's name
+      }
+    }
+    write_config();
+    std::ofstream o(path("it_begin.txt"));  // Write_it_ToFile, S:6742-6760
+    o << it << std::endl;
+  }
+
+  void write_config() {
+    c.at("p_BC_xb") = p_BC_xb;
+    c.at("solve_is_finished") = solve_is_finished;
+    std::string err = sts::write_config(c, path("EnteredData.txt"));
+    if (!err.empty()) die(err);
+  }
+
+  // SolveContinuityEquation, S:10621-10714
+  void continuity() {
+    std::vector<double> cs(Na), cf(Na);
+    double norms[3];
+    ck(sts_continuity(ctx, cs.data(), cf.data(), norms), "sts_continuity");
+    if (c.flag("ToWriteSolvedDataToBinaryFile")) {
+      write_bin(path("ContinuityEquation_stationary.bin"), cs.data(), Na);
+      write_bin(path("ContinuityEquation.bin"), cf.data(), Na);
+      if (c.flag("ToWriteSolvedDataToNewFiles")) {
+        write_bin(path("ContinuityEquation_stationary." + num(it) + ".bin"), cs.data(), Na);
+        write_bin(path("ContinuityEquation." + num(it) + ".bin"), cf.data(), Na);
+      }
+    } else {
+      write_text(path("ContinuityEquation_stationary.txt"), cs.data(), Na);
+      write_text(path("ContinuityEquation.txt"), cf.data(), Na);
+      if (c.flag("ToWriteSolvedDataToNewFiles")) {
+        write_text(path("ContinuityEquation_stationary." + num(it) + ".txt"), cs.data(), Na);
+        write_text(path("ContinuityEquation." + num(it) + ".txt"), cf.data(), Na);
+      }
+    }
+  }
+
+  // WriteTempDataToFile, S:5838-5942
+  void write_tmp() {
+    std::ofstream o;
+    if (tmp_first && !c.flag("ToReadSolvedDataFromFile")) o.open(path("Tmp.txt"));
+    else o.open(path("Tmp.txt"), std::ios::out | std::ios::app);
+    tmp_first = false;
+    if (!o) die("cannot write Tmp.txt");
+    o << "it = " << it << "\tit_counter = " << it_counter << "\tIter = " << Iter << "\tIter_p_c = " << Iter_p_c
+      << "\tIter_T = " << c.Iter_T << std::endl;
+    unsigned sum = 0;
+    for (unsigned k = 0; k < Iter; k++) {
+      o << "\t" << k << "\t" << max_Iter_p_c[k] << std::setiosflags(std::ios::scientific) << std::setprecision(15)
+        << "\t" << max_res_p0[k] << "\t" << max_res_p[k] << "\t" << max_res_T0[k] << "\t" << max_res_T[k] << std::endl;
+      sum += max_Iter_p_c[k];
+    }
+    o << "\t\t" << sum << std::endl;
+    o << std::setiosflags(std::ios::scientific) << std::setprecision(15) << "\t" << max_res_u0 << "\t" << max_res_u
+      << std::endl
+      << "\t" << max_res_v0 << "\t" << max_res_v << std::endl;
+  }
+
+  // WriteIterationsInLoopsToFile, S:5945-6006
+  void write_iters() {
+    std::ofstream o;
+    if (iters_first) {
+      if (c.flag("ToReadSolvedDataFromFile")) {
+        o.open(path("Iterations_in_loops.txt"), std::ios::out | std::ios::app);
+      } else {
+        o.open(path("Iterations_in_loops.txt"));
+        o << "it" << "\tit_counter" << "\tIter" << "\tsum_max_Iter_p_c" << std::endl;
+      }
+    } else {
+      o.open(path("Iterations_in_loops.txt"), std::ios::out | std::ios::app);
+    }
+    iters_first = false;
+    unsigned sum = 0;
+    for (unsigned k = 0; k < Iter; k++) sum += max_Iter_p_c[k];
+    o << it << "\t" << it_counter << "\t\t" << Iter << "\t" << sum << std::endl;
+  }
+
+  // S:2013-2071 / S:4347-4361 for the current inflow bookkeeping
+  void scalars() {
+    const unsigned type = (unsigned)c.get("TypeOfNonDimensiolization");
+    double cT = c.cT, cPch = c.cPch;
+    if (type == 1 || type == 2) {
+      cT = (1.0 / c.get("Re")) * (u_given_xb_in_it * p_BC_xb / c.get("T_BC_xb"));
+      if (type == 2) cPch = u_given_xb_in_it * u_given_xb_in_it / (c.get("gamma1") * c.get("Ma") * c.get("Ma"));
+      else cPch = 0.5;
+    } else {
+      cT = c.get("Kn") * 5.0 * sqrt(M_PI) / 16.0;
+      cPch = 0.5;
+    }
+    ck(sts_set_scalars(ctx, cT, cPch), "sts_set_scalars");
+  }
+
+  // BoundaryConditions_Pressure_begin_iteration_for_it, S:7430-7647, for
+  // p_BC_xb_correction_method == 0 and no pressure-ratio correction.
+  void pressure_begin(double probe) {
+    if (c.flag("GivenReKn")) c.at("u_given_xb") = c.get("Re") * c.get("Kn") * sqrt(15.0 * M_PI / 128.0);
+    u_given_xb_in_it = c.get("u_given_xb");
+    if (c.flag("Given_velocity_on_xb")) p_BC_xb = probe;
+  }
+
+  void run() {
+    time_solve_start = time(nullptr);
+    Nx = c.Nx;
+    Ny = c.Ny;
+    Na = (size_t)Nx * (size_t)Ny;
+    mask.resize(Na);
+    for (size_t k = 0; k < Na; k++) mask[k] = c.vol[k] == 0;
+    for (const char *f : {"ToContinueFromInterpolation", "ToStartFromExactSolutionForChannelFlowWithPressureBC",
+                          "Pressure_ratio_correct"})
+      if (c.flag(f)) die(std::string(f) + " = 1 is not provided");
+    if ((unsigned)c.get("p_BC_xb_correction_method") != 0) die("p_BC_xb_correction_method != 0 is not provided");
+    ht = c.get("ht");
+    N_I = (unsigned)c.get("N_I");
+    N_I_p_c = (unsigned)c.get("N_I_p_c");
+    if (N_I < 1) N_I = 1;
+    if (N_I_p_c < 1 || N_I_p_c > 62) die("N_I_p_c must be 1..62");
+    p_BC_xb = c.get("p_BC_xb");
+    u_given_xb_in_it = c.u_given_xb_in_it;
+    time_solve_stop = (double)time_solve_start + c.get("max_time_for_solve") * 3600.0;
+    solve_is_finished = false;
+    const double Nt = c.get("Nt");
+
+    // mesh files written by the serial build, S:2249-2259
+    write_text(path("x_f.txt"), c.x_f.data(), c.x_f.size());
+    write_text(path("hx.txt"), c.hx.data(), c.hx.size());
+    write_text(path("x_v.txt"), c.x_v.data(), c.x_v.size());
+    write_text(path("y_f.txt"), c.y_f.data(), c.y_f.size());
+    write_text(path("hy.txt"), c.hy.data(), c.hy.size());
+    write_text(path("y_v.txt"), c.y_v.data(), c.y_v.size());
+    if (!c.flag("ToReadSolvedDataFromFile")) write_config();
+
+    c.fill_grid(grid);
+    ck(sts_create(&grid, 64, &ctx), "sts_create");
+    max_res_p.assign(N_I, 0.0);
+    max_res_T.assign(N_I, 0.0);
+    max_res_p0.assign(N_I, 0.0);
+    max_res_T0.assign(N_I, 0.0);
+    max_Iter_p_c.assign(N_I, 0u);
+
+    std::vector<double> u, v, p, T, rho;
+    host_fields(u, v, p, T, rho);
+    if (c.flag("ToReadSolvedDataFromFile")) {  // ReadSolvedDataFromFile, S:5734-5835
+      const bool bin = c.flag("ToReadSolvedDataFromBinaryFile");
+      const std::string ext = bin ? ".bin" : ".txt";
+      read_field(path("u" + ext), u.data(), Na, bin);
+      read_field(path("v" + ext), v.data(), Na, bin);
+      read_field(path("p" + ext), p.data(), Na, bin);
+      read_field(path("Temper" + ext), T.data(), Na, bin);
+      for (size_t k = 0; k < Na; k++) rho[k] = mask[k] ? p[k] / T[k] : 0;
+      set(STS_U, u);
+      set(STS_V, v);
+      set(STS_P, p);
+      set(STS_T, T);
+      set(STS_RHO, rho);
+      ck(sts_rho_in_middle_points(ctx), "rho_in_middle_points");
+      std::ifstream itf(path("it_begin.txt"));  // Read_it_FromFile, S:6762-6787
+      if (!(itf >> it_begin)) die("cannot read it_begin.txt");
+      it = it_begin;
+    } else {
+      set(STS_U, u);
+      set(STS_V, v);
+      set(STS_P, p);
+      set(STS_T, T);
+      set(STS_RHO, rho);
+      ck(sts_apply_bodies(ctx), "apply_bodies");  // S:2507
+      if (c.flag("Pressure_BC")) pressure_columns(true);
+      ck(sts_save_previous(ctx), "save_previous");  // S:2838-2845
+      ck(sts_rho_in_middle_points(ctx), "rho_in_middle_points");
+      ck(sts_apply_bodies(ctx), "apply_bodies");
+      ck(sts_boundary_conditions(ctx, STS_BC_VELOCITIES, nullptr), "boundary_conditions");
+      ck(sts_save_previous(ctx), "save_previous");
+      ck(sts_rho_in_middle_points(ctx), "rho_in_middle_points");
+      it_begin = 0;
+      it = it_begin;
+      write_solved();
+    }
+    // S:2870-2930: coefficient arrays start at zero
+    const std::vector<double> zero(Na, 0.0);
+    for (int f : {STS_D_P_C_12, STS_D_P_C_34, STS_APX, STS_APY, STS_BPX, STS_BPY, STS_GAMMA_YF, STS_GAMMA_XF,
+                  STS_FX, STS_FY, STS_U_PSEUDO, STS_V_PSEUDO, STS_D_UX, STS_D_UY, STS_D_VX, STS_D_VY, STS_DTX,
+                  STS_DTY, STS_SCT1, STS_RHO_U, STS_RHO_V})
+      set(f, zero);
+    Iter = 0;
+    Iter_pr1 = Iter;
+    Iter_pr2 = Iter_pr1;
+    if (c.flag("Pressure_BC") && !c.flag("ToReadSolvedDataFromFile")) pressure_columns(false);
+    ck(sts_properties(ctx), "properties");  // S:2948-2953
+    ck(sts_rho_in_middle_points(ctx), "rho_in_middle_points");
+    it = it_begin;
+    it_counter = 0;
+    countinue_time_step = true;
+    ck(sts_save_previous(ctx), "save_previous");  // S:2975-2983
+    scalars();                                    // S:3043-3057
+    ck(sts_rho_in_middle_points(ctx), "rho_in_middle_points");
+    double probe = 0;
+    ck(sts_boundary_conditions(ctx, STS_BC_SLIP_U | STS_BC_SLIP_V | STS_BC_VELOCITIES, &probe), "bc");
+    pressure_begin(probe);
+    ck(sts_boundary_conditions(ctx, STS_BC_JUMP | STS_BC_TEMPERATURE | STS_BC_PRESSURE, nullptr), "bc");
+    ck(sts_fluxes_diffusion(ctx), "fluxes_diffusion");
+
+    const bool fused = (unsigned)c.get("TypeOfNonDimensiolization") == 3;
+    int save_ctr = 0, temp_ctr = 0;
+    const int Nt_save = (int)c.get("Nt_save_solved_data"), Nt_temp = (int)c.get("Nt_save_temp_data");
+    const auto t0 = std::chrono::steady_clock::now();
+    time_solve_start = time(nullptr);
+    do {  // loop 1
+      ck(sts_save_previous(ctx), "save_previous");
+      for (unsigned k = 0; k < Iter; k++) {
+        max_res_p[k] = max_res_T[k] = max_res_p0[k] = max_res_T0[k] = 0;
+        max_Iter_p_c[k] = 0;
+      }
+      it += ht;
+      it_counter++;
+      Iter_pr2 = Iter_pr1;
+      Iter_pr1 = Iter;
+      Iter = 0;
+      continue_iter = true;
+      do {  // loop 2
+        double r[4], r0[2];
+        if (fused) {
+          ck(sts_outer_iteration(ctx, (int)N_I_p_c, r, r0, &probe), "outer_iteration");
+        } else {
+          // cT of the diffusion terms is recomputed after the correction
+          // (S:4347) from the inflow values of the previous iteration.
+          ck(sts_assemble(ctx), "assemble");
+          std::vector<double> rs(2 * N_I_p_c);
+          ck(sts_pt_jacobi(ctx, (int)N_I_p_c, rs.data()), "pt_jacobi");
+          double ruv[2];
+          ck(sts_correct(ctx, ruv), "correct");
+          scalars();
+          ck(sts_boundary_conditions(ctx, STS_BC_SLIP_U | STS_BC_SLIP_V | STS_BC_VELOCITIES, &probe), "bc");
+          ck(sts_boundary_conditions(ctx, STS_BC_JUMP | STS_BC_TEMPERATURE | STS_BC_PRESSURE, nullptr), "bc");
+          ck(sts_fluxes_diffusion(ctx), "fluxes_diffusion");
+          r[0] = ruv[0];
+          r[1] = ruv[1];
+          r[2] = rs[2 * (N_I_p_c - 1)];
+          r[3] = rs[2 * (N_I_p_c - 1) + 1];
+          r0[0] = rs[0];
+          r0[1] = rs[1];
+        }
+        outer_total++;
+        Iter_p_c = N_I_p_c;
+        max_res_u = r[0];
+        max_res_v = r[1];
+        max_res_p[Iter] = r[2];
+        max_res_T[Iter] = r[3];
+        max_res_p0[Iter] = r0[0];
+        max_res_T0[Iter] = r0[1];
+        max_Iter_p_c[Iter] = Iter_p_c;
+        Iter++;
+        if (Iter == 1) {
+          max_res_u0 = max_res_u;
+          max_res_v0 = max_res_v;
+        }
+        const double eV = c.get("MaxError_Velocities"), ep = c.get("MaxError_p_c"), eT = c.get("MaxError_T");
+        const double es = c.get("MaxError_to_stop_program");
+        continue_iter = (Iter < N_I && (eV < max_res_u || eV < max_res_v || ep < max_res_p[Iter - 1] ||
+                                        eT < max_res_T[Iter - 1]));
+        countinue_time_step =
+            (it_counter < Nt && (double)time(nullptr) < time_solve_stop &&
+             ((!(Iter <= 1 && Iter_p_c <= 2)) || es < max_res_u || es < max_res_v || es < max_res_p0[Iter - 1] ||
+              es < max_res_T0[Iter - 1]));
+        pressure_begin(probe);  // S:4480, after the velocity conditions
+        if (c.flag("ToSolveContinuityEquation")) continuity();
+      } while (continue_iter);
+      if (c.flag("use_interpolation_for_u_BC_by_time")) {  // S:8102-8144
+        const double tb = c.get("t_BC_b"), te = c.get("t_BC_e"), ub = c.get("u_BC_xb_t_BC_b"),
+                     ue = c.get("u_BC_xb_t_BC_e");
+        double ug;
+        if (it <= tb) ug = ub;
+        else if (tb < it && it <= te) ug = ((ue - ub) / (te - tb)) * (it - tb) + ub;
+        else ug = ue;
+        c.at("u_gas_nd") = ug;
+        c.at("u_given_xb") = ug;
+        ck(sts_set_inflow_u(ctx, ug), "set_inflow_u");
+      }
+      if (++save_ctr >= Nt_save) {
+        write_solved();
+        if (c.flag("ToSolveContinuityEquation")) continuity();
+        save_ctr = 0;
+      }
+      if (++temp_ctr >= Nt_temp) {
+        write_tmp();
+        write_iters();
+        temp_ctr = 0;
+      }
+    } while (countinue_time_step);
+    const double loop_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    solve_is_finished = Iter <= 1 && Iter_p_c <= 1 && max_res_u < c.get("MaxError_to_stop_program") &&
+                        max_res_v < c.get("MaxError_to_stop_program") &&
+                        max_res_p[Iter - 1] < c.get("MaxError_to_stop_program") &&
+                        max_res_T[Iter - 1] < c.get("MaxError_to_stop_program");
+    c.at("max_time_for_solve") = (double)(time(nullptr) - time_solve_start) / 3600.0;
+    write_solved();
+    if (c.flag("ToSolveContinuityEquation")) continuity();
+    write_tmp();
+    loop_seconds = loop_s;
+    ck(sts_destroy(ctx), "sts_destroy");
+  }
+  double loop_seconds = 0;
+};
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  std::string dir = ".", timing;
+  for (int a = 1; a < argc; a++) {
+    std::string s = argv[a];
+    if (s == "--timing" && a + 1 < argc) timing = argv[++a];
+    else if (s == "-h" || s == "--help") {
+      std::cout << "usage: sts_gpu [case_dir] [--timing file.json]" << std::endl;
+      return 0;
+    } else dir = s;
+  }
+  Driver d;
+  d.dir = dir;
+  std::string err = sts::load_case(dir, d.c);
+  if (!err.empty()) die(err);
+  d.run();
+  if (!timing.empty()) {
+    std::ofstream o(timing);
+    o << std::setprecision(17) << "{\"Nx\": " << d.Nx << ", \"Ny\": " << d.Ny << ", \"steps\": " << d.it_counter
+      << ", \"outer_iterations\": " << d.outer_total << ", \"loop_seconds\": " << d.loop_seconds << "}" << std::endl;
+  }
+  return 0;
+}

@@ -632,10 +632,13 @@ __global__ void __launch_bounds__(BX *BY) k_flux_diff(KGrid g, KFields fl) {
   FLD(FX)[ij] = hy[j] * FLD(RHO_U)[ij] * FLD(U)[ij];
   FLD(FY)[ij] = hx[i] * FLD(RHO_V)[ij] * FLD(V)[ij];
   const bool m00 = m[i_1j_1], m10 = m[ij_1], m01 = m[i_1j], m11 = m[ij];
-  double Gy = 0.0, Gx = 0.0;  // all four cells body: the reference leaves its initial 0
+  // All four cells body: the reference leaves Gamma untouched (S:10477-10543).
+  double Gy = 0.0, Gx = 0.0;
   if (m00 && m10 && m01 && m11) {
     Gy = 0.5 * (Li(sT[i_1j_1], sT[i_1j], hy[jm], hy[j]) + Li(sT[ij_1], sT[ij], hy[jm], hy[j]));
     Gx = 0.5 * (Li(sT[i_1j_1], sT[ij_1], hx[i_1], hx[i]) + Li(sT[i_1j], sT[ij], hx[i_1], hx[i]));
+    FLD(GAMMA_YF)[ij] = Gy;
+    FLD(GAMMA_XF)[ij] = Gx;
   } else if (m00 || m10 || m01 || m11) {
     Gy = 0.5 * (bd(m00 && m01) * Li(sT[i_1j_1], sT[i_1j], hy[jm], hy[j]) + bd(m00 != m01) * sqrt(Twv[i_1j])
                 + bd(m10 && m11) * Li(sT[ij_1], sT[ij], hy[jm], hy[j]) + bd(m10 != m11) * sqrt(Twv[ij])
@@ -643,9 +646,9 @@ __global__ void __launch_bounds__(BX *BY) k_flux_diff(KGrid g, KFields fl) {
     Gx = 0.5 * (bd(m00 && m10) * Li(sT[i_1j_1], sT[ij_1], hx[i_1], hx[i]) + bd(m00 != m10) * sqrt(Twu[ij_1])
                 + bd(m01 && m11) * Li(sT[i_1j], sT[ij], hx[i_1], hx[i]) + bd(m01 != m11) * sqrt(Twu[ij])
                 + bd((!(m00 && m10)) || (!(m01 && m11))) * Li(sqrt(Twv[i_1j]), sqrt(Twv[ij]), hx[i_1], hx[i]));
+    FLD(GAMMA_YF)[ij] = Gy;
+    FLD(GAMMA_XF)[ij] = Gx;
   }
-  FLD(GAMMA_YF)[ij] = Gy;
-  FLD(GAMMA_XF)[ij] = Gx;
   FLD(D_UX)[ij] = cT * sT[i_1j] * (hy[j] / hx[i_1]);
   FLD(D_UY)[ij] = ((m11 && m01) || (m10 && m00))
                       ? cT * Gy * ((hx[i] + hx[i_1]) / (bd(m11 * m01) * hy[j] + bd(m10 * m00) * hy[jm]))
@@ -723,6 +726,15 @@ __global__ void __launch_bounds__(BX *BY) k_set_inflow(KGrid g, KFields fl, doub
   if (i >= g.Nx_u0 || j >= g.Ny) return;
   const unsigned k = (unsigned)i + (unsigned)j * (unsigned)g.Nx;
   if (g.m[k]) FLD(U)[k] = u_in;
+}
+
+// SolveFluxes alone (S:10440-10454), as SolveContinuityEquation calls it.
+__global__ void __launch_bounds__(BX *BY) k_fluxes(KGrid g, KFields fl) {
+  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  if (i >= g.Nx || j >= g.Ny) return;
+  const unsigned ij = (unsigned)i + (unsigned)j * (unsigned)g.Nx;
+  FLD(FX)[ij] = g.hy[j] * FLD(RHO_U)[ij] * FLD(U)[ij];
+  FLD(FY)[ij] = g.hx[i] * FLD(RHO_V)[ij] * FLD(V)[ij];
 }
 
 // S:10621-10646 plus its norms over fluid cells (one partial per block).
@@ -837,6 +849,7 @@ int continuity_blocks(const KGrid &g) {
 }
 hipError_t launch_continuity(const KGrid &g, const KFields &f, double *cs, double *cf, double *partial,
                              hipStream_t s) {
+  hipLaunchKernelGGL(k_fluxes, grid2d(g), dim3(BX, BY), 0, s, g, f);
   hipLaunchKernelGGL(k_continuity, grid2d(g), dim3(BX, BY), 0, s, g, f, cs, cf, partial);
   return hipGetLastError();
 }

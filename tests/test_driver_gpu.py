@@ -1,0 +1,64 @@
+"""End-to-end parity of the GPU program (sts_gpu) with the reference solver:
+the same case directory in, the same files out, byte for byte (FP64)."""
+import importlib
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from casefiles import REF_BIN, have_reference, run_reference
+from golden_cases import RUN_CASES, fresh_input, ref_dir
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "simple-ts_amd", "sts_gpu")
+SAME_BYTES = ["u.bin", "v.bin", "p.bin", "Temper.bin", "vol_inf_fb.bin", "x_f.txt", "y_f.txt", "hx.txt", "hy.txt",
+              "x_v.txt", "y_v.txt", "Tmp.txt", "Iterations_in_loops.txt", "it_begin.txt",
+              "ContinuityEquation.bin", "ContinuityEquation_stationary.bin"]
+
+
+def run_driver(d, timeout=300):
+    print(f"sts_gpu {d}", flush=True)
+    subprocess.run([DRIVER, d], check=True, timeout=timeout, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+
+
+def compare_dirs(mine, ref):
+    bad = []
+    for f in SAME_BYTES:
+        rp = os.path.join(ref, f)
+        if not os.path.exists(rp):
+            continue
+        a = open(os.path.join(mine, f), "rb").read()
+        b = open(rp, "rb").read()
+        if a != b:
+            bad.append(f)
+    # EnteredData.txt: line 44 holds the run's elapsed hours
+    a = open(os.path.join(mine, "EnteredData.txt")).read().split("\n")
+    b = open(os.path.join(ref, "EnteredData.txt")).read().split("\n")
+    if [x for k, x in enumerate(a) if k != 43] != [x for k, x in enumerate(b) if k != 43]:
+        bad.append("EnteredData.txt")
+    return bad
+
+
+@pytest.mark.parametrize("name", RUN_CASES)
+def test_driver_matches_reference_files(name, tmp_path):
+    d = fresh_input(name, tmp_path)
+    run_driver(d)
+    bad = compare_dirs(d, ref_dir(name))
+    assert not bad, f"{name}: files differing from the reference's: {bad}"
+
+
+@pytest.mark.skipif(not have_reference(), reason="oracle/_ref/sts_ref not built")
+def test_driver_matches_live_reference_subsonic_square(tmp_path):
+    """The subsonic confined square on the 240x48 grid for 12 steps, each
+    converged to 1e-8, against the reference run in the same job."""
+    cases = importlib.import_module("simple-ts_amd.cases")
+    mine = cases.confined_square(str(tmp_path / "gpu"), 240, 48, 85, 0.00194, 12)
+    ref = cases.confined_square(str(tmp_path / "ref"), 240, 48, 85, 0.00194, 12)
+    run_reference(ref)
+    run_driver(mine)
+    bad = compare_dirs(mine, ref)
+    assert not bad, f"files differing from the reference's: {bad}"
