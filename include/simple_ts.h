@@ -161,9 +161,17 @@ int sts_continuity(sts_ctx *ctx, double *cont_stat, double *cont, double *norms)
 int sts_outer_iteration(sts_ctx *ctx, int n_pc, double *res, double *res_p0T0,
                         double *p_probe);
 
-/* Wall time of the context's stream work since the last call, in ms, and
-   the number of kernels launched (diagnostics). */
+/* The HIP stream all work of the context is issued on (diagnostics). */
 int sts_stream_info(sts_ctx *ctx, void **hip_stream);
+
+/* Kernel timing (diagnostics, no counterpart in the reference): while
+   enabled, every launch of the chosen kernel is bracketed by HIP events on
+   the context's stream; sts_kernel_time reports the launches and their summed
+   duration since it was enabled.  kernel: STS_K_*, or -1 to disable. */
+enum sts_kernel { STS_K_ASSEMBLE = 0, STS_K_SWEEP, STS_K_CORRECT, STS_K_WALL, STS_K_EDGES, STS_K_FLUX_DIFF,
+                  STS_K_COUNT };
+int sts_time_kernel(sts_ctx *ctx, int kernel);
+int sts_kernel_time(sts_ctx *ctx, long long *launches, double *total_ms);
 
 #ifdef __cplusplus
 }

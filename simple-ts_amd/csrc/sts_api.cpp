@@ -49,6 +49,12 @@ struct sts_ctx {
   unsigned long long *h_red = nullptr;        // pinned host mirror of red (+ probe)
   double *partial = nullptr;                  // continuity norms, one triple per block
   double *cont_stat = nullptr, *cont = nullptr;
+  // kernel timing: event pairs around launches of kernel `timed`
+  int timed = -1;
+  std::vector<hipEvent_t> ev;  // 2 per pending launch
+  int ev_used = 0;
+  long long timed_launches = 0;
+  double timed_ms = 0;
 
   KFields fields() const {
     KFields k;
@@ -58,6 +64,60 @@ struct sts_ctx {
 };
 
 extern "C" const char *sts_last_error(void) { return g_err.c_str(); }
+
+namespace {
+constexpr int EV_POOL = 256;
+
+// Folds the durations of completed event pairs into the totals.
+int drain_events(sts_ctx *c) {
+  if (c->ev_used == 0) return STS_OK;
+  HIP_OK(hipEventSynchronize(c->ev[2 * (c->ev_used - 1) + 1]));
+  for (int k = 0; k < c->ev_used; k++) {
+    float ms = 0;
+    HIP_OK(hipEventElapsedTime(&ms, c->ev[2 * k], c->ev[2 * k + 1]));
+    c->timed_ms += ms;
+    c->timed_launches++;
+  }
+  c->ev_used = 0;
+  return STS_OK;
+}
+
+// Records the start event when kernel `id` is timed; returns the pair slot or -1.
+int timed_begin(sts_ctx *c, int id) {
+  if (c->timed != id) return -1;
+  if (c->ev_used == EV_POOL && drain_events(c)) return -1;
+  const int k = c->ev_used++;
+  (void)hipEventRecord(c->ev[2 * k], c->stream);
+  return k;
+}
+
+void timed_end(sts_ctx *c, int k) {
+  if (k >= 0) (void)hipEventRecord(c->ev[2 * k + 1], c->stream);
+}
+}  // namespace
+
+extern "C" int sts_time_kernel(sts_ctx *c, int kernel) {
+  if (!c || kernel < -1 || kernel >= STS_K_COUNT) return fail(STS_ERR_ARG, "bad kernel id");
+  if (c->ev.empty()) {
+    c->ev.resize(2 * EV_POOL);
+    for (hipEvent_t &e : c->ev) HIP_OK(hipEventCreate(&e));
+  }
+  int rc = drain_events(c);
+  if (rc) return rc;
+  c->timed = kernel;
+  c->timed_launches = 0;
+  c->timed_ms = 0;
+  return STS_OK;
+}
+
+extern "C" int sts_kernel_time(sts_ctx *c, long long *launches, double *total_ms) {
+  if (!c) return fail(STS_ERR_ARG, "null context");
+  int rc = drain_events(c);
+  if (rc) return rc;
+  if (launches) *launches = c->timed_launches;
+  if (total_ms) *total_ms = c->timed_ms;
+  return STS_OK;
+}
 
 static unsigned bc_bits(const sts_grid *g) {
   unsigned b = 0;
@@ -96,6 +156,7 @@ static int check_box(const char *name, const int *b, int N) {
 
 extern "C" int sts_destroy(sts_ctx *c) {
   if (!c) return STS_OK;
+  for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
   for (int q = 0; q < STS_NFIELDS; q++) (void)hipFree(c->f[q]);
   (void)hipFree(c->p_alt);
   (void)hipFree(c->T_alt);
@@ -249,7 +310,9 @@ extern "C" int sts_save_previous(sts_ctx *c) {
 
 extern "C" int sts_assemble(sts_ctx *c) {
   if (!c) return fail(STS_ERR_ARG, "null context");
+  const int t = timed_begin(c, STS_K_ASSEMBLE);
   HIP_OK(launch_assemble(c->g, c->fields(), nullptr, 0, c->stream));
+  timed_end(c, t);
   return STS_OK;
 }
 
@@ -263,8 +326,10 @@ static double bits_to_double(unsigned long long b) {
 static int run_sweeps(sts_ctx *c, int n) {
   for (int s = 0; s < n; s++) {
     KFields kf = c->fields();
+    const int t = timed_begin(c, STS_K_SWEEP);
     HIP_OK(launch_sweep(c->g, kf, c->f[STS_P], c->f[STS_T], c->p_alt, c->T_alt, c->red + 2 + 2 * s,
                         c->red + 3 + 2 * s, c->stream));
+    timed_end(c, t);
     std::swap(c->f[STS_P], c->p_alt);
     std::swap(c->f[STS_T], c->T_alt);
   }
@@ -299,7 +364,9 @@ extern "C" int sts_pt_jacobi(sts_ctx *c, int n, double *res) {
 extern "C" int sts_correct(sts_ctx *c, double *res) {
   if (!c) return fail(STS_ERR_ARG, "null context");
   HIP_OK(hipMemsetAsync(c->red, 0, sizeof(unsigned long long) * 2, c->stream));
+  const int t = timed_begin(c, STS_K_CORRECT);
   HIP_OK(launch_correct(c->g, c->fields(), c->red, c->red + 1, 1, c->stream));
+  timed_end(c, t);
   int rc = read_red(c, 2, false);
   if (rc) return rc;
   if (res) {
@@ -327,8 +394,16 @@ static int enqueue_bcs(sts_ctx *c, int parts, bool probe) {
   // The jump condition reads only T and rho, which the velocity conditions do
   // not touch, so it runs with the slip conditions ahead of the edge copies;
   // the temperature copies still see the wall values' source unchanged.
-  if (wall) HIP_OK(launch_wall(c->g, c->fields(), wall, c->stream));
-  if (edge || probe) HIP_OK(launch_edges(c->g, c->fields(), edge, probe ? c->probe : nullptr, c->stream));
+  if (wall) {
+    const int t = timed_begin(c, STS_K_WALL);
+    HIP_OK(launch_wall(c->g, c->fields(), wall, c->stream));
+    timed_end(c, t);
+  }
+  if (edge || probe) {
+    const int t = timed_begin(c, STS_K_EDGES);
+    HIP_OK(launch_edges(c->g, c->fields(), edge, probe ? c->probe : nullptr, c->stream));
+    timed_end(c, t);
+  }
   return STS_OK;
 }
 
@@ -345,7 +420,9 @@ extern "C" int sts_boundary_conditions(sts_ctx *c, int parts, double *p_probe) {
 
 extern "C" int sts_fluxes_diffusion(sts_ctx *c) {
   if (!c) return fail(STS_ERR_ARG, "null context");
+  const int t = timed_begin(c, STS_K_FLUX_DIFF);
   HIP_OK(launch_flux_diff(c->g, c->fields(), c->stream));
+  timed_end(c, t);
   return STS_OK;
 }
 
@@ -402,12 +479,18 @@ extern "C" int sts_continuity(sts_ctx *c, double *cont_stat, double *cont, doubl
 extern "C" int sts_outer_iteration(sts_ctx *c, int n_pc, double *res, double *res_p0T0, double *p_probe) {
   if (!c || n_pc < 1 || n_pc > MAX_SWEEPS) return fail(STS_ERR_ARG, "N_I_p_c must be 1..62");
   const int nslots = 2 + 2 * n_pc;
+  int t = timed_begin(c, STS_K_ASSEMBLE);
   HIP_OK(launch_assemble(c->g, c->fields(), c->red, nslots, c->stream));
+  timed_end(c, t);
   int rc = run_sweeps(c, n_pc);
   if (rc) return rc;
+  t = timed_begin(c, STS_K_CORRECT);
   HIP_OK(launch_correct(c->g, c->fields(), c->red, c->red + 1, 1, c->stream));
+  timed_end(c, t);
   if ((rc = enqueue_bcs(c, STS_BC_ALL, true))) return rc;
+  t = timed_begin(c, STS_K_FLUX_DIFF);
   HIP_OK(launch_flux_diff(c->g, c->fields(), c->stream));
+  timed_end(c, t);
   if ((rc = read_red(c, nslots, true))) return rc;
   if (res) {
     res[0] = bits_to_double(c->h_red[0]);

@@ -379,6 +379,9 @@ struct Driver {
     ck(sts_fluxes_diffusion(ctx), "fluxes_diffusion");
 
     const bool fused = (unsigned)c.get("TypeOfNonDimensiolization") == 3;
+    auto t_timed = std::chrono::steady_clock::now();
+    long long outer_at_start = 0;
+    if (warmup_steps == 0) start_timing(t_timed, outer_at_start);
     int save_ctr = 0, temp_ctr = 0;
     const int Nt_save = (int)c.get("Nt_save_solved_data"), Nt_temp = (int)c.get("Nt_save_temp_data");
     const auto t0 = std::chrono::steady_clock::now();
@@ -464,8 +467,14 @@ struct Driver {
         write_iters();
         temp_ctr = 0;
       }
+      if (it_counter == warmup_steps) start_timing(t_timed, outer_at_start);
     } while (countinue_time_step);
-    const double loop_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    const auto t_end = std::chrono::steady_clock::now();
+    const double loop_s = std::chrono::duration<double>(t_end - t0).count();
+    timed_seconds = std::chrono::duration<double>(t_end - t_timed).count();
+    timed_steps = it_counter - warmup_steps;
+    timed_outer = outer_total - outer_at_start;
+    if (time_kernel >= 0) ck(sts_kernel_time(ctx, &kernel_launches, &kernel_ms), "sts_kernel_time");
     solve_is_finished = Iter <= 1 && Iter_p_c <= 1 && max_res_u < c.get("MaxError_to_stop_program") &&
                         max_res_v < c.get("MaxError_to_stop_program") &&
                         max_res_p[Iter - 1] < c.get("MaxError_to_stop_program") &&
@@ -478,29 +487,58 @@ struct Driver {
     ck(sts_destroy(ctx), "sts_destroy");
   }
   double loop_seconds = 0;
+  // timing of the steps after the warm-up ones (benchmarking)
+  unsigned warmup_steps = 0;
+  int time_kernel = -1;
+  double timed_seconds = 0, kernel_ms = 0;
+  unsigned timed_steps = 0;
+  long long timed_outer = 0, kernel_launches = 0;
+
+  void start_timing(std::chrono::steady_clock::time_point &t, long long &outer0) {
+    if (time_kernel >= 0) ck(sts_time_kernel(ctx, time_kernel), "sts_time_kernel");
+    t = std::chrono::steady_clock::now();
+    outer0 = outer_total;
+  }
 };
 
 }  // namespace
 
 int main(int argc, char **argv) {
   std::string dir = ".", timing;
+  unsigned warmup = 0;
+  int kernel = -1;
+  const char *knames[] = {"assemble", "sweep", "correct", "wall", "edges", "flux_diff"};
   for (int a = 1; a < argc; a++) {
     std::string s = argv[a];
     if (s == "--timing" && a + 1 < argc) timing = argv[++a];
-    else if (s == "-h" || s == "--help") {
-      std::cout << "usage: sts_gpu [case_dir] [--timing file.json]" << std::endl;
+    else if (s == "--warmup" && a + 1 < argc) warmup = (unsigned)std::atoi(argv[++a]);
+    else if (s == "--time-kernel" && a + 1 < argc) {
+      std::string k = argv[++a];
+      for (int q = 0; q < STS_K_COUNT; q++)
+        if (k == knames[q]) kernel = q;
+      if (kernel < 0) die("unknown kernel " + k);
+    } else if (s == "-h" || s == "--help") {
+      std::cout << "usage: sts_gpu [case_dir] [--timing file.json] [--warmup steps] [--time-kernel "
+                   "assemble|sweep|correct|wall|edges|flux_diff]"
+                << std::endl;
       return 0;
     } else dir = s;
   }
   Driver d;
   d.dir = dir;
+  d.warmup_steps = warmup;
+  d.time_kernel = kernel;
   std::string err = sts::load_case(dir, d.c);
   if (!err.empty()) die(err);
   d.run();
   if (!timing.empty()) {
     std::ofstream o(timing);
     o << std::setprecision(17) << "{\"Nx\": " << d.Nx << ", \"Ny\": " << d.Ny << ", \"steps\": " << d.it_counter
-      << ", \"outer_iterations\": " << d.outer_total << ", \"loop_seconds\": " << d.loop_seconds << "}" << std::endl;
+      << ", \"outer_iterations\": " << d.outer_total << ", \"loop_seconds\": " << d.loop_seconds
+      << ", \"warmup_steps\": " << d.warmup_steps << ", \"timed_steps\": " << d.timed_steps
+      << ", \"timed_outer_iterations\": " << d.timed_outer << ", \"timed_seconds\": " << d.timed_seconds
+      << ", \"kernel\": \"" << (kernel >= 0 ? knames[kernel] : "") << "\", \"kernel_launches\": "
+      << d.kernel_launches << ", \"kernel_ms\": " << d.kernel_ms << "}" << std::endl;
   }
   return 0;
 }
