@@ -161,6 +161,27 @@ int sts_continuity(sts_ctx *ctx, double *cont_stat, double *cont, double *norms)
 int sts_outer_iteration(sts_ctx *ctx, int n_pc, double *res, double *res_p0T0,
                         double *p_probe);
 
+/* Loop-2 control of one time step (S:4406-4450). */
+typedef struct sts_loop_control {
+  int N_I;            /* cap on outer iterations per step */
+  int N_I_p_c;        /* fused T/p sweeps per outer iteration (loop 3) */
+  double tol_uv;      /* MaxError_Velocities */
+  double tol_p;       /* MaxError_p_c */
+  double tol_T;       /* MaxError_T */
+  int batch;          /* outer iterations enqueued before the first host check; 0 = from the last step */
+} sts_loop_control;
+
+/* One time step of loop 1 (S:3230-4608): the prologue S:3233-3239, then
+   outer iterations (as sts_outer_iteration) until the reference's loop-2 test
+   S:4433-4437 stops them or N_I is reached.  The test is evaluated on the GPU
+   after each outer iteration; iterations are enqueued in batches and those
+   past the stopping one exit without work, so the host waits about once per
+   step.  *iterations gets Iter; res_log (may be NULL) gets 6 doubles per outer
+   iteration: max|du|, max|dv|, max|dp|, max|dT| (last sweep), max|dp|,
+   max|dT| (first sweep).  p_probe as in sts_boundary_conditions, for the last
+   outer iteration.  Returns STS_ERR_NONFINITE if a residual is not finite. */
+int sts_time_step(sts_ctx *ctx, const sts_loop_control *lc, int *iterations, double *res_log, double *p_probe);
+
 /* The HIP stream all work of the context is issued on (diagnostics). */
 int sts_stream_info(sts_ctx *ctx, void **hip_stream);
 

@@ -70,6 +70,11 @@ class Grid(ctypes.Structure):
     _fields_ = grid_fields(False)
 
 
+class LoopControl(ctypes.Structure):
+    _fields_ = [("N_I", ctypes.c_int), ("N_I_p_c", ctypes.c_int), ("tol_uv", ctypes.c_double),
+                ("tol_p", ctypes.c_double), ("tol_T", ctypes.c_double), ("batch", ctypes.c_int)]
+
+
 class Problem:
     """A SIMPLE-TS case as the time step sees it: mesh, body mask and table,
     solve boxes, scalars and boundary-condition switches.  Keeps the numpy
@@ -145,6 +150,9 @@ def lib():
             ("sts_fluxes_diffusion", [vp]), ("sts_apply_bodies", [vp]), ("sts_jump_overwrite", [vp]),
             ("sts_set_inflow_u", [vp, c_double]), ("sts_continuity", [vp, _DP, _DP, _DP]),
             ("sts_outer_iteration", [vp, c_int, _DP, _DP, _DP]), ("sts_stream_info", [vp, ctypes.POINTER(vp)]),
+            ("sts_time_step", [vp, ctypes.POINTER(LoopControl), ctypes.POINTER(c_int), _DP, _DP]),
+            ("sts_time_kernel", [vp, c_int]),
+            ("sts_kernel_time", [vp, ctypes.POINTER(ctypes.c_longlong), _DP]),
         ]:
             fn = getattr(L, name)
             fn.argtypes = args
@@ -248,6 +256,15 @@ class Context:
         cs, cf, nr = np.zeros(n), np.zeros(n), np.zeros(3)
         _check(lib().sts_continuity(self.h, _dp(cs), _dp(cf), _dp(nr)), "sts_continuity")
         return cs, cf, nr
+
+    def time_step(self, N_I, N_I_p_c, tol_uv, tol_p, tol_T, batch=0):  # S:3230-4608
+        """One time step with loop 2 controlled on the GPU; returns (Iter,
+        residual log [Iter, 6], p probe)."""
+        lc = LoopControl(N_I, N_I_p_c, tol_uv, tol_p, tol_T, batch)
+        it = ctypes.c_int()
+        log, pr = np.zeros(6 * N_I), np.zeros(1)
+        _check(lib().sts_time_step(self.h, ctypes.byref(lc), ctypes.byref(it), _dp(log), _dp(pr)), "sts_time_step")
+        return it.value, log[:6 * it.value].reshape(-1, 6), float(pr[0])
 
     def outer_iteration(self, n_pc):  # S:3263-4608
         r, r0, pr = np.zeros(4), np.zeros(2), np.zeros(1)

@@ -5,6 +5,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -49,6 +50,12 @@ struct sts_ctx {
   unsigned long long *h_red = nullptr;        // pinned host mirror of red (+ probe)
   double *partial = nullptr;                  // continuity norms, one triple per block
   double *cont_stat = nullptr, *cont = nullptr;
+  // device-side loop 2
+  LoopCtrl *ctrl = nullptr;
+  LoopCtrl *h_ctrl = nullptr;          // pinned mirror
+  unsigned long long *log = nullptr;   // 6 residual maxima per outer iteration
+  int log_cap = 0;
+  int last_iters = 0;
   // kernel timing: event pairs around launches of kernel `timed`
   int timed = -1;
   std::vector<hipEvent_t> ev;  // 2 per pending launch
@@ -66,7 +73,7 @@ struct sts_ctx {
 extern "C" const char *sts_last_error(void) { return g_err.c_str(); }
 
 namespace {
-constexpr int EV_POOL = 256;
+constexpr int EV_POOL = 4096;
 
 // Folds the durations of completed event pairs into the totals.
 int drain_events(sts_ctx *c) {
@@ -171,6 +178,9 @@ extern "C" int sts_destroy(sts_ctx *c) {
   (void)hipFree(c->cont_stat);
   (void)hipFree(c->cont);
   if (c->h_red) (void)hipHostFree(c->h_red);
+  if (c->h_ctrl) (void)hipHostFree(c->h_ctrl);
+  (void)hipFree(c->ctrl);
+  (void)hipFree(c->log);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return STS_OK;
@@ -503,5 +513,98 @@ extern "C" int sts_outer_iteration(sts_ctx *c, int n_pc, double *res, double *re
     res_p0T0[1] = bits_to_double(c->h_red[3]);
   }
   if (p_probe) std::memcpy(p_probe, &c->h_red[NRED], sizeof(double));
+  return STS_OK;
+}
+
+// Enqueues outer iteration k of the current step (iteration parity of the
+// p/T ping-pong follows k * n_pc).
+static int enqueue_outer(sts_ctx *c, int k, const sts_loop_control *lc, double *p0, double *T0, double *p1,
+                         double *T1) {
+  const int n = lc->N_I_p_c;
+  const Gate gate{c->ctrl, k};
+  unsigned long long *lk = c->log + 6 * (size_t)k;
+  auto fields_at = [&](int parity) {
+    KFields kf = c->fields();
+    kf.f[STS_P] = parity ? p1 : p0;
+    kf.f[STS_T] = parity ? T1 : T0;
+    return kf;
+  };
+  const int par0 = (int)(((long long)k * n) & 1);
+  int t = timed_begin(c, STS_K_ASSEMBLE);
+  HIP_OK(launch_assemble(c->g, fields_at(par0), nullptr, 0, c->stream, gate));
+  timed_end(c, t);
+  for (int s = 0; s < n; s++) {
+    const int pin = (par0 + s) & 1;
+    KFields kf = fields_at(pin);
+    unsigned long long *rp = (s == n - 1) ? lk + 2 : nullptr, *rT = (s == n - 1) ? lk + 3 : nullptr;
+    unsigned long long *rp0 = (s == 0) ? lk + 4 : nullptr, *rT0 = (s == 0) ? lk + 5 : nullptr;
+    t = timed_begin(c, STS_K_SWEEP);
+    HIP_OK(launch_sweep(c->g, kf, pin ? p1 : p0, pin ? T1 : T0, pin ? p0 : p1, pin ? T0 : T1, rp, rT, c->stream, rp0,
+                        rT0, gate));
+    timed_end(c, t);
+  }
+  const KFields kf = fields_at((par0 + n) & 1);
+  t = timed_begin(c, STS_K_CORRECT);
+  HIP_OK(launch_correct(c->g, kf, lk + 0, lk + 1, 1, c->stream, gate));
+  timed_end(c, t);
+  t = timed_begin(c, STS_K_WALL);
+  HIP_OK(launch_wall(c->g, kf, STS_BC_SLIP_U | STS_BC_SLIP_V | STS_BC_JUMP, c->stream, gate));
+  timed_end(c, t);
+  const LoopTest lt{c->ctrl, lk, lc->N_I, lc->tol_uv, lc->tol_p, lc->tol_T};
+  t = timed_begin(c, STS_K_EDGES);
+  HIP_OK(launch_edges(c->g, kf, STS_BC_VELOCITIES | STS_BC_TEMPERATURE | STS_BC_PRESSURE, c->probe, c->stream, gate,
+                      lt));
+  timed_end(c, t);
+  t = timed_begin(c, STS_K_FLUX_DIFF);
+  HIP_OK(launch_flux_diff(c->g, kf, c->stream, gate));
+  timed_end(c, t);
+  return STS_OK;
+}
+
+extern "C" int sts_time_step(sts_ctx *c, const sts_loop_control *lc, int *iterations, double *res_log,
+                             double *p_probe) {
+  if (!c || !lc || lc->N_I < 1 || lc->N_I_p_c < 1 || lc->N_I_p_c > MAX_SWEEPS)
+    return fail(STS_ERR_ARG, "bad loop control");
+  if (lc->N_I > c->log_cap) {
+    (void)hipFree(c->log);
+    c->log = nullptr;
+    HIP_OK(hipMalloc(&c->log, sizeof(unsigned long long) * 6 * (size_t)lc->N_I));
+    c->log_cap = lc->N_I;
+  }
+  if (!c->ctrl) {
+    HIP_OK(hipMalloc(&c->ctrl, sizeof(LoopCtrl)));
+    HIP_OK(hipHostMalloc(&c->h_ctrl, sizeof(LoopCtrl), hipHostMallocDefault));
+  }
+  int rc = sts_save_previous(c);
+  if (rc) return rc;
+  HIP_OK(launch_loop_open(c->ctrl, c->log, 6 * lc->N_I, c->stream));
+  double *const p0 = c->f[STS_P], *const T0 = c->f[STS_T], *const p1 = c->p_alt, *const T1 = c->T_alt;
+  int k = 0, batch = lc->batch > 0 ? lc->batch : (c->last_iters > 2 ? c->last_iters - 1 : 1);
+  int stop = -1;
+  while (stop < 0) {
+    const int end = std::min(lc->N_I, k + batch);
+    for (; k < end; k++)
+      if ((rc = enqueue_outer(c, k, lc, p0, T0, p1, T1))) return rc;
+    HIP_OK(hipMemcpyAsync(c->h_ctrl, c->ctrl, sizeof(LoopCtrl), hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    if (c->h_ctrl->stop_at != 0x7fffffff) stop = c->h_ctrl->stop_at;
+    else if (k >= lc->N_I) return fail(STS_ERR_ARG, "loop 2 did not stop at N_I");  // the test always stops there
+    batch = 2;
+  }
+  const int iters = stop + 1;
+  c->last_iters = iters;
+  if (((long long)iters * lc->N_I_p_c) & 1) {  // odd number of sweeps in total: result in the partner buffers
+    std::swap(c->f[STS_P], c->p_alt);
+    std::swap(c->f[STS_T], c->T_alt);
+  }
+  std::vector<unsigned long long> lg(6 * (size_t)iters);
+  HIP_OK(hipMemcpyAsync(lg.data(), c->log, sizeof(unsigned long long) * lg.size(), hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipMemcpyAsync(&c->h_red[NRED], c->probe, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  if (iterations) *iterations = iters;
+  if (res_log)
+    for (size_t q = 0; q < lg.size(); q++) res_log[q] = bits_to_double(lg[q]);
+  if (p_probe) std::memcpy(p_probe, &c->h_red[NRED], sizeof(double));
+  if (c->h_ctrl->nonfinite) return fail(STS_ERR_NONFINITE, "non-finite residual in outer iteration " + std::to_string(iters));
   return STS_OK;
 }

@@ -40,6 +40,30 @@ struct KFields {
   double *f[STS_NFIELDS];
 };
 
+// Device-side loop-2 control (S:4406-4450 evaluated on the GPU): the outer
+// iterations of one time step are enqueued ahead; iteration k runs only while
+// k <= stop_at, and the edge kernel of iteration k ends the loop by setting
+// stop_at = k when the reference's test says so.
+struct LoopCtrl {
+  int stop_at;    // last outer iteration (0-based) to run, INT_MAX while open
+  int nonfinite;  // a residual was not finite
+};
+
+struct Gate {
+  const LoopCtrl *ctrl;  // nullptr: always run
+  int k;                 // outer-iteration index of this launch
+};
+
+__device__ __forceinline__ bool gated_out(const Gate &g) { return g.ctrl && g.k > g.ctrl->stop_at; }
+
+// Loop-2 test of iteration k (S:4433-4437) from its residual log entry.
+struct LoopTest {
+  LoopCtrl *ctrl;
+  const unsigned long long *log_k;  // max|du|, max|dv|, max|dp|, max|dT| (last sweep), first-sweep dp, dT
+  int N_I;
+  double tol_uv, tol_p, tol_T;
+};
+
 // Clamped neighbour indices of cell (i, j).
 struct Nbr {
   unsigned i, j, ij, i_1, i_2, i1, i_1j, i1j, ij_1, ij1, i_1j_1, i_1j1, i1j_1, i1j1, i_2j;

@@ -386,8 +386,11 @@ struct Driver {
     const int Nt_save = (int)c.get("Nt_save_solved_data"), Nt_temp = (int)c.get("Nt_save_temp_data");
     const auto t0 = std::chrono::steady_clock::now();
     time_solve_start = time(nullptr);
+    const double eV = c.get("MaxError_Velocities"), ep = c.get("MaxError_p_c"), eT = c.get("MaxError_T");
+    const double es = c.get("MaxError_to_stop_program");
+    std::vector<double> lg(6 * (size_t)N_I);
     do {  // loop 1
-      ck(sts_save_previous(ctx), "save_previous");
+      if (!fused) ck(sts_save_previous(ctx), "save_previous");
       for (unsigned k = 0; k < Iter; k++) {
         max_res_p[k] = max_res_T[k] = max_res_p0[k] = max_res_T0[k] = 0;
         max_Iter_p_c[k] = 0;
@@ -398,11 +401,34 @@ struct Driver {
       Iter_pr1 = Iter;
       Iter = 0;
       continue_iter = true;
-      do {  // loop 2
-        double r[4], r0[2];
-        if (fused) {
-          ck(sts_outer_iteration(ctx, (int)N_I_p_c, r, r0, &probe), "outer_iteration");
-        } else {
+      if (fused) {
+        // loop 2 with the test on the GPU (same test, same order of work)
+        sts_loop_control lc{(int)N_I, (int)N_I_p_c, eV, ep, eT, 0};
+        int iters = 0;
+        ck(sts_time_step(ctx, &lc, &iters, lg.data(), &probe), "time_step");
+        outer_total += iters;
+        Iter_p_c = N_I_p_c;
+        for (int k = 0; k < iters; k++) {
+          max_res_p[k] = lg[6 * k + 2];
+          max_res_T[k] = lg[6 * k + 3];
+          max_res_p0[k] = lg[6 * k + 4];
+          max_res_T0[k] = lg[6 * k + 5];
+          max_Iter_p_c[k] = Iter_p_c;
+        }
+        Iter = (unsigned)iters;
+        max_res_u = lg[6 * (iters - 1) + 0];
+        max_res_v = lg[6 * (iters - 1) + 1];
+        max_res_u0 = lg[0];
+        max_res_v0 = lg[1];
+        countinue_time_step =
+            (it_counter < Nt && (double)time(nullptr) < time_solve_stop &&
+             ((!(Iter <= 1 && Iter_p_c <= 2)) || es < max_res_u || es < max_res_v || es < max_res_p0[Iter - 1] ||
+              es < max_res_T0[Iter - 1]));
+        pressure_begin(probe);
+        if (c.flag("ToSolveContinuityEquation")) continuity();
+      } else {
+        do {  // loop 2
+          double r[4], r0[2];
           // cT of the diffusion terms is recomputed after the correction
           // (S:4347) from the inflow values of the previous iteration.
           ck(sts_assemble(ctx), "assemble");
@@ -420,32 +446,30 @@ struct Driver {
           r[3] = rs[2 * (N_I_p_c - 1) + 1];
           r0[0] = rs[0];
           r0[1] = rs[1];
-        }
-        outer_total++;
-        Iter_p_c = N_I_p_c;
-        max_res_u = r[0];
-        max_res_v = r[1];
-        max_res_p[Iter] = r[2];
-        max_res_T[Iter] = r[3];
-        max_res_p0[Iter] = r0[0];
-        max_res_T0[Iter] = r0[1];
-        max_Iter_p_c[Iter] = Iter_p_c;
-        Iter++;
-        if (Iter == 1) {
-          max_res_u0 = max_res_u;
-          max_res_v0 = max_res_v;
-        }
-        const double eV = c.get("MaxError_Velocities"), ep = c.get("MaxError_p_c"), eT = c.get("MaxError_T");
-        const double es = c.get("MaxError_to_stop_program");
-        continue_iter = (Iter < N_I && (eV < max_res_u || eV < max_res_v || ep < max_res_p[Iter - 1] ||
-                                        eT < max_res_T[Iter - 1]));
-        countinue_time_step =
-            (it_counter < Nt && (double)time(nullptr) < time_solve_stop &&
-             ((!(Iter <= 1 && Iter_p_c <= 2)) || es < max_res_u || es < max_res_v || es < max_res_p0[Iter - 1] ||
-              es < max_res_T0[Iter - 1]));
-        pressure_begin(probe);  // S:4480, after the velocity conditions
-        if (c.flag("ToSolveContinuityEquation")) continuity();
-      } while (continue_iter);
+          outer_total++;
+          Iter_p_c = N_I_p_c;
+          max_res_u = r[0];
+          max_res_v = r[1];
+          max_res_p[Iter] = r[2];
+          max_res_T[Iter] = r[3];
+          max_res_p0[Iter] = r0[0];
+          max_res_T0[Iter] = r0[1];
+          max_Iter_p_c[Iter] = Iter_p_c;
+          Iter++;
+          if (Iter == 1) {
+            max_res_u0 = max_res_u;
+            max_res_v0 = max_res_v;
+          }
+          continue_iter = (Iter < N_I && (eV < max_res_u || eV < max_res_v || ep < max_res_p[Iter - 1] ||
+                                          eT < max_res_T[Iter - 1]));
+          countinue_time_step =
+              (it_counter < Nt && (double)time(nullptr) < time_solve_stop &&
+               ((!(Iter <= 1 && Iter_p_c <= 2)) || es < max_res_u || es < max_res_v ||
+                es < max_res_p0[Iter - 1] || es < max_res_T0[Iter - 1]));
+          pressure_begin(probe);  // S:4480, after the velocity conditions
+          if (c.flag("ToSolveContinuityEquation")) continuity();
+        } while (continue_iter);
+      }
       if (c.flag("use_interpolation_for_u_BC_by_time")) {  // S:8102-8144
         const double tb = c.get("t_BC_b"), te = c.get("t_BC_e"), ub = c.get("u_BC_xb_t_BC_b"),
                      ue = c.get("u_BC_xb_t_BC_e");

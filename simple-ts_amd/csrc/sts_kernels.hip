@@ -34,10 +34,13 @@ __device__ __forceinline__ unsigned long long wave_max(unsigned long long v) {
   return v;
 }
 
-// Block-wide max of up to two values, one atomicMax per block and value.
+// Block-wide max of two values, one atomicMax per block and destination;
+// destinations may be null, and a second pair (dst2, dst3) receives the same
+// maxima (the first and the last sweep of a single-sweep loop 3 coincide).
 template <int NV>
 __device__ __forceinline__ void block_max_atomic(unsigned long long (&v)[NV], unsigned long long *dst0,
-                                                 unsigned long long *dst1) {
+                                                 unsigned long long *dst1, unsigned long long *dst2 = nullptr,
+                                                 unsigned long long *dst3 = nullptr) {
   __shared__ unsigned long long sm[NV][16];
   const int tid = threadIdx.x + threadIdx.y * blockDim.x;
   const int lane = tid & 63, wid = tid >> 6;
@@ -54,7 +57,9 @@ __device__ __forceinline__ void block_max_atomic(unsigned long long (&v)[NV], un
       unsigned long long b = 0;
       for (int q = 0; q < nw; q++) b = (sm[k][q] > b) ? sm[k][q] : b;
       unsigned long long *d = (k == 0) ? dst0 : dst1;
-      if (b) atomicMax(d, b);
+      unsigned long long *e = (k == 0) ? dst2 : dst3;
+      if (b && d) atomicMax(d, b);
+      if (b && e) atomicMax(e, b);
     }
   }
 }
@@ -161,7 +166,9 @@ __device__ __forceinline__ void cell_energy_source(const KGrid &g, const KFields
                   + p_pr[ij] * hx[i] * hy[j];
 }
 
-__global__ void __launch_bounds__(BX *BY) k_assemble(KGrid g, KFields fl, unsigned long long *red, int nred) {
+__global__ void __launch_bounds__(BX *BY) k_assemble(KGrid g, KFields fl, unsigned long long *red, int nred,
+                                                     Gate gate) {
+  if (gated_out(gate)) return;
   const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
   if (red && blockIdx.x == 0 && blockIdx.y == 0) {
     const int t = threadIdx.x + threadIdx.y * BX;
@@ -184,7 +191,9 @@ __global__ void __launch_bounds__(BX *BY) k_assemble(KGrid g, KFields fl, unsign
 __global__ void __launch_bounds__(BX *BY) k_sweep(KGrid g, KFields fl, const double *__restrict__ Po,
                                                   const double *__restrict__ To, double *__restrict__ Pn,
                                                   double *__restrict__ Tn, unsigned long long *red_p,
-                                                  unsigned long long *red_T) {
+                                                  unsigned long long *red_T, unsigned long long *red_p2,
+                                                  unsigned long long *red_T2, Gate gate) {
+  if (gated_out(gate)) return;
   const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
   unsigned long long r[2] = {0ull, 0ull};
   if (i < g.Nx && j < g.Ny) {
@@ -228,7 +237,7 @@ __global__ void __launch_bounds__(BX *BY) k_sweep(KGrid g, KFields fl, const dou
       Tn[ij] = To[ij];
     }
   }
-  block_max_atomic<2>(r, red_p, red_T);
+  if (red_p || red_T || red_p2 || red_T2) block_max_atomic<2>(r, red_p, red_T, red_p2, red_T2);
 }
 
 // ---------------------------------------------------------------------------
@@ -242,7 +251,8 @@ __device__ __forceinline__ double rho_of(const uint8_t *m, const double *p, cons
 }
 
 __global__ void __launch_bounds__(BX *BY) k_correct(KGrid g, KFields fl, unsigned long long *red_u,
-                                                    unsigned long long *red_v, int do_correct) {
+                                                    unsigned long long *red_v, int do_correct, Gate gate) {
+  if (gated_out(gate)) return;
   const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
   unsigned long long r[2] = {0ull, 0ull};
   if (i < g.Nx && j < g.Ny) {
@@ -415,7 +425,8 @@ __device__ __forceinline__ double wall_T(const KGrid &g, uint32_t k, double T_fl
   return b->T_body;
 }
 
-__global__ void __launch_bounds__(BX *BY) k_wall(KGrid g, KFields fl, int parts) {
+__global__ void __launch_bounds__(BX *BY) k_wall(KGrid g, KFields fl, int parts, Gate gate) {
+  if (gated_out(gate)) return;
   const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
   if (i >= g.Nx || j >= g.Ny) return;
   const unsigned Nx = (unsigned)g.Nx, Ny = (unsigned)g.Ny;
@@ -517,7 +528,25 @@ __device__ __forceinline__ void edge_y(const KGrid &g, double *a, int b0, int b1
   }
 }
 
-__global__ void __launch_bounds__(EDGE_THREADS) k_edges(KGrid g, KFields fl, int parts, double *probe) {
+__global__ void __launch_bounds__(EDGE_THREADS) k_edges(KGrid g, KFields fl, int parts, double *probe, Gate gate,
+                                                        LoopTest lt) {
+  if (gated_out(gate)) return;
+  if (lt.ctrl && threadIdx.x == 0) {
+    // S:4433-4437 for iteration gate.k (Iter = k + 1 after the increment)
+    double r[4];
+    bool fin = true;
+    for (int q = 0; q < 4; q++) {
+      r[q] = __longlong_as_double((long long)lt.log_k[q]);
+      fin = fin && isfinite(r[q]);
+    }
+    bool cont = (gate.k + 1 < lt.N_I) &&
+                (lt.tol_uv < r[0] || lt.tol_uv < r[1] || lt.tol_p < r[2] || lt.tol_T < r[3]);
+    if (!fin) {
+      lt.ctrl->nonfinite = 1;
+      cont = false;
+    }
+    if (!cont) lt.ctrl->stop_at = gate.k;
+  }
   const int Nx = g.Nx, Ny = g.Ny;
   const uint8_t *m = g.m;
   const unsigned bc = g.bc;
@@ -620,7 +649,8 @@ __global__ void __launch_bounds__(EDGE_THREADS) k_edges(KGrid g, KFields fl, int
 // A10 + A11: fluxes (S:10440-10454) and diffusion coefficients
 // (S:10457-10618) over all cells.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(BX *BY) k_flux_diff(KGrid g, KFields fl) {
+__global__ void __launch_bounds__(BX *BY) k_flux_diff(KGrid g, KFields fl, Gate gate) {
+  if (gated_out(gate)) return;
   const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
   if (i >= g.Nx || j >= g.Ny) return;
   const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny);
@@ -728,6 +758,16 @@ __global__ void __launch_bounds__(BX *BY) k_set_inflow(KGrid g, KFields fl, doub
   if (g.m[k]) FLD(U)[k] = u_in;
 }
 
+// Opens the loop-2 control of a time step and clears its residual log.
+__global__ void k_loop_open(LoopCtrl *ctrl, unsigned long long *log, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0) {
+    ctrl->stop_at = 0x7fffffff;
+    ctrl->nonfinite = 0;
+  }
+  for (int q = t; q < n; q += gridDim.x * blockDim.x) log[q] = 0ull;
+}
+
 // SolveFluxes alone (S:10440-10454), as SolveContinuityEquation calls it.
 __global__ void __launch_bounds__(BX *BY) k_fluxes(KGrid g, KFields fl) {
   const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
@@ -789,18 +829,20 @@ __global__ void __launch_bounds__(BX *BY) k_continuity(KGrid g, KFields fl, doub
 
 static inline dim3 grid2d(const KGrid &g) { return dim3((g.Nx + BX - 1) / BX, (g.Ny + BY - 1) / BY); }
 
-hipError_t launch_assemble(const KGrid &g, const KFields &f, unsigned long long *red, int nred, hipStream_t s) {
-  hipLaunchKernelGGL(k_assemble, grid2d(g), dim3(BX, BY), 0, s, g, f, red, nred);
+hipError_t launch_assemble(const KGrid &g, const KFields &f, unsigned long long *red, int nred, hipStream_t s,
+                           Gate gate) {
+  hipLaunchKernelGGL(k_assemble, grid2d(g), dim3(BX, BY), 0, s, g, f, red, nred, gate);
   return hipGetLastError();
 }
 hipError_t launch_sweep(const KGrid &g, const KFields &f, const double *Po, const double *To, double *Pn, double *Tn,
-                        unsigned long long *rp, unsigned long long *rT, hipStream_t s) {
-  hipLaunchKernelGGL(k_sweep, grid2d(g), dim3(BX, BY), 0, s, g, f, Po, To, Pn, Tn, rp, rT);
+                        unsigned long long *rp, unsigned long long *rT, hipStream_t s, unsigned long long *rp2,
+                        unsigned long long *rT2, Gate gate) {
+  hipLaunchKernelGGL(k_sweep, grid2d(g), dim3(BX, BY), 0, s, g, f, Po, To, Pn, Tn, rp, rT, rp2, rT2, gate);
   return hipGetLastError();
 }
 hipError_t launch_correct(const KGrid &g, const KFields &f, unsigned long long *ru, unsigned long long *rv,
-                          int do_correct, hipStream_t s) {
-  hipLaunchKernelGGL(k_correct, grid2d(g), dim3(BX, BY), 0, s, g, f, ru, rv, do_correct);
+                          int do_correct, hipStream_t s, Gate gate) {
+  hipLaunchKernelGGL(k_correct, grid2d(g), dim3(BX, BY), 0, s, g, f, ru, rv, do_correct, gate);
   return hipGetLastError();
 }
 hipError_t launch_properties(const KGrid &g, const KFields &f, hipStream_t s) {
@@ -811,16 +853,17 @@ hipError_t launch_rho_mid(const KGrid &g, const KFields &f, hipStream_t s) {
   hipLaunchKernelGGL(k_rho_mid, grid2d(g), dim3(BX, BY), 0, s, g, f);
   return hipGetLastError();
 }
-hipError_t launch_wall(const KGrid &g, const KFields &f, int parts, hipStream_t s) {
-  hipLaunchKernelGGL(k_wall, grid2d(g), dim3(BX, BY), 0, s, g, f, parts);
+hipError_t launch_wall(const KGrid &g, const KFields &f, int parts, hipStream_t s, Gate gate) {
+  hipLaunchKernelGGL(k_wall, grid2d(g), dim3(BX, BY), 0, s, g, f, parts, gate);
   return hipGetLastError();
 }
-hipError_t launch_edges(const KGrid &g, const KFields &f, int parts, double *probe, hipStream_t s) {
-  hipLaunchKernelGGL(k_edges, dim3(1), dim3(EDGE_THREADS), 0, s, g, f, parts, probe);
+hipError_t launch_edges(const KGrid &g, const KFields &f, int parts, double *probe, hipStream_t s, Gate gate,
+                        LoopTest lt) {
+  hipLaunchKernelGGL(k_edges, dim3(1), dim3(EDGE_THREADS), 0, s, g, f, parts, probe, gate, lt);
   return hipGetLastError();
 }
-hipError_t launch_flux_diff(const KGrid &g, const KFields &f, hipStream_t s) {
-  hipLaunchKernelGGL(k_flux_diff, grid2d(g), dim3(BX, BY), 0, s, g, f);
+hipError_t launch_flux_diff(const KGrid &g, const KFields &f, hipStream_t s, Gate gate) {
+  hipLaunchKernelGGL(k_flux_diff, grid2d(g), dim3(BX, BY), 0, s, g, f, gate);
   return hipGetLastError();
 }
 hipError_t launch_apply_bodies(const KGrid &g, const KFields &f, hipStream_t s) {
@@ -841,6 +884,10 @@ hipError_t launch_copy5(const double *const *src, double *const *dst, int n, hip
 hipError_t launch_set_inflow(const KGrid &g, const KFields &f, double u_in, hipStream_t s) {
   dim3 gr((g.Nx_u0 + BX - 1) / BX > 0 ? (g.Nx_u0 + BX - 1) / BX : 1, (g.Ny + BY - 1) / BY);
   hipLaunchKernelGGL(k_set_inflow, gr, dim3(BX, BY), 0, s, g, f, u_in);
+  return hipGetLastError();
+}
+hipError_t launch_loop_open(LoopCtrl *ctrl, unsigned long long *log, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_loop_open, dim3(16), dim3(256), 0, s, ctrl, log, n);
   return hipGetLastError();
 }
 int continuity_blocks(const KGrid &g) {

@@ -62,3 +62,18 @@ def test_driver_matches_live_reference_subsonic_square(tmp_path):
     run_driver(mine)
     bad = compare_dirs(mine, ref)
     assert not bad, f"files differing from the reference's: {bad}"
+
+
+@pytest.mark.skipif(not have_reference(), reason="oracle/_ref/sts_ref not built")
+@pytest.mark.parametrize("n_pc", [1, 3])
+def test_driver_odd_sweep_counts_match_live_reference(n_pc, tmp_path):
+    """Odd N_I_p_c leaves each outer iteration's p/T in the partner buffers;
+    the device-side loop must follow that parity."""
+    cases = importlib.import_module("simple-ts_amd.cases")
+    kw = dict(N_I=400, N_I_p_c=n_pc, extra=dict(Nt_save_temp_data=1))
+    mine = cases.confined_square(str(tmp_path / "gpu"), 60, 14, 85, 0.00194, 4, **kw)
+    ref = cases.confined_square(str(tmp_path / "ref"), 60, 14, 85, 0.00194, 4, **kw)
+    run_reference(ref)
+    run_driver(mine)
+    bad = compare_dirs(mine, ref)
+    assert not bad, f"files differing from the reference's: {bad}"
