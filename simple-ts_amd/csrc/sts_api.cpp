@@ -5,6 +5,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <algorithm>
 #include <cstring>
 #include <string>
@@ -56,6 +57,7 @@ struct sts_ctx {
   unsigned long long *log = nullptr;   // 6 residual maxima per outer iteration
   int log_cap = 0;
   int last_iters = 0;
+  bool fuse_sweeps = true;  // pairs of loop-3 sweeps in one k_sweep2 pass
   // kernel timing: event pairs around launches of kernel `timed`
   int timed = -1;
   std::vector<hipEvent_t> ev;  // 2 per pending launch
@@ -250,6 +252,7 @@ extern "C" int sts_create(const sts_grid *gi, int precision, sts_ctx **out) {
   c->Nx = gi->Nx;
   c->Ny = gi->Ny;
   c->Na = (int)Na;
+  if (const char *e = std::getenv("STS_FUSED_SWEEPS")) c->fuse_sweeps = std::atoi(e) != 0;
   int arc = alloc_state(c, gi, mask);
   if (arc) {
     sts_destroy(c);
@@ -332,13 +335,21 @@ static double bits_to_double(unsigned long long b) {
   return d;
 }
 
-// n sweeps reading slot base 2+2s; pointer identity of p/T follows the parity.
+// n sweeps, residuals of sweep s in slots 2+2s, 3+2s; pairs of sweeps run
+// fused in one pass.  Each pass leaves its result in the partner buffers.
 static int run_sweeps(sts_ctx *c, int n) {
-  for (int s = 0; s < n; s++) {
+  for (int s = 0; s < n;) {
     KFields kf = c->fields();
     const int t = timed_begin(c, STS_K_SWEEP);
-    HIP_OK(launch_sweep(c->g, kf, c->f[STS_P], c->f[STS_T], c->p_alt, c->T_alt, c->red + 2 + 2 * s,
-                        c->red + 3 + 2 * s, c->stream));
+    if (c->fuse_sweeps && s + 1 < n) {
+      HIP_OK(launch_sweep2(c->g, kf, c->f[STS_P], c->f[STS_T], c->p_alt, c->T_alt, c->red + 2 + 2 * s,
+                           c->red + 3 + 2 * s, c->red + 4 + 2 * s, c->red + 5 + 2 * s, c->stream));
+      s += 2;
+    } else {
+      HIP_OK(launch_sweep(c->g, kf, c->f[STS_P], c->f[STS_T], c->p_alt, c->T_alt, c->red + 2 + 2 * s,
+                          c->red + 3 + 2 * s, c->stream));
+      s += 1;
+    }
     timed_end(c, t);
     std::swap(c->f[STS_P], c->p_alt);
     std::swap(c->f[STS_T], c->T_alt);
@@ -516,8 +527,11 @@ extern "C" int sts_outer_iteration(sts_ctx *c, int n_pc, double *res, double *re
   return STS_OK;
 }
 
-// Enqueues outer iteration k of the current step (iteration parity of the
-// p/T ping-pong follows k * n_pc).
+// Buffer flips (passes) of one loop 3 of n sweeps.
+static int sweep_passes(const sts_ctx *c, int n) { return c->fuse_sweeps ? n / 2 + n % 2 : n; }
+
+// Enqueues outer iteration k of the current step; the p/T buffer parity at
+// its start follows the passes of the iterations before it.
 static int enqueue_outer(sts_ctx *c, int k, const sts_loop_control *lc, double *p0, double *T0, double *p1,
                          double *T1) {
   const int n = lc->N_I_p_c;
@@ -529,21 +543,28 @@ static int enqueue_outer(sts_ctx *c, int k, const sts_loop_control *lc, double *
     kf.f[STS_T] = parity ? T1 : T0;
     return kf;
   };
-  const int par0 = (int)(((long long)k * n) & 1);
+  int par = (int)(((long long)k * sweep_passes(c, n)) & 1);
   int t = timed_begin(c, STS_K_ASSEMBLE);
-  HIP_OK(launch_assemble(c->g, fields_at(par0), nullptr, 0, c->stream, gate));
+  HIP_OK(launch_assemble(c->g, fields_at(par), nullptr, 0, c->stream, gate));
   timed_end(c, t);
-  for (int s = 0; s < n; s++) {
-    const int pin = (par0 + s) & 1;
-    KFields kf = fields_at(pin);
-    unsigned long long *rp = (s == n - 1) ? lk + 2 : nullptr, *rT = (s == n - 1) ? lk + 3 : nullptr;
-    unsigned long long *rp0 = (s == 0) ? lk + 4 : nullptr, *rT0 = (s == 0) ? lk + 5 : nullptr;
+  for (int s = 0; s < n;) {
+    KFields kf = fields_at(par);
+    double *Pi = par ? p1 : p0, *Ti = par ? T1 : T0, *Po = par ? p0 : p1, *To = par ? T0 : T1;
     t = timed_begin(c, STS_K_SWEEP);
-    HIP_OK(launch_sweep(c->g, kf, pin ? p1 : p0, pin ? T1 : T0, pin ? p0 : p1, pin ? T0 : T1, rp, rT, c->stream, rp0,
-                        rT0, gate));
+    if (c->fuse_sweeps && s + 1 < n) {
+      // sweep s (first?) and s+1 (last?)
+      HIP_OK(launch_sweep2(c->g, kf, Pi, Ti, Po, To, s == 0 ? lk + 4 : nullptr, s == 0 ? lk + 5 : nullptr,
+                           s + 1 == n - 1 ? lk + 2 : nullptr, s + 1 == n - 1 ? lk + 3 : nullptr, c->stream, gate));
+      s += 2;
+    } else {
+      HIP_OK(launch_sweep(c->g, kf, Pi, Ti, Po, To, s == n - 1 ? lk + 2 : nullptr, s == n - 1 ? lk + 3 : nullptr,
+                          c->stream, s == 0 ? lk + 4 : nullptr, s == 0 ? lk + 5 : nullptr, gate));
+      s += 1;
+    }
     timed_end(c, t);
+    par ^= 1;
   }
-  const KFields kf = fields_at((par0 + n) & 1);
+  const KFields kf = fields_at(par);
   t = timed_begin(c, STS_K_CORRECT);
   HIP_OK(launch_correct(c->g, kf, lk + 0, lk + 1, 1, c->stream, gate));
   timed_end(c, t);
@@ -593,7 +614,7 @@ extern "C" int sts_time_step(sts_ctx *c, const sts_loop_control *lc, int *iterat
   }
   const int iters = stop + 1;
   c->last_iters = iters;
-  if (((long long)iters * lc->N_I_p_c) & 1) {  // odd number of sweeps in total: result in the partner buffers
+  if (((long long)iters * sweep_passes(c, lc->N_I_p_c)) & 1) {  // odd number of passes: result in the partners
     std::swap(c->f[STS_P], c->p_alt);
     std::swap(c->f[STS_T], c->T_alt);
   }

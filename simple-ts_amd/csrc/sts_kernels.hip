@@ -45,6 +45,7 @@ __device__ __forceinline__ void block_max_atomic(unsigned long long (&v)[NV], un
   const int tid = threadIdx.x + threadIdx.y * blockDim.x;
   const int lane = tid & 63, wid = tid >> 6;
   const int nw = (blockDim.x * blockDim.y + 63) >> 6;
+  __syncthreads();  // a previous call's reader may still be reading sm
 #pragma unroll
   for (int k = 0; k < NV; k++) {
     unsigned long long w = wave_max(v[k]);
@@ -183,11 +184,58 @@ __global__ void __launch_bounds__(BX *BY) k_assemble(KGrid g, KFields fl, unsign
 }
 
 // ---------------------------------------------------------------------------
-// A5: one fused T/p Jacobi sweep (S:3805-4027).  Reads (Po, To), writes
-// (Pn, Tn) on every cell: the update set gets the new values, every other
-// cell is copied through, which restates the loop-3 entry copy S:3763-3764
-// and the pointer swap S:3792-3795 without a separate pass.
+// A5: fused T/p Jacobi sweep (S:3805-4027).
 // ---------------------------------------------------------------------------
+// Old p and T of a cell and of its four (clamped) neighbours.
+struct PTStencil {
+  double Pc, Tc, Pw, Tw, Pe, Te, Ps, Ts, Pn, Tn;
+};
+
+// The reference's T update (S:3837-3857) then p update (S:3884-3895) of one
+// cell of the update set, with the new T in the p denominator (S:3894).  A
+// body neighbour contributes the fluid temperature on its wall (S:3838-3848:
+// m*T_old + !m*T_wall with T_wall finite, i.e. a select).
+__device__ __forceinline__ void pt_update(const KGrid &g, const KFields &fl, const Nbr &n, const PTStencil &x,
+                                          double &pn, double &tn) {
+  const unsigned ij = n.ij, i1j = n.i1j, ij1 = n.ij1;
+  const uint8_t *m = g.m;
+  const double *Fx = FLD(FX), *Fy = FLD(FY), *DTx = FLD(DTX), *DTy = FLD(DTY), *u = FLD(U), *v = FLD(V);
+  const double *apx = FLD(APX), *apy = FLD(APY), *bpx = FLD(BPX), *bpy = FLD(BPY);
+  const double ht = g.ht;
+  const double hxi = g.hx[n.i], hyj = g.hy[n.j];
+  double aT1 = maxi0(Fx[ij]) + DTx[ij];
+  double aT2 = maxi0(-Fx[i1j]) + DTx[i1j];
+  double aT3 = maxi0(Fy[ij]) + DTy[ij];
+  double aT4 = maxi0(-Fy[ij1]) + DTy[ij1];
+  double X1 = m[n.i_1j] ? x.Tw : FLD(TW_U)[ij];
+  double X2 = m[i1j] ? x.Te : FLD(TW_U)[i1j];
+  double X3 = m[n.ij_1] ? x.Ts : FLD(TW_V)[ij];
+  double X4 = m[ij1] ? x.Tn : FLD(TW_V)[ij1];
+  tn = (ht * (aT1 * X1 + aT2 * X2 + aT3 * X3 + aT4 * X4
+              + (1.0 - g.gamma1) * x.Pc * ((u[i1j] - u[ij]) * hyj + (v[ij1] - v[ij]) * hxi))
+        + FLD(SCT1)[ij])
+       / (ht * (aT1 + aT2 + aT3 + aT4 + Fx[i1j] - Fx[ij] + Fy[ij1] - Fy[ij]) + hxi * hyj * FLD(RHO)[ij]);
+  double s = apx[ij] * x.Pw + apx[i1j] * x.Pe + apy[ij] * x.Ps + apy[ij1] * x.Pn + bpx[ij] - bpx[i1j] + bpy[ij]
+             - bpy[ij1];
+  // S:3887-3888: inflow/outflow flux terms of the given-velocity columns
+  if (g.given_xb && (int)n.i == g.Nx_o0) s = s + Fx[ij];
+  if (g.given_xe && (int)n.i == g.Nx_o1 - 1) s = s - Fx[i1j];
+  pn = (ht * s + FLD(P_PR)[ij] * hxi * hyj / (FLD(T_PR)[ij])) /
+       (ht * (apx[ij] + apx[i1j] + apy[ij] + apy[ij1]) + hxi * hyj / (tn));
+}
+
+__device__ __forceinline__ bool in_pt_set(const KGrid &g, int i, int j, unsigned ij) {
+  return in_box(i, j, g.Nx_o0, g.Nx_o1, g.Ny_o0, g.Ny_o1) && g.m[ij];
+}
+
+__device__ __forceinline__ PTStencil stencil_global(const double *Po, const double *To, const Nbr &n) {
+  return PTStencil{Po[n.ij], To[n.ij], Po[n.i_1j], To[n.i_1j], Po[n.i1j], To[n.i1j],
+                   Po[n.ij_1], To[n.ij_1], Po[n.ij1], To[n.ij1]};
+}
+
+// One sweep: reads (Po, To), writes (Pn, Tn) on every cell: the update set
+// gets the new values, every other cell is copied through, which restates
+// the loop-3 entry copy S:3763-3764 and the pointer swap S:3792-3795.
 __global__ void __launch_bounds__(BX *BY) k_sweep(KGrid g, KFields fl, const double *__restrict__ Po,
                                                   const double *__restrict__ To, double *__restrict__ Pn,
                                                   double *__restrict__ Tn, unsigned long long *red_p,
@@ -198,46 +246,88 @@ __global__ void __launch_bounds__(BX *BY) k_sweep(KGrid g, KFields fl, const dou
   unsigned long long r[2] = {0ull, 0ull};
   if (i < g.Nx && j < g.Ny) {
     const unsigned ij = (unsigned)i + (unsigned)j * (unsigned)g.Nx;
-    const uint8_t *m = g.m;
-    if (in_box(i, j, g.Nx_o0, g.Nx_o1, g.Ny_o0, g.Ny_o1) && m[ij]) {
+    if (in_pt_set(g, i, j, ij)) {
       const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny);
-      const unsigned i_1j = n.i_1j, i1j = n.i1j, ij_1 = n.ij_1, ij1 = n.ij1;
-      const double *Fx = FLD(FX), *Fy = FLD(FY), *DTx = FLD(DTX), *DTy = FLD(DTY), *u = FLD(U), *v = FLD(V);
-      const double *apx = FLD(APX), *apy = FLD(APY), *bpx = FLD(BPX), *bpy = FLD(BPY);
-      const double ht = g.ht;
-      const double hxi = g.hx[i], hyj = g.hy[j];
-      double aT1 = maxi0(Fx[ij]) + DTx[ij];
-      double aT2 = maxi0(-Fx[i1j]) + DTx[i1j];
-      double aT3 = maxi0(Fy[ij]) + DTy[ij];
-      double aT4 = maxi0(-Fy[ij1]) + DTy[ij1];
-      // S:3838-3848: a body neighbour contributes the fluid temperature on
-      // its wall instead (m*T_old + !m*T_wall, with T_wall finite).
-      double X1 = m[i_1j] ? To[i_1j] : FLD(TW_U)[ij];
-      double X2 = m[i1j] ? To[i1j] : FLD(TW_U)[i1j];
-      double X3 = m[ij_1] ? To[ij_1] : FLD(TW_V)[ij];
-      double X4 = m[ij1] ? To[ij1] : FLD(TW_V)[ij1];
-      const double Pij = Po[ij];
-      double tn = (ht * (aT1 * X1 + aT2 * X2 + aT3 * X3 + aT4 * X4
-                         + (1.0 - g.gamma1) * Pij * ((u[i1j] - u[ij]) * hyj + (v[ij1] - v[ij]) * hxi))
-                   + FLD(SCT1)[ij])
-                  / (ht * (aT1 + aT2 + aT3 + aT4 + Fx[i1j] - Fx[ij] + Fy[ij1] - Fy[ij]) + hxi * hyj * FLD(RHO)[ij]);
-      double s = apx[ij] * Po[i_1j] + apx[i1j] * Po[i1j] + apy[ij] * Po[ij_1] + apy[ij1] * Po[ij1] + bpx[ij]
-                 - bpx[i1j] + bpy[ij] - bpy[ij1];
-      // S:3887-3888: inflow/outflow flux terms of the given-velocity columns
-      if (g.given_xb && i == g.Nx_o0) s = s + Fx[ij];
-      if (g.given_xe && i == g.Nx_o1 - 1) s = s - Fx[i1j];
-      double pn = (ht * s + FLD(P_PR)[ij] * hxi * hyj / (FLD(T_PR)[ij]))
-                  / (ht * (apx[ij] + apx[i1j] + apy[ij] + apy[ij1]) + hxi * hyj / (tn));
+      const PTStencil x = stencil_global(Po, To, n);
+      double pn, tn;
+      pt_update(g, fl, n, x, pn, tn);
       Tn[ij] = tn;
       Pn[ij] = pn;
-      r[0] = absbits(pn - Pij);
-      r[1] = absbits(tn - To[ij]);
+      r[0] = absbits(pn - x.Pc);
+      r[1] = absbits(tn - x.Tc);
     } else {
       Pn[ij] = Po[ij];
       Tn[ij] = To[ij];
     }
   }
   if (red_p || red_T || red_p2 || red_T2) block_max_atomic<2>(r, red_p, red_T, red_p2, red_T2);
+}
+
+// Two sweeps in one pass (temporal blocking): each block first sweeps its
+// tile plus a one-cell halo from (Po, To) into LDS, then sweeps the tile from
+// LDS into (Pn, Tn).  Same arithmetic as two k_sweep launches; the
+// intermediate p and T never go to HBM.  Residual maxima of both sweeps over
+// the tile cells go to (r1p, r1T) and (r2p, r2T).
+constexpr int S2X = 64, S2Y = 8, S2RX = S2X + 2, S2RY = S2Y + 2;
+
+__global__ void __launch_bounds__(S2X *S2Y) k_sweep2(KGrid g, KFields fl, const double *__restrict__ Po,
+                                                     const double *__restrict__ To, double *__restrict__ Pn,
+                                                     double *__restrict__ Tn, unsigned long long *r1p,
+                                                     unsigned long long *r1T, unsigned long long *r2p,
+                                                     unsigned long long *r2T, Gate gate) {
+  if (gated_out(gate)) return;
+  __shared__ double sP[S2RY][S2RX], sT[S2RY][S2RX];
+  const int Nx = g.Nx, Ny = g.Ny;
+  const int i0 = blockIdx.x * S2X, j0 = blockIdx.y * S2Y;
+  const int tid = threadIdx.x + threadIdx.y * S2X;
+  unsigned long long r1[2] = {0ull, 0ull};
+  for (int q = tid; q < S2RX * S2RY; q += S2X * S2Y) {
+    const int ry = q / S2RX, rx = q - ry * S2RX;
+    const int i = i0 - 1 + rx, j = j0 - 1 + ry;
+    if (i < 0 || j < 0 || i >= Nx || j >= Ny) continue;
+    const unsigned ij = (unsigned)i + (unsigned)j * (unsigned)Nx;
+    double pn, tn;
+    if (in_pt_set(g, i, j, ij)) {
+      const Nbr n((unsigned)i, (unsigned)j, (unsigned)Nx, (unsigned)Ny);
+      const PTStencil x = stencil_global(Po, To, n);
+      pt_update(g, fl, n, x, pn, tn);
+      if (rx >= 1 && rx <= S2X && ry >= 1 && ry <= S2Y) {
+        r1[0] = (absbits(pn - x.Pc) > r1[0]) ? absbits(pn - x.Pc) : r1[0];
+        r1[1] = (absbits(tn - x.Tc) > r1[1]) ? absbits(tn - x.Tc) : r1[1];
+      }
+    } else {
+      pn = Po[ij];
+      tn = To[ij];
+    }
+    sP[ry][rx] = pn;
+    sT[ry][rx] = tn;
+  }
+  __syncthreads();
+  unsigned long long r2[2] = {0ull, 0ull};
+  const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
+  if (i < Nx && j < Ny) {
+    const unsigned ij = (unsigned)i + (unsigned)j * (unsigned)Nx;
+    const int cx = threadIdx.x + 1, cy = threadIdx.y + 1;
+    if (in_pt_set(g, i, j, ij)) {
+      const Nbr n((unsigned)i, (unsigned)j, (unsigned)Nx, (unsigned)Ny);
+      // clamped neighbours map to the cell itself at the domain edge
+      const int wx = cx - (int)(n.i - n.i_1), ex = cx + (int)(n.i1 - n.i);
+      const int sy = cy - (int)((n.ij - n.ij_1) / (unsigned)Nx), ny = cy + (int)((n.ij1 - n.ij) / (unsigned)Nx);
+      const PTStencil x{sP[cy][cx], sT[cy][cx], sP[cy][wx], sT[cy][wx], sP[cy][ex], sT[cy][ex],
+                        sP[sy][cx], sT[sy][cx], sP[ny][cx], sT[ny][cx]};
+      double pn, tn;
+      pt_update(g, fl, n, x, pn, tn);
+      Tn[ij] = tn;
+      Pn[ij] = pn;
+      r2[0] = absbits(pn - x.Pc);
+      r2[1] = absbits(tn - x.Tc);
+    } else {
+      Pn[ij] = sP[cy][cx];
+      Tn[ij] = sT[cy][cx];
+    }
+  }
+  if (r1p || r1T) block_max_atomic<2>(r1, r1p, r1T);
+  if (r2p || r2T) block_max_atomic<2>(r2, r2p, r2T);
 }
 
 // ---------------------------------------------------------------------------
@@ -838,6 +928,13 @@ hipError_t launch_sweep(const KGrid &g, const KFields &f, const double *Po, cons
                         unsigned long long *rp, unsigned long long *rT, hipStream_t s, unsigned long long *rp2,
                         unsigned long long *rT2, Gate gate) {
   hipLaunchKernelGGL(k_sweep, grid2d(g), dim3(BX, BY), 0, s, g, f, Po, To, Pn, Tn, rp, rT, rp2, rT2, gate);
+  return hipGetLastError();
+}
+hipError_t launch_sweep2(const KGrid &g, const KFields &f, const double *Po, const double *To, double *Pn, double *Tn,
+                         unsigned long long *r1p, unsigned long long *r1T, unsigned long long *r2p,
+                         unsigned long long *r2T, hipStream_t s, Gate gate) {
+  dim3 gr((g.Nx + S2X - 1) / S2X, (g.Ny + S2Y - 1) / S2Y);
+  hipLaunchKernelGGL(k_sweep2, gr, dim3(S2X, S2Y), 0, s, g, f, Po, To, Pn, Tn, r1p, r1T, r2p, r2T, gate);
   return hipGetLastError();
 }
 hipError_t launch_correct(const KGrid &g, const KFields &f, unsigned long long *ru, unsigned long long *rv,

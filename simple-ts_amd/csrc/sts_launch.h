@@ -10,6 +10,9 @@ hipError_t launch_sweep(const KGrid &g, const KFields &f, const double *Po, cons
                         unsigned long long *rp, unsigned long long *rT, hipStream_t s,
                         unsigned long long *rp2 = nullptr, unsigned long long *rT2 = nullptr,
                         Gate gate = Gate{nullptr, 0});
+hipError_t launch_sweep2(const KGrid &g, const KFields &f, const double *Po, const double *To, double *Pn, double *Tn,
+                         unsigned long long *r1p, unsigned long long *r1T, unsigned long long *r2p,
+                         unsigned long long *r2T, hipStream_t s, Gate gate = Gate{nullptr, 0});
 hipError_t launch_correct(const KGrid &g, const KFields &f, unsigned long long *ru, unsigned long long *rv,
                           int do_correct, hipStream_t s, Gate gate = Gate{nullptr, 0});
 hipError_t launch_properties(const KGrid &g, const KFields &f, hipStream_t s);

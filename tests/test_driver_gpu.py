@@ -20,9 +20,11 @@ SAME_BYTES = ["u.bin", "v.bin", "p.bin", "Temper.bin", "vol_inf_fb.bin", "x_f.tx
               "ContinuityEquation.bin", "ContinuityEquation_stationary.bin"]
 
 
-def run_driver(d, timeout=300):
+def run_driver(d, timeout=300, env=None):
     print(f"sts_gpu {d}", flush=True)
-    subprocess.run([DRIVER, d], check=True, timeout=timeout, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    e = dict(os.environ)
+    e.update(env or {})
+    subprocess.run([DRIVER, d], check=True, timeout=timeout, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, env=e)
 
 
 def compare_dirs(mine, ref):
@@ -43,10 +45,12 @@ def compare_dirs(mine, ref):
     return bad
 
 
+@pytest.mark.parametrize("fused", ["1", "0"])
 @pytest.mark.parametrize("name", RUN_CASES)
-def test_driver_matches_reference_files(name, tmp_path):
+def test_driver_matches_reference_files(name, fused, tmp_path):
+    """fused: loop 3 as one two-sweep pass (k_sweep2) or as single sweeps."""
     d = fresh_input(name, tmp_path)
-    run_driver(d)
+    run_driver(d, env={"STS_FUSED_SWEEPS": fused})
     bad = compare_dirs(d, ref_dir(name))
     assert not bad, f"{name}: files differing from the reference's: {bad}"
 
