@@ -45,6 +45,10 @@ def run_case(case_dir, max_steps=None, on_step=None):
         if V(flag):
             raise NotImplementedError(flag)
     N_I, N_I_p_c = max(1, int(V("N_I"))), int(V("N_I_p_c"))
+    top = (prob.Ny - 1) * prob.Nx  # S:6906-6912: rho_v one row up, outside the arrays on the top row
+    if (V("drhodt_durhodx_dvrhody_0_BC_xb") and m[top + prob.boxes["Nx_u"][0] - 1]) or (
+            V("drhodt_durhodx_dvrhody_0_BC_xe") and m[top + prob.boxes["Nx_u"][1] - 1]):
+        raise NotImplementedError("drhodt_durhodx_dvrhody edge condition on a fluid cell of the top row")
     Nt = V("Nt") if max_steps is None else min(V("Nt"), max_steps)
     p_BC_xb = V("p_BC_xb")
     u_given_xb = V("u_given_xb")

@@ -47,8 +47,10 @@ def case_specs():
     """name -> function(dir) writing the case input."""
     return {
         # the shipped microchannel on a coarser mesh: pressure-driven, slip walls
-        "shipped_coarse": lambda d: cases.write_case(d, dict(hxmin=0.1, hxmax=0.1, hymin=0.1, hymax=0.1, Nt=6,
-                                                             Nt_save_temp_data=1, **CONV)),
+        # (y_e keeps the top row inside the upper body: a fluid top row would make
+        # the reference's drhodt_... edge condition read past its arrays, S:6906)
+        "shipped_coarse": lambda d: cases.write_case(d, dict(hxmin=0.1, hxmax=0.1, hymin=0.1, hymax=0.1, y_e=2.15,
+                                                             Nt=6, Nt_save_temp_data=1, **CONV)),
         # subsonic confined square with time-dependent inflow and continuity output
         "square": lambda d: cases.confined_square(
             d, 60, 14, 85, 0.00194, 4, extra=dict(ToSolveContinuityEquation=1, use_interpolation_for_u_BC_by_time=1,

@@ -58,10 +58,11 @@ def test_driver_matches_reference_files(name, fused, tmp_path):
 @pytest.mark.skipif(not have_reference(), reason="oracle/_ref/sts_ref not built")
 def test_driver_matches_live_reference_subsonic_square(tmp_path):
     """The subsonic confined square on the 240x48 grid for 12 steps, each
-    converged to 1e-8, against the reference run in the same job."""
+    converged to 1e-8, against the reference run in the same job (dt =
+    0.005 H/u_in: at 0.01 H/u_in the reference itself diverges on this grid)."""
     cases = importlib.import_module("simple-ts_amd.cases")
-    mine = cases.confined_square(str(tmp_path / "gpu"), 240, 48, 85, 0.00194, 12)
-    ref = cases.confined_square(str(tmp_path / "ref"), 240, 48, 85, 0.00194, 12)
+    mine = cases.confined_square(str(tmp_path / "gpu"), 240, 48, 85, 0.00194, 12, dt_over_tin=0.005)
+    ref = cases.confined_square(str(tmp_path / "ref"), 240, 48, 85, 0.00194, 12, dt_over_tin=0.005)
     run_reference(ref)
     run_driver(mine)
     bad = compare_dirs(mine, ref)
@@ -69,10 +70,11 @@ def test_driver_matches_live_reference_subsonic_square(tmp_path):
 
 
 @pytest.mark.skipif(not have_reference(), reason="oracle/_ref/sts_ref not built")
-@pytest.mark.parametrize("n_pc", [1, 3])
+@pytest.mark.parametrize("n_pc", [3, 5])
 def test_driver_odd_sweep_counts_match_live_reference(n_pc, tmp_path):
     """Odd N_I_p_c leaves each outer iteration's p/T in the partner buffers;
-    the device-side loop must follow that parity."""
+    the device-side loop must follow that parity.  (N_I_p_c = 1 diverges in
+    the reference, as its input notes warn.)"""
     cases = importlib.import_module("simple-ts_amd.cases")
     kw = dict(N_I=400, N_I_p_c=n_pc, extra=dict(Nt_save_temp_data=1))
     mine = cases.confined_square(str(tmp_path / "gpu"), 60, 14, 85, 0.00194, 4, **kw)
