@@ -46,7 +46,9 @@ struct sts_ctx {
   DevBody *bodies = nullptr;
   double *f[STS_NFIELDS] = {};
   double *p_alt = nullptr, *T_alt = nullptr;  // loop-3 ping-pong partners of p, Temper
-  unsigned long long *red = nullptr;          // residual maxima as IEEE bit patterns
+  unsigned long long *red = nullptr;          // slotted residual maxima as IEEE bit patterns
+  unsigned long long *red_out = nullptr;      // red folded over its slots
+  unsigned long long *lred = nullptr;         // slotted maxima of the current outer iteration (loop 2)
   double *probe = nullptr;
   unsigned long long *h_red = nullptr;        // pinned host mirror of red (+ probe)
   double *partial = nullptr;                  // continuity norms, one triple per block
@@ -175,6 +177,8 @@ extern "C" int sts_destroy(sts_ctx *c) {
   (void)hipFree(c->vol);
   (void)hipFree(c->bodies);
   (void)hipFree(c->red);
+  (void)hipFree(c->red_out);
+  (void)hipFree(c->lred);
   (void)hipFree(c->probe);
   (void)hipFree(c->partial);
   (void)hipFree(c->cont_stat);
@@ -204,7 +208,11 @@ static int alloc_state(sts_ctx *c, const sts_grid *gi, const std::vector<uint8_t
   HIP_OK(hipMalloc(&c->vol, sizeof(uint32_t) * Na));
   const size_t nb = (size_t)(gi->nbodies > 0 ? gi->nbodies : 1);
   HIP_OK(hipMalloc(&c->bodies, sizeof(DevBody) * nb));
-  HIP_OK(hipMalloc(&c->red, sizeof(unsigned long long) * NRED));
+  HIP_OK(hipMalloc(&c->red, sizeof(unsigned long long) * NSLOT * SLOT_STRIDE));
+  HIP_OK(hipMalloc(&c->red_out, sizeof(unsigned long long) * NRED));
+  HIP_OK(hipMalloc(&c->lred, sizeof(unsigned long long) * NSLOT * SLOT_STRIDE));
+  HIP_OK(hipMemsetAsync(c->red, 0, sizeof(unsigned long long) * NSLOT * SLOT_STRIDE, c->stream));
+  HIP_OK(hipMemsetAsync(c->lred, 0, sizeof(unsigned long long) * NSLOT * SLOT_STRIDE, c->stream));
   HIP_OK(hipMalloc(&c->probe, sizeof(double)));
   HIP_OK(hipHostMalloc(&c->h_red, sizeof(unsigned long long) * (NRED + 1), hipHostMallocDefault));
   HIP_OK(hipMemcpyAsync(c->hx, gi->hx, sizeof(double) * gi->Nx, hipMemcpyHostToDevice, c->stream));
@@ -324,7 +332,7 @@ extern "C" int sts_save_previous(sts_ctx *c) {
 extern "C" int sts_assemble(sts_ctx *c) {
   if (!c) return fail(STS_ERR_ARG, "null context");
   const int t = timed_begin(c, STS_K_ASSEMBLE);
-  HIP_OK(launch_assemble(c->g, c->fields(), nullptr, 0, c->stream));
+  HIP_OK(launch_assemble(c->g, c->fields(), c->stream));
   timed_end(c, t);
   return STS_OK;
 }
@@ -357,8 +365,12 @@ static int run_sweeps(sts_ctx *c, int n) {
   return STS_OK;
 }
 
+// Folds words 0..nslots-1 of the slotted residual buffer (clearing them) and
+// brings them to h_red.
 static int read_red(sts_ctx *c, int nslots, bool with_probe) {
-  HIP_OK(hipMemcpyAsync(c->h_red, c->red, sizeof(unsigned long long) * nslots, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(launch_fold(c->red, nslots, c->red_out, c->stream));
+  HIP_OK(hipMemcpyAsync(c->h_red, c->red_out, sizeof(unsigned long long) * nslots, hipMemcpyDeviceToHost,
+                        c->stream));
   if (with_probe)
     HIP_OK(hipMemcpyAsync(c->h_red + NRED, c->probe, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_OK(hipStreamSynchronize(c->stream));
@@ -370,7 +382,7 @@ static int read_red(sts_ctx *c, int nslots, bool with_probe) {
 
 extern "C" int sts_pt_jacobi(sts_ctx *c, int n, double *res) {
   if (!c || n < 1 || n > MAX_SWEEPS) return fail(STS_ERR_ARG, "sweep count must be 1..62");
-  HIP_OK(hipMemsetAsync(c->red, 0, sizeof(unsigned long long) * NRED, c->stream));
+  HIP_OK(hipMemsetAsync(c->red, 0, sizeof(unsigned long long) * NSLOT * SLOT_STRIDE, c->stream));
   int rc = run_sweeps(c, n);
   if (rc) return rc;
   if ((rc = read_red(c, 2 + 2 * n, false))) return rc;
@@ -384,7 +396,7 @@ extern "C" int sts_pt_jacobi(sts_ctx *c, int n, double *res) {
 
 extern "C" int sts_correct(sts_ctx *c, double *res) {
   if (!c) return fail(STS_ERR_ARG, "null context");
-  HIP_OK(hipMemsetAsync(c->red, 0, sizeof(unsigned long long) * 2, c->stream));
+  HIP_OK(hipMemsetAsync(c->red, 0, sizeof(unsigned long long) * NSLOT * SLOT_STRIDE, c->stream));
   const int t = timed_begin(c, STS_K_CORRECT);
   HIP_OK(launch_correct(c->g, c->fields(), c->red, c->red + 1, 1, c->stream));
   timed_end(c, t);
@@ -500,8 +512,9 @@ extern "C" int sts_continuity(sts_ctx *c, double *cont_stat, double *cont, doubl
 extern "C" int sts_outer_iteration(sts_ctx *c, int n_pc, double *res, double *res_p0T0, double *p_probe) {
   if (!c || n_pc < 1 || n_pc > MAX_SWEEPS) return fail(STS_ERR_ARG, "N_I_p_c must be 1..62");
   const int nslots = 2 + 2 * n_pc;
+  HIP_OK(hipMemsetAsync(c->red, 0, sizeof(unsigned long long) * NSLOT * SLOT_STRIDE, c->stream));
   int t = timed_begin(c, STS_K_ASSEMBLE);
-  HIP_OK(launch_assemble(c->g, c->fields(), c->red, nslots, c->stream));
+  HIP_OK(launch_assemble(c->g, c->fields(), c->stream));
   timed_end(c, t);
   int rc = run_sweeps(c, n_pc);
   if (rc) return rc;
@@ -536,7 +549,7 @@ static int enqueue_outer(sts_ctx *c, int k, const sts_loop_control *lc, double *
                          double *T1) {
   const int n = lc->N_I_p_c;
   const Gate gate{c->ctrl, k};
-  unsigned long long *lk = c->log + 6 * (size_t)k;
+  unsigned long long *lk = c->log + 6 * (size_t)k, *lr = c->lred;
   auto fields_at = [&](int parity) {
     KFields kf = c->fields();
     kf.f[STS_P] = parity ? p1 : p0;
@@ -545,7 +558,7 @@ static int enqueue_outer(sts_ctx *c, int k, const sts_loop_control *lc, double *
   };
   int par = (int)(((long long)k * sweep_passes(c, n)) & 1);
   int t = timed_begin(c, STS_K_ASSEMBLE);
-  HIP_OK(launch_assemble(c->g, fields_at(par), nullptr, 0, c->stream, gate));
+  HIP_OK(launch_assemble(c->g, fields_at(par), c->stream, gate));
   timed_end(c, t);
   for (int s = 0; s < n;) {
     KFields kf = fields_at(par);
@@ -553,12 +566,12 @@ static int enqueue_outer(sts_ctx *c, int k, const sts_loop_control *lc, double *
     t = timed_begin(c, STS_K_SWEEP);
     if (c->fuse_sweeps && s + 1 < n) {
       // sweep s (first?) and s+1 (last?)
-      HIP_OK(launch_sweep2(c->g, kf, Pi, Ti, Po, To, s == 0 ? lk + 4 : nullptr, s == 0 ? lk + 5 : nullptr,
-                           s + 1 == n - 1 ? lk + 2 : nullptr, s + 1 == n - 1 ? lk + 3 : nullptr, c->stream, gate));
+      HIP_OK(launch_sweep2(c->g, kf, Pi, Ti, Po, To, s == 0 ? lr + 4 : nullptr, s == 0 ? lr + 5 : nullptr,
+                           s + 1 == n - 1 ? lr + 2 : nullptr, s + 1 == n - 1 ? lr + 3 : nullptr, c->stream, gate));
       s += 2;
     } else {
-      HIP_OK(launch_sweep(c->g, kf, Pi, Ti, Po, To, s == n - 1 ? lk + 2 : nullptr, s == n - 1 ? lk + 3 : nullptr,
-                          c->stream, s == 0 ? lk + 4 : nullptr, s == 0 ? lk + 5 : nullptr, gate));
+      HIP_OK(launch_sweep(c->g, kf, Pi, Ti, Po, To, s == n - 1 ? lr + 2 : nullptr, s == n - 1 ? lr + 3 : nullptr,
+                          c->stream, s == 0 ? lr + 4 : nullptr, s == 0 ? lr + 5 : nullptr, gate));
       s += 1;
     }
     timed_end(c, t);
@@ -566,12 +579,12 @@ static int enqueue_outer(sts_ctx *c, int k, const sts_loop_control *lc, double *
   }
   const KFields kf = fields_at(par);
   t = timed_begin(c, STS_K_CORRECT);
-  HIP_OK(launch_correct(c->g, kf, lk + 0, lk + 1, 1, c->stream, gate));
+  HIP_OK(launch_correct(c->g, kf, lr + 0, lr + 1, 1, c->stream, gate));
   timed_end(c, t);
   t = timed_begin(c, STS_K_WALL);
   HIP_OK(launch_wall(c->g, kf, STS_BC_SLIP_U | STS_BC_SLIP_V | STS_BC_JUMP, c->stream, gate));
   timed_end(c, t);
-  const LoopTest lt{c->ctrl, lk, lc->N_I, lc->tol_uv, lc->tol_p, lc->tol_T};
+  const LoopTest lt{c->ctrl, lr, lk, lc->N_I, lc->tol_uv, lc->tol_p, lc->tol_T};
   t = timed_begin(c, STS_K_EDGES);
   HIP_OK(launch_edges(c->g, kf, STS_BC_VELOCITIES | STS_BC_TEMPERATURE | STS_BC_PRESSURE, c->probe, c->stream, gate,
                       lt));
@@ -598,7 +611,7 @@ extern "C" int sts_time_step(sts_ctx *c, const sts_loop_control *lc, int *iterat
   }
   int rc = sts_save_previous(c);
   if (rc) return rc;
-  HIP_OK(launch_loop_open(c->ctrl, c->log, 6 * lc->N_I, c->stream));
+  HIP_OK(launch_loop_open(c->ctrl, c->log, 6 * lc->N_I, c->lred, c->stream));
   double *const p0 = c->f[STS_P], *const T0 = c->f[STS_T], *const p1 = c->p_alt, *const T1 = c->T_alt;
   int k = 0, batch = lc->batch > 0 ? lc->batch : (c->last_iters > 2 ? c->last_iters - 1 : 1);
   int stop = -1;

@@ -56,10 +56,21 @@ struct Gate {
 
 __device__ __forceinline__ bool gated_out(const Gate &g) { return g.ctrl && g.k > g.ctrl->stop_at; }
 
-// Loop-2 test of iteration k (S:4433-4437) from its residual log entry.
+// Residual maxima are accumulated with one atomicMax per workgroup into one
+// of NSLOT copies of each destination (copy = workgroup index mod NSLOT,
+// copies SLOT_STRIDE words apart), so no address takes more than a few dozen
+// atomics per launch; the reader folds the copies.  A destination pointer
+// passed to a kernel is the address of copy 0.
+constexpr int NSLOT = 64, SLOT_STRIDE = 128;
+
+// Loop-2 test of iteration k (S:4433-4437).  The kernels of iteration k leave
+// their maxima in the slotted words lred[0..5]: max|du|, max|dv|, max|dp|,
+// max|dT| (last sweep), first-sweep dp, dT.  The test folds them into log_k,
+// clears lred for the next iteration and decides.
 struct LoopTest {
   LoopCtrl *ctrl;
-  const unsigned long long *log_k;  // max|du|, max|dv|, max|dp|, max|dT| (last sweep), first-sweep dp, dT
+  unsigned long long *lred;
+  unsigned long long *log_k;
   int N_I;
   double tol_uv, tol_p, tol_T;
 };

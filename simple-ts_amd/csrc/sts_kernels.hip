@@ -34,9 +34,10 @@ __device__ __forceinline__ unsigned long long wave_max(unsigned long long v) {
   return v;
 }
 
-// Block-wide max of two values, one atomicMax per block and destination;
-// destinations may be null, and a second pair (dst2, dst3) receives the same
-// maxima (the first and the last sweep of a single-sweep loop 3 coincide).
+// Block-wide max of two values, one atomicMax per block and destination, into
+// the block's slot copy of each destination; destinations may be null, and a
+// second pair (dst2, dst3) receives the same maxima (the first and the last
+// sweep of a single-sweep loop 3 coincide).
 template <int NV>
 __device__ __forceinline__ void block_max_atomic(unsigned long long (&v)[NV], unsigned long long *dst0,
                                                  unsigned long long *dst1, unsigned long long *dst2 = nullptr,
@@ -53,14 +54,16 @@ __device__ __forceinline__ void block_max_atomic(unsigned long long (&v)[NV], un
   }
   __syncthreads();
   if (tid == 0) {
+    const unsigned slot = ((unsigned)blockIdx.x + (unsigned)blockIdx.y * gridDim.x) % (unsigned)NSLOT;
+    const unsigned off = slot * (unsigned)SLOT_STRIDE;
 #pragma unroll
     for (int k = 0; k < NV; k++) {
       unsigned long long b = 0;
       for (int q = 0; q < nw; q++) b = (sm[k][q] > b) ? sm[k][q] : b;
       unsigned long long *d = (k == 0) ? dst0 : dst1;
       unsigned long long *e = (k == 0) ? dst2 : dst3;
-      if (b && d) atomicMax(d, b);
-      if (b && e) atomicMax(e, b);
+      if (b && d) atomicMax(d + off, b);
+      if (b && e) atomicMax(e + off, b);
     }
   }
 }
@@ -167,14 +170,9 @@ __device__ __forceinline__ void cell_energy_source(const KGrid &g, const KFields
                   + p_pr[ij] * hx[i] * hy[j];
 }
 
-__global__ void __launch_bounds__(BX *BY) k_assemble(KGrid g, KFields fl, unsigned long long *red, int nred,
-                                                     Gate gate) {
+__global__ void __launch_bounds__(BX *BY) k_assemble(KGrid g, KFields fl, Gate gate) {
   if (gated_out(gate)) return;
   const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-  if (red && blockIdx.x == 0 && blockIdx.y == 0) {
-    const int t = threadIdx.x + threadIdx.y * BX;
-    if (t < nred) red[t] = 0ull;
-  }
   if (i >= g.Nx || j >= g.Ny) return;
   const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny);
   const uint8_t *m = g.m;
@@ -621,6 +619,15 @@ __device__ __forceinline__ void edge_y(const KGrid &g, double *a, int b0, int b1
 __global__ void __launch_bounds__(EDGE_THREADS) k_edges(KGrid g, KFields fl, int parts, double *probe, Gate gate,
                                                         LoopTest lt) {
   if (gated_out(gate)) return;
+  if (lt.ctrl && threadIdx.x < NSLOT) {  // wave 0: fold the slots of iteration gate.k
+    unsigned long long *sl = lt.lred + threadIdx.x * SLOT_STRIDE;
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+      const unsigned long long w = wave_max(sl[q]);
+      sl[q] = 0ull;
+      if (threadIdx.x == 0) lt.log_k[q] = w;
+    }
+  }
   if (lt.ctrl && threadIdx.x == 0) {
     // S:4433-4437 for iteration gate.k (Iter = k + 1 after the increment)
     double r[4];
@@ -848,14 +855,30 @@ __global__ void __launch_bounds__(BX *BY) k_set_inflow(KGrid g, KFields fl, doub
   if (g.m[k]) FLD(U)[k] = u_in;
 }
 
-// Opens the loop-2 control of a time step and clears its residual log.
-__global__ void k_loop_open(LoopCtrl *ctrl, unsigned long long *log, int n) {
+// Opens the loop-2 control of a time step and clears its residual log and
+// the slotted residual words.
+__global__ void k_loop_open(LoopCtrl *ctrl, unsigned long long *log, int n, unsigned long long *lred) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t == 0) {
     ctrl->stop_at = 0x7fffffff;
     ctrl->nonfinite = 0;
   }
   for (int q = t; q < n; q += gridDim.x * blockDim.x) log[q] = 0ull;
+  for (int q = t; q < NSLOT * SLOT_STRIDE; q += gridDim.x * blockDim.x) lred[q] = 0ull;
+}
+
+// Folds the NSLOT copies of words 0..n-1 of a slotted buffer into out[0..n-1]
+// and clears the copies.
+__global__ void k_fold(unsigned long long *slots, int n, unsigned long long *out) {
+  for (int q = threadIdx.x; q < n; q += blockDim.x) {
+    unsigned long long b = 0ull;
+    for (int s = 0; s < NSLOT; s++) {
+      const unsigned long long w = slots[q + s * SLOT_STRIDE];
+      b = (w > b) ? w : b;
+      slots[q + s * SLOT_STRIDE] = 0ull;
+    }
+    out[q] = b;
+  }
 }
 
 // SolveFluxes alone (S:10440-10454), as SolveContinuityEquation calls it.
@@ -919,9 +942,8 @@ __global__ void __launch_bounds__(BX *BY) k_continuity(KGrid g, KFields fl, doub
 
 static inline dim3 grid2d(const KGrid &g) { return dim3((g.Nx + BX - 1) / BX, (g.Ny + BY - 1) / BY); }
 
-hipError_t launch_assemble(const KGrid &g, const KFields &f, unsigned long long *red, int nred, hipStream_t s,
-                           Gate gate) {
-  hipLaunchKernelGGL(k_assemble, grid2d(g), dim3(BX, BY), 0, s, g, f, red, nred, gate);
+hipError_t launch_assemble(const KGrid &g, const KFields &f, hipStream_t s, Gate gate) {
+  hipLaunchKernelGGL(k_assemble, grid2d(g), dim3(BX, BY), 0, s, g, f, gate);
   return hipGetLastError();
 }
 hipError_t launch_sweep(const KGrid &g, const KFields &f, const double *Po, const double *To, double *Pn, double *Tn,
@@ -983,8 +1005,13 @@ hipError_t launch_set_inflow(const KGrid &g, const KFields &f, double u_in, hipS
   hipLaunchKernelGGL(k_set_inflow, gr, dim3(BX, BY), 0, s, g, f, u_in);
   return hipGetLastError();
 }
-hipError_t launch_loop_open(LoopCtrl *ctrl, unsigned long long *log, int n, hipStream_t s) {
-  hipLaunchKernelGGL(k_loop_open, dim3(16), dim3(256), 0, s, ctrl, log, n);
+hipError_t launch_loop_open(LoopCtrl *ctrl, unsigned long long *log, int n, unsigned long long *lred,
+                            hipStream_t s) {
+  hipLaunchKernelGGL(k_loop_open, dim3(16), dim3(256), 0, s, ctrl, log, n, lred);
+  return hipGetLastError();
+}
+hipError_t launch_fold(unsigned long long *slots, int n, unsigned long long *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_fold, dim3(1), dim3(128), 0, s, slots, n, out);
   return hipGetLastError();
 }
 int continuity_blocks(const KGrid &g) {
