@@ -50,7 +50,7 @@ def workload_case(dst, steps):
     return cases.confined_square(dst, NX, NY, RE, KN, steps, dt_over_tin=DT_OVER_TIN)
 
 
-def run_gpu(steps, warmup, kernel="sweep"):
+def run_gpu(steps, warmup, kernel="sweep", args=()):
     pkg = importlib.import_module("simple-ts_amd")
     if not os.access(pkg.DRIVER, os.X_OK):
         raise SystemExit(f"{pkg.DRIVER} is not built; run __graft_entry__.build()")
@@ -59,8 +59,8 @@ def run_gpu(steps, warmup, kernel="sweep"):
         d = workload_case(os.path.join(work, "gpu"), warmup + steps)
         timing = os.path.join(work, "timing.json")
         t0 = time.time()
-        subprocess.run([pkg.DRIVER, d, "--warmup", str(warmup), "--time-kernel", kernel, "--timing", timing],
-                       check=True)
+        subprocess.run([pkg.DRIVER, d, "--warmup", str(warmup), "--time-kernel", kernel, "--timing", timing,
+                        *args], check=True)
         wall = time.time() - t0
         with open(timing) as f:
             t = json.load(f)
@@ -101,12 +101,15 @@ def main():
     ap.add_argument("--steps", type=int, default=490)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--mixed-args", default="", help="extra sts_gpu arguments of the mixed run (space separated)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the workload does not shard "
               f"across GPUs in this build)", file=sys.stderr)
         sys.exit(2)
     t = run_gpu(args.steps, args.warmup)
+    tm = run_gpu(args.steps, args.warmup, "sweep32", ["--mixed"] + args.mixed_args.split())
+    print(json.dumps({"mixed_timing": tm, "fp64_timing": t}), file=sys.stderr)
     na = NX * NY
     value = na * t["timed_steps"] / t["timed_seconds"] / 1e6
     ms_step = 1e3 * t["timed_seconds"] / max(1, t["timed_steps"])

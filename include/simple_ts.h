@@ -93,7 +93,8 @@ const char *sts_last_error(void);
 
 /* Allocates the device state for one case on the current HIP device.  All
    fields start at zero, as the reference zero-fills them before loop 1
-   (S:2872-2930).  precision: 64 (FP64, reference arithmetic). */
+   (S:2872-2930).  precision: 64 (FP64, reference arithmetic), or 32: the
+   same FP64 state plus an FP32 copy for sts_time_step_mixed. */
 int sts_create(const sts_grid *g, int precision, sts_ctx **out);
 int sts_destroy(sts_ctx *ctx);
 
@@ -182,13 +183,35 @@ typedef struct sts_loop_control {
    outer iteration.  Returns STS_ERR_NONFINITE if a residual is not finite. */
 int sts_time_step(sts_ctx *ctx, const sts_loop_control *lc, int *iterations, double *res_log, double *p_probe);
 
+/* Mixed-precision time step (no counterpart in the reference; needs a context
+   created with precision 32).  After the prologue, loop 2 first runs on an
+   FP32 copy of the state -- assembly, sweeps, correction, boundary conditions,
+   fluxes and diffusion coefficients all in FP32 -- until its successive
+   differences fall below the FP32 tolerances, stagnate (the largest of
+   max|d*|/tol shrinks by less than stall_ratio over stall_w iterations) or
+   N_I32 iterations have run.  The FP32 state is then widened to FP64 and loop
+   2 continues in FP64 exactly as in sts_time_step: the reference's test on
+   FP64 successive differences alone decides convergence, so the step ends at
+   the same tolerance as the FP64 path.  res_log (room for 6*(N_I32+N_I)
+   doubles) gets 6 doubles per outer iteration, FP32 iterations first.
+   N_I32 = 0 gives sts_time_step. */
+typedef struct sts_mixed_control {
+  int N_I32;                          /* cap on FP32 outer iterations */
+  double tol32_uv, tol32_p, tol32_T;  /* FP32-phase tolerances */
+  int stall_w;                        /* stagnation window (0: off) */
+  double stall_ratio;                 /* required shrink over the window */
+} sts_mixed_control;
+int sts_time_step_mixed(sts_ctx *ctx, const sts_loop_control *lc, const sts_mixed_control *mc, int *iters32,
+                        int *iters64, double *res_log, double *p_probe);
+
 /* The HIP stream all work of the context is issued on (diagnostics). */
 int sts_stream_info(sts_ctx *ctx, void **hip_stream);
 
 /* Kernel timing (diagnostics, no counterpart in the reference): while
    enabled, every launch of the chosen kernel is bracketed by HIP events on
    the context's stream; sts_kernel_time reports the launches and their summed
-   duration since it was enabled.  kernel: STS_K_*, or -1 to disable. */
+   duration since it was enabled.  kernel: STS_K_* for the FP64 kernel,
+   STS_K_* + STS_K_COUNT for its FP32 instance, or -1 to disable. */
 enum sts_kernel { STS_K_ASSEMBLE = 0, STS_K_SWEEP, STS_K_CORRECT, STS_K_WALL, STS_K_EDGES, STS_K_FLUX_DIFF,
                   STS_K_COUNT };
 int sts_time_kernel(sts_ctx *ctx, int kernel);

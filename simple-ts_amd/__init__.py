@@ -50,6 +50,9 @@ class Body(ctypes.Structure):
 _I2 = ctypes.c_int * 2
 _DP = ctypes.POINTER(ctypes.c_double)
 _UP = ctypes.POINTER(ctypes.c_uint)
+_FP = ctypes.POINTER(ctypes.c_float)
+_BP = ctypes.POINTER(ctypes.c_ubyte)
+_IP = ctypes.POINTER(ctypes.c_int)
 
 
 def grid_fields(with_mask):
@@ -73,6 +76,16 @@ class Grid(ctypes.Structure):
 class LoopControl(ctypes.Structure):
     _fields_ = [("N_I", ctypes.c_int), ("N_I_p_c", ctypes.c_int), ("tol_uv", ctypes.c_double),
                 ("tol_p", ctypes.c_double), ("tol_T", ctypes.c_double), ("batch", ctypes.c_int)]
+
+
+class MixedControl(ctypes.Structure):
+    _fields_ = [("N_I32", ctypes.c_int), ("tol32_uv", ctypes.c_double), ("tol32_p", ctypes.c_double),
+                ("tol32_T", ctypes.c_double), ("stall_w", ctypes.c_int), ("stall_ratio", ctypes.c_double)]
+
+
+class UNetNorm(ctypes.Structure):
+    _fields_ = [("u_ref", ctypes.c_double), ("p_ref", ctypes.c_double), ("dp_scale", ctypes.c_double),
+                ("dT_scale", ctypes.c_double)]
 
 
 class Problem:
@@ -151,12 +164,28 @@ def lib():
             ("sts_set_inflow_u", [vp, c_double]), ("sts_continuity", [vp, _DP, _DP, _DP]),
             ("sts_outer_iteration", [vp, c_int, _DP, _DP, _DP]), ("sts_stream_info", [vp, ctypes.POINTER(vp)]),
             ("sts_time_step", [vp, ctypes.POINTER(LoopControl), ctypes.POINTER(c_int), _DP, _DP]),
+            ("sts_time_step_mixed", [vp, ctypes.POINTER(LoopControl), ctypes.POINTER(MixedControl),
+                                     ctypes.POINTER(c_int), ctypes.POINTER(c_int), _DP, _DP]),
             ("sts_time_kernel", [vp, c_int]),
+            ("sts_unet_create", [c_int, c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(vp)]),
+            ("sts_unet_destroy", [vp]), ("sts_unet_set_params", [vp, _FP]), ("sts_unet_get_params", [vp, _FP]),
+            ("sts_unet_get_grads", [vp, _FP]), ("sts_unet_forward", [vp, c_int, _FP, _FP]),
+            ("sts_unet_backward_mse", [vp, c_int, _FP, _BP, _DP]), ("sts_unet_backward", [vp, c_int, _FP]),
+            ("sts_unet_adam", [vp, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float]),
+            ("sts_unet_dataset", [vp, c_int]),
+            ("sts_unet_record", [vp, c_int, vp, ctypes.POINTER(UNetNorm)]),
+            ("sts_unet_train_step", [vp, _IP, c_int, ctypes.c_float, _DP]),
+            ("sts_unet_eval", [vp, _IP, c_int, _DP]),
+            ("sts_time_step_ws", [vp, vp, ctypes.POINTER(UNetNorm), ctypes.POINTER(LoopControl),
+                                  ctypes.POINTER(MixedControl), ctypes.POINTER(c_int), ctypes.POINTER(c_int),
+                                  ctypes.POINTER(c_int), _DP, _DP]),
             ("sts_kernel_time", [vp, ctypes.POINTER(ctypes.c_longlong), _DP]),
         ]:
             fn = getattr(L, name)
             fn.argtypes = args
             fn.restype = c_int
+        L.sts_unet_param_count.argtypes = [vp]
+        L.sts_unet_param_count.restype = ctypes.c_longlong
         _lib = L
     return _lib
 
@@ -266,10 +295,127 @@ class Context:
         _check(lib().sts_time_step(self.h, ctypes.byref(lc), ctypes.byref(it), _dp(log), _dp(pr)), "sts_time_step")
         return it.value, log[:6 * it.value].reshape(-1, 6), float(pr[0])
 
+    def time_step_mixed(self, N_I, N_I_p_c, tol_uv, tol_p, tol_T, N_I32, tol32, stall_w=4, stall_ratio=0.7,
+                        batch=0):
+        """Mixed-precision time step (FP32 phase, then FP64 phase with the
+        reference's test); needs precision=32.  tol32 = (uv, p, T) tolerances
+        of the FP32 phase.  Returns (FP32 iterations, FP64 iterations,
+        residual log [n32 + n64, 6], p probe)."""
+        lc = LoopControl(N_I, N_I_p_c, tol_uv, tol_p, tol_T, batch)
+        mc = MixedControl(N_I32, tol32[0], tol32[1], tol32[2], stall_w, stall_ratio)
+        n32, n64 = ctypes.c_int(), ctypes.c_int()
+        log, pr = np.zeros(6 * (N_I + N_I32)), np.zeros(1)
+        _check(lib().sts_time_step_mixed(self.h, ctypes.byref(lc), ctypes.byref(mc), ctypes.byref(n32),
+                                         ctypes.byref(n64), _dp(log), _dp(pr)), "sts_time_step_mixed")
+        n = n32.value + n64.value
+        return n32.value, n64.value, log[:6 * n].reshape(-1, 6), float(pr[0])
+
+    def time_step_ws(self, net, norm, N_I, N_I_p_c, tol_uv, tol_p, tol_T, mixed=None, batch=0):
+        """Time step warm-started by the U-Net `net` (UNet) with normalisation
+        `norm` (UNetNorm); mixed = None or (N_I32, tol32, stall_w,
+        stall_ratio).  Returns (used, n32, n64, residual log, p probe)."""
+        lc = LoopControl(N_I, N_I_p_c, tol_uv, tol_p, tol_T, batch)
+        mc, n_i32 = None, 0
+        if mixed is not None:
+            n_i32, tol32, w, r = mixed
+            mc = MixedControl(n_i32, tol32[0], tol32[1], tol32[2], w, r)
+        used, n32, n64 = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        log, pr = np.zeros(6 * (N_I + n_i32)), np.zeros(1)
+        _check(lib().sts_time_step_ws(self.h, net.h, ctypes.byref(norm), ctypes.byref(lc),
+                                      ctypes.byref(mc) if mc is not None else None, ctypes.byref(used),
+                                      ctypes.byref(n32), ctypes.byref(n64), _dp(log), _dp(pr)), "sts_time_step_ws")
+        n = n32.value + n64.value
+        return used.value, n32.value, n64.value, log[:6 * n].reshape(-1, 6), float(pr[0])
+
     def outer_iteration(self, n_pc):  # S:3263-4608
         r, r0, pr = np.zeros(4), np.zeros(2), np.zeros(1)
         _check(lib().sts_outer_iteration(self.h, n_pc, _dp(r), _dp(r0), _dp(pr)), "sts_outer_iteration")
         return r, r0, float(pr[0])
+
+
+# ---------------------------------------------------------------------------
+# Warm-start U-Net (include/sts_unet.h)
+# ---------------------------------------------------------------------------
+def _fp(a):
+    return a.ctypes.data_as(_FP)
+
+
+class UNet:
+    """The BF16 U-Net on the GPU.  Arrays are float32 NHWC numpy arrays."""
+
+    def __init__(self, cin, cout, base, max_n, H, W):
+        self.cin, self.cout, self.base, self.max_n, self.H, self.W = cin, cout, base, max_n, H, W
+        h = ctypes.c_void_p()
+        _check(lib().sts_unet_create(cin, cout, base, max_n, H, W, ctypes.byref(h)), "sts_unet_create")
+        self.h = h
+        self.n_params = int(lib().sts_unet_param_count(h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sts_unet_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, flat):
+        flat = np.ascontiguousarray(flat, dtype=np.float32)
+        assert flat.size == self.n_params
+        _check(lib().sts_unet_set_params(self.h, _fp(flat)), "sts_unet_set_params")
+
+    def params(self):
+        out = np.zeros(self.n_params, np.float32)
+        _check(lib().sts_unet_get_params(self.h, _fp(out)), "sts_unet_get_params")
+        return out
+
+    def grads(self):
+        out = np.zeros(self.n_params, np.float32)
+        _check(lib().sts_unet_get_grads(self.h, _fp(out)), "sts_unet_get_grads")
+        return out
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        n = x.shape[0]
+        y = np.zeros((n, self.H, self.W, self.cout), np.float32)
+        _check(lib().sts_unet_forward(self.h, n, _fp(x), _fp(y)), "sts_unet_forward")
+        return y
+
+    def backward_mse(self, t, mask):
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        loss = np.zeros(1)
+        _check(lib().sts_unet_backward_mse(self.h, t.shape[0], _fp(t), m.ctypes.data_as(_BP), _dp(loss)),
+               "sts_unet_backward_mse")
+        return float(loss[0])
+
+    def backward(self, dy):
+        dy = np.ascontiguousarray(dy, dtype=np.float32)
+        _check(lib().sts_unet_backward(self.h, dy.shape[0], _fp(dy)), "sts_unet_backward")
+
+    def adam(self, lr, b1=0.9, b2=0.999, eps=1e-8):
+        _check(lib().sts_unet_adam(self.h, lr, b1, b2, eps), "sts_unet_adam")
+
+    def dataset(self, slots):
+        _check(lib().sts_unet_dataset(self.h, slots), "sts_unet_dataset")
+
+    def record(self, slot, ctx, norm):
+        _check(lib().sts_unet_record(self.h, slot, ctx.h, ctypes.byref(norm)), "sts_unet_record")
+
+    def train_step(self, slots, lr):
+        s = np.ascontiguousarray(slots, dtype=np.int32)
+        loss = np.zeros(1)
+        _check(lib().sts_unet_train_step(self.h, s.ctypes.data_as(_IP), s.size, lr, _dp(loss)),
+               "sts_unet_train_step")
+        return float(loss[0])
+
+    def eval(self, slots):
+        s = np.ascontiguousarray(slots, dtype=np.int32)
+        loss = np.zeros(1)
+        _check(lib().sts_unet_eval(self.h, s.ctypes.data_as(_IP), s.size, _dp(loss)), "sts_unet_eval")
+        return float(loss[0])
 
 
 # ---------------------------------------------------------------------------

@@ -13,6 +13,7 @@
 
 #include "simple_ts.h"
 #include "sts_device.h"
+#include "sts_internal.h"
 #include "sts_launch.h"
 
 namespace {
@@ -59,6 +60,15 @@ struct sts_ctx {
   unsigned long long *log = nullptr;   // 6 residual maxima per outer iteration
   int log_cap = 0;
   int last_iters = 0;
+  // FP32 copy of the state for the mixed-precision time step (precision 32)
+  bool mixed = false;
+  KGridT<float> g32{};
+  float *hx32 = nullptr, *hy32 = nullptr;
+  float *f32[STS_NFIELDS] = {};
+  float *p_alt32 = nullptr, *T_alt32 = nullptr;
+  int last_iters32 = 0, last_iters64m = 0;
+  long long fluid_cells = 0;
+  double *p_ws = nullptr, *T_ws = nullptr;  // predicted starting guess (warm start)
   bool fuse_sweeps = true;  // pairs of loop-3 sweeps in one k_sweep2 pass
   // kernel timing: event pairs around launches of kernel `timed`
   int timed = -1;
@@ -70,6 +80,11 @@ struct sts_ctx {
   KFields fields() const {
     KFields k;
     for (int q = 0; q < STS_NFIELDS; q++) k.f[q] = f[q];
+    return k;
+  }
+  KFieldsT<float> fields32() const {
+    KFieldsT<float> k;
+    for (int q = 0; q < STS_NFIELDS; q++) k.f[q] = f32[q];
     return k;
   }
 };
@@ -108,7 +123,7 @@ void timed_end(sts_ctx *c, int k) {
 }  // namespace
 
 extern "C" int sts_time_kernel(sts_ctx *c, int kernel) {
-  if (!c || kernel < -1 || kernel >= STS_K_COUNT) return fail(STS_ERR_ARG, "bad kernel id");
+  if (!c || kernel < -1 || kernel >= 2 * STS_K_COUNT) return fail(STS_ERR_ARG, "bad kernel id");
   if (c->ev.empty()) {
     c->ev.resize(2 * EV_POOL);
     for (hipEvent_t &e : c->ev) HIP_OK(hipEventCreate(&e));
@@ -171,6 +186,13 @@ extern "C" int sts_destroy(sts_ctx *c) {
   for (int q = 0; q < STS_NFIELDS; q++) (void)hipFree(c->f[q]);
   (void)hipFree(c->p_alt);
   (void)hipFree(c->T_alt);
+  for (int q = 0; q < STS_NFIELDS; q++) (void)hipFree(c->f32[q]);
+  (void)hipFree(c->p_ws);
+  (void)hipFree(c->T_ws);
+  (void)hipFree(c->p_alt32);
+  (void)hipFree(c->T_alt32);
+  (void)hipFree(c->hx32);
+  (void)hipFree(c->hy32);
   (void)hipFree(c->hx);
   (void)hipFree(c->hy);
   (void)hipFree(c->m);
@@ -229,10 +251,44 @@ static int alloc_state(sts_ctx *c, const sts_grid *gi, const std::vector<uint8_t
   return STS_OK;
 }
 
+// FP32 grid: the same case with FP32 mesh steps and scalars.
+static void narrow_grid(sts_ctx *c) {
+  const KGrid &g = c->g;
+  KGridT<float> &h = c->g32;
+  h.Nx = g.Nx; h.Ny = g.Ny;
+  h.hx = c->hx32; h.hy = c->hy32;
+  h.m = g.m; h.vol = g.vol; h.bodies = g.bodies; h.nbodies = g.nbodies;
+  h.Nx_u0 = g.Nx_u0; h.Nx_u1 = g.Nx_u1; h.Ny_u0 = g.Ny_u0; h.Ny_u1 = g.Ny_u1;
+  h.Nx_v0 = g.Nx_v0; h.Nx_v1 = g.Nx_v1; h.Ny_v0 = g.Ny_v0; h.Ny_v1 = g.Ny_v1;
+  h.Nx_o0 = g.Nx_o0; h.Nx_o1 = g.Nx_o1; h.Ny_o0 = g.Ny_o0; h.Ny_o1 = g.Ny_o1;
+  h.ht = (float)g.ht; h.gamma1 = (float)g.gamma1; h.Kn = (float)g.Kn; h.cT = (float)g.cT;
+  h.cT1 = (float)g.cT1; h.cT2 = (float)g.cT2; h.cPch = (float)g.cPch; h.c_Fr_v = (float)g.c_Fr_v;
+  h.given_xb = g.given_xb; h.given_xe = g.given_xe; h.kn_local = g.kn_local;
+  h.bc = g.bc;
+}
+
+static int alloc_fp32(sts_ctx *c, const sts_grid *gi) {
+  const size_t bytes = sizeof(float) * (size_t)c->Na;
+  for (int q = 0; q < STS_NFIELDS; q++) {
+    HIP_OK(hipMalloc(&c->f32[q], bytes));
+    HIP_OK(hipMemsetAsync(c->f32[q], 0, bytes, c->stream));
+  }
+  HIP_OK(hipMalloc(&c->p_alt32, bytes));
+  HIP_OK(hipMalloc(&c->T_alt32, bytes));
+  HIP_OK(hipMalloc(&c->hx32, sizeof(float) * gi->Nx));
+  HIP_OK(hipMalloc(&c->hy32, sizeof(float) * gi->Ny));
+  std::vector<float> hx(gi->hx, gi->hx + gi->Nx), hy(gi->hy, gi->hy + gi->Ny);
+  HIP_OK(hipMemcpyAsync(c->hx32, hx.data(), sizeof(float) * gi->Nx, hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipMemcpyAsync(c->hy32, hy.data(), sizeof(float) * gi->Ny, hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  narrow_grid(c);
+  return STS_OK;
+}
+
 extern "C" int sts_create(const sts_grid *gi, int precision, sts_ctx **out) {
   if (!gi || !out) return fail(STS_ERR_ARG, "null argument");
   *out = nullptr;
-  if (precision != 64) return fail(STS_ERR_UNSUPPORTED, "only precision 64 is provided by sts_create");
+  if (precision != 64 && precision != 32) return fail(STS_ERR_UNSUPPORTED, "precision must be 64 or 32");
   if (gi->Nx < 3 || gi->Ny < 3 || !gi->hx || !gi->hy || !gi->vol_inf_fb)
     return fail(STS_ERR_ARG, "grid needs Nx, Ny >= 3, hx, hy and vol_inf_fb");
   if (gi->nbodies < 0 || (gi->nbodies > 0 && !gi->bodies)) return fail(STS_ERR_ARG, "bad body table");
@@ -257,6 +313,7 @@ extern "C" int sts_create(const sts_grid *gi, int precision, sts_ctx **out) {
       (gi->drhodt_xe && mask[top + (size_t)(gi->Nx_u[1] - 1)]))
     return fail(STS_ERR_UNSUPPORTED, "drhodt_durhodx_dvrhody edge condition on a fluid cell of the top row");
   sts_ctx *c = new sts_ctx();
+  for (uint8_t b : mask) c->fluid_cells += b;
   c->Nx = gi->Nx;
   c->Ny = gi->Ny;
   c->Na = (int)Na;
@@ -290,6 +347,13 @@ extern "C" int sts_create(const sts_grid *gi, int precision, sts_ctx **out) {
   g.given_xe = gi->given_xe;
   g.kn_local = gi->kn_local;
   g.bc = bc_bits(gi);
+  if (precision == 32) {
+    c->mixed = true;
+    if ((arc = alloc_fp32(c, gi))) {
+      sts_destroy(c);
+      return arc;
+    }
+  }
   *out = c;
   return STS_OK;
 }
@@ -312,6 +376,7 @@ extern "C" int sts_set_scalars(sts_ctx *c, double cT, double cPch) {
   if (!c) return fail(STS_ERR_ARG, "null context");
   c->g.cT = cT;
   c->g.cPch = cPch;
+  if (c->mixed) narrow_grid(c);
   return STS_OK;
 }
 
@@ -367,14 +432,14 @@ static int run_sweeps(sts_ctx *c, int n) {
 
 // Folds words 0..nslots-1 of the slotted residual buffer (clearing them) and
 // brings them to h_red.
-static int read_red(sts_ctx *c, int nslots, bool with_probe) {
+static int read_red(sts_ctx *c, int nslots, bool with_probe, bool check_finite = true) {
   HIP_OK(launch_fold(c->red, nslots, c->red_out, c->stream));
   HIP_OK(hipMemcpyAsync(c->h_red, c->red_out, sizeof(unsigned long long) * nslots, hipMemcpyDeviceToHost,
                         c->stream));
   if (with_probe)
     HIP_OK(hipMemcpyAsync(c->h_red + NRED, c->probe, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_OK(hipStreamSynchronize(c->stream));
-  for (int q = 0; q < nslots; q++)
+  for (int q = 0; q < nslots && check_finite; q++)
     if (!std::isfinite(bits_to_double(c->h_red[q])))
       return fail(STS_ERR_NONFINITE, "non-finite residual (slot " + std::to_string(q) + ")");
   return STS_OK;
@@ -543,102 +608,267 @@ extern "C" int sts_outer_iteration(sts_ctx *c, int n_pc, double *res, double *re
 // Buffer flips (passes) of one loop 3 of n sweeps.
 static int sweep_passes(const sts_ctx *c, int n) { return c->fuse_sweeps ? n / 2 + n % 2 : n; }
 
-// Enqueues outer iteration k of the current step; the p/T buffer parity at
-// its start follows the passes of the iterations before it.
-static int enqueue_outer(sts_ctx *c, int k, const sts_loop_control *lc, double *p0, double *T0, double *p1,
-                         double *T1) {
-  const int n = lc->N_I_p_c;
+// One precision's view of the state for loop 2: grid, fields (p and T as
+// they stand at the start of the step) and the p/T ping-pong partners.
+template <typename R>
+struct Loop2State {
+  const KGridT<R> *g;
+  KFieldsT<R> f;
+  R *p_alt, *T_alt;
+  int kid;  // timing id offset: 0 for FP64 kernels, STS_K_COUNT for FP32
+};
+
+// Enqueues outer iteration k of the current loop 2; the p/T buffer parity at
+// its start follows the passes of the iterations before it.  The kernels
+// leave their residual maxima in the slotted words lred; the edge kernel folds
+// them into log[6k..6k+5] and runs the loop test.
+template <typename R>
+static int enqueue_outer(sts_ctx *c, const Loop2State<R> &st, int k, int n, LoopTest lt) {
+  const KGridT<R> &g = *st.g;
   const Gate gate{c->ctrl, k};
-  unsigned long long *lk = c->log + 6 * (size_t)k, *lr = c->lred;
+  R *const p0 = st.f.f[STS_P], *const T0 = st.f.f[STS_T], *const p1 = st.p_alt, *const T1 = st.T_alt;
+  unsigned long long *lr = c->lred;
   auto fields_at = [&](int parity) {
-    KFields kf = c->fields();
+    KFieldsT<R> kf = st.f;
     kf.f[STS_P] = parity ? p1 : p0;
     kf.f[STS_T] = parity ? T1 : T0;
     return kf;
   };
   int par = (int)(((long long)k * sweep_passes(c, n)) & 1);
-  int t = timed_begin(c, STS_K_ASSEMBLE);
-  HIP_OK(launch_assemble(c->g, fields_at(par), c->stream, gate));
+  int t = timed_begin(c, st.kid + STS_K_ASSEMBLE);
+  HIP_OK(launch_assemble(g, fields_at(par), c->stream, gate));
   timed_end(c, t);
   for (int s = 0; s < n;) {
-    KFields kf = fields_at(par);
-    double *Pi = par ? p1 : p0, *Ti = par ? T1 : T0, *Po = par ? p0 : p1, *To = par ? T0 : T1;
-    t = timed_begin(c, STS_K_SWEEP);
+    KFieldsT<R> kf = fields_at(par);
+    R *Pi = par ? p1 : p0, *Ti = par ? T1 : T0, *Po = par ? p0 : p1, *To = par ? T0 : T1;
+    t = timed_begin(c, st.kid + STS_K_SWEEP);
     if (c->fuse_sweeps && s + 1 < n) {
       // sweep s (first?) and s+1 (last?)
-      HIP_OK(launch_sweep2(c->g, kf, Pi, Ti, Po, To, s == 0 ? lr + 4 : nullptr, s == 0 ? lr + 5 : nullptr,
+      HIP_OK(launch_sweep2(g, kf, Pi, Ti, Po, To, s == 0 ? lr + 4 : nullptr, s == 0 ? lr + 5 : nullptr,
                            s + 1 == n - 1 ? lr + 2 : nullptr, s + 1 == n - 1 ? lr + 3 : nullptr, c->stream, gate));
       s += 2;
     } else {
-      HIP_OK(launch_sweep(c->g, kf, Pi, Ti, Po, To, s == n - 1 ? lr + 2 : nullptr, s == n - 1 ? lr + 3 : nullptr,
-                          c->stream, s == 0 ? lr + 4 : nullptr, s == 0 ? lr + 5 : nullptr, gate));
+      HIP_OK(launch_sweep(g, kf, (const R *)Pi, (const R *)Ti, Po, To, s == n - 1 ? lr + 2 : nullptr,
+                          s == n - 1 ? lr + 3 : nullptr, c->stream, s == 0 ? lr + 4 : nullptr,
+                          s == 0 ? lr + 5 : nullptr, gate));
       s += 1;
     }
     timed_end(c, t);
     par ^= 1;
   }
-  const KFields kf = fields_at(par);
-  t = timed_begin(c, STS_K_CORRECT);
-  HIP_OK(launch_correct(c->g, kf, lr + 0, lr + 1, 1, c->stream, gate));
+  const KFieldsT<R> kf = fields_at(par);
+  t = timed_begin(c, st.kid + STS_K_CORRECT);
+  HIP_OK(launch_correct(g, kf, lr + 0, lr + 1, 1, c->stream, gate));
   timed_end(c, t);
-  t = timed_begin(c, STS_K_WALL);
-  HIP_OK(launch_wall(c->g, kf, STS_BC_SLIP_U | STS_BC_SLIP_V | STS_BC_JUMP, c->stream, gate));
+  t = timed_begin(c, st.kid + STS_K_WALL);
+  HIP_OK(launch_wall(g, kf, STS_BC_SLIP_U | STS_BC_SLIP_V | STS_BC_JUMP, c->stream, gate));
   timed_end(c, t);
-  const LoopTest lt{c->ctrl, lr, lk, lc->N_I, lc->tol_uv, lc->tol_p, lc->tol_T};
-  t = timed_begin(c, STS_K_EDGES);
-  HIP_OK(launch_edges(c->g, kf, STS_BC_VELOCITIES | STS_BC_TEMPERATURE | STS_BC_PRESSURE, c->probe, c->stream, gate,
+  lt.lred = lr;
+  lt.log_k += 6 * (size_t)k;
+  t = timed_begin(c, st.kid + STS_K_EDGES);
+  HIP_OK(launch_edges(g, kf, STS_BC_VELOCITIES | STS_BC_TEMPERATURE | STS_BC_PRESSURE, c->probe, c->stream, gate,
                       lt));
   timed_end(c, t);
-  t = timed_begin(c, STS_K_FLUX_DIFF);
-  HIP_OK(launch_flux_diff(c->g, kf, c->stream, gate));
+  t = timed_begin(c, st.kid + STS_K_FLUX_DIFF);
+  HIP_OK(launch_flux_diff(g, kf, c->stream, gate));
   timed_end(c, t);
   return STS_OK;
+}
+
+// Runs one loop 2 on the device: outer iterations are enqueued in batches
+// (the first of `batch`), each gated on the loop control, and the host checks
+// the control once per batch.  lt carries the test (tolerances, cap N_I,
+// stagnation) and the log base.  On return *iters holds the iterations run
+// and (p, T) of the state are where the last pass left them.
+template <typename R>
+static int run_loop2(sts_ctx *c, Loop2State<R> &st, R *&p, R *&T, R *&p_alt, R *&T_alt, int n_pc, LoopTest lt,
+                     int batch, int *iters) {
+  HIP_OK(launch_loop_open(c->ctrl, lt.log_k, 6 * lt.N_I, c->lred, c->stream));
+  int k = 0, stop = -1, rc;
+  while (stop < 0) {
+    const int end = std::min(lt.N_I, k + batch);
+    for (; k < end; k++)
+      if ((rc = enqueue_outer(c, st, k, n_pc, lt))) return rc;
+    HIP_OK(hipMemcpyAsync(c->h_ctrl, c->ctrl, sizeof(LoopCtrl), hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    if (c->h_ctrl->stop_at != 0x7fffffff) stop = c->h_ctrl->stop_at;
+    else if (k >= lt.N_I) return fail(STS_ERR_ARG, "loop 2 did not stop at N_I");  // the test always stops there
+    batch = 2;
+  }
+  *iters = stop + 1;
+  if (((long long)*iters * sweep_passes(c, n_pc)) & 1) {  // odd number of passes: result in the partners
+    std::swap(p, p_alt);
+    std::swap(T, T_alt);
+  }
+  if (c->h_ctrl->nonfinite) return fail(STS_ERR_NONFINITE, "non-finite residual in outer iteration " + std::to_string(*iters));
+  return STS_OK;
+}
+
+static int ensure_loop_buffers(sts_ctx *c, int n_iter) {
+  if (n_iter > c->log_cap) {
+    (void)hipFree(c->log);
+    c->log = nullptr;
+    HIP_OK(hipMalloc(&c->log, sizeof(unsigned long long) * 6 * (size_t)n_iter));
+    c->log_cap = n_iter;
+  }
+  if (!c->ctrl) {
+    HIP_OK(hipMalloc(&c->ctrl, sizeof(LoopCtrl)));
+    HIP_OK(hipHostMalloc(&c->h_ctrl, sizeof(LoopCtrl), hipHostMallocDefault));
+  }
+  return STS_OK;
+}
+
+// Copies n_iter iterations of the residual log and the probe to the host.
+static int finish_step(sts_ctx *c, int n_iter, double *res_log, double *p_probe) {
+  std::vector<unsigned long long> lg(6 * (size_t)n_iter);
+  if (n_iter)
+    HIP_OK(hipMemcpyAsync(lg.data(), c->log, sizeof(unsigned long long) * lg.size(), hipMemcpyDeviceToHost,
+                          c->stream));
+  HIP_OK(hipMemcpyAsync(&c->h_red[NRED], c->probe, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  if (res_log)
+    for (size_t q = 0; q < lg.size(); q++) res_log[q] = bits_to_double(lg[q]);
+  if (p_probe) std::memcpy(p_probe, &c->h_red[NRED], sizeof(double));
+  return STS_OK;
+}
+
+static LoopTest loop_test(sts_ctx *c, unsigned long long *log, int N_I, double tol_uv, double tol_p, double tol_T,
+                          int stall_w, double stall_ratio) {
+  return LoopTest{c->ctrl, c->lred, log, N_I, tol_uv, tol_p, tol_T, stall_w, stall_ratio};
 }
 
 extern "C" int sts_time_step(sts_ctx *c, const sts_loop_control *lc, int *iterations, double *res_log,
                              double *p_probe) {
   if (!c || !lc || lc->N_I < 1 || lc->N_I_p_c < 1 || lc->N_I_p_c > MAX_SWEEPS)
     return fail(STS_ERR_ARG, "bad loop control");
-  if (lc->N_I > c->log_cap) {
-    (void)hipFree(c->log);
-    c->log = nullptr;
-    HIP_OK(hipMalloc(&c->log, sizeof(unsigned long long) * 6 * (size_t)lc->N_I));
-    c->log_cap = lc->N_I;
-  }
-  if (!c->ctrl) {
-    HIP_OK(hipMalloc(&c->ctrl, sizeof(LoopCtrl)));
-    HIP_OK(hipHostMalloc(&c->h_ctrl, sizeof(LoopCtrl), hipHostMallocDefault));
-  }
-  int rc = sts_save_previous(c);
-  if (rc) return rc;
-  HIP_OK(launch_loop_open(c->ctrl, c->log, 6 * lc->N_I, c->lred, c->stream));
-  double *const p0 = c->f[STS_P], *const T0 = c->f[STS_T], *const p1 = c->p_alt, *const T1 = c->T_alt;
-  int k = 0, batch = lc->batch > 0 ? lc->batch : (c->last_iters > 2 ? c->last_iters - 1 : 1);
-  int stop = -1;
-  while (stop < 0) {
-    const int end = std::min(lc->N_I, k + batch);
-    for (; k < end; k++)
-      if ((rc = enqueue_outer(c, k, lc, p0, T0, p1, T1))) return rc;
-    HIP_OK(hipMemcpyAsync(c->h_ctrl, c->ctrl, sizeof(LoopCtrl), hipMemcpyDeviceToHost, c->stream));
-    HIP_OK(hipStreamSynchronize(c->stream));
-    if (c->h_ctrl->stop_at != 0x7fffffff) stop = c->h_ctrl->stop_at;
-    else if (k >= lc->N_I) return fail(STS_ERR_ARG, "loop 2 did not stop at N_I");  // the test always stops there
-    batch = 2;
-  }
-  const int iters = stop + 1;
+  int rc;
+  if ((rc = ensure_loop_buffers(c, lc->N_I))) return rc;
+  if ((rc = sts_save_previous(c))) return rc;
+  Loop2State<double> st{&c->g, c->fields(), c->p_alt, c->T_alt, 0};
+  const int batch = lc->batch > 0 ? lc->batch : (c->last_iters > 2 ? c->last_iters - 1 : 1);
+  int iters = 0;
+  rc = run_loop2(c, st, c->f[STS_P], c->f[STS_T], c->p_alt, c->T_alt, lc->N_I_p_c,
+                 loop_test(c, c->log, lc->N_I, lc->tol_uv, lc->tol_p, lc->tol_T, 0, 0.0), batch, &iters);
   c->last_iters = iters;
-  if (((long long)iters * sweep_passes(c, lc->N_I_p_c)) & 1) {  // odd number of passes: result in the partners
-    std::swap(c->f[STS_P], c->p_alt);
-    std::swap(c->f[STS_T], c->T_alt);
-  }
-  std::vector<unsigned long long> lg(6 * (size_t)iters);
-  HIP_OK(hipMemcpyAsync(lg.data(), c->log, sizeof(unsigned long long) * lg.size(), hipMemcpyDeviceToHost, c->stream));
-  HIP_OK(hipMemcpyAsync(&c->h_red[NRED], c->probe, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIP_OK(hipStreamSynchronize(c->stream));
   if (iterations) *iterations = iters;
-  if (res_log)
-    for (size_t q = 0; q < lg.size(); q++) res_log[q] = bits_to_double(lg[q]);
-  if (p_probe) std::memcpy(p_probe, &c->h_red[NRED], sizeof(double));
-  if (c->h_ctrl->nonfinite) return fail(STS_ERR_NONFINITE, "non-finite residual in outer iteration " + std::to_string(iters));
+  int rc2 = finish_step(c, iters, res_log, p_probe);
+  return rc ? rc : rc2;
+}
+
+// Fields an outer iteration reads before it writes them: the FP32 phase hands
+// these to the FP64 phase.
+static constexpr unsigned long long bit(int f) { return 1ull << f; }
+static constexpr unsigned long long WIDEN_MASK =
+    bit(STS_U) | bit(STS_V) | bit(STS_P) | bit(STS_T) | bit(STS_RHO) | bit(STS_SQRT_T) | bit(STS_RHO_U) |
+    bit(STS_RHO_V) | bit(STS_FX) | bit(STS_FY) | bit(STS_GAMMA_XF) | bit(STS_GAMMA_YF) | bit(STS_D_UX) |
+    bit(STS_D_UY) | bit(STS_D_VX) | bit(STS_D_VY) | bit(STS_DTX) | bit(STS_DTY) | bit(STS_TW_U) | bit(STS_TW_V);
+
+// Loop 2 after the prologue: the FP32 phase (when mc asks for one), then the
+// FP64 phase whose test is the reference's.  Logs go to c->log, FP32 first.
+static int run_phases(sts_ctx *c, const sts_loop_control *lc, const sts_mixed_control *mc, int *n32p, int *n64p) {
+  int rc, n32 = 0, n64 = 0;
+  *n32p = *n64p = 0;
+  if (mc && mc->N_I32 > 0) {
+    const unsigned long long all = (1ull << STS_NFIELDS) - 1ull;
+    HIP_OK(launch_convert(c->fields(), c->fields32(), all, c->Na, c->stream));
+    Loop2State<float> s32{&c->g32, c->fields32(), c->p_alt32, c->T_alt32, STS_K_COUNT};
+    const int b32 = lc->batch > 0 ? lc->batch : (c->last_iters32 > 2 ? c->last_iters32 - 1 : 1);
+    rc = run_loop2(c, s32, c->f32[STS_P], c->f32[STS_T], c->p_alt32, c->T_alt32, lc->N_I_p_c,
+                   loop_test(c, c->log, mc->N_I32, mc->tol32_uv, mc->tol32_p, mc->tol32_T, mc->stall_w,
+                             mc->stall_ratio),
+                   b32, &n32);
+    c->last_iters32 = n32;
+    *n32p = n32;
+    if (rc) return rc;
+    HIP_OK(launch_convert(c->fields32(), c->fields(), WIDEN_MASK, c->Na, c->stream));
+  }
+  Loop2State<double> s64{&c->g, c->fields(), c->p_alt, c->T_alt, 0};
+  int &last = mc ? c->last_iters64m : c->last_iters;
+  const int b64 = lc->batch > 0 ? lc->batch : (last > 2 ? last - 1 : 1);
+  rc = run_loop2(c, s64, c->f[STS_P], c->f[STS_T], c->p_alt, c->T_alt, lc->N_I_p_c,
+                 loop_test(c, c->log + 6 * (size_t)n32, lc->N_I, lc->tol_uv, lc->tol_p, lc->tol_T, 0, 0.0), b64,
+                 &n64);
+  last = n64;
+  *n64p = n64;
+  return rc;
+}
+
+static int check_mixed(const sts_ctx *c, const sts_loop_control *lc, const sts_mixed_control *mc) {
+  if (!lc || lc->N_I < 1 || lc->N_I_p_c < 1 || lc->N_I_p_c > MAX_SWEEPS) return fail(STS_ERR_ARG, "bad loop control");
+  if (mc && (mc->N_I32 < 0 || mc->stall_w < 0)) return fail(STS_ERR_ARG, "bad mixed control");
+  if (mc && mc->N_I32 > 0 && !c->mixed) return fail(STS_ERR_UNSUPPORTED, "the context was not created with precision 32");
   return STS_OK;
+}
+
+extern "C" int sts_time_step_mixed(sts_ctx *c, const sts_loop_control *lc, const sts_mixed_control *mc,
+                                   int *iters32, int *iters64, double *res_log, double *p_probe) {
+  if (!c || !mc) return fail(STS_ERR_ARG, "null argument");
+  int rc;
+  if ((rc = check_mixed(c, lc, mc))) return rc;
+  if ((rc = ensure_loop_buffers(c, lc->N_I + mc->N_I32))) return rc;
+  if ((rc = sts_save_previous(c))) return rc;
+  int n32 = 0, n64 = 0;
+  rc = run_phases(c, lc, mc, &n32, &n64);
+  if (iters32) *iters32 = n32;
+  if (iters64) *iters64 = n64;
+  int rc2 = finish_step(c, n32 + n64, res_log, p_probe);
+  return rc ? rc : rc2;
+}
+
+// ---------------------------------------------------------------------------
+// Warm start (include/sts_unet.h)
+// ---------------------------------------------------------------------------
+int sts_internal_view(sts_ctx *c, StsView *v) {
+  if (!c || !v) return fail(STS_ERR_ARG, "null argument");
+  v->Nx = c->Nx;
+  v->Ny = c->Ny;
+  for (int q = 0; q < STS_NFIELDS; q++) v->f[q] = c->f[q];
+  v->mask = c->m;
+  v->fluid_cells = c->fluid_cells;
+  v->stream = c->stream;
+  return STS_OK;
+}
+
+void sts_internal_set_error(const std::string &msg) { g_err = msg; }
+
+extern "C" int sts_time_step_ws(sts_ctx *c, sts_unet *net, const sts_unet_norm *nrm, const sts_loop_control *lc,
+                                const sts_mixed_control *mc, int *used, int *iters32, int *iters64, double *res_log,
+                                double *p_probe) {
+  if (!c || !net || !nrm) return fail(STS_ERR_ARG, "null argument");
+  int rc;
+  if ((rc = check_mixed(c, lc, mc))) return rc;
+  if ((rc = ensure_loop_buffers(c, lc->N_I + (mc ? mc->N_I32 : 0)))) return rc;
+  if (!c->p_ws) {
+    HIP_OK(hipMalloc(&c->p_ws, sizeof(double) * (size_t)c->Na));
+    HIP_OK(hipMalloc(&c->T_ws, sizeof(double) * (size_t)c->Na));
+  }
+  if ((rc = sts_save_previous(c))) return rc;
+  StsView v;
+  sts_internal_view(c, &v);
+  if ((rc = sts_internal_unet_predict(net, v, nrm, c->p_ws, c->T_ws))) return rc;
+  // First sweep of iteration 1 from both guesses; the coefficients do not
+  // depend on the guess (assembly reads neither p nor T).
+  HIP_OK(hipMemsetAsync(c->red, 0, sizeof(unsigned long long) * NSLOT * SLOT_STRIDE, c->stream));
+  HIP_OK(launch_assemble(c->g, c->fields(), c->stream));
+  HIP_OK(launch_sweep(c->g, c->fields(), (const double *)c->f[STS_P], (const double *)c->f[STS_T], c->p_alt, c->T_alt,
+                      c->red + 2, c->red + 3, c->stream));
+  HIP_OK(launch_sweep(c->g, c->fields(), (const double *)c->p_ws, (const double *)c->T_ws, c->p_alt, c->T_alt,
+                      c->red + 4, c->red + 5, c->stream));
+  if ((rc = read_red(c, 6, false, false))) return rc;
+  if (!std::isfinite(bits_to_double(c->h_red[2])) || !std::isfinite(bits_to_double(c->h_red[3])))
+    return fail(STS_ERR_NONFINITE, "non-finite residual of the first sweep");
+  const double rA = std::max(bits_to_double(c->h_red[2]) / lc->tol_p, bits_to_double(c->h_red[3]) / lc->tol_T);
+  const double rB = std::max(bits_to_double(c->h_red[4]) / lc->tol_p, bits_to_double(c->h_red[5]) / lc->tol_T);
+  const bool take = rB < rA;
+  if (take) {
+    HIP_OK(hipMemcpyAsync(c->f[STS_P], c->p_ws, sizeof(double) * (size_t)c->Na, hipMemcpyDeviceToDevice, c->stream));
+    HIP_OK(hipMemcpyAsync(c->f[STS_T], c->T_ws, sizeof(double) * (size_t)c->Na, hipMemcpyDeviceToDevice, c->stream));
+  }
+  if (used) *used = take ? 1 : 0;
+  int n32 = 0, n64 = 0;
+  rc = run_phases(c, lc, mc, &n32, &n64);
+  if (iters32) *iters32 = n32;
+  if (iters64) *iters64 = n64;
+  int rc2 = finish_step(c, n32 + n64, res_log, p_probe);
+  return rc ? rc : rc2;
 }

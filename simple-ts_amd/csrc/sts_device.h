@@ -23,22 +23,29 @@ enum : unsigned {
   BC_DTDX_XB = 1u << 17, BC_DTDX_XE = 1u << 18, BC_DTDY_YB = 1u << 19, BC_DTDY_YE = 1u << 20,
 };
 
-struct KGrid {
+// Grid, scalars and switches of one case; R is the arithmetic type of the
+// kernels that take it (double, or float for the FP32 phase of the mixed
+// precision time step).
+template <typename R>
+struct KGridT {
   int Nx, Ny;
-  const double *hx, *hy;
+  const R *hx, *hy;
   const uint8_t *m;       // vol_inf_fb_bool
   const uint32_t *vol;    // vol_inf_fb
   const DevBody *bodies;
   int nbodies;
   int Nx_u0, Nx_u1, Ny_u0, Ny_u1, Nx_v0, Nx_v1, Ny_v0, Ny_v1, Nx_o0, Nx_o1, Ny_o0, Ny_o1;
-  double ht, gamma1, Kn, cT, cT1, cT2, cPch, c_Fr_v;
+  R ht, gamma1, Kn, cT, cT1, cT2, cPch, c_Fr_v;
   int given_xb, given_xe, kn_local;
   unsigned bc;
 };
+using KGrid = KGridT<double>;
 
-struct KFields {
-  double *f[STS_NFIELDS];
+template <typename R>
+struct KFieldsT {
+  R *f[STS_NFIELDS];
 };
+using KFields = KFieldsT<double>;
 
 // Device-side loop-2 control (S:4406-4450 evaluated on the GPU): the outer
 // iterations of one time step are enqueued ahead; iteration k runs only while
@@ -73,6 +80,8 @@ struct LoopTest {
   unsigned long long *log_k;
   int N_I;
   double tol_uv, tol_p, tol_T;
+  int stall_w;         // > 0: also stop when the residuals stagnate (see k_edges)
+  double stall_ratio;
 };
 
 // Clamped neighbour indices of cell (i, j).
@@ -98,16 +107,19 @@ struct Nbr {
   }
 };
 
-__device__ __forceinline__ double maxi0(double a) {  // maximum(0, a), S:121
-  return (0.0 > a) ? 0.0 : a;
+template <typename R>
+__device__ __forceinline__ R maxi0(R a) {  // maximum(0, a), S:121
+  return (R(0.0) > a) ? R(0.0) : a;
 }
-__device__ __forceinline__ double upwind_rho(double r0, double r1, double v1) {  // S:130
-  return (v1 == 0) ? (0.5 * (r0 + r1)) : ((v1 > 0) ? r0 : r1);
+template <typename R>
+__device__ __forceinline__ R upwind_rho(R r0, R r1, R v1) {  // S:130
+  return (v1 == R(0)) ? (R(0.5) * (r0 + r1)) : ((v1 > R(0)) ? r0 : r1);
 }
-__device__ __forceinline__ double Li(double f1, double f2, double h1, double h2) {  // S:133
+template <typename R>
+__device__ __forceinline__ R Li(R f1, R f2, R h1, R h2) {  // S:133
   return (f1 * h2 + f2 * h1) / (h1 + h2);
 }
-__device__ __forceinline__ double Hi(double f1, double f2, double h1, double h2) {  // S:136
+template <typename R>
+__device__ __forceinline__ R Hi(R f1, R f2, R h1, R h2) {  // S:136
   return (h1 + h2) * f1 * f2 / (h1 * f2 + h2 * f1);
 }
-__device__ __forceinline__ double bd(unsigned b) { return (double)b; }

@@ -310,7 +310,7 @@ struct Driver {
     if (!c.flag("ToReadSolvedDataFromFile")) write_config();
 
     c.fill_grid(grid);
-    ck(sts_create(&grid, 64, &ctx), "sts_create");
+    ck(sts_create(&grid, mixed ? 32 : 64, &ctx), "sts_create");
     max_res_p.assign(N_I, 0.0);
     max_res_T.assign(N_I, 0.0);
     max_res_p0.assign(N_I, 0.0);
@@ -388,7 +388,8 @@ struct Driver {
     time_solve_start = time(nullptr);
     const double eV = c.get("MaxError_Velocities"), ep = c.get("MaxError_p_c"), eT = c.get("MaxError_T");
     const double es = c.get("MaxError_to_stop_program");
-    std::vector<double> lg(6 * (size_t)N_I);
+    if (mc.N_I32 < 0) mc.N_I32 = (int)N_I;
+    std::vector<double> lg(6 * (size_t)N_I), lg32(6 * ((size_t)N_I + (size_t)mc.N_I32));
     do {  // loop 1
       if (!fused) ck(sts_save_previous(ctx), "save_previous");
       for (unsigned k = 0; k < Iter; k++) {
@@ -405,7 +406,15 @@ struct Driver {
         // loop 2 with the test on the GPU (same test, same order of work)
         sts_loop_control lc{(int)N_I, (int)N_I_p_c, eV, ep, eT, 0};
         int iters = 0;
-        ck(sts_time_step(ctx, &lc, &iters, lg.data(), &probe), "time_step");
+        if (mixed) {
+          // FP32 phase first; the FP64 phase's iterations are the reference's Iter
+          int n32 = 0;
+          ck(sts_time_step_mixed(ctx, &lc, &mc, &n32, &iters, lg32.data(), &probe), "time_step_mixed");
+          std::copy(lg32.begin() + 6 * (size_t)n32, lg32.begin() + 6 * (size_t)(n32 + iters), lg.begin());
+          outer_total32 += n32;
+        } else {
+          ck(sts_time_step(ctx, &lc, &iters, lg.data(), &probe), "time_step");
+        }
         outer_total += iters;
         Iter_p_c = N_I_p_c;
         for (int k = 0; k < iters; k++) {
@@ -511,6 +520,10 @@ struct Driver {
     ck(sts_destroy(ctx), "sts_destroy");
   }
   double loop_seconds = 0;
+  // mixed-precision time steps (sts_time_step_mixed) when set
+  bool mixed = false;
+  sts_mixed_control mc{0, 1e-6, 1e-6, 1e-6, 4, 0.7};
+  long long outer_total32 = 0, outer32_at_start = 0;
   // timing of the steps after the warm-up ones (benchmarking)
   unsigned warmup_steps = 0;
   int time_kernel = -1;
@@ -522,6 +535,7 @@ struct Driver {
     if (time_kernel >= 0) ck(sts_time_kernel(ctx, time_kernel), "sts_time_kernel");
     t = std::chrono::steady_clock::now();
     outer0 = outer_total;
+    outer32_at_start = outer_total32;
   }
 };
 
@@ -531,19 +545,33 @@ int main(int argc, char **argv) {
   std::string dir = ".", timing;
   unsigned warmup = 0;
   int kernel = -1;
-  const char *knames[] = {"assemble", "sweep", "correct", "wall", "edges", "flux_diff"};
+  const char *knames[] = {"assemble",   "sweep",   "correct",   "wall",   "edges",   "flux_diff",
+                          "assemble32", "sweep32", "correct32", "wall32", "edges32", "flux_diff32"};
+  bool mixed = false;
+  sts_mixed_control mc{0, 1e-6, 1e-6, 1e-6, 4, 0.7};
+  int n32 = -1;
   for (int a = 1; a < argc; a++) {
     std::string s = argv[a];
     if (s == "--timing" && a + 1 < argc) timing = argv[++a];
     else if (s == "--warmup" && a + 1 < argc) warmup = (unsigned)std::atoi(argv[++a]);
     else if (s == "--time-kernel" && a + 1 < argc) {
       std::string k = argv[++a];
-      for (int q = 0; q < STS_K_COUNT; q++)
+      for (int q = 0; q < 2 * STS_K_COUNT; q++)
         if (k == knames[q]) kernel = q;
       if (kernel < 0) die("unknown kernel " + k);
+    } else if (s == "--mixed") {
+      mixed = true;
+    } else if (s == "--n32" && a + 1 < argc) {
+      n32 = std::atoi(argv[++a]);
+    } else if (s == "--tol32" && a + 1 < argc) {
+      mc.tol32_uv = mc.tol32_p = mc.tol32_T = std::atof(argv[++a]);
+    } else if (s == "--stall" && a + 2 < argc) {
+      mc.stall_w = std::atoi(argv[++a]);
+      mc.stall_ratio = std::atof(argv[++a]);
     } else if (s == "-h" || s == "--help") {
       std::cout << "usage: sts_gpu [case_dir] [--timing file.json] [--warmup steps] [--time-kernel "
-                   "assemble|sweep|correct|wall|edges|flux_diff]"
+                   "assemble|sweep|correct|wall|edges|flux_diff[32]] [--mixed [--n32 cap] [--tol32 tol] "
+                   "[--stall window ratio]]"
                 << std::endl;
       return 0;
     } else dir = s;
@@ -552,6 +580,9 @@ int main(int argc, char **argv) {
   d.dir = dir;
   d.warmup_steps = warmup;
   d.time_kernel = kernel;
+  d.mixed = mixed;
+  mc.N_I32 = n32;
+  d.mc = mc;
   std::string err = sts::load_case(dir, d.c);
   if (!err.empty()) die(err);
   d.run();
@@ -562,7 +593,9 @@ int main(int argc, char **argv) {
       << ", \"warmup_steps\": " << d.warmup_steps << ", \"timed_steps\": " << d.timed_steps
       << ", \"timed_outer_iterations\": " << d.timed_outer << ", \"timed_seconds\": " << d.timed_seconds
       << ", \"kernel\": \"" << (kernel >= 0 ? knames[kernel] : "") << "\", \"kernel_launches\": "
-      << d.kernel_launches << ", \"kernel_ms\": " << d.kernel_ms << "}" << std::endl;
+      << d.kernel_launches << ", \"kernel_ms\": " << d.kernel_ms << ", \"mixed\": " << (mixed ? 1 : 0)
+      << ", \"fp32_iterations\": " << d.outer_total32
+      << ", \"timed_fp32_iterations\": " << d.outer_total32 - d.outer32_at_start << "}" << std::endl;
   }
   return 0;
 }

@@ -20,9 +20,11 @@ __device__ __forceinline__ bool in_box(int i, int j, int x0, int x1, int y0, int
 }
 
 // Non-negative doubles order like their bit patterns; a NaN (0x7ff8...)
-// compares above +inf and so survives the max and is reported.
-__device__ __forceinline__ unsigned long long absbits(double r) {
-  return (unsigned long long)__double_as_longlong(fabs(r));
+// compares above +inf and so survives the max and is reported.  FP32
+// differences are widened to double (exactly) first.
+template <typename R>
+__device__ __forceinline__ unsigned long long absbits(R r) {
+  return (unsigned long long)__double_as_longlong(fabs((double)r));
 }
 
 __device__ __forceinline__ unsigned long long wave_max(unsigned long long v) {
@@ -71,106 +73,110 @@ __device__ __forceinline__ void block_max_atomic(unsigned long long (&v)[NV], un
 // ---------------------------------------------------------------------------
 // A2 + A3 + A4: coefficient assembly (S:3269-3756)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void cell_assemble_u(const KGrid &g, const KFields &fl, const Nbr &n) {
+template <typename R>
+__device__ __forceinline__ void cell_assemble_u(const KGridT<R> &g, const KFieldsT<R> &fl, const Nbr &n) {
   const uint8_t *m = g.m;
-  const double *hx = g.hx, *hy = g.hy;
-  const double *Fx = FLD(FX), *Fy = FLD(FY), *u = FLD(U), *v = FLD(V), *sqrt_T = FLD(SQRT_T);
-  const double *Gamma_yf = FLD(GAMMA_YF), *D_ux = FLD(D_UX), *D_uy = FLD(D_UY);
-  const double *rho_pr = FLD(RHO_PR), *u_pr = FLD(U_PR), *rho = FLD(RHO), *rho_u = FLD(RHO_U);
+  const R *hx = g.hx, *hy = g.hy;
+  const R *Fx = FLD(FX), *Fy = FLD(FY), *u = FLD(U), *v = FLD(V), *sqrt_T = FLD(SQRT_T);
+  const R *Gamma_yf = FLD(GAMMA_YF), *D_ux = FLD(D_UX), *D_uy = FLD(D_UY);
+  const R *rho_pr = FLD(RHO_PR), *u_pr = FLD(U_PR), *rho = FLD(RHO), *rho_u = FLD(RHO_U);
   const unsigned i = n.i, j = n.j, ij = n.ij, i_1j = n.i_1j, i1j = n.i1j, ij_1 = n.ij_1, ij1 = n.ij1;
   const unsigned i_1j_1 = n.i_1j_1, i_1j1 = n.i_1j1, i_2j = n.i_2j, i_1 = n.i_1;
-  const double cT = g.cT, ht = g.ht;
-  double F_xu1 = 0.5 * (Fx[i_1j] + Fx[ij]);
-  double F_xu2 = 0.5 * (Fx[ij] + Fx[i1j]);
-  double Scu = cT * (bd(m[i1j]) * u[i1j] * sqrt_T[ij] * hy[j] / (hx[i] * 3.0)
-                     + bd(m[i_2j]) * u[i_1j] * sqrt_T[i_1j] * hy[j] / (hx[i_1] * 3.0)
-                     + Gamma_yf[ij1] * (v[ij1] * bd(m[ij1]) - v[i_1j1] * bd(m[i_1j1]))
-                     - Gamma_yf[ij] * (v[ij] * bd(m[ij_1]) - v[i_1j] * bd(m[i_1j_1]))
-                     - sqrt_T[ij] * (v[ij1] * bd(m[ij1]) - v[ij] * bd(m[ij_1])) * 2.0 / 3.0
-                     + sqrt_T[i_1j] * (v[i_1j1] * bd(m[i_1j1]) - v[i_1j] * bd(m[i_1j_1])) * 2.0 / 3.0);
-  double Spu = -(D_ux[ij] + D_ux[i1j]) / 3.0;
-  double au1 = maxi0(F_xu1) + D_ux[ij];
-  double au2 = maxi0(-F_xu2) + D_ux[i1j];
-  double au3 = 0.5 * (maxi0(Fy[i_1j]) + maxi0(Fy[ij])) + D_uy[ij];
-  double au4 = 0.5 * (maxi0(-Fy[i_1j1]) + maxi0(-Fy[ij1])) + D_uy[ij1];
-  double bu3 = (rho_pr[i_1j] * hx[i_1] + rho_pr[ij] * hx[i]) * u_pr[ij] * hy[j] * 0.5 / ht + Scu;
-  double au0 = au1 + au2 + au3 + au4 + 0.5 * (-Fy[i_1j] - Fy[ij] + Fy[i_1j1] + Fy[ij1]) + F_xu2 - F_xu1 - Spu
-               + (rho[i_1j] * hx[i_1] + rho[ij] * hx[i]) * hy[j] * 0.5 / ht;
-  double ups = (au1 * bd(m[i1j]) * u[i_1j] + au2 * bd(m[i_2j]) * u[i1j] + au3 * u[ij_1] + au4 * u[ij1] + bu3) / au0;
-  double dpc = g.cPch * hy[j] / au0;
-  double h = hy[j] * rho_u[ij];
+  const R cT = g.cT, ht = g.ht;
+  R F_xu1 = R(0.5) * (Fx[i_1j] + Fx[ij]);
+  R F_xu2 = R(0.5) * (Fx[ij] + Fx[i1j]);
+  R Scu = cT * (R(m[i1j]) * u[i1j] * sqrt_T[ij] * hy[j] / (hx[i] * R(3.0))
+                     + R(m[i_2j]) * u[i_1j] * sqrt_T[i_1j] * hy[j] / (hx[i_1] * R(3.0))
+                     + Gamma_yf[ij1] * (v[ij1] * R(m[ij1]) - v[i_1j1] * R(m[i_1j1]))
+                     - Gamma_yf[ij] * (v[ij] * R(m[ij_1]) - v[i_1j] * R(m[i_1j_1]))
+                     - sqrt_T[ij] * (v[ij1] * R(m[ij1]) - v[ij] * R(m[ij_1])) * R(2.0) / R(3.0)
+                     + sqrt_T[i_1j] * (v[i_1j1] * R(m[i_1j1]) - v[i_1j] * R(m[i_1j_1])) * R(2.0) / R(3.0));
+  R Spu = -(D_ux[ij] + D_ux[i1j]) / R(3.0);
+  R au1 = maxi0(F_xu1) + D_ux[ij];
+  R au2 = maxi0(-F_xu2) + D_ux[i1j];
+  R au3 = R(0.5) * (maxi0(Fy[i_1j]) + maxi0(Fy[ij])) + D_uy[ij];
+  R au4 = R(0.5) * (maxi0(-Fy[i_1j1]) + maxi0(-Fy[ij1])) + D_uy[ij1];
+  R bu3 = (rho_pr[i_1j] * hx[i_1] + rho_pr[ij] * hx[i]) * u_pr[ij] * hy[j] * R(0.5) / ht + Scu;
+  R au0 = au1 + au2 + au3 + au4 + R(0.5) * (-Fy[i_1j] - Fy[ij] + Fy[i_1j1] + Fy[ij1]) + F_xu2 - F_xu1 - Spu
+               + (rho[i_1j] * hx[i_1] + rho[ij] * hx[i]) * hy[j] * R(0.5) / ht;
+  R ups = (au1 * R(m[i1j]) * u[i_1j] + au2 * R(m[i_2j]) * u[i1j] + au3 * u[ij_1] + au4 * u[ij1] + bu3) / au0;
+  R dpc = g.cPch * hy[j] / au0;
+  R h = hy[j] * rho_u[ij];
   FLD(U_PSEUDO)[ij] = ups;
   FLD(D_P_C_12)[ij] = dpc;
   FLD(APX)[ij] = dpc * h;
-  FLD(BPX)[ij] = bd(m[i_1j] * m[ij]) * h * ups;
+  FLD(BPX)[ij] = R(m[i_1j] * m[ij]) * h * ups;
 }
 
-__device__ __forceinline__ void cell_assemble_v(const KGrid &g, const KFields &fl, const Nbr &n) {
+template <typename R>
+__device__ __forceinline__ void cell_assemble_v(const KGridT<R> &g, const KFieldsT<R> &fl, const Nbr &n) {
   const uint8_t *m = g.m;
   const unsigned Nx = (unsigned)g.Nx;
-  const double *hx = g.hx, *hy = g.hy;
-  const double *Fx = FLD(FX), *Fy = FLD(FY), *u = FLD(U), *v = FLD(V), *sqrt_T = FLD(SQRT_T);
-  const double *Gamma_xf = FLD(GAMMA_XF), *D_vx = FLD(D_VX), *D_vy = FLD(D_VY);
-  const double *rho_pr = FLD(RHO_PR), *v_pr = FLD(V_PR), *rho = FLD(RHO), *rho_v = FLD(RHO_V);
+  const R *hx = g.hx, *hy = g.hy;
+  const R *Fx = FLD(FX), *Fy = FLD(FY), *u = FLD(U), *v = FLD(V), *sqrt_T = FLD(SQRT_T);
+  const R *Gamma_xf = FLD(GAMMA_XF), *D_vx = FLD(D_VX), *D_vy = FLD(D_VY);
+  const R *rho_pr = FLD(RHO_PR), *v_pr = FLD(V_PR), *rho = FLD(RHO), *rho_v = FLD(RHO_V);
   const unsigned i = n.i, j = n.j, ij = n.ij, i_1j = n.i_1j, i1j = n.i1j, ij_1 = n.ij_1, ij1 = n.ij1;
   const unsigned i_1j_1 = n.i_1j_1, i1j_1 = n.i1j_1;
-  const double cT = g.cT, ht = g.ht;
+  const R cT = g.cT, ht = g.ht;
   // S:3399 ij_1 - Nx*(0 <= (int)(j-2)); S:3400 hy[j - 1*(0 <= (int)(j-1))]
   const unsigned jm2 = ij_1 - Nx * (0 <= (int)(j - 2));
-  const double hyjm = hy[j - 1 * (0 <= (int)(j - 1))];
-  double F_yv3 = 0.5 * (Fy[ij_1] + Fy[ij]);
-  double F_yv4 = 0.5 * (Fy[ij] + Fy[ij1]);
-  double Scv = cT * (bd(m[ij1]) * v[ij1] * sqrt_T[ij] * hx[i] / (hy[j] * 3.0)
-                     + bd(m[jm2]) * v[ij_1] * sqrt_T[ij_1] * hx[i] / (hyjm * 3.0)
-                     + Gamma_xf[i1j] * (u[i1j] * bd(m[i1j]) - u[i1j_1] * bd(m[i1j_1]))
-                     - Gamma_xf[ij] * (u[ij] * bd(m[i_1j]) - u[ij_1] * bd(m[i_1j_1]))
-                     - sqrt_T[ij] * (u[i1j] * bd(m[i1j]) - u[ij] * bd(m[i_1j])) * 2.0 / 3.0
-                     + sqrt_T[ij_1] * (u[i1j_1] * bd(m[i1j_1]) - u[ij_1] * bd(m[i_1j_1])) * 2.0 / 3.0)
-               - g.c_Fr_v * (rho[ij_1] * hx[i] * 0.5 * hyjm + rho[ij] * hx[i] * 0.5 * hy[j]);
-  double Spv = -(D_vy[ij] + D_vy[ij1]) / 3.0;
-  double av1 = 0.5 * (maxi0(Fx[ij]) + maxi0(Fx[ij_1])) + D_vx[ij];
-  double av2 = 0.5 * (maxi0(-Fx[i1j]) + maxi0(-Fx[i1j_1])) + D_vx[i1j];
-  double av3 = maxi0(F_yv3) + D_vy[ij];
-  double av4 = maxi0(-F_yv4) + D_vy[ij1];
-  double bv3 = (rho_pr[ij_1] * hy[j - 1] + rho_pr[ij] * hy[j]) * v_pr[ij] * hx[i] * 0.5 / ht + Scv;
-  double av0 = av1 + av2 + av3 + av4 + 0.5 * (Fx[i1j] - Fx[ij] + Fx[i1j_1] - Fx[ij_1]) + F_yv4 - F_yv3 - Spv
-               + (rho[ij_1] * hy[j - 1] + rho[ij] * hy[j]) * hx[i] * 0.5 / ht;
-  double vps = (av1 * v[i_1j] + av2 * v[i1j] + av3 * bd(m[jm2]) * v[ij_1] + av4 * bd(m[ij1]) * v[ij1] + bv3) / av0;
-  double dpc = g.cPch * hx[i] / av0;
-  double h = hx[i] * rho_v[ij];
+  const R hyjm = hy[j - 1 * (0 <= (int)(j - 1))];
+  R F_yv3 = R(0.5) * (Fy[ij_1] + Fy[ij]);
+  R F_yv4 = R(0.5) * (Fy[ij] + Fy[ij1]);
+  R Scv = cT * (R(m[ij1]) * v[ij1] * sqrt_T[ij] * hx[i] / (hy[j] * R(3.0))
+                     + R(m[jm2]) * v[ij_1] * sqrt_T[ij_1] * hx[i] / (hyjm * R(3.0))
+                     + Gamma_xf[i1j] * (u[i1j] * R(m[i1j]) - u[i1j_1] * R(m[i1j_1]))
+                     - Gamma_xf[ij] * (u[ij] * R(m[i_1j]) - u[ij_1] * R(m[i_1j_1]))
+                     - sqrt_T[ij] * (u[i1j] * R(m[i1j]) - u[ij] * R(m[i_1j])) * R(2.0) / R(3.0)
+                     + sqrt_T[ij_1] * (u[i1j_1] * R(m[i1j_1]) - u[ij_1] * R(m[i_1j_1])) * R(2.0) / R(3.0))
+               - g.c_Fr_v * (rho[ij_1] * hx[i] * R(0.5) * hyjm + rho[ij] * hx[i] * R(0.5) * hy[j]);
+  R Spv = -(D_vy[ij] + D_vy[ij1]) / R(3.0);
+  R av1 = R(0.5) * (maxi0(Fx[ij]) + maxi0(Fx[ij_1])) + D_vx[ij];
+  R av2 = R(0.5) * (maxi0(-Fx[i1j]) + maxi0(-Fx[i1j_1])) + D_vx[i1j];
+  R av3 = maxi0(F_yv3) + D_vy[ij];
+  R av4 = maxi0(-F_yv4) + D_vy[ij1];
+  R bv3 = (rho_pr[ij_1] * hy[j - 1] + rho_pr[ij] * hy[j]) * v_pr[ij] * hx[i] * R(0.5) / ht + Scv;
+  R av0 = av1 + av2 + av3 + av4 + R(0.5) * (Fx[i1j] - Fx[ij] + Fx[i1j_1] - Fx[ij_1]) + F_yv4 - F_yv3 - Spv
+               + (rho[ij_1] * hy[j - 1] + rho[ij] * hy[j]) * hx[i] * R(0.5) / ht;
+  R vps = (av1 * v[i_1j] + av2 * v[i1j] + av3 * R(m[jm2]) * v[ij_1] + av4 * R(m[ij1]) * v[ij1] + bv3) / av0;
+  R dpc = g.cPch * hx[i] / av0;
+  R h = hx[i] * rho_v[ij];
   FLD(V_PSEUDO)[ij] = vps;
   FLD(D_P_C_34)[ij] = dpc;
   FLD(APY)[ij] = dpc * h;
-  FLD(BPY)[ij] = bd(m[ij - Nx] * m[ij]) * h * vps;
+  FLD(BPY)[ij] = R(m[ij - Nx] * m[ij]) * h * vps;
 }
 
-__device__ __forceinline__ void cell_energy_source(const KGrid &g, const KFields &fl, const Nbr &n) {
+template <typename R>
+__device__ __forceinline__ void cell_energy_source(const KGridT<R> &g, const KFieldsT<R> &fl, const Nbr &n) {
   const uint8_t *m = g.m;
   const unsigned Nx = (unsigned)g.Nx;
-  const double *hx = g.hx, *hy = g.hy;
-  const double *u = FLD(U), *v = FLD(V), *sqrt_T = FLD(SQRT_T), *p_pr = FLD(P_PR);
+  const R *hx = g.hx, *hy = g.hy;
+  const R *u = FLD(U), *v = FLD(V), *sqrt_T = FLD(SQRT_T), *p_pr = FLD(P_PR);
   const unsigned i = n.i, j = n.j, ij = n.ij, i_1j = n.i_1j, i1j = n.i1j, ij_1 = n.ij_1, ij1 = n.ij1;
   const unsigned i_1j_1 = n.i_1j_1, i_1j1 = n.i_1j1, i1j_1 = n.i1j_1, i1j1 = n.i1j1, i_1 = n.i_1, i1 = n.i1;
   // S:3729-3740: the upper neighbour row is guarded by j < Nx-1
   const unsigned jm = j - 1 * (0 < j), jp = j + 1 * (j < (Nx - 1));
-  double dudx_T = (bd(m[i1j]) * u[i1j] - bd(m[i_1j]) * u[ij]) / hx[i];
-  double dvdy_T = (bd(m[ij1]) * v[ij1] - bd(m[ij_1]) * v[ij]) / hy[j];
-  double s0 = ((u[ij] - u[ij_1]) / (bd(m[i_1j_1] && m[ij_1]) * hy[jm] + hy[j]))
-              + ((v[ij] - v[i_1j]) / (bd(m[i_1j_1] && m[i_1j]) * hx[i_1] + hx[i]));
-  double s1 = ((u[i1j] - u[i1j_1]) / (bd(m[ij_1] && m[i1j_1]) * hy[jm] + hy[j]))
-              + ((v[i1j] - v[ij]) / (hx[i] + bd(m[i1j_1] && m[i1j]) * hx[i1]));
-  double s2 = ((u[ij1] - u[ij]) / (hy[j] + bd(m[i_1j1] && m[ij1]) * hy[jp]))
-              + ((v[ij1] - v[i_1j1]) / (bd(m[i_1j] && m[i_1j1]) * hx[i_1] + hx[i]));
-  double s3 = ((u[i1j1] - u[i1j]) / (hy[j] + bd(m[ij1] && m[i1j1]) * hy[jp]))
-              + ((v[i1j1] - v[ij1]) / (hx[i] + bd(m[i1j] && m[i1j1]) * hx[i1]));
-  double dsum = dudx_T + dvdy_T;
+  R dudx_T = (R(m[i1j]) * u[i1j] - R(m[i_1j]) * u[ij]) / hx[i];
+  R dvdy_T = (R(m[ij1]) * v[ij1] - R(m[ij_1]) * v[ij]) / hy[j];
+  R s0 = ((u[ij] - u[ij_1]) / (R(m[i_1j_1] && m[ij_1]) * hy[jm] + hy[j]))
+              + ((v[ij] - v[i_1j]) / (R(m[i_1j_1] && m[i_1j]) * hx[i_1] + hx[i]));
+  R s1 = ((u[i1j] - u[i1j_1]) / (R(m[ij_1] && m[i1j_1]) * hy[jm] + hy[j]))
+              + ((v[i1j] - v[ij]) / (hx[i] + R(m[i1j_1] && m[i1j]) * hx[i1]));
+  R s2 = ((u[ij1] - u[ij]) / (hy[j] + R(m[i_1j1] && m[ij1]) * hy[jp]))
+              + ((v[ij1] - v[i_1j1]) / (R(m[i_1j] && m[i_1j1]) * hx[i_1] + hx[i]));
+  R s3 = ((u[i1j1] - u[i1j]) / (hy[j] + R(m[ij1] && m[i1j1]) * hy[jp]))
+              + ((v[i1j1] - v[ij1]) / (hx[i] + R(m[i1j] && m[i1j1]) * hx[i1]));
+  R dsum = dudx_T + dvdy_T;
   FLD(SCT1)[ij] = g.ht * sqrt_T[ij] * g.cT2 * hx[i] * hy[j]
-                      * (2.0 * (dudx_T * dudx_T + dvdy_T * dvdy_T) + s0 * s0 + s1 * s1 + s2 * s2 + s3 * s3
-                         - dsum * dsum * 2.0 / 3.0)
+                      * (R(2.0) * (dudx_T * dudx_T + dvdy_T * dvdy_T) + s0 * s0 + s1 * s1 + s2 * s2 + s3 * s3
+                         - dsum * dsum * R(2.0) / R(3.0))
                   + p_pr[ij] * hx[i] * hy[j];
 }
 
-__global__ void __launch_bounds__(BX *BY) k_assemble(KGrid g, KFields fl, Gate gate) {
+template <typename R>
+__global__ void __launch_bounds__(BX *BY) k_assemble(KGridT<R> g, KFieldsT<R> fl, Gate gate) {
   if (gated_out(gate)) return;
   const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
   if (i >= g.Nx || j >= g.Ny) return;
@@ -185,35 +191,37 @@ __global__ void __launch_bounds__(BX *BY) k_assemble(KGrid g, KFields fl, Gate g
 // A5: fused T/p Jacobi sweep (S:3805-4027).
 // ---------------------------------------------------------------------------
 // Old p and T of a cell and of its four (clamped) neighbours.
+template <typename R>
 struct PTStencil {
-  double Pc, Tc, Pw, Tw, Pe, Te, Ps, Ts, Pn, Tn;
+  R Pc, Tc, Pw, Tw, Pe, Te, Ps, Ts, Pn, Tn;
 };
 
 // The reference's T update (S:3837-3857) then p update (S:3884-3895) of one
 // cell of the update set, with the new T in the p denominator (S:3894).  A
 // body neighbour contributes the fluid temperature on its wall (S:3838-3848:
 // m*T_old + !m*T_wall with T_wall finite, i.e. a select).
-__device__ __forceinline__ void pt_update(const KGrid &g, const KFields &fl, const Nbr &n, const PTStencil &x,
-                                          double &pn, double &tn) {
+template <typename R>
+__device__ __forceinline__ void pt_update(const KGridT<R> &g, const KFieldsT<R> &fl, const Nbr &n, const PTStencil<R> &x,
+                                          R &pn, R &tn) {
   const unsigned ij = n.ij, i1j = n.i1j, ij1 = n.ij1;
   const uint8_t *m = g.m;
-  const double *Fx = FLD(FX), *Fy = FLD(FY), *DTx = FLD(DTX), *DTy = FLD(DTY), *u = FLD(U), *v = FLD(V);
-  const double *apx = FLD(APX), *apy = FLD(APY), *bpx = FLD(BPX), *bpy = FLD(BPY);
-  const double ht = g.ht;
-  const double hxi = g.hx[n.i], hyj = g.hy[n.j];
-  double aT1 = maxi0(Fx[ij]) + DTx[ij];
-  double aT2 = maxi0(-Fx[i1j]) + DTx[i1j];
-  double aT3 = maxi0(Fy[ij]) + DTy[ij];
-  double aT4 = maxi0(-Fy[ij1]) + DTy[ij1];
-  double X1 = m[n.i_1j] ? x.Tw : FLD(TW_U)[ij];
-  double X2 = m[i1j] ? x.Te : FLD(TW_U)[i1j];
-  double X3 = m[n.ij_1] ? x.Ts : FLD(TW_V)[ij];
-  double X4 = m[ij1] ? x.Tn : FLD(TW_V)[ij1];
+  const R *Fx = FLD(FX), *Fy = FLD(FY), *DTx = FLD(DTX), *DTy = FLD(DTY), *u = FLD(U), *v = FLD(V);
+  const R *apx = FLD(APX), *apy = FLD(APY), *bpx = FLD(BPX), *bpy = FLD(BPY);
+  const R ht = g.ht;
+  const R hxi = g.hx[n.i], hyj = g.hy[n.j];
+  R aT1 = maxi0(Fx[ij]) + DTx[ij];
+  R aT2 = maxi0(-Fx[i1j]) + DTx[i1j];
+  R aT3 = maxi0(Fy[ij]) + DTy[ij];
+  R aT4 = maxi0(-Fy[ij1]) + DTy[ij1];
+  R X1 = m[n.i_1j] ? x.Tw : FLD(TW_U)[ij];
+  R X2 = m[i1j] ? x.Te : FLD(TW_U)[i1j];
+  R X3 = m[n.ij_1] ? x.Ts : FLD(TW_V)[ij];
+  R X4 = m[ij1] ? x.Tn : FLD(TW_V)[ij1];
   tn = (ht * (aT1 * X1 + aT2 * X2 + aT3 * X3 + aT4 * X4
-              + (1.0 - g.gamma1) * x.Pc * ((u[i1j] - u[ij]) * hyj + (v[ij1] - v[ij]) * hxi))
+              + (R(1.0) - g.gamma1) * x.Pc * ((u[i1j] - u[ij]) * hyj + (v[ij1] - v[ij]) * hxi))
         + FLD(SCT1)[ij])
        / (ht * (aT1 + aT2 + aT3 + aT4 + Fx[i1j] - Fx[ij] + Fy[ij1] - Fy[ij]) + hxi * hyj * FLD(RHO)[ij]);
-  double s = apx[ij] * x.Pw + apx[i1j] * x.Pe + apy[ij] * x.Ps + apy[ij1] * x.Pn + bpx[ij] - bpx[i1j] + bpy[ij]
+  R s = apx[ij] * x.Pw + apx[i1j] * x.Pe + apy[ij] * x.Ps + apy[ij1] * x.Pn + bpx[ij] - bpx[i1j] + bpy[ij]
              - bpy[ij1];
   // S:3887-3888: inflow/outflow flux terms of the given-velocity columns
   if (g.given_xb && (int)n.i == g.Nx_o0) s = s + Fx[ij];
@@ -222,21 +230,24 @@ __device__ __forceinline__ void pt_update(const KGrid &g, const KFields &fl, con
        (ht * (apx[ij] + apx[i1j] + apy[ij] + apy[ij1]) + hxi * hyj / (tn));
 }
 
-__device__ __forceinline__ bool in_pt_set(const KGrid &g, int i, int j, unsigned ij) {
+template <typename R>
+__device__ __forceinline__ bool in_pt_set(const KGridT<R> &g, int i, int j, unsigned ij) {
   return in_box(i, j, g.Nx_o0, g.Nx_o1, g.Ny_o0, g.Ny_o1) && g.m[ij];
 }
 
-__device__ __forceinline__ PTStencil stencil_global(const double *Po, const double *To, const Nbr &n) {
-  return PTStencil{Po[n.ij], To[n.ij], Po[n.i_1j], To[n.i_1j], Po[n.i1j], To[n.i1j],
+template <typename R>
+__device__ __forceinline__ PTStencil<R> stencil_global(const R *Po, const R *To, const Nbr &n) {
+  return PTStencil<R>{Po[n.ij], To[n.ij], Po[n.i_1j], To[n.i_1j], Po[n.i1j], To[n.i1j],
                    Po[n.ij_1], To[n.ij_1], Po[n.ij1], To[n.ij1]};
 }
 
 // One sweep: reads (Po, To), writes (Pn, Tn) on every cell: the update set
 // gets the new values, every other cell is copied through, which restates
 // the loop-3 entry copy S:3763-3764 and the pointer swap S:3792-3795.
-__global__ void __launch_bounds__(BX *BY) k_sweep(KGrid g, KFields fl, const double *__restrict__ Po,
-                                                  const double *__restrict__ To, double *__restrict__ Pn,
-                                                  double *__restrict__ Tn, unsigned long long *red_p,
+template <typename R>
+__global__ void __launch_bounds__(BX *BY) k_sweep(KGridT<R> g, KFieldsT<R> fl, const R *__restrict__ Po,
+                                                  const R *__restrict__ To, R *__restrict__ Pn,
+                                                  R *__restrict__ Tn, unsigned long long *red_p,
                                                   unsigned long long *red_T, unsigned long long *red_p2,
                                                   unsigned long long *red_T2, Gate gate) {
   if (gated_out(gate)) return;
@@ -246,8 +257,8 @@ __global__ void __launch_bounds__(BX *BY) k_sweep(KGrid g, KFields fl, const dou
     const unsigned ij = (unsigned)i + (unsigned)j * (unsigned)g.Nx;
     if (in_pt_set(g, i, j, ij)) {
       const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny);
-      const PTStencil x = stencil_global(Po, To, n);
-      double pn, tn;
+      const PTStencil<R> x = stencil_global(Po, To, n);
+      R pn, tn;
       pt_update(g, fl, n, x, pn, tn);
       Tn[ij] = tn;
       Pn[ij] = pn;
@@ -268,13 +279,14 @@ __global__ void __launch_bounds__(BX *BY) k_sweep(KGrid g, KFields fl, const dou
 // the tile cells go to (r1p, r1T) and (r2p, r2T).
 constexpr int S2X = 64, S2Y = 8, S2RX = S2X + 2, S2RY = S2Y + 2;
 
-__global__ void __launch_bounds__(S2X *S2Y) k_sweep2(KGrid g, KFields fl, const double *__restrict__ Po,
-                                                     const double *__restrict__ To, double *__restrict__ Pn,
-                                                     double *__restrict__ Tn, unsigned long long *r1p,
+template <typename R>
+__global__ void __launch_bounds__(S2X *S2Y) k_sweep2(KGridT<R> g, KFieldsT<R> fl, const R *__restrict__ Po,
+                                                     const R *__restrict__ To, R *__restrict__ Pn,
+                                                     R *__restrict__ Tn, unsigned long long *r1p,
                                                      unsigned long long *r1T, unsigned long long *r2p,
                                                      unsigned long long *r2T, Gate gate) {
   if (gated_out(gate)) return;
-  __shared__ double sP[S2RY][S2RX], sT[S2RY][S2RX];
+  __shared__ R sP[S2RY][S2RX], sT[S2RY][S2RX];
   const int Nx = g.Nx, Ny = g.Ny;
   const int i0 = blockIdx.x * S2X, j0 = blockIdx.y * S2Y;
   const int tid = threadIdx.x + threadIdx.y * S2X;
@@ -284,10 +296,10 @@ __global__ void __launch_bounds__(S2X *S2Y) k_sweep2(KGrid g, KFields fl, const 
     const int i = i0 - 1 + rx, j = j0 - 1 + ry;
     if (i < 0 || j < 0 || i >= Nx || j >= Ny) continue;
     const unsigned ij = (unsigned)i + (unsigned)j * (unsigned)Nx;
-    double pn, tn;
+    R pn, tn;
     if (in_pt_set(g, i, j, ij)) {
       const Nbr n((unsigned)i, (unsigned)j, (unsigned)Nx, (unsigned)Ny);
-      const PTStencil x = stencil_global(Po, To, n);
+      const PTStencil<R> x = stencil_global(Po, To, n);
       pt_update(g, fl, n, x, pn, tn);
       if (rx >= 1 && rx <= S2X && ry >= 1 && ry <= S2Y) {
         r1[0] = (absbits(pn - x.Pc) > r1[0]) ? absbits(pn - x.Pc) : r1[0];
@@ -311,9 +323,9 @@ __global__ void __launch_bounds__(S2X *S2Y) k_sweep2(KGrid g, KFields fl, const 
       // clamped neighbours map to the cell itself at the domain edge
       const int wx = cx - (int)(n.i - n.i_1), ex = cx + (int)(n.i1 - n.i);
       const int sy = cy - (int)((n.ij - n.ij_1) / (unsigned)Nx), ny = cy + (int)((n.ij1 - n.ij) / (unsigned)Nx);
-      const PTStencil x{sP[cy][cx], sT[cy][cx], sP[cy][wx], sT[cy][wx], sP[cy][ex], sT[cy][ex],
+      const PTStencil<R> x{sP[cy][cx], sT[cy][cx], sP[cy][wx], sT[cy][wx], sP[cy][ex], sT[cy][ex],
                         sP[sy][cx], sT[sy][cx], sP[ny][cx], sT[ny][cx]};
-      double pn, tn;
+      R pn, tn;
       pt_update(g, fl, n, x, pn, tn);
       Tn[ij] = tn;
       Pn[ij] = pn;
@@ -334,11 +346,13 @@ __global__ void __launch_bounds__(S2X *S2Y) k_sweep2(KGrid g, KFields fl, const 
 // ij needs rho of the west/south cell; it is recomputed here from p and T
 // with the same operations, so no second pass is needed.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double rho_of(const uint8_t *m, const double *p, const double *T, unsigned c) {
-  return m[c] ? p[c] / T[c] : 0.0;
+template <typename R>
+__device__ __forceinline__ R rho_of(const uint8_t *m, const R *p, const R *T, unsigned c) {
+  return m[c] ? p[c] / T[c] : R(0.0);
 }
 
-__global__ void __launch_bounds__(BX *BY) k_correct(KGrid g, KFields fl, unsigned long long *red_u,
+template <typename R>
+__global__ void __launch_bounds__(BX *BY) k_correct(KGridT<R> g, KFieldsT<R> fl, unsigned long long *red_u,
                                                     unsigned long long *red_v, int do_correct, Gate gate) {
   if (gated_out(gate)) return;
   const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
@@ -347,32 +361,32 @@ __global__ void __launch_bounds__(BX *BY) k_correct(KGrid g, KFields fl, unsigne
     const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny);
     const uint8_t *m = g.m;
     const unsigned ij = n.ij, i_1j = n.i_1j, ij_1 = n.ij_1;
-    const double *p = FLD(P), *T = FLD(T);
-    double uf = FLD(U)[ij], vf = FLD(V)[ij];
+    const R *p = FLD(P), *T = FLD(T);
+    R uf = FLD(U)[ij], vf = FLD(V)[ij];
     if (do_correct) {
       if (in_box(i, j, g.Nx_u0, g.Nx_u1, g.Ny_u0, g.Ny_u1) && m[i_1j] && m[ij]) {
-        double un = FLD(U_PSEUDO)[ij] - (p[ij] - p[i_1j]) * FLD(D_P_C_12)[ij];
+        R un = FLD(U_PSEUDO)[ij] - (p[ij] - p[i_1j]) * FLD(D_P_C_12)[ij];
         r[0] = absbits(un - uf);
         uf = un;
         FLD(U)[ij] = un;
       }
       if (in_box(i, j, g.Nx_v0, g.Nx_v1, g.Ny_v0, g.Ny_v1) && m[ij_1] && m[ij]) {
-        double vn = FLD(V_PSEUDO)[ij] - (p[ij] - p[ij - (unsigned)g.Nx]) * FLD(D_P_C_34)[ij];
+        R vn = FLD(V_PSEUDO)[ij] - (p[ij] - p[ij - (unsigned)g.Nx]) * FLD(D_P_C_34)[ij];
         r[1] = absbits(vn - vf);
         vf = vn;
         FLD(V)[ij] = vn;
       }
     }
-    const double rij = rho_of(m, p, T, ij);
+    const R rij = rho_of(m, p, T, ij);
     FLD(SQRT_T)[ij] = sqrt(T[ij]);
     FLD(RHO)[ij] = rij;
-    double ru, rv;
+    R ru, rv;
     if (m[i_1j] && m[ij]) ru = upwind_rho(rho_of(m, p, T, i_1j), rij, uf);
     else if (m[i_1j] != m[ij]) ru = m[i_1j] ? rho_of(m, p, T, i_1j) : rij;
-    else ru = 0.0;
+    else ru = R(0.0);
     if (m[ij_1] && m[ij]) rv = upwind_rho(rho_of(m, p, T, ij_1), rij, vf);
     else if (m[ij_1] != m[ij]) rv = m[ij_1] ? rho_of(m, p, T, ij_1) : rij;
-    else rv = 0.0;
+    else rv = R(0.0);
     FLD(RHO_U)[ij] = ru;
     FLD(RHO_V)[ij] = rv;
   }
@@ -380,30 +394,32 @@ __global__ void __launch_bounds__(BX *BY) k_correct(KGrid g, KFields fl, unsigne
 }
 
 // A7 alone (S:4323-4330)
-__global__ void __launch_bounds__(BX *BY) k_properties(KGrid g, KFields fl) {
+template <typename R>
+__global__ void __launch_bounds__(BX *BY) k_properties(KGridT<R> g, KFieldsT<R> fl) {
   const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
   if (i >= g.Nx || j >= g.Ny) return;
   const unsigned c = (unsigned)i + (unsigned)j * (unsigned)g.Nx;
-  const double T = FLD(T)[c];
+  const R T = FLD(T)[c];
   FLD(SQRT_T)[c] = sqrt(T);
-  FLD(RHO)[c] = g.m[c] ? FLD(P)[c] / T : 0.0;
+  FLD(RHO)[c] = g.m[c] ? FLD(P)[c] / T : R(0.0);
 }
 
 // A9 alone (S:10382-10437), from the stored rho
-__global__ void __launch_bounds__(BX *BY) k_rho_mid(KGrid g, KFields fl) {
+template <typename R>
+__global__ void __launch_bounds__(BX *BY) k_rho_mid(KGridT<R> g, KFieldsT<R> fl) {
   const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
   if (i >= g.Nx || j >= g.Ny) return;
   const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny);
   const uint8_t *m = g.m;
-  const double *rho = FLD(RHO);
+  const R *rho = FLD(RHO);
   const unsigned ij = n.ij, i_1j = n.i_1j, ij_1 = n.ij_1;
-  double ru, rv;
+  R ru, rv;
   if (m[i_1j] && m[ij]) ru = upwind_rho(rho[i_1j], rho[ij], FLD(U)[ij]);
   else if (m[i_1j] != m[ij]) ru = m[i_1j] ? rho[i_1j] : rho[ij];
-  else ru = 0.0;
+  else ru = R(0.0);
   if (m[ij_1] && m[ij]) rv = upwind_rho(rho[ij_1], rho[ij], FLD(V)[ij]);
   else if (m[ij_1] != m[ij]) rv = m[ij_1] ? rho[ij_1] : rho[ij];
-  else rv = 0.0;
+  else rv = R(0.0);
   FLD(RHO_U)[ij] = ru;
   FLD(RHO_V)[ij] = rv;
 }
@@ -415,18 +431,21 @@ __global__ void __launch_bounds__(BX *BY) k_rho_mid(KGrid g, KFields fl) {
 // last writer of that walk is evaluated here.  Every value read is a face or
 // cell that no writer of this pass modifies, so faces are independent.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ const DevBody *body_of(const KGrid &g, uint32_t k) {
+template <typename R>
+__device__ __forceinline__ const DevBody *body_of(const KGridT<R> &g, uint32_t k) {
   return (k >= 1u && (int)k <= g.nbodies) ? &g.bodies[k - 1] : nullptr;
 }
 
 // Slip cell for u: body cell with fluid below or above, slip on (S:10169-10176).
-__device__ __forceinline__ const DevBody *slip_cell_u(const KGrid &g, const Nbr &c) {
+template <typename R>
+__device__ __forceinline__ const DevBody *slip_cell_u(const KGridT<R> &g, const Nbr &c) {
   const uint32_t *vol = g.vol;
   if (!((0u < vol[c.ij]) && ((0u == vol[c.ij_1]) || (0u == vol[c.ij1])))) return nullptr;
   const DevBody *b = body_of(g, vol[c.ij]);
   return (b && b->is_slip) ? b : nullptr;
 }
-__device__ __forceinline__ const DevBody *slip_cell_v(const KGrid &g, const Nbr &c) {
+template <typename R>
+__device__ __forceinline__ const DevBody *slip_cell_v(const KGridT<R> &g, const Nbr &c) {
   const uint32_t *vol = g.vol;
   if (!((0u < vol[c.ij]) && ((0u == vol[c.i_1j]) || (0u == vol[c.i1j])))) return nullptr;
   const DevBody *b = body_of(g, vol[c.ij]);
@@ -435,85 +454,89 @@ __device__ __forceinline__ const DevBody *slip_cell_v(const KGrid &g, const Nbr 
 
 // The two writes of slip cell c for u (S:7155-7206): which==0 -> u[c.ij],
 // which==1 -> u[c.i1j].  Returns true and the value if the write happens.
-__device__ __forceinline__ bool slip_u_write(const KGrid &g, const KFields &fl, const Nbr &c, const DevBody *b,
-                                             int which, double &val) {
+template <typename R>
+__device__ __forceinline__ bool slip_u_write(const KGridT<R> &g, const KFieldsT<R> &fl, const Nbr &c, const DevBody *b,
+                                             int which, R &val) {
   const uint8_t *m = g.m;
-  const double *u = FLD(U), *rho_u = FLD(RHO_U);
-  const double FK = b->F_slip * g.Kn, ub = b->u_body;
+  const R *u = FLD(U), *rho_u = FLD(RHO_U);
+  const R FK = R(b->F_slip) * g.Kn, ub = R(b->u_body);
   if (0u == g.vol[c.ij_1]) {
-    const double hh = g.hy[c.j - 1 * (0 < c.j)];
+    const R hh = g.hy[c.j - 1 * (0 < c.j)];
     if (which == 0) {
       if (!m[c.i_1j_1]) return false;
-      double a = g.kn_local ? FK / (rho_u[c.ij_1] * 0.5 * hh) : FK / (0.5 * hh);
-      val = (u[c.ij_1] * a + ub) / (1.0 + a);
+      R a = g.kn_local ? FK / (rho_u[c.ij_1] * R(0.5) * hh) : FK / (R(0.5) * hh);
+      val = (u[c.ij_1] * a + ub) / (R(1.0) + a);
     } else {
       if (!m[c.i1j_1]) return false;
-      double a = g.kn_local ? FK / (rho_u[c.i1j_1] * 0.5 * hh) : FK / (0.5 * hh);
-      val = (u[c.i1j_1] * a + ub) / (1.0 + a);
+      R a = g.kn_local ? FK / (rho_u[c.i1j_1] * R(0.5) * hh) : FK / (R(0.5) * hh);
+      val = (u[c.i1j_1] * a + ub) / (R(1.0) + a);
     }
   } else {
-    const double hh = g.hy[c.j + 1 * (c.j < (unsigned)(g.Ny - 1))];
+    const R hh = g.hy[c.j + 1 * (c.j < (unsigned)(g.Ny - 1))];
     if (which == 0) {
       if (!m[c.i_1j1]) return false;
-      double a = g.kn_local ? FK / (rho_u[c.ij1] * 0.5 * hh) : FK / (0.5 * hh);
-      val = (u[c.ij1] * a + ub) / (1.0 + a);
+      R a = g.kn_local ? FK / (rho_u[c.ij1] * R(0.5) * hh) : FK / (R(0.5) * hh);
+      val = (u[c.ij1] * a + ub) / (R(1.0) + a);
     } else {
       if (!m[c.i1j1]) return false;
-      double a = g.kn_local ? FK / (rho_u[c.i1j1] * 0.5 * hh) : FK / (0.5 * hh);
-      val = (u[c.i1j1] * a + ub) / (1.0 + a);
+      R a = g.kn_local ? FK / (rho_u[c.i1j1] * R(0.5) * hh) : FK / (R(0.5) * hh);
+      val = (u[c.i1j1] * a + ub) / (R(1.0) + a);
     }
   }
   return true;
 }
 
 // S:7273-7324: which==0 -> v[c.ij], which==1 -> v[c.ij1]
-__device__ __forceinline__ bool slip_v_write(const KGrid &g, const KFields &fl, const Nbr &c, const DevBody *b,
-                                             int which, double &val) {
+template <typename R>
+__device__ __forceinline__ bool slip_v_write(const KGridT<R> &g, const KFieldsT<R> &fl, const Nbr &c, const DevBody *b,
+                                             int which, R &val) {
   const uint8_t *m = g.m;
-  const double *v = FLD(V), *rho_v = FLD(RHO_V);
-  const double FK = b->F_slip * g.Kn, vb = b->v_body;
+  const R *v = FLD(V), *rho_v = FLD(RHO_V);
+  const R FK = R(b->F_slip) * g.Kn, vb = R(b->v_body);
   if (0u == g.vol[c.i_1j]) {
-    const double hh = g.hx[c.i_1];
+    const R hh = g.hx[c.i_1];
     if (which == 0) {
       if (!m[c.i_1j_1]) return false;
-      double a = g.kn_local ? FK / (rho_v[c.i_1j] * 0.5 * hh) : FK / (0.5 * hh);
-      val = (v[c.i_1j] * a + vb) / (1.0 + a);
+      R a = g.kn_local ? FK / (rho_v[c.i_1j] * R(0.5) * hh) : FK / (R(0.5) * hh);
+      val = (v[c.i_1j] * a + vb) / (R(1.0) + a);
     } else {
       if (!m[c.i_1j1]) return false;
-      double a = g.kn_local ? FK / (rho_v[c.i_1j1] * 0.5 * hh) : FK / (0.5 * hh);
-      val = (v[c.i_1j1] * a + vb) / (1.0 + a);
+      R a = g.kn_local ? FK / (rho_v[c.i_1j1] * R(0.5) * hh) : FK / (R(0.5) * hh);
+      val = (v[c.i_1j1] * a + vb) / (R(1.0) + a);
     }
   } else {
-    const double hh = g.hx[c.i1];
+    const R hh = g.hx[c.i1];
     if (which == 0) {
       if (!m[c.i1j_1]) return false;
-      double a = g.kn_local ? FK / (rho_v[c.i1j] * 0.5 * hh) : FK / (0.5 * hh);
-      val = (v[c.i1j] * a + vb) / (1.0 + a);
+      R a = g.kn_local ? FK / (rho_v[c.i1j] * R(0.5) * hh) : FK / (R(0.5) * hh);
+      val = (v[c.i1j] * a + vb) / (R(1.0) + a);
     } else {
       if (!m[c.i1j1]) return false;
-      double a = g.kn_local ? FK / (rho_v[c.i1j1] * 0.5 * hh) : FK / (0.5 * hh);
-      val = (v[c.i1j1] * a + vb) / (1.0 + a);
+      R a = g.kn_local ? FK / (rho_v[c.i1j1] * R(0.5) * hh) : FK / (R(0.5) * hh);
+      val = (v[c.i1j1] * a + vb) / (R(1.0) + a);
     }
   }
   return true;
 }
 
 // Wall temperature seen from a fluid cell through a face with body k (S:7762-7797).
-__device__ __forceinline__ double wall_T(const KGrid &g, uint32_t k, double T_fluid, double rho_fluid,
-                                         double half_h) {
+template <typename R>
+__device__ __forceinline__ R wall_T(const KGridT<R> &g, uint32_t k, R T_fluid, R rho_fluid,
+                                         R half_h) {
   const DevBody *b = body_of(g, k);
-  if (!b) return 0.0;
+  if (!b) return R(0.0);
   if (b->is_jump) {
-    double FK = g.kn_local ? (b->F_jump * g.Kn) / rho_fluid : b->F_jump * g.Kn;
-    double a = FK / half_h;
-    return (a * T_fluid + b->T_body) / (1.0 + a);
+    R FK = g.kn_local ? (R(b->F_jump) * g.Kn) / rho_fluid : R(b->F_jump) * g.Kn;
+    R a = FK / half_h;
+    return (a * T_fluid + R(b->T_body)) / (R(1.0) + a);
   } else if (b->dTdn) {
     return T_fluid;
   }
-  return b->T_body;
+  return R(b->T_body);
 }
 
-__global__ void __launch_bounds__(BX *BY) k_wall(KGrid g, KFields fl, int parts, Gate gate) {
+template <typename R>
+__global__ void __launch_bounds__(BX *BY) k_wall(KGridT<R> g, KFieldsT<R> fl, int parts, Gate gate) {
   if (gated_out(gate)) return;
   const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
   if (i >= g.Nx || j >= g.Ny) return;
@@ -521,7 +544,7 @@ __global__ void __launch_bounds__(BX *BY) k_wall(KGrid g, KFields fl, int parts,
   const unsigned k = (unsigned)i + (unsigned)j * Nx;
   const uint8_t *m = g.m;
   if (parts & STS_BC_SLIP_U) {
-    double val = 0.0;
+    R val = R(0.0);
     bool w = false;
     if (i >= 1) {  // cell (i-1, j) writes u[i1j]
       const Nbr c((unsigned)i - 1u, (unsigned)j, Nx, Ny);
@@ -537,7 +560,7 @@ __global__ void __launch_bounds__(BX *BY) k_wall(KGrid g, KFields fl, int parts,
     if (w) FLD(U)[k] = val;
   }
   if (parts & STS_BC_SLIP_V) {
-    double val = 0.0;
+    R val = R(0.0);
     bool w = false;
     if (j >= 1) {  // cell (i, j-1) writes v[ij1]
       const Nbr c((unsigned)i, (unsigned)j - 1u, Nx, Ny);
@@ -553,16 +576,16 @@ __global__ void __launch_bounds__(BX *BY) k_wall(KGrid g, KFields fl, int parts,
     if (w) FLD(V)[k] = val;
   }
   if (parts & STS_BC_JUMP) {
-    const double *T = FLD(T), *rho = FLD(RHO);
+    const R *T = FLD(T), *rho = FLD(RHO);
     if (i >= 1) {
       const unsigned w_ = k - 1u;
-      if (m[k] && !m[w_]) FLD(TW_U)[k] = wall_T(g, g.vol[w_], T[k], rho[k], 0.5 * g.hx[i]);
-      else if (m[w_] && !m[k]) FLD(TW_U)[k] = wall_T(g, g.vol[k], T[w_], rho[w_], 0.5 * g.hx[i - 1]);
+      if (m[k] && !m[w_]) FLD(TW_U)[k] = wall_T(g, g.vol[w_], T[k], rho[k], R(0.5) * g.hx[i]);
+      else if (m[w_] && !m[k]) FLD(TW_U)[k] = wall_T(g, g.vol[k], T[w_], rho[w_], R(0.5) * g.hx[i - 1]);
     }
     if (j >= 1) {
       const unsigned s_ = k - Nx;
-      if (m[k] && !m[s_]) FLD(TW_V)[k] = wall_T(g, g.vol[s_], T[k], rho[k], 0.5 * g.hy[j]);
-      else if (m[s_] && !m[k]) FLD(TW_V)[k] = wall_T(g, g.vol[k], T[s_], rho[s_], 0.5 * g.hy[j - 1]);
+      if (m[k] && !m[s_]) FLD(TW_V)[k] = wall_T(g, g.vol[s_], T[k], rho[k], R(0.5) * g.hy[j]);
+      else if (m[s_] && !m[k]) FLD(TW_V)[k] = wall_T(g, g.vol[k], T[s_], rho[s_], R(0.5) * g.hy[j - 1]);
     }
   }
 }
@@ -576,7 +599,8 @@ __global__ void __launch_bounds__(BX *BY) k_wall(KGrid g, KFields fl, int parts,
 // ---------------------------------------------------------------------------
 constexpr int EDGE_THREADS = 1024;
 
-__device__ __forceinline__ void edge_x(const KGrid &g, double *a, int b0, int b1, bool lo, bool hi) {
+template <typename R>
+__device__ __forceinline__ void edge_x(const KGridT<R> &g, R *a, int b0, int b1, bool lo, bool hi) {
   const int Nx = g.Nx, Ny = g.Ny;
   const uint8_t *m = g.m;
   if (lo) {
@@ -596,7 +620,8 @@ __device__ __forceinline__ void edge_x(const KGrid &g, double *a, int b0, int b1
   }
 }
 
-__device__ __forceinline__ void edge_y(const KGrid &g, double *a, int b0, int b1, bool lo, bool hi) {
+template <typename R>
+__device__ __forceinline__ void edge_y(const KGridT<R> &g, R *a, int b0, int b1, bool lo, bool hi) {
   const int Nx = g.Nx, Ny = g.Ny;
   const uint8_t *m = g.m;
   if (lo) {
@@ -616,7 +641,8 @@ __device__ __forceinline__ void edge_y(const KGrid &g, double *a, int b0, int b1
   }
 }
 
-__global__ void __launch_bounds__(EDGE_THREADS) k_edges(KGrid g, KFields fl, int parts, double *probe, Gate gate,
+template <typename R>
+__global__ void __launch_bounds__(EDGE_THREADS) k_edges(KGridT<R> g, KFieldsT<R> fl, int parts, double *probe, Gate gate,
                                                         LoopTest lt) {
   if (gated_out(gate)) return;
   if (lt.ctrl && threadIdx.x < NSLOT) {  // wave 0: fold the slots of iteration gate.k
@@ -638,6 +664,18 @@ __global__ void __launch_bounds__(EDGE_THREADS) k_edges(KGrid g, KFields fl, int
     }
     bool cont = (gate.k + 1 < lt.N_I) &&
                 (lt.tol_uv < r[0] || lt.tol_uv < r[1] || lt.tol_p < r[2] || lt.tol_T < r[3]);
+    if (cont && lt.stall_w > 0 && gate.k >= 2 * lt.stall_w) {
+      // stagnation (FP32 phase): the largest residual relative to its
+      // tolerance fell by less than stall_ratio over the last stall_w iterations
+      const unsigned long long *lb = lt.log_k - 6 * lt.stall_w;
+      const double tol[4] = {lt.tol_uv, lt.tol_uv, lt.tol_p, lt.tol_T};
+      double now = 0.0, then = 0.0;
+      for (int q = 0; q < 4; q++) {
+        now = fmax(now, r[q] / tol[q]);
+        then = fmax(then, __longlong_as_double((long long)lb[q]) / tol[q]);
+      }
+      if (now > lt.stall_ratio * then) cont = false;
+    }
     if (!fin) {
       lt.ctrl->nonfinite = 1;
       cont = false;
@@ -647,9 +685,9 @@ __global__ void __launch_bounds__(EDGE_THREADS) k_edges(KGrid g, KFields fl, int
   const int Nx = g.Nx, Ny = g.Ny;
   const uint8_t *m = g.m;
   const unsigned bc = g.bc;
-  double *u = FLD(U), *v = FLD(V);
-  const double *rho = FLD(RHO), *rho_pr = FLD(RHO_PR), *rho_u = FLD(RHO_U), *rho_v = FLD(RHO_V);
-  const double *hx = g.hx, *hy = g.hy;
+  R *u = FLD(U), *v = FLD(V);
+  const R *rho = FLD(RHO), *rho_pr = FLD(RHO_PR), *rho_u = FLD(RHO_U), *rho_v = FLD(RHO_V);
+  const R *hx = g.hx, *hy = g.hy;
   if (parts & STS_BC_VELOCITIES) {
     edge_x(g, v, g.Nx_v0, g.Nx_v1, bc & BC_DVDX_XB, false);
     __syncthreads();
@@ -700,8 +738,8 @@ __global__ void __launch_bounds__(EDGE_THREADS) k_edges(KGrid g, KFields fl, int
     }
     __syncthreads();
     if (bc & BC_UMINMOUT_XE) {  // S:7031-7059 (serial build): the last row's rho_u
-      const double Min = rho_u[g.Nx_u0 + (Ny - 1) * Nx];
-      const double Mout = rho_u[g.Nx_u1 + (Ny - 1) * Nx];
+      const R Min = rho_u[g.Nx_u0 + (Ny - 1) * Nx];
+      const R Mout = rho_u[g.Nx_u1 + (Ny - 1) * Nx];
       for (int j = threadIdx.x; j < Ny; j += EDGE_THREADS) {
         int node = g.Nx_u1 - 1 + j * Nx;
         u[node + 1] = u[node] * Min / Mout;
@@ -714,7 +752,7 @@ __global__ void __launch_bounds__(EDGE_THREADS) k_edges(KGrid g, KFields fl, int
     __syncthreads();
   }
   if (parts & STS_BC_TEMPERATURE) {
-    double *T = FLD(T);
+    R *T = FLD(T);
     edge_x(g, T, g.Nx_o0, g.Nx_o1, bc & BC_DTDX_XB, false);
     __syncthreads();
     edge_x(g, T, g.Nx_o0, g.Nx_o1, false, bc & BC_DTDX_XE);
@@ -725,7 +763,7 @@ __global__ void __launch_bounds__(EDGE_THREADS) k_edges(KGrid g, KFields fl, int
     __syncthreads();
   }
   if (parts & STS_BC_PRESSURE) {
-    double *p = FLD(P);
+    R *p = FLD(P);
     edge_x(g, p, g.Nx_o0, g.Nx_o1, bc & BC_DPDX_XB, false);
     __syncthreads();
     edge_x(g, p, g.Nx_o0, g.Nx_o1, false, bc & BC_DPDX_XE);
@@ -738,7 +776,7 @@ __global__ void __launch_bounds__(EDGE_THREADS) k_edges(KGrid g, KFields fl, int
   if (probe && threadIdx.x == 0) {
     // S:7512-7513: the cell p_BC_xb is read from (not touched by the edge copies)
     int jj = int((g.Ny_o1 - 1.0) - g.Ny_o0);
-    probe[0] = (jj >= 0 && jj < Ny) ? FLD(P)[g.Nx_o0 + jj * Nx] : 0.0;
+    probe[0] = (jj >= 0 && jj < Ny) ? (double)FLD(P)[g.Nx_o0 + jj * Nx] : 0.0;
   }
 }
 
@@ -746,52 +784,53 @@ __global__ void __launch_bounds__(EDGE_THREADS) k_edges(KGrid g, KFields fl, int
 // A10 + A11: fluxes (S:10440-10454) and diffusion coefficients
 // (S:10457-10618) over all cells.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(BX *BY) k_flux_diff(KGrid g, KFields fl, Gate gate) {
+template <typename R>
+__global__ void __launch_bounds__(BX *BY) k_flux_diff(KGridT<R> g, KFieldsT<R> fl, Gate gate) {
   if (gated_out(gate)) return;
   const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
   if (i >= g.Nx || j >= g.Ny) return;
   const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny);
   const uint8_t *m = g.m;
-  const double *hx = g.hx, *hy = g.hy, *sT = FLD(SQRT_T), *Twu = FLD(TW_U), *Twv = FLD(TW_V);
+  const R *hx = g.hx, *hy = g.hy, *sT = FLD(SQRT_T), *Twu = FLD(TW_U), *Twv = FLD(TW_V);
   const unsigned ij = n.ij, i_1j = n.i_1j, ij_1 = n.ij_1, i_1j_1 = n.i_1j_1, i_1 = n.i_1;
   const unsigned jm = (unsigned)j - 1u * (0 < j);
-  const double cT = g.cT, cT1 = g.cT1;
+  const R cT = g.cT, cT1 = g.cT1;
   FLD(FX)[ij] = hy[j] * FLD(RHO_U)[ij] * FLD(U)[ij];
   FLD(FY)[ij] = hx[i] * FLD(RHO_V)[ij] * FLD(V)[ij];
   const bool m00 = m[i_1j_1], m10 = m[ij_1], m01 = m[i_1j], m11 = m[ij];
   // All four cells body: the reference leaves Gamma untouched (S:10477-10543).
-  double Gy = 0.0, Gx = 0.0;
+  R Gy = R(0.0), Gx = R(0.0);
   if (m00 && m10 && m01 && m11) {
-    Gy = 0.5 * (Li(sT[i_1j_1], sT[i_1j], hy[jm], hy[j]) + Li(sT[ij_1], sT[ij], hy[jm], hy[j]));
-    Gx = 0.5 * (Li(sT[i_1j_1], sT[ij_1], hx[i_1], hx[i]) + Li(sT[i_1j], sT[ij], hx[i_1], hx[i]));
+    Gy = R(0.5) * (Li(sT[i_1j_1], sT[i_1j], hy[jm], hy[j]) + Li(sT[ij_1], sT[ij], hy[jm], hy[j]));
+    Gx = R(0.5) * (Li(sT[i_1j_1], sT[ij_1], hx[i_1], hx[i]) + Li(sT[i_1j], sT[ij], hx[i_1], hx[i]));
     FLD(GAMMA_YF)[ij] = Gy;
     FLD(GAMMA_XF)[ij] = Gx;
   } else if (m00 || m10 || m01 || m11) {
-    Gy = 0.5 * (bd(m00 && m01) * Li(sT[i_1j_1], sT[i_1j], hy[jm], hy[j]) + bd(m00 != m01) * sqrt(Twv[i_1j])
-                + bd(m10 && m11) * Li(sT[ij_1], sT[ij], hy[jm], hy[j]) + bd(m10 != m11) * sqrt(Twv[ij])
-                + bd((!(m00 && m01)) || (!(m10 && m11))) * Li(sqrt(Twu[ij_1]), sqrt(Twu[ij]), hy[jm], hy[j]));
-    Gx = 0.5 * (bd(m00 && m10) * Li(sT[i_1j_1], sT[ij_1], hx[i_1], hx[i]) + bd(m00 != m10) * sqrt(Twu[ij_1])
-                + bd(m01 && m11) * Li(sT[i_1j], sT[ij], hx[i_1], hx[i]) + bd(m01 != m11) * sqrt(Twu[ij])
-                + bd((!(m00 && m10)) || (!(m01 && m11))) * Li(sqrt(Twv[i_1j]), sqrt(Twv[ij]), hx[i_1], hx[i]));
+    Gy = R(0.5) * (R(m00 && m01) * Li(sT[i_1j_1], sT[i_1j], hy[jm], hy[j]) + R(m00 != m01) * sqrt(Twv[i_1j])
+                + R(m10 && m11) * Li(sT[ij_1], sT[ij], hy[jm], hy[j]) + R(m10 != m11) * sqrt(Twv[ij])
+                + R((!(m00 && m01)) || (!(m10 && m11))) * Li(sqrt(Twu[ij_1]), sqrt(Twu[ij]), hy[jm], hy[j]));
+    Gx = R(0.5) * (R(m00 && m10) * Li(sT[i_1j_1], sT[ij_1], hx[i_1], hx[i]) + R(m00 != m10) * sqrt(Twu[ij_1])
+                + R(m01 && m11) * Li(sT[i_1j], sT[ij], hx[i_1], hx[i]) + R(m01 != m11) * sqrt(Twu[ij])
+                + R((!(m00 && m10)) || (!(m01 && m11))) * Li(sqrt(Twv[i_1j]), sqrt(Twv[ij]), hx[i_1], hx[i]));
     FLD(GAMMA_YF)[ij] = Gy;
     FLD(GAMMA_XF)[ij] = Gx;
   }
   FLD(D_UX)[ij] = cT * sT[i_1j] * (hy[j] / hx[i_1]);
   FLD(D_UY)[ij] = ((m11 && m01) || (m10 && m00))
-                      ? cT * Gy * ((hx[i] + hx[i_1]) / (bd(m11 * m01) * hy[j] + bd(m10 * m00) * hy[jm]))
-                      : 0.0;
+                      ? cT * Gy * ((hx[i] + hx[i_1]) / (R(m11 * m01) * hy[j] + R(m10 * m00) * hy[jm]))
+                      : R(0.0);
   FLD(D_VX)[ij] = ((m11 && m10) || (m01 && m00))
-                      ? cT * Gx * ((hy[j] + hy[jm]) / (bd(m11 * m10) * hx[i] + bd(m01 * m00) * hx[i_1]))
-                      : 0.0;
+                      ? cT * Gx * ((hy[j] + hy[jm]) / (R(m11 * m10) * hx[i] + R(m01 * m00) * hx[i_1]))
+                      : R(0.0);
   FLD(D_VY)[ij] = cT * sT[ij_1] * (hx[i] / hy[jm]);
-  FLD(DTX)[ij] = (m11 || m01) ? cT1 * (hy[j] / (0.5 * (bd(m11) * hx[i] + bd(m01) * hx[i_1])))
-                                    * (bd((!m11) || (!m01)) * sqrt(Twu[ij])
-                                       + bd(m11 * m01) * Hi(sT[i_1j], sT[ij], hx[i_1], hx[i]))
-                              : 0.0;
-  FLD(DTY)[ij] = (m11 || m10) ? cT1 * (hx[i] / (0.5 * (bd(m11) * hy[j] + bd(m10) * hy[jm])))
-                                    * (bd((!m11) || (!m10)) * sqrt(Twv[ij])
-                                       + bd(m11 * m10) * Hi(sT[ij_1], sT[ij], hy[jm], hy[j]))
-                              : 0.0;
+  FLD(DTX)[ij] = (m11 || m01) ? cT1 * (hy[j] / (R(0.5) * (R(m11) * hx[i] + R(m01) * hx[i_1])))
+                                    * (R((!m11) || (!m01)) * sqrt(Twu[ij])
+                                       + R(m11 * m01) * Hi(sT[i_1j], sT[ij], hx[i_1], hx[i]))
+                              : R(0.0);
+  FLD(DTY)[ij] = (m11 || m10) ? cT1 * (hx[i] / (R(0.5) * (R(m11) * hy[j] + R(m10) * hy[jm])))
+                                    * (R((!m11) || (!m10)) * sqrt(Twv[ij])
+                                       + R(m11 * m10) * Hi(sT[ij_1], sT[ij], hy[jm], hy[j]))
+                              : R(0.0);
 }
 
 // ---------------------------------------------------------------------------
@@ -867,6 +906,16 @@ __global__ void k_loop_open(LoopCtrl *ctrl, unsigned long long *log, int n, unsi
   for (int q = t; q < NSLOT * SLOT_STRIDE; q += gridDim.x * blockDim.x) lred[q] = 0ull;
 }
 
+// Element-wise type conversion of the fields selected by mask (bit f = field
+// f) between the FP64 state and its FP32 copy (round to nearest / exact).
+template <typename S, typename D>
+__global__ void __launch_bounds__(256) k_convert(KFieldsT<S> src, KFieldsT<D> dst, unsigned long long mask, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  for (int f = 0; f < STS_NFIELDS; f++)
+    if ((mask >> f) & 1ull) dst.f[f][t] = (D)src.f[f][t];
+}
+
 // Folds the NSLOT copies of words 0..n-1 of a slotted buffer into out[0..n-1]
 // and clears the copies.
 __global__ void k_fold(unsigned long long *slots, int n, unsigned long long *out) {
@@ -940,49 +989,78 @@ __global__ void __launch_bounds__(BX *BY) k_continuity(KGrid g, KFields fl, doub
 // ---------------------------------------------------------------------------
 #include "sts_launch.h"
 
-static inline dim3 grid2d(const KGrid &g) { return dim3((g.Nx + BX - 1) / BX, (g.Ny + BY - 1) / BY); }
+template <typename R>
+static inline dim3 grid2d(const KGridT<R> &g) {
+  return dim3((g.Nx + BX - 1) / BX, (g.Ny + BY - 1) / BY);
+}
 
-hipError_t launch_assemble(const KGrid &g, const KFields &f, hipStream_t s, Gate gate) {
-  hipLaunchKernelGGL(k_assemble, grid2d(g), dim3(BX, BY), 0, s, g, f, gate);
+template <typename R>
+hipError_t launch_assemble(const KGridT<R> &g, const KFieldsT<R> &f, hipStream_t s, Gate gate) {
+  hipLaunchKernelGGL(k_assemble<R>, grid2d(g), dim3(BX, BY), 0, s, g, f, gate);
   return hipGetLastError();
 }
-hipError_t launch_sweep(const KGrid &g, const KFields &f, const double *Po, const double *To, double *Pn, double *Tn,
+template <typename R>
+hipError_t launch_sweep(const KGridT<R> &g, const KFieldsT<R> &f, const R *Po, const R *To, R *Pn, R *Tn,
                         unsigned long long *rp, unsigned long long *rT, hipStream_t s, unsigned long long *rp2,
                         unsigned long long *rT2, Gate gate) {
-  hipLaunchKernelGGL(k_sweep, grid2d(g), dim3(BX, BY), 0, s, g, f, Po, To, Pn, Tn, rp, rT, rp2, rT2, gate);
+  hipLaunchKernelGGL(k_sweep<R>, grid2d(g), dim3(BX, BY), 0, s, g, f, Po, To, Pn, Tn, rp, rT, rp2, rT2, gate);
   return hipGetLastError();
 }
-hipError_t launch_sweep2(const KGrid &g, const KFields &f, const double *Po, const double *To, double *Pn, double *Tn,
+template <typename R>
+hipError_t launch_sweep2(const KGridT<R> &g, const KFieldsT<R> &f, const R *Po, const R *To, R *Pn, R *Tn,
                          unsigned long long *r1p, unsigned long long *r1T, unsigned long long *r2p,
                          unsigned long long *r2T, hipStream_t s, Gate gate) {
   dim3 gr((g.Nx + S2X - 1) / S2X, (g.Ny + S2Y - 1) / S2Y);
-  hipLaunchKernelGGL(k_sweep2, gr, dim3(S2X, S2Y), 0, s, g, f, Po, To, Pn, Tn, r1p, r1T, r2p, r2T, gate);
+  hipLaunchKernelGGL(k_sweep2<R>, gr, dim3(S2X, S2Y), 0, s, g, f, Po, To, Pn, Tn, r1p, r1T, r2p, r2T, gate);
   return hipGetLastError();
 }
-hipError_t launch_correct(const KGrid &g, const KFields &f, unsigned long long *ru, unsigned long long *rv,
+template <typename R>
+hipError_t launch_correct(const KGridT<R> &g, const KFieldsT<R> &f, unsigned long long *ru, unsigned long long *rv,
                           int do_correct, hipStream_t s, Gate gate) {
-  hipLaunchKernelGGL(k_correct, grid2d(g), dim3(BX, BY), 0, s, g, f, ru, rv, do_correct, gate);
+  hipLaunchKernelGGL(k_correct<R>, grid2d(g), dim3(BX, BY), 0, s, g, f, ru, rv, do_correct, gate);
   return hipGetLastError();
 }
+template <typename R>
+hipError_t launch_wall(const KGridT<R> &g, const KFieldsT<R> &f, int parts, hipStream_t s, Gate gate) {
+  hipLaunchKernelGGL(k_wall<R>, grid2d(g), dim3(BX, BY), 0, s, g, f, parts, gate);
+  return hipGetLastError();
+}
+template <typename R>
+hipError_t launch_edges(const KGridT<R> &g, const KFieldsT<R> &f, int parts, double *probe, hipStream_t s,
+                        Gate gate, LoopTest lt) {
+  hipLaunchKernelGGL(k_edges<R>, dim3(1), dim3(EDGE_THREADS), 0, s, g, f, parts, probe, gate, lt);
+  return hipGetLastError();
+}
+template <typename R>
+hipError_t launch_flux_diff(const KGridT<R> &g, const KFieldsT<R> &f, hipStream_t s, Gate gate) {
+  hipLaunchKernelGGL(k_flux_diff<R>, grid2d(g), dim3(BX, BY), 0, s, g, f, gate);
+  return hipGetLastError();
+}
+
+#define STS_INSTANTIATE(R)                                                                                     \
+  template hipError_t launch_assemble<R>(const KGridT<R> &, const KFieldsT<R> &, hipStream_t, Gate);           \
+  template hipError_t launch_sweep<R>(const KGridT<R> &, const KFieldsT<R> &, const R *, const R *, R *, R *,  \
+                                      unsigned long long *, unsigned long long *, hipStream_t,                 \
+                                      unsigned long long *, unsigned long long *, Gate);                       \
+  template hipError_t launch_sweep2<R>(const KGridT<R> &, const KFieldsT<R> &, const R *, const R *, R *, R *, \
+                                       unsigned long long *, unsigned long long *, unsigned long long *,       \
+                                       unsigned long long *, hipStream_t, Gate);                               \
+  template hipError_t launch_correct<R>(const KGridT<R> &, const KFieldsT<R> &, unsigned long long *,          \
+                                        unsigned long long *, int, hipStream_t, Gate);                         \
+  template hipError_t launch_wall<R>(const KGridT<R> &, const KFieldsT<R> &, int, hipStream_t, Gate);          \
+  template hipError_t launch_edges<R>(const KGridT<R> &, const KFieldsT<R> &, int, double *, hipStream_t, Gate, \
+                                      LoopTest);                                                               \
+  template hipError_t launch_flux_diff<R>(const KGridT<R> &, const KFieldsT<R> &, hipStream_t, Gate);
+STS_INSTANTIATE(double)
+STS_INSTANTIATE(float)
+#undef STS_INSTANTIATE
+
 hipError_t launch_properties(const KGrid &g, const KFields &f, hipStream_t s) {
-  hipLaunchKernelGGL(k_properties, grid2d(g), dim3(BX, BY), 0, s, g, f);
+  hipLaunchKernelGGL(k_properties<double>, grid2d(g), dim3(BX, BY), 0, s, g, f);
   return hipGetLastError();
 }
 hipError_t launch_rho_mid(const KGrid &g, const KFields &f, hipStream_t s) {
-  hipLaunchKernelGGL(k_rho_mid, grid2d(g), dim3(BX, BY), 0, s, g, f);
-  return hipGetLastError();
-}
-hipError_t launch_wall(const KGrid &g, const KFields &f, int parts, hipStream_t s, Gate gate) {
-  hipLaunchKernelGGL(k_wall, grid2d(g), dim3(BX, BY), 0, s, g, f, parts, gate);
-  return hipGetLastError();
-}
-hipError_t launch_edges(const KGrid &g, const KFields &f, int parts, double *probe, hipStream_t s, Gate gate,
-                        LoopTest lt) {
-  hipLaunchKernelGGL(k_edges, dim3(1), dim3(EDGE_THREADS), 0, s, g, f, parts, probe, gate, lt);
-  return hipGetLastError();
-}
-hipError_t launch_flux_diff(const KGrid &g, const KFields &f, hipStream_t s, Gate gate) {
-  hipLaunchKernelGGL(k_flux_diff, grid2d(g), dim3(BX, BY), 0, s, g, f, gate);
+  hipLaunchKernelGGL(k_rho_mid<double>, grid2d(g), dim3(BX, BY), 0, s, g, f);
   return hipGetLastError();
 }
 hipError_t launch_apply_bodies(const KGrid &g, const KFields &f, hipStream_t s) {
@@ -1010,6 +1088,16 @@ hipError_t launch_loop_open(LoopCtrl *ctrl, unsigned long long *log, int n, unsi
   hipLaunchKernelGGL(k_loop_open, dim3(16), dim3(256), 0, s, ctrl, log, n, lred);
   return hipGetLastError();
 }
+template <typename S, typename D>
+hipError_t launch_convert(const KFieldsT<S> &src, const KFieldsT<D> &dst, unsigned long long mask, int n,
+                          hipStream_t s) {
+  hipLaunchKernelGGL((k_convert<S, D>), dim3((n + 255) / 256), dim3(256), 0, s, src, dst, mask, n);
+  return hipGetLastError();
+}
+template hipError_t launch_convert<double, float>(const KFieldsT<double> &, const KFieldsT<float> &,
+                                                  unsigned long long, int, hipStream_t);
+template hipError_t launch_convert<float, double>(const KFieldsT<float> &, const KFieldsT<double> &,
+                                                  unsigned long long, int, hipStream_t);
 hipError_t launch_fold(unsigned long long *slots, int n, unsigned long long *out, hipStream_t s) {
   hipLaunchKernelGGL(k_fold, dim3(1), dim3(128), 0, s, slots, n, out);
   return hipGetLastError();

@@ -1,0 +1,510 @@
+// Kernels of the warm-start U-Net (gfx950): BF16 activations and weights,
+// FP32 accumulation on the matrix cores (v_mfma_f32_16x16x32_bf16).
+//
+// Tensors are NHWC with channels fastest; a pixel index p runs over
+// (n, j, i) with i fastest.  Every BF16 value is stored as its 16 raw bits.
+//
+// 3x3 convolution = implicit GEMM  Y[p][co] = sum_k A[p][k] * Wt[co][k],
+// k = tap * Cx + ci (Cx = channel stride of the input, a multiple of 8, so
+// the 8 consecutive k of one MFMA lane are 8 channels of one tap: one 16-byte
+// load), Kp = k range rounded up to 32 with zero weights.  A 16x16x32 MFMA
+// takes from lane l = r + 16 g the A row r (pixel) and the B column r (output
+// channel), both at k = 8g..8g+7, and leaves D rows 4g..4g+3 of column r.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "unet_launch.h"
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16;
+
+struct alignas(16) U16x8 {
+  u16 v[8];
+};
+
+__device__ __forceinline__ u16 f2bf(float f) {  // round to nearest even
+  unsigned b = __float_as_uint(f);
+  b += 0x7FFFu + ((b >> 16) & 1u);
+  return (u16)(b >> 16);
+}
+__device__ __forceinline__ float bf2f(u16 h) { return __uint_as_float((unsigned)h << 16); }
+
+__device__ __forceinline__ bf16x8 load8(const u16 *p) {
+  const U16x8 q = *reinterpret_cast<const U16x8 *>(p);
+  return __builtin_bit_cast(bf16x8, q);
+}
+__device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, U16x8{}); }
+
+// ---------------------------------------------------------------------------
+// Convolution (forward, and backward-data with flipped weights)
+// Block: 4 waves; wave w computes pixels [p0, p0+32) x channels [c0, c0+32)
+// with p0 = (4 blockIdx.x + w) * 32, c0 = 32 blockIdx.y.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r = lane & 15, g = lane >> 4;
+  const int npix = a.N * a.H * a.W;
+  const int p0 = (blockIdx.x * 4 + wave) * 32, c0 = blockIdx.y * 32;
+  if (p0 >= npix) return;
+  const int rad = a.ks / 2, taps = a.ks * a.ks;
+  int pn[2], pj[2], pi[2];
+  bool pv[2];
+#pragma unroll
+  for (int s = 0; s < 2; s++) {
+    const int p = p0 + 16 * s + r;
+    pv[s] = p < npix;
+    const int q = pv[s] ? p : 0;
+    pi[s] = q % a.W;
+    const int t = q / a.W;
+    pj[s] = t % a.H;
+    pn[s] = t / a.H;
+  }
+  floatx4 acc[2][2];
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++) acc[x][y] = floatx4{0.f, 0.f, 0.f, 0.f};
+  for (int kc = 0; kc < a.Kp; kc += 32) {
+    const int k = kc + 8 * g;
+    const int t = k / a.Cx, ci = k - t * a.Cx;
+    bf16x8 A[2], B[2];
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+      A[s] = zero8();
+      if (t < taps && pv[s]) {
+        const int jj = pj[s] + t / a.ks - rad, ii = pi[s] + t % a.ks - rad;
+        if (jj >= 0 && jj < a.H && ii >= 0 && ii < a.W)
+          A[s] = load8(a.X + ((size_t)(pn[s] * a.H + jj) * a.W + ii) * a.Cx + ci);
+      }
+      const int co = c0 + 16 * s + r;
+      B[s] = (co < a.Cout) ? load8(a.Wt + (size_t)co * a.Kp + k) : zero8();
+    }
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+      for (int y = 0; y < 2; y++) acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[x], B[y], acc[x][y], 0, 0, 0);
+  }
+#pragma unroll
+  for (int y = 0; y < 2; y++) {
+    const int co = c0 + 16 * y + r;
+    if (co >= a.Cout) continue;
+    const float bias = a.bias ? a.bias[co] : 0.f;
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int p = p0 + 16 * x + 4 * g + q;
+        if (p >= npix) continue;
+        float v = acc[x][y][q] + bias;
+        if (a.relu) v = fmaxf(v, 0.f);
+        a.Y[(size_t)p * a.Cout + co] = f2bf(v);
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Weight gradient dW[t][ci][co] += sum_p X[p + off(t)][ci] * dZ[p][co]:
+// GEMM with M = co, N = k = t*Cx + ci, reduced over pixels.  Wave w of block
+// (bx, by, bz) owns k tile [64 bx + 16 w, +16), co tile [16 by, +16), pixels
+// [bz * chunk, +chunk); partial sums go to dW with atomicAdd.  W (row length)
+// is a multiple of 8, so the 8 pixels of a lane are one row segment.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_wgrad(WgradArgs a) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r = lane & 15, g = lane >> 4;
+  const int npix = a.N * a.H * a.W;
+  const int kk = blockIdx.x * 64 + wave * 16 + r;  // B column of this lane
+  const int co = blockIdx.y * 16 + r;              // A row of this lane
+  const int rad = a.ks / 2, taps = a.ks * a.ks;
+  const int t = kk / a.Cx, ci = kk - t * a.Cx;
+  const bool kv = t < taps && ci < a.Cin;
+  const int dy = kv ? t / a.ks - rad : 0, dx = kv ? t % a.ks - rad : 0;
+  const bool cv = co < a.Cout;
+  const int pbeg = blockIdx.z * a.chunk, pend = min(npix, pbeg + a.chunk);
+  floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+  for (int pc = pbeg; pc < pend; pc += 32) {
+    const int p = pc + 8 * g;  // 8 pixels p..p+7, one row segment
+    U16x8 av{}, bv{};
+    if (p < pend) {
+      const int i0 = p % a.W, tt = p / a.W, j = tt % a.H, n = tt / a.H;
+      if (cv) {
+#pragma unroll
+        for (int q = 0; q < 8; q++) av.v[q] = a.dZ[(size_t)(p + q) * a.Cout + co];
+      }
+      const int jj = j + dy;
+      if (kv && jj >= 0 && jj < a.H) {
+        const size_t rowb = (size_t)(n * a.H + jj) * a.W;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+          const int ii = i0 + q + dx;
+          bv.v[q] = (ii >= 0 && ii < a.W) ? a.X[(rowb + ii) * a.Cx + ci] : (u16)0;
+        }
+      }
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av), __builtin_bit_cast(bf16x8, bv), acc,
+                                                  0, 0, 0);
+  }
+  // D rows = co (4g + q of the tile), column = k tile entry r
+  const int col_k = blockIdx.x * 64 + wave * 16 + r;
+  const int tc = col_k / a.Cx, cc = col_k - tc * a.Cx;
+  if (tc >= taps || cc >= a.Cin) return;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int c = blockIdx.y * 16 + 4 * g + q;
+    if (c < a.Cout) atomicAdd(&a.dW[((size_t)tc * a.Cin + cc) * a.Cout + c], acc[q]);
+  }
+}
+
+// db[co] += sum_p dZ[p][co]; block = 256 threads over a pixel chunk.
+__global__ void __launch_bounds__(256) k_bgrad(const u16 *dZ, int npix, int Cout, int chunk, float *db) {
+  const int pbeg = blockIdx.x * chunk, pend = min(npix, pbeg + chunk);
+  for (int co = threadIdx.x; co < Cout; co += blockDim.x) {
+    float s = 0.f;
+    for (int p = pbeg; p < pend; p++) s += bf2f(dZ[(size_t)p * Cout + co]);
+    atomicAdd(&db[co], s);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Element-wise and layout kernels
+// ---------------------------------------------------------------------------
+// 2x2 average pool: X [N][H][W][C] -> Y [N][H/2][W/2][C]
+__global__ void k_pool(const u16 *X, int N, int H, int W, int C, u16 *Y) {
+  const size_t n_out = (size_t)N * (H / 2) * (W / 2) * C;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_out) return;
+  const int c = (int)(e % C);
+  size_t q = e / C;
+  const int i = (int)(q % (W / 2));
+  q /= (W / 2);
+  const int j = (int)(q % (H / 2));
+  const int n = (int)(q / (H / 2));
+  const size_t b = ((size_t)(n * H + 2 * j) * W + 2 * i) * C + c;
+  const float s = (bf2f(X[b]) + bf2f(X[b + C])) + (bf2f(X[b + (size_t)W * C]) + bf2f(X[b + (size_t)W * C + C]));
+  Y[e] = f2bf(s * 0.25f);
+}
+
+// concat [nearest-up(B), S]: B [N][H/2][W/2][Cb], S [N][H][W][Cs] -> O [N][H][W][Cb+Cs]
+__global__ void k_upcat(const u16 *B, int Cb, const u16 *S, int Cs, int N, int H, int W, u16 *O) {
+  const int C = Cb + Cs;
+  const size_t n_out = (size_t)N * H * W * C;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_out) return;
+  const int c = (int)(e % C);
+  const size_t p = e / C;
+  const int i = (int)(p % W);
+  const size_t t = p / W;
+  const int j = (int)(t % H), n = (int)(t / H);
+  O[e] = (c < Cb) ? B[((size_t)(n * (H / 2) + j / 2) * (W / 2) + i / 2) * Cb + c] : S[p * Cs + (c - Cb)];
+}
+
+// dZ = G * (Act > 0)
+__global__ void k_relu_mask(const u16 *G, const u16 *Act, size_t n, u16 *dZ) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  dZ[e] = (bf2f(Act[e]) > 0.f) ? G[e] : (u16)0;
+}
+
+// Backward of nearest-up on channels [0, Cu) of D [N][H][W][Cd]:
+// O [N][H/2][W/2][Cu] = round(sum of the 4 children)
+__global__ void k_up_back(const u16 *D, int Cd, int Cu, int N, int H, int W, u16 *O) {
+  const size_t n_out = (size_t)N * (H / 2) * (W / 2) * Cu;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_out) return;
+  const int c = (int)(e % Cu);
+  size_t q = e / Cu;
+  const int i = (int)(q % (W / 2));
+  q /= (W / 2);
+  const int j = (int)(q % (H / 2)), n = (int)(q / (H / 2));
+  const size_t b = ((size_t)(n * H + 2 * j) * W + 2 * i) * Cd + c;
+  const size_t rw = (size_t)W * Cd;
+  const float s = (bf2f(D[b]) + bf2f(D[b + Cd])) + (bf2f(D[b + rw]) + bf2f(D[b + rw + Cd]));
+  O[e] = f2bf(s);
+}
+
+// Backward of the pool plus the skip branch: O [N][H][W][C] =
+// round(round(P[parent] / 4) + Skip[p][off + c]) with Skip of stride Cs.
+__global__ void k_pool_back_add(const u16 *P, const u16 *Skip, int Cs, int off, int N, int H, int W, int C, u16 *O) {
+  const size_t n_out = (size_t)N * H * W * C;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_out) return;
+  const int c = (int)(e % C);
+  const size_t p = e / C;
+  const int i = (int)(p % W);
+  const size_t t = p / W;
+  const int j = (int)(t % H), n = (int)(t / H);
+  const float pb = bf2f(f2bf(bf2f(P[((size_t)(n * (H / 2) + j / 2) * (W / 2) + i / 2) * C + c]) * 0.25f));
+  O[e] = f2bf(pb + bf2f(Skip[p * Cs + off + c]));
+}
+
+// Head (1x1 conv, linear): Y[p][o] = sum_c X[p][c] * Wb[c][o] + b[o], FP32 out.
+__global__ void k_head(const u16 *X, int C, const u16 *Wb, const float *b, int Co, int npix, float *Y) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npix) return;
+  for (int o = 0; o < Co; o++) {
+    float s = 0.f;
+    for (int c = 0; c < C; c++) s += bf2f(X[(size_t)p * C + c]) * bf2f(Wb[c * Co + o]);
+    Y[(size_t)p * Co + o] = s + b[o];
+  }
+}
+
+// Head backward: dX[p][c] = round(sum_o dY[p][o] Wb[c][o]); dW[c][o] and
+// db[o] accumulate per block, then atomically.
+__global__ void __launch_bounds__(256) k_head_back(const float *dY, const u16 *X, int C, const u16 *Wb, int Co,
+                                                   int npix, u16 *dX, float *dW, float *db) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool ok = p < npix;
+  for (int c = 0; c < C; c++) {
+    float s = 0.f;
+    if (ok)
+      for (int o = 0; o < Co; o++) s += dY[(size_t)p * Co + o] * bf2f(Wb[c * Co + o]);
+    if (ok) dX[(size_t)p * C + c] = f2bf(s);
+  }
+  __shared__ float red[256];
+  for (int q = 0; q < C * Co; q++) {  // dW[c][o], one block reduction each
+    const int c = q / Co, o = q - c * Co;
+    red[threadIdx.x] = ok ? dY[(size_t)p * Co + o] * bf2f(X[(size_t)p * C + c]) : 0.f;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+      if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(&dW[q], red[0]);
+    __syncthreads();
+  }
+  for (int o = 0; o < Co; o++) {
+    red[threadIdx.x] = ok ? dY[(size_t)p * Co + o] : 0.f;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+      if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(&db[o], red[0]);
+    __syncthreads();
+  }
+}
+
+// Masked MSE: dY = (Y - T) * m / count; loss += sum((Y - T)^2 m) / (2 count)
+__global__ void __launch_bounds__(256) k_mse(const float *Y, const float *T, const uint8_t *M, int Co, int npix,
+                                             float inv_count, float *dY, double *loss) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  double s = 0.0;
+  if (p < npix) {
+    const float m = M[p] ? 1.f : 0.f;
+    for (int o = 0; o < Co; o++) {
+      const float e = (Y[(size_t)p * Co + o] - T[(size_t)p * Co + o]) * m;
+      dY[(size_t)p * Co + o] = e * inv_count;
+      s += (double)e * (double)e;
+    }
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(loss, red[0] * 0.5 * (double)inv_count);
+}
+
+// Adam on FP32 master parameters.
+__global__ void k_adam(float *P, const float *G, float *m, float *v, size_t n, float lr, float b1, float b2, float eps,
+                       float c1, float c2) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const float g = G[e];
+  const float mm = b1 * m[e] + (1.f - b1) * g;
+  const float vv = b2 * v[e] + (1.f - b2) * g * g;
+  m[e] = mm;
+  v[e] = vv;
+  const float mh = mm / c1, vh = vv / c2;
+  P[e] -= lr * mh / (sqrtf(vh) + eps);
+}
+
+// Weight layouts from the FP32 master W[t][ci][co]:
+//   Wt[co][t*Cx + ci]       forward (zero for ci >= Cin and k >= taps*Cx)
+//   Wb[ci][t'*Co + co] = W[taps-1-t'][ci][co]   backward data (flipped taps)
+//   Wh[ci][co]              1x1 head
+__global__ void k_pack_fwd(const float *W, int taps, int Cin, int Cx, int Co, int Kp, u16 *Wt) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)Co * Kp) return;
+  const int co = (int)(e / Kp), k = (int)(e % Kp);
+  const int t = k / Cx, ci = k - t * Cx;
+  Wt[e] = (t < taps && ci < Cin) ? f2bf(W[((size_t)t * Cin + ci) * Co + co]) : (u16)0;
+}
+__global__ void k_pack_bwd(const float *W, int taps, int Cin, int Co, int Kpb, u16 *Wb) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)Cin * Kpb) return;
+  const int ci = (int)(e / Kpb), k = (int)(e % Kpb);
+  const int t = k / Co, co = k - t * Co;
+  Wb[e] = (t < taps) ? f2bf(W[((size_t)(taps - 1 - t) * Cin + ci) * Co + co]) : (u16)0;
+}
+__global__ void k_pack_head(const float *W, int n, u16 *Wh) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) Wh[e] = f2bf(W[e]);
+}
+
+// FP32 NHWC [n][C] -> BF16 with channel stride Cx (zero padding channels).
+__global__ void k_to_bf16(const float *X, int C, int Cx, size_t npix, u16 *Y) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= npix * Cx) return;
+  const size_t p = e / Cx;
+  const int c = (int)(e % Cx);
+  Y[e] = (c < C) ? f2bf(X[p * C + c]) : (u16)0;
+}
+
+// Solver state -> network input [H][W][Cx] (H = Ny rows j, W = Nx columns i):
+// (u/u_ref, v/u_ref, p/p_ref - 1, T - 1, fluid mask), zero padding channels.
+__global__ void k_pack_state(const double *u, const double *v, const double *p, const double *T, const uint8_t *m,
+                             int npix, int Cx, double inv_u, double inv_p, u16 *X) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= npix) return;
+  u16 *o = X + (size_t)q * Cx;
+  o[0] = f2bf((float)(u[q] * inv_u));
+  o[1] = f2bf((float)(v[q] * inv_u));
+  o[2] = f2bf((float)(p[q] * inv_p - 1.0));
+  o[3] = f2bf((float)(T[q] - 1.0));
+  o[4] = m[q] ? f2bf(1.f) : (u16)0;
+  for (int c = 5; c < Cx; c++) o[c] = 0;
+}
+
+// Training target (dp / (p_ref dp_scale), dT / dT_scale) and mask.
+__global__ void k_pack_target(const double *p, const double *p_pr, const double *T, const double *T_pr,
+                              const uint8_t *m, int npix, double sp, double sT, float *Tg, uint8_t *M) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= npix) return;
+  Tg[2 * (size_t)q] = (float)((p[q] - p_pr[q]) * sp);
+  Tg[2 * (size_t)q + 1] = (float)((T[q] - T_pr[q]) * sT);
+  M[q] = m[q];
+}
+
+// Predicted guess: p' = p + y0 * (p_ref dp_scale), T' = T + y1 * dT_scale on
+// fluid cells, copies elsewhere.
+__global__ void k_apply_pred(const float *Y, const double *p, const double *T, const uint8_t *m, int npix, double sp,
+                             double sT, double *pw, double *Tw) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= npix) return;
+  pw[q] = m[q] ? p[q] + (double)Y[2 * (size_t)q] * sp : p[q];
+  Tw[q] = m[q] ? T[q] + (double)Y[2 * (size_t)q + 1] * sT : T[q];
+}
+
+// Gathers dataset samples into a batch.
+__global__ void k_gather(const u16 *Xs, const float *Ts, const uint8_t *Ms, const int *slots, int n, int npix, int Cx,
+                         int Co, u16 *X, float *Tg, uint8_t *M) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)n * npix) return;
+  const int b = (int)(e / npix), q = (int)(e % npix);
+  const size_t src = (size_t)slots[b] * npix + q;
+  for (int c = 0; c < Cx; c++) X[e * Cx + c] = Xs[src * Cx + c];
+  for (int c = 0; c < Co; c++) Tg[e * Co + c] = Ts[src * Co + c];
+  M[e] = Ms[src];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Launch wrappers
+// ---------------------------------------------------------------------------
+static inline unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+hipError_t unet_conv(const ConvArgs &a, hipStream_t s) {
+  const size_t npix = (size_t)a.N * a.H * a.W;
+  hipLaunchKernelGGL(k_conv, dim3(nblk(npix, 128), nblk(a.Cout, 32)), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t unet_wgrad(const WgradArgs &a, hipStream_t s) {
+  const size_t npix = (size_t)a.N * a.H * a.W;
+  const int Kp = ((a.ks * a.ks * a.Cx + 31) / 32) * 32;
+  hipLaunchKernelGGL(k_wgrad, dim3(nblk(Kp, 64), nblk(a.Cout, 16), nblk(npix, a.chunk)), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t unet_bgrad(const uint16_t *dZ, int npix, int Cout, float *db, hipStream_t s) {
+  const int chunk = 1024;
+  hipLaunchKernelGGL(k_bgrad, dim3(nblk(npix, chunk)), dim3(256), 0, s, dZ, npix, Cout, chunk, db);
+  return hipGetLastError();
+}
+hipError_t unet_pool(const uint16_t *X, int N, int H, int W, int C, uint16_t *Y, hipStream_t s) {
+  hipLaunchKernelGGL(k_pool, dim3(nblk((size_t)N * (H / 2) * (W / 2) * C, 256)), dim3(256), 0, s, X, N, H, W, C, Y);
+  return hipGetLastError();
+}
+hipError_t unet_upcat(const uint16_t *B, int Cb, const uint16_t *S, int Cs, int N, int H, int W, uint16_t *O,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(k_upcat, dim3(nblk((size_t)N * H * W * (Cb + Cs), 256)), dim3(256), 0, s, B, Cb, S, Cs, N, H, W,
+                     O);
+  return hipGetLastError();
+}
+hipError_t unet_relu_mask(const uint16_t *G, const uint16_t *Act, size_t n, uint16_t *dZ, hipStream_t s) {
+  hipLaunchKernelGGL(k_relu_mask, dim3(nblk(n, 256)), dim3(256), 0, s, G, Act, n, dZ);
+  return hipGetLastError();
+}
+hipError_t unet_up_back(const uint16_t *D, int Cd, int Cu, int N, int H, int W, uint16_t *O, hipStream_t s) {
+  hipLaunchKernelGGL(k_up_back, dim3(nblk((size_t)N * (H / 2) * (W / 2) * Cu, 256)), dim3(256), 0, s, D, Cd, Cu, N, H,
+                     W, O);
+  return hipGetLastError();
+}
+hipError_t unet_pool_back_add(const uint16_t *P, const uint16_t *Skip, int Cs, int off, int N, int H, int W, int C,
+                              uint16_t *O, hipStream_t s) {
+  hipLaunchKernelGGL(k_pool_back_add, dim3(nblk((size_t)N * H * W * C, 256)), dim3(256), 0, s, P, Skip, Cs, off, N, H,
+                     W, C, O);
+  return hipGetLastError();
+}
+hipError_t unet_head(const uint16_t *X, int C, const uint16_t *Wh, const float *b, int Co, int npix, float *Y,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(k_head, dim3(nblk(npix, 256)), dim3(256), 0, s, X, C, Wh, b, Co, npix, Y);
+  return hipGetLastError();
+}
+hipError_t unet_head_back(const float *dY, const uint16_t *X, int C, const uint16_t *Wh, int Co, int npix,
+                          uint16_t *dX, float *dW, float *db, hipStream_t s) {
+  hipLaunchKernelGGL(k_head_back, dim3(nblk(npix, 256)), dim3(256), 0, s, dY, X, C, Wh, Co, npix, dX, dW, db);
+  return hipGetLastError();
+}
+hipError_t unet_mse(const float *Y, const float *T, const uint8_t *M, int Co, int npix, float inv_count, float *dY,
+                    double *loss, hipStream_t s) {
+  hipLaunchKernelGGL(k_mse, dim3(nblk(npix, 256)), dim3(256), 0, s, Y, T, M, Co, npix, inv_count, dY, loss);
+  return hipGetLastError();
+}
+hipError_t unet_adam(float *P, const float *G, float *m, float *v, size_t n, float lr, float b1, float b2, float eps,
+                     float c1, float c2, hipStream_t s) {
+  hipLaunchKernelGGL(k_adam, dim3(nblk(n, 256)), dim3(256), 0, s, P, G, m, v, n, lr, b1, b2, eps, c1, c2);
+  return hipGetLastError();
+}
+hipError_t unet_pack_fwd(const float *W, int taps, int Cin, int Cx, int Co, int Kp, uint16_t *Wt, hipStream_t s) {
+  hipLaunchKernelGGL(k_pack_fwd, dim3(nblk((size_t)Co * Kp, 256)), dim3(256), 0, s, W, taps, Cin, Cx, Co, Kp, Wt);
+  return hipGetLastError();
+}
+hipError_t unet_pack_bwd(const float *W, int taps, int Cin, int Co, int Kpb, uint16_t *Wb, hipStream_t s) {
+  hipLaunchKernelGGL(k_pack_bwd, dim3(nblk((size_t)Cin * Kpb, 256)), dim3(256), 0, s, W, taps, Cin, Co, Kpb, Wb);
+  return hipGetLastError();
+}
+hipError_t unet_pack_head(const float *W, int n, uint16_t *Wh, hipStream_t s) {
+  hipLaunchKernelGGL(k_pack_head, dim3(nblk(n, 256)), dim3(256), 0, s, W, n, Wh);
+  return hipGetLastError();
+}
+hipError_t unet_to_bf16(const float *X, int C, int Cx, size_t npix, uint16_t *Y, hipStream_t s) {
+  hipLaunchKernelGGL(k_to_bf16, dim3(nblk(npix * Cx, 256)), dim3(256), 0, s, X, C, Cx, npix, Y);
+  return hipGetLastError();
+}
+hipError_t unet_pack_state(const double *u, const double *v, const double *p, const double *T, const uint8_t *m,
+                           int npix, int Cx, double inv_u, double inv_p, uint16_t *X, hipStream_t s) {
+  hipLaunchKernelGGL(k_pack_state, dim3(nblk(npix, 256)), dim3(256), 0, s, u, v, p, T, m, npix, Cx, inv_u, inv_p, X);
+  return hipGetLastError();
+}
+hipError_t unet_pack_target(const double *p, const double *p_pr, const double *T, const double *T_pr,
+                            const uint8_t *m, int npix, double sp, double sT, float *Tg, uint8_t *M, hipStream_t s) {
+  hipLaunchKernelGGL(k_pack_target, dim3(nblk(npix, 256)), dim3(256), 0, s, p, p_pr, T, T_pr, m, npix, sp, sT, Tg, M);
+  return hipGetLastError();
+}
+hipError_t unet_apply_pred(const float *Y, const double *p, const double *T, const uint8_t *m, int npix, double sp,
+                           double sT, double *pw, double *Tw, hipStream_t s) {
+  hipLaunchKernelGGL(k_apply_pred, dim3(nblk(npix, 256)), dim3(256), 0, s, Y, p, T, m, npix, sp, sT, pw, Tw);
+  return hipGetLastError();
+}
+hipError_t unet_gather(const uint16_t *Xs, const float *Ts, const uint8_t *Ms, const int *slots, int n, int npix,
+                       int Cx, int Co, uint16_t *X, float *Tg, uint8_t *M, hipStream_t s) {
+  hipLaunchKernelGGL(k_gather, dim3(nblk((size_t)n * npix, 256)), dim3(256), 0, s, Xs, Ts, Ms, slots, n, npix, Cx, Co,
+                     X, Tg, M);
+  return hipGetLastError();
+}

@@ -7,7 +7,8 @@ import re
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADERS = {"simple_ts.h": "libsimple_ts_hip.so", "simple_ts_host.h": "libsimple_ts_host.so"}
+HEADERS = {"simple_ts.h": "libsimple_ts_hip.so", "simple_ts_host.h": "libsimple_ts_host.so",
+           "sts_unet.h": "libsimple_ts_hip.so"}
 
 
 def declared(header):
