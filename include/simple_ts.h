@@ -204,6 +204,27 @@ typedef struct sts_mixed_control {
 int sts_time_step_mixed(sts_ctx *ctx, const sts_loop_control *lc, const sts_mixed_control *mc, int *iters32,
                         int *iters64, double *res_log, double *p_probe);
 
+/* Batched time steps (no counterpart in the reference).  The cases of n
+   contexts (any grid sizes) advance one time step together: every kernel of
+   loop 2 is launched once for all of them over a list of tiles of every
+   case, each workgroup following its own case's loop-2 control, so a case
+   that has converged drops out while the others iterate.  Each case's
+   arithmetic is that of sts_time_step / sts_time_step_mixed (mc may be
+   NULL), bit for bit.  All contexts need the same N_I_p_c; their own streams
+   must be idle.  iters32/iters64 (may be NULL) get n counts each. */
+typedef struct sts_batch sts_batch;
+int sts_batch_create(sts_ctx *const *ctxs, int n, sts_batch **out);
+int sts_batch_destroy(sts_batch *b);
+int sts_batch_time_step(sts_batch *b, const sts_loop_control *lc, const sts_mixed_control *mc, int *iters32,
+                        int *iters64);
+/* Tile shapes of the batched launches: variant v of the cell kernels is
+   64x4, 64x8, 128x2, 32x8 cells, of the fused sweep pass 64x8, 64x4, 128x4,
+   64x16 (v = 0..3).  sts_batch_autotune times each variant `reps` times on
+   the current state, keeps the fastest and reports the mean milliseconds per
+   variant in ms[8] (cell kernels, then sweeps; may be NULL). */
+int sts_batch_set_tiles(sts_batch *b, int cell_variant, int sweep_variant);
+int sts_batch_autotune(sts_batch *b, int reps, int *cell_variant, int *sweep_variant, double *ms);
+
 /* The HIP stream all work of the context is issued on (diagnostics). */
 int sts_stream_info(sts_ctx *ctx, void **hip_stream);
 

@@ -167,6 +167,10 @@ def lib():
             ("sts_time_step_mixed", [vp, ctypes.POINTER(LoopControl), ctypes.POINTER(MixedControl),
                                      ctypes.POINTER(c_int), ctypes.POINTER(c_int), _DP, _DP]),
             ("sts_time_kernel", [vp, c_int]),
+            ("sts_batch_create", [ctypes.POINTER(vp), c_int, ctypes.POINTER(vp)]), ("sts_batch_destroy", [vp]),
+            ("sts_batch_time_step", [vp, ctypes.POINTER(LoopControl), ctypes.POINTER(MixedControl), _IP, _IP]),
+            ("sts_batch_set_tiles", [vp, c_int, c_int]),
+            ("sts_batch_autotune", [vp, c_int, _IP, _IP, _DP]),
             ("sts_unet_create", [c_int, c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(vp)]),
             ("sts_unet_destroy", [vp]), ("sts_unet_set_params", [vp, _FP]), ("sts_unet_get_params", [vp, _FP]),
             ("sts_unet_get_grads", [vp, _FP]), ("sts_unet_forward", [vp, c_int, _FP, _FP]),
@@ -331,6 +335,54 @@ class Context:
         r, r0, pr = np.zeros(4), np.zeros(2), np.zeros(1)
         _check(lib().sts_outer_iteration(self.h, n_pc, _dp(r), _dp(r0), _dp(pr)), "sts_outer_iteration")
         return r, r0, float(pr[0])
+
+
+class Batch:
+    """Several contexts stepped together in shared launches (sts_batch)."""
+
+    NVARIANT = 4
+
+    def __init__(self, contexts):
+        self.contexts = list(contexts)
+        arr = (ctypes.c_void_p * len(self.contexts))(*[c.h.value for c in self.contexts])
+        h = ctypes.c_void_p()
+        _check(lib().sts_batch_create(arr, len(self.contexts), ctypes.byref(h)), "sts_batch_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sts_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def time_step(self, N_I, N_I_p_c, tol_uv, tol_p, tol_T, mixed=None, batch=0):
+        """One time step of every case; mixed = None or (N_I32, tol32,
+        stall_w, stall_ratio).  Returns (FP32 counts, FP64 counts)."""
+        n = len(self.contexts)
+        lc = LoopControl(N_I, N_I_p_c, tol_uv, tol_p, tol_T, batch)
+        mc = None
+        if mixed is not None:
+            n_i32, tol32, w, r = mixed
+            mc = MixedControl(n_i32, tol32[0], tol32[1], tol32[2], w, r)
+        a, b = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        _check(lib().sts_batch_time_step(self.h, ctypes.byref(lc), ctypes.byref(mc) if mc is not None else None,
+                                         a.ctypes.data_as(_IP), b.ctypes.data_as(_IP)), "sts_batch_time_step")
+        return a.tolist(), b.tolist()
+
+    def set_tiles(self, cell_variant, sweep_variant):
+        _check(lib().sts_batch_set_tiles(self.h, cell_variant, sweep_variant), "sts_batch_set_tiles")
+
+    def autotune(self, reps=5):
+        cv, sv = np.zeros(1, np.int32), np.zeros(1, np.int32)
+        ms = np.zeros(2 * self.NVARIANT)
+        _check(lib().sts_batch_autotune(self.h, reps, cv.ctypes.data_as(_IP), sv.ctypes.data_as(_IP), _dp(ms)),
+               "sts_batch_autotune")
+        return int(cv[0]), int(sv[0]), ms
 
 
 # ---------------------------------------------------------------------------

@@ -872,3 +872,293 @@ extern "C" int sts_time_step_ws(sts_ctx *c, sts_unet *net, const sts_unet_norm *
   int rc2 = finish_step(c, n32 + n64, res_log, p_probe);
   return rc ? rc : rc2;
 }
+
+// ---------------------------------------------------------------------------
+// Batched time steps (include/simple_ts.h, sts_batch_*)
+// ---------------------------------------------------------------------------
+struct sts_batch {
+  std::vector<sts_ctx *> cs;
+  hipStream_t stream = nullptr;
+  BatchCase<double> *C64 = nullptr;
+  BatchCase<float> *C32 = nullptr;
+  LoopCtrl *ctrl = nullptr, *h_ctrl = nullptr;
+  unsigned long long *lred = nullptr, *log = nullptr;
+  int log_cap = 0;
+  BatchTile *cell_t[NVARIANT] = {}, *sweep_t[NVARIANT] = {};
+  int n_cell[NVARIANT] = {}, n_sweep[NVARIANT] = {};
+  int cell_v = 0, sweep_v = 0;
+  int last_iters32 = 0, last_iters64 = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+extern "C" int sts_batch_destroy(sts_batch *b) {
+  if (!b) return STS_OK;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (void *p : {(void *)b->C64, (void *)b->C32, (void *)b->ctrl, (void *)b->lred, (void *)b->log}) (void)hipFree(p);
+  if (b->h_ctrl) (void)hipHostFree(b->h_ctrl);
+  for (int v = 0; v < NVARIANT; v++) {
+    (void)hipFree(b->cell_t[v]);
+    (void)hipFree(b->sweep_t[v]);
+  }
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return STS_OK;
+}
+
+static int build_tiles(sts_batch *b, const int (*shape)[2], BatchTile **out, int *count) {
+  for (int v = 0; v < NVARIANT; v++) {
+    std::vector<BatchTile> t;
+    const int TX = shape[v][0], TY = shape[v][1];
+    for (int c = 0; c < (int)b->cs.size(); c++) {
+      const int nx = (b->cs[c]->Nx + TX - 1) / TX, ny = (b->cs[c]->Ny + TY - 1) / TY;
+      for (int y = 0; y < ny; y++)
+        for (int x = 0; x < nx; x++) t.push_back(BatchTile{c, x, y});
+    }
+    HIP_OK(hipMalloc(&out[v], sizeof(BatchTile) * (t.empty() ? 1 : t.size())));
+    HIP_OK(hipMemcpyAsync(out[v], t.data(), sizeof(BatchTile) * t.size(), hipMemcpyHostToDevice, b->stream));
+    count[v] = (int)t.size();
+  }
+  return STS_OK;
+}
+
+extern "C" int sts_batch_create(sts_ctx *const *ctxs, int n, sts_batch **out) {
+  if (!ctxs || !out || n < 1) return fail(STS_ERR_ARG, "need at least one context");
+  *out = nullptr;
+  sts_batch *b = new sts_batch();
+  for (int i = 0; i < n; i++) {
+    if (!ctxs[i]) {
+      sts_batch_destroy(b);
+      return fail(STS_ERR_ARG, "null context in batch");
+    }
+    b->cs.push_back(ctxs[i]);
+  }
+  auto bail = [&](int rc) {
+    sts_batch_destroy(b);
+    return rc;
+  };
+#define B_OK(expr)                                                                                            \
+  do {                                                                                                        \
+    hipError_t e_ = (expr);                                                                                   \
+    if (e_ != hipSuccess) return bail(fail(STS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_))); \
+  } while (0)
+  B_OK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  B_OK(hipMalloc(&b->C64, sizeof(BatchCase<double>) * n));
+  B_OK(hipMalloc(&b->C32, sizeof(BatchCase<float>) * n));
+  B_OK(hipMalloc(&b->ctrl, sizeof(LoopCtrl) * n));
+  B_OK(hipHostMalloc(&b->h_ctrl, sizeof(LoopCtrl) * n, hipHostMallocDefault));
+  B_OK(hipMalloc(&b->lred, sizeof(unsigned long long) * NSLOT * SLOT_STRIDE * (size_t)n));
+  B_OK(hipEventCreate(&b->ev0));
+  B_OK(hipEventCreate(&b->ev1));
+#undef B_OK
+  int rc;
+  if ((rc = build_tiles(b, CELL_TILES, b->cell_t, b->n_cell)) || (rc = build_tiles(b, SWEEP_TILES, b->sweep_t, b->n_sweep)))
+    return bail(rc);
+  if (hipStreamSynchronize(b->stream) != hipSuccess) return bail(fail(STS_ERR_HIP, "stream sync"));
+  *out = b;
+  return STS_OK;
+}
+
+extern "C" int sts_batch_set_tiles(sts_batch *b, int cell_variant, int sweep_variant) {
+  if (!b || cell_variant < 0 || cell_variant >= NVARIANT || sweep_variant < 0 || sweep_variant >= NVARIANT)
+    return fail(STS_ERR_ARG, "bad tile variant");
+  b->cell_v = cell_variant;
+  b->sweep_v = sweep_variant;
+  return STS_OK;
+}
+
+// Case table of one precision at the current p/T pointers of every case.
+template <typename R>
+static int upload_cases(sts_batch *b, BatchCase<R> *dst, int N_I, double tol_uv, double tol_p, double tol_T,
+                        int stall_w, double stall_ratio, size_t log_off, bool gated) {
+  std::vector<BatchCase<R>> h(b->cs.size());
+  for (size_t i = 0; i < b->cs.size(); i++) {
+    sts_ctx *c = b->cs[i];
+    BatchCase<R> &x = h[i];
+    if constexpr (sizeof(R) == sizeof(double)) {
+      x.g = c->g;
+      x.f = c->fields();
+      x.p_alt = c->p_alt;
+      x.T_alt = c->T_alt;
+    } else {
+      x.g = c->g32;
+      x.f = c->fields32();
+      x.p_alt = c->p_alt32;
+      x.T_alt = c->T_alt32;
+    }
+    x.ctrl = gated ? b->ctrl + i : nullptr;
+    x.lred = b->lred + i * (size_t)NSLOT * SLOT_STRIDE;
+    x.log = b->log ? b->log + i * 6 * (size_t)b->log_cap + log_off : nullptr;
+    x.probe = c->probe;
+    x.N_I = N_I;
+    x.stall_w = stall_w;
+    x.tol_uv = tol_uv;
+    x.tol_p = tol_p;
+    x.tol_T = tol_T;
+    x.stall_ratio = stall_ratio;
+  }
+  HIP_OK(hipMemcpyAsync(dst, h.data(), sizeof(BatchCase<R>) * h.size(), hipMemcpyHostToDevice, b->stream));
+  return STS_OK;
+}
+
+// Loop 2 of every case in shared launches; iters[i] gets case i's count.
+template <typename R>
+static int run_batch_loop2(sts_batch *b, BatchCase<R> *C, int n_pc, int N_I, int batch, std::vector<int> &iters) {
+  const int n = (int)b->cs.size(), passes = sweep_passes(b->cs[0], n_pc);
+  hipStream_t s = b->stream;
+  const int cv = b->cell_v, sv = b->sweep_v;
+  BatchTile *ct = b->cell_t[cv], *st = b->sweep_t[sv];
+  const int nct = b->n_cell[cv], nst = b->n_sweep[sv];
+  HIP_OK(launch_batch<R>(BK_LOOP_OPEN, 0, C, n, nullptr, 0, 0, 0, 0, 0, s));
+  int k = 0;
+  while (true) {
+    const int end = std::min(N_I, k + batch);
+    for (; k < end; k++) {
+      int par = (int)(((long long)k * passes) & 1);
+      HIP_OK(launch_batch<R>(BK_ASSEMBLE, cv, C, n, ct, nct, k, par, 0, 0, s));
+      for (int q = 0; q < n_pc;) {
+        if (b->cs[0]->fuse_sweeps && q + 1 < n_pc) {
+          HIP_OK(launch_batch<R>(BK_SWEEP2, sv, C, n, st, nst, k, par, q == 0, q + 1 == n_pc - 1, s));
+          q += 2;
+        } else {
+          HIP_OK(launch_batch<R>(BK_SWEEP, cv, C, n, ct, nct, k, par, q == 0, q == n_pc - 1, s));
+          q += 1;
+        }
+        par ^= 1;
+      }
+      HIP_OK(launch_batch<R>(BK_CORRECT, cv, C, n, ct, nct, k, par, 0, 0, s));
+      HIP_OK(launch_batch<R>(BK_WALL, cv, C, n, ct, nct, k, par, STS_BC_SLIP_U | STS_BC_SLIP_V | STS_BC_JUMP, 0, s));
+      HIP_OK(launch_batch<R>(BK_EDGES, 0, C, n, nullptr, 0, k, par,
+                             STS_BC_VELOCITIES | STS_BC_TEMPERATURE | STS_BC_PRESSURE, 0, s));
+      HIP_OK(launch_batch<R>(BK_FLUX_DIFF, cv, C, n, ct, nct, k, par, 0, 0, s));
+    }
+    HIP_OK(hipMemcpyAsync(b->h_ctrl, b->ctrl, sizeof(LoopCtrl) * n, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    bool open = false;
+    for (int i = 0; i < n; i++) open = open || b->h_ctrl[i].stop_at == 0x7fffffff;
+    if (!open) break;
+    if (k >= N_I) return fail(STS_ERR_ARG, "loop 2 did not stop at N_I");
+    batch = 2;
+  }
+  iters.assign(n, 0);
+  for (int i = 0; i < n; i++) {
+    iters[i] = b->h_ctrl[i].stop_at + 1;
+    sts_ctx *c = b->cs[i];
+    if (((long long)iters[i] * passes) & 1) {
+      if constexpr (sizeof(R) == sizeof(double)) {
+        std::swap(c->f[STS_P], c->p_alt);
+        std::swap(c->f[STS_T], c->T_alt);
+      } else {
+        std::swap(c->f32[STS_P], c->p_alt32);
+        std::swap(c->f32[STS_T], c->T_alt32);
+      }
+    }
+  }
+  for (int i = 0; i < n; i++)
+    if (b->h_ctrl[i].nonfinite)
+      return fail(STS_ERR_NONFINITE, "non-finite residual in case " + std::to_string(i) + ", outer iteration " +
+                                         std::to_string(iters[i]));
+  return STS_OK;
+}
+
+extern "C" int sts_batch_time_step(sts_batch *b, const sts_loop_control *lc, const sts_mixed_control *mc,
+                                   int *iters32, int *iters64) {
+  if (!b) return fail(STS_ERR_ARG, "null batch");
+  int rc;
+  const int n = (int)b->cs.size();
+  for (sts_ctx *c : b->cs) {
+    if ((rc = check_mixed(c, lc, mc))) return rc;
+    if (c->fuse_sweeps != b->cs[0]->fuse_sweeps) return fail(STS_ERR_ARG, "cases differ in sweep fusion");
+    HIP_OK(hipStreamSynchronize(c->stream));
+  }
+  const int cap = lc->N_I + (mc ? mc->N_I32 : 0);
+  if (cap > b->log_cap) {
+    (void)hipFree(b->log);
+    b->log = nullptr;
+    HIP_OK(hipMalloc(&b->log, sizeof(unsigned long long) * 6 * (size_t)cap * n));
+    b->log_cap = cap;
+  }
+  hipStream_t s = b->stream;
+  for (sts_ctx *c : b->cs) {  // prologue S:3233-3239
+    const double *src[5] = {c->f[STS_P], c->f[STS_U], c->f[STS_V], c->f[STS_T], c->f[STS_RHO]};
+    double *dst[5] = {c->f[STS_P_PR], c->f[STS_U_PR], c->f[STS_V_PR], c->f[STS_T_PR], c->f[STS_RHO_PR]};
+    HIP_OK(launch_copy5(src, dst, c->Na, s));
+  }
+  std::vector<int> n32(n, 0), n64(n, 0);
+  if (mc && mc->N_I32 > 0) {
+    const unsigned long long all = (1ull << STS_NFIELDS) - 1ull;
+    for (sts_ctx *c : b->cs) HIP_OK(launch_convert(c->fields(), c->fields32(), all, c->Na, s));
+    if ((rc = upload_cases<float>(b, b->C32, mc->N_I32, mc->tol32_uv, mc->tol32_p, mc->tol32_T, mc->stall_w,
+                                  mc->stall_ratio, 0, true)))
+      return rc;
+    const int b32 = lc->batch > 0 ? lc->batch : (b->last_iters32 > 2 ? b->last_iters32 - 1 : 1);
+    if ((rc = run_batch_loop2<float>(b, b->C32, lc->N_I_p_c, mc->N_I32, b32, n32))) return rc;
+    b->last_iters32 = *std::min_element(n32.begin(), n32.end());
+    for (sts_ctx *c : b->cs) HIP_OK(launch_convert(c->fields32(), c->fields(), WIDEN_MASK, c->Na, s));
+  }
+  // the FP64 logs follow the FP32 ones of each case (offset = the largest FP32 count)
+  const int off = mc ? mc->N_I32 : 0;
+  if ((rc = upload_cases<double>(b, b->C64, lc->N_I, lc->tol_uv, lc->tol_p, lc->tol_T, 0, 0.0, 6 * (size_t)off, true)))
+    return rc;
+  const int b64 = lc->batch > 0 ? lc->batch : (b->last_iters64 > 2 ? b->last_iters64 - 1 : 1);
+  if ((rc = run_batch_loop2<double>(b, b->C64, lc->N_I_p_c, lc->N_I, b64, n64))) return rc;
+  b->last_iters64 = *std::min_element(n64.begin(), n64.end());
+  for (int i = 0; i < n; i++) {
+    if (iters32) iters32[i] = n32[i];
+    if (iters64) iters64[i] = n64[i];
+    b->cs[i]->last_iters = n64[i];
+  }
+  HIP_OK(hipStreamSynchronize(s));
+  return STS_OK;
+}
+
+// Times the bulk kernels of each tile variant on the current state (the
+// timed launches only rewrite what the next outer iteration recomputes before
+// reading it: assembly outputs, fluxes and diffusion coefficients from
+// unchanged inputs, and the p/T partners), then keeps the fastest.
+extern "C" int sts_batch_autotune(sts_batch *b, int reps, int *cell_variant, int *sweep_variant, double *ms) {
+  if (!b || reps < 1) return fail(STS_ERR_ARG, "bad autotune arguments");
+  const int n = (int)b->cs.size();
+  for (sts_ctx *c : b->cs) HIP_OK(hipStreamSynchronize(c->stream));
+  int rc;
+  if ((rc = upload_cases<double>(b, b->C64, 1, 0, 0, 0, 0, 0.0, 0, false))) return rc;
+  hipStream_t s = b->stream;
+  double t_cell[NVARIANT], t_sweep[NVARIANT];
+  for (int v = 0; v < NVARIANT; v++) {
+    for (int pass = 0; pass < 2; pass++) {  // pass 0 warms up
+      HIP_OK(hipEventRecord(b->ev0, s));
+      for (int r = 0; r < reps; r++) {
+        HIP_OK(launch_batch<double>(BK_ASSEMBLE, v, b->C64, n, b->cell_t[v], b->n_cell[v], 0, 0, 0, 0, s));
+        HIP_OK(launch_batch<double>(BK_FLUX_DIFF, v, b->C64, n, b->cell_t[v], b->n_cell[v], 0, 0, 0, 0, s));
+      }
+      HIP_OK(hipEventRecord(b->ev1, s));
+      HIP_OK(hipEventSynchronize(b->ev1));
+      float f = 0;
+      HIP_OK(hipEventElapsedTime(&f, b->ev0, b->ev1));
+      t_cell[v] = f / reps;
+      HIP_OK(hipEventRecord(b->ev0, s));
+      for (int r = 0; r < reps; r++)
+        HIP_OK(launch_batch<double>(BK_SWEEP2, v, b->C64, n, b->sweep_t[v], b->n_sweep[v], 0, 0, 0, 0, s));
+      HIP_OK(hipEventRecord(b->ev1, s));
+      HIP_OK(hipEventSynchronize(b->ev1));
+      HIP_OK(hipEventElapsedTime(&f, b->ev0, b->ev1));
+      t_sweep[v] = f / reps;
+    }
+  }
+  int bc = 0, bs = 0;
+  for (int v = 1; v < NVARIANT; v++) {
+    if (t_cell[v] < t_cell[bc]) bc = v;
+    if (t_sweep[v] < t_sweep[bs]) bs = v;
+  }
+  b->cell_v = bc;
+  b->sweep_v = bs;
+  if (cell_variant) *cell_variant = bc;
+  if (sweep_variant) *sweep_variant = bs;
+  if (ms)
+    for (int v = 0; v < NVARIANT; v++) {
+      ms[v] = t_cell[v];
+      ms[NVARIANT + v] = t_sweep[v];
+    }
+  return STS_OK;
+}

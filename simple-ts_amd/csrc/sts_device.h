@@ -123,3 +123,24 @@ template <typename R>
 __device__ __forceinline__ R Hi(R f1, R f2, R h1, R h2) {  // S:136
   return (h1 + h2) * f1 * f2 / (h1 * f2 + h2 * f1);
 }
+
+// One case of a batch run in shared launches (device-resident): grid and
+// fields with p and T as they stand at the start of the step, the p/T
+// ping-pong partners, loop control, residual words and log, loop-2 test.
+template <typename R>
+struct BatchCase {
+  KGridT<R> g;
+  KFieldsT<R> f;
+  R *p_alt, *T_alt;
+  LoopCtrl *ctrl;
+  unsigned long long *lred;  // slotted residual words
+  unsigned long long *log;   // 6 words per outer iteration
+  double *probe;
+  int N_I, stall_w;
+  double tol_uv, tol_p, tol_T, stall_ratio;
+};
+
+// Work item of a batched launch: tile (x, y) of case c.
+struct BatchTile {
+  int c, x, y;
+};

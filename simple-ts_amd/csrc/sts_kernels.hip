@@ -176,15 +176,19 @@ __device__ __forceinline__ void cell_energy_source(const KGridT<R> &g, const KFi
 }
 
 template <typename R>
-__global__ void __launch_bounds__(BX *BY) k_assemble(KGridT<R> g, KFieldsT<R> fl, Gate gate) {
-  if (gated_out(gate)) return;
-  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+__device__ __forceinline__ void assemble_cell(const KGridT<R> &g, const KFieldsT<R> &fl, int i, int j) {
   if (i >= g.Nx || j >= g.Ny) return;
   const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny);
   const uint8_t *m = g.m;
   if (in_box(i, j, g.Nx_u0, g.Nx_u1, g.Ny_u0, g.Ny_u1) && m[n.i_1j] && m[n.ij]) cell_assemble_u(g, fl, n);
   if (in_box(i, j, g.Nx_v0, g.Nx_v1, g.Ny_v0, g.Ny_v1) && m[n.ij_1] && m[n.ij]) cell_assemble_v(g, fl, n);
   if (in_box(i, j, g.Nx_o0, g.Nx_o1, g.Ny_o0, g.Ny_o1) && m[n.ij]) cell_energy_source(g, fl, n);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(BX *BY) k_assemble(KGridT<R> g, KFieldsT<R> fl, Gate gate) {
+  if (gated_out(gate)) return;
+  assemble_cell(g, fl, (int)(blockIdx.x * BX + threadIdx.x), (int)(blockIdx.y * BY + threadIdx.y));
 }
 
 // ---------------------------------------------------------------------------
@@ -280,19 +284,17 @@ __global__ void __launch_bounds__(BX *BY) k_sweep(KGridT<R> g, KFieldsT<R> fl, c
 constexpr int S2X = 64, S2Y = 8, S2RX = S2X + 2, S2RY = S2Y + 2;
 
 template <typename R>
-__global__ void __launch_bounds__(S2X *S2Y) k_sweep2(KGridT<R> g, KFieldsT<R> fl, const R *__restrict__ Po,
-                                                     const R *__restrict__ To, R *__restrict__ Pn,
-                                                     R *__restrict__ Tn, unsigned long long *r1p,
-                                                     unsigned long long *r1T, unsigned long long *r2p,
-                                                     unsigned long long *r2T, Gate gate) {
-  if (gated_out(gate)) return;
-  __shared__ R sP[S2RY][S2RX], sT[S2RY][S2RX];
+__device__ __forceinline__ void sweep2_tile_body(const KGridT<R> &g, const KFieldsT<R> &fl, const R *__restrict__ Po,
+                                                 const R *__restrict__ To, R *__restrict__ Pn, R *__restrict__ Tn,
+                                                 int i0, int j0, int tx, int ty, int SX, int SY, R *sP, R *sT,
+                                                 unsigned long long *r1p, unsigned long long *r1T,
+                                                 unsigned long long *r2p, unsigned long long *r2T) {
   const int Nx = g.Nx, Ny = g.Ny;
-  const int i0 = blockIdx.x * S2X, j0 = blockIdx.y * S2Y;
-  const int tid = threadIdx.x + threadIdx.y * S2X;
+  const int RX = SX + 2, RY = SY + 2;
+  const int tid = tx + ty * SX;
   unsigned long long r1[2] = {0ull, 0ull};
-  for (int q = tid; q < S2RX * S2RY; q += S2X * S2Y) {
-    const int ry = q / S2RX, rx = q - ry * S2RX;
+  for (int q = tid; q < RX * RY; q += SX * SY) {
+    const int ry = q / RX, rx = q - ry * RX;
     const int i = i0 - 1 + rx, j = j0 - 1 + ry;
     if (i < 0 || j < 0 || i >= Nx || j >= Ny) continue;
     const unsigned ij = (unsigned)i + (unsigned)j * (unsigned)Nx;
@@ -301,7 +303,7 @@ __global__ void __launch_bounds__(S2X *S2Y) k_sweep2(KGridT<R> g, KFieldsT<R> fl
       const Nbr n((unsigned)i, (unsigned)j, (unsigned)Nx, (unsigned)Ny);
       const PTStencil<R> x = stencil_global(Po, To, n);
       pt_update(g, fl, n, x, pn, tn);
-      if (rx >= 1 && rx <= S2X && ry >= 1 && ry <= S2Y) {
+      if (rx >= 1 && rx <= SX && ry >= 1 && ry <= SY) {
         r1[0] = (absbits(pn - x.Pc) > r1[0]) ? absbits(pn - x.Pc) : r1[0];
         r1[1] = (absbits(tn - x.Tc) > r1[1]) ? absbits(tn - x.Tc) : r1[1];
       }
@@ -309,22 +311,23 @@ __global__ void __launch_bounds__(S2X *S2Y) k_sweep2(KGridT<R> g, KFieldsT<R> fl
       pn = Po[ij];
       tn = To[ij];
     }
-    sP[ry][rx] = pn;
-    sT[ry][rx] = tn;
+    sP[ry * RX + rx] = pn;
+    sT[ry * RX + rx] = tn;
   }
   __syncthreads();
   unsigned long long r2[2] = {0ull, 0ull};
-  const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
+  const int i = i0 + tx, j = j0 + ty;
   if (i < Nx && j < Ny) {
     const unsigned ij = (unsigned)i + (unsigned)j * (unsigned)Nx;
-    const int cx = threadIdx.x + 1, cy = threadIdx.y + 1;
+    const int cx = tx + 1, cy = ty + 1;
     if (in_pt_set(g, i, j, ij)) {
       const Nbr n((unsigned)i, (unsigned)j, (unsigned)Nx, (unsigned)Ny);
       // clamped neighbours map to the cell itself at the domain edge
       const int wx = cx - (int)(n.i - n.i_1), ex = cx + (int)(n.i1 - n.i);
       const int sy = cy - (int)((n.ij - n.ij_1) / (unsigned)Nx), ny = cy + (int)((n.ij1 - n.ij) / (unsigned)Nx);
-      const PTStencil<R> x{sP[cy][cx], sT[cy][cx], sP[cy][wx], sT[cy][wx], sP[cy][ex], sT[cy][ex],
-                        sP[sy][cx], sT[sy][cx], sP[ny][cx], sT[ny][cx]};
+      const PTStencil<R> x{sP[cy * RX + cx], sT[cy * RX + cx], sP[cy * RX + wx], sT[cy * RX + wx],
+                           sP[cy * RX + ex], sT[cy * RX + ex], sP[sy * RX + cx], sT[sy * RX + cx],
+                           sP[ny * RX + cx], sT[ny * RX + cx]};
       R pn, tn;
       pt_update(g, fl, n, x, pn, tn);
       Tn[ij] = tn;
@@ -332,12 +335,33 @@ __global__ void __launch_bounds__(S2X *S2Y) k_sweep2(KGridT<R> g, KFieldsT<R> fl
       r2[0] = absbits(pn - x.Pc);
       r2[1] = absbits(tn - x.Tc);
     } else {
-      Pn[ij] = sP[cy][cx];
-      Tn[ij] = sT[cy][cx];
+      Pn[ij] = sP[cy * RX + cx];
+      Tn[ij] = sT[cy * RX + cx];
     }
   }
   if (r1p || r1T) block_max_atomic<2>(r1, r1p, r1T);
   if (r2p || r2T) block_max_atomic<2>(r2, r2p, r2T);
+}
+
+// Tile (i0, j0) of SX x SY cells with its one-cell halo in LDS.
+template <typename R, int SX, int SY>
+__device__ __forceinline__ void sweep2_tile(const KGridT<R> &g, const KFieldsT<R> &fl, const R *__restrict__ Po,
+                                            const R *__restrict__ To, R *__restrict__ Pn, R *__restrict__ Tn, int i0,
+                                            int j0, int tx, int ty, unsigned long long *r1p, unsigned long long *r1T,
+                                            unsigned long long *r2p, unsigned long long *r2T) {
+  __shared__ R sP[(SY + 2) * (SX + 2)], sT[(SY + 2) * (SX + 2)];
+  sweep2_tile_body(g, fl, Po, To, Pn, Tn, i0, j0, tx, ty, SX, SY, sP, sT, r1p, r1T, r2p, r2T);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(S2X *S2Y) k_sweep2(KGridT<R> g, KFieldsT<R> fl, const R *__restrict__ Po,
+                                                     const R *__restrict__ To, R *__restrict__ Pn,
+                                                     R *__restrict__ Tn, unsigned long long *r1p,
+                                                     unsigned long long *r1T, unsigned long long *r2p,
+                                                     unsigned long long *r2T, Gate gate) {
+  if (gated_out(gate)) return;
+  sweep2_tile<R, S2X, S2Y>(g, fl, Po, To, Pn, Tn, (int)blockIdx.x * S2X, (int)blockIdx.y * S2Y, (int)threadIdx.x,
+                           (int)threadIdx.y, r1p, r1T, r2p, r2T);
 }
 
 // ---------------------------------------------------------------------------
@@ -352,11 +376,8 @@ __device__ __forceinline__ R rho_of(const uint8_t *m, const R *p, const R *T, un
 }
 
 template <typename R>
-__global__ void __launch_bounds__(BX *BY) k_correct(KGridT<R> g, KFieldsT<R> fl, unsigned long long *red_u,
-                                                    unsigned long long *red_v, int do_correct, Gate gate) {
-  if (gated_out(gate)) return;
-  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
-  unsigned long long r[2] = {0ull, 0ull};
+__device__ __forceinline__ void correct_cell(const KGridT<R> &g, const KFieldsT<R> &fl, int i, int j, int do_correct,
+                                             unsigned long long (&r)[2]) {
   if (i < g.Nx && j < g.Ny) {
     const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny);
     const uint8_t *m = g.m;
@@ -390,6 +411,14 @@ __global__ void __launch_bounds__(BX *BY) k_correct(KGridT<R> g, KFieldsT<R> fl,
     FLD(RHO_U)[ij] = ru;
     FLD(RHO_V)[ij] = rv;
   }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(BX *BY) k_correct(KGridT<R> g, KFieldsT<R> fl, unsigned long long *red_u,
+                                                    unsigned long long *red_v, int do_correct, Gate gate) {
+  if (gated_out(gate)) return;
+  unsigned long long r[2] = {0ull, 0ull};
+  correct_cell(g, fl, (int)(blockIdx.x * BX + threadIdx.x), (int)(blockIdx.y * BY + threadIdx.y), do_correct, r);
   if (do_correct) block_max_atomic<2>(r, red_u, red_v);
 }
 
@@ -536,9 +565,7 @@ __device__ __forceinline__ R wall_T(const KGridT<R> &g, uint32_t k, R T_fluid, R
 }
 
 template <typename R>
-__global__ void __launch_bounds__(BX *BY) k_wall(KGridT<R> g, KFieldsT<R> fl, int parts, Gate gate) {
-  if (gated_out(gate)) return;
-  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+__device__ __forceinline__ void wall_cell(const KGridT<R> &g, const KFieldsT<R> &fl, int i, int j, int parts) {
   if (i >= g.Nx || j >= g.Ny) return;
   const unsigned Nx = (unsigned)g.Nx, Ny = (unsigned)g.Ny;
   const unsigned k = (unsigned)i + (unsigned)j * Nx;
@@ -588,6 +615,12 @@ __global__ void __launch_bounds__(BX *BY) k_wall(KGridT<R> g, KFieldsT<R> fl, in
       else if (m[s_] && !m[k]) FLD(TW_V)[k] = wall_T(g, g.vol[k], T[s_], rho[s_], R(0.5) * g.hy[j - 1]);
     }
   }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(BX *BY) k_wall(KGridT<R> g, KFieldsT<R> fl, int parts, Gate gate) {
+  if (gated_out(gate)) return;
+  wall_cell(g, fl, (int)(blockIdx.x * BX + threadIdx.x), (int)(blockIdx.y * BY + threadIdx.y), parts);
 }
 
 // ---------------------------------------------------------------------------
@@ -642,9 +675,8 @@ __device__ __forceinline__ void edge_y(const KGridT<R> &g, R *a, int b0, int b1,
 }
 
 template <typename R>
-__global__ void __launch_bounds__(EDGE_THREADS) k_edges(KGridT<R> g, KFieldsT<R> fl, int parts, double *probe, Gate gate,
-                                                        LoopTest lt) {
-  if (gated_out(gate)) return;
+__device__ void edges_block(const KGridT<R> &g, const KFieldsT<R> &fl, int parts, double *probe, const Gate &gate,
+                            const LoopTest &lt) {
   if (lt.ctrl && threadIdx.x < NSLOT) {  // wave 0: fold the slots of iteration gate.k
     unsigned long long *sl = lt.lred + threadIdx.x * SLOT_STRIDE;
 #pragma unroll
@@ -780,14 +812,19 @@ __global__ void __launch_bounds__(EDGE_THREADS) k_edges(KGridT<R> g, KFieldsT<R>
   }
 }
 
+template <typename R>
+__global__ void __launch_bounds__(EDGE_THREADS) k_edges(KGridT<R> g, KFieldsT<R> fl, int parts, double *probe,
+                                                        Gate gate, LoopTest lt) {
+  if (gated_out(gate)) return;
+  edges_block(g, fl, parts, probe, gate, lt);
+}
+
 // ---------------------------------------------------------------------------
 // A10 + A11: fluxes (S:10440-10454) and diffusion coefficients
 // (S:10457-10618) over all cells.
 // ---------------------------------------------------------------------------
 template <typename R>
-__global__ void __launch_bounds__(BX *BY) k_flux_diff(KGridT<R> g, KFieldsT<R> fl, Gate gate) {
-  if (gated_out(gate)) return;
-  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+__device__ __forceinline__ void flux_diff_cell(const KGridT<R> &g, const KFieldsT<R> &fl, int i, int j) {
   if (i >= g.Nx || j >= g.Ny) return;
   const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny);
   const uint8_t *m = g.m;
@@ -831,6 +868,12 @@ __global__ void __launch_bounds__(BX *BY) k_flux_diff(KGridT<R> g, KFieldsT<R> f
                                     * (R((!m11) || (!m10)) * sqrt(Twv[ij])
                                        + R(m11 * m10) * Hi(sT[ij_1], sT[ij], hy[jm], hy[j]))
                               : R(0.0);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(BX *BY) k_flux_diff(KGridT<R> g, KFieldsT<R> fl, Gate gate) {
+  if (gated_out(gate)) return;
+  flux_diff_cell(g, fl, (int)(blockIdx.x * BX + threadIdx.x), (int)(blockIdx.y * BY + threadIdx.y));
 }
 
 // ---------------------------------------------------------------------------
@@ -982,6 +1025,135 @@ __global__ void __launch_bounds__(BX *BY) k_continuity(KGrid g, KFields fl, doub
   }
 }
 
+// ---------------------------------------------------------------------------
+// Batched loop-2 kernels: one launch covers the tiles of every case of a
+// batch (BatchTile list); each workgroup takes its case's grid and fields
+// from the BatchCase table and obeys that case's loop control.  The per-cell
+// work is the same code as the single-case kernels above.  par = 1 selects
+// the p/T partners as the current p, T.
+// ---------------------------------------------------------------------------
+template <typename R>
+__device__ __forceinline__ KFieldsT<R> fields_at(const BatchCase<R> &c, int par) {
+  KFieldsT<R> f = c.f;
+  if (par) {
+    f.f[STS_P] = c.p_alt;
+    f.f[STS_T] = c.T_alt;
+  }
+  return f;
+}
+
+template <typename R, int TX, int TY>
+__global__ void __launch_bounds__(TX *TY) kb_assemble(const BatchCase<R> *C, const BatchTile *T, int k, int par) {
+  const BatchTile t = T[blockIdx.x];
+  const BatchCase<R> &c = C[t.c];
+  if (gated_out(Gate{c.ctrl, k})) return;
+  const KFieldsT<R> f = fields_at(c, par);
+  assemble_cell(c.g, f, t.x * TX + (int)threadIdx.x % TX, t.y * TY + (int)threadIdx.x / TX);
+}
+
+// Pass of two fused sweeps from parity par; first/last: whether it holds
+// loop 3's first / last sweep (their residuals go to the loop-2 test).
+template <typename R, int SX, int SY>
+__global__ void __launch_bounds__(SX *SY) kb_sweep2(const BatchCase<R> *C, const BatchTile *T, int k, int par,
+                                                    int first, int last) {
+  const BatchTile t = T[blockIdx.x];
+  const BatchCase<R> &c = C[t.c];
+  if (gated_out(Gate{c.ctrl, k})) return;
+  const KFieldsT<R> f = fields_at(c, par);
+  const R *Pi = par ? c.p_alt : c.f.f[STS_P], *Ti = par ? c.T_alt : c.f.f[STS_T];
+  R *Po = par ? c.f.f[STS_P] : c.p_alt, *To = par ? c.f.f[STS_T] : c.T_alt;
+  sweep2_tile<R, SX, SY>(c.g, f, Pi, Ti, Po, To, t.x * SX, t.y * SY, (int)threadIdx.x % SX, (int)threadIdx.x / SX,
+                         first ? c.lred + 4 : nullptr, first ? c.lred + 5 : nullptr, last ? c.lred + 2 : nullptr,
+                         last ? c.lred + 3 : nullptr);
+}
+
+// Single sweep (odd sweep counts).
+template <typename R, int TX, int TY>
+__global__ void __launch_bounds__(TX *TY) kb_sweep(const BatchCase<R> *C, const BatchTile *T, int k, int par,
+                                                   int first, int last) {
+  const BatchTile t = T[blockIdx.x];
+  const BatchCase<R> &c = C[t.c];
+  if (gated_out(Gate{c.ctrl, k})) return;
+  const KFieldsT<R> fl = fields_at(c, par);
+  const KGridT<R> &g = c.g;
+  const R *Po = par ? c.p_alt : c.f.f[STS_P], *To = par ? c.T_alt : c.f.f[STS_T];
+  R *Pn = par ? c.f.f[STS_P] : c.p_alt, *Tn = par ? c.f.f[STS_T] : c.T_alt;
+  const int i = t.x * TX + (int)threadIdx.x % TX, j = t.y * TY + (int)threadIdx.x / TX;
+  unsigned long long r[2] = {0ull, 0ull};
+  if (i < g.Nx && j < g.Ny) {
+    const unsigned ij = (unsigned)i + (unsigned)j * (unsigned)g.Nx;
+    if (in_pt_set(g, i, j, ij)) {
+      const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny);
+      const PTStencil<R> x = stencil_global(Po, To, n);
+      R pn, tn;
+      pt_update(g, fl, n, x, pn, tn);
+      Tn[ij] = tn;
+      Pn[ij] = pn;
+      r[0] = absbits(pn - x.Pc);
+      r[1] = absbits(tn - x.Tc);
+    } else {
+      Pn[ij] = Po[ij];
+      Tn[ij] = To[ij];
+    }
+  }
+  block_max_atomic<2>(r, last ? c.lred + 2 : nullptr, last ? c.lred + 3 : nullptr, first ? c.lred + 4 : nullptr,
+                      first ? c.lred + 5 : nullptr);
+}
+
+template <typename R, int TX, int TY>
+__global__ void __launch_bounds__(TX *TY) kb_correct(const BatchCase<R> *C, const BatchTile *T, int k, int par) {
+  const BatchTile t = T[blockIdx.x];
+  const BatchCase<R> &c = C[t.c];
+  if (gated_out(Gate{c.ctrl, k})) return;
+  const KFieldsT<R> f = fields_at(c, par);
+  unsigned long long r[2] = {0ull, 0ull};
+  correct_cell(c.g, f, t.x * TX + (int)threadIdx.x % TX, t.y * TY + (int)threadIdx.x / TX, 1, r);
+  block_max_atomic<2>(r, c.lred + 0, c.lred + 1);
+}
+
+template <typename R, int TX, int TY>
+__global__ void __launch_bounds__(TX *TY) kb_wall(const BatchCase<R> *C, const BatchTile *T, int k, int par,
+                                                  int parts) {
+  const BatchTile t = T[blockIdx.x];
+  const BatchCase<R> &c = C[t.c];
+  if (gated_out(Gate{c.ctrl, k})) return;
+  const KFieldsT<R> f = fields_at(c, par);
+  wall_cell(c.g, f, t.x * TX + (int)threadIdx.x % TX, t.y * TY + (int)threadIdx.x / TX, parts);
+}
+
+template <typename R, int TX, int TY>
+__global__ void __launch_bounds__(TX *TY) kb_flux_diff(const BatchCase<R> *C, const BatchTile *T, int k, int par) {
+  const BatchTile t = T[blockIdx.x];
+  const BatchCase<R> &c = C[t.c];
+  if (gated_out(Gate{c.ctrl, k})) return;
+  const KFieldsT<R> f = fields_at(c, par);
+  flux_diff_cell(c.g, f, t.x * TX + (int)threadIdx.x % TX, t.y * TY + (int)threadIdx.x / TX);
+}
+
+// Edge conditions and the loop-2 test: one workgroup per case.
+template <typename R>
+__global__ void __launch_bounds__(EDGE_THREADS) kb_edges(const BatchCase<R> *C, int k, int par, int parts) {
+  const BatchCase<R> &c = C[blockIdx.x];
+  const Gate gate{c.ctrl, k};
+  if (gated_out(gate)) return;
+  const KFieldsT<R> f = fields_at(c, par);
+  const LoopTest lt{c.ctrl, c.lred, c.log + 6 * (size_t)k, c.N_I, c.tol_uv, c.tol_p, c.tol_T, c.stall_w,
+                    c.stall_ratio};
+  edges_block(c.g, f, parts, c.probe, gate, lt);
+}
+
+// Opens every case's loop 2: control, log (6 N_I words), slotted words.
+template <typename R>
+__global__ void __launch_bounds__(256) kb_loop_open(const BatchCase<R> *C) {
+  const BatchCase<R> &c = C[blockIdx.x];
+  if (threadIdx.x == 0) {
+    c.ctrl->stop_at = 0x7fffffff;
+    c.ctrl->nonfinite = 0;
+  }
+  for (int q = threadIdx.x; q < 6 * c.N_I; q += blockDim.x) c.log[q] = 0ull;
+  for (int q = threadIdx.x; q < NSLOT * SLOT_STRIDE; q += blockDim.x) c.lred[q] = 0ull;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1112,3 +1284,49 @@ hipError_t launch_continuity(const KGrid &g, const KFields &f, double *cs, doubl
   hipLaunchKernelGGL(k_continuity, grid2d(g), dim3(BX, BY), 0, s, g, f, cs, cf, partial);
   return hipGetLastError();
 }
+
+// ---------------------------------------------------------------------------
+// Batched launches; tile shape variants (the autotuner's choices)
+// ---------------------------------------------------------------------------
+const int CELL_TILES[NVARIANT][2] = {{64, 4}, {64, 8}, {128, 2}, {32, 8}};
+const int SWEEP_TILES[NVARIANT][2] = {{64, 8}, {64, 4}, {128, 4}, {64, 16}};
+
+#define CELL_SWITCH(v, KERNEL, ...)                                                                   \
+  switch (v) {                                                                                        \
+    case 0: hipLaunchKernelGGL((KERNEL<R, 64, 4>), dim3(ntiles), dim3(256), 0, s, __VA_ARGS__); break;  \
+    case 1: hipLaunchKernelGGL((KERNEL<R, 64, 8>), dim3(ntiles), dim3(512), 0, s, __VA_ARGS__); break;  \
+    case 2: hipLaunchKernelGGL((KERNEL<R, 128, 2>), dim3(ntiles), dim3(256), 0, s, __VA_ARGS__); break; \
+    default: hipLaunchKernelGGL((KERNEL<R, 32, 8>), dim3(ntiles), dim3(256), 0, s, __VA_ARGS__); break; \
+  }
+
+template <typename R>
+hipError_t launch_batch(int kernel, int variant, const BatchCase<R> *C, int ncases, const BatchTile *T, int ntiles,
+                        int k, int par, int a, int b, hipStream_t s) {
+  if (kernel != BK_EDGES && kernel != BK_LOOP_OPEN && ntiles == 0) return hipSuccess;
+  switch (kernel) {
+    case BK_ASSEMBLE: CELL_SWITCH(variant, kb_assemble, C, T, k, par); break;
+    case BK_SWEEP: CELL_SWITCH(variant, kb_sweep, C, T, k, par, a, b); break;
+    case BK_CORRECT: CELL_SWITCH(variant, kb_correct, C, T, k, par); break;
+    case BK_WALL: CELL_SWITCH(variant, kb_wall, C, T, k, par, a); break;
+    case BK_FLUX_DIFF: CELL_SWITCH(variant, kb_flux_diff, C, T, k, par); break;
+    case BK_SWEEP2:
+      switch (variant) {
+        case 0: hipLaunchKernelGGL((kb_sweep2<R, 64, 8>), dim3(ntiles), dim3(512), 0, s, C, T, k, par, a, b); break;
+        case 1: hipLaunchKernelGGL((kb_sweep2<R, 64, 4>), dim3(ntiles), dim3(256), 0, s, C, T, k, par, a, b); break;
+        case 2: hipLaunchKernelGGL((kb_sweep2<R, 128, 4>), dim3(ntiles), dim3(512), 0, s, C, T, k, par, a, b); break;
+        default:
+          hipLaunchKernelGGL((kb_sweep2<R, 64, 16>), dim3(ntiles), dim3(1024), 0, s, C, T, k, par, a, b);
+          break;
+      }
+      break;
+    case BK_EDGES: hipLaunchKernelGGL(kb_edges<R>, dim3(ncases), dim3(EDGE_THREADS), 0, s, C, k, par, a); break;
+    case BK_LOOP_OPEN: hipLaunchKernelGGL(kb_loop_open<R>, dim3(ncases), dim3(256), 0, s, C); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+#undef CELL_SWITCH
+template hipError_t launch_batch<double>(int, int, const BatchCase<double> *, int, const BatchTile *, int, int, int,
+                                         int, int, hipStream_t);
+template hipError_t launch_batch<float>(int, int, const BatchCase<float> *, int, const BatchTile *, int, int, int,
+                                        int, int, hipStream_t);

@@ -45,3 +45,15 @@ hipError_t launch_fold(unsigned long long *slots, int n, unsigned long long *out
 int continuity_blocks(const KGrid &g);
 hipError_t launch_continuity(const KGrid &g, const KFields &f, double *cs, double *cf, double *partial,
                              hipStream_t s);
+
+// Batched loop-2 launches over the tiles T[0..ntiles) of the cases
+// C[0..ncases).  kernel: BK_*; variant: index into CELL_TILES (cell
+// kernels) or SWEEP_TILES (BK_SWEEP2).  (a, b) = (first, last) for the
+// sweeps, a = parts for BK_WALL and BK_EDGES.
+enum { BK_ASSEMBLE, BK_SWEEP2, BK_SWEEP, BK_CORRECT, BK_WALL, BK_EDGES, BK_FLUX_DIFF, BK_LOOP_OPEN };
+constexpr int NVARIANT = 4;
+extern const int CELL_TILES[NVARIANT][2];   // (TX, TY)
+extern const int SWEEP_TILES[NVARIANT][2];  // (SX, SY)
+template <typename R>
+hipError_t launch_batch(int kernel, int variant, const BatchCase<R> *C, int ncases, const BatchTile *T, int ntiles,
+                        int k, int par, int a, int b, hipStream_t s);

@@ -1,0 +1,83 @@
+"""Batched time steps: cases of different grid sizes stepped together in
+shared launches give, case by case, exactly the fields and iteration counts
+of the same cases stepped one by one -- for every tile shape, after the
+autotuner has run, and for the mixed-precision step."""
+import importlib
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+# (Nx, Ny, Ma, Kn, side, xc)
+CASES = [(60, 14, 0.1, 0.00194, 0.25, 4.0), (96, 20, 1.2, 0.01, 0.3, 3.0), (128, 32, 2.0, 0.003, 0.2, 5.0)]
+
+
+def runs(tmp_path, tag, precision=64):
+    ws = importlib.import_module("simple-ts_amd.warmstart")
+    out = []
+    for k, (Nx, Ny, Ma, Kn, side, xc) in enumerate(CASES):
+        prm = dict(Ma=Ma, Kn=Kn, Re=Ma / (Kn * ws.SQ), side=side, xc=xc)
+        d = ws.write_case(str(tmp_path / f"{tag}{k}"), Nx, Ny, prm, 10)
+        out.append(ws.Run(d, precision))
+    return out
+
+
+def state(r):
+    return [r.ctx.get(k) for k in ("u", "v", "p", "Temper", "rho")]
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+def test_batch_matches_single_cases(variant, tmp_path):
+    pkg = importlib.import_module("simple-ts_amd")
+    single, batched = runs(tmp_path, "s"), runs(tmp_path, "b")
+    B = pkg.Batch([r.ctx for r in batched])
+    B.set_tiles(variant, variant)
+    loop = single[0].loop
+    for step in range(3):
+        it_single = [r.ctx.time_step(*loop)[0] for r in single]
+        _, it_batch = B.time_step(*loop)
+        print(f"variant {variant} step {step}: single {it_single}, batched {it_batch}", flush=True)
+        assert it_batch == it_single
+    for a, b in zip(single, batched):
+        for x, y in zip(state(a), state(b)):
+            assert np.array_equal(x, y)
+    B.close()
+    for r in single + batched:
+        r.close()
+
+
+def test_autotune_then_step_matches(tmp_path):
+    pkg = importlib.import_module("simple-ts_amd")
+    single, batched = runs(tmp_path, "s"), runs(tmp_path, "b")
+    B = pkg.Batch([r.ctx for r in batched])
+    cv, sv, ms = B.autotune(3)
+    print(f"autotune: cell variant {cv}, sweep variant {sv}, ms {np.round(ms, 4).tolist()}", flush=True)
+    assert 0 <= cv < 4 and 0 <= sv < 4 and np.all(ms > 0)
+    loop = single[0].loop
+    for step in range(2):
+        assert B.time_step(*loop)[1] == [r.ctx.time_step(*loop)[0] for r in single]
+    for a, b in zip(single, batched):
+        for x, y in zip(state(a), state(b)):
+            assert np.array_equal(x, y)
+    B.close()
+    for r in single + batched:
+        r.close()
+
+
+def test_mixed_batch_matches_single_mixed(tmp_path):
+    pkg = importlib.import_module("simple-ts_amd")
+    single, batched = runs(tmp_path, "s", 32), runs(tmp_path, "b", 32)
+    B = pkg.Batch([r.ctx for r in batched])
+    loop = single[0].loop
+    mixed = (loop[0], (1e-6, 1e-6, 1e-6), 4, 0.7)
+    for step in range(2):
+        ref = [r.ctx.time_step_mixed(*loop, mixed[0], mixed[1], mixed[2], mixed[3])[:2] for r in single]
+        n32, n64 = B.time_step(*loop, mixed=mixed)
+        assert list(zip(n32, n64)) == ref
+    for a, b in zip(single, batched):
+        for x, y in zip(state(a), state(b)):
+            assert np.array_equal(x, y)
+    B.close()
+    for r in single + batched:
+        r.close()
