@@ -4,25 +4,35 @@ Workload: supersonic flow past a confined square in a 2-D channel (channel
 15H x H, square of side H/4 at x = 4H on the axis), Ma 2.43 / Kn 0.00283 /
 Re 1415, uniform staggered grid 1920 x 384, first-order upwind, every time
 step iterated to the 1e-8 successive-difference tolerance of the reference
-(loop 2), two fused p/T Jacobi sweeps per outer iteration, FP64.  Synthetic
-input: the case is generated in the reference's input format
-(EnteredData.txt + solve_polyhedrons_b.txt), uniform initial field.  The time
-step keeps the Courant number of the 240 x 48 case at dt = 0.01 H/u_in, i.e.
-dt = 0.01 (240/1920) H/u_in: with dt = 0.01 H/u_in on this grid the reference
-solver itself diverges in the first step.
+(loop 2), two fused p/T Jacobi sweeps per outer iteration.  Synthetic input:
+the case is generated in the reference's input format (EnteredData.txt +
+solve_polyhedrons_b.txt), uniform initial field.  The time step keeps the
+Courant number of the 240 x 48 case at dt = 0.01 H/u_in, i.e. dt = 0.01
+(240/1920) H/u_in: with dt = 0.01 H/u_in on this grid the reference solver
+itself diverges in the first step.
 
-A "step" is one converged time step of the GPU program (sts_gpu, the C++
-mirror of the reference's main()).  The first W steps of the run are not
-timed; the next K are.  value = Nx*Ny*K / timed seconds / 1e6 (Mcell.steps/s).
+Three paths run the same W + K steps; the first W are not timed:
+  fp64       the plain FP64 path (sts_gpu, the C++ mirror of the reference's
+             main()), bit-identical to the reference;
+  mixed      loop 2 in FP32 until it stagnates, then FP64 outer iterations
+             ending on the reference's own test (sts_time_step_mixed);
+  surrogate  the mixed path warm-started by the trained BF16 U-Net
+             (sts_time_step_ws), when trained parameters are present
+             (simple-ts_amd/unet_b32.npz, written by train_unet.py).
+value = Nx*Ny*K / timed seconds / 1e6 (Mcell.steps/s) of the last path run
+in the order mixed, surrogate (the headline path); the FP64 path's figure,
+the speed-ups and every field's relative L2 error against the FP64 path
+after the W + K steps are reported beside it.
+
+roofline: the FP64 fused p/T Jacobi pass (k_sweep2, two sweeps per launch),
+2 x 144 algorithmic bytes per cell per launch (16 FP64 arrays read and 2
+written per sweep) over all Nx*Ny cells, divided by its average launch
+duration from HIP events recorded around every launch in the timed region,
+against the 8 TB/s HBM peak.
 
 The CPU baseline is the reference solver itself (oracle/_ref/sts_ref, serial,
-one pinned core) on the first steps of the same case: the time of step 2
-is the difference of a 2-step and a 1-step run (setup excluded), i.e. step 2.
-
-roofline: the fused p/T Jacobi sweep kernel, 144 algorithmic bytes per cell
-(16 FP64 arrays read, 2 written) per launch over all Nx*Ny cells, divided by
-its average launch duration from HIP events recorded around every launch in
-the timed region, against the 8 TB/s HBM peak.
+one pinned core) on the first steps of the same case: the time of step 2 is
+the difference of a 2-step and a 1-step run (setup excluded).
 """
 import argparse
 import importlib
@@ -34,6 +44,8 @@ import sys
 import tempfile
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -42,7 +54,9 @@ NX, NY = 1920, 384
 RE, KN = 1415.0, 0.00283
 DT_OVER_TIN = 0.01 * 240 / NX
 HBM_PEAK_GBS = 8000.0
-SWEEP_BYTES_PER_CELL = 144  # FP64: Fx Fy DTx DTy T_old p_old u v ScT1 rho apx apy bpx bpy p_pr T_pr + T p
+SWEEP_BYTES_PER_CELL = 144  # FP64, per sweep: Fx Fy DTx DTy T_old p_old u v ScT1 rho apx apy bpx bpy p_pr T_pr + T p
+WEIGHTS = os.path.join(ROOT, "simple-ts_amd", "unet_b32.npz")
+MIXED = (5000, (1e-6, 1e-6, 1e-6), 4, 0.7)  # N_I32, FP32 tolerances, stagnation window and ratio
 
 
 def workload_case(dst, steps):
@@ -50,7 +64,12 @@ def workload_case(dst, steps):
     return cases.confined_square(dst, NX, NY, RE, KN, steps, dt_over_tin=DT_OVER_TIN)
 
 
-def run_gpu(steps, warmup, kernel="sweep", args=()):
+def read_fields(d):
+    return {f: np.fromfile(os.path.join(d, f + ".bin"), dtype=np.float64) for f in ("u", "v", "p", "Temper")}
+
+
+def run_driver(steps, warmup, kernel="sweep"):
+    """The FP64 program; returns its timing record and final fields."""
     pkg = importlib.import_module("simple-ts_amd")
     if not os.access(pkg.DRIVER, os.X_OK):
         raise SystemExit(f"{pkg.DRIVER} is not built; run __graft_entry__.build()")
@@ -58,16 +77,54 @@ def run_gpu(steps, warmup, kernel="sweep", args=()):
     try:
         d = workload_case(os.path.join(work, "gpu"), warmup + steps)
         timing = os.path.join(work, "timing.json")
-        t0 = time.time()
-        subprocess.run([pkg.DRIVER, d, "--warmup", str(warmup), "--time-kernel", kernel, "--timing", timing,
-                        *args], check=True)
-        wall = time.time() - t0
+        print(f"[bench] fp64 program: {warmup} + {steps} steps", file=sys.stderr, flush=True)
+        subprocess.run([pkg.DRIVER, d, "--warmup", str(warmup), "--time-kernel", kernel, "--timing", timing],
+                       check=True, stdout=subprocess.DEVNULL)
         with open(timing) as f:
             t = json.load(f)
-        t["process_seconds"] = wall
-        return t
+        return t, read_fields(d)
     finally:
         shutil.rmtree(work, ignore_errors=True)
+
+
+def run_python_path(steps, warmup, net=None):
+    """Mixed path (net None) or surrogate-warm-started mixed path."""
+    ws = importlib.import_module("simple-ts_amd.warmstart")
+    work = tempfile.mkdtemp(prefix="sts_bench_")
+    try:
+        run = ws.Run(workload_case(os.path.join(work, "c"), warmup + steps), precision=32)
+        n32 = n64 = taken = 0
+        t0 = None
+        print(f"[bench] {'surrogate' if net else 'mixed'} path: {warmup} + {steps} steps", file=sys.stderr,
+              flush=True)
+        for s in range(warmup + steps):
+            if s == warmup:
+                t0 = time.perf_counter()
+            if net is None:
+                a, b, _, _ = run.ctx.time_step_mixed(*run.loop, *MIXED)
+            else:
+                u, a, b = run.step_ws(net, mixed=MIXED)
+                taken += u if s >= warmup else 0
+            if s >= warmup:
+                n32, n64 = n32 + a, n64 + b
+        dt = time.perf_counter() - t0
+        run.ctx.jump_overwrite()  # as the program does before it writes the fields
+        f = {k: run.ctx.get(k) for k in ("u", "v", "p", "Temper")}
+        run.close()
+        return dict(seconds=dt, fp32_it=n32, fp64_it=n64, taken=taken), f
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
+def load_net():
+    if not os.path.exists(WEIGHTS):
+        return None, None
+    pkg = importlib.import_module("simple-ts_amd")
+    z = np.load(WEIGHTS, allow_pickle=False)
+    net = pkg.UNet(int(z["cin"]), int(z["cout"]), int(z["base"]), 1, NY, NX)
+    net.set_params(z["params"])
+    return net, {"samples": int(z["samples"]), "train_grid": z["grid"].tolist(),
+                 "holdout_loss": float(z["holdout_loss"][-1])}
 
 
 def run_cpu_baseline():
@@ -83,6 +140,7 @@ def run_cpu_baseline():
             cmd = [ref]
             if shutil.which("taskset"):
                 cmd = ["taskset", "-c", "0"] + cmd
+            print(f"[bench] cpu reference: {n} step(s)", file=sys.stderr, flush=True)
             t0 = time.perf_counter()
             subprocess.run(cmd, cwd=d, check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
             times[n] = time.perf_counter() - t0
@@ -95,42 +153,68 @@ def run_cpu_baseline():
         shutil.rmtree(work, ignore_errors=True)
 
 
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=490)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--mixed-args", default="", help="extra sts_gpu arguments of the mixed run (space separated)")
+    ap.add_argument("--fp64-only", action="store_true", help="run only the FP64 program")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the workload does not shard "
               f"across GPUs in this build)", file=sys.stderr)
         sys.exit(2)
-    t = run_gpu(args.steps, args.warmup)
-    tm = run_gpu(args.steps, args.warmup, "sweep32", ["--mixed"] + args.mixed_args.split())
-    print(json.dumps({"mixed_timing": tm, "fp64_timing": t}), file=sys.stderr)
     na = NX * NY
-    value = na * t["timed_steps"] / t["timed_seconds"] / 1e6
-    ms_step = 1e3 * t["timed_seconds"] / max(1, t["timed_steps"])
+    t, f64 = run_driver(args.steps, args.warmup)
+    fp64_value = na * t["timed_steps"] / t["timed_seconds"] / 1e6
+    paths = {"fp64": {"value": fp64_value, "ms_per_step": 1e3 * t["timed_seconds"] / t["timed_steps"],
+                      "outer_iterations_per_step": t["timed_outer_iterations"] / t["timed_steps"]}}
+    best, best_name = fp64_value, "fp64"
+    if not args.fp64_only:
+        for name in ("mixed", "surrogate"):
+            net = info = None
+            if name == "surrogate":
+                net, info = load_net()
+                if net is None:
+                    continue
+            r, f = run_python_path(args.steps, args.warmup, net)
+            v = na * args.steps / r["seconds"] / 1e6
+            paths[name] = {"value": v, "ms_per_step": 1e3 * r["seconds"] / args.steps,
+                           "fp32_outer_iterations_per_step": r["fp32_it"] / args.steps,
+                           "fp64_outer_iterations_per_step": r["fp64_it"] / args.steps,
+                           "speedup_vs_fp64": v / fp64_value,
+                           "field_rel_l2_vs_fp64": {k: rel_l2(f[k], f64[k]) for k in f}}
+            if net is not None:
+                paths[name]["prediction_taken_steps"] = r["taken"]
+                paths[name]["network"] = info
+                net.close()
+            best, best_name = v, name
     roof = None
     if t.get("kernel_launches"):
         avg_s = t["kernel_ms"] / t["kernel_launches"] / 1e3
-        achieved = SWEEP_BYTES_PER_CELL * na / avg_s / 1e9
+        achieved = 2 * SWEEP_BYTES_PER_CELL * na / avg_s / 1e9
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_sweep (fused p/T Jacobi sweep)",
-                "avg_launch_us": avg_s * 1e6, "launches": t["kernel_launches"]}
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "kernel": "k_sweep2 (FP64 fused pass of two p/T Jacobi sweeps)",
+                "bytes_per_launch": 2 * SWEEP_BYTES_PER_CELL * na, "avg_launch_us": avg_s * 1e6,
+                "launches": t["kernel_launches"]}
     cpu = None if args.no_cpu else run_cpu_baseline()
+    path_desc = {"fp64": "plain FP64", "mixed": "FP32 loop-2 phase + FP64 phase (no warm start)",
+                 "surrogate": "U-Net warm start + FP32 loop-2 phase + FP64 phase"}
     line = {
-        "metric": METRIC, "value": value, "unit": "Mcell·steps/s", "n_gpus": 1, "steps": t["timed_steps"],
-        "warmup": t["warmup_steps"], "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64",
+        "metric": METRIC, "value": best, "unit": "Mcell·steps/s", "n_gpus": 1, "steps": t["timed_steps"],
+        "warmup": t["warmup_steps"], "ms_per_step": paths[best_name]["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if best_name == "fp64" else "f32+f64",
         "data": "synthetic: confined-square channel generated in the reference's input format, uniform start",
         "config": {"workload": f"supersonic confined square, Ma 2.43 Kn {KN} Re {RE:g}, {NX}x{NY}, "
-                               f"dt {DT_OVER_TIN:g} H/u_in, each step converged to 1e-8, FP64", "Nx": NX, "Ny": NY,
-                   "outer_iterations_per_step": t["timed_outer_iterations"] / max(1, t["timed_steps"]),
-                   "path": "plain FP64"},
-        "roofline": roof, "cpu_baseline": cpu,
+                               f"dt {DT_OVER_TIN:g} H/u_in, each step converged to 1e-8", "Nx": NX, "Ny": NY,
+                   "path": path_desc[best_name], "speedup_vs_fp64": best / fp64_value},
+        "paths": paths, "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(line))
 

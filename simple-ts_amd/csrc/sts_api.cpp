@@ -825,6 +825,10 @@ int sts_internal_view(sts_ctx *c, StsView *v) {
   for (int q = 0; q < STS_NFIELDS; q++) v->f[q] = c->f[q];
   v->mask = c->m;
   v->fluid_cells = c->fluid_cells;
+  v->box_o[0] = c->g.Nx_o0;
+  v->box_o[1] = c->g.Nx_o1;
+  v->box_o[2] = c->g.Ny_o0;
+  v->box_o[3] = c->g.Ny_o1;
   v->stream = c->stream;
   return STS_OK;
 }

@@ -15,6 +15,7 @@ struct StsView {
   const double *f[STS_NFIELDS];
   const uint8_t *mask;
   long long fluid_cells;
+  int box_o[4];  // p/T update box: Nx_o0, Nx_o1, Ny_o0, Ny_o1
   hipStream_t stream;
 };
 

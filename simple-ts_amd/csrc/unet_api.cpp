@@ -473,8 +473,8 @@ int sts_internal_unet_predict(sts_unet *u, const StsView &v, const sts_unet_norm
   U_OK(unet_pack_state(f[STS_U], f[STS_V], f[STS_P], f[STS_T], v.mask, np, u->Cx0, 1.0 / nrm->u_ref, 1.0 / nrm->p_ref,
                        u->x, u->stream));
   if ((rc = forward_dev(u, 1))) return rc;
-  U_OK(unet_apply_pred(u->y, f[STS_P], f[STS_T], v.mask, np, nrm->p_ref * nrm->dp_scale, nrm->dT_scale, pw, Tw,
-                       u->stream));
+  U_OK(unet_apply_pred(u->y, f[STS_P], f[STS_T], v.mask, v.Nx, np, v.box_o, nrm->p_ref * nrm->dp_scale,
+                       nrm->dT_scale, pw, Tw, u->stream));
   U_OK(hipStreamSynchronize(u->stream));
   return STS_OK;
 }

@@ -382,13 +382,16 @@ __global__ void k_pack_target(const double *p, const double *p_pr, const double 
 }
 
 // Predicted guess: p' = p + y0 * (p_ref dp_scale), T' = T + y1 * dT_scale on
-// fluid cells, copies elsewhere.
-__global__ void k_apply_pred(const float *Y, const double *p, const double *T, const uint8_t *m, int npix, double sp,
-                             double sT, double *pw, double *Tw) {
+// the cells loop 3 updates (fluid cells of the p/T box [x0,x1) x [y0,y1)),
+// copies elsewhere: fixed-value cells keep their boundary values.
+__global__ void k_apply_pred(const float *Y, const double *p, const double *T, const uint8_t *m, int Nx, int npix,
+                             int x0, int x1, int y0, int y1, double sp, double sT, double *pw, double *Tw) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= npix) return;
-  pw[q] = m[q] ? p[q] + (double)Y[2 * (size_t)q] * sp : p[q];
-  Tw[q] = m[q] ? T[q] + (double)Y[2 * (size_t)q + 1] * sT : T[q];
+  const int i = q % Nx, j = q / Nx;
+  const bool upd = m[q] && i >= x0 && i < x1 && j >= y0 && j < y1;
+  pw[q] = upd ? p[q] + (double)Y[2 * (size_t)q] * sp : p[q];
+  Tw[q] = upd ? T[q] + (double)Y[2 * (size_t)q + 1] * sT : T[q];
 }
 
 // Gathers dataset samples into a batch.
@@ -497,9 +500,10 @@ hipError_t unet_pack_target(const double *p, const double *p_pr, const double *T
   hipLaunchKernelGGL(k_pack_target, dim3(nblk(npix, 256)), dim3(256), 0, s, p, p_pr, T, T_pr, m, npix, sp, sT, Tg, M);
   return hipGetLastError();
 }
-hipError_t unet_apply_pred(const float *Y, const double *p, const double *T, const uint8_t *m, int npix, double sp,
-                           double sT, double *pw, double *Tw, hipStream_t s) {
-  hipLaunchKernelGGL(k_apply_pred, dim3(nblk(npix, 256)), dim3(256), 0, s, Y, p, T, m, npix, sp, sT, pw, Tw);
+hipError_t unet_apply_pred(const float *Y, const double *p, const double *T, const uint8_t *m, int Nx, int npix,
+                           const int box[4], double sp, double sT, double *pw, double *Tw, hipStream_t s) {
+  hipLaunchKernelGGL(k_apply_pred, dim3(nblk(npix, 256)), dim3(256), 0, s, Y, p, T, m, Nx, npix, box[0], box[1],
+                     box[2], box[3], sp, sT, pw, Tw);
   return hipGetLastError();
 }
 hipError_t unet_gather(const uint16_t *Xs, const float *Ts, const uint8_t *Ms, const int *slots, int n, int npix,

@@ -56,7 +56,7 @@ hipError_t unet_pack_state(const double *u, const double *v, const double *p, co
                            int npix, int Cx, double inv_u, double inv_p, uint16_t *X, hipStream_t s);
 hipError_t unet_pack_target(const double *p, const double *p_pr, const double *T, const double *T_pr,
                             const uint8_t *m, int npix, double sp, double sT, float *Tg, uint8_t *M, hipStream_t s);
-hipError_t unet_apply_pred(const float *Y, const double *p, const double *T, const uint8_t *m, int npix, double sp,
-                           double sT, double *pw, double *Tw, hipStream_t s);
+hipError_t unet_apply_pred(const float *Y, const double *p, const double *T, const uint8_t *m, int Nx, int npix,
+                           const int box[4], double sp, double sT, double *pw, double *Tw, hipStream_t s);
 hipError_t unet_gather(const uint16_t *Xs, const float *Ts, const uint8_t *Ms, const int *slots, int n, int npix,
                        int Cx, int Co, uint16_t *X, float *Tg, uint8_t *M, hipStream_t s);

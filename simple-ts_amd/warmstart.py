@@ -22,6 +22,27 @@ CASES = importlib.import_module(__package__ + ".cases")
 SQ = math.sqrt(15 * math.pi / 128)  # u_in = Re Kn sqrt(15 pi / 128) (S:2022)
 
 
+def conv_shapes(cin, cout, base):
+    """(ci, co, k) of the network's convolutions in parameter order
+    (include/sts_unet.h)."""
+    B = base
+    return [(cin, B, 3), (B, B, 3), (B, 2 * B, 3), (2 * B, 2 * B, 3), (2 * B, 4 * B, 3), (4 * B, 4 * B, 3),
+            (4 * B, 8 * B, 3), (8 * B, 8 * B, 3), (12 * B, 4 * B, 3), (4 * B, 4 * B, 3), (6 * B, 2 * B, 3),
+            (2 * B, 2 * B, 3), (3 * B, B, 3), (B, B, 3), (B, cout, 1)]
+
+
+def he_init(cin, cout, base, seed):
+    """Flat FP32 parameters: He-normal weights (the linear head: 1/fan-in
+    variance) and zero biases."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    shapes = conv_shapes(cin, cout, base)
+    for q, (ci, co, k) in enumerate(shapes):
+        var = (2.0 if q < len(shapes) - 1 else 1.0) / (k * k * ci)
+        parts += [rng.standard_normal(k * k * ci * co) * math.sqrt(var), np.zeros(co)]
+    return np.concatenate(parts).astype(np.float32)
+
+
 def case_params(rng):
     """One seeded draw of the training distribution."""
     Ma = rng.uniform(0.1, 2.5)
@@ -126,7 +147,7 @@ def train(net, n_samples, epochs, batch, lr, seed, holdout=0.1, log=print):
     for ep in range(epochs):
         rng.shuffle(tr)
         losses = [net.train_step(tr[b:b + batch], lr) for b in range(0, len(tr) - batch + 1, batch)]
-        hl = [net.eval(hold[b:b + batch]) for b in range(0, len(hold) - batch + 1, batch)] or [float("nan")]
+        hl = [net.eval(hold[b:b + batch]) for b in range(0, len(hold), batch)]
         hist.append(float(np.mean(losses)))
         hist_hold.append(float(np.mean(hl)))
         log(f"epoch {ep}: train loss {hist[-1]:.4e}, held-out {hist_hold[-1]:.4e}")

@@ -18,7 +18,7 @@ def runs(tmp_path, tag, precision=64):
     out = []
     for k, (Nx, Ny, Ma, Kn, side, xc) in enumerate(CASES):
         prm = dict(Ma=Ma, Kn=Kn, Re=Ma / (Kn * ws.SQ), side=side, xc=xc)
-        d = ws.write_case(str(tmp_path / f"{tag}{k}"), Nx, Ny, prm, 10)
+        d = ws.write_case(str(tmp_path / f"{tag}{k}"), Nx, Ny, prm, 10, dt_over_tin=0.005)
         out.append(ws.Run(d, precision))
     return out
 
