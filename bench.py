@@ -164,7 +164,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--fp64-only", action="store_true", help="run only the FP64 program")
+    ap.add_argument("--write-case", metavar="DIR", help="only write the workload case (warmup + steps steps) to DIR")
     args = ap.parse_args()
+    if args.write_case:
+        print(workload_case(args.write_case, args.warmup + args.steps))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the workload does not shard "
               f"across GPUs in this build)", file=sys.stderr)

@@ -203,35 +203,78 @@ struct PTStencil {
 // The reference's T update (S:3837-3857) then p update (S:3884-3895) of one
 // cell of the update set, with the new T in the p denominator (S:3894).  A
 // body neighbour contributes the fluid temperature on its wall (S:3838-3848:
-// m*T_old + !m*T_wall with T_wall finite, i.e. a select).
+// m*T_old + !m*T_wall with T_wall finite, i.e. a select).  The per-cell
+// factors that do not depend on the old p, T are formed once (load) with the
+// reference's operations and kept, so a cell swept twice reads its
+// coefficient arrays once; update() then performs the rest of each
+// expression in the reference's order.
+template <typename R>
+struct PTCoef {
+  R aT1, aT2, aT3, aT4;  // T neighbour coefficients (S:3959-3962)
+  R X1, X2, X3, X4;      // wall temperature of a body neighbour
+  R div, sct, denT;      // velocity divergence term, ScT1, T denominator
+  R a1, a2, a3, a4;      // p neighbour coefficients apx_W, apx_E, apy_S, apy_N
+  R b1, b2, b3, b4;      // bpx_W, bpx_E, bpy_S, bpy_N
+  R gxb, gxe;            // given-velocity flux terms (S:3887-3888)
+  R q, Ap, hh;           // p_pr hx hy / T_pr, ht * sum(a), hx hy
+  R c1, ht;
+  unsigned fluid;  // bit q: neighbour q (W, E, S, N) is fluid
+  bool has_gxb, has_gxe;
+
+  __device__ __forceinline__ void load(const KGridT<R> &g, const KFieldsT<R> &fl, const Nbr &n) {
+    const unsigned ij = n.ij, i1j = n.i1j, ij1 = n.ij1;
+    const uint8_t *m = g.m;
+    const R *Fx = FLD(FX), *Fy = FLD(FY), *DTx = FLD(DTX), *DTy = FLD(DTY), *u = FLD(U), *v = FLD(V);
+    const R *apx = FLD(APX), *apy = FLD(APY), *bpx = FLD(BPX), *bpy = FLD(BPY);
+    ht = g.ht;
+    c1 = R(1.0) - g.gamma1;
+    const R hxi = g.hx[n.i], hyj = g.hy[n.j];
+    aT1 = maxi0(Fx[ij]) + DTx[ij];
+    aT2 = maxi0(-Fx[i1j]) + DTx[i1j];
+    aT3 = maxi0(Fy[ij]) + DTy[ij];
+    aT4 = maxi0(-Fy[ij1]) + DTy[ij1];
+    fluid = (m[n.i_1j] ? 1u : 0u) | (m[i1j] ? 2u : 0u) | (m[n.ij_1] ? 4u : 0u) | (m[ij1] ? 8u : 0u);
+    X1 = (fluid & 1u) ? R(0) : FLD(TW_U)[ij];
+    X2 = (fluid & 2u) ? R(0) : FLD(TW_U)[i1j];
+    X3 = (fluid & 4u) ? R(0) : FLD(TW_V)[ij];
+    X4 = (fluid & 8u) ? R(0) : FLD(TW_V)[ij1];
+    div = (u[i1j] - u[ij]) * hyj + (v[ij1] - v[ij]) * hxi;
+    sct = FLD(SCT1)[ij];
+    denT = ht * (aT1 + aT2 + aT3 + aT4 + Fx[i1j] - Fx[ij] + Fy[ij1] - Fy[ij]) + hxi * hyj * FLD(RHO)[ij];
+    a1 = apx[ij];
+    a2 = apx[i1j];
+    a3 = apy[ij];
+    a4 = apy[ij1];
+    b1 = bpx[ij];
+    b2 = bpx[i1j];
+    b3 = bpy[ij];
+    b4 = bpy[ij1];
+    has_gxb = g.given_xb && (int)n.i == g.Nx_o0;
+    has_gxe = g.given_xe && (int)n.i == g.Nx_o1 - 1;
+    gxb = has_gxb ? Fx[ij] : R(0);
+    gxe = has_gxe ? Fx[i1j] : R(0);
+    q = FLD(P_PR)[ij] * hxi * hyj / (FLD(T_PR)[ij]);
+    Ap = ht * (a1 + a2 + a3 + a4);
+    hh = hxi * hyj;
+  }
+
+  __device__ __forceinline__ void update(const PTStencil<R> &x, R &pn, R &tn) const {
+    const R t1 = (fluid & 1u) ? x.Tw : X1, t2 = (fluid & 2u) ? x.Te : X2;
+    const R t3 = (fluid & 4u) ? x.Ts : X3, t4 = (fluid & 8u) ? x.Tn : X4;
+    tn = (ht * (aT1 * t1 + aT2 * t2 + aT3 * t3 + aT4 * t4 + c1 * x.Pc * div) + sct) / denT;
+    R s = a1 * x.Pw + a2 * x.Pe + a3 * x.Ps + a4 * x.Pn + b1 - b2 + b3 - b4;
+    if (has_gxb) s = s + gxb;
+    if (has_gxe) s = s - gxe;
+    pn = (ht * s + q) / (Ap + hh / (tn));
+  }
+};
+
 template <typename R>
 __device__ __forceinline__ void pt_update(const KGridT<R> &g, const KFieldsT<R> &fl, const Nbr &n, const PTStencil<R> &x,
                                           R &pn, R &tn) {
-  const unsigned ij = n.ij, i1j = n.i1j, ij1 = n.ij1;
-  const uint8_t *m = g.m;
-  const R *Fx = FLD(FX), *Fy = FLD(FY), *DTx = FLD(DTX), *DTy = FLD(DTY), *u = FLD(U), *v = FLD(V);
-  const R *apx = FLD(APX), *apy = FLD(APY), *bpx = FLD(BPX), *bpy = FLD(BPY);
-  const R ht = g.ht;
-  const R hxi = g.hx[n.i], hyj = g.hy[n.j];
-  R aT1 = maxi0(Fx[ij]) + DTx[ij];
-  R aT2 = maxi0(-Fx[i1j]) + DTx[i1j];
-  R aT3 = maxi0(Fy[ij]) + DTy[ij];
-  R aT4 = maxi0(-Fy[ij1]) + DTy[ij1];
-  R X1 = m[n.i_1j] ? x.Tw : FLD(TW_U)[ij];
-  R X2 = m[i1j] ? x.Te : FLD(TW_U)[i1j];
-  R X3 = m[n.ij_1] ? x.Ts : FLD(TW_V)[ij];
-  R X4 = m[ij1] ? x.Tn : FLD(TW_V)[ij1];
-  tn = (ht * (aT1 * X1 + aT2 * X2 + aT3 * X3 + aT4 * X4
-              + (R(1.0) - g.gamma1) * x.Pc * ((u[i1j] - u[ij]) * hyj + (v[ij1] - v[ij]) * hxi))
-        + FLD(SCT1)[ij])
-       / (ht * (aT1 + aT2 + aT3 + aT4 + Fx[i1j] - Fx[ij] + Fy[ij1] - Fy[ij]) + hxi * hyj * FLD(RHO)[ij]);
-  R s = apx[ij] * x.Pw + apx[i1j] * x.Pe + apy[ij] * x.Ps + apy[ij1] * x.Pn + bpx[ij] - bpx[i1j] + bpy[ij]
-             - bpy[ij1];
-  // S:3887-3888: inflow/outflow flux terms of the given-velocity columns
-  if (g.given_xb && (int)n.i == g.Nx_o0) s = s + Fx[ij];
-  if (g.given_xe && (int)n.i == g.Nx_o1 - 1) s = s - Fx[i1j];
-  pn = (ht * s + FLD(P_PR)[ij] * hxi * hyj / (FLD(T_PR)[ij])) /
-       (ht * (apx[ij] + apx[i1j] + apy[ij] + apy[ij1]) + hxi * hyj / (tn));
+  PTCoef<R> c;
+  c.load(g, fl, n);
+  c.update(x, pn, tn);
 }
 
 template <typename R>
@@ -292,52 +335,79 @@ __device__ __forceinline__ void sweep2_tile_body(const KGridT<R> &g, const KFiel
   const int Nx = g.Nx, Ny = g.Ny;
   const int RX = SX + 2, RY = SY + 2;
   const int tid = tx + ty * SX;
-  unsigned long long r1[2] = {0ull, 0ull};
-  for (int q = tid; q < RX * RY; q += SX * SY) {
-    const int ry = q / RX, rx = q - ry * RX;
-    const int i = i0 - 1 + rx, j = j0 - 1 + ry;
-    if (i < 0 || j < 0 || i >= Nx || j >= Ny) continue;
-    const unsigned ij = (unsigned)i + (unsigned)j * (unsigned)Nx;
-    R pn, tn;
-    if (in_pt_set(g, i, j, ij)) {
-      const Nbr n((unsigned)i, (unsigned)j, (unsigned)Nx, (unsigned)Ny);
-      const PTStencil<R> x = stencil_global(Po, To, n);
-      pt_update(g, fl, n, x, pn, tn);
-      if (rx >= 1 && rx <= SX && ry >= 1 && ry <= SY) {
-        r1[0] = (absbits(pn - x.Pc) > r1[0]) ? absbits(pn - x.Pc) : r1[0];
-        r1[1] = (absbits(tn - x.Tc) > r1[1]) ? absbits(tn - x.Tc) : r1[1];
-      }
+  // Sweep 1 on the one-cell halo ring of the tile (its cells belong to the
+  // neighbouring tiles, which count their residuals).
+  const int NR = 2 * RX + 2 * SY;
+  for (int q = tid; q < NR; q += SX * SY) {
+    int rx, ry;
+    if (q < RX) {
+      rx = q;
+      ry = 0;
+    } else if (q < 2 * RX) {
+      rx = q - RX;
+      ry = RY - 1;
+    } else if (q < 2 * RX + SY) {
+      rx = 0;
+      ry = 1 + (q - 2 * RX);
     } else {
-      pn = Po[ij];
-      tn = To[ij];
+      rx = RX - 1;
+      ry = 1 + (q - 2 * RX - SY);
+    }
+    const int hi = i0 - 1 + rx, hj = j0 - 1 + ry;
+    if (hi < 0 || hj < 0 || hi >= Nx || hj >= Ny) continue;
+    const unsigned hij = (unsigned)hi + (unsigned)hj * (unsigned)Nx;
+    R pn, tn;
+    if (in_pt_set(g, hi, hj, hij)) {
+      const Nbr hn((unsigned)hi, (unsigned)hj, (unsigned)Nx, (unsigned)Ny);
+      pt_update(g, fl, hn, stencil_global(Po, To, hn), pn, tn);
+    } else {
+      pn = Po[hij];
+      tn = To[hij];
     }
     sP[ry * RX + rx] = pn;
     sT[ry * RX + rx] = tn;
   }
-  __syncthreads();
-  unsigned long long r2[2] = {0ull, 0ull};
+  // Sweep 1 on the thread's own cell; its factors stay in registers.
   const int i = i0 + tx, j = j0 + ty;
-  if (i < Nx && j < Ny) {
-    const unsigned ij = (unsigned)i + (unsigned)j * (unsigned)Nx;
-    const int cx = tx + 1, cy = ty + 1;
-    if (in_pt_set(g, i, j, ij)) {
-      const Nbr n((unsigned)i, (unsigned)j, (unsigned)Nx, (unsigned)Ny);
-      // clamped neighbours map to the cell itself at the domain edge
-      const int wx = cx - (int)(n.i - n.i_1), ex = cx + (int)(n.i1 - n.i);
-      const int sy = cy - (int)((n.ij - n.ij_1) / (unsigned)Nx), ny = cy + (int)((n.ij1 - n.ij) / (unsigned)Nx);
-      const PTStencil<R> x{sP[cy * RX + cx], sT[cy * RX + cx], sP[cy * RX + wx], sT[cy * RX + wx],
-                           sP[cy * RX + ex], sT[cy * RX + ex], sP[sy * RX + cx], sT[sy * RX + cx],
-                           sP[ny * RX + cx], sT[ny * RX + cx]};
-      R pn, tn;
-      pt_update(g, fl, n, x, pn, tn);
-      Tn[ij] = tn;
-      Pn[ij] = pn;
-      r2[0] = absbits(pn - x.Pc);
-      r2[1] = absbits(tn - x.Tc);
-    } else {
-      Pn[ij] = sP[cy * RX + cx];
-      Tn[ij] = sT[cy * RX + cx];
-    }
+  const bool in = i < Nx && j < Ny;
+  const unsigned ij = in ? (unsigned)i + (unsigned)j * (unsigned)Nx : 0u;
+  const bool upd = in && in_pt_set(g, i, j, ij);
+  const Nbr n((unsigned)(in ? i : 0), (unsigned)(in ? j : 0), (unsigned)Nx, (unsigned)Ny);
+  const int cx = tx + 1, cy = ty + 1;
+  unsigned long long r1[2] = {0ull, 0ull};
+  PTCoef<R> c;
+  if (upd) {
+    c.load(g, fl, n);
+    const PTStencil<R> x = stencil_global(Po, To, n);
+    R pn, tn;
+    c.update(x, pn, tn);
+    r1[0] = absbits(pn - x.Pc);
+    r1[1] = absbits(tn - x.Tc);
+    sP[cy * RX + cx] = pn;
+    sT[cy * RX + cx] = tn;
+  } else if (in) {
+    sP[cy * RX + cx] = Po[ij];
+    sT[cy * RX + cx] = To[ij];
+  }
+  __syncthreads();
+  // Sweep 2 on the own cell from LDS.
+  unsigned long long r2[2] = {0ull, 0ull};
+  if (upd) {
+    // clamped neighbours map to the cell itself at the domain edge
+    const int wx = cx - (int)(n.i - n.i_1), ex = cx + (int)(n.i1 - n.i);
+    const int sy = cy - (int)((n.ij - n.ij_1) / (unsigned)Nx), ny = cy + (int)((n.ij1 - n.ij) / (unsigned)Nx);
+    const PTStencil<R> x{sP[cy * RX + cx], sT[cy * RX + cx], sP[cy * RX + wx], sT[cy * RX + wx],
+                         sP[cy * RX + ex], sT[cy * RX + ex], sP[sy * RX + cx], sT[sy * RX + cx],
+                         sP[ny * RX + cx], sT[ny * RX + cx]};
+    R pn, tn;
+    c.update(x, pn, tn);
+    Tn[ij] = tn;
+    Pn[ij] = pn;
+    r2[0] = absbits(pn - x.Pc);
+    r2[1] = absbits(tn - x.Tc);
+  } else if (in) {
+    Pn[ij] = sP[cy * RX + cx];
+    Tn[ij] = sT[cy * RX + cx];
   }
   if (r1p || r1T) block_max_atomic<2>(r1, r1p, r1T);
   if (r2p || r2T) block_max_atomic<2>(r2, r2p, r2T);
