@@ -913,12 +913,20 @@ extern "C" int sts_batch_destroy(sts_batch *b) {
 
 static int build_tiles(sts_batch *b, const int (*shape)[2], BatchTile **out, int *count) {
   for (int v = 0; v < NVARIANT; v++) {
-    std::vector<BatchTile> t;
+    std::vector<BatchTile> lt, t;
     const int TX = shape[v][0], TY = shape[v][1];
     for (int c = 0; c < (int)b->cs.size(); c++) {
       const int nx = (b->cs[c]->Nx + TX - 1) / TX, ny = (b->cs[c]->Ny + TY - 1) / TY;
       for (int y = 0; y < ny; y++)
-        for (int x = 0; x < nx; x++) t.push_back(BatchTile{c, x, y});
+        for (int x = 0; x < nx; x++) lt.push_back(BatchTile{c, x, y});
+    }
+    // Workgroup b runs on chiplet b % 8 (round-robin dispatch); give every
+    // chiplet one contiguous run of the tile list (same order as xcd_tile).
+    const int nt = (int)lt.size(), per = nt / 8, rem = nt % 8;
+    t.resize(lt.size());
+    for (int bid = 0; bid < nt; bid++) {
+      const int x = bid % 8;
+      t[bid] = lt[x * per + std::min(x, rem) + bid / 8];
     }
     HIP_OK(hipMalloc(&out[v], sizeof(BatchTile) * (t.empty() ? 1 : t.size())));
     HIP_OK(hipMemcpyAsync(out[v], t.data(), sizeof(BatchTile) * t.size(), hipMemcpyHostToDevice, b->stream));

@@ -15,6 +15,26 @@ namespace {
 
 constexpr int BX = 64, BY = 4;
 
+// Workgroups are dispatched to the chiplets (XCDs, each with its own L2)
+// round-robin by linear id.  Tile t of a launch of nt tiles is given to the
+// workgroup ids that make every XCD work through one contiguous run of tiles
+// (whole rows of tiles), so tiles sharing halo cache lines meet in one L2.
+constexpr int NXCD = 8;
+__device__ __forceinline__ int xcd_tile(int bid, int nt) {
+  const int per = nt / NXCD, rem = nt % NXCD, x = bid % NXCD;
+  return x * per + min(x, rem) + bid / NXCD;
+}
+
+// Origin (in cells) of this workgroup's TX x TY tile of an Nx-wide grid
+// launched as a 1-D grid of tiles.
+template <int TX, int TY>
+__device__ __forceinline__ void tile_origin(int Nx, int &i0, int &j0) {
+  const int nbx = (Nx + TX - 1) / TX;
+  const int t = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+  i0 = (t % nbx) * TX;
+  j0 = (t / nbx) * TY;
+}
+
 __device__ __forceinline__ bool in_box(int i, int j, int x0, int x1, int y0, int y1) {
   return i >= x0 && i < x1 && j >= y0 && j < y1;
 }
@@ -188,7 +208,9 @@ __device__ __forceinline__ void assemble_cell(const KGridT<R> &g, const KFieldsT
 template <typename R>
 __global__ void __launch_bounds__(BX *BY) k_assemble(KGridT<R> g, KFieldsT<R> fl, Gate gate) {
   if (gated_out(gate)) return;
-  assemble_cell(g, fl, (int)(blockIdx.x * BX + threadIdx.x), (int)(blockIdx.y * BY + threadIdx.y));
+  int i0, j0;
+  tile_origin<BX, BY>(g.Nx, i0, j0);
+  assemble_cell(g, fl, i0 + (int)threadIdx.x, j0 + (int)threadIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -298,7 +320,9 @@ __global__ void __launch_bounds__(BX *BY) k_sweep(KGridT<R> g, KFieldsT<R> fl, c
                                                   unsigned long long *red_T, unsigned long long *red_p2,
                                                   unsigned long long *red_T2, Gate gate) {
   if (gated_out(gate)) return;
-  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  int i0, j0;
+  tile_origin<BX, BY>(g.Nx, i0, j0);
+  const int i = i0 + (int)threadIdx.x, j = j0 + (int)threadIdx.y;
   unsigned long long r[2] = {0ull, 0ull};
   if (i < g.Nx && j < g.Ny) {
     const unsigned ij = (unsigned)i + (unsigned)j * (unsigned)g.Nx;
@@ -430,7 +454,9 @@ __global__ void __launch_bounds__(S2X *S2Y) k_sweep2(KGridT<R> g, KFieldsT<R> fl
                                                      unsigned long long *r1T, unsigned long long *r2p,
                                                      unsigned long long *r2T, Gate gate) {
   if (gated_out(gate)) return;
-  sweep2_tile<R, S2X, S2Y>(g, fl, Po, To, Pn, Tn, (int)blockIdx.x * S2X, (int)blockIdx.y * S2Y, (int)threadIdx.x,
+  int i0, j0;
+  tile_origin<S2X, S2Y>(g.Nx, i0, j0);
+  sweep2_tile<R, S2X, S2Y>(g, fl, Po, To, Pn, Tn, i0, j0, (int)threadIdx.x,
                            (int)threadIdx.y, r1p, r1T, r2p, r2T);
 }
 
@@ -488,7 +514,9 @@ __global__ void __launch_bounds__(BX *BY) k_correct(KGridT<R> g, KFieldsT<R> fl,
                                                     unsigned long long *red_v, int do_correct, Gate gate) {
   if (gated_out(gate)) return;
   unsigned long long r[2] = {0ull, 0ull};
-  correct_cell(g, fl, (int)(blockIdx.x * BX + threadIdx.x), (int)(blockIdx.y * BY + threadIdx.y), do_correct, r);
+  int i0, j0;
+  tile_origin<BX, BY>(g.Nx, i0, j0);
+  correct_cell(g, fl, i0 + (int)threadIdx.x, j0 + (int)threadIdx.y, do_correct, r);
   if (do_correct) block_max_atomic<2>(r, red_u, red_v);
 }
 
@@ -690,7 +718,9 @@ __device__ __forceinline__ void wall_cell(const KGridT<R> &g, const KFieldsT<R> 
 template <typename R>
 __global__ void __launch_bounds__(BX *BY) k_wall(KGridT<R> g, KFieldsT<R> fl, int parts, Gate gate) {
   if (gated_out(gate)) return;
-  wall_cell(g, fl, (int)(blockIdx.x * BX + threadIdx.x), (int)(blockIdx.y * BY + threadIdx.y), parts);
+  int i0, j0;
+  tile_origin<BX, BY>(g.Nx, i0, j0);
+  wall_cell(g, fl, i0 + (int)threadIdx.x, j0 + (int)threadIdx.y, parts);
 }
 
 // ---------------------------------------------------------------------------
@@ -943,7 +973,9 @@ __device__ __forceinline__ void flux_diff_cell(const KGridT<R> &g, const KFields
 template <typename R>
 __global__ void __launch_bounds__(BX *BY) k_flux_diff(KGridT<R> g, KFieldsT<R> fl, Gate gate) {
   if (gated_out(gate)) return;
-  flux_diff_cell(g, fl, (int)(blockIdx.x * BX + threadIdx.x), (int)(blockIdx.y * BY + threadIdx.y));
+  int i0, j0;
+  tile_origin<BX, BY>(g.Nx, i0, j0);
+  flux_diff_cell(g, fl, i0 + (int)threadIdx.x, j0 + (int)threadIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -1235,36 +1267,40 @@ template <typename R>
 static inline dim3 grid2d(const KGridT<R> &g) {
   return dim3((g.Nx + BX - 1) / BX, (g.Ny + BY - 1) / BY);
 }
+// 1-D grid of TX x TY tiles (kernels that place their tiles with tile_origin)
+template <int TX, int TY, typename R>
+static inline dim3 grid_tiles(const KGridT<R> &g) {
+  return dim3(((g.Nx + TX - 1) / TX) * ((g.Ny + TY - 1) / TY));
+}
 
 template <typename R>
 hipError_t launch_assemble(const KGridT<R> &g, const KFieldsT<R> &f, hipStream_t s, Gate gate) {
-  hipLaunchKernelGGL(k_assemble<R>, grid2d(g), dim3(BX, BY), 0, s, g, f, gate);
+  hipLaunchKernelGGL(k_assemble<R>, (grid_tiles<BX, BY>(g)), dim3(BX, BY), 0, s, g, f, gate);
   return hipGetLastError();
 }
 template <typename R>
 hipError_t launch_sweep(const KGridT<R> &g, const KFieldsT<R> &f, const R *Po, const R *To, R *Pn, R *Tn,
                         unsigned long long *rp, unsigned long long *rT, hipStream_t s, unsigned long long *rp2,
                         unsigned long long *rT2, Gate gate) {
-  hipLaunchKernelGGL(k_sweep<R>, grid2d(g), dim3(BX, BY), 0, s, g, f, Po, To, Pn, Tn, rp, rT, rp2, rT2, gate);
+  hipLaunchKernelGGL(k_sweep<R>, (grid_tiles<BX, BY>(g)), dim3(BX, BY), 0, s, g, f, Po, To, Pn, Tn, rp, rT, rp2, rT2, gate);
   return hipGetLastError();
 }
 template <typename R>
 hipError_t launch_sweep2(const KGridT<R> &g, const KFieldsT<R> &f, const R *Po, const R *To, R *Pn, R *Tn,
                          unsigned long long *r1p, unsigned long long *r1T, unsigned long long *r2p,
                          unsigned long long *r2T, hipStream_t s, Gate gate) {
-  dim3 gr((g.Nx + S2X - 1) / S2X, (g.Ny + S2Y - 1) / S2Y);
-  hipLaunchKernelGGL(k_sweep2<R>, gr, dim3(S2X, S2Y), 0, s, g, f, Po, To, Pn, Tn, r1p, r1T, r2p, r2T, gate);
+  hipLaunchKernelGGL(k_sweep2<R>, (grid_tiles<S2X, S2Y>(g)), dim3(S2X, S2Y), 0, s, g, f, Po, To, Pn, Tn, r1p, r1T, r2p, r2T, gate);
   return hipGetLastError();
 }
 template <typename R>
 hipError_t launch_correct(const KGridT<R> &g, const KFieldsT<R> &f, unsigned long long *ru, unsigned long long *rv,
                           int do_correct, hipStream_t s, Gate gate) {
-  hipLaunchKernelGGL(k_correct<R>, grid2d(g), dim3(BX, BY), 0, s, g, f, ru, rv, do_correct, gate);
+  hipLaunchKernelGGL(k_correct<R>, (grid_tiles<BX, BY>(g)), dim3(BX, BY), 0, s, g, f, ru, rv, do_correct, gate);
   return hipGetLastError();
 }
 template <typename R>
 hipError_t launch_wall(const KGridT<R> &g, const KFieldsT<R> &f, int parts, hipStream_t s, Gate gate) {
-  hipLaunchKernelGGL(k_wall<R>, grid2d(g), dim3(BX, BY), 0, s, g, f, parts, gate);
+  hipLaunchKernelGGL(k_wall<R>, (grid_tiles<BX, BY>(g)), dim3(BX, BY), 0, s, g, f, parts, gate);
   return hipGetLastError();
 }
 template <typename R>
@@ -1275,7 +1311,7 @@ hipError_t launch_edges(const KGridT<R> &g, const KFieldsT<R> &f, int parts, dou
 }
 template <typename R>
 hipError_t launch_flux_diff(const KGridT<R> &g, const KFieldsT<R> &f, hipStream_t s, Gate gate) {
-  hipLaunchKernelGGL(k_flux_diff<R>, grid2d(g), dim3(BX, BY), 0, s, g, f, gate);
+  hipLaunchKernelGGL(k_flux_diff<R>, (grid_tiles<BX, BY>(g)), dim3(BX, BY), 0, s, g, f, gate);
   return hipGetLastError();
 }
 
