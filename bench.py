@@ -28,7 +28,11 @@ roofline: the FP64 fused p/T Jacobi pass (k_sweep2, two sweeps per launch),
 2 x 144 algorithmic bytes per cell per launch (16 FP64 arrays read and 2
 written per sweep) over all Nx*Ny cells, divided by its average launch
 duration from HIP events recorded around every launch in the timed region,
-against the 8 TB/s HBM peak.
+against the 8 TB/s HBM peak.  traffic: the memory bytes one launch really
+moves, from two rocprofv3 counter passes over a 3-step run of the same case
+(FETCH_SIZE, then WRITE_SIZE, median KiB per launch), each scaled by what the
+counter reports for the 5-array copy of the time-step prologue (k_copy5,
+5 x 8 x Nx*Ny bytes read and written).
 
 The CPU baseline is the reference solver itself (oracle/_ref/sts_ref, serial,
 one pinned core) on the first steps of the same case: the time of step 2 is
@@ -116,6 +120,57 @@ def run_python_path(steps, warmup, net=None):
         shutil.rmtree(work, ignore_errors=True)
 
 
+def counter_pass(counter, case_dir, out_dir):
+    """Median KiB per launch of k_sweep2<double> and k_copy5 for one counter."""
+    import csv
+    import statistics
+    pkg = importlib.import_module("simple-ts_amd")
+    prof = shutil.which("rocprofv3")
+    if not prof:
+        return None
+    d = os.path.join(out_dir, counter)
+    cmd = ["timeout", "-s", "KILL", "120", prof, "--pmc", counter, "--kernel-include-regex", "k_sweep2|k_copy5",
+           "--output-format", "csv", "-d", d, "-o", "c", "--", pkg.DRIVER, case_dir]
+    print(f"[bench] counter pass {counter}", file=sys.stderr, flush=True)
+    env = dict(os.environ, TMPDIR=out_dir)
+    r = subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, cwd=out_dir, env=env)
+    path = os.path.join(d, "c_counter_collection.csv")
+    if r.returncode != 0 or not os.path.exists(path):
+        return None
+    by = {"sweep": [], "copy": []}
+    for row in csv.DictReader(open(path)):
+        name = row["Kernel_Name"]
+        key = "sweep" if "k_sweep2<double>" in name else ("copy" if "k_copy5" in name else None)
+        if key:
+            by[key].append(float(row["Counter_Value"]))
+    if not by["sweep"] or not by["copy"]:
+        return None
+    top = max(by["sweep"])
+    live = [v for v in by["sweep"] if v > 0.05 * top]  # launches past the loop-2 stop return at once
+    return statistics.median(live), statistics.median(by["copy"])
+
+
+def measure_traffic():
+    """HBM bytes per k_sweep2 launch: FETCH_SIZE + WRITE_SIZE, each scaled by
+    the same counter's reading for k_copy5 (known bytes)."""
+    work = tempfile.mkdtemp(prefix="sts_pmc_")
+    try:
+        d = workload_case(os.path.join(work, "c"), 3)
+        copy_bytes = 5 * 8 * NX * NY
+        total, factors = 0.0, {}
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            r = counter_pass(counter, d, work)
+            if r is None:
+                return None, None
+            sweep_kib, copy_kib = r
+            f = copy_bytes / (copy_kib * 1024.0)
+            factors[counter] = f
+            total += sweep_kib * 1024.0 * f
+        return total, factors
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
 def load_net():
     if not os.path.exists(WEIGHTS):
         return None, None
@@ -164,6 +219,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--fp64-only", action="store_true", help="run only the FP64 program")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes")
     ap.add_argument("--write-case", metavar="DIR", help="only write the workload case (warmup + steps steps) to DIR")
     args = ap.parse_args()
     if args.write_case:
@@ -202,8 +258,9 @@ def main():
     if t.get("kernel_launches"):
         avg_s = t["kernel_ms"] / t["kernel_launches"] / 1e3
         achieved = 2 * SWEEP_BYTES_PER_CELL * na / avg_s / 1e9
+        traffic, factors = (None, None) if args.no_traffic else measure_traffic()
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "counter_scale": factors,
                 "kernel": "k_sweep2 (FP64 fused pass of two p/T Jacobi sweeps)",
                 "bytes_per_launch": 2 * SWEEP_BYTES_PER_CELL * na, "avg_launch_us": avg_s * 1e6,
                 "launches": t["kernel_launches"]}

@@ -126,7 +126,8 @@ extern "C" int sts_time_kernel(sts_ctx *c, int kernel) {
   if (!c || kernel < -1 || kernel >= 2 * STS_K_COUNT) return fail(STS_ERR_ARG, "bad kernel id");
   if (c->ev.empty()) {
     c->ev.resize(2 * EV_POOL);
-    for (hipEvent_t &e : c->ev) HIP_OK(hipEventCreate(&e));
+    // timing only: no system-scope fence (cache writeback) at each record
+    for (hipEvent_t &e : c->ev) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
   }
   int rc = drain_events(c);
   if (rc) return rc;
@@ -961,8 +962,8 @@ extern "C" int sts_batch_create(sts_ctx *const *ctxs, int n, sts_batch **out) {
   B_OK(hipMalloc(&b->ctrl, sizeof(LoopCtrl) * n));
   B_OK(hipHostMalloc(&b->h_ctrl, sizeof(LoopCtrl) * n, hipHostMallocDefault));
   B_OK(hipMalloc(&b->lred, sizeof(unsigned long long) * NSLOT * SLOT_STRIDE * (size_t)n));
-  B_OK(hipEventCreate(&b->ev0));
-  B_OK(hipEventCreate(&b->ev1));
+  B_OK(hipEventCreateWithFlags(&b->ev0, hipEventDisableSystemFence));
+  B_OK(hipEventCreateWithFlags(&b->ev1, hipEventDisableSystemFence));
 #undef B_OK
   int rc;
   if ((rc = build_tiles(b, CELL_TILES, b->cell_t, b->n_cell)) || (rc = build_tiles(b, SWEEP_TILES, b->sweep_t, b->n_sweep)))
