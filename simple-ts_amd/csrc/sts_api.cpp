@@ -68,6 +68,8 @@ struct sts_ctx {
   float *p_alt32 = nullptr, *T_alt32 = nullptr;
   int last_iters32 = 0, last_iters64m = 0;
   long long fluid_cells = 0;
+  unsigned *wall_cells = nullptr;  // cells next to a body/fluid interface (k_wall_list)
+  int n_wall_cells = 0;
   double *p_ws = nullptr, *T_ws = nullptr;  // predicted starting guess (warm start)
   bool fuse_sweeps = true;  // pairs of loop-3 sweeps in one k_sweep2 pass
   // kernel timing: event pairs around launches of kernel `timed`
@@ -189,6 +191,7 @@ extern "C" int sts_destroy(sts_ctx *c) {
   (void)hipFree(c->T_alt);
   for (int q = 0; q < STS_NFIELDS; q++) (void)hipFree(c->f32[q]);
   (void)hipFree(c->p_ws);
+  (void)hipFree(c->wall_cells);
   (void)hipFree(c->T_ws);
   (void)hipFree(c->p_alt32);
   (void)hipFree(c->T_alt32);
@@ -323,6 +326,29 @@ extern "C" int sts_create(const sts_grid *gi, int precision, sts_ctx **out) {
   if (arc) {
     sts_destroy(c);
     return arc;
+  }
+  {
+    // the wall conditions write only at cells k with k, k-1 or k-Nx on a
+    // body/fluid interface (a cell with a 4-neighbour of the other kind)
+    const int Nx = gi->Nx, Ny = gi->Ny;
+    auto iface = [&](int i, int j) {
+      const uint8_t a = mask[(size_t)i + (size_t)j * Nx];
+      return (i > 0 && mask[(size_t)i - 1 + (size_t)j * Nx] != a) ||
+             (i + 1 < Nx && mask[(size_t)i + 1 + (size_t)j * Nx] != a) ||
+             (j > 0 && mask[(size_t)i + (size_t)(j - 1) * Nx] != a) ||
+             (j + 1 < Ny && mask[(size_t)i + (size_t)(j + 1) * Nx] != a);
+    };
+    std::vector<unsigned> wl;
+    for (int j = 0; j < Ny; j++)
+      for (int i = 0; i < Nx; i++)
+        if (iface(i, j) || (i > 0 && iface(i - 1, j)) || (j > 0 && iface(i, j - 1)))
+          wl.push_back((unsigned)i + (unsigned)j * (unsigned)Nx);
+    c->n_wall_cells = (int)wl.size();
+    if (hipMalloc(&c->wall_cells, sizeof(unsigned) * (wl.empty() ? 1 : wl.size())) != hipSuccess ||
+        hipMemcpy(c->wall_cells, wl.data(), sizeof(unsigned) * wl.size(), hipMemcpyHostToDevice) != hipSuccess) {
+      sts_destroy(c);
+      return fail(STS_ERR_HIP, "wall cell list");
+    }
   }
   KGrid &g = c->g;
   g.Nx = gi->Nx;
@@ -662,7 +688,8 @@ static int enqueue_outer(sts_ctx *c, const Loop2State<R> &st, int k, int n, Loop
   HIP_OK(launch_correct(g, kf, lr + 0, lr + 1, 1, c->stream, gate));
   timed_end(c, t);
   t = timed_begin(c, st.kid + STS_K_WALL);
-  HIP_OK(launch_wall(g, kf, STS_BC_SLIP_U | STS_BC_SLIP_V | STS_BC_JUMP, c->stream, gate));
+  HIP_OK(launch_wall(g, kf, STS_BC_SLIP_U | STS_BC_SLIP_V | STS_BC_JUMP, c->stream, gate, c->wall_cells,
+                     c->n_wall_cells));
   timed_end(c, t);
   lt.lred = lr;
   lt.log_k += 6 * (size_t)k;

@@ -723,6 +723,18 @@ __global__ void __launch_bounds__(BX *BY) k_wall(KGridT<R> g, KFieldsT<R> fl, in
   wall_cell(g, fl, i0 + (int)threadIdx.x, j0 + (int)threadIdx.y, parts);
 }
 
+// The same over a list of the only cells that can be written: those next to
+// a body/fluid interface (the list is built once from the mask).
+template <typename R>
+__global__ void __launch_bounds__(256) k_wall_list(KGridT<R> g, KFieldsT<R> fl, const unsigned *cells, int n,
+                                                   int parts, Gate gate) {
+  if (gated_out(gate)) return;
+  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (t >= n) return;
+  const unsigned k = cells[t];
+  wall_cell(g, fl, (int)(k % (unsigned)g.Nx), (int)(k / (unsigned)g.Nx), parts);
+}
+
 // ---------------------------------------------------------------------------
 // A12 edge part: Neumann copies and inflow/outflow mass conditions at the
 // solve-box edges, S:6790-7096 (velocities), S:7650-7741 (temperature),
@@ -776,8 +788,17 @@ __device__ __forceinline__ void edge_y(const KGridT<R> &g, R *a, int b0, int b1,
 
 template <typename R>
 __device__ void edges_block(const KGridT<R> &g, const KFieldsT<R> &fl, int parts, double *probe, const Gate &gate,
-                            const LoopTest &lt) {
-  if (lt.ctrl && threadIdx.x < NSLOT) {  // wave 0: fold the slots of iteration gate.k
+                            const LoopTest &lt, int group) {
+  // The velocity, temperature and pressure conditions touch disjoint arrays
+  // (u, v read rho and the face densities, which none of them writes), so
+  // three workgroups can run them side by side: group 0 = loop test and
+  // velocities, 1 = temperature, 2 = pressure and the probe.
+  if (group == 1) parts &= STS_BC_TEMPERATURE;
+  if (group == 2) parts &= STS_BC_PRESSURE;
+  if (group == 0) parts &= STS_BC_VELOCITIES;
+  if (group >= 0 && group != 2) probe = nullptr;
+  const bool test = lt.ctrl && group <= 0;
+  if (test && threadIdx.x < NSLOT) {  // wave 0: fold the slots of iteration gate.k
     unsigned long long *sl = lt.lred + threadIdx.x * SLOT_STRIDE;
 #pragma unroll
     for (int q = 0; q < 6; q++) {
@@ -786,7 +807,7 @@ __device__ void edges_block(const KGridT<R> &g, const KFieldsT<R> &fl, int parts
       if (threadIdx.x == 0) lt.log_k[q] = w;
     }
   }
-  if (lt.ctrl && threadIdx.x == 0) {
+  if (test && threadIdx.x == 0) {
     // S:4433-4437 for iteration gate.k (Iter = k + 1 after the increment)
     double r[4];
     bool fin = true;
@@ -916,7 +937,7 @@ template <typename R>
 __global__ void __launch_bounds__(EDGE_THREADS) k_edges(KGridT<R> g, KFieldsT<R> fl, int parts, double *probe,
                                                         Gate gate, LoopTest lt) {
   if (gated_out(gate)) return;
-  edges_block(g, fl, parts, probe, gate, lt);
+  edges_block(g, fl, parts, probe, gate, lt, (int)blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1235,13 +1256,13 @@ __global__ void __launch_bounds__(TX *TY) kb_flux_diff(const BatchCase<R> *C, co
 // Edge conditions and the loop-2 test: one workgroup per case.
 template <typename R>
 __global__ void __launch_bounds__(EDGE_THREADS) kb_edges(const BatchCase<R> *C, int k, int par, int parts) {
-  const BatchCase<R> &c = C[blockIdx.x];
+  const BatchCase<R> &c = C[blockIdx.x / 3];
   const Gate gate{c.ctrl, k};
   if (gated_out(gate)) return;
   const KFieldsT<R> f = fields_at(c, par);
   const LoopTest lt{c.ctrl, c.lred, c.log + 6 * (size_t)k, c.N_I, c.tol_uv, c.tol_p, c.tol_T, c.stall_w,
                     c.stall_ratio};
-  edges_block(c.g, f, parts, c.probe, gate, lt);
+  edges_block(c.g, f, parts, c.probe, gate, lt, (int)(blockIdx.x % 3));
 }
 
 // Opens every case's loop 2: control, log (6 N_I words), slotted words.
@@ -1299,14 +1320,21 @@ hipError_t launch_correct(const KGridT<R> &g, const KFieldsT<R> &f, unsigned lon
   return hipGetLastError();
 }
 template <typename R>
-hipError_t launch_wall(const KGridT<R> &g, const KFieldsT<R> &f, int parts, hipStream_t s, Gate gate) {
-  hipLaunchKernelGGL(k_wall<R>, (grid_tiles<BX, BY>(g)), dim3(BX, BY), 0, s, g, f, parts, gate);
+hipError_t launch_wall(const KGridT<R> &g, const KFieldsT<R> &f, int parts, hipStream_t s, Gate gate,
+                       const unsigned *cells, int ncells) {
+  if (cells) {
+    if (ncells > 0)
+      hipLaunchKernelGGL(k_wall_list<R>, dim3((ncells + 255) / 256), dim3(256), 0, s, g, f, cells, ncells, parts,
+                         gate);
+  } else {
+    hipLaunchKernelGGL(k_wall<R>, (grid_tiles<BX, BY>(g)), dim3(BX, BY), 0, s, g, f, parts, gate);
+  }
   return hipGetLastError();
 }
 template <typename R>
 hipError_t launch_edges(const KGridT<R> &g, const KFieldsT<R> &f, int parts, double *probe, hipStream_t s,
                         Gate gate, LoopTest lt) {
-  hipLaunchKernelGGL(k_edges<R>, dim3(1), dim3(EDGE_THREADS), 0, s, g, f, parts, probe, gate, lt);
+  hipLaunchKernelGGL(k_edges<R>, dim3(3), dim3(EDGE_THREADS), 0, s, g, f, parts, probe, gate, lt);
   return hipGetLastError();
 }
 template <typename R>
@@ -1325,7 +1353,8 @@ hipError_t launch_flux_diff(const KGridT<R> &g, const KFieldsT<R> &f, hipStream_
                                        unsigned long long *, hipStream_t, Gate);                               \
   template hipError_t launch_correct<R>(const KGridT<R> &, const KFieldsT<R> &, unsigned long long *,          \
                                         unsigned long long *, int, hipStream_t, Gate);                         \
-  template hipError_t launch_wall<R>(const KGridT<R> &, const KFieldsT<R> &, int, hipStream_t, Gate);          \
+  template hipError_t launch_wall<R>(const KGridT<R> &, const KFieldsT<R> &, int, hipStream_t, Gate,           \
+                                     const unsigned *, int);                                                  \
   template hipError_t launch_edges<R>(const KGridT<R> &, const KFieldsT<R> &, int, double *, hipStream_t, Gate, \
                                       LoopTest);                                                               \
   template hipError_t launch_flux_diff<R>(const KGridT<R> &, const KFieldsT<R> &, hipStream_t, Gate);
@@ -1425,7 +1454,7 @@ hipError_t launch_batch(int kernel, int variant, const BatchCase<R> *C, int ncas
           break;
       }
       break;
-    case BK_EDGES: hipLaunchKernelGGL(kb_edges<R>, dim3(ncases), dim3(EDGE_THREADS), 0, s, C, k, par, a); break;
+    case BK_EDGES: hipLaunchKernelGGL(kb_edges<R>, dim3(3 * ncases), dim3(EDGE_THREADS), 0, s, C, k, par, a); break;
     case BK_LOOP_OPEN: hipLaunchKernelGGL(kb_loop_open<R>, dim3(ncases), dim3(256), 0, s, C); break;
     default: return hipErrorInvalidValue;
   }

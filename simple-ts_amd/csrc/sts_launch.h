@@ -19,9 +19,10 @@ hipError_t launch_sweep2(const KGridT<R> &g, const KFieldsT<R> &f, const R *Po, 
 template <typename R>
 hipError_t launch_correct(const KGridT<R> &g, const KFieldsT<R> &f, unsigned long long *ru, unsigned long long *rv,
                           int do_correct, hipStream_t s, Gate gate = Gate{nullptr, 0});
+// cells: list of the cells next to a body/fluid interface (or null: all cells)
 template <typename R>
 hipError_t launch_wall(const KGridT<R> &g, const KFieldsT<R> &f, int parts, hipStream_t s,
-                       Gate gate = Gate{nullptr, 0});
+                       Gate gate = Gate{nullptr, 0}, const unsigned *cells = nullptr, int ncells = 0);
 template <typename R>
 hipError_t launch_edges(const KGridT<R> &g, const KFieldsT<R> &f, int parts, double *probe, hipStream_t s,
                         Gate gate = Gate{nullptr, 0},
