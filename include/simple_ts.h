@@ -200,6 +200,8 @@ typedef struct sts_mixed_control {
   double tol32_uv, tol32_p, tol32_T;  /* FP32-phase tolerances */
   int stall_w;                        /* stagnation window (0: off) */
   double stall_ratio;                 /* required shrink over the window */
+  int extrapolate;                    /* 1: start loop 2 from 2 x^n - x^(n-1) (u, v, p, T on their update
+                                         sets, the rest rebuilt from them) instead of x^n */
 } sts_mixed_control;
 int sts_time_step_mixed(sts_ctx *ctx, const sts_loop_control *lc, const sts_mixed_control *mc, int *iters32,
                         int *iters64, double *res_log, double *p_probe);

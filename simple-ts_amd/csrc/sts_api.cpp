@@ -71,6 +71,8 @@ struct sts_ctx {
   unsigned *wall_cells = nullptr;  // cells next to a body/fluid interface (k_wall_list)
   int n_wall_cells = 0;
   double *p_ws = nullptr, *T_ws = nullptr;  // predicted starting guess (warm start)
+  double *prev[4] = {};                      // u, v, p, T at the start of the previous step (extrapolation)
+  bool have_prev = false;
   bool fuse_sweeps = true;  // pairs of loop-3 sweeps in one k_sweep2 pass
   // kernel timing: event pairs around launches of kernel `timed`
   int timed = -1;
@@ -191,6 +193,7 @@ extern "C" int sts_destroy(sts_ctx *c) {
   (void)hipFree(c->T_alt);
   for (int q = 0; q < STS_NFIELDS; q++) (void)hipFree(c->f32[q]);
   (void)hipFree(c->p_ws);
+  for (double *q : c->prev) (void)hipFree(q);
   (void)hipFree(c->wall_cells);
   (void)hipFree(c->T_ws);
   (void)hipFree(c->p_alt32);
@@ -387,6 +390,7 @@ extern "C" int sts_create(const sts_grid *gi, int precision, sts_ctx **out) {
 
 extern "C" int sts_set_field(sts_ctx *c, int field, const double *host) {
   if (!c || !host || field < 0 || field >= STS_NFIELDS) return fail(STS_ERR_ARG, "bad field or pointer");
+  c->have_prev = false;  // the state no longer continues the previous steps
   HIP_OK(hipMemcpyAsync(c->f[field], host, sizeof(double) * (size_t)c->Na, hipMemcpyHostToDevice, c->stream));
   HIP_OK(hipStreamSynchronize(c->stream));
   return STS_OK;
@@ -791,6 +795,34 @@ static constexpr unsigned long long WIDEN_MASK =
     bit(STS_RHO_V) | bit(STS_FX) | bit(STS_FY) | bit(STS_GAMMA_XF) | bit(STS_GAMMA_YF) | bit(STS_D_UX) |
     bit(STS_D_UY) | bit(STS_D_VX) | bit(STS_D_VY) | bit(STS_DTX) | bit(STS_DTY) | bit(STS_TW_U) | bit(STS_TW_V);
 
+// After the prologue: loop 2 starts from 2 x^n - x^(n-1) for u, v, p, T (the
+// first time only x^n is stored), and every field derived from them is
+// rebuilt the way the end of an outer iteration builds it.
+static int extrapolate_start(sts_ctx *c, bool on) {
+  if (!on) {
+    c->have_prev = false;
+    return STS_OK;
+  }
+  const size_t bytes = sizeof(double) * (size_t)c->Na;
+  if (!c->prev[0])
+    for (double *&q : c->prev) HIP_OK(hipMalloc(&q, bytes));
+  if (!c->have_prev) {
+    const int src[4] = {STS_U_PR, STS_V_PR, STS_P_PR, STS_T_PR};
+    for (int q = 0; q < 4; q++)
+      HIP_OK(hipMemcpyAsync(c->prev[q], c->f[src[q]], bytes, hipMemcpyDeviceToDevice, c->stream));
+    c->have_prev = true;
+    return STS_OK;
+  }
+  const KFields kf = c->fields();
+  HIP_OK(launch_extrapolate(c->g, kf, c->prev, c->stream));
+  HIP_OK(launch_correct(c->g, kf, nullptr, nullptr, 0, c->stream));
+  HIP_OK(launch_wall(c->g, kf, STS_BC_SLIP_U | STS_BC_SLIP_V | STS_BC_JUMP, c->stream, Gate{nullptr, 0},
+                     c->wall_cells, c->n_wall_cells));
+  HIP_OK(launch_edges(c->g, kf, STS_BC_VELOCITIES | STS_BC_TEMPERATURE | STS_BC_PRESSURE, nullptr, c->stream));
+  HIP_OK(launch_flux_diff(c->g, kf, c->stream));
+  return STS_OK;
+}
+
 // Loop 2 after the prologue: the FP32 phase (when mc asks for one), then the
 // FP64 phase whose test is the reference's.  Logs go to c->log, FP32 first.
 static int run_phases(sts_ctx *c, const sts_loop_control *lc, const sts_mixed_control *mc, int *n32p, int *n64p) {
@@ -835,6 +867,7 @@ extern "C" int sts_time_step_mixed(sts_ctx *c, const sts_loop_control *lc, const
   if ((rc = check_mixed(c, lc, mc))) return rc;
   if ((rc = ensure_loop_buffers(c, lc->N_I + mc->N_I32))) return rc;
   if ((rc = sts_save_previous(c))) return rc;
+  if ((rc = extrapolate_start(c, mc->extrapolate != 0))) return rc;
   int n32 = 0, n64 = 0;
   rc = run_phases(c, lc, mc, &n32, &n64);
   if (iters32) *iters32 = n32;
@@ -875,6 +908,7 @@ extern "C" int sts_time_step_ws(sts_ctx *c, sts_unet *net, const sts_unet_norm *
     HIP_OK(hipMalloc(&c->T_ws, sizeof(double) * (size_t)c->Na));
   }
   if ((rc = sts_save_previous(c))) return rc;
+  if ((rc = extrapolate_start(c, mc && mc->extrapolate != 0))) return rc;
   StsView v;
   sts_internal_view(c, &v);
   if ((rc = sts_internal_unet_predict(net, v, nrm, c->p_ws, c->T_ws))) return rc;
@@ -1107,6 +1141,7 @@ extern "C" int sts_batch_time_step(sts_batch *b, const sts_loop_control *lc, con
   if (!b) return fail(STS_ERR_ARG, "null batch");
   int rc;
   const int n = (int)b->cs.size();
+  if (mc && mc->extrapolate) return fail(STS_ERR_UNSUPPORTED, "batched steps start from x^n (no extrapolation)");
   for (sts_ctx *c : b->cs) {
     if ((rc = check_mixed(c, lc, mc))) return rc;
     if (c->fuse_sweeps != b->cs[0]->fuse_sweeps) return fail(STS_ERR_ARG, "cases differ in sweep fusion");

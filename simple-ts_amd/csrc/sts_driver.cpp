@@ -522,7 +522,7 @@ struct Driver {
   double loop_seconds = 0;
   // mixed-precision time steps (sts_time_step_mixed) when set
   bool mixed = false;
-  sts_mixed_control mc{0, 1e-6, 1e-6, 1e-6, 4, 0.7};
+  sts_mixed_control mc{0, 1e-6, 1e-6, 1e-6, 4, 0.7, 0};
   long long outer_total32 = 0, outer32_at_start = 0;
   // timing of the steps after the warm-up ones (benchmarking)
   unsigned warmup_steps = 0;
@@ -548,7 +548,7 @@ int main(int argc, char **argv) {
   const char *knames[] = {"assemble",   "sweep",   "correct",   "wall",   "edges",   "flux_diff",
                           "assemble32", "sweep32", "correct32", "wall32", "edges32", "flux_diff32"};
   bool mixed = false;
-  sts_mixed_control mc{0, 1e-6, 1e-6, 1e-6, 4, 0.7};
+  sts_mixed_control mc{0, 1e-6, 1e-6, 1e-6, 4, 0.7, 0};
   int n32 = -1;
   for (int a = 1; a < argc; a++) {
     std::string s = argv[a];
@@ -563,6 +563,8 @@ int main(int argc, char **argv) {
       mixed = true;
     } else if (s == "--n32" && a + 1 < argc) {
       n32 = std::atoi(argv[++a]);
+    } else if (s == "--extrapolate") {
+      mc.extrapolate = 1;
     } else if (s == "--tol32" && a + 1 < argc) {
       mc.tol32_uv = mc.tol32_p = mc.tol32_T = std::atof(argv[++a]);
     } else if (s == "--stall" && a + 2 < argc) {

@@ -1053,6 +1053,29 @@ __global__ void __launch_bounds__(BX *BY) k_copy5(const double *a0, const double
   b4[t] = a4[t];
 }
 
+// Start of loop 2 from a linear extrapolation in time: x = 2 x^n - x^(n-1)
+// for u, v, p, T on their update sets (x^n = the *_pr fields after the
+// prologue); x^(n-1) (prev) then takes x^n.
+__global__ void __launch_bounds__(BX *BY) k_extrapolate(KGrid g, KFields fl, double *pu, double *pv, double *pp,
+                                                        double *pT) {
+  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  if (i >= g.Nx || j >= g.Ny) return;
+  const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny);
+  const unsigned k = n.ij;
+  const uint8_t *m = g.m;
+  const double un = FLD(U_PR)[k], vn = FLD(V_PR)[k], pn = FLD(P_PR)[k], Tn = FLD(T_PR)[k];
+  if (in_box(i, j, g.Nx_u0, g.Nx_u1, g.Ny_u0, g.Ny_u1) && m[n.i_1j] && m[k]) FLD(U)[k] = 2.0 * un - pu[k];
+  if (in_box(i, j, g.Nx_v0, g.Nx_v1, g.Ny_v0, g.Ny_v1) && m[n.ij_1] && m[k]) FLD(V)[k] = 2.0 * vn - pv[k];
+  if (in_box(i, j, g.Nx_o0, g.Nx_o1, g.Ny_o0, g.Ny_o1) && m[k]) {
+    FLD(P)[k] = 2.0 * pn - pp[k];
+    FLD(T)[k] = 2.0 * Tn - pT[k];
+  }
+  pu[k] = un;
+  pv[k] = vn;
+  pp[k] = pn;
+  pT[k] = Tn;
+}
+
 __global__ void __launch_bounds__(BX *BY) k_set_inflow(KGrid g, KFields fl, double u_in) {
   const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
   if (i >= g.Nx_u0 || j >= g.Ny) return;
@@ -1383,6 +1406,10 @@ hipError_t launch_copy5(const double *const *src, double *const *dst, int n, hip
   dim3 gr(blocks < 65535 ? blocks : 65535, (blocks + 65534) / 65535);
   hipLaunchKernelGGL(k_copy5, gr, dim3(BX, BY), 0, s, src[0], src[1], src[2], src[3], src[4], dst[0], dst[1], dst[2],
                      dst[3], dst[4], n);
+  return hipGetLastError();
+}
+hipError_t launch_extrapolate(const KGrid &g, const KFields &f, double *const prev[4], hipStream_t s) {
+  hipLaunchKernelGGL(k_extrapolate, grid2d(g), dim3(BX, BY), 0, s, g, f, prev[0], prev[1], prev[2], prev[3]);
   return hipGetLastError();
 }
 hipError_t launch_set_inflow(const KGrid &g, const KFields &f, double u_in, hipStream_t s) {
