@@ -178,3 +178,55 @@ def rayleigh_benard(dst, Nx, Ny, Kn=0.05, Fr=1.0, steps=10, dt=0.01, W=2.0, H=1.
               dvdx_0_BC_xb=0, dvdx_0_BC_xe=0, dTdx_0_BC_xe=0, To_Use_Kn_local_in_wall_BC=1, N_SubDomains_x=1)
     ov.update(extra or {})
     return write_case(dst, ov, walls)
+
+
+# Mode-3 constants (S:2013-2071): gamma = 5/3, Ma = sqrt(6/5), Pr = 2/3,
+# c_mu = (5/16) sqrt(2 pi / gamma); cT = Kn 5 sqrt(pi)/16 multiplies sqrt(T)
+# in the momentum diffusion, cT1 = gamma c_mu Kn / (Ma Pr) in the heat
+# diffusion, and c_Fr_v = 2 / (gamma Fr Ma^2) = 1/Fr is the gravity term.
+def rayleigh_fr(Ra, Kn, T_hot=2.0, T_cold=1.0, p=1.0):
+    """Fr that gives a box of height H = 1 the Rayleigh number Ra.
+
+    The reference's momentum equation is rho Du/Dt = ... + div(cT sqrt(T)
+    grad u) - c_Fr_v rho e_y and its temperature equation rho DT/Dt = ... +
+    div(cT1 sqrt(T) grad T) (the ht*sum(aT) + hx*hy*rho denominators of
+    S:3973-3983), so at the mean state (T_m, rho_m = p/T_m) the kinematic
+    viscosity is nu = cT sqrt(T_m)/rho_m, the thermal diffusivity
+    alpha = cT1 sqrt(T_m)/rho_m, the gravity g = c_Fr_v, and an ideal gas has
+    beta = 1/T_m.  Ra = g beta dT H^3 / (nu alpha) then gives
+    Fr = dT rho_m^2 / (Ra T_m^2 cT cT1).  This relation is not in the
+    reference; it is the Boussinesq reading of its nondimensional equations."""
+    gamma, Ma, Pr = 5.0 / 3.0, math.sqrt(6.0 / 5.0), 2.0 / 3.0
+    c_mu = (5.0 / 16.0) * math.sqrt(2.0 * math.pi / gamma)
+    cT = Kn * 5.0 * math.sqrt(math.pi) / 16.0
+    cT1 = gamma * c_mu * Kn / (Ma * Pr)
+    Tm = 0.5 * (T_hot + T_cold)
+    rho_m = p / Tm
+    return (T_hot - T_cold) * rho_m ** 2 / (Ra * Tm ** 2 * cT * cT1)
+
+
+def batch_cases(root, n=64, steps=300, seed=0, nx_range=(256, 1024), Ra=3000.0):
+    """The mixed set of independent cases stepped together: seeded grids
+    Nx ~ U{256..1024} (multiples of 64) with Nx:Ny = 4:1; the first half are
+    confined squares with Ma ~ U[0.1, 2.5], Kn ~ log-U[1e-3, 5e-2] and
+    dt = 0.005 (240/Nx) H/u_in; the second half Rayleigh-Benard boxes (2:1,
+    T = 2 below, 1 above, Kn = 0.05, Fr from rayleigh_fr(Ra)) with
+    dt = 8e-4 (256/Nx)^2.  Returns [(case_dir, description)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    sq = math.sqrt(15 * math.pi / 128)
+    for k in range(n):
+        Nx = int(rng.integers(nx_range[0] // 64, nx_range[1] // 64 + 1)) * 64
+        Ny = Nx // 4
+        d = os.path.join(root, f"case{k:02d}")
+        if k < n // 2:
+            Ma = rng.uniform(0.1, 2.5)
+            Kn = 10 ** rng.uniform(math.log10(1e-3), math.log10(5e-2))
+            confined_square(d, Nx, Ny, Ma / (Kn * sq), Kn, steps, dt_over_tin=0.005 * 240 / Nx)
+            out.append((d, f"square {Nx}x{Ny} Ma {Ma:.2f} Kn {Kn:.4f}"))
+        else:
+            Fr = rayleigh_fr(Ra, 0.05)
+            # the buoyancy-driven start needs dt ~ 1/Nx^2 (1e-3 is the limit at Nx = 256)
+            rayleigh_benard(d, Nx, Ny, Kn=0.05, Fr=Fr, steps=steps, dt=8e-4 * (256.0 / Nx) ** 2)
+            out.append((d, f"rayleigh-benard {Nx}x{Ny} Ra {Ra:g} Fr {Fr:.4f}"))
+    return out
