@@ -227,6 +227,14 @@ int sts_batch_time_step(sts_batch *b, const sts_loop_control *lc, const sts_mixe
 int sts_batch_set_tiles(sts_batch *b, int cell_variant, int sweep_variant);
 int sts_batch_autotune(sts_batch *b, int reps, int *cell_variant, int *sweep_variant, double *ms);
 
+/* Drag-coefficient sums of body `body` (0-based, circles first, the
+   numbering of vol_inf_fb minus one), WriteDragCoefficient S:8203-8262: over
+   the body's cells of the p/T box in the reference's order, for the faces
+   bottom, front, top, behind (in that order) the friction integral, the
+   pressure integral and the face length -- sums[12].  The caller applies the
+   nondimensional factors of S:8265-8283 and writes CD.<k>.txt. */
+int sts_drag_sums(sts_ctx *ctx, int body, double *sums);
+
 /* The HIP stream all work of the context is issued on (diagnostics). */
 int sts_stream_info(sts_ctx *ctx, void **hip_stream);
 

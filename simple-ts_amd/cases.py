@@ -55,7 +55,7 @@ def write_circles(dst, circles):
             f.write(f"{int(b.get('jump', 0))}\t//is_TemperatureJumpBC_body\n")
             f.write(f"{fmt(b.get('F_jump', 2.1904))}\t//F_TemperatureJump\n")
             f.write(f"{fmt(b.get('dTdn', 0.0))}\t//dTdn_on_wall_0\n")
-            f.write("0\t//ToSolveDragCoefficientForThisBody\n")
+            f.write(f"{int(b.get('drag', 0))}\t//ToSolveDragCoefficientForThisBody\n")
 
 
 def write_case(dst, overrides=None, polyhedrons=None, template=SHIPPED, circles=None):
@@ -96,7 +96,7 @@ def write_case(dst, overrides=None, polyhedrons=None, template=SHIPPED, circles=
                 f.write(f"{int(b.get('jump', 0))}\t//is_TemperatureJumpBC_body\n")
                 f.write(f"{fmt(b.get('F_jump', 2.1904))}\t//F_TemperatureJump\n")
                 f.write(f"{fmt(b.get('dTdn', 0.0))}\t//dTdn_on_wall_0\n")
-                f.write("0\t//ToSolveDragCoefficientForThisBody\n")
+                f.write(f"{int(b.get('drag', 0))}\t//ToSolveDragCoefficientForThisBody\n")
                 for x, y in v:
                     f.write(f"{fmt(x)}\t{fmt(y)}\n")
     if circles:
@@ -111,7 +111,7 @@ def u_inlet(Re, Kn):
 
 
 def confined_square(dst, Nx, Ny, Re, Kn, steps, dt_over_tin=0.01, L=15.0, H=1.0, side=0.25, xc=4.0, yc=0.5,
-                    tol=1e-8, N_I=5000, N_I_p_c=2, extra=None, wall_slip=0, circles=None):
+                    tol=1e-8, N_I=5000, N_I_p_c=2, extra=None, wall_slip=0, circles=None, drag=0):
     """Flow past a confined square in a channel of length L and height H:
     square of side `side` centred at (xc, yc), isothermal no-slip walls (T=1),
     uniform inlet velocity u_in = u_inlet(Re, Kn) held on the first two u
@@ -133,7 +133,7 @@ def confined_square(dst, Nx, Ny, Re, Kn, steps, dt_over_tin=0.01, L=15.0, H=1.0,
     ]
     if side > 0:
         walls.append(dict(vertices=[(xc - s2, yc - s2), (xc + s2, yc - s2), (xc + s2, yc + s2), (xc - s2, yc + s2)],
-                          p1=(xc - 0.3 * s2, yc - 0.2 * s2), p2=(xc + 0.3 * s2, yc + 0.4 * s2)))
+                          p1=(xc - 0.3 * s2, yc - 0.2 * s2), p2=(xc + 0.3 * s2, yc + 0.4 * s2), drag=drag))
     ov = dict(u_gas_nd=u_in, v_gas_nd=0.0, T_gas_nd=1.0, p_gas_nd=1.0, Kn=Kn, Re=Re, TypeOfNonDimensiolization=3,
               Nt=steps, Nx=Nx, Ny=Ny, N_I=N_I, N_I_p_c=N_I_p_c, N_I_T=0, ht=dt_over_tin * H / u_in,
               x_b=0.0, x_e=L, y_b=-0.5 * hy, y_e=H + 1.5 * hy, kind_of_mesh=5, hxmin=hx, hxmax=hx, hymin=hy,

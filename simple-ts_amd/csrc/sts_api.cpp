@@ -70,6 +70,12 @@ struct sts_ctx {
   long long fluid_cells = 0;
   unsigned *wall_cells = nullptr;  // cells next to a body/fluid interface (k_wall_list)
   int n_wall_cells = 0;
+  // drag coefficients: host copy of vol_inf_fb, per-body cell lists, buffers
+  std::vector<uint32_t> h_vol;
+  std::vector<unsigned *> drag_cells;
+  std::vector<int> drag_n;
+  double *drag_terms = nullptr, *drag_out = nullptr;
+  size_t drag_cap = 0;
   double *p_ws = nullptr, *T_ws = nullptr;  // predicted starting guess (warm start)
   double *prev[4] = {};                      // u, v, p, T at the start of the previous step (extrapolation)
   bool have_prev = false;
@@ -195,6 +201,9 @@ extern "C" int sts_destroy(sts_ctx *c) {
   (void)hipFree(c->p_ws);
   for (double *q : c->prev) (void)hipFree(q);
   (void)hipFree(c->wall_cells);
+  for (unsigned *q : c->drag_cells) (void)hipFree(q);
+  (void)hipFree(c->drag_terms);
+  (void)hipFree(c->drag_out);
   (void)hipFree(c->T_ws);
   (void)hipFree(c->p_alt32);
   (void)hipFree(c->T_alt32);
@@ -321,6 +330,7 @@ extern "C" int sts_create(const sts_grid *gi, int precision, sts_ctx **out) {
     return fail(STS_ERR_UNSUPPORTED, "drhodt_durhodx_dvrhody edge condition on a fluid cell of the top row");
   sts_ctx *c = new sts_ctx();
   for (uint8_t b : mask) c->fluid_cells += b;
+  c->h_vol.assign(gi->vol_inf_fb, gi->vol_inf_fb + Na);
   c->Nx = gi->Nx;
   c->Ny = gi->Ny;
   c->Na = (int)Na;
@@ -1235,5 +1245,41 @@ extern "C" int sts_batch_autotune(sts_batch *b, int reps, int *cell_variant, int
       ms[v] = t_cell[v];
       ms[NVARIANT + v] = t_sweep[v];
     }
+  return STS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Drag coefficient sums (WriteDragCoefficient, S:8147-8430)
+// ---------------------------------------------------------------------------
+extern "C" int sts_drag_sums(sts_ctx *c, int body, double *sums) {
+  if (!c || !sums || body < 0 || body >= c->g.nbodies) return fail(STS_ERR_ARG, "bad body index");
+  if (c->drag_cells.empty()) {
+    c->drag_cells.assign(c->g.nbodies, nullptr);
+    c->drag_n.assign(c->g.nbodies, -1);
+  }
+  if (c->drag_n[body] < 0) {
+    // the body's cells of the p/T box in the reference's loop order (i outer, j inner)
+    std::vector<unsigned> cells;
+    const KGrid &g = c->g;
+    for (int i = g.Nx_o0; i < g.Nx_o1; i++)
+      for (int j = g.Ny_o0; j < g.Ny_o1; j++) {
+        const unsigned node = (unsigned)i + (unsigned)j * (unsigned)g.Nx;
+        if ((long long)c->h_vol[node] - 1 == body) cells.push_back(node);
+      }
+    HIP_OK(hipMalloc(&c->drag_cells[body], sizeof(unsigned) * (cells.empty() ? 1 : cells.size())));
+    HIP_OK(hipMemcpy(c->drag_cells[body], cells.data(), sizeof(unsigned) * cells.size(), hipMemcpyHostToDevice));
+    c->drag_n[body] = (int)cells.size();
+  }
+  const size_t need = 12 * (size_t)(c->drag_n[body] > 0 ? c->drag_n[body] : 1);
+  if (need > c->drag_cap) {
+    (void)hipFree(c->drag_terms);
+    c->drag_terms = nullptr;
+    HIP_OK(hipMalloc(&c->drag_terms, sizeof(double) * need));
+    c->drag_cap = need;
+  }
+  if (!c->drag_out) HIP_OK(hipMalloc(&c->drag_out, sizeof(double) * 12));
+  HIP_OK(launch_drag(c->g, c->fields(), c->drag_cells[body], c->drag_n[body], c->drag_terms, c->drag_out, c->stream));
+  HIP_OK(hipMemcpyAsync(sums, c->drag_out, sizeof(double) * 12, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
   return STS_OK;
 }

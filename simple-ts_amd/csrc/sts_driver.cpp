@@ -236,6 +236,66 @@ struct Driver {
       << "\t" << max_res_v0 << "\t" << max_res_v << std::endl;
   }
 
+  // WriteDragCoefficient, S:8147-8430: for every body marked
+  // ToSolveDragCoefficientForThisBody (circles first), the face sums from the
+  // GPU, the nondimensional factors of the case and one line of CD.<k>.txt.
+  // As in the reference, only the first file written in a run is truncated
+  // and given the header line.
+  bool cd_first = true;
+  void write_drag() {
+    const size_t nc = c.circles.size(), nb = nc + c.polys.size();
+    const unsigned type = (unsigned)c.get("TypeOfNonDimensiolization");
+    const bool restart = c.flag("ToReadSolvedDataFromFile");
+    for (size_t k = 0; k < nb; k++) {
+      if (!(k < nc ? c.circles[k].drag : c.polys[k - nc].drag)) continue;
+      double s[12];
+      ck(sts_drag_sums(ctx, (int)k, s), "sts_drag_sums");
+      double c_CD_p = 0, c_CD_fr = 0;
+      const double rho_b = p_BC_xb / c.get("T_BC_xb");
+      if (type == 1) {
+        c_CD_p = 1.0 / (u_given_xb_in_it * u_given_xb_in_it * rho_b);
+        c_CD_fr = 2.0 / (c.get("Re") * u_given_xb_in_it);
+      } else if (type == 2) {
+        c_CD_p = 2.0 / (c.get("gamma1") * c.get("Ma") * c.get("Ma") * rho_b);
+        c_CD_fr = 2.0 / (c.get("Re") * u_given_xb_in_it);
+      } else if (type == 3) {
+        c_CD_p = 1.0 / (u_given_xb_in_it * u_given_xb_in_it * rho_b);
+        c_CD_fr = (5.0 / 8.0) * sqrt(M_PI) * c.get("Kn") / (u_given_xb_in_it * u_given_xb_in_it * rho_b);
+      }
+      const double fr_bottom = c_CD_fr * s[0], p_bottom = c_CD_p * s[1], fr_front = c_CD_fr * s[3],
+                   p_front = c_CD_p * s[4], fr_top = c_CD_fr * s[6], p_top = c_CD_p * s[7],
+                   fr_behind = c_CD_fr * s[9], p_behind = c_CD_p * s[10];
+      const double S_bottom = s[2], S_front = s[5], S_top = s[8], S_behind = s[11];
+      const double CD_fr_x = (fr_top / S_top) + (fr_bottom / S_bottom);
+      const double CD_p_x = (p_front / S_front) - (p_behind / S_behind);
+      const std::string name = path("CD." + std::to_string(k) + ".txt");
+      std::ofstream o;
+      if (cd_first && !restart) o.open(name);
+      else o.open(name, std::ios::out | std::ios::app);
+      if (!o) {
+        std::cout << "The program can not open file " << "CD." << k << ".txt" << " to write data." << std::endl;
+        continue;
+      }
+      const int w = 30;
+      if (!restart && cd_first) {
+        for (const char *h : {"//it (number of step by time) 00", "ht (step by time) 01", "CD_fr_x (CD from friction) 02",
+                              "CD_p_x (CD from pressure) 03", "S_for_CD_bottom 04", "S_for_CD_front 05",
+                              "S_for_CD_top 06", "S_for_CD_behind 07", "CD_fr_bottom 08", "CD_p_bottom 09",
+                              "CD_fr_front 10", "CD_p_front 11", "CD_fr_top 12", "CD_p_top 13", "CD_fr_behind 14",
+                              "CD_p_behind 15"})
+          o << std::setw(w) << h;
+        o << std::endl;
+      }
+      o << std::setw(w) << it;
+      o << std::setiosflags(std::ios::scientific) << std::setprecision(10);
+      for (double x : {ht, CD_fr_x, CD_p_x, S_bottom, S_front, S_top, S_behind, fr_bottom, p_bottom, fr_front,
+                       p_front, fr_top, p_top, fr_behind, p_behind})
+        o << std::setw(w) << x;
+      o << std::endl;
+      cd_first = false;
+    }
+  }
+
   // WriteIterationsInLoopsToFile, S:5945-6006
   void write_iters() {
     std::ofstream o;
@@ -382,7 +442,8 @@ struct Driver {
     auto t_timed = std::chrono::steady_clock::now();
     long long outer_at_start = 0;
     if (warmup_steps == 0) start_timing(t_timed, outer_at_start);
-    int save_ctr = 0, temp_ctr = 0;
+    int save_ctr = 0, temp_ctr = 0, drag_ctr = 0;
+    const int Nt_drag = (int)c.get("Nt_save_DragCoefficient");
     const int Nt_save = (int)c.get("Nt_save_solved_data"), Nt_temp = (int)c.get("Nt_save_temp_data");
     const auto t0 = std::chrono::steady_clock::now();
     time_solve_start = time(nullptr);
@@ -495,6 +556,10 @@ struct Driver {
         if (c.flag("ToSolveContinuityEquation")) continuity();
         save_ctr = 0;
       }
+      if (++drag_ctr >= Nt_drag) {  // S:4653-4661
+        write_drag();
+        drag_ctr = 0;
+      }
       if (++temp_ctr >= Nt_temp) {
         write_tmp();
         write_iters();
@@ -515,6 +580,7 @@ struct Driver {
     c.at("max_time_for_solve") = (double)(time(nullptr) - time_solve_start) / 3600.0;
     write_solved();
     if (c.flag("ToSolveContinuityEquation")) continuity();
+    write_drag();  // S:4713
     write_tmp();
     loop_seconds = loop_s;
     ck(sts_destroy(ctx), "sts_destroy");

@@ -1083,6 +1083,57 @@ __global__ void __launch_bounds__(BX *BY) k_set_inflow(KGrid g, KFields fl, doub
   if (g.m[k]) FLD(U)[k] = u_in;
 }
 
+// WriteDragCoefficient (S:8203-8262): the friction and pressure terms of
+// the four faces of each listed body cell, 12 per cell in the order
+// (bottom, front, top, behind) x (friction, pressure, face length); a face
+// whose neighbour is not fluid contributes +0 (an exact no-op in the sums).
+__global__ void __launch_bounds__(256) k_drag_terms(KGrid g, KFields fl, const unsigned *cells, int n, double *terms) {
+  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (t >= n) return;
+  const unsigned Nx = (unsigned)g.Nx, Ny = (unsigned)g.Ny, k = cells[t];
+  const Nbr c(k % Nx, k / Nx, Nx, Ny);
+  const uint8_t *m = g.m;
+  const double *hx = g.hx, *hy = g.hy, *u = FLD(U), *v = FLD(V), *p = FLD(P);
+  const unsigned i = c.i, j = c.j;
+  const unsigned ij = c.ij, ij_1 = c.ij_1, ij1 = c.ij1, i_1j = c.i_1j, i1j = c.i1j;
+  const unsigned i1j_1 = c.i1j_1, i1j1 = c.i1j1, i_1j1 = c.i_1j1;
+  double *o = terms + 12 * (size_t)t;
+  for (int q = 0; q < 12; q++) o[q] = 0.0;
+  if (m[ij_1]) {
+    const double hb = hy[j - 1 * (j < (Ny - 1))];
+    o[0] = hx[i] * sqrt(FLD(TW_V)[ij]) * ((u[ij_1] - u[ij]) / hb + (u[i1j_1] - u[i1j]) / hb);
+    o[1] = p[ij_1] * hx[i];
+    o[2] = hx[i];
+  }
+  if (m[i_1j]) {
+    o[3] = hy[j] * sqrt(FLD(TW_U)[ij]) * ((v[i_1j] - v[ij]) / hx[c.i_1] + (v[i_1j1] - v[ij1]) / hx[c.i_1]);
+    o[4] = p[i_1j] * hy[j];
+    o[5] = hy[j];
+  }
+  if (m[ij1]) {
+    const double ht_ = hy[j + 1 * (0 < j)];
+    o[6] = hx[i] * sqrt(FLD(TW_V)[ij1]) * ((u[ij1] - u[ij]) / ht_ + (u[i1j1] - u[i1j]) / ht_);
+    o[7] = p[ij1] * hx[i];
+    o[8] = hx[i];
+  }
+  if (m[i1j]) {
+    o[9] = hy[j] * sqrt(FLD(TW_U)[i1j]) * ((v[i1j] - v[ij]) / hx[c.i1] + (v[i1j1] - v[ij1]) / hx[c.i1]);
+    o[10] = p[i1j] * hy[j];
+    o[11] = hy[j];
+  }
+}
+
+// The sums of k_drag_terms in the reference's cell order (one thread: the
+// order of the additions is the reference's, so the sums are bit-identical).
+__global__ void k_drag_sum(const double *terms, int n, double *out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s[12];
+  for (int q = 0; q < 12; q++) s[q] = 0.0;
+  for (int t = 0; t < n; t++)
+    for (int q = 0; q < 12; q++) s[q] += terms[12 * (size_t)t + q];
+  for (int q = 0; q < 12; q++) out[q] = s[q];
+}
+
 // Opens the loop-2 control of a time step and clears its residual log and
 // the slotted residual words.
 __global__ void k_loop_open(LoopCtrl *ctrl, unsigned long long *log, int n, unsigned long long *lred) {
@@ -1432,6 +1483,12 @@ template hipError_t launch_convert<double, float>(const KFieldsT<double> &, cons
                                                   unsigned long long, int, hipStream_t);
 template hipError_t launch_convert<float, double>(const KFieldsT<float> &, const KFieldsT<double> &,
                                                   unsigned long long, int, hipStream_t);
+hipError_t launch_drag(const KGrid &g, const KFields &f, const unsigned *cells, int n, double *terms, double *out,
+                       hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_drag_terms, dim3((n + 255) / 256), dim3(256), 0, s, g, f, cells, n, terms);
+  hipLaunchKernelGGL(k_drag_sum, dim3(1), dim3(64), 0, s, terms, n, out);
+  return hipGetLastError();
+}
 hipError_t launch_fold(unsigned long long *slots, int n, unsigned long long *out, hipStream_t s) {
   hipLaunchKernelGGL(k_fold, dim3(1), dim3(128), 0, s, slots, n, out);
   return hipGetLastError();

@@ -44,6 +44,8 @@ template <typename S, typename D>
 hipError_t launch_convert(const KFieldsT<S> &src, const KFieldsT<D> &dst, unsigned long long mask, int n,
                           hipStream_t s);
 hipError_t launch_fold(unsigned long long *slots, int n, unsigned long long *out, hipStream_t s);
+hipError_t launch_drag(const KGrid &g, const KFields &f, const unsigned *cells, int n, double *terms, double *out,
+                       hipStream_t s);
 int continuity_blocks(const KGrid &g);
 hipError_t launch_continuity(const KGrid &g, const KFields &f, double *cs, double *cf, double *partial,
                              hipStream_t s);

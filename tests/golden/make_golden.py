@@ -32,7 +32,7 @@ REF_DBG = os.path.join(ROOT, "oracle", "_ref", "sts_ref_dbg")
 
 KEEP = ["u.bin", "v.bin", "p.bin", "Temper.bin", "vol_inf_fb.bin", "x_f.txt", "y_f.txt", "hx.txt", "hy.txt",
         "x_v.txt", "y_v.txt", "Tmp.txt", "Iterations_in_loops.txt", "it_begin.txt", "EnteredData.txt",
-        "ContinuityEquation.bin", "ContinuityEquation_stationary.bin"]
+        "ContinuityEquation.bin", "ContinuityEquation_stationary.bin", "CD.0.txt", "CD.1.txt", "CD.2.txt", "CD.3.txt"]
 
 MESH_KEEP = ["vol_inf_fb.bin", "x_f.txt", "y_f.txt", "hx.txt", "hy.txt", "x_v.txt", "y_v.txt", "EnteredData.txt"]
 
@@ -58,6 +58,10 @@ def case_specs():
                                                   u_BC_xb_t_BC_e=0.2, Nt_save_solved_data=2)),
         # supersonic confined square
         "square_super": lambda d: cases.confined_square(d, 60, 14, 1415, 0.00283, 4),
+        # drag coefficients of a circle and the square (WriteDragCoefficient, S:8147-8430)
+        "square_cd": lambda d: cases.confined_square(
+            d, 60, 14, 85, 0.00194, 5, drag=1, extra=dict(Nt_save_DragCoefficient=2),
+            circles=[dict(x_c=8.0, y_c=0.5, r_c=0.2, drag=1)]),
         # heated box with gravity and temperature jump
         "rb_jump": lambda d: cases.rayleigh_benard(d, 24, 12, Fr=1.0, steps=3, N_I=40, tol=0.0, jump=True),
         # circle body in a slip channel, local Kn on the walls
