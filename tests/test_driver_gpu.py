@@ -84,3 +84,37 @@ def test_driver_odd_sweep_counts_match_live_reference(n_pc, tmp_path):
     run_driver(mine)
     bad = compare_dirs(mine, ref)
     assert not bad, f"files differing from the reference's: {bad}"
+
+
+def _set_entries(case_dir, **values):
+    """Rewrites entries of a case's EnteredData.txt (by the reference's line
+    order) in place."""
+    cases = importlib.import_module("simple-ts_amd.cases")
+    names = cases.entry_names()
+    path = os.path.join(case_dir, "EnteredData.txt")
+    lines = open(path).read().split("\n")
+    for k, name in enumerate(names):
+        if name in values:
+            val, rest = lines[k].split("\t", 1)
+            lines[k] = cases.fmt(values[name]) + "\t" + rest
+    open(path, "w").write("\n".join(lines))
+
+
+@pytest.mark.skipif(not have_reference(), reason="oracle/_ref/sts_ref not built")
+def test_driver_restart_matches_live_reference(tmp_path):
+    """Restart (ToReadSolvedDataFromFile, S:5734-5835, S:6762-6787): both
+    programs run 3 steps, then continue 2 more from the files they wrote;
+    every file of the continued runs is the reference's, byte for byte."""
+    cases = importlib.import_module("simple-ts_amd.cases")
+    kw = dict(extra=dict(Nt_save_solved_data=3, Nt_save_temp_data=1))
+    mine = cases.confined_square(str(tmp_path / "gpu"), 60, 14, 85, 0.00194, 3, **kw)
+    ref = cases.confined_square(str(tmp_path / "ref"), 60, 14, 85, 0.00194, 3, **kw)
+    run_reference(ref)
+    run_driver(mine)
+    assert not compare_dirs(mine, ref)
+    for d in (mine, ref):
+        _set_entries(d, ToReadSolvedDataFromFile=1, Nt=2)
+    run_reference(ref)
+    run_driver(mine)
+    bad = compare_dirs(mine, ref)
+    assert not bad, f"files differing from the reference's after the restart: {bad}"
