@@ -36,6 +36,9 @@ int sts_unet_get_grads(sts_unet *net, float *host);
 
 /* y[n][H][W][cout] = net(x[n][H][W][cin]). */
 int sts_unet_forward(sts_unet *net, int n, const float *x, float *y);
+/* The same outputs, bit for bit, without keeping what the backward pass
+   needs (the decoder reads its concatenated inputs in place). */
+int sts_unet_infer(sts_unet *net, int n, const float *x, float *y);
 /* After sts_unet_forward of the same batch: gradients (overwritten) of
    L = sum over cells with mask != 0 of |y - t|^2 / (2 * count * cout),
    count = number of such cells; *loss gets L. */

@@ -175,6 +175,7 @@ def lib():
             ("sts_unet_create", [c_int, c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(vp)]),
             ("sts_unet_destroy", [vp]), ("sts_unet_set_params", [vp, _FP]), ("sts_unet_get_params", [vp, _FP]),
             ("sts_unet_get_grads", [vp, _FP]), ("sts_unet_forward", [vp, c_int, _FP, _FP]),
+            ("sts_unet_infer", [vp, c_int, _FP, _FP]),
             ("sts_unet_backward_mse", [vp, c_int, _FP, _BP, _DP]), ("sts_unet_backward", [vp, c_int, _FP]),
             ("sts_unet_adam", [vp, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float]),
             ("sts_unet_dataset", [vp, c_int]),
@@ -434,6 +435,13 @@ class UNet:
         n = x.shape[0]
         y = np.zeros((n, self.H, self.W, self.cout), np.float32)
         _check(lib().sts_unet_forward(self.h, n, _fp(x), _fp(y)), "sts_unet_forward")
+        return y
+
+    def infer(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        n = x.shape[0]
+        y = np.zeros((n, self.H, self.W, self.cout), np.float32)
+        _check(lib().sts_unet_infer(self.h, n, _fp(x), _fp(y)), "sts_unet_infer")
         return y
 
     def backward_mse(self, t, mask):

@@ -111,11 +111,21 @@ int pack_weights(sts_unet *u) {
   return STS_OK;
 }
 
-int conv_fwd(sts_unet *u, int k, const u16 *X, int n, u16 *Y) {
+int conv_fwd(sts_unet *u, int k, const u16 *X, int n, u16 *Y, const u16 *skip = nullptr, int Cb = 0) {
   const Conv &c = u->L[k];
-  ConvArgs a{X, c.Cx, n, u->h(c.level), u->w(c.level), 3, c.Wt, c.Kp, u->P + c.b_off, c.Cout, Y, 1};
+  ConvArgs a{X, c.Cx, n, u->h(c.level), u->w(c.level), 3, c.Wt, c.Kp, u->P + c.b_off, c.Cout, Y, 1, skip, Cb};
   U_OK(unet_conv(a, u->stream));
   return STS_OK;
+}
+
+// Decoder conv on [nearest-up(half), skip]: materialised (training keeps the
+// concatenation for the backward pass) or read in place by the convolution.
+int conv_cat(sts_unet *u, int k, const u16 *half, int Cb, const u16 *skip, int Cs, int n, u16 *cat, bool keep,
+             u16 *Y) {
+  const int l = u->L[k].level;
+  if (!keep) return conv_fwd(u, k, half, n, Y, skip, Cb);
+  U_OK(unet_upcat(half, Cb, skip, Cs, n, u->h(l), u->w(l), cat, u->stream));
+  return conv_fwd(u, k, cat, n, Y);
 }
 
 // Backward of conv k + ReLU: g = dL/d(output) (BF16); writes the parameter
@@ -135,8 +145,9 @@ int conv_back(sts_unet *u, int k, const u16 *Xin, int n, const u16 *g, u16 *dX) 
   return STS_OK;
 }
 
-// Forward of the batch already in u->x; result in u->y.
-int forward_dev(sts_unet *u, int n) {
+// Forward of the batch already in u->x; result in u->y.  keep: materialise
+// the concatenations the backward pass reads.
+int forward_dev(sts_unet *u, int n, bool keep = true) {
   int rc;
   const int B = u->base;
   if ((rc = conv_fwd(u, E1A, u->x, n, u->act[E1A])) || (rc = conv_fwd(u, E1B, u->act[E1A], n, u->act[E1B])))
@@ -150,14 +161,14 @@ int forward_dev(sts_unet *u, int n) {
   U_OK(unet_pool(u->act[E3B], n, u->h(2), u->w(2), 4 * B, u->pl[2], u->stream));
   if ((rc = conv_fwd(u, BA, u->pl[2], n, u->act[BA])) || (rc = conv_fwd(u, BB, u->act[BA], n, u->act[BB])))
     return rc;
-  U_OK(unet_upcat(u->act[BB], 8 * B, u->act[E3B], 4 * B, n, u->h(2), u->w(2), u->cat[2], u->stream));
-  if ((rc = conv_fwd(u, D3A, u->cat[2], n, u->act[D3A])) || (rc = conv_fwd(u, D3B, u->act[D3A], n, u->act[D3B])))
+  if ((rc = conv_cat(u, D3A, u->act[BB], 8 * B, u->act[E3B], 4 * B, n, u->cat[2], keep, u->act[D3A])) ||
+      (rc = conv_fwd(u, D3B, u->act[D3A], n, u->act[D3B])))
     return rc;
-  U_OK(unet_upcat(u->act[D3B], 4 * B, u->act[E2B], 2 * B, n, u->h(1), u->w(1), u->cat[1], u->stream));
-  if ((rc = conv_fwd(u, D2A, u->cat[1], n, u->act[D2A])) || (rc = conv_fwd(u, D2B, u->act[D2A], n, u->act[D2B])))
+  if ((rc = conv_cat(u, D2A, u->act[D3B], 4 * B, u->act[E2B], 2 * B, n, u->cat[1], keep, u->act[D2A])) ||
+      (rc = conv_fwd(u, D2B, u->act[D2A], n, u->act[D2B])))
     return rc;
-  U_OK(unet_upcat(u->act[D2B], 2 * B, u->act[E1B], B, n, u->h(0), u->w(0), u->cat[0], u->stream));
-  if ((rc = conv_fwd(u, D1A, u->cat[0], n, u->act[D1A])) || (rc = conv_fwd(u, D1B, u->act[D1A], n, u->act[D1B])))
+  if ((rc = conv_cat(u, D1A, u->act[D2B], 2 * B, u->act[E1B], B, n, u->cat[0], keep, u->act[D1A])) ||
+      (rc = conv_fwd(u, D1B, u->act[D1A], n, u->act[D1B])))
     return rc;
   const Conv &hd = u->L[HEAD];
   U_OK(unet_head(u->act[D1B], B, hd.Wt, u->P + hd.b_off, u->cout, (int)u->npix(n, 0), u->y, u->stream));
@@ -347,6 +358,19 @@ extern "C" int sts_unet_forward(sts_unet *u, int n, const float *x, float *y) {
   return STS_OK;
 }
 
+extern "C" int sts_unet_infer(sts_unet *u, int n, const float *x, float *y) {
+  if (!u || !x || !y) return ufail(STS_ERR_ARG, "null argument");
+  int rc = check_n(u, n);
+  if (rc) return rc;
+  const size_t np = u->npix(n, 0);
+  U_OK(hipMemcpyAsync(u->xin, x, sizeof(float) * np * u->cin, hipMemcpyHostToDevice, u->stream));
+  U_OK(unet_to_bf16(u->xin, u->cin, u->Cx0, np, u->x, u->stream));
+  if ((rc = forward_dev(u, n, false))) return rc;
+  U_OK(hipMemcpyAsync(y, u->y, sizeof(float) * np * u->cout, hipMemcpyDeviceToHost, u->stream));
+  U_OK(hipStreamSynchronize(u->stream));
+  return STS_OK;
+}
+
 extern "C" int sts_unet_backward(sts_unet *u, int n, const float *dy) {
   if (!u || !dy) return ufail(STS_ERR_ARG, "null argument");
   int rc = check_n(u, n);
@@ -472,7 +496,7 @@ int sts_internal_unet_predict(sts_unet *u, const StsView &v, const sts_unet_norm
   const double *const *f = v.f;
   U_OK(unet_pack_state(f[STS_U], f[STS_V], f[STS_P], f[STS_T], v.mask, np, u->Cx0, 1.0 / nrm->u_ref, 1.0 / nrm->p_ref,
                        u->x, u->stream));
-  if ((rc = forward_dev(u, 1))) return rc;
+  if ((rc = forward_dev(u, 1, false))) return rc;
   U_OK(unet_apply_pred(u->y, f[STS_P], f[STS_T], v.mask, v.Nx, np, v.box_o, nrm->p_ref * nrm->dp_scale,
                        nrm->dT_scale, pw, Tw, u->stream));
   U_OK(hipStreamSynchronize(u->stream));

@@ -40,20 +40,24 @@ __device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, U1
 
 // ---------------------------------------------------------------------------
 // Convolution (forward, and backward-data with flipped weights)
-// Block: 4 waves; wave w computes pixels [p0, p0+32) x channels [c0, c0+32)
-// with p0 = (4 blockIdx.x + w) * 32, c0 = 32 blockIdx.y.
+// Block: 4 waves stacked along the pixels; a wave computes TM x TN tiles of
+// 16 pixels x 16 channels: pixels [p0, p0 + 16 TM) x channels
+// [c0, c0 + 16 TN), p0 = (4 blockIdx.x + w) * 16 TM, c0 = 16 TN blockIdx.y.
+// Per 32-wide k chunk a lane loads TM + TN 16-byte fragments for TM * TN
+// MFMAs; every output sums its k chunks in the same order for any TM, TN.
 // ---------------------------------------------------------------------------
+template <int TM, int TN>
 __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int r = lane & 15, g = lane >> 4;
   const int npix = a.N * a.H * a.W;
-  const int p0 = (blockIdx.x * 4 + wave) * 32, c0 = blockIdx.y * 32;
+  const int p0 = (blockIdx.x * 4 + wave) * (16 * TM), c0 = blockIdx.y * (16 * TN);
   if (p0 >= npix) return;
   const int rad = a.ks / 2, taps = a.ks * a.ks;
-  int pn[2], pj[2], pi[2];
-  bool pv[2];
+  int pn[TM], pj[TM], pi[TM];
+  bool pv[TM];
 #pragma unroll
-  for (int s = 0; s < 2; s++) {
+  for (int s = 0; s < TM; s++) {
     const int p = p0 + 16 * s + r;
     pv[s] = p < npix;
     const int q = pv[s] ? p : 0;
@@ -62,38 +66,48 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
     pj[s] = t % a.H;
     pn[s] = t / a.H;
   }
-  floatx4 acc[2][2];
+  floatx4 acc[TM][TN];
 #pragma unroll
-  for (int x = 0; x < 2; x++)
+  for (int x = 0; x < TM; x++)
 #pragma unroll
-    for (int y = 0; y < 2; y++) acc[x][y] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int y = 0; y < TN; y++) acc[x][y] = floatx4{0.f, 0.f, 0.f, 0.f};
   for (int kc = 0; kc < a.Kp; kc += 32) {
     const int k = kc + 8 * g;
     const int t = k / a.Cx, ci = k - t * a.Cx;
-    bf16x8 A[2], B[2];
+    const int dy = t / a.ks - rad, dx = t % a.ks - rad;
+    bf16x8 A[TM], B[TN];
 #pragma unroll
-    for (int s = 0; s < 2; s++) {
+    for (int s = 0; s < TM; s++) {
       A[s] = zero8();
       if (t < taps && pv[s]) {
-        const int jj = pj[s] + t / a.ks - rad, ii = pi[s] + t % a.ks - rad;
-        if (jj >= 0 && jj < a.H && ii >= 0 && ii < a.W)
-          A[s] = load8(a.X + ((size_t)(pn[s] * a.H + jj) * a.W + ii) * a.Cx + ci);
+        const int jj = pj[s] + dy, ii = pi[s] + dx;
+        if (jj >= 0 && jj < a.H && ii >= 0 && ii < a.W) {
+          if (!a.X2)
+            A[s] = load8(a.X + ((size_t)(pn[s] * a.H + jj) * a.W + ii) * a.Cx + ci);
+          else if (ci < a.Cb)  // nearest-upsampled half-size tensor (channels [0, Cb))
+            A[s] = load8(a.X + ((size_t)(pn[s] * (a.H / 2) + jj / 2) * (a.W / 2) + ii / 2) * a.Cb + ci);
+          else  // skip tensor (channels [Cb, Cx))
+            A[s] = load8(a.X2 + ((size_t)(pn[s] * a.H + jj) * a.W + ii) * (a.Cx - a.Cb) + (ci - a.Cb));
+        }
       }
+    }
+#pragma unroll
+    for (int s = 0; s < TN; s++) {
       const int co = c0 + 16 * s + r;
       B[s] = (co < a.Cout) ? load8(a.Wt + (size_t)co * a.Kp + k) : zero8();
     }
 #pragma unroll
-    for (int x = 0; x < 2; x++)
+    for (int x = 0; x < TM; x++)
 #pragma unroll
-      for (int y = 0; y < 2; y++) acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[x], B[y], acc[x][y], 0, 0, 0);
+      for (int y = 0; y < TN; y++) acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[x], B[y], acc[x][y], 0, 0, 0);
   }
 #pragma unroll
-  for (int y = 0; y < 2; y++) {
+  for (int y = 0; y < TN; y++) {
     const int co = c0 + 16 * y + r;
     if (co >= a.Cout) continue;
     const float bias = a.bias ? a.bias[co] : 0.f;
 #pragma unroll
-    for (int x = 0; x < 2; x++)
+    for (int x = 0; x < TM; x++)
 #pragma unroll
       for (int q = 0; q < 4; q++) {
         const int p = p0 + 16 * x + 4 * g + q;
@@ -244,11 +258,17 @@ __global__ void k_pool_back_add(const u16 *P, const u16 *Skip, int Cs, int off, 
 __global__ void k_head(const u16 *X, int C, const u16 *Wb, const float *b, int Co, int npix, float *Y) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= npix) return;
-  for (int o = 0; o < Co; o++) {
-    float s = 0.f;
-    for (int c = 0; c < C; c++) s += bf2f(X[(size_t)p * C + c]) * bf2f(Wb[c * Co + o]);
-    Y[(size_t)p * Co + o] = s + b[o];
+  float s[8];
+  for (int o = 0; o < Co; o++) s[o] = 0.f;
+  for (int c0 = 0; c0 < C; c0 += 8) {  // C is a multiple of 8: 16-byte loads
+    const U16x8 x = *reinterpret_cast<const U16x8 *>(X + (size_t)p * C + c0);
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const float xv = bf2f(x.v[q]);
+      for (int o = 0; o < Co; o++) s[o] += xv * bf2f(Wb[(c0 + q) * Co + o]);
+    }
   }
+  for (int o = 0; o < Co; o++) Y[(size_t)p * Co + o] = s[o] + b[o];
 }
 
 // Head backward: dX[p][c] = round(sum_o dY[p][o] Wb[c][o]); dW[c][o] and
@@ -415,7 +435,11 @@ static inline unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1
 
 hipError_t unet_conv(const ConvArgs &a, hipStream_t s) {
   const size_t npix = (size_t)a.N * a.H * a.W;
-  hipLaunchKernelGGL(k_conv, dim3(nblk(npix, 128), nblk(a.Cout, 32)), dim3(256), 0, s, a);
+  // 4 pixel tiles per wave; 4 channel tiles where there are 64+ channels
+  if (a.Cout >= 64)
+    hipLaunchKernelGGL((k_conv<4, 4>), dim3(nblk(npix, 256), nblk(a.Cout, 64)), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_conv<4, 2>), dim3(nblk(npix, 256), nblk(a.Cout, 32)), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 hipError_t unet_wgrad(const WgradArgs &a, hipStream_t s) {

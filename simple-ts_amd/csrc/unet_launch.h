@@ -16,6 +16,11 @@ struct ConvArgs {
   int Cout;
   uint16_t *Y;  // [N][H][W][Cout], BF16
   int relu;
+  // Optional concatenated input read in place: channels [0, Cb) are X at
+  // half resolution, nearest-upsampled; channels [Cb, Cx) are X2 (stride
+  // Cx - Cb).  X2 == null: X is one tensor of stride Cx.
+  const uint16_t *X2 = nullptr;
+  int Cb = 0;
 };
 
 // Weight gradient dW[t][ci][co] += sum_p X[p+off(t)][ci] dZ[p][co].

@@ -74,6 +74,16 @@ def test_forward_matches_oracle(base):
 
 
 @pytest.mark.parametrize("base", [8, 32])
+def test_inference_equals_forward(base):
+    """The inference schedule (concatenations read in place) gives the
+    forward pass's outputs bit for bit."""
+    N, H, W = 2, 16, 64
+    P, x, _, _ = problem(base, N, H, W, 7)
+    net = net_for(base, N, H, W, P)
+    assert np.array_equal(net.infer(x), net.forward(x))
+
+
+@pytest.mark.parametrize("base", [8, 32])
 def test_backward_matches_oracle(base):
     N, H, W = 2, 16, 64
     P, x, t, m = problem(base, N, H, W, 2)
