@@ -63,6 +63,10 @@ typedef struct sts_grid {
   int dvdx_0_xb, dvdx_0_xe, dvdy_0_yb, dvdy_0_ye;
   int dpdx_0_xb, dpdx_0_xe, dpdy_0_yb, dpdy_0_ye;
   int dTdx_0_xb, dTdx_0_xe, dTdy_0_yb, dTdy_0_ye;
+  /* Periodic_boundary_conditions_about_OX: x neighbours wrap (S:201-232),
+     the x solve boxes span 0..Nx (S:6055-6102) and the box-edge conditions
+     are off (S:6795, S:7340, S:7436, S:7655). */
+  int periodic_x;
 } sts_grid;
 
 /* Field identifiers, the reference's array names (S:245-1792). */

@@ -2,8 +2,7 @@
 stages (test infrastructure only).
 
 Setup sequence S:2476-3226 and loops 1-3 S:3230-4691 of SIMPLE_TS_MPI.cpp for
-the options the time-step library provides (non-periodic, no exact-solution
-start, p_BC_xb_correction_method 0, no pressure-ratio correction).  Mesh, mask
+the options the time-step library provides (no exact-solution start, p_BC_xb_correction_method 0, no pressure-ratio correction).  Mesh, mask
 and boxes come from the host setup library (simple_ts_host.h), which is
 checked separately against the reference's own mesh and mask files.
 """
@@ -64,6 +63,8 @@ def run_case(case_dir, max_steps=None, on_step=None):
 
     def pressure_begin():
         nonlocal u_given_xb, u_in_it, p_BC_xb
+        if prob.periodic_x:  # S:7436
+            return
         if V("GivenReKn"):
             u_given_xb = V("Re") * V("Kn") * math.sqrt(15.0 * math.pi / 128.0)
         u_in_it = u_given_xb
@@ -148,7 +149,7 @@ def run_case(case_dir, max_steps=None, on_step=None):
                 or es < rpt[0][1])
             if not cont_iter:
                 break
-        if V("use_interpolation_for_u_BC_by_time"):  # S:8102-8144
+        if V("use_interpolation_for_u_BC_by_time") and not prob.periodic_x:  # S:8102-8144
             tb, te, ub, ue = V("t_BC_b"), V("t_BC_e"), V("u_BC_xb_t_BC_b"), V("u_BC_xb_t_BC_e")
             if it <= tb:
                 ug = ub

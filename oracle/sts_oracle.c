@@ -16,30 +16,43 @@
 #define LI(f1, f2, h1, h2) (((f1) * (h2) + (f2) * (h1)) / ((h1) + (h2)))
 #define HI(f1, f2, h1, h2) (((h1) + (h2)) * (f1) * (f2) / ((h1) * (f2) + (h2) * (f1)))
 
-/* Clamped neighbour indices, S:140-153 (ij_function). */
+/* Neighbour indices: clamped, S:140-153 (ij_function), or wrapping in x,
+   S:201-232 (ij_function_PeriodicBC; j stays clamped). */
 typedef struct {
   unsigned i, j, ij, i_1, i_2, i1, i_1j, i1j, ij_1, ij1, i_1j_1, i_1j1, i1j_1, i1j1, i_2j;
 } nb_t;
 
-static void nbr(const ora_grid *g, unsigned i, unsigned j, nb_t *n) {
+static void nbr_px(const ora_grid *g, unsigned i, unsigned j, int px, nb_t *n) {
   const unsigned Nx = (unsigned)g->Nx, Ny = (unsigned)g->Ny;
   unsigned ij = i + j * Nx;
   n->i = i;
   n->j = j;
   n->ij = ij;
-  n->i_1 = i - 1 * (0 < i);
-  n->i_2 = i - 1 * (0 < i) - 1 * (1 < i);
-  n->i1 = i + 1 * (i < (Nx - 1));
-  n->i_1j = ij - 1 * (0 < i);
-  n->i1j = ij + 1 * (i < (Nx - 1));
-  n->ij_1 = ij - Nx * (0 < j);
-  n->ij1 = ij + Nx * (j < (Ny - 1));
-  n->i_1j_1 = ij - 1 * (0 < i) - Nx * (0 < j);
-  n->i_1j1 = ij - 1 * (0 < i) + Nx * (j < (Ny - 1));
-  n->i1j_1 = ij + 1 * (i < (Nx - 1)) - Nx * (0 < j);
-  n->i1j1 = ij + 1 * (i < (Nx - 1)) + Nx * (j < (Ny - 1));
+  if (px) {
+    n->i_1 = (i == 0) ? Nx - 1 : i - 1;
+    n->i_2 = (i == 1) ? Nx - 1 : (i == 0) ? Nx - 2 : i - 2;
+    n->i1 = (i == Nx - 1) ? 0 : i + 1;
+  } else {
+    n->i_1 = i - 1 * (0 < i);
+    n->i_2 = i - 1 * (0 < i) - 1 * (1 < i);
+    n->i1 = i + 1 * (i < (Nx - 1));
+  }
+  const unsigned js = j - 1 * (0 < j), jn = j + 1 * (j < (Ny - 1));
+  n->i_1j = n->i_1 + j * Nx;
+  n->i1j = n->i1 + j * Nx;
+  n->ij_1 = i + js * Nx;
+  n->ij1 = i + jn * Nx;
+  n->i_1j_1 = n->i_1 + js * Nx;
+  n->i_1j1 = n->i_1 + jn * Nx;
+  n->i1j_1 = n->i1 + js * Nx;
+  n->i1j1 = n->i1 + jn * Nx;
   n->i_2j = n->i_2 + j * Nx;
 }
+
+/* The hot-path call sites take the wrapping form when the case is periodic
+   about OX (S:3276, S:3381, S:3709, S:3813, S:4180, S:4224, S:7112, S:7230,
+   S:7752, S:8058, S:10174-10354, S:10390, S:10465). */
+static void nbr(const ora_grid *g, unsigned i, unsigned j, nb_t *n) { nbr_px(g, i, j, g->periodic_x, n); }
 
 /* Body parameters of vol_inf_fb value k >= 1 (S:7114-7130). */
 static const ora_body *body_of(const ora_grid *g, unsigned k) {
@@ -398,6 +411,7 @@ void ora_bc_slip_v(const ora_grid *g, ora_fields *f) {
 
 /* S:6790-7096: edge copies and inflow/outflow mass conditions for u and v. */
 void ora_bc_velocities(const ora_grid *g, ora_fields *f) {
+  if (g->periodic_x) return; /* S:6795 */
   const unsigned char *m = g->m;
   const unsigned Nx = (unsigned)g->Nx, Ny = (unsigned)g->Ny;
   const double *hx = g->hx, *hy = g->hy;
@@ -588,12 +602,14 @@ static void edge_copy_y(const ora_grid *g, double *a, int b0, int b1, int lo_on,
 
 /* S:7650-7741 */
 void ora_bc_temperature(const ora_grid *g, ora_fields *f) {
+  if (g->periodic_x) return; /* S:7655 */
   edge_copy_x(g, f->T, g->Nx_o[0], g->Nx_o[1], g->dTdx_0_xb, g->dTdx_0_xe);
   edge_copy_y(g, f->T, g->Ny_o[0], g->Ny_o[1], g->dTdy_0_yb, g->dTdy_0_ye);
 }
 
 /* S:7335-7428 */
 void ora_bc_pressure(const ora_grid *g, ora_fields *f) {
+  if (g->periodic_x) return; /* S:7340 */
   edge_copy_x(g, f->p, g->Nx_o[0], g->Nx_o[1], g->dpdx_0_xb, g->dpdx_0_xe);
   edge_copy_y(g, f->p, g->Ny_o[0], g->Ny_o[1], g->dpdy_0_yb, g->dpdy_0_ye);
 }
@@ -666,14 +682,14 @@ void ora_diffusion(const ora_grid *g, ora_fields *f) {
     }
 }
 
-/* S:6470-6584 */
+/* S:6470-6584 (clamped indices in every case, S:6479) */
 void ora_apply_bodies(const ora_grid *g, ora_fields *f) {
   const unsigned *vol = g->vol;
   unsigned i, j;
   for (i = 0; i < (unsigned)g->Nx; i++)
     for (j = 0; j < (unsigned)g->Ny; j++) {
       nb_t n;
-      nbr(g, i, j, &n);
+      nbr_px(g, i, j, 0, &n);
       const ora_body *b = body_of(g, vol[n.ij]);
       if (!b) continue;
       f->u[n.ij] = b->u_body;
@@ -682,7 +698,7 @@ void ora_apply_bodies(const ora_grid *g, ora_fields *f) {
   for (i = 0; i < (unsigned)g->Nx; i++)
     for (j = 0; j < (unsigned)g->Ny; j++) {
       nb_t n;
-      nbr(g, i, j, &n);
+      nbr_px(g, i, j, 0, &n);
       const ora_body *b = body_of(g, vol[n.ij]);
       if (!b) continue;
       f->v[n.ij] = b->v_body;

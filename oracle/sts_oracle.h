@@ -46,6 +46,7 @@ typedef struct ora_grid {
   int dvdx_0_xb, dvdx_0_xe, dvdy_0_yb, dvdy_0_ye;
   int dpdx_0_xb, dpdx_0_xe, dpdy_0_yb, dpdy_0_ye;
   int dTdx_0_xb, dTdx_0_xe, dTdy_0_yb, dTdy_0_ye;
+  int periodic_x;                   /* Periodic_boundary_conditions_about_OX */
 } ora_grid;
 
 typedef struct ora_fields {

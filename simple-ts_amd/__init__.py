@@ -66,6 +66,7 @@ def grid_fields(with_mask):
     f += [(n, ctypes.c_double) for n in ("ht", "gamma1", "Kn", "cT", "cT1", "cT2", "cPch", "c_Fr_v")]
     f += [(n, ctypes.c_int) for n in ("given_xb", "given_xe", "kn_local")]
     f += [(n, ctypes.c_int) for n in BC_FLAGS]
+    f.append(("periodic_x", ctypes.c_int))
     return f
 
 
@@ -97,7 +98,7 @@ class Problem:
     SCALARS = ("ht", "gamma1", "Kn", "cT", "cT1", "cT2", "cPch", "c_Fr_v")
 
     def __init__(self, hx, hy, vol_inf_fb, bodies, boxes, scalars, given_xb=0, given_xe=0, kn_local=0,
-                 bc=None):
+                 bc=None, periodic_x=0):
         self.hx = np.ascontiguousarray(hx, dtype=np.float64)
         self.hy = np.ascontiguousarray(hy, dtype=np.float64)
         self.Nx, self.Ny = self.hx.size, self.hy.size
@@ -109,6 +110,7 @@ class Problem:
         self.boxes = {k: tuple(int(x) for x in v) for k, v in boxes.items()}
         self.scalars = dict(scalars)
         self.given_xb, self.given_xe, self.kn_local = int(given_xb), int(given_xe), int(kn_local)
+        self.periodic_x = int(periodic_x)
         self.bc = {k: 0 for k in BC_FLAGS}
         if bc:
             for k, v in bc.items():
@@ -138,6 +140,7 @@ class Problem:
         for k in self.SCALARS:
             setattr(st, k, float(self.scalars[k]))
         st.given_xb, st.given_xe, st.kn_local = self.given_xb, self.given_xe, self.kn_local
+        st.periodic_x = self.periodic_x
         for k, v in self.bc.items():
             setattr(st, k, v)
         return st
@@ -536,7 +539,7 @@ class Case:
                             dTdn_on_wall_0=g.bodies[k].dTdn_on_wall_0) for k in range(g.nbodies)]
         self.boxes = {k: tuple(getattr(g, k)) for k in ("Nx_u", "Ny_u", "Nx_v", "Ny_v", "Nx_o", "Ny_o")}
         self.scalars = {k: getattr(g, k) for k in Problem.SCALARS}
-        self.flags = dict(given_xb=g.given_xb, given_xe=g.given_xe, kn_local=g.kn_local)
+        self.flags = dict(given_xb=g.given_xb, given_xe=g.given_xe, kn_local=g.kn_local, periodic_x=g.periodic_x)
         self.bc = {k: getattr(g, k) for k in BC_FLAGS}
 
     def value(self, name):

@@ -150,12 +150,16 @@ def confined_square(dst, Nx, Ny, Re, Kn, steps, dt_over_tin=0.01, L=15.0, H=1.0,
 
 
 def rayleigh_benard(dst, Nx, Ny, Kn=0.05, Fr=1.0, steps=10, dt=0.01, W=2.0, H=1.0, T_hot=2.0, T_cold=1.0,
-                    tol=1e-8, N_I=5000, N_I_p_c=2, jump=False, extra=None):
+                    tol=1e-8, N_I=5000, N_I_p_c=2, jump=False, extra=None, periodic=False, circles=None):
     """Closed box W*H of gas at rest between a hot bottom wall (T_hot) and a
     cold top wall (T_cold), side walls at the mean temperature, gravity along
     -y with c_Fr_v = 2/(gamma*Fr*Ma^2) (S:2066).  As in confined_square, the
-    fluid occupies columns 1..Nx-3 and rows 1..Ny-3 of the Nx*Ny array."""
-    hx, hy = W / (Nx - 3), H / (Ny - 3)
+    fluid occupies columns 1..Nx-3 and rows 1..Ny-3 of the Nx*Ny array.
+
+    periodic: no side walls; the layer is periodic about OX with period W
+    over all Nx columns (cell centres at (i + 1/2) W/Nx).  circles: extra
+    bodies (write_circles dicts), e.g. a heated cylinder."""
+    hx, hy = (W / Nx if periodic else W / (Nx - 3)), H / (Ny - 3)
     Tm = 0.5 * (T_hot + T_cold)
     big = 1e5
     walls = [
@@ -163,21 +167,77 @@ def rayleigh_benard(dst, Nx, Ny, Kn=0.05, Fr=1.0, steps=10, dt=0.01, W=2.0, H=1.
              T_body=T_hot, jump=int(jump)),
         dict(vertices=[(-big, H), (big, H), (big, big), (-big, big)], p1=(0.0, 5e4), p2=(1.0, 6e4), T_body=T_cold,
              jump=int(jump)),
-        dict(vertices=[(-big, 0.0), (0.0, 0.0), (0.0, H), (-big, H)], p1=(-5e4, 0.3 * H), p2=(-6e4, 0.6 * H),
-             T_body=Tm, jump=int(jump)),
-        dict(vertices=[(W, 0.0), (big, 0.0), (big, H), (W, H)], p1=(5e4, 0.3 * H), p2=(6e4, 0.6 * H), T_body=Tm,
-             jump=int(jump)),
     ]
+    if not periodic:
+        walls += [
+            dict(vertices=[(-big, 0.0), (0.0, 0.0), (0.0, H), (-big, H)], p1=(-5e4, 0.3 * H), p2=(-6e4, 0.6 * H),
+                 T_body=Tm, jump=int(jump)),
+            dict(vertices=[(W, 0.0), (big, 0.0), (big, H), (W, H)], p1=(5e4, 0.3 * H), p2=(6e4, 0.6 * H),
+                 T_body=Tm, jump=int(jump)),
+        ]
+    x_b, x_e = (0.0, W) if periodic else (-0.5 * hx, W + 1.5 * hx)
     ov = dict(u_gas_nd=0.0, v_gas_nd=0.0, T_gas_nd=Tm, p_gas_nd=1.0, Kn=Kn, Fr=Fr, SolveGravityField=1,
               TypeOfNonDimensiolization=3, Nt=steps, Nx=Nx, Ny=Ny, N_I=N_I, N_I_p_c=N_I_p_c, N_I_T=0, ht=dt,
-              x_b=-0.5 * hx, x_e=W + 1.5 * hx, y_b=-0.5 * hy, y_e=H + 1.5 * hy, kind_of_mesh=5, hxmin=hx, hxmax=hx,
+              x_b=x_b, x_e=x_e, y_b=-0.5 * hy, y_e=H + 1.5 * hy, kind_of_mesh=5, hxmin=hx, hxmax=hx,
               hymin=hy, hymax=hy, MaxError_Velocities=tol, MaxError_p_c=tol, MaxError_T=tol,
               MaxError_to_stop_program=1e-14, max_time_for_solve=1000.0, Nt_save_solved_data=10 ** 9,
               Nt_save_DragCoefficient=10 ** 9, Nt_save_temp_data=1, Given_velocity_on_xb=0, Given_velocity_on_xe=0,
               Pressure_BC=0, GivenReKn=0, drhodt_durhodx_dvrhody_0_BC_xb=0, drhodt_durhodx_dvrhody_0_BC_xe=0,
-              dvdx_0_BC_xb=0, dvdx_0_BC_xe=0, dTdx_0_BC_xe=0, To_Use_Kn_local_in_wall_BC=1, N_SubDomains_x=1)
+              dvdx_0_BC_xb=0, dvdx_0_BC_xe=0, dTdx_0_BC_xe=0, To_Use_Kn_local_in_wall_BC=1, N_SubDomains_x=1,
+              Periodic_boundary_conditions_about_OX=int(periodic))
+    if circles:
+        ov["ToImport_data_for_circles_from_file_b"] = 1
     ov.update(extra or {})
-    return write_case(dst, ov, walls)
+    return write_case(dst, ov, walls, circles=circles)
+
+
+def periodic_channel(dst, Nx, Ny, u0, Kn, steps, dt=0.01, W=2.0, H=1.0, tol=1e-8, N_I=5000, N_I_p_c=2,
+                     slip=1, jump=1, seam_block=True, circles=None, top_wall=True, extra=None):
+    """Channel periodic about OX (period W over all Nx columns) between
+    walls at y = 0 and y = H (T = 1, velocity slip and temperature jump
+    switchable), gas starting at a uniform velocity u0 that the walls and
+    bodies slow down.  seam_block: staggered blocks at the periodic seam
+    x = 0 = W: one from the seam to x = 0.15 W over 0.4-0.8 H, one over
+    0.2-0.6 H that ends one column short of the seam, and one over 0.08-0.2 H
+    that reaches the seam from the left, so that across the seam every
+    fluid/body combination of the last two and first two columns occurs.
+    top_wall=False leaves the top rows fluid and asks for zero-gradient
+    conditions there, which the reference switches off for periodic cases
+    (S:6795, S:7340, S:7655): the top rows then keep their initial state."""
+    hx, hy = W / Nx, H / (Ny - 3)
+    big = 1e5
+    walls = [
+        dict(vertices=[(-big, -big), (big, -big), (big, 0.0), (-big, 0.0)], p1=(0.0, -5e4), p2=(1.0, -6e4),
+             slip=slip, jump=jump),
+    ]
+    if top_wall:
+        walls.append(dict(vertices=[(-big, H), (big, H), (big, big), (-big, big)], p1=(0.0, 5e4), p2=(1.0, 6e4),
+                          slip=slip, jump=jump, T_body=1.0))
+    if seam_block:
+        w, xr = 0.15 * W, W - 1.2 * hx
+        walls += [
+            dict(vertices=[(-1.0, 0.4 * H), (w, 0.4 * H), (w, 0.8 * H), (-1.0, 0.8 * H)], p1=(0.2 * w, 0.5 * H),
+                 p2=(0.3 * w, 0.7 * H), slip=slip, jump=jump, T_body=1.3),
+            dict(vertices=[(xr - w, 0.2 * H), (xr, 0.2 * H), (xr, 0.6 * H), (xr - w, 0.6 * H)],
+                 p1=(xr - 0.5 * w, 0.3 * H), p2=(xr - 0.4 * w, 0.5 * H), slip=slip, jump=jump, T_body=0.8),
+            dict(vertices=[(W - 3.5 * hx, 0.08 * H), (W + 1.0, 0.08 * H), (W + 1.0, 0.2 * H), (W - 3.5 * hx, 0.2 * H)],
+                 p1=(W - 2.0 * hx, 0.1 * H), p2=(W - 1.0 * hx, 0.15 * H), slip=slip, jump=jump, T_body=1.1),
+        ]
+    ov = dict(u_gas_nd=u0, v_gas_nd=0.0, T_gas_nd=1.0, p_gas_nd=1.0, Kn=Kn, TypeOfNonDimensiolization=3,
+              Nt=steps, Nx=Nx, Ny=Ny, N_I=N_I, N_I_p_c=N_I_p_c, N_I_T=0, ht=dt, x_b=0.0, x_e=W, y_b=-0.5 * hy,
+              y_e=H + 1.5 * hy, kind_of_mesh=5, hxmin=hx, hxmax=hx, hymin=hy, hymax=hy, MaxError_Velocities=tol,
+              MaxError_p_c=tol, MaxError_T=tol, MaxError_to_stop_program=1e-14, max_time_for_solve=1000.0,
+              Nt_save_solved_data=10 ** 9, Nt_save_DragCoefficient=10 ** 9, Nt_save_temp_data=1,
+              Given_velocity_on_xb=0, Given_velocity_on_xe=0, Pressure_BC=0, GivenReKn=0, SolveGravityField=0,
+              drhodt_durhodx_dvrhody_0_BC_xb=0, drhodt_durhodx_dvrhody_0_BC_xe=0, dudx_0_BC_xe=0, dvdx_0_BC_xb=0,
+              dvdx_0_BC_xe=0, dTdx_0_BC_xe=0, dpdx_0_BC_xb=0, To_Use_Kn_local_in_wall_BC=1, N_SubDomains_x=1,
+              Periodic_boundary_conditions_about_OX=1)
+    if not top_wall:
+        ov.update(dudy_0_BC_ye=1, dvdy_0_BC_ye=1, dpdy_0_BC_ye=1, dTdy_0_BC_ye=1, dudx_0_BC_xe=1, dTdx_0_BC_xe=1)
+    if circles:
+        ov["ToImport_data_for_circles_from_file_b"] = 1
+    ov.update(extra or {})
+    return write_case(dst, ov, walls, circles=circles)
 
 
 # Mode-3 constants (S:2013-2071): gamma = 5/3, Ma = sqrt(6/5), Pr = 2/3,

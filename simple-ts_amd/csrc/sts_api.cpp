@@ -185,7 +185,12 @@ static unsigned bc_bits(const sts_grid *g) {
 // Solve boxes must leave the first and last column/row out, as every
 // non-periodic box of define_area_for_solving (S:6009-6267) does: the
 // stencils then stay inside the grid with the reference's clamped indices.
-static int check_box(const char *name, const int *b, int N) {
+// Periodic about OX, the x boxes span every column (S:6055-6102).
+static int check_box(const char *name, const int *b, int N, bool wrap = false) {
+  if (wrap) {
+    if (b[0] != 0 || b[1] != N) return fail(STS_ERR_ARG, std::string("periodic solve box ") + name + " must be 0..N");
+    return STS_OK;
+  }
   if (b[0] < 1 || b[1] > N - 1 || b[0] >= b[1])
     return fail(STS_ERR_ARG, std::string("solve box ") + name + " must satisfy 1 <= lo < hi <= N-1");
   return STS_OK;
@@ -281,6 +286,7 @@ static void narrow_grid(sts_ctx *c) {
   h.cT1 = (float)g.cT1; h.cT2 = (float)g.cT2; h.cPch = (float)g.cPch; h.c_Fr_v = (float)g.c_Fr_v;
   h.given_xb = g.given_xb; h.given_xe = g.given_xe; h.kn_local = g.kn_local;
   h.bc = g.bc;
+  h.px = g.px;
 }
 
 static int alloc_fp32(sts_ctx *c, const sts_grid *gi) {
@@ -309,10 +315,12 @@ extern "C" int sts_create(const sts_grid *gi, int precision, sts_ctx **out) {
     return fail(STS_ERR_ARG, "grid needs Nx, Ny >= 3, hx, hy and vol_inf_fb");
   if (gi->nbodies < 0 || (gi->nbodies > 0 && !gi->bodies)) return fail(STS_ERR_ARG, "bad body table");
   int rc;
-  if ((rc = check_box("Nx_u", gi->Nx_u, gi->Nx)) || (rc = check_box("Ny_u", gi->Ny_u, gi->Ny)) ||
-      (rc = check_box("Nx_v", gi->Nx_v, gi->Nx)) || (rc = check_box("Ny_v", gi->Ny_v, gi->Ny)) ||
-      (rc = check_box("Nx_o", gi->Nx_o, gi->Nx)) || (rc = check_box("Ny_o", gi->Ny_o, gi->Ny)))
+  const bool px = gi->periodic_x != 0;
+  if ((rc = check_box("Nx_u", gi->Nx_u, gi->Nx, px)) || (rc = check_box("Ny_u", gi->Ny_u, gi->Ny)) ||
+      (rc = check_box("Nx_v", gi->Nx_v, gi->Nx, px)) || (rc = check_box("Ny_v", gi->Ny_v, gi->Ny)) ||
+      (rc = check_box("Nx_o", gi->Nx_o, gi->Nx, px)) || (rc = check_box("Ny_o", gi->Ny_o, gi->Ny)))
     return rc;
+  if (px && gi->Nx < 4) return fail(STS_ERR_ARG, "periodic grid needs Nx >= 4");
   const long long Na = (long long)gi->Nx * gi->Ny;
   if (Na > (1ll << 31) - 1) return fail(STS_ERR_ARG, "grid too large for 32-bit node indices");
   std::vector<uint8_t> mask((size_t)Na);
@@ -341,20 +349,23 @@ extern "C" int sts_create(const sts_grid *gi, int precision, sts_ctx **out) {
     return arc;
   }
   {
-    // the wall conditions write only at cells k with k, k-1 or k-Nx on a
-    // body/fluid interface (a cell with a 4-neighbour of the other kind)
+    // the wall conditions write only at cells k with k, its west neighbour
+    // or k-Nx on a body/fluid interface (a cell with a 4-neighbour of the
+    // other kind); periodic about OX, west and east wrap
     const int Nx = gi->Nx, Ny = gi->Ny;
+    auto west = [&](int i) { return i > 0 ? i - 1 : (px ? Nx - 1 : -1); };
+    auto east = [&](int i) { return i + 1 < Nx ? i + 1 : (px ? 0 : -1); };
     auto iface = [&](int i, int j) {
       const uint8_t a = mask[(size_t)i + (size_t)j * Nx];
-      return (i > 0 && mask[(size_t)i - 1 + (size_t)j * Nx] != a) ||
-             (i + 1 < Nx && mask[(size_t)i + 1 + (size_t)j * Nx] != a) ||
+      const int w = west(i), e = east(i);
+      return (w >= 0 && mask[(size_t)w + (size_t)j * Nx] != a) || (e >= 0 && mask[(size_t)e + (size_t)j * Nx] != a) ||
              (j > 0 && mask[(size_t)i + (size_t)(j - 1) * Nx] != a) ||
              (j + 1 < Ny && mask[(size_t)i + (size_t)(j + 1) * Nx] != a);
     };
     std::vector<unsigned> wl;
     for (int j = 0; j < Ny; j++)
       for (int i = 0; i < Nx; i++)
-        if (iface(i, j) || (i > 0 && iface(i - 1, j)) || (j > 0 && iface(i, j - 1)))
+        if (iface(i, j) || (west(i) >= 0 && iface(west(i), j)) || (j > 0 && iface(i, j - 1)))
           wl.push_back((unsigned)i + (unsigned)j * (unsigned)Nx);
     c->n_wall_cells = (int)wl.size();
     if (hipMalloc(&c->wall_cells, sizeof(unsigned) * (wl.empty() ? 1 : wl.size())) != hipSuccess ||
@@ -387,6 +398,7 @@ extern "C" int sts_create(const sts_grid *gi, int precision, sts_ctx **out) {
   g.given_xe = gi->given_xe;
   g.kn_local = gi->kn_local;
   g.bc = bc_bits(gi);
+  g.px = px ? 1 : 0;
   if (precision == 32) {
     c->mixed = true;
     if ((arc = alloc_fp32(c, gi))) {

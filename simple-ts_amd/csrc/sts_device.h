@@ -38,6 +38,7 @@ struct KGridT {
   R ht, gamma1, Kn, cT, cT1, cT2, cPch, c_Fr_v;
   int given_xb, given_xe, kn_local;
   unsigned bc;
+  int px;  // periodic about OX: x neighbours wrap, box-edge conditions off
 };
 using KGrid = KGridT<double>;
 
@@ -85,25 +86,35 @@ struct LoopTest {
 };
 
 // Clamped neighbour indices of cell (i, j).
+// Neighbour indices of cell (i, j): clamped at the grid edge (ij_function,
+// S:140-153), or, with px, wrapping in x (ij_function_PeriodicBC,
+// S:201-232; j stays clamped).
 struct Nbr {
   unsigned i, j, ij, i_1, i_2, i1, i_1j, i1j, ij_1, ij1, i_1j_1, i_1j1, i1j_1, i1j1, i_2j;
-  __device__ __forceinline__ Nbr(unsigned ii, unsigned jj, unsigned Nx, unsigned Ny) {
+  __device__ __forceinline__ Nbr(unsigned ii, unsigned jj, unsigned Nx, unsigned Ny, bool px) {
     i = ii;
     j = jj;
     ij = ii + jj * Nx;
-    const unsigned l = (0u < ii), r = (ii < Nx - 1u), d = (0u < jj), u = (jj < Ny - 1u);
-    i_1 = ii - l;
-    i_2 = ii - l - (1u < ii);
-    i1 = ii + r;
-    i_1j = ij - l;
-    i1j = ij + r;
-    ij_1 = ij - Nx * d;
-    ij1 = ij + Nx * u;
-    i_1j_1 = ij - l - Nx * d;
-    i_1j1 = ij - l + Nx * u;
-    i1j_1 = ij + r - Nx * d;
-    i1j1 = ij + r + Nx * u;
-    i_2j = i_2 + jj * Nx;
+    if (px) {
+      i_1 = (ii == 0u) ? Nx - 1u : ii - 1u;
+      i_2 = (ii == 1u) ? Nx - 1u : (ii == 0u) ? Nx - 2u : ii - 2u;
+      i1 = (ii == Nx - 1u) ? 0u : ii + 1u;
+    } else {
+      const unsigned l = (0u < ii);
+      i_1 = ii - l;
+      i_2 = ii - l - (1u < ii);
+      i1 = ii + (ii < Nx - 1u);
+    }
+    const unsigned rs = (jj - (0u < jj)) * Nx, rn = (jj + (jj < Ny - 1u)) * Nx, rj = jj * Nx;
+    i_1j = i_1 + rj;
+    i1j = i1 + rj;
+    ij_1 = ii + rs;
+    ij1 = ii + rn;
+    i_1j_1 = i_1 + rs;
+    i_1j1 = i_1 + rn;
+    i1j_1 = i1 + rs;
+    i1j1 = i1 + rn;
+    i_2j = i_2 + rj;
   }
 };
 

@@ -333,6 +333,7 @@ struct Driver {
   // BoundaryConditions_Pressure_begin_iteration_for_it, S:7430-7647, for
   // p_BC_xb_correction_method == 0 and no pressure-ratio correction.
   void pressure_begin(double probe) {
+    if (c.flag("Periodic_boundary_conditions_about_OX")) return;  // S:7436
     if (c.flag("GivenReKn")) c.at("u_given_xb") = c.get("Re") * c.get("Kn") * sqrt(15.0 * M_PI / 128.0);
     u_given_xb_in_it = c.get("u_given_xb");
     if (c.flag("Given_velocity_on_xb")) p_BC_xb = probe;
@@ -540,7 +541,8 @@ struct Driver {
           if (c.flag("ToSolveContinuityEquation")) continuity();
         } while (continue_iter);
       }
-      if (c.flag("use_interpolation_for_u_BC_by_time")) {  // S:8102-8144
+      if (c.flag("use_interpolation_for_u_BC_by_time") &&
+          !c.flag("Periodic_boundary_conditions_about_OX")) {  // S:8102-8144 (S:8105)
         const double tb = c.get("t_BC_b"), te = c.get("t_BC_e"), ub = c.get("u_BC_xb_t_BC_b"),
                      ue = c.get("u_BC_xb_t_BC_e");
         double ug;

@@ -691,11 +691,14 @@ std::string load_case(const std::string &dir, Case &c) {
     c.bodies.push_back(sts_body{b.u_body, b.v_body, b.p_body, b.T_body, b.is_slip, b.F_slip, b.is_jump, b.F_jump,
                                 b.dTdn});
 
-  // Solve boxes, S:6009-6267 (non-periodic serial build)
-  if (c.flag("Periodic_boundary_conditions_about_OX"))
-    return "periodic boundary conditions about OX are not provided";
+  // Solve boxes, S:6009-6267 (serial build): periodic about OX spans every
+  // column (S:6055-6058, S:6098-6102); y limits are those of the other cases
+  // (S:6140-6144, S:6173-6177).
   const bool pbc = c.flag("Pressure_BC"), gxb = c.flag("Given_velocity_on_xb"), gxe = c.flag("Given_velocity_on_xe");
-  if (pbc) {
+  if (c.flag("Periodic_boundary_conditions_about_OX")) {
+    c.Nx_u[0] = c.Nx_v[0] = c.Nx_o[0] = 0;
+    c.Nx_u[1] = c.Nx_v[1] = c.Nx_o[1] = Nx;
+  } else if (pbc) {
     c.Nx_u[0] = c.Nx_v[0] = c.Nx_o[0] = 1;
     c.Nx_u[1] = Nx - 1;
     c.Nx_v[1] = Nx - 2;
@@ -762,6 +765,7 @@ void Case::fill_grid(sts_grid &g) const {
   g.dTdx_0_xe = flag("dTdx_0_BC_xe");
   g.dTdy_0_yb = flag("dTdy_0_BC_yb");
   g.dTdy_0_ye = flag("dTdy_0_BC_ye");
+  g.periodic_x = flag("Periodic_boundary_conditions_about_OX");
 }
 
 // WriteEnteredDataToFile, S:5044-5238: std::setprecision(15), default float

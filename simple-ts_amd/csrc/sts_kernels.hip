@@ -198,7 +198,7 @@ __device__ __forceinline__ void cell_energy_source(const KGridT<R> &g, const KFi
 template <typename R>
 __device__ __forceinline__ void assemble_cell(const KGridT<R> &g, const KFieldsT<R> &fl, int i, int j) {
   if (i >= g.Nx || j >= g.Ny) return;
-  const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny);
+  const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny, g.px);
   const uint8_t *m = g.m;
   if (in_box(i, j, g.Nx_u0, g.Nx_u1, g.Ny_u0, g.Ny_u1) && m[n.i_1j] && m[n.ij]) cell_assemble_u(g, fl, n);
   if (in_box(i, j, g.Nx_v0, g.Nx_v1, g.Ny_v0, g.Ny_v1) && m[n.ij_1] && m[n.ij]) cell_assemble_v(g, fl, n);
@@ -310,6 +310,21 @@ __device__ __forceinline__ PTStencil<R> stencil_global(const R *Po, const R *To,
                    Po[n.ij_1], To[n.ij_1], Po[n.ij1], To[n.ij1]};
 }
 
+// The first-sweep value of one cell from (Po, To): the update set gets the
+// new values, any other cell keeps its old ones.
+template <typename R>
+__device__ __forceinline__ void sweep1_cell(const KGridT<R> &g, const KFieldsT<R> &fl, const R *__restrict__ Po,
+                                            const R *__restrict__ To, int i, int j, R &pn, R &tn) {
+  const unsigned ij = (unsigned)i + (unsigned)j * (unsigned)g.Nx;
+  if (in_pt_set(g, i, j, ij)) {
+    const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny, g.px);
+    pt_update(g, fl, n, stencil_global(Po, To, n), pn, tn);
+  } else {
+    pn = Po[ij];
+    tn = To[ij];
+  }
+}
+
 // One sweep: reads (Po, To), writes (Pn, Tn) on every cell: the update set
 // gets the new values, every other cell is copied through, which restates
 // the loop-3 entry copy S:3763-3764 and the pointer swap S:3792-3795.
@@ -327,7 +342,7 @@ __global__ void __launch_bounds__(BX *BY) k_sweep(KGridT<R> g, KFieldsT<R> fl, c
   if (i < g.Nx && j < g.Ny) {
     const unsigned ij = (unsigned)i + (unsigned)j * (unsigned)g.Nx;
     if (in_pt_set(g, i, j, ij)) {
-      const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny);
+      const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny, g.px);
       const PTStencil<R> x = stencil_global(Po, To, n);
       R pn, tn;
       pt_update(g, fl, n, x, pn, tn);
@@ -379,15 +394,8 @@ __device__ __forceinline__ void sweep2_tile_body(const KGridT<R> &g, const KFiel
     }
     const int hi = i0 - 1 + rx, hj = j0 - 1 + ry;
     if (hi < 0 || hj < 0 || hi >= Nx || hj >= Ny) continue;
-    const unsigned hij = (unsigned)hi + (unsigned)hj * (unsigned)Nx;
     R pn, tn;
-    if (in_pt_set(g, hi, hj, hij)) {
-      const Nbr hn((unsigned)hi, (unsigned)hj, (unsigned)Nx, (unsigned)Ny);
-      pt_update(g, fl, hn, stencil_global(Po, To, hn), pn, tn);
-    } else {
-      pn = Po[hij];
-      tn = To[hij];
-    }
+    sweep1_cell(g, fl, Po, To, hi, hj, pn, tn);
     sP[ry * RX + rx] = pn;
     sT[ry * RX + rx] = tn;
   }
@@ -396,7 +404,7 @@ __device__ __forceinline__ void sweep2_tile_body(const KGridT<R> &g, const KFiel
   const bool in = i < Nx && j < Ny;
   const unsigned ij = in ? (unsigned)i + (unsigned)j * (unsigned)Nx : 0u;
   const bool upd = in && in_pt_set(g, i, j, ij);
-  const Nbr n((unsigned)(in ? i : 0), (unsigned)(in ? j : 0), (unsigned)Nx, (unsigned)Ny);
+  const Nbr n((unsigned)(in ? i : 0), (unsigned)(in ? j : 0), (unsigned)Nx, (unsigned)Ny, g.px);
   const int cx = tx + 1, cy = ty + 1;
   unsigned long long r1[2] = {0ull, 0ull};
   PTCoef<R> c;
@@ -417,12 +425,17 @@ __device__ __forceinline__ void sweep2_tile_body(const KGridT<R> &g, const KFiel
   // Sweep 2 on the own cell from LDS.
   unsigned long long r2[2] = {0ull, 0ull};
   if (upd) {
-    // clamped neighbours map to the cell itself at the domain edge
-    const int wx = cx - (int)(n.i - n.i_1), ex = cx + (int)(n.i1 - n.i);
+    // clamped neighbours map to the cell itself at the domain edge; a
+    // neighbour across the periodic seam is not in this tile's LDS, so its
+    // first-sweep value is formed again from (Po, To) with the same operations
+    const bool seam_w = g.px && n.i == 0u, seam_e = g.px && n.i == (unsigned)Nx - 1u;
+    const int wx = seam_w ? cx : cx - (int)(n.i - n.i_1), ex = seam_e ? cx : cx + (int)(n.i1 - n.i);
     const int sy = cy - (int)((n.ij - n.ij_1) / (unsigned)Nx), ny = cy + (int)((n.ij1 - n.ij) / (unsigned)Nx);
-    const PTStencil<R> x{sP[cy * RX + cx], sT[cy * RX + cx], sP[cy * RX + wx], sT[cy * RX + wx],
-                         sP[cy * RX + ex], sT[cy * RX + ex], sP[sy * RX + cx], sT[sy * RX + cx],
-                         sP[ny * RX + cx], sT[ny * RX + cx]};
+    PTStencil<R> x{sP[cy * RX + cx], sT[cy * RX + cx], sP[cy * RX + wx], sT[cy * RX + wx],
+                   sP[cy * RX + ex], sT[cy * RX + ex], sP[sy * RX + cx], sT[sy * RX + cx],
+                   sP[ny * RX + cx], sT[ny * RX + cx]};
+    if (seam_w) sweep1_cell(g, fl, Po, To, Nx - 1, j, x.Pw, x.Tw);
+    if (seam_e) sweep1_cell(g, fl, Po, To, 0, j, x.Pe, x.Te);
     R pn, tn;
     c.update(x, pn, tn);
     Tn[ij] = tn;
@@ -475,7 +488,7 @@ template <typename R>
 __device__ __forceinline__ void correct_cell(const KGridT<R> &g, const KFieldsT<R> &fl, int i, int j, int do_correct,
                                              unsigned long long (&r)[2]) {
   if (i < g.Nx && j < g.Ny) {
-    const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny);
+    const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny, g.px);
     const uint8_t *m = g.m;
     const unsigned ij = n.ij, i_1j = n.i_1j, ij_1 = n.ij_1;
     const R *p = FLD(P), *T = FLD(T);
@@ -536,7 +549,7 @@ template <typename R>
 __global__ void __launch_bounds__(BX *BY) k_rho_mid(KGridT<R> g, KFieldsT<R> fl) {
   const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
   if (i >= g.Nx || j >= g.Ny) return;
-  const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny);
+  const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny, g.px);
   const uint8_t *m = g.m;
   const R *rho = FLD(RHO);
   const unsigned ij = n.ij, i_1j = n.i_1j, ij_1 = n.ij_1;
@@ -668,19 +681,29 @@ __device__ __forceinline__ void wall_cell(const KGridT<R> &g, const KFieldsT<R> 
   const unsigned Nx = (unsigned)g.Nx, Ny = (unsigned)g.Ny;
   const unsigned k = (unsigned)i + (unsigned)j * Nx;
   const uint8_t *m = g.m;
+  const bool px = g.px != 0;
   if (parts & STS_BC_SLIP_U) {
     R val = R(0.0);
     bool w = false;
-    if (i >= 1) {  // cell (i-1, j) writes u[i1j]
-      const Nbr c((unsigned)i - 1u, (unsigned)j, Nx, Ny);
-      const DevBody *b = slip_cell_u(g, c);
-      if (b) w |= slip_u_write(g, fl, c, b, 1, val);
-    }
-    const Nbr c((unsigned)i, (unsigned)j, Nx, Ny);
+    const Nbr c((unsigned)i, (unsigned)j, Nx, Ny, px);
     const DevBody *b = slip_cell_u(g, c);
-    if (b) {
-      w |= slip_u_write(g, fl, c, b, 0, val);
-      if ((unsigned)i == Nx - 1u) w |= slip_u_write(g, fl, c, b, 1, val);  // clamped i1j == ij
+    if (px && i == 0) {
+      // periodic: u[ij] of the first column is written by the cell itself,
+      // then by the last column's cell (node ij + Nx - 1, later in the walk)
+      if (b) w |= slip_u_write(g, fl, c, b, 0, val);
+      const Nbr cw(Nx - 1u, (unsigned)j, Nx, Ny, true);
+      const DevBody *bw = slip_cell_u(g, cw);
+      if (bw) w |= slip_u_write(g, fl, cw, bw, 1, val);
+    } else {
+      if (i >= 1) {  // cell (i-1, j) writes u[i1j]
+        const Nbr cw((unsigned)i - 1u, (unsigned)j, Nx, Ny, px);
+        const DevBody *bw = slip_cell_u(g, cw);
+        if (bw) w |= slip_u_write(g, fl, cw, bw, 1, val);
+      }
+      if (b) {
+        w |= slip_u_write(g, fl, c, b, 0, val);
+        if (!px && (unsigned)i == Nx - 1u) w |= slip_u_write(g, fl, c, b, 1, val);  // clamped i1j == ij
+      }
     }
     if (w) FLD(U)[k] = val;
   }
@@ -688,11 +711,11 @@ __device__ __forceinline__ void wall_cell(const KGridT<R> &g, const KFieldsT<R> 
     R val = R(0.0);
     bool w = false;
     if (j >= 1) {  // cell (i, j-1) writes v[ij1]
-      const Nbr c((unsigned)i, (unsigned)j - 1u, Nx, Ny);
+      const Nbr c((unsigned)i, (unsigned)j - 1u, Nx, Ny, px);
       const DevBody *b = slip_cell_v(g, c);
       if (b) w |= slip_v_write(g, fl, c, b, 1, val);
     }
-    const Nbr c((unsigned)i, (unsigned)j, Nx, Ny);
+    const Nbr c((unsigned)i, (unsigned)j, Nx, Ny, px);
     const DevBody *b = slip_cell_v(g, c);
     if (b) {
       w |= slip_v_write(g, fl, c, b, 0, val);
@@ -702,10 +725,10 @@ __device__ __forceinline__ void wall_cell(const KGridT<R> &g, const KFieldsT<R> 
   }
   if (parts & STS_BC_JUMP) {
     const R *T = FLD(T), *rho = FLD(RHO);
-    if (i >= 1) {
-      const unsigned w_ = k - 1u;
+    if (i >= 1 || px) {  // face between cell iw (west, across the seam if periodic) and cell i
+      const unsigned iw = (i >= 1) ? (unsigned)i - 1u : Nx - 1u, w_ = iw + (unsigned)j * Nx;
       if (m[k] && !m[w_]) FLD(TW_U)[k] = wall_T(g, g.vol[w_], T[k], rho[k], R(0.5) * g.hx[i]);
-      else if (m[w_] && !m[k]) FLD(TW_U)[k] = wall_T(g, g.vol[k], T[w_], rho[w_], R(0.5) * g.hx[i - 1]);
+      else if (m[w_] && !m[k]) FLD(TW_U)[k] = wall_T(g, g.vol[k], T[w_], rho[w_], R(0.5) * g.hx[iw]);
     }
     if (j >= 1) {
       const unsigned s_ = k - Nx;
@@ -796,6 +819,8 @@ __device__ void edges_block(const KGridT<R> &g, const KFieldsT<R> &fl, int parts
   if (group == 1) parts &= STS_BC_TEMPERATURE;
   if (group == 2) parts &= STS_BC_PRESSURE;
   if (group == 0) parts &= STS_BC_VELOCITIES;
+  // periodic about OX: the box-edge conditions are off (S:6795, S:7340, S:7655)
+  if (g.px) parts &= ~(STS_BC_VELOCITIES | STS_BC_TEMPERATURE | STS_BC_PRESSURE);
   if (group >= 0 && group != 2) probe = nullptr;
   const bool test = lt.ctrl && group <= 0;
   if (test && threadIdx.x < NSLOT) {  // wave 0: fold the slots of iteration gate.k
@@ -947,7 +972,7 @@ __global__ void __launch_bounds__(EDGE_THREADS) k_edges(KGridT<R> g, KFieldsT<R>
 template <typename R>
 __device__ __forceinline__ void flux_diff_cell(const KGridT<R> &g, const KFieldsT<R> &fl, int i, int j) {
   if (i >= g.Nx || j >= g.Ny) return;
-  const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny);
+  const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny, g.px);
   const uint8_t *m = g.m;
   const R *hx = g.hx, *hy = g.hy, *sT = FLD(SQRT_T), *Twu = FLD(TW_U), *Twv = FLD(TW_V);
   const unsigned ij = n.ij, i_1j = n.i_1j, ij_1 = n.ij_1, i_1j_1 = n.i_1j_1, i_1 = n.i_1;
@@ -1026,7 +1051,8 @@ __global__ void __launch_bounds__(BX *BY) k_apply_bodies(KGrid g, KFields fl) {
 
 // S:8048-8099 after the wall temperatures are refreshed: each body cell gets
 // the wall temperature of the face to its last fluid neighbour in the
-// reference's walk order (above, right, left, below).
+// reference's walk order (above, right, left, below; periodic about OX, the
+// neighbour across the seam ranks by its node number).
 __global__ void __launch_bounds__(BX *BY) k_jump_overwrite(KGrid g, KFields fl) {
   const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
   if (i >= g.Nx || j >= g.Ny) return;
@@ -1035,9 +1061,12 @@ __global__ void __launch_bounds__(BX *BY) k_jump_overwrite(KGrid g, KFields fl) 
   const uint8_t *m = g.m;
   if (m[k]) return;
   double *T = FLD(T);
+  const bool seam_w = g.px && i == 0, seam_e = g.px && i == g.Nx - 1;
   if (j + 1 < g.Ny && m[k + Nx]) T[k] = FLD(TW_V)[k + Nx];
+  else if (seam_w && m[k + Nx - 1u]) T[k] = FLD(TW_U)[k];  // last column, node k + Nx - 1
   else if (i + 1 < g.Nx && m[k + 1u]) T[k] = FLD(TW_U)[k + 1u];
   else if (i >= 1 && m[k - 1u]) T[k] = FLD(TW_U)[k];
+  else if (seam_e && m[k + 1u - Nx]) T[k] = FLD(TW_U)[k + 1u - Nx];  // first column, node k - Nx + 1
   else if (j >= 1 && m[k - Nx]) T[k] = FLD(TW_V)[k];
 }
 
@@ -1060,7 +1089,7 @@ __global__ void __launch_bounds__(BX *BY) k_extrapolate(KGrid g, KFields fl, dou
                                                         double *pT) {
   const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
   if (i >= g.Nx || j >= g.Ny) return;
-  const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny);
+  const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny, g.px);
   const unsigned k = n.ij;
   const uint8_t *m = g.m;
   const double un = FLD(U_PR)[k], vn = FLD(V_PR)[k], pn = FLD(P_PR)[k], Tn = FLD(T_PR)[k];
@@ -1091,7 +1120,7 @@ __global__ void __launch_bounds__(256) k_drag_terms(KGrid g, KFields fl, const u
   const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (t >= n) return;
   const unsigned Nx = (unsigned)g.Nx, Ny = (unsigned)g.Ny, k = cells[t];
-  const Nbr c(k % Nx, k / Nx, Nx, Ny);
+  const Nbr c(k % Nx, k / Nx, Nx, Ny, false);  // clamped in every case (S:8214)
   const uint8_t *m = g.m;
   const double *hx = g.hx, *hy = g.hy, *u = FLD(U), *v = FLD(V), *p = FLD(P);
   const unsigned i = c.i, j = c.j;
@@ -1280,7 +1309,7 @@ __global__ void __launch_bounds__(TX *TY) kb_sweep(const BatchCase<R> *C, const 
   if (i < g.Nx && j < g.Ny) {
     const unsigned ij = (unsigned)i + (unsigned)j * (unsigned)g.Nx;
     if (in_pt_set(g, i, j, ij)) {
-      const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny);
+      const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny, g.px);
       const PTStencil<R> x = stencil_global(Po, To, n);
       R pn, tn;
       pt_update(g, fl, n, x, pn, tn);

@@ -12,9 +12,12 @@ import numpy as np
 sts = importlib.import_module("simple-ts_amd")
 
 
-def boxes_for(Nx, Ny, pressure_bc=False, given_xb=False, given_xe=False):
-    """Solve boxes of the non-periodic serial build (S:6009-6267)."""
-    if pressure_bc:
+def boxes_for(Nx, Ny, pressure_bc=False, given_xb=False, given_xe=False, periodic=False):
+    """Solve boxes of the serial build (S:6009-6267); periodic about OX the
+    x boxes span every column (S:6055-6102)."""
+    if periodic:
+        xu = xv = xo = (0, Nx)
+    elif pressure_bc:
         xu, xv, xo = (1, Nx - 1), (1, Nx - 2), (1, Nx - 2)
     else:
         xu = (2 if given_xb else 1, Nx - 1)
@@ -24,7 +27,7 @@ def boxes_for(Nx, Ny, pressure_bc=False, given_xb=False, given_xe=False):
 
 
 def channel_case(Nx=40, Ny=16, seed=0, slip=(1, 0, 1, 1), jump=(0, 1, 1, 0), kn_local=1, given_xb=1,
-                 gravity=0.0, bc=None, uniform=False):
+                 gravity=0.0, bc=None, uniform=False, periodic=False):
     rng = np.random.default_rng(seed)
     if uniform:
         hx = np.full(Nx, 0.05)
@@ -40,6 +43,15 @@ def channel_case(Nx=40, Ny=16, seed=0, slip=(1, 0, 1, 1), jump=(0, 1, 1, 0), kn_
     vol[c:d, a:b] = 3                 # square
     vol[1, Nx // 2 + 3] = 4           # one-cell notch on the bottom wall
     vol[Ny - 2, Nx // 2 - 2: Nx // 2] = 4
+    if periodic:
+        # bodies against the periodic seam: a block across it, one that stops
+        # a column short of it, and single cells in the first and last columns
+        r = Ny // 2
+        vol[r - 2:r + 1, Nx - 3:] = 3
+        vol[r - 2:r + 1, :2] = 3
+        vol[r + 2:r + 4, Nx - 4:Nx - 1] = 4
+        vol[2, 0] = 4
+        vol[Ny - 3, Nx - 1] = 4
     bodies = []
     for k in range(4):
         bodies.append(dict(u_body=0.0, v_body=0.0, p_body=0.0, T_body=1.0 + 0.25 * k, is_slip=int(slip[k]),
@@ -54,8 +66,9 @@ def channel_case(Nx=40, Ny=16, seed=0, slip=(1, 0, 1, 1), jump=(0, 1, 1, 0), kn_
                 c_Fr_v=gravity)
     if bc is None:
         bc = dict(dvdx_0_xb=1, dvdx_0_xe=1, dudx_0_xe=1, dTdx_0_xe=1, dpdx_0_xb=1, dTdy_0_yb=1, dudy_0_ye=1)
-    return sts.Problem(hx, hy, vol.reshape(-1), bodies, boxes_for(Nx, Ny, given_xb=bool(given_xb)), scal,
-                       given_xb=given_xb, kn_local=kn_local, bc=bc)
+    return sts.Problem(hx, hy, vol.reshape(-1), bodies,
+                       boxes_for(Nx, Ny, given_xb=bool(given_xb), periodic=periodic), scal,
+                       given_xb=given_xb, kn_local=kn_local, bc=bc, periodic_x=int(periodic))
 
 
 def random_state(problem, seed=1, zero_frac=0.05):

@@ -14,7 +14,8 @@ reference's input format:
                  DEBUG_PROGRAM build dumps (after 1 step of 2 outer
                  iterations)
 
-Run from the repository root:  python tests/golden/make_golden.py
+Run from the repository root:  python tests/golden/make_golden.py [case ...]
+(no names: every case).
 """
 import importlib
 import os
@@ -66,6 +67,17 @@ def case_specs():
         "rb_jump": lambda d: cases.rayleigh_benard(d, 24, 12, Fr=1.0, steps=3, N_I=40, tol=0.0, jump=True),
         # circle body in a slip channel, local Kn on the walls
         "circle": lambda d: _circle_case(d),
+        # periodic about OX: channel with slip/jump walls, staggered blocks
+        # across the seam and a heated cylinder, gas started at u = 0.3
+        "periodic_channel": lambda d: cases.periodic_channel(
+            d, 40, 14, 0.3, 0.02, 4, circles=[dict(x_c=1.0, y_c=0.45, r_c=0.18, T_body=1.2, slip=1, jump=1)]),
+        # periodic about OX with an open top whose zero-gradient conditions
+        # the reference switches off for periodic cases
+        "periodic_open": lambda d: cases.periodic_channel(d, 40, 14, 0.3, 0.02, 3, top_wall=False),
+        # periodic Rayleigh-Benard layer with a hot cylinder
+        "periodic_rb": lambda d: cases.rayleigh_benard(
+            d, 32, 12, Fr=0.5, steps=3, N_I=60, jump=True, periodic=True,
+            circles=[dict(x_c=0.6, y_c=0.3, r_c=0.15, T_body=2.0, jump=1)]),
         # mesh kinds 1-4 (setup checks)
         "mesh1": lambda d: _mesh_case(d, 1),
         "mesh2": lambda d: _mesh_case(d, 2),
@@ -94,7 +106,10 @@ def main():
     for path in (REF, REF_DBG):
         if not os.access(path, os.X_OK):
             sys.exit(f"{path} missing: make -f oracle/Makefile.ref")
+    only = set(sys.argv[1:])
     for name, make in case_specs().items():
+        if only and name not in only:
+            continue
         base = os.path.join(HERE, name)
         shutil.rmtree(base, ignore_errors=True)
         inp, ref = os.path.join(base, "input"), os.path.join(base, "ref")
@@ -106,6 +121,8 @@ def main():
             if f not in keep:
                 os.remove(os.path.join(ref, f))
         print(name, sorted(os.listdir(ref)))
+    if only and "square_dbg" not in only:
+        return
     # DEBUG_PROGRAM dumps of one step with two outer iterations
     base = os.path.join(HERE, "square_dbg")
     shutil.rmtree(base, ignore_errors=True)

@@ -6,7 +6,8 @@ import shutil
 import numpy as np
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-RUN_CASES = ["shipped_coarse", "square", "square_super", "rb_jump", "circle", "square_cd"]
+RUN_CASES = ["shipped_coarse", "square", "square_super", "rb_jump", "circle", "square_cd", "periodic_channel",
+             "periodic_open", "periodic_rb"]
 MESH_CASES = ["mesh1", "mesh2", "mesh3", "mesh4"]
 ALL_CASES = RUN_CASES + MESH_CASES + ["square_dbg"]
 

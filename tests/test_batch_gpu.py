@@ -20,6 +20,10 @@ def runs(tmp_path, tag, precision=64):
         prm = dict(Ma=Ma, Kn=Kn, Re=Ma / (Kn * ws.SQ), side=side, xc=xc)
         d = ws.write_case(str(tmp_path / f"{tag}{k}"), Nx, Ny, prm, 10, dt_over_tin=0.005)
         out.append(ws.Run(d, precision))
+    # a channel periodic about OX, with bodies across the seam
+    cs = importlib.import_module("simple-ts_amd.cases")
+    d = cs.periodic_channel(str(tmp_path / f"{tag}p"), 100, 22, 0.3, 0.02, 10, dt=0.005)
+    out.append(ws.Run(d, precision))
     return out
 
 

@@ -71,6 +71,25 @@ def test_driver_matches_live_reference_subsonic_square(tmp_path):
 
 
 @pytest.mark.skipif(not have_reference(), reason="oracle/_ref/sts_ref not built")
+@pytest.mark.parametrize("Nx", [256, 200])
+def test_driver_matches_live_reference_periodic_layer(Nx, tmp_path):
+    """A Rayleigh-Benard layer periodic about OX with a heated cylinder, on
+    a grid whose width is (256) and is not (200) a whole number of sweep
+    tiles, 6 steps each converged to 1e-8, against the reference."""
+    cases = importlib.import_module("simple-ts_amd.cases")
+
+    def make(d):
+        return cases.rayleigh_benard(str(d), Nx, 40, W=4.0, Fr=cases.rayleigh_fr(3000.0, 0.05), steps=6, dt=2e-3,
+                                     periodic=True, jump=True,
+                                     circles=[dict(x_c=3.9, y_c=0.35, r_c=0.2, T_body=2.0, jump=1)])
+    mine, ref = make(tmp_path / "gpu"), make(tmp_path / "ref")
+    run_reference(ref)
+    run_driver(mine)
+    bad = compare_dirs(mine, ref)
+    assert not bad, f"files differing from the reference's: {bad}"
+
+
+@pytest.mark.skipif(not have_reference(), reason="oracle/_ref/sts_ref not built")
 @pytest.mark.parametrize("n_pc", [3, 5])
 def test_driver_odd_sweep_counts_match_live_reference(n_pc, tmp_path):
     """Odd N_I_p_c leaves each outer iteration's p/T in the partner buffers;

@@ -13,6 +13,10 @@ CASES = {
     "channel130x37_noslip": dict(Nx=130, Ny=37, seed=5, slip=(0, 0, 0, 0), jump=(0, 0, 0, 0), given_xb=0,
                                  bc=dict(dudx_0_xb=1, durhodx_0_xb=1, drhodt_xe=1, dvdy_0_yb=1, dvdy_0_ye=1,
                                          dpdx_0_xe=1, dpdy_0_yb=1, dTdx_0_xb=1, dTdy_0_ye=1, dudy_0_yb=1)),
+    # periodic about OX; the edge switches are set and must have no effect
+    "periodic70x20": dict(Nx=70, Ny=20, seed=7, given_xb=0, periodic=True),
+    "periodic128x24_gravity": dict(Nx=128, Ny=24, seed=9, given_xb=0, periodic=True, gravity=0.4,
+                                   bc=dict(dudx_0_xb=1, dvdy_0_yb=1, dpdx_0_xe=1, dTdx_0_xb=1, dTdy_0_ye=1)),
 }
 
 STAGES = ["save_previous", "assemble", "pt_jacobi1", "pt_jacobi2", "pt_jacobi3", "correct", "properties",
