@@ -375,11 +375,19 @@ __device__ __forceinline__ void sweep2_tile_body(const KGridT<R> &g, const KFiel
   const int RX = SX + 2, RY = SY + 2;
   const int tid = tx + ty * SX;
   // Sweep 1 on the one-cell halo ring of the tile (its cells belong to the
-  // neighbouring tiles, which count their residuals).
+  // neighbouring tiles, which count their residuals).  Periodic about OX, the
+  // ring column left of the first grid column holds the last one, and the
+  // column right of the last grid column holds the first one: if the last
+  // column lies inside the tile, SY extra entries fill that LDS column.
   const int NR = 2 * RX + 2 * SY;
-  for (int q = tid; q < NR; q += SX * SY) {
+  const bool east_extra = g.px && i0 <= Nx - 1 && Nx - 1 < i0 + SX - 1;
+  const int NRx = NR + (east_extra ? SY : 0);
+  for (int q = tid; q < NRx; q += SX * SY) {
     int rx, ry;
-    if (q < RX) {
+    if (q >= NR) {
+      rx = Nx - i0 + 1;
+      ry = 1 + (q - NR);
+    } else if (q < RX) {
       rx = q;
       ry = 0;
     } else if (q < 2 * RX) {
@@ -392,7 +400,9 @@ __device__ __forceinline__ void sweep2_tile_body(const KGridT<R> &g, const KFiel
       rx = RX - 1;
       ry = 1 + (q - 2 * RX - SY);
     }
-    const int hi = i0 - 1 + rx, hj = j0 - 1 + ry;
+    int hi = i0 - 1 + rx;
+    const int hj = j0 - 1 + ry;
+    if (g.px) hi = (hi == -1) ? Nx - 1 : (hi == Nx) ? 0 : hi;
     if (hi < 0 || hj < 0 || hi >= Nx || hj >= Ny) continue;
     R pn, tn;
     sweep1_cell(g, fl, Po, To, hi, hj, pn, tn);
@@ -425,17 +435,13 @@ __device__ __forceinline__ void sweep2_tile_body(const KGridT<R> &g, const KFiel
   // Sweep 2 on the own cell from LDS.
   unsigned long long r2[2] = {0ull, 0ull};
   if (upd) {
-    // clamped neighbours map to the cell itself at the domain edge; a
-    // neighbour across the periodic seam is not in this tile's LDS, so its
-    // first-sweep value is formed again from (Po, To) with the same operations
-    const bool seam_w = g.px && n.i == 0u, seam_e = g.px && n.i == (unsigned)Nx - 1u;
-    const int wx = seam_w ? cx : cx - (int)(n.i - n.i_1), ex = seam_e ? cx : cx + (int)(n.i1 - n.i);
+    // clamped neighbours map to the cell itself at the domain edge; periodic
+    // about OX, the LDS column beside the seam holds the wrapped neighbour
+    const int wx = cx - (g.px ? 1 : (int)(n.i - n.i_1)), ex = cx + (g.px ? 1 : (int)(n.i1 - n.i));
     const int sy = cy - (int)((n.ij - n.ij_1) / (unsigned)Nx), ny = cy + (int)((n.ij1 - n.ij) / (unsigned)Nx);
-    PTStencil<R> x{sP[cy * RX + cx], sT[cy * RX + cx], sP[cy * RX + wx], sT[cy * RX + wx],
-                   sP[cy * RX + ex], sT[cy * RX + ex], sP[sy * RX + cx], sT[sy * RX + cx],
-                   sP[ny * RX + cx], sT[ny * RX + cx]};
-    if (seam_w) sweep1_cell(g, fl, Po, To, Nx - 1, j, x.Pw, x.Tw);
-    if (seam_e) sweep1_cell(g, fl, Po, To, 0, j, x.Pe, x.Te);
+    const PTStencil<R> x{sP[cy * RX + cx], sT[cy * RX + cx], sP[cy * RX + wx], sT[cy * RX + wx],
+                         sP[cy * RX + ex], sT[cy * RX + ex], sP[sy * RX + cx], sT[sy * RX + cx],
+                         sP[ny * RX + cx], sT[ny * RX + cx]};
     R pn, tn;
     c.update(x, pn, tn);
     Tn[ij] = tn;
@@ -809,57 +815,77 @@ __device__ __forceinline__ void edge_y(const KGridT<R> &g, R *a, int b0, int b1,
   }
 }
 
+// The loop-2 test of iteration gate.k (S:4433-4437, Iter = k + 1 after the
+// increment), one wave: fold the slot copies of the six residual words,
+// clear them, log the maxima, and end the loop by setting stop_at = k.  The
+// residuals stay in registers; the stagnation window's older log entries are
+// fetched alongside the slots.
+__device__ __forceinline__ void loop_test_wave(const Gate &gate, const LoopTest &lt) {
+  if (!lt.ctrl || threadIdx.x >= NSLOT) return;
+  const bool stall = lt.stall_w > 0 && gate.k >= 2 * lt.stall_w;
+  unsigned long long old[4] = {0ull, 0ull, 0ull, 0ull};
+  if (stall && threadIdx.x == 0) {
+    const unsigned long long *lb = lt.log_k - 6 * lt.stall_w;
+#pragma unroll
+    for (int q = 0; q < 4; q++) old[q] = lb[q];
+  }
+  unsigned long long *sl = lt.lred + threadIdx.x * SLOT_STRIDE;
+  unsigned long long w[6];
+#pragma unroll
+  for (int q = 0; q < 6; q++) w[q] = sl[q];
+#pragma unroll
+  for (int q = 0; q < 6; q++) {
+    sl[q] = 0ull;
+    w[q] = wave_max(w[q]);
+  }
+  if (threadIdx.x != 0) return;
+#pragma unroll
+  for (int q = 0; q < 6; q++) lt.log_k[q] = w[q];
+  double r[4];
+  bool fin = true;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    r[q] = __longlong_as_double((long long)w[q]);
+    fin = fin && isfinite(r[q]);
+  }
+  bool cont = (gate.k + 1 < lt.N_I) && (lt.tol_uv < r[0] || lt.tol_uv < r[1] || lt.tol_p < r[2] || lt.tol_T < r[3]);
+  if (cont && stall) {
+    // stagnation (FP32 phase): the largest residual relative to its
+    // tolerance fell by less than stall_ratio over the last stall_w iterations
+    const double tol[4] = {lt.tol_uv, lt.tol_uv, lt.tol_p, lt.tol_T};
+    double now = 0.0, then = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      now = fmax(now, r[q] / tol[q]);
+      then = fmax(then, __longlong_as_double((long long)old[q]) / tol[q]);
+    }
+    if (now > lt.stall_ratio * then) cont = false;
+  }
+  if (!fin) {
+    lt.ctrl->nonfinite = 1;
+    cont = false;
+  }
+  if (!cont) lt.ctrl->stop_at = gate.k;
+}
+
+constexpr int EDGE_GROUPS = 4;
+
 template <typename R>
 __device__ void edges_block(const KGridT<R> &g, const KFieldsT<R> &fl, int parts, double *probe, const Gate &gate,
                             const LoopTest &lt, int group) {
   // The velocity, temperature and pressure conditions touch disjoint arrays
-  // (u, v read rho and the face densities, which none of them writes), so
-  // three workgroups can run them side by side: group 0 = loop test and
-  // velocities, 1 = temperature, 2 = pressure and the probe.
+  // (u, v read rho and the face densities, which none of them writes), and
+  // the loop test reads only the residual words, so four workgroups run them
+  // side by side: group 0 = velocities, 1 = temperature, 2 = pressure and the
+  // probe, 3 = loop test (group < 0: everything in one workgroup).
+  if (group == 3 || group < 0) loop_test_wave(gate, lt);
+  if (group == 3) return;
   if (group == 1) parts &= STS_BC_TEMPERATURE;
   if (group == 2) parts &= STS_BC_PRESSURE;
   if (group == 0) parts &= STS_BC_VELOCITIES;
   // periodic about OX: the box-edge conditions are off (S:6795, S:7340, S:7655)
   if (g.px) parts &= ~(STS_BC_VELOCITIES | STS_BC_TEMPERATURE | STS_BC_PRESSURE);
   if (group >= 0 && group != 2) probe = nullptr;
-  const bool test = lt.ctrl && group <= 0;
-  if (test && threadIdx.x < NSLOT) {  // wave 0: fold the slots of iteration gate.k
-    unsigned long long *sl = lt.lred + threadIdx.x * SLOT_STRIDE;
-#pragma unroll
-    for (int q = 0; q < 6; q++) {
-      const unsigned long long w = wave_max(sl[q]);
-      sl[q] = 0ull;
-      if (threadIdx.x == 0) lt.log_k[q] = w;
-    }
-  }
-  if (test && threadIdx.x == 0) {
-    // S:4433-4437 for iteration gate.k (Iter = k + 1 after the increment)
-    double r[4];
-    bool fin = true;
-    for (int q = 0; q < 4; q++) {
-      r[q] = __longlong_as_double((long long)lt.log_k[q]);
-      fin = fin && isfinite(r[q]);
-    }
-    bool cont = (gate.k + 1 < lt.N_I) &&
-                (lt.tol_uv < r[0] || lt.tol_uv < r[1] || lt.tol_p < r[2] || lt.tol_T < r[3]);
-    if (cont && lt.stall_w > 0 && gate.k >= 2 * lt.stall_w) {
-      // stagnation (FP32 phase): the largest residual relative to its
-      // tolerance fell by less than stall_ratio over the last stall_w iterations
-      const unsigned long long *lb = lt.log_k - 6 * lt.stall_w;
-      const double tol[4] = {lt.tol_uv, lt.tol_uv, lt.tol_p, lt.tol_T};
-      double now = 0.0, then = 0.0;
-      for (int q = 0; q < 4; q++) {
-        now = fmax(now, r[q] / tol[q]);
-        then = fmax(then, __longlong_as_double((long long)lb[q]) / tol[q]);
-      }
-      if (now > lt.stall_ratio * then) cont = false;
-    }
-    if (!fin) {
-      lt.ctrl->nonfinite = 1;
-      cont = false;
-    }
-    if (!cont) lt.ctrl->stop_at = gate.k;
-  }
   const int Nx = g.Nx, Ny = g.Ny;
   const uint8_t *m = g.m;
   const unsigned bc = g.bc;
@@ -1359,13 +1385,13 @@ __global__ void __launch_bounds__(TX *TY) kb_flux_diff(const BatchCase<R> *C, co
 // Edge conditions and the loop-2 test: one workgroup per case.
 template <typename R>
 __global__ void __launch_bounds__(EDGE_THREADS) kb_edges(const BatchCase<R> *C, int k, int par, int parts) {
-  const BatchCase<R> &c = C[blockIdx.x / 3];
+  const BatchCase<R> &c = C[blockIdx.x / EDGE_GROUPS];
   const Gate gate{c.ctrl, k};
   if (gated_out(gate)) return;
   const KFieldsT<R> f = fields_at(c, par);
   const LoopTest lt{c.ctrl, c.lred, c.log + 6 * (size_t)k, c.N_I, c.tol_uv, c.tol_p, c.tol_T, c.stall_w,
                     c.stall_ratio};
-  edges_block(c.g, f, parts, c.probe, gate, lt, (int)(blockIdx.x % 3));
+  edges_block(c.g, f, parts, c.probe, gate, lt, (int)(blockIdx.x % EDGE_GROUPS));
 }
 
 // Opens every case's loop 2: control, log (6 N_I words), slotted words.
@@ -1437,7 +1463,8 @@ hipError_t launch_wall(const KGridT<R> &g, const KFieldsT<R> &f, int parts, hipS
 template <typename R>
 hipError_t launch_edges(const KGridT<R> &g, const KFieldsT<R> &f, int parts, double *probe, hipStream_t s,
                         Gate gate, LoopTest lt) {
-  hipLaunchKernelGGL(k_edges<R>, dim3(3), dim3(EDGE_THREADS), 0, s, g, f, parts, probe, gate, lt);
+  hipLaunchKernelGGL(k_edges<R>, dim3(lt.ctrl ? EDGE_GROUPS : EDGE_GROUPS - 1), dim3(EDGE_THREADS), 0, s, g, f, parts,
+                     probe, gate, lt);
   return hipGetLastError();
 }
 template <typename R>
@@ -1567,7 +1594,9 @@ hipError_t launch_batch(int kernel, int variant, const BatchCase<R> *C, int ncas
           break;
       }
       break;
-    case BK_EDGES: hipLaunchKernelGGL(kb_edges<R>, dim3(3 * ncases), dim3(EDGE_THREADS), 0, s, C, k, par, a); break;
+    case BK_EDGES:
+      hipLaunchKernelGGL(kb_edges<R>, dim3(EDGE_GROUPS * ncases), dim3(EDGE_THREADS), 0, s, C, k, par, a);
+      break;
     case BK_LOOP_OPEN: hipLaunchKernelGGL(kb_loop_open<R>, dim3(ncases), dim3(256), 0, s, C); break;
     default: return hipErrorInvalidValue;
   }
