@@ -68,8 +68,8 @@ struct sts_ctx {
   float *p_alt32 = nullptr, *T_alt32 = nullptr;
   int last_iters32 = 0, last_iters64m = 0;
   long long fluid_cells = 0;
-  unsigned *wall_cells = nullptr;  // cells next to a body/fluid interface (k_wall_list)
-  int n_wall_cells = 0;
+  WallOp *wall_ops = nullptr;  // the wall pass's writes, planned once from the mask (k_wall_ops)
+  int n_wall_ops = 0;
   // drag coefficients: host copy of vol_inf_fb, per-body cell lists, buffers
   std::vector<uint32_t> h_vol;
   std::vector<unsigned *> drag_cells;
@@ -205,7 +205,7 @@ extern "C" int sts_destroy(sts_ctx *c) {
   for (int q = 0; q < STS_NFIELDS; q++) (void)hipFree(c->f32[q]);
   (void)hipFree(c->p_ws);
   for (double *q : c->prev) (void)hipFree(q);
-  (void)hipFree(c->wall_cells);
+  (void)hipFree(c->wall_ops);
   for (unsigned *q : c->drag_cells) (void)hipFree(q);
   (void)hipFree(c->drag_terms);
   (void)hipFree(c->drag_out);
@@ -307,6 +307,35 @@ static int alloc_fp32(sts_ctx *c, const sts_grid *gi) {
   return STS_OK;
 }
 
+// The wall pass's writes of the listed cells, planned on the device from the
+// mask and body table and kept in a compact list (at least one slot, so that
+// an empty list still selects the list kernel).
+static int plan_wall(sts_ctx *c, const std::vector<unsigned> &cells) {
+  const int n = (int)cells.size();
+  std::vector<WallOp> all((size_t)4 * n), ops;
+  if (n > 0) {
+    unsigned *d_cells = nullptr;
+    WallOp *d_all = nullptr;
+    hipError_t e = hipMalloc(&d_cells, sizeof(unsigned) * n);
+    if (e == hipSuccess) e = hipMalloc(&d_all, sizeof(WallOp) * 4 * (size_t)n);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_cells, cells.data(), sizeof(unsigned) * n, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = launch_wall_plan(c->g, d_cells, n, d_all, c->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(all.data(), d_all, sizeof(WallOp) * all.size(), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d_cells);
+    (void)hipFree(d_all);
+    HIP_OK(e);
+    for (const WallOp &o : all)
+      if ((o.kb & 7u) != WOP_NONE) ops.push_back(o);
+  }
+  c->n_wall_ops = (int)ops.size();
+  HIP_OK(hipMalloc(&c->wall_ops, sizeof(WallOp) * (ops.empty() ? 1 : ops.size())));
+  if (!ops.empty())
+    HIP_OK(hipMemcpy(c->wall_ops, ops.data(), sizeof(WallOp) * ops.size(), hipMemcpyHostToDevice));
+  return STS_OK;
+}
+
 extern "C" int sts_create(const sts_grid *gi, int precision, sts_ctx **out) {
   if (!gi || !out) return fail(STS_ERR_ARG, "null argument");
   *out = nullptr;
@@ -348,6 +377,7 @@ extern "C" int sts_create(const sts_grid *gi, int precision, sts_ctx **out) {
     sts_destroy(c);
     return arc;
   }
+  std::vector<unsigned> wl;
   {
     // the wall conditions write only at cells k with k, its west neighbour
     // or k-Nx on a body/fluid interface (a cell with a 4-neighbour of the
@@ -362,17 +392,10 @@ extern "C" int sts_create(const sts_grid *gi, int precision, sts_ctx **out) {
              (j > 0 && mask[(size_t)i + (size_t)(j - 1) * Nx] != a) ||
              (j + 1 < Ny && mask[(size_t)i + (size_t)(j + 1) * Nx] != a);
     };
-    std::vector<unsigned> wl;
     for (int j = 0; j < Ny; j++)
       for (int i = 0; i < Nx; i++)
         if (iface(i, j) || (west(i) >= 0 && iface(west(i), j)) || (j > 0 && iface(i, j - 1)))
           wl.push_back((unsigned)i + (unsigned)j * (unsigned)Nx);
-    c->n_wall_cells = (int)wl.size();
-    if (hipMalloc(&c->wall_cells, sizeof(unsigned) * (wl.empty() ? 1 : wl.size())) != hipSuccess ||
-        hipMemcpy(c->wall_cells, wl.data(), sizeof(unsigned) * wl.size(), hipMemcpyHostToDevice) != hipSuccess) {
-      sts_destroy(c);
-      return fail(STS_ERR_HIP, "wall cell list");
-    }
   }
   KGrid &g = c->g;
   g.Nx = gi->Nx;
@@ -399,6 +422,10 @@ extern "C" int sts_create(const sts_grid *gi, int precision, sts_ctx **out) {
   g.kn_local = gi->kn_local;
   g.bc = bc_bits(gi);
   g.px = px ? 1 : 0;
+  if ((arc = plan_wall(c, wl))) {
+    sts_destroy(c);
+    return arc;
+  }
   if (precision == 32) {
     c->mixed = true;
     if ((arc = alloc_fp32(c, gi))) {
@@ -714,8 +741,8 @@ static int enqueue_outer(sts_ctx *c, const Loop2State<R> &st, int k, int n, Loop
   HIP_OK(launch_correct(g, kf, lr + 0, lr + 1, 1, c->stream, gate));
   timed_end(c, t);
   t = timed_begin(c, st.kid + STS_K_WALL);
-  HIP_OK(launch_wall(g, kf, STS_BC_SLIP_U | STS_BC_SLIP_V | STS_BC_JUMP, c->stream, gate, c->wall_cells,
-                     c->n_wall_cells));
+  HIP_OK(launch_wall(g, kf, STS_BC_SLIP_U | STS_BC_SLIP_V | STS_BC_JUMP, c->stream, gate, c->wall_ops,
+                     c->n_wall_ops));
   timed_end(c, t);
   lt.lred = lr;
   lt.log_k += 6 * (size_t)k;
@@ -839,7 +866,7 @@ static int extrapolate_start(sts_ctx *c, bool on) {
   HIP_OK(launch_extrapolate(c->g, kf, c->prev, c->stream));
   HIP_OK(launch_correct(c->g, kf, nullptr, nullptr, 0, c->stream));
   HIP_OK(launch_wall(c->g, kf, STS_BC_SLIP_U | STS_BC_SLIP_V | STS_BC_JUMP, c->stream, Gate{nullptr, 0},
-                     c->wall_cells, c->n_wall_cells));
+                     c->wall_ops, c->n_wall_ops));
   HIP_OK(launch_edges(c->g, kf, STS_BC_VELOCITIES | STS_BC_TEMPERATURE | STS_BC_PRESSURE, nullptr, c->stream));
   HIP_OK(launch_flux_diff(c->g, kf, c->stream));
   return STS_OK;
@@ -974,6 +1001,8 @@ struct sts_batch {
   int log_cap = 0;
   BatchTile *cell_t[NVARIANT] = {}, *sweep_t[NVARIANT] = {};
   int n_cell[NVARIANT] = {}, n_sweep[NVARIANT] = {};
+  BatchTile *wall_t = nullptr;  // chunks of the cases' planned wall writes
+  int n_wall = 0;
   int cell_v = 0, sweep_v = 0;
   int last_iters32 = 0, last_iters64 = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -988,6 +1017,7 @@ extern "C" int sts_batch_destroy(sts_batch *b) {
     (void)hipFree(b->cell_t[v]);
     (void)hipFree(b->sweep_t[v]);
   }
+  (void)hipFree(b->wall_t);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -1051,6 +1081,16 @@ extern "C" int sts_batch_create(sts_ctx *const *ctxs, int n, sts_batch **out) {
   int rc;
   if ((rc = build_tiles(b, CELL_TILES, b->cell_t, b->n_cell)) || (rc = build_tiles(b, SWEEP_TILES, b->sweep_t, b->n_sweep)))
     return bail(rc);
+  {
+    std::vector<BatchTile> wt;
+    for (int c = 0; c < n; c++)
+      for (int x = 0; x * WALL_OPS_PER_TILE < b->cs[c]->n_wall_ops; x++) wt.push_back(BatchTile{c, x, 0});
+    b->n_wall = (int)wt.size();
+    if (hipMalloc(&b->wall_t, sizeof(BatchTile) * (wt.empty() ? 1 : wt.size())) != hipSuccess ||
+        (!wt.empty() && hipMemcpyAsync(b->wall_t, wt.data(), sizeof(BatchTile) * wt.size(), hipMemcpyHostToDevice,
+                                       b->stream) != hipSuccess))
+      return bail(fail(STS_ERR_HIP, "wall tiles"));
+  }
   if (hipStreamSynchronize(b->stream) != hipSuccess) return bail(fail(STS_ERR_HIP, "stream sync"));
   *out = b;
   return STS_OK;
@@ -1093,6 +1133,8 @@ static int upload_cases(sts_batch *b, BatchCase<R> *dst, int N_I, double tol_uv,
     x.tol_p = tol_p;
     x.tol_T = tol_T;
     x.stall_ratio = stall_ratio;
+    x.wops = c->wall_ops;
+    x.nwops = c->n_wall_ops;
   }
   HIP_OK(hipMemcpyAsync(dst, h.data(), sizeof(BatchCase<R>) * h.size(), hipMemcpyHostToDevice, b->stream));
   return STS_OK;
@@ -1124,7 +1166,8 @@ static int run_batch_loop2(sts_batch *b, BatchCase<R> *C, int n_pc, int N_I, int
         par ^= 1;
       }
       HIP_OK(launch_batch<R>(BK_CORRECT, cv, C, n, ct, nct, k, par, 0, 0, s));
-      HIP_OK(launch_batch<R>(BK_WALL, cv, C, n, ct, nct, k, par, STS_BC_SLIP_U | STS_BC_SLIP_V | STS_BC_JUMP, 0, s));
+      HIP_OK(launch_batch<R>(BK_WALL_OPS, 0, C, n, b->wall_t, b->n_wall, k, par,
+                             STS_BC_SLIP_U | STS_BC_SLIP_V | STS_BC_JUMP, 0, s));
       HIP_OK(launch_batch<R>(BK_EDGES, 0, C, n, nullptr, 0, k, par,
                              STS_BC_VELOCITIES | STS_BC_TEMPERATURE | STS_BC_PRESSURE, 0, s));
       HIP_OK(launch_batch<R>(BK_FLUX_DIFF, cv, C, n, ct, nct, k, par, 0, 0, s));

@@ -135,6 +135,18 @@ __device__ __forceinline__ R Hi(R f1, R f2, R h1, R h2) {  // S:136
   return (h1 + h2) * f1 * f2 / (h1 * f2 + h2 * f1);
 }
 
+// One write of the wall pass.  Which faces are written, from which face or
+// cell, with which body and mesh step, follows from the mask and the body
+// table alone, which are fixed for a case; only the values change.
+enum : unsigned { WOP_U = 0u, WOP_V = 1u, WOP_TWU = 2u, WOP_TWV = 3u, WOP_NONE = 7u };
+struct WallOp {
+  unsigned dst;  // face written: U, V, TW_U or TW_V [dst]
+  unsigned src;  // slip: the fluid-side face read; wall temperature: the fluid cell
+  unsigned h;    // hy row (U, TW_V) or hx column (V, TW_U) of the half cell
+  unsigned kb;   // kind | body index << 3
+};
+constexpr int WALL_OPS_PER_TILE = 256;
+
 // One case of a batch run in shared launches (device-resident): grid and
 // fields with p and T as they stand at the start of the step, the p/T
 // ping-pong partners, loop control, residual words and log, loop-2 test.
@@ -149,6 +161,8 @@ struct BatchCase {
   double *probe;
   int N_I, stall_w;
   double tol_uv, tol_p, tol_T, stall_ratio;
+  const WallOp *wops;  // the case's planned wall writes
+  int nwops;
 };
 
 // Work item of a batched launch: tile (x, y) of case c.

@@ -363,7 +363,7 @@ __global__ void __launch_bounds__(BX *BY) k_sweep(KGridT<R> g, KFieldsT<R> fl, c
 // LDS into (Pn, Tn).  Same arithmetic as two k_sweep launches; the
 // intermediate p and T never go to HBM.  Residual maxima of both sweeps over
 // the tile cells go to (r1p, r1T) and (r2p, r2T).
-constexpr int S2X = 64, S2Y = 8, S2RX = S2X + 2, S2RY = S2Y + 2;
+constexpr int S2X = 64, S2Y = 8;
 
 template <typename R>
 __device__ __forceinline__ void sweep2_tile_body(const KGridT<R> &g, const KFieldsT<R> &fl, const R *__restrict__ Po,
@@ -599,78 +599,72 @@ __device__ __forceinline__ const DevBody *slip_cell_v(const KGridT<R> &g, const 
 }
 
 // The two writes of slip cell c for u (S:7155-7206): which==0 -> u[c.ij],
-// which==1 -> u[c.i1j].  Returns true and the value if the write happens.
+// which==1 -> u[c.i1j].  Decided by the mask alone: returns false if the
+// write does not happen, else the face it reads (src) and the hy row of the
+// half cell (h).
 template <typename R>
-__device__ __forceinline__ bool slip_u_write(const KGridT<R> &g, const KFieldsT<R> &fl, const Nbr &c, const DevBody *b,
-                                             int which, R &val) {
+__device__ __forceinline__ bool slip_u_src(const KGridT<R> &g, const Nbr &c, int which, unsigned &src, unsigned &h) {
   const uint8_t *m = g.m;
-  const R *u = FLD(U), *rho_u = FLD(RHO_U);
-  const R FK = R(b->F_slip) * g.Kn, ub = R(b->u_body);
   if (0u == g.vol[c.ij_1]) {
-    const R hh = g.hy[c.j - 1 * (0 < c.j)];
+    h = c.j - 1 * (0 < c.j);
     if (which == 0) {
       if (!m[c.i_1j_1]) return false;
-      R a = g.kn_local ? FK / (rho_u[c.ij_1] * R(0.5) * hh) : FK / (R(0.5) * hh);
-      val = (u[c.ij_1] * a + ub) / (R(1.0) + a);
+      src = c.ij_1;
     } else {
       if (!m[c.i1j_1]) return false;
-      R a = g.kn_local ? FK / (rho_u[c.i1j_1] * R(0.5) * hh) : FK / (R(0.5) * hh);
-      val = (u[c.i1j_1] * a + ub) / (R(1.0) + a);
+      src = c.i1j_1;
     }
   } else {
-    const R hh = g.hy[c.j + 1 * (c.j < (unsigned)(g.Ny - 1))];
+    h = c.j + 1 * (c.j < (unsigned)(g.Ny - 1));
     if (which == 0) {
       if (!m[c.i_1j1]) return false;
-      R a = g.kn_local ? FK / (rho_u[c.ij1] * R(0.5) * hh) : FK / (R(0.5) * hh);
-      val = (u[c.ij1] * a + ub) / (R(1.0) + a);
+      src = c.ij1;
     } else {
       if (!m[c.i1j1]) return false;
-      R a = g.kn_local ? FK / (rho_u[c.i1j1] * R(0.5) * hh) : FK / (R(0.5) * hh);
-      val = (u[c.i1j1] * a + ub) / (R(1.0) + a);
+      src = c.i1j1;
     }
   }
   return true;
 }
 
-// S:7273-7324: which==0 -> v[c.ij], which==1 -> v[c.ij1]
+// S:7273-7324: which==0 -> v[c.ij], which==1 -> v[c.ij1]; h is an hx column.
 template <typename R>
-__device__ __forceinline__ bool slip_v_write(const KGridT<R> &g, const KFieldsT<R> &fl, const Nbr &c, const DevBody *b,
-                                             int which, R &val) {
+__device__ __forceinline__ bool slip_v_src(const KGridT<R> &g, const Nbr &c, int which, unsigned &src, unsigned &h) {
   const uint8_t *m = g.m;
-  const R *v = FLD(V), *rho_v = FLD(RHO_V);
-  const R FK = R(b->F_slip) * g.Kn, vb = R(b->v_body);
   if (0u == g.vol[c.i_1j]) {
-    const R hh = g.hx[c.i_1];
+    h = c.i_1;
     if (which == 0) {
       if (!m[c.i_1j_1]) return false;
-      R a = g.kn_local ? FK / (rho_v[c.i_1j] * R(0.5) * hh) : FK / (R(0.5) * hh);
-      val = (v[c.i_1j] * a + vb) / (R(1.0) + a);
+      src = c.i_1j;
     } else {
       if (!m[c.i_1j1]) return false;
-      R a = g.kn_local ? FK / (rho_v[c.i_1j1] * R(0.5) * hh) : FK / (R(0.5) * hh);
-      val = (v[c.i_1j1] * a + vb) / (R(1.0) + a);
+      src = c.i_1j1;
     }
   } else {
-    const R hh = g.hx[c.i1];
+    h = c.i1;
     if (which == 0) {
       if (!m[c.i1j_1]) return false;
-      R a = g.kn_local ? FK / (rho_v[c.i1j] * R(0.5) * hh) : FK / (R(0.5) * hh);
-      val = (v[c.i1j] * a + vb) / (R(1.0) + a);
+      src = c.i1j;
     } else {
       if (!m[c.i1j1]) return false;
-      R a = g.kn_local ? FK / (rho_v[c.i1j1] * R(0.5) * hh) : FK / (R(0.5) * hh);
-      val = (v[c.i1j1] * a + vb) / (R(1.0) + a);
+      src = c.i1j1;
     }
   }
   return true;
 }
 
-// Wall temperature seen from a fluid cell through a face with body k (S:7762-7797).
+// Slip velocity (S:7160-7205): (vel a + vb) / (1 + a) with
+// a = F Kn / (rho_face h/2) (local Kn) or F Kn / (h/2).
 template <typename R>
-__device__ __forceinline__ R wall_T(const KGridT<R> &g, uint32_t k, R T_fluid, R rho_fluid,
-                                         R half_h) {
-  const DevBody *b = body_of(g, k);
-  if (!b) return R(0.0);
+__device__ __forceinline__ R slip_value(const KGridT<R> &g, const DevBody *b, R vb, R vel, R rho_face, R hh) {
+  const R FK = R(b->F_slip) * g.Kn;
+  const R a = g.kn_local ? FK / (rho_face * R(0.5) * hh) : FK / (R(0.5) * hh);
+  return (vel * a + vb) / (R(1.0) + a);
+}
+
+// Wall temperature seen from a fluid cell through a face with body b (S:7762-7797).
+template <typename R>
+__device__ __forceinline__ R wall_T(const KGridT<R> &g, const DevBody *b, R T_fluid, R rho_fluid, R half_h) {
   if (b->is_jump) {
     R FK = g.kn_local ? (R(b->F_jump) * g.Kn) / rho_fluid : R(b->F_jump) * g.Kn;
     R a = FK / half_h;
@@ -681,67 +675,104 @@ __device__ __forceinline__ R wall_T(const KGridT<R> &g, uint32_t k, R T_fluid, R
   return R(b->T_body);
 }
 
+__device__ __forceinline__ void set_op(WallOp &o, unsigned kind, unsigned dst, unsigned src, unsigned h,
+                                       unsigned body) {
+  o.dst = dst;
+  o.src = src;
+  o.h = h;
+  o.kb = kind | (body << 3);
+}
+
+// The writes of face k = (i, j) for the requested parts: per array the last
+// writer of the reference's walk (increasing node order; periodic about OX
+// the last column's cell follows the first column's).  op[q] for U, V, TW_U,
+// TW_V; kind WOP_NONE where nothing is written.
 template <typename R>
-__device__ __forceinline__ void wall_cell(const KGridT<R> &g, const KFieldsT<R> &fl, int i, int j, int parts) {
+__device__ __forceinline__ void plan_wall_cell(const KGridT<R> &g, int i, int j, int parts, WallOp (&op)[4]) {
+#pragma unroll
+  for (int q = 0; q < 4; q++) op[q].kb = WOP_NONE;
   if (i >= g.Nx || j >= g.Ny) return;
   const unsigned Nx = (unsigned)g.Nx, Ny = (unsigned)g.Ny;
   const unsigned k = (unsigned)i + (unsigned)j * Nx;
   const uint8_t *m = g.m;
   const bool px = g.px != 0;
+  unsigned src = 0u, h = 0u;
   if (parts & STS_BC_SLIP_U) {
-    R val = R(0.0);
-    bool w = false;
     const Nbr c((unsigned)i, (unsigned)j, Nx, Ny, px);
     const DevBody *b = slip_cell_u(g, c);
+    const unsigned bi = b ? (unsigned)(b - g.bodies) : 0u;
     if (px && i == 0) {
       // periodic: u[ij] of the first column is written by the cell itself,
       // then by the last column's cell (node ij + Nx - 1, later in the walk)
-      if (b) w |= slip_u_write(g, fl, c, b, 0, val);
+      if (b && slip_u_src(g, c, 0, src, h)) set_op(op[0], WOP_U, k, src, h, bi);
       const Nbr cw(Nx - 1u, (unsigned)j, Nx, Ny, true);
       const DevBody *bw = slip_cell_u(g, cw);
-      if (bw) w |= slip_u_write(g, fl, cw, bw, 1, val);
+      if (bw && slip_u_src(g, cw, 1, src, h)) set_op(op[0], WOP_U, k, src, h, (unsigned)(bw - g.bodies));
     } else {
       if (i >= 1) {  // cell (i-1, j) writes u[i1j]
         const Nbr cw((unsigned)i - 1u, (unsigned)j, Nx, Ny, px);
         const DevBody *bw = slip_cell_u(g, cw);
-        if (bw) w |= slip_u_write(g, fl, cw, bw, 1, val);
+        if (bw && slip_u_src(g, cw, 1, src, h)) set_op(op[0], WOP_U, k, src, h, (unsigned)(bw - g.bodies));
       }
       if (b) {
-        w |= slip_u_write(g, fl, c, b, 0, val);
-        if (!px && (unsigned)i == Nx - 1u) w |= slip_u_write(g, fl, c, b, 1, val);  // clamped i1j == ij
+        if (slip_u_src(g, c, 0, src, h)) set_op(op[0], WOP_U, k, src, h, bi);
+        if (!px && (unsigned)i == Nx - 1u && slip_u_src(g, c, 1, src, h))  // clamped i1j == ij
+          set_op(op[0], WOP_U, k, src, h, bi);
       }
     }
-    if (w) FLD(U)[k] = val;
   }
   if (parts & STS_BC_SLIP_V) {
-    R val = R(0.0);
-    bool w = false;
     if (j >= 1) {  // cell (i, j-1) writes v[ij1]
       const Nbr c((unsigned)i, (unsigned)j - 1u, Nx, Ny, px);
       const DevBody *b = slip_cell_v(g, c);
-      if (b) w |= slip_v_write(g, fl, c, b, 1, val);
+      if (b && slip_v_src(g, c, 1, src, h)) set_op(op[1], WOP_V, k, src, h, (unsigned)(b - g.bodies));
     }
     const Nbr c((unsigned)i, (unsigned)j, Nx, Ny, px);
     const DevBody *b = slip_cell_v(g, c);
     if (b) {
-      w |= slip_v_write(g, fl, c, b, 0, val);
-      if ((unsigned)j == Ny - 1u) w |= slip_v_write(g, fl, c, b, 1, val);  // clamped ij1 == ij
+      const unsigned bi = (unsigned)(b - g.bodies);
+      if (slip_v_src(g, c, 0, src, h)) set_op(op[1], WOP_V, k, src, h, bi);
+      if ((unsigned)j == Ny - 1u && slip_v_src(g, c, 1, src, h))  // clamped ij1 == ij
+        set_op(op[1], WOP_V, k, src, h, bi);
     }
-    if (w) FLD(V)[k] = val;
   }
   if (parts & STS_BC_JUMP) {
-    const R *T = FLD(T), *rho = FLD(RHO);
     if (i >= 1 || px) {  // face between cell iw (west, across the seam if periodic) and cell i
       const unsigned iw = (i >= 1) ? (unsigned)i - 1u : Nx - 1u, w_ = iw + (unsigned)j * Nx;
-      if (m[k] && !m[w_]) FLD(TW_U)[k] = wall_T(g, g.vol[w_], T[k], rho[k], R(0.5) * g.hx[i]);
-      else if (m[w_] && !m[k]) FLD(TW_U)[k] = wall_T(g, g.vol[k], T[w_], rho[w_], R(0.5) * g.hx[iw]);
+      if (m[k] && !m[w_]) set_op(op[2], WOP_TWU, k, k, (unsigned)i, g.vol[w_] - 1u);
+      else if (m[w_] && !m[k]) set_op(op[2], WOP_TWU, k, w_, iw, g.vol[k] - 1u);
     }
     if (j >= 1) {
       const unsigned s_ = k - Nx;
-      if (m[k] && !m[s_]) FLD(TW_V)[k] = wall_T(g, g.vol[s_], T[k], rho[k], R(0.5) * g.hy[j]);
-      else if (m[s_] && !m[k]) FLD(TW_V)[k] = wall_T(g, g.vol[k], T[s_], rho[s_], R(0.5) * g.hy[j - 1]);
+      if (m[k] && !m[s_]) set_op(op[3], WOP_TWV, k, k, (unsigned)j, g.vol[s_] - 1u);
+      else if (m[s_] && !m[k]) set_op(op[3], WOP_TWV, k, s_, (unsigned)j - 1u, g.vol[k] - 1u);
     }
   }
+}
+
+// Value of one planned write; every value read is a face or cell that no
+// write of the pass modifies, so the writes are independent.
+template <typename R>
+__device__ __forceinline__ void apply_wall_op(const KGridT<R> &g, const KFieldsT<R> &fl, const WallOp &o) {
+  const unsigned kind = o.kb & 7u;
+  if (kind == WOP_NONE) return;
+  const DevBody *b = &g.bodies[o.kb >> 3];
+  if (kind == WOP_U)
+    FLD(U)[o.dst] = slip_value(g, b, R(b->u_body), FLD(U)[o.src], FLD(RHO_U)[o.src], g.hy[o.h]);
+  else if (kind == WOP_V)
+    FLD(V)[o.dst] = slip_value(g, b, R(b->v_body), FLD(V)[o.src], FLD(RHO_V)[o.src], g.hx[o.h]);
+  else if (kind == WOP_TWU)
+    FLD(TW_U)[o.dst] = wall_T(g, b, FLD(T)[o.src], FLD(RHO)[o.src], R(0.5) * g.hx[o.h]);
+  else
+    FLD(TW_V)[o.dst] = wall_T(g, b, FLD(T)[o.src], FLD(RHO)[o.src], R(0.5) * g.hy[o.h]);
+}
+
+template <typename R>
+__device__ __forceinline__ void wall_cell(const KGridT<R> &g, const KFieldsT<R> &fl, int i, int j, int parts) {
+  WallOp op[4];
+  plan_wall_cell(g, i, j, parts, op);
+#pragma unroll
+  for (int q = 0; q < 4; q++) apply_wall_op(g, fl, op[q]);
 }
 
 template <typename R>
@@ -752,16 +783,29 @@ __global__ void __launch_bounds__(BX *BY) k_wall(KGridT<R> g, KFieldsT<R> fl, in
   wall_cell(g, fl, i0 + (int)threadIdx.x, j0 + (int)threadIdx.y, parts);
 }
 
-// The same over a list of the only cells that can be written: those next to
-// a body/fluid interface (the list is built once from the mask).
+// The planned writes of a case (built once from its mask), one per thread.
 template <typename R>
-__global__ void __launch_bounds__(256) k_wall_list(KGridT<R> g, KFieldsT<R> fl, const unsigned *cells, int n,
-                                                   int parts, Gate gate) {
+__global__ void __launch_bounds__(256) k_wall_ops(KGridT<R> g, KFieldsT<R> fl, const WallOp *__restrict__ ops, int n,
+                                                  int parts, Gate gate) {
   if (gated_out(gate)) return;
   const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (t >= n) return;
+  const WallOp o = ops[t];
+  const unsigned kind = o.kb & 7u;
+  const int need = kind == WOP_U ? STS_BC_SLIP_U : kind == WOP_V ? STS_BC_SLIP_V : STS_BC_JUMP;
+  if (parts & need) apply_wall_op(g, fl, o);
+}
+
+// Planning pass: the four planned writes of each listed cell (out[4 t + q]).
+__global__ void __launch_bounds__(256) k_wall_plan(KGrid g, const unsigned *cells, int n, WallOp *out) {
+  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (t >= n) return;
   const unsigned k = cells[t];
-  wall_cell(g, fl, (int)(k % (unsigned)g.Nx), (int)(k / (unsigned)g.Nx), parts);
+  WallOp op[4];
+  plan_wall_cell(g, (int)(k % (unsigned)g.Nx), (int)(k / (unsigned)g.Nx), STS_BC_SLIP_U | STS_BC_SLIP_V | STS_BC_JUMP,
+                 op);
+#pragma unroll
+  for (int q = 0; q < 4; q++) out[4 * (size_t)t + q] = op[q];
 }
 
 // ---------------------------------------------------------------------------
@@ -1373,6 +1417,20 @@ __global__ void __launch_bounds__(TX *TY) kb_wall(const BatchCase<R> *C, const B
   wall_cell(c.g, f, t.x * TX + (int)threadIdx.x % TX, t.y * TY + (int)threadIdx.x / TX, parts);
 }
 
+template <typename R>
+__global__ void __launch_bounds__(WALL_OPS_PER_TILE) kb_wall_ops(const BatchCase<R> *C, const BatchTile *T, int k,
+                                                                 int par, int parts) {
+  const BatchTile t = T[blockIdx.x];
+  const BatchCase<R> &c = C[t.c];
+  if (gated_out(Gate{c.ctrl, k})) return;
+  const int q = t.x * WALL_OPS_PER_TILE + (int)threadIdx.x;
+  if (q >= c.nwops) return;
+  const WallOp o = c.wops[q];
+  const unsigned kind = o.kb & 7u;
+  const int need = kind == WOP_U ? STS_BC_SLIP_U : kind == WOP_V ? STS_BC_SLIP_V : STS_BC_JUMP;
+  if (parts & need) apply_wall_op(c.g, fields_at(c, par), o);
+}
+
 template <typename R, int TX, int TY>
 __global__ void __launch_bounds__(TX *TY) kb_flux_diff(const BatchCase<R> *C, const BatchTile *T, int k, int par) {
   const BatchTile t = T[blockIdx.x];
@@ -1450,11 +1508,10 @@ hipError_t launch_correct(const KGridT<R> &g, const KFieldsT<R> &f, unsigned lon
 }
 template <typename R>
 hipError_t launch_wall(const KGridT<R> &g, const KFieldsT<R> &f, int parts, hipStream_t s, Gate gate,
-                       const unsigned *cells, int ncells) {
-  if (cells) {
-    if (ncells > 0)
-      hipLaunchKernelGGL(k_wall_list<R>, dim3((ncells + 255) / 256), dim3(256), 0, s, g, f, cells, ncells, parts,
-                         gate);
+                       const WallOp *ops, int nops) {
+  if (ops) {
+    if (nops > 0)
+      hipLaunchKernelGGL(k_wall_ops<R>, dim3((nops + 255) / 256), dim3(256), 0, s, g, f, ops, nops, parts, gate);
   } else {
     hipLaunchKernelGGL(k_wall<R>, (grid_tiles<BX, BY>(g)), dim3(BX, BY), 0, s, g, f, parts, gate);
   }
@@ -1484,7 +1541,7 @@ hipError_t launch_flux_diff(const KGridT<R> &g, const KFieldsT<R> &f, hipStream_
   template hipError_t launch_correct<R>(const KGridT<R> &, const KFieldsT<R> &, unsigned long long *,          \
                                         unsigned long long *, int, hipStream_t, Gate);                         \
   template hipError_t launch_wall<R>(const KGridT<R> &, const KFieldsT<R> &, int, hipStream_t, Gate,           \
-                                     const unsigned *, int);                                                  \
+                                     const WallOp *, int);                                                    \
   template hipError_t launch_edges<R>(const KGridT<R> &, const KFieldsT<R> &, int, double *, hipStream_t, Gate, \
                                       LoopTest);                                                               \
   template hipError_t launch_flux_diff<R>(const KGridT<R> &, const KFieldsT<R> &, hipStream_t, Gate);
@@ -1545,6 +1602,11 @@ hipError_t launch_drag(const KGrid &g, const KFields &f, const unsigned *cells, 
   hipLaunchKernelGGL(k_drag_sum, dim3(1), dim3(64), 0, s, terms, n, out);
   return hipGetLastError();
 }
+hipError_t launch_wall_plan(const KGrid &g, const unsigned *cells, int n, WallOp *out, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_wall_plan, dim3((n + 255) / 256), dim3(256), 0, s, g, cells, n, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_fold(unsigned long long *slots, int n, unsigned long long *out, hipStream_t s) {
   hipLaunchKernelGGL(k_fold, dim3(1), dim3(128), 0, s, slots, n, out);
   return hipGetLastError();
@@ -1583,6 +1645,10 @@ hipError_t launch_batch(int kernel, int variant, const BatchCase<R> *C, int ncas
     case BK_SWEEP: CELL_SWITCH(variant, kb_sweep, C, T, k, par, a, b); break;
     case BK_CORRECT: CELL_SWITCH(variant, kb_correct, C, T, k, par); break;
     case BK_WALL: CELL_SWITCH(variant, kb_wall, C, T, k, par, a); break;
+    case BK_WALL_OPS:
+      if (ntiles > 0)
+        hipLaunchKernelGGL(kb_wall_ops<R>, dim3(ntiles), dim3(WALL_OPS_PER_TILE), 0, s, C, T, k, par, a);
+      break;
     case BK_FLUX_DIFF: CELL_SWITCH(variant, kb_flux_diff, C, T, k, par); break;
     case BK_SWEEP2:
       switch (variant) {

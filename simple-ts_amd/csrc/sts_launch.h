@@ -22,7 +22,9 @@ hipError_t launch_correct(const KGridT<R> &g, const KFieldsT<R> &f, unsigned lon
 // cells: list of the cells next to a body/fluid interface (or null: all cells)
 template <typename R>
 hipError_t launch_wall(const KGridT<R> &g, const KFieldsT<R> &f, int parts, hipStream_t s,
-                       Gate gate = Gate{nullptr, 0}, const unsigned *cells = nullptr, int ncells = 0);
+                       Gate gate = Gate{nullptr, 0}, const WallOp *ops = nullptr, int nops = 0);
+// The planned writes of the listed cells: out[4 t + q], q = U, V, TW_U, TW_V.
+hipError_t launch_wall_plan(const KGrid &g, const unsigned *cells, int n, WallOp *out, hipStream_t s);
 template <typename R>
 hipError_t launch_edges(const KGridT<R> &g, const KFieldsT<R> &f, int parts, double *probe, hipStream_t s,
                         Gate gate = Gate{nullptr, 0},
@@ -54,7 +56,9 @@ hipError_t launch_continuity(const KGrid &g, const KFields &f, double *cs, doubl
 // C[0..ncases).  kernel: BK_*; variant: index into CELL_TILES (cell
 // kernels) or SWEEP_TILES (BK_SWEEP2).  (a, b) = (first, last) for the
 // sweeps, a = parts for BK_WALL and BK_EDGES.
-enum { BK_ASSEMBLE, BK_SWEEP2, BK_SWEEP, BK_CORRECT, BK_WALL, BK_EDGES, BK_FLUX_DIFF, BK_LOOP_OPEN };
+enum { BK_ASSEMBLE, BK_SWEEP2, BK_SWEEP, BK_CORRECT, BK_WALL, BK_EDGES, BK_FLUX_DIFF, BK_LOOP_OPEN, BK_WALL_OPS };
+// BK_WALL_OPS: tile (c, x) = planned wall writes [x W, x W + W) of case c,
+// W = WALL_OPS_PER_TILE.
 constexpr int NVARIANT = 4;
 extern const int CELL_TILES[NVARIANT][2];   // (TX, TY)
 extern const int SWEEP_TILES[NVARIANT][2];  // (SX, SY)
