@@ -204,8 +204,15 @@ typedef struct sts_mixed_control {
   double tol32_uv, tol32_p, tol32_T;  /* FP32-phase tolerances */
   int stall_w;                        /* stagnation window (0: off) */
   double stall_ratio;                 /* required shrink over the window */
-  int extrapolate;                    /* 1: start loop 2 from 2 x^n - x^(n-1) (u, v, p, T on their update
-                                         sets, the rest rebuilt from them) instead of x^n */
+  int extrapolate;                    /* q = 1..5: start loop 2 from the degree-q polynomial in time
+                                         through the last q+1 converged steps (q = 1: 2 x^n - x^(n-1);
+                                         q = 2: 3 x^n - 3 x^(n-1) + x^(n-2); ...) for u, v, p, T on every
+                                         cell, the derived fields rebuilt from them, instead of x^n; a
+                                         cell whose guess is not finite (or p, T not positive) keeps x^n;
+                                         fewer stored steps give a lower degree; 0: off */
+  int fp32_above;                     /* > 0: run the FP32 phase only when the context's previous step
+                                         took more than this many outer iterations (a well-started step
+                                         goes straight to FP64); 0: always */
 } sts_mixed_control;
 int sts_time_step_mixed(sts_ctx *ctx, const sts_loop_control *lc, const sts_mixed_control *mc, int *iters32,
                         int *iters64, double *res_log, double *p_probe);

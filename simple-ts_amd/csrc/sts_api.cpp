@@ -67,6 +67,7 @@ struct sts_ctx {
   float *f32[STS_NFIELDS] = {};
   float *p_alt32 = nullptr, *T_alt32 = nullptr;
   int last_iters32 = 0, last_iters64m = 0;
+  int last_total = -1;  // outer iterations of the previous mixed or warm-started step (-1: none)
   long long fluid_cells = 0;
   WallOp *wall_ops = nullptr;  // the wall pass's writes, planned once from the mask (k_wall_ops)
   int n_wall_ops = 0;
@@ -77,8 +78,8 @@ struct sts_ctx {
   double *drag_terms = nullptr, *drag_out = nullptr;
   size_t drag_cap = 0;
   double *p_ws = nullptr, *T_ws = nullptr;  // predicted starting guess (warm start)
-  double *prev[4] = {};                      // u, v, p, T at the start of the previous step (extrapolation)
-  bool have_prev = false;
+  ExtrapHist hist{};  // u, v, p, T of the last converged steps (extrapolated starts)
+  int n_hist = 0;     // stored levels
   bool fuse_sweeps = true;  // pairs of loop-3 sweeps in one k_sweep2 pass
   // kernel timing: event pairs around launches of kernel `timed`
   int timed = -1;
@@ -204,7 +205,8 @@ extern "C" int sts_destroy(sts_ctx *c) {
   (void)hipFree(c->T_alt);
   for (int q = 0; q < STS_NFIELDS; q++) (void)hipFree(c->f32[q]);
   (void)hipFree(c->p_ws);
-  for (double *q : c->prev) (void)hipFree(q);
+  for (auto &lv : c->hist.f)
+    for (double *q : lv) (void)hipFree(q);
   (void)hipFree(c->wall_ops);
   for (unsigned *q : c->drag_cells) (void)hipFree(q);
   (void)hipFree(c->drag_terms);
@@ -439,7 +441,8 @@ extern "C" int sts_create(const sts_grid *gi, int precision, sts_ctx **out) {
 
 extern "C" int sts_set_field(sts_ctx *c, int field, const double *host) {
   if (!c || !host || field < 0 || field >= STS_NFIELDS) return fail(STS_ERR_ARG, "bad field or pointer");
-  c->have_prev = false;  // the state no longer continues the previous steps
+  c->n_hist = 0;  // the state no longer continues the previous steps
+  c->last_total = -1;
   HIP_OK(hipMemcpyAsync(c->f[field], host, sizeof(double) * (size_t)c->Na, hipMemcpyHostToDevice, c->stream));
   HIP_OK(hipStreamSynchronize(c->stream));
   return STS_OK;
@@ -844,26 +847,25 @@ static constexpr unsigned long long WIDEN_MASK =
     bit(STS_RHO_V) | bit(STS_FX) | bit(STS_FY) | bit(STS_GAMMA_XF) | bit(STS_GAMMA_YF) | bit(STS_D_UX) |
     bit(STS_D_UY) | bit(STS_D_VX) | bit(STS_D_VY) | bit(STS_DTX) | bit(STS_DTY) | bit(STS_TW_U) | bit(STS_TW_V);
 
-// After the prologue: loop 2 starts from 2 x^n - x^(n-1) for u, v, p, T (the
-// first time only x^n is stored), and every field derived from them is
-// rebuilt the way the end of an outer iteration builds it.
-static int extrapolate_start(sts_ctx *c, bool on) {
-  if (!on) {
-    c->have_prev = false;
+// After the prologue: loop 2 starts from the polynomial of degree `order`
+// (capped by the stored steps) through x^n and the previous converged steps
+// for u, v, p, T, and every field derived from them is rebuilt the way the
+// end of an outer iteration builds it.  The history then advances a step.
+static int extrapolate_start(sts_ctx *c, int order) {
+  if (order <= 0) {
+    c->n_hist = 0;
     return STS_OK;
   }
+  if (order > MAX_EXTRAP) return fail(STS_ERR_ARG, "extrapolation degree must be 0..5");
   const size_t bytes = sizeof(double) * (size_t)c->Na;
-  if (!c->prev[0])
-    for (double *&q : c->prev) HIP_OK(hipMalloc(&q, bytes));
-  if (!c->have_prev) {
-    const int src[4] = {STS_U_PR, STS_V_PR, STS_P_PR, STS_T_PR};
-    for (int q = 0; q < 4; q++)
-      HIP_OK(hipMemcpyAsync(c->prev[q], c->f[src[q]], bytes, hipMemcpyDeviceToDevice, c->stream));
-    c->have_prev = true;
-    return STS_OK;
-  }
+  if (!c->hist.f[0][0])
+    for (auto &lv : c->hist.f)
+      for (double *&q : lv) HIP_OK(hipMalloc(&q, bytes));
+  const int keep = c->n_hist, use = std::min(order, keep);
   const KFields kf = c->fields();
-  HIP_OK(launch_extrapolate(c->g, kf, c->prev, c->stream));
+  HIP_OK(launch_extrapolate(c->g, kf, c->hist, use, keep, c->stream));
+  c->n_hist = std::min(keep + 1, MAX_EXTRAP);
+  if (use == 0) return STS_OK;
   HIP_OK(launch_correct(c->g, kf, nullptr, nullptr, 0, c->stream));
   HIP_OK(launch_wall(c->g, kf, STS_BC_SLIP_U | STS_BC_SLIP_V | STS_BC_JUMP, c->stream, Gate{nullptr, 0},
                      c->wall_ops, c->n_wall_ops));
@@ -877,7 +879,8 @@ static int extrapolate_start(sts_ctx *c, bool on) {
 static int run_phases(sts_ctx *c, const sts_loop_control *lc, const sts_mixed_control *mc, int *n32p, int *n64p) {
   int rc, n32 = 0, n64 = 0;
   *n32p = *n64p = 0;
-  if (mc && mc->N_I32 > 0) {
+  const bool fp32 = mc && mc->N_I32 > 0 && (mc->fp32_above <= 0 || c->last_total < 0 || c->last_total > mc->fp32_above);
+  if (fp32) {
     const unsigned long long all = (1ull << STS_NFIELDS) - 1ull;
     HIP_OK(launch_convert(c->fields(), c->fields32(), all, c->Na, c->stream));
     Loop2State<float> s32{&c->g32, c->fields32(), c->p_alt32, c->T_alt32, STS_K_COUNT};
@@ -899,6 +902,7 @@ static int run_phases(sts_ctx *c, const sts_loop_control *lc, const sts_mixed_co
                  &n64);
   last = n64;
   *n64p = n64;
+  c->last_total = n32 + n64;
   return rc;
 }
 
@@ -916,7 +920,7 @@ extern "C" int sts_time_step_mixed(sts_ctx *c, const sts_loop_control *lc, const
   if ((rc = check_mixed(c, lc, mc))) return rc;
   if ((rc = ensure_loop_buffers(c, lc->N_I + mc->N_I32))) return rc;
   if ((rc = sts_save_previous(c))) return rc;
-  if ((rc = extrapolate_start(c, mc->extrapolate != 0))) return rc;
+  if ((rc = extrapolate_start(c, mc->extrapolate))) return rc;
   int n32 = 0, n64 = 0;
   rc = run_phases(c, lc, mc, &n32, &n64);
   if (iters32) *iters32 = n32;
@@ -957,7 +961,7 @@ extern "C" int sts_time_step_ws(sts_ctx *c, sts_unet *net, const sts_unet_norm *
     HIP_OK(hipMalloc(&c->T_ws, sizeof(double) * (size_t)c->Na));
   }
   if ((rc = sts_save_previous(c))) return rc;
-  if ((rc = extrapolate_start(c, mc && mc->extrapolate != 0))) return rc;
+  if ((rc = extrapolate_start(c, mc ? mc->extrapolate : 0))) return rc;
   StsView v;
   sts_internal_view(c, &v);
   if ((rc = sts_internal_unet_predict(net, v, nrm, c->p_ws, c->T_ws))) return rc;

@@ -147,6 +147,12 @@ struct WallOp {
 };
 constexpr int WALL_OPS_PER_TILE = 256;
 
+// Converged u, v, p, T of the last steps: f[h] = x^(n-1-h) (extrapolated starts).
+constexpr int MAX_EXTRAP = 5;
+struct ExtrapHist {
+  double *f[MAX_EXTRAP][4];  // [h][u, v, p, T]
+};
+
 // One case of a batch run in shared launches (device-resident): grid and
 // fields with p and T as they stand at the start of the step, the p/T
 // ping-pong partners, loop control, residual words and log, loop-2 test.

@@ -1155,24 +1155,50 @@ __global__ void __launch_bounds__(BX *BY) k_copy5(const double *a0, const double
 // Start of loop 2 from a linear extrapolation in time: x = 2 x^n - x^(n-1)
 // for u, v, p, T on their update sets (x^n = the *_pr fields after the
 // prologue); x^(n-1) (prev) then takes x^n.
-__global__ void __launch_bounds__(BX *BY) k_extrapolate(KGrid g, KFields fl, double *pu, double *pv, double *pp,
-                                                        double *pT) {
+// Polynomial extrapolation in time of degree `order` through x^n and the
+// stored steps, as Newton's backward-difference sum x0 + D1 + ... + Dq
+// (D1 = x0 - x1, D(k+1) = Dk(x0) - Dk(x1)), so that a value constant in time
+// (fixed inflow, outflow pressure, body values) is kept exactly; then the
+// history shifts (keep = number of levels stored).  Every cell is
+// extrapolated: besides the update sets, the cells the boundary conditions
+// copy into (e.g. the inflow pressure, which sets the inflow density) carry
+// the step's slowest-converging change.
+__device__ __forceinline__ double extrap(int order, const double (&x)[MAX_EXTRAP + 1]) {
+  double d[MAX_EXTRAP + 1];
+#pragma unroll
+  for (int h = 0; h <= MAX_EXTRAP; h++) d[h] = x[h];
+  double s = x[0];
+#pragma unroll
+  for (int k = 1; k <= MAX_EXTRAP; k++) {
+    if (k > order) break;
+#pragma unroll
+    for (int h = 0; h + k <= MAX_EXTRAP; h++) d[h] = d[h] - d[h + 1];  // d[h] = D^k x^(n-h)
+    s += d[0];
+  }
+  return s;
+}
+
+__global__ void __launch_bounds__(BX *BY) k_extrapolate(KGrid g, KFields fl, ExtrapHist hist, int order, int keep) {
   const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
   if (i >= g.Nx || j >= g.Ny) return;
-  const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny, g.px);
-  const unsigned k = n.ij;
-  const uint8_t *m = g.m;
-  const double un = FLD(U_PR)[k], vn = FLD(V_PR)[k], pn = FLD(P_PR)[k], Tn = FLD(T_PR)[k];
-  if (in_box(i, j, g.Nx_u0, g.Nx_u1, g.Ny_u0, g.Ny_u1) && m[n.i_1j] && m[k]) FLD(U)[k] = 2.0 * un - pu[k];
-  if (in_box(i, j, g.Nx_v0, g.Nx_v1, g.Ny_v0, g.Ny_v1) && m[n.ij_1] && m[k]) FLD(V)[k] = 2.0 * vn - pv[k];
-  if (in_box(i, j, g.Nx_o0, g.Nx_o1, g.Ny_o0, g.Ny_o1) && m[k]) {
-    FLD(P)[k] = 2.0 * pn - pp[k];
-    FLD(T)[k] = 2.0 * Tn - pT[k];
+  const unsigned k = (unsigned)i + (unsigned)j * (unsigned)g.Nx;
+  const int src[4] = {STS_U_PR, STS_V_PR, STS_P_PR, STS_T_PR}, dst[4] = {STS_U, STS_V, STS_P, STS_T};
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    double x[MAX_EXTRAP + 1];
+    x[0] = fl.f[src[q]][k];
+#pragma unroll
+    for (int h = 0; h < MAX_EXTRAP; h++) x[h + 1] = (h < keep) ? hist.f[h][q][k] : 0.0;
+    if (order > 0) {
+      // a guess that is not finite, or a pressure or temperature that is not
+      // positive, keeps the previous step's value in that cell
+      const double e = extrap(order, x);
+      const bool ok = isfinite(e) && (q < 2 || e > 0.0 || x[0] <= 0.0);
+      fl.f[dst[q]][k] = ok ? e : x[0];
+    }
+    for (int h = (keep < MAX_EXTRAP ? keep : MAX_EXTRAP - 1); h > 0; h--) hist.f[h][q][k] = x[h];
+    hist.f[0][q][k] = x[0];
   }
-  pu[k] = un;
-  pv[k] = vn;
-  pp[k] = pn;
-  pT[k] = Tn;
 }
 
 __global__ void __launch_bounds__(BX *BY) k_set_inflow(KGrid g, KFields fl, double u_in) {
@@ -1572,8 +1598,9 @@ hipError_t launch_copy5(const double *const *src, double *const *dst, int n, hip
                      dst[3], dst[4], n);
   return hipGetLastError();
 }
-hipError_t launch_extrapolate(const KGrid &g, const KFields &f, double *const prev[4], hipStream_t s) {
-  hipLaunchKernelGGL(k_extrapolate, grid2d(g), dim3(BX, BY), 0, s, g, f, prev[0], prev[1], prev[2], prev[3]);
+hipError_t launch_extrapolate(const KGrid &g, const KFields &f, const ExtrapHist &hist, int order, int keep,
+                              hipStream_t s) {
+  hipLaunchKernelGGL(k_extrapolate, grid2d(g), dim3(BX, BY), 0, s, g, f, hist, order, keep);
   return hipGetLastError();
 }
 hipError_t launch_set_inflow(const KGrid &g, const KFields &f, double u_in, hipStream_t s) {
