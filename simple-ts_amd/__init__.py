@@ -368,7 +368,8 @@ class Batch:
 
     def time_step(self, N_I, N_I_p_c, tol_uv, tol_p, tol_T, mixed=None, batch=0):
         """One time step of every case; mixed = None or (N_I32, tol32,
-        stall_w, stall_ratio).  Returns (FP32 counts, FP64 counts)."""
+        stall_w, stall_ratio[, extrapolate[, fp32_above]]).  Returns (FP32
+        counts, FP64 counts)."""
         n = len(self.contexts)
         lc = LoopControl(N_I, N_I_p_c, tol_uv, tol_p, tol_T, batch)
         mc = None

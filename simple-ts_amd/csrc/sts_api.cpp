@@ -851,7 +851,8 @@ static constexpr unsigned long long WIDEN_MASK =
 // (capped by the stored steps) through x^n and the previous converged steps
 // for u, v, p, T, and every field derived from them is rebuilt the way the
 // end of an outer iteration builds it.  The history then advances a step.
-static int extrapolate_start(sts_ctx *c, int order) {
+static int extrapolate_start(sts_ctx *c, int order, hipStream_t s = nullptr) {
+  if (!s) s = c->stream;
   if (order <= 0) {
     c->n_hist = 0;
     return STS_OK;
@@ -863,15 +864,21 @@ static int extrapolate_start(sts_ctx *c, int order) {
       for (double *&q : lv) HIP_OK(hipMalloc(&q, bytes));
   const int keep = c->n_hist, use = std::min(order, keep);
   const KFields kf = c->fields();
-  HIP_OK(launch_extrapolate(c->g, kf, c->hist, use, keep, c->stream));
+  HIP_OK(launch_extrapolate(c->g, kf, c->hist, use, keep, s));
   c->n_hist = std::min(keep + 1, MAX_EXTRAP);
   if (use == 0) return STS_OK;
-  HIP_OK(launch_correct(c->g, kf, nullptr, nullptr, 0, c->stream));
-  HIP_OK(launch_wall(c->g, kf, STS_BC_SLIP_U | STS_BC_SLIP_V | STS_BC_JUMP, c->stream, Gate{nullptr, 0},
-                     c->wall_ops, c->n_wall_ops));
-  HIP_OK(launch_edges(c->g, kf, STS_BC_VELOCITIES | STS_BC_TEMPERATURE | STS_BC_PRESSURE, nullptr, c->stream));
-  HIP_OK(launch_flux_diff(c->g, kf, c->stream));
+  HIP_OK(launch_correct(c->g, kf, nullptr, nullptr, 0, s));
+  HIP_OK(launch_wall(c->g, kf, STS_BC_SLIP_U | STS_BC_SLIP_V | STS_BC_JUMP, s, Gate{nullptr, 0}, c->wall_ops,
+                     c->n_wall_ops));
+  HIP_OK(launch_edges(c->g, kf, STS_BC_VELOCITIES | STS_BC_TEMPERATURE | STS_BC_PRESSURE, nullptr, s));
+  HIP_OK(launch_flux_diff(c->g, kf, s));
   return STS_OK;
+}
+
+// Whether this step of context c runs an FP32 phase (mc asks for one and the
+// gate, if any, lets it: the previous step needed more than fp32_above).
+static bool fp32_phase(const sts_ctx *c, const sts_mixed_control *mc) {
+  return mc && mc->N_I32 > 0 && (mc->fp32_above <= 0 || c->last_total < 0 || c->last_total > mc->fp32_above);
 }
 
 // Loop 2 after the prologue: the FP32 phase (when mc asks for one), then the
@@ -879,8 +886,7 @@ static int extrapolate_start(sts_ctx *c, int order) {
 static int run_phases(sts_ctx *c, const sts_loop_control *lc, const sts_mixed_control *mc, int *n32p, int *n64p) {
   int rc, n32 = 0, n64 = 0;
   *n32p = *n64p = 0;
-  const bool fp32 = mc && mc->N_I32 > 0 && (mc->fp32_above <= 0 || c->last_total < 0 || c->last_total > mc->fp32_above);
-  if (fp32) {
+  if (fp32_phase(c, mc)) {
     const unsigned long long all = (1ull << STS_NFIELDS) - 1ull;
     HIP_OK(launch_convert(c->fields(), c->fields32(), all, c->Na, c->stream));
     Loop2State<float> s32{&c->g32, c->fields32(), c->p_alt32, c->T_alt32, STS_K_COUNT};
@@ -1111,7 +1117,8 @@ extern "C" int sts_batch_set_tiles(sts_batch *b, int cell_variant, int sweep_var
 // Case table of one precision at the current p/T pointers of every case.
 template <typename R>
 static int upload_cases(sts_batch *b, BatchCase<R> *dst, int N_I, double tol_uv, double tol_p, double tol_T,
-                        int stall_w, double stall_ratio, size_t log_off, bool gated) {
+                        int stall_w, double stall_ratio, size_t log_off, bool gated,
+                        const std::vector<char> *skip = nullptr) {
   std::vector<BatchCase<R>> h(b->cs.size());
   for (size_t i = 0; i < b->cs.size(); i++) {
     sts_ctx *c = b->cs[i];
@@ -1139,6 +1146,7 @@ static int upload_cases(sts_batch *b, BatchCase<R> *dst, int N_I, double tol_uv,
     x.stall_ratio = stall_ratio;
     x.wops = c->wall_ops;
     x.nwops = c->n_wall_ops;
+    x.skip = skip ? (*skip)[i] : 0;
   }
   HIP_OK(hipMemcpyAsync(dst, h.data(), sizeof(BatchCase<R>) * h.size(), hipMemcpyHostToDevice, b->stream));
   return STS_OK;
@@ -1210,7 +1218,6 @@ extern "C" int sts_batch_time_step(sts_batch *b, const sts_loop_control *lc, con
   if (!b) return fail(STS_ERR_ARG, "null batch");
   int rc;
   const int n = (int)b->cs.size();
-  if (mc && mc->extrapolate) return fail(STS_ERR_UNSUPPORTED, "batched steps start from x^n (no extrapolation)");
   for (sts_ctx *c : b->cs) {
     if ((rc = check_mixed(c, lc, mc))) return rc;
     if (c->fuse_sweeps != b->cs[0]->fuse_sweeps) return fail(STS_ERR_ARG, "cases differ in sweep fusion");
@@ -1228,18 +1235,32 @@ extern "C" int sts_batch_time_step(sts_batch *b, const sts_loop_control *lc, con
     const double *src[5] = {c->f[STS_P], c->f[STS_U], c->f[STS_V], c->f[STS_T], c->f[STS_RHO]};
     double *dst[5] = {c->f[STS_P_PR], c->f[STS_U_PR], c->f[STS_V_PR], c->f[STS_T_PR], c->f[STS_RHO_PR]};
     HIP_OK(launch_copy5(src, dst, c->Na, s));
+    if ((rc = extrapolate_start(c, mc ? mc->extrapolate : 0, s))) return rc;
   }
   std::vector<int> n32(n, 0), n64(n, 0);
-  if (mc && mc->N_I32 > 0) {
+  // cases the FP32 gate sends straight to FP64 sit the FP32 phase out
+  std::vector<char> skip32(n, 1);
+  bool any32 = false;
+  for (int i = 0; i < n; i++) {
+    skip32[i] = fp32_phase(b->cs[i], mc) ? 0 : 1;
+    any32 = any32 || !skip32[i];
+  }
+  if (any32) {
     const unsigned long long all = (1ull << STS_NFIELDS) - 1ull;
-    for (sts_ctx *c : b->cs) HIP_OK(launch_convert(c->fields(), c->fields32(), all, c->Na, s));
+    for (int i = 0; i < n; i++)
+      if (!skip32[i]) HIP_OK(launch_convert(b->cs[i]->fields(), b->cs[i]->fields32(), all, b->cs[i]->Na, s));
     if ((rc = upload_cases<float>(b, b->C32, mc->N_I32, mc->tol32_uv, mc->tol32_p, mc->tol32_T, mc->stall_w,
-                                  mc->stall_ratio, 0, true)))
+                                  mc->stall_ratio, 0, true, &skip32)))
       return rc;
     const int b32 = lc->batch > 0 ? lc->batch : (b->last_iters32 > 2 ? b->last_iters32 - 1 : 1);
     if ((rc = run_batch_loop2<float>(b, b->C32, lc->N_I_p_c, mc->N_I32, b32, n32))) return rc;
-    b->last_iters32 = *std::min_element(n32.begin(), n32.end());
-    for (sts_ctx *c : b->cs) HIP_OK(launch_convert(c->fields32(), c->fields(), WIDEN_MASK, c->Na, s));
+    int lo = mc->N_I32;
+    for (int i = 0; i < n; i++)
+      if (!skip32[i]) lo = std::min(lo, n32[i]);
+    b->last_iters32 = lo;
+    for (int i = 0; i < n; i++)
+      if (!skip32[i])
+        HIP_OK(launch_convert(b->cs[i]->fields32(), b->cs[i]->fields(), WIDEN_MASK, b->cs[i]->Na, s));
   }
   // the FP64 logs follow the FP32 ones of each case (offset = the largest FP32 count)
   const int off = mc ? mc->N_I32 : 0;
@@ -1252,6 +1273,7 @@ extern "C" int sts_batch_time_step(sts_batch *b, const sts_loop_control *lc, con
     if (iters32) iters32[i] = n32[i];
     if (iters64) iters64[i] = n64[i];
     b->cs[i]->last_iters = n64[i];
+    b->cs[i]->last_total = n32[i] + n64[i];
   }
   HIP_OK(hipStreamSynchronize(s));
   return STS_OK;

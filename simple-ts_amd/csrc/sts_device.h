@@ -169,6 +169,7 @@ struct BatchCase {
   double tol_uv, tol_p, tol_T, stall_ratio;
   const WallOp *wops;  // the case's planned wall writes
   int nwops;
+  int skip;  // the case sits this loop out (its loop control opens closed)
 };
 
 // Work item of a batched launch: tile (x, y) of case c.

@@ -1483,7 +1483,7 @@ template <typename R>
 __global__ void __launch_bounds__(256) kb_loop_open(const BatchCase<R> *C) {
   const BatchCase<R> &c = C[blockIdx.x];
   if (threadIdx.x == 0) {
-    c.ctrl->stop_at = 0x7fffffff;
+    c.ctrl->stop_at = c.skip ? -1 : 0x7fffffff;
     c.ctrl->nonfinite = 0;
   }
   for (int q = threadIdx.x; q < 6 * c.N_I; q += blockDim.x) c.log[q] = 0ull;

@@ -85,3 +85,27 @@ def test_mixed_batch_matches_single_mixed(tmp_path):
     B.close()
     for r in single + batched:
         r.close()
+
+
+@pytest.mark.parametrize("fp32_above", [0, 18])
+def test_predicted_batch_matches_single(fp32_above, tmp_path):
+    """Predicted starts (degree 3) and the FP32 gate in shared launches: case
+    by case the single-case counts and fields, bit for bit, including the
+    steps where some cases skip the FP32 phase and others run it."""
+    pkg = importlib.import_module("simple-ts_amd")
+    single, batched = runs(tmp_path, "s", 32), runs(tmp_path, "b", 32)
+    B = pkg.Batch([r.ctx for r in batched])
+    loop = single[0].loop
+    mixed = (loop[0], (1e-6, 1e-6, 1e-6), 4, 0.7, 3, fp32_above)
+    for step in range(6):
+        ref = [r.ctx.time_step_mixed(*loop, *mixed[:4], extrapolate=mixed[4], fp32_above=mixed[5])[:2]
+               for r in single]
+        n32, n64 = B.time_step(*loop, mixed=mixed)
+        print(f"step {step}: single {ref}, batched {list(zip(n32, n64))}", flush=True)
+        assert list(zip(n32, n64)) == ref
+    for a, b in zip(single, batched):
+        for x, y in zip(state(a), state(b)):
+            assert np.array_equal(x, y)
+    B.close()
+    for r in single + batched:
+        r.close()
