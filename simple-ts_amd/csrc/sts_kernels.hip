@@ -528,14 +528,91 @@ __device__ __forceinline__ void correct_cell(const KGridT<R> &g, const KFieldsT<
   }
 }
 
+// correct_cell in two phases, so that a thread can issue the loads of
+// several cells before it computes any of them.
+template <typename R>
+struct CorrectIn {
+  unsigned ij, i_1j, ij_1;
+  bool in, cu, cv;                    // cell in grid; u, v corrected
+  bool mc, mw, ms;                    // fluid: cell, west, south
+  R uf, vf, ups, vps, d12, d34, pc, pw, ps, Tc, Tw, Ts;
+  __device__ __forceinline__ void load(const KGridT<R> &g, const KFieldsT<R> &fl, int i, int j, int do_correct) {
+    in = i < g.Nx && j < g.Ny;
+    if (!in) return;
+    const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny, g.px);
+    const uint8_t *m = g.m;
+    ij = n.ij;
+    i_1j = n.i_1j;
+    ij_1 = n.ij_1;
+    mc = m[ij];
+    mw = m[i_1j];
+    ms = m[ij_1];
+    cu = do_correct && in_box(i, j, g.Nx_u0, g.Nx_u1, g.Ny_u0, g.Ny_u1) && mw && mc;
+    cv = do_correct && in_box(i, j, g.Nx_v0, g.Nx_v1, g.Ny_v0, g.Ny_v1) && ms && mc;
+    const R *p = FLD(P), *T = FLD(T);
+    uf = FLD(U)[ij];
+    vf = FLD(V)[ij];
+    ups = cu ? FLD(U_PSEUDO)[ij] : R(0);
+    d12 = cu ? FLD(D_P_C_12)[ij] : R(0);
+    vps = cv ? FLD(V_PSEUDO)[ij] : R(0);
+    d34 = cv ? FLD(D_P_C_34)[ij] : R(0);
+    pc = p[ij];
+    pw = p[i_1j];
+    ps = p[ij_1];
+    Tc = T[ij];
+    Tw = T[i_1j];
+    Ts = T[ij_1];
+  }
+  // S:4170-4305 (u, v), S:4323-4330 (sqrt_T, rho), S:10382-10437 (face densities)
+  __device__ __forceinline__ void apply(const KFieldsT<R> &fl, unsigned long long (&r)[2]) const {
+    if (!in) return;
+    R u = uf, v = vf;
+    if (cu) {
+      const R un = ups - (pc - pw) * d12;
+      const unsigned long long e = absbits(un - uf);
+      r[0] = e > r[0] ? e : r[0];
+      u = un;
+      FLD(U)[ij] = un;
+    }
+    if (cv) {  // p[ij - Nx] is p[ij_1] inside the v box (j >= 1)
+      const R vn = vps - (pc - ps) * d34;
+      const unsigned long long e = absbits(vn - vf);
+      r[1] = e > r[1] ? e : r[1];
+      v = vn;
+      FLD(V)[ij] = vn;
+    }
+    const R rij = mc ? pc / Tc : R(0.0);
+    FLD(SQRT_T)[ij] = sqrt(Tc);
+    FLD(RHO)[ij] = rij;
+    const R rw = mw ? pw / Tw : R(0.0), rs = ms ? ps / Ts : R(0.0);
+    R ru, rv;
+    if (mw && mc) ru = upwind_rho(rw, rij, u);
+    else if (mw != mc) ru = mw ? rw : rij;
+    else ru = R(0.0);
+    if (ms && mc) rv = upwind_rho(rs, rij, v);
+    else if (ms != mc) rv = ms ? rs : rij;
+    else rv = R(0.0);
+    FLD(RHO_U)[ij] = ru;
+    FLD(RHO_V)[ij] = rv;
+  }
+};
+
+// CORRECT_ROWS cells per thread, rows BY apart, all loads issued first.
+constexpr int CORRECT_ROWS = 2;
+
 template <typename R>
 __global__ void __launch_bounds__(BX *BY) k_correct(KGridT<R> g, KFieldsT<R> fl, unsigned long long *red_u,
                                                     unsigned long long *red_v, int do_correct, Gate gate) {
   if (gated_out(gate)) return;
   unsigned long long r[2] = {0ull, 0ull};
   int i0, j0;
-  tile_origin<BX, BY>(g.Nx, i0, j0);
-  correct_cell(g, fl, i0 + (int)threadIdx.x, j0 + (int)threadIdx.y, do_correct, r);
+  tile_origin<BX, BY * CORRECT_ROWS>(g.Nx, i0, j0);
+  CorrectIn<R> c[CORRECT_ROWS];
+#pragma unroll
+  for (int q = 0; q < CORRECT_ROWS; q++)
+    c[q].load(g, fl, i0 + (int)threadIdx.x, j0 + (int)threadIdx.y + q * BY, do_correct);
+#pragma unroll
+  for (int q = 0; q < CORRECT_ROWS; q++) c[q].apply(fl, r);
   if (do_correct) block_max_atomic<2>(r, red_u, red_v);
 }
 
@@ -1039,59 +1116,104 @@ __global__ void __launch_bounds__(EDGE_THREADS) k_edges(KGridT<R> g, KFieldsT<R>
 // A10 + A11: fluxes (S:10440-10454) and diffusion coefficients
 // (S:10457-10618) over all cells.
 // ---------------------------------------------------------------------------
+// Inputs of one face/cell (load), then the reference's expressions on them
+// (apply); a thread issues the loads of all its cells before computing any.
+template <typename R>
+struct FluxIn {
+  bool in, m00, m10, m01, m11;  // fluid: (i-1,j-1), (i,j-1), (i-1,j), (i,j)
+  unsigned ij;
+  R hxi, hxw, hyj, hys;  // hx[i], hx[i-1], hy[j], hy[j-1] (clamped or wrapped as the reference's)
+  R ru, u, rv, v;
+  R s00, s10, s01, s11;      // sqrt_T at the four cells
+  R twu10, twu11, twv01, twv11;  // Temper_of_fluid_on_the_wall_u[ij_1], [ij], _v[i_1j], [ij]
+  __device__ __forceinline__ void load(const KGridT<R> &g, const KFieldsT<R> &fl, int i, int j) {
+    in = i < g.Nx && j < g.Ny;
+    if (!in) return;
+    const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny, g.px);
+    const uint8_t *m = g.m;
+    const R *sT = FLD(SQRT_T), *Twu = FLD(TW_U), *Twv = FLD(TW_V);
+    ij = n.ij;
+    const unsigned jm = (unsigned)j - 1u * (0 < j);
+    hxi = g.hx[i];
+    hxw = g.hx[n.i_1];
+    hyj = g.hy[j];
+    hys = g.hy[jm];
+    ru = FLD(RHO_U)[ij];
+    u = FLD(U)[ij];
+    rv = FLD(RHO_V)[ij];
+    v = FLD(V)[ij];
+    m00 = m[n.i_1j_1];
+    m10 = m[n.ij_1];
+    m01 = m[n.i_1j];
+    m11 = m[ij];
+    s00 = sT[n.i_1j_1];
+    s10 = sT[n.ij_1];
+    s01 = sT[n.i_1j];
+    s11 = sT[ij];
+    twu10 = Twu[n.ij_1];
+    twu11 = Twu[ij];
+    twv01 = Twv[n.i_1j];
+    twv11 = Twv[ij];
+  }
+  __device__ __forceinline__ void apply(const KGridT<R> &g, const KFieldsT<R> &fl) const {
+    if (!in) return;
+    const R cT = g.cT, cT1 = g.cT1;
+    FLD(FX)[ij] = hyj * ru * u;
+    FLD(FY)[ij] = hxi * rv * v;
+    // All four cells body: the reference leaves Gamma untouched (S:10477-10543).
+    R Gy = R(0.0), Gx = R(0.0);
+    if (m00 && m10 && m01 && m11) {
+      Gy = R(0.5) * (Li(s00, s01, hys, hyj) + Li(s10, s11, hys, hyj));
+      Gx = R(0.5) * (Li(s00, s10, hxw, hxi) + Li(s01, s11, hxw, hxi));
+      FLD(GAMMA_YF)[ij] = Gy;
+      FLD(GAMMA_XF)[ij] = Gx;
+    } else if (m00 || m10 || m01 || m11) {
+      Gy = R(0.5) * (R(m00 && m01) * Li(s00, s01, hys, hyj) + R(m00 != m01) * sqrt(twv01)
+                     + R(m10 && m11) * Li(s10, s11, hys, hyj) + R(m10 != m11) * sqrt(twv11)
+                     + R((!(m00 && m01)) || (!(m10 && m11))) * Li(sqrt(twu10), sqrt(twu11), hys, hyj));
+      Gx = R(0.5) * (R(m00 && m10) * Li(s00, s10, hxw, hxi) + R(m00 != m10) * sqrt(twu10)
+                     + R(m01 && m11) * Li(s01, s11, hxw, hxi) + R(m01 != m11) * sqrt(twu11)
+                     + R((!(m00 && m10)) || (!(m01 && m11))) * Li(sqrt(twv01), sqrt(twv11), hxw, hxi));
+      FLD(GAMMA_YF)[ij] = Gy;
+      FLD(GAMMA_XF)[ij] = Gx;
+    }
+    FLD(D_UX)[ij] = cT * s01 * (hyj / hxw);
+    FLD(D_UY)[ij] = ((m11 && m01) || (m10 && m00))
+                        ? cT * Gy * ((hxi + hxw) / (R(m11 * m01) * hyj + R(m10 * m00) * hys))
+                        : R(0.0);
+    FLD(D_VX)[ij] = ((m11 && m10) || (m01 && m00))
+                        ? cT * Gx * ((hyj + hys) / (R(m11 * m10) * hxi + R(m01 * m00) * hxw))
+                        : R(0.0);
+    FLD(D_VY)[ij] = cT * s10 * (hxi / hys);
+    FLD(DTX)[ij] = (m11 || m01) ? cT1 * (hyj / (R(0.5) * (R(m11) * hxi + R(m01) * hxw)))
+                                      * (R((!m11) || (!m01)) * sqrt(twu11) + R(m11 * m01) * Hi(s01, s11, hxw, hxi))
+                                : R(0.0);
+    FLD(DTY)[ij] = (m11 || m10) ? cT1 * (hxi / (R(0.5) * (R(m11) * hyj + R(m10) * hys)))
+                                      * (R((!m11) || (!m10)) * sqrt(twv11) + R(m11 * m10) * Hi(s10, s11, hys, hyj))
+                                : R(0.0);
+  }
+};
+
 template <typename R>
 __device__ __forceinline__ void flux_diff_cell(const KGridT<R> &g, const KFieldsT<R> &fl, int i, int j) {
-  if (i >= g.Nx || j >= g.Ny) return;
-  const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny, g.px);
-  const uint8_t *m = g.m;
-  const R *hx = g.hx, *hy = g.hy, *sT = FLD(SQRT_T), *Twu = FLD(TW_U), *Twv = FLD(TW_V);
-  const unsigned ij = n.ij, i_1j = n.i_1j, ij_1 = n.ij_1, i_1j_1 = n.i_1j_1, i_1 = n.i_1;
-  const unsigned jm = (unsigned)j - 1u * (0 < j);
-  const R cT = g.cT, cT1 = g.cT1;
-  FLD(FX)[ij] = hy[j] * FLD(RHO_U)[ij] * FLD(U)[ij];
-  FLD(FY)[ij] = hx[i] * FLD(RHO_V)[ij] * FLD(V)[ij];
-  const bool m00 = m[i_1j_1], m10 = m[ij_1], m01 = m[i_1j], m11 = m[ij];
-  // All four cells body: the reference leaves Gamma untouched (S:10477-10543).
-  R Gy = R(0.0), Gx = R(0.0);
-  if (m00 && m10 && m01 && m11) {
-    Gy = R(0.5) * (Li(sT[i_1j_1], sT[i_1j], hy[jm], hy[j]) + Li(sT[ij_1], sT[ij], hy[jm], hy[j]));
-    Gx = R(0.5) * (Li(sT[i_1j_1], sT[ij_1], hx[i_1], hx[i]) + Li(sT[i_1j], sT[ij], hx[i_1], hx[i]));
-    FLD(GAMMA_YF)[ij] = Gy;
-    FLD(GAMMA_XF)[ij] = Gx;
-  } else if (m00 || m10 || m01 || m11) {
-    Gy = R(0.5) * (R(m00 && m01) * Li(sT[i_1j_1], sT[i_1j], hy[jm], hy[j]) + R(m00 != m01) * sqrt(Twv[i_1j])
-                + R(m10 && m11) * Li(sT[ij_1], sT[ij], hy[jm], hy[j]) + R(m10 != m11) * sqrt(Twv[ij])
-                + R((!(m00 && m01)) || (!(m10 && m11))) * Li(sqrt(Twu[ij_1]), sqrt(Twu[ij]), hy[jm], hy[j]));
-    Gx = R(0.5) * (R(m00 && m10) * Li(sT[i_1j_1], sT[ij_1], hx[i_1], hx[i]) + R(m00 != m10) * sqrt(Twu[ij_1])
-                + R(m01 && m11) * Li(sT[i_1j], sT[ij], hx[i_1], hx[i]) + R(m01 != m11) * sqrt(Twu[ij])
-                + R((!(m00 && m10)) || (!(m01 && m11))) * Li(sqrt(Twv[i_1j]), sqrt(Twv[ij]), hx[i_1], hx[i]));
-    FLD(GAMMA_YF)[ij] = Gy;
-    FLD(GAMMA_XF)[ij] = Gx;
-  }
-  FLD(D_UX)[ij] = cT * sT[i_1j] * (hy[j] / hx[i_1]);
-  FLD(D_UY)[ij] = ((m11 && m01) || (m10 && m00))
-                      ? cT * Gy * ((hx[i] + hx[i_1]) / (R(m11 * m01) * hy[j] + R(m10 * m00) * hy[jm]))
-                      : R(0.0);
-  FLD(D_VX)[ij] = ((m11 && m10) || (m01 && m00))
-                      ? cT * Gx * ((hy[j] + hy[jm]) / (R(m11 * m10) * hx[i] + R(m01 * m00) * hx[i_1]))
-                      : R(0.0);
-  FLD(D_VY)[ij] = cT * sT[ij_1] * (hx[i] / hy[jm]);
-  FLD(DTX)[ij] = (m11 || m01) ? cT1 * (hy[j] / (R(0.5) * (R(m11) * hx[i] + R(m01) * hx[i_1])))
-                                    * (R((!m11) || (!m01)) * sqrt(Twu[ij])
-                                       + R(m11 * m01) * Hi(sT[i_1j], sT[ij], hx[i_1], hx[i]))
-                              : R(0.0);
-  FLD(DTY)[ij] = (m11 || m10) ? cT1 * (hx[i] / (R(0.5) * (R(m11) * hy[j] + R(m10) * hy[jm])))
-                                    * (R((!m11) || (!m10)) * sqrt(Twv[ij])
-                                       + R(m11 * m10) * Hi(sT[ij_1], sT[ij], hy[jm], hy[j]))
-                              : R(0.0);
+  FluxIn<R> c;
+  c.load(g, fl, i, j);
+  c.apply(g, fl);
 }
+
+// FLUX_ROWS cells per thread, rows BY apart, all loads issued first.
+constexpr int FLUX_ROWS = 2;
 
 template <typename R>
 __global__ void __launch_bounds__(BX *BY) k_flux_diff(KGridT<R> g, KFieldsT<R> fl, Gate gate) {
   if (gated_out(gate)) return;
   int i0, j0;
-  tile_origin<BX, BY>(g.Nx, i0, j0);
-  flux_diff_cell(g, fl, i0 + (int)threadIdx.x, j0 + (int)threadIdx.y);
+  tile_origin<BX, BY * FLUX_ROWS>(g.Nx, i0, j0);
+  FluxIn<R> c[FLUX_ROWS];
+#pragma unroll
+  for (int q = 0; q < FLUX_ROWS; q++) c[q].load(g, fl, i0 + (int)threadIdx.x, j0 + (int)threadIdx.y + q * BY);
+#pragma unroll
+  for (int q = 0; q < FLUX_ROWS; q++) c[q].apply(g, fl);
 }
 
 // ---------------------------------------------------------------------------
@@ -1529,7 +1651,8 @@ hipError_t launch_sweep2(const KGridT<R> &g, const KFieldsT<R> &f, const R *Po, 
 template <typename R>
 hipError_t launch_correct(const KGridT<R> &g, const KFieldsT<R> &f, unsigned long long *ru, unsigned long long *rv,
                           int do_correct, hipStream_t s, Gate gate) {
-  hipLaunchKernelGGL(k_correct<R>, (grid_tiles<BX, BY>(g)), dim3(BX, BY), 0, s, g, f, ru, rv, do_correct, gate);
+  hipLaunchKernelGGL(k_correct<R>, (grid_tiles<BX, BY * CORRECT_ROWS>(g)), dim3(BX, BY), 0, s, g, f, ru, rv, do_correct,
+                     gate);
   return hipGetLastError();
 }
 template <typename R>
@@ -1552,7 +1675,7 @@ hipError_t launch_edges(const KGridT<R> &g, const KFieldsT<R> &f, int parts, dou
 }
 template <typename R>
 hipError_t launch_flux_diff(const KGridT<R> &g, const KFieldsT<R> &f, hipStream_t s, Gate gate) {
-  hipLaunchKernelGGL(k_flux_diff<R>, (grid_tiles<BX, BY>(g)), dim3(BX, BY), 0, s, g, f, gate);
+  hipLaunchKernelGGL(k_flux_diff<R>, (grid_tiles<BX, BY * FLUX_ROWS>(g)), dim3(BX, BY), 0, s, g, f, gate);
   return hipGetLastError();
 }
 
