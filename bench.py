@@ -15,23 +15,21 @@ Every path runs the same W + K steps; the first W are not timed, and every
 step ends on the reference's loop-2 test (1e-8):
   fp64             the plain FP64 path (sts_gpu, the C++ mirror of the
                    reference's main()), bit-identical to the reference;
-  predicted        FP64 loop 2 started from the degree-4 polynomial in time
-                   through the last five converged steps (every cell, the
-                   derived fields rebuilt), sts_time_step_mixed with no
-                   FP32 phase;
-  warm_mixed       the headline path: the same predicted start, an FP32
-                   loop-2 phase on the steps whose predecessor needed more
-                   than 25 outer iterations, FP64 outer iterations ending on
-                   the reference's own test;
+  predicted        FP64 loop 2 started from the degree-2 polynomial in time
+                   through the last three converged steps (every cell, the
+                   derived fields rebuilt), guarded: an attempt that has not
+                   met the test within the recent iteration count of steps
+                   from the previous state is restarted from that state
+                   (sts_time_step_mixed, extrapolate = 2, guard = 64);
+  warm_mixed       the headline path: the same guarded predicted start, plus
+                   an FP32 loop-2 phase (at most 200 outer iterations) on
+                   steps whose predecessor needed more than 80;
   mixed            FP32 phase then FP64 phase from the previous step's state
                    (no predicted start);
   surrogate        the mixed path warm-started by the trained BF16 U-Net with
                    its residual-based fallback (sts_time_step_ws), when
                    trained parameters are present (simple-ts_amd/unet_b32.npz,
-                   written by train_unet.py);
-  surrogate_on_predicted  warm_mixed with the U-Net prediction tried on top
-                   of the predicted start (kept only if it lowers the first
-                   sweep's residual).
+                   written by train_unet.py).
 value = Nx*Ny*K / timed seconds / 1e6 (Mcell.steps/s) of warm_mixed; every
 path's figure, its speed-up over fp64 and every field's relative L2 error
 against the fp64 path after the W + K steps are reported beside it.
@@ -73,14 +71,15 @@ HBM_PEAK_GBS = 8000.0
 SWEEP_BYTES_PER_CELL = 144  # FP64, per sweep: Fx Fy DTx DTy T_old p_old u v ScT1 rho apx apy bpx bpy p_pr T_pr + T p
 WEIGHTS = os.path.join(ROOT, "simple-ts_amd", "unet_b32.npz")
 MIXED = (5000, (1e-7, 1e-7, 1e-7), 6, 0.95)  # N_I32, FP32 tolerances, stagnation window and ratio
-DEGREE = 4        # predicted start: polynomial degree in time
-FP32_ABOVE = 25   # warm_mixed: FP32 phase after a step that needed more outer iterations than this
-PATHS = {  # name: (FP32 phase, extrapolation degree, FP32 gate, network)
-    "predicted": (False, DEGREE, 0, False),
-    "warm_mixed": (True, DEGREE, FP32_ABOVE, False),
-    "mixed": (True, 0, 0, False),
-    "surrogate": (True, 0, 0, True),
-    "surrogate_on_predicted": (True, DEGREE, FP32_ABOVE, True),
+DEGREE = 2        # predicted start: polynomial degree in time
+GUARD = 64        # predicted start: restart from x^n past the budget, pauses up to this many steps
+FP32_ABOVE = 80   # warm_mixed: FP32 phase only after a step that needed more outer iterations than this
+N_I32_WARM = 200  # warm_mixed: cap on the FP32 phase
+PATHS = {  # name: (FP32 cap, extrapolation degree, FP32 gate, guard, network)
+    "predicted": (0, DEGREE, 0, GUARD, False),
+    "warm_mixed": (N_I32_WARM, DEGREE, FP32_ABOVE, GUARD, False),
+    "mixed": (MIXED[0], 0, 0, 0, False),
+    "surrogate": (MIXED[0], 0, 0, 0, True),
 }
 HEADLINE = "warm_mixed"
 
@@ -115,8 +114,7 @@ def run_driver(steps, warmup, kernel="sweep"):
 
 def run_python_path(steps, warmup, name, net=None):
     """One of PATHS through the Python binding of the C ABI."""
-    fp32, degree, above, _ = PATHS[name]
-    n_i32 = MIXED[0] if fp32 else 0
+    n_i32, degree, above, guard, _ = PATHS[name]
     ws = importlib.import_module("simple-ts_amd.warmstart")
     work = tempfile.mkdtemp(prefix="sts_bench_")
     try:
@@ -129,17 +127,18 @@ def run_python_path(steps, warmup, name, net=None):
                 t0 = time.perf_counter()
             if net is None:
                 a, b, _, _ = run.ctx.time_step_mixed(*run.loop, n_i32, *MIXED[1:], extrapolate=degree,
-                                                     fp32_above=above)
+                                                     fp32_above=above, guard=guard)
             else:
-                u, a, b = run.step_ws(net, mixed=(n_i32, *MIXED[1:], degree, above))
+                u, a, b = run.step_ws(net, mixed=(n_i32, *MIXED[1:], degree, above, guard))
                 taken += u if s >= warmup else 0
             if s >= warmup:
                 n32, n64 = n32 + a, n64 + b
         dt = time.perf_counter() - t0
+        stats = run.ctx.predict_stats()
         run.ctx.jump_overwrite()  # as the program does before it writes the fields
         f = {k: run.ctx.get(k) for k in ("u", "v", "p", "Temper")}
         run.close()
-        return dict(seconds=dt, fp32_it=n32, fp64_it=n64, taken=taken), f
+        return dict(seconds=dt, fp32_it=n32, fp64_it=n64, taken=taken, stats=stats), f
     finally:
         shutil.rmtree(work, ignore_errors=True)
 
@@ -262,7 +261,8 @@ def run_batch_workload(n, steps, warmup, mc):
                 def step():
                     for r in runs:
                         if mixed:
-                            r.ctx.time_step_mixed(*r.loop, *mc[:4], extrapolate=mc[4], fp32_above=mc[5])
+                            r.ctx.time_step_mixed(*r.loop, *mc[:4], extrapolate=mc[4], fp32_above=mc[5],
+                                                  guard=mc[6])
                         else:
                             r.ctx.time_step(*r.loop)
             t0 = None
@@ -310,7 +310,7 @@ def main():
     if args.workload == "batch":
         out = {"fp64": run_batch_workload(args.cases, args.steps, args.warmup, None),
                "warm_mixed": run_batch_workload(args.cases, args.steps, args.warmup,
-                                                (*MIXED, DEGREE, FP32_ABOVE))}
+                                                (N_I32_WARM, *MIXED[1:], DEGREE, FP32_ABOVE, GUARD))}
         r = out["warm_mixed"]
         print(json.dumps({
             "metric": "Mcell·steps/s to 1e-8 residual, batched uneven cases", "value": r["batched"],
@@ -332,7 +332,7 @@ def main():
     if not args.fp64_only:
         for name in PATHS:
             net = info = None
-            if PATHS[name][3]:
+            if PATHS[name][4]:
                 net, info = load_net()
                 if net is None:
                     continue
@@ -343,6 +343,11 @@ def main():
                            "fp64_outer_iterations_per_step": r["fp64_it"] / args.steps,
                            "speedup_vs_fp64": v / fp64_value,
                            "field_rel_l2_vs_fp64": {k: rel_l2(f[k], f64[k]) for k in f}}
+            if PATHS[name][3]:
+                s = r["stats"]
+                paths[name]["guard"] = {"predicted_steps": s[0], "restarted": s[1],
+                                        "abandoned_fp32_iterations": s[2], "abandoned_fp64_iterations": s[3],
+                                        "steps_from_previous_state": s[4]}
             if net is not None:
                 paths[name]["prediction_taken_steps"] = r["taken"]
                 paths[name]["network"] = info
@@ -362,8 +367,10 @@ def main():
     cpu = None if args.no_cpu else run_cpu_baseline()
     path_desc = {"fp64": "plain FP64",
                  "warm_mixed": f"start predicted by degree-{DEGREE} extrapolation in time (every cell, derived "
-                               f"fields rebuilt) + FP32 loop-2 phase after steps of more than {FP32_ABOVE} outer "
-                               f"iterations + FP64 phase on the reference's test"}
+                               f"fields rebuilt), guarded (restart from the previous state when loop 2 exceeds "
+                               f"the recent iteration count of such starts) + FP32 loop-2 phase (at most "
+                               f"{N_I32_WARM}) after steps of more than {FP32_ABOVE} outer iterations + FP64 "
+                               f"phase on the reference's test"}
     line = {
         "metric": METRIC, "value": best, "unit": "Mcell·steps/s", "n_gpus": 1, "steps": t["timed_steps"],
         "warmup": t["warmup_steps"], "ms_per_step": paths[best_name]["ms_per_step"], "higher_is_better": True,

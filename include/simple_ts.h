@@ -213,9 +213,22 @@ typedef struct sts_mixed_control {
   int fp32_above;                     /* > 0: run the FP32 phase only when the context's previous step
                                          took more than this many outer iterations (a well-started step
                                          goes straight to FP64); 0: always */
+  int guard;                          /* > 0 (with extrapolate): a predicted step whose FP64 phase has not
+                                         met the reference's test within as many outer iterations as
+                                         steps from x^n recently took (their running mean) is restarted
+                                         from x^n, FP64 only, and the degree in use drops by one (it
+                                         climbs back after 4 accepted predicted steps); an abandoned
+                                         degree-1 attempt pauses prediction for 1, 2, 4, ... steps (at
+                                         most guard).  0: every step predicted at the full degree */
 } sts_mixed_control;
 int sts_time_step_mixed(sts_ctx *ctx, const sts_loop_control *lc, const sts_mixed_control *mc, int *iters32,
                         int *iters64, double *res_log, double *p_probe);
+/* Counters of the guarded predicted start since the context was created:
+   out[0] predicted steps, out[1] of them restarted from x^n (batched steps:
+   that went over the budget), out[2] FP32 and out[3] FP64 outer iterations
+   spent in the abandoned attempts (not included in the counts
+   sts_time_step_mixed returns), out[4] steps run from x^n by the guard. */
+int sts_predict_stats(sts_ctx *ctx, long long out[5]);
 
 /* Batched time steps (no counterpart in the reference).  The cases of n
    contexts (any grid sizes) advance one time step together: every kernel of

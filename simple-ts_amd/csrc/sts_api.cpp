@@ -68,6 +68,12 @@ struct sts_ctx {
   float *p_alt32 = nullptr, *T_alt32 = nullptr;
   int last_iters32 = 0, last_iters64m = 0;
   int last_total = -1;  // outer iterations of the previous mixed or warm-started step (-1: none)
+  // guarded predicted start: running mean of the FP64 iterations of steps
+  // from x^n (-1: none yet), pause left and next pause length, counters
+  double plain_mean = -1.0;
+  int pred_pause = 0, pred_backoff = 1;
+  int pred_degree = -1, pred_wins = 0;  // degree in use (-1: the requested one), successes since a change
+  long long pstat[5] = {0, 0, 0, 0, 0};
   long long fluid_cells = 0;
   WallOp *wall_ops = nullptr;  // the wall pass's writes, planned once from the mask (k_wall_ops)
   int n_wall_ops = 0;
@@ -443,6 +449,11 @@ extern "C" int sts_set_field(sts_ctx *c, int field, const double *host) {
   if (!c || !host || field < 0 || field >= STS_NFIELDS) return fail(STS_ERR_ARG, "bad field or pointer");
   c->n_hist = 0;  // the state no longer continues the previous steps
   c->last_total = -1;
+  c->plain_mean = -1.0;
+  c->pred_pause = 0;
+  c->pred_backoff = 1;
+  c->pred_degree = -1;
+  c->pred_wins = 0;
   HIP_OK(hipMemcpyAsync(c->f[field], host, sizeof(double) * (size_t)c->Na, hipMemcpyHostToDevice, c->stream));
   HIP_OK(hipStreamSynchronize(c->stream));
   return STS_OK;
@@ -851,7 +862,8 @@ static constexpr unsigned long long WIDEN_MASK =
 // (capped by the stored steps) through x^n and the previous converged steps
 // for u, v, p, T, and every field derived from them is rebuilt the way the
 // end of an outer iteration builds it.  The history then advances a step.
-static int extrapolate_start(sts_ctx *c, int order, hipStream_t s = nullptr) {
+static int extrapolate_start(sts_ctx *c, int order, hipStream_t s = nullptr, bool store_only = false,
+                             int degree_cap = MAX_EXTRAP) {
   if (!s) s = c->stream;
   if (order <= 0) {
     c->n_hist = 0;
@@ -862,10 +874,18 @@ static int extrapolate_start(sts_ctx *c, int order, hipStream_t s = nullptr) {
   if (!c->hist.f[0][0])
     for (auto &lv : c->hist.f)
       for (double *&q : lv) HIP_OK(hipMalloc(&q, bytes));
-  const int keep = c->n_hist, use = std::min(order, keep);
+  // `order` past steps are kept: x^n goes to a free level, or over the oldest
+  // one, and that level becomes level 0 (the rest move down by renaming)
+  const int keep = std::min(c->n_hist, order), use = store_only ? 0 : std::min(keep, degree_cap);
+  const int slot = keep < order ? keep : order - 1;
   const KFields kf = c->fields();
-  HIP_OK(launch_extrapolate(c->g, kf, c->hist, use, keep, s));
-  c->n_hist = std::min(keep + 1, MAX_EXTRAP);
+  HIP_OK(launch_extrapolate(c->g, kf, c->hist, use, slot, s));
+  double *top[4];
+  for (int q = 0; q < 4; q++) top[q] = c->hist.f[slot][q];
+  for (int h = slot; h > 0; h--)
+    for (int q = 0; q < 4; q++) c->hist.f[h][q] = c->hist.f[h - 1][q];
+  for (int q = 0; q < 4; q++) c->hist.f[0][q] = top[q];
+  c->n_hist = std::min(keep + 1, order);
   if (use == 0) return STS_OK;
   HIP_OK(launch_correct(c->g, kf, nullptr, nullptr, 0, s));
   HIP_OK(launch_wall(c->g, kf, STS_BC_SLIP_U | STS_BC_SLIP_V | STS_BC_JUMP, s, Gate{nullptr, 0}, c->wall_ops,
@@ -919,6 +939,34 @@ static int check_mixed(const sts_ctx *c, const sts_loop_control *lc, const sts_m
   return STS_OK;
 }
 
+// Loop 2 from x^n again after an abandoned predicted attempt: u, v, p, T
+// back to the step's start values (the *_PR fields) and the derived fields
+// rebuilt from them, as the end of the previous step left them.
+static int restart_from_previous(sts_ctx *c) {
+  const size_t bytes = sizeof(double) * (size_t)c->Na;
+  const int src[4] = {STS_U_PR, STS_V_PR, STS_P_PR, STS_T_PR}, dst[4] = {STS_U, STS_V, STS_P, STS_T};
+  for (int q = 0; q < 4; q++)
+    HIP_OK(hipMemcpyAsync(c->f[dst[q]], c->f[src[q]], bytes, hipMemcpyDeviceToDevice, c->stream));
+  const KFields kf = c->fields();
+  HIP_OK(launch_correct(c->g, kf, nullptr, nullptr, 0, c->stream));
+  HIP_OK(launch_wall(c->g, kf, STS_BC_SLIP_U | STS_BC_SLIP_V | STS_BC_JUMP, c->stream, Gate{nullptr, 0},
+                     c->wall_ops, c->n_wall_ops));
+  HIP_OK(launch_edges(c->g, kf, STS_BC_VELOCITIES | STS_BC_TEMPERATURE | STS_BC_PRESSURE, nullptr, c->stream));
+  HIP_OK(launch_flux_diff(c->g, kf, c->stream));
+  return STS_OK;
+}
+
+// Whether the last FP64 log entry of a loop 2 of n outer iterations (log
+// offset off) meets the reference's test (S:4433-4437).
+static int log_converged(sts_ctx *c, size_t off, int n, const sts_loop_control *lc, bool *ok) {
+  unsigned long long w[4];
+  HIP_OK(hipMemcpyAsync(w, c->log + off + 6 * (size_t)(n - 1), sizeof w, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  const double r[4] = {bits_to_double(w[0]), bits_to_double(w[1]), bits_to_double(w[2]), bits_to_double(w[3])};
+  *ok = !(lc->tol_uv < r[0] || lc->tol_uv < r[1] || lc->tol_p < r[2] || lc->tol_T < r[3]);
+  return STS_OK;
+}
+
 extern "C" int sts_time_step_mixed(sts_ctx *c, const sts_loop_control *lc, const sts_mixed_control *mc,
                                    int *iters32, int *iters64, double *res_log, double *p_probe) {
   if (!c || !mc) return fail(STS_ERR_ARG, "null argument");
@@ -926,13 +974,66 @@ extern "C" int sts_time_step_mixed(sts_ctx *c, const sts_loop_control *lc, const
   if ((rc = check_mixed(c, lc, mc))) return rc;
   if ((rc = ensure_loop_buffers(c, lc->N_I + mc->N_I32))) return rc;
   if ((rc = sts_save_previous(c))) return rc;
-  if ((rc = extrapolate_start(c, mc->extrapolate))) return rc;
   int n32 = 0, n64 = 0;
-  rc = run_phases(c, lc, mc, &n32, &n64);
+  if (mc->guard <= 0 || mc->extrapolate <= 0) {
+    if ((rc = extrapolate_start(c, mc->extrapolate))) return rc;
+    rc = run_phases(c, lc, mc, &n32, &n64);
+  } else {
+    // guarded predicted start: the degree drops by one after an abandoned
+    // attempt and climbs back after 4 accepted ones; an abandoned degree-1
+    // attempt pauses prediction
+    if (c->pred_degree < 0 || c->pred_degree > mc->extrapolate) c->pred_degree = mc->extrapolate;
+    const bool predict = c->n_hist > 0 && c->plain_mean > 0.0 && c->pred_pause == 0;
+    if ((rc = extrapolate_start(c, mc->extrapolate, nullptr, !predict, c->pred_degree))) return rc;
+    bool plain = !predict;
+    if (predict) {
+      c->pstat[0]++;
+      sts_loop_control lcp = *lc;
+      lcp.N_I = std::max(1, std::min(lc->N_I, (int)std::ceil(c->plain_mean)));
+      rc = run_phases(c, &lcp, mc, &n32, &n64);
+      bool ok = true;
+      if (!rc && n64 >= lcp.N_I && lcp.N_I < lc->N_I) rc = log_converged(c, 6 * (size_t)n32, n64, lc, &ok);
+      if (rc && rc != STS_ERR_NONFINITE) return rc;
+      if (rc == STS_ERR_NONFINITE || !ok) {  // abandon the attempt
+        c->pstat[1]++;
+        c->pstat[2] += n32;
+        c->pstat[3] += n64;
+        c->pred_wins = 0;
+        if (c->pred_degree > 1) {
+          c->pred_degree--;
+        } else {
+          c->pred_pause = c->pred_backoff;
+          c->pred_backoff = std::min(2 * c->pred_backoff, mc->guard);
+        }
+        if ((rc = restart_from_previous(c))) return rc;
+        plain = true;
+      } else {
+        c->pred_backoff = 1;
+        if (++c->pred_wins >= 4 && c->pred_degree < mc->extrapolate) {
+          c->pred_degree++;
+          c->pred_wins = 0;
+        }
+      }
+    } else if (c->pred_pause > 0) {
+      c->pred_pause--;
+    }
+    if (plain) {
+      c->pstat[4] += !predict;
+      n32 = 0;
+      rc = run_phases(c, lc, nullptr, &n32, &n64);
+      if (!rc) c->plain_mean = c->plain_mean < 0.0 ? n64 : 0.5 * (c->plain_mean + n64);
+    }
+  }
   if (iters32) *iters32 = n32;
   if (iters64) *iters64 = n64;
   int rc2 = finish_step(c, n32 + n64, res_log, p_probe);
   return rc ? rc : rc2;
+}
+
+extern "C" int sts_predict_stats(sts_ctx *c, long long out[5]) {
+  if (!c || !out) return fail(STS_ERR_ARG, "null argument");
+  for (int q = 0; q < 5; q++) out[q] = c->pstat[q];
+  return STS_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -1235,7 +1336,21 @@ extern "C" int sts_batch_time_step(sts_batch *b, const sts_loop_control *lc, con
     const double *src[5] = {c->f[STS_P], c->f[STS_U], c->f[STS_V], c->f[STS_T], c->f[STS_RHO]};
     double *dst[5] = {c->f[STS_P_PR], c->f[STS_U_PR], c->f[STS_V_PR], c->f[STS_T_PR], c->f[STS_RHO_PR]};
     HIP_OK(launch_copy5(src, dst, c->Na, s));
-    if ((rc = extrapolate_start(c, mc ? mc->extrapolate : 0, s))) return rc;
+  }
+  // predicted starts; with a guard, per case: a predicted step that needed
+  // more outer iterations than the case's recent steps from x^n pauses its
+  // prediction for 1, 2, 4, ... steps (no restart inside shared launches)
+  std::vector<char> predicted(n, 0);
+  for (int i = 0; i < n; i++) {
+    sts_ctx *c = b->cs[i];
+    const int q = mc ? mc->extrapolate : 0;
+    bool pred = q > 0;
+    if (pred && mc->guard > 0) {
+      pred = c->n_hist > 0 && c->plain_mean > 0.0 && c->pred_pause == 0;
+      if (!pred && c->pred_pause > 0) c->pred_pause--;
+    }
+    predicted[i] = pred && c->n_hist > 0;
+    if ((rc = extrapolate_start(c, q, s, !pred))) return rc;
   }
   std::vector<int> n32(n, 0), n64(n, 0);
   // cases the FP32 gate sends straight to FP64 sit the FP32 phase out
@@ -1272,8 +1387,24 @@ extern "C" int sts_batch_time_step(sts_batch *b, const sts_loop_control *lc, con
   for (int i = 0; i < n; i++) {
     if (iters32) iters32[i] = n32[i];
     if (iters64) iters64[i] = n64[i];
-    b->cs[i]->last_iters = n64[i];
-    b->cs[i]->last_total = n32[i] + n64[i];
+    sts_ctx *c = b->cs[i];
+    c->last_iters = n64[i];
+    c->last_total = n32[i] + n64[i];
+    if (mc && mc->extrapolate > 0 && mc->guard > 0) {
+      if (!predicted[i]) {
+        c->plain_mean = c->plain_mean < 0.0 ? n64[i] : 0.5 * (c->plain_mean + n64[i]);
+        c->pstat[4]++;
+      } else {
+        c->pstat[0]++;
+        if (n64[i] > c->plain_mean) {
+          c->pstat[1]++;
+          c->pred_pause = c->pred_backoff;
+          c->pred_backoff = std::min(2 * c->pred_backoff, mc->guard);
+        } else {
+          c->pred_backoff = 1;
+        }
+      }
+    }
   }
   HIP_OK(hipStreamSynchronize(s));
   return STS_OK;

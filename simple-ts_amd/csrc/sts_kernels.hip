@@ -1300,7 +1300,9 @@ __device__ __forceinline__ double extrap(int order, const double (&x)[MAX_EXTRAP
   return s;
 }
 
-__global__ void __launch_bounds__(BX *BY) k_extrapolate(KGrid g, KFields fl, ExtrapHist hist, int order, int keep) {
+// Reads x^n and hist[0 .. order-1]; stores x^n into hist[slot] (the host then
+// rotates the level pointers, so no level is copied).
+__global__ void __launch_bounds__(BX *BY) k_extrapolate(KGrid g, KFields fl, ExtrapHist hist, int order, int slot) {
   const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
   if (i >= g.Nx || j >= g.Ny) return;
   const unsigned k = (unsigned)i + (unsigned)j * (unsigned)g.Nx;
@@ -1310,7 +1312,7 @@ __global__ void __launch_bounds__(BX *BY) k_extrapolate(KGrid g, KFields fl, Ext
     double x[MAX_EXTRAP + 1];
     x[0] = fl.f[src[q]][k];
 #pragma unroll
-    for (int h = 0; h < MAX_EXTRAP; h++) x[h + 1] = (h < keep) ? hist.f[h][q][k] : 0.0;
+    for (int h = 0; h < MAX_EXTRAP; h++) x[h + 1] = (h < order) ? hist.f[h][q][k] : 0.0;
     if (order > 0) {
       // a guess that is not finite, or a pressure or temperature that is not
       // positive, keeps the previous step's value in that cell
@@ -1318,8 +1320,7 @@ __global__ void __launch_bounds__(BX *BY) k_extrapolate(KGrid g, KFields fl, Ext
       const bool ok = isfinite(e) && (q < 2 || e > 0.0 || x[0] <= 0.0);
       fl.f[dst[q]][k] = ok ? e : x[0];
     }
-    for (int h = (keep < MAX_EXTRAP ? keep : MAX_EXTRAP - 1); h > 0; h--) hist.f[h][q][k] = x[h];
-    hist.f[0][q][k] = x[0];
+    hist.f[slot][q][k] = x[0];
   }
 }
 
@@ -1721,9 +1722,9 @@ hipError_t launch_copy5(const double *const *src, double *const *dst, int n, hip
                      dst[3], dst[4], n);
   return hipGetLastError();
 }
-hipError_t launch_extrapolate(const KGrid &g, const KFields &f, const ExtrapHist &hist, int order, int keep,
+hipError_t launch_extrapolate(const KGrid &g, const KFields &f, const ExtrapHist &hist, int order, int slot,
                               hipStream_t s) {
-  hipLaunchKernelGGL(k_extrapolate, grid2d(g), dim3(BX, BY), 0, s, g, f, hist, order, keep);
+  hipLaunchKernelGGL(k_extrapolate, grid2d(g), dim3(BX, BY), 0, s, g, f, hist, order, slot);
   return hipGetLastError();
 }
 hipError_t launch_set_inflow(const KGrid &g, const KFields &f, double u_in, hipStream_t s) {

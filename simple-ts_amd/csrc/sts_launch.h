@@ -40,8 +40,8 @@ hipError_t launch_jump_overwrite(const KGrid &g, const KFields &f, hipStream_t s
 hipError_t launch_copy5(const double *const *src, double *const *dst, int n, hipStream_t s);
 hipError_t launch_set_inflow(const KGrid &g, const KFields &f, double u_in, hipStream_t s);
 // u, v, p, T extrapolated from x^n (the *_PR fields) and hist[h] = x^(n-1-h)
-// (h < order), then the history shifts by one step; order 0 only stores.
-hipError_t launch_extrapolate(const KGrid &g, const KFields &f, const ExtrapHist &hist, int order, int keep,
+// (h < order); x^n is stored into hist[slot]; order 0 only stores.
+hipError_t launch_extrapolate(const KGrid &g, const KFields &f, const ExtrapHist &hist, int order, int slot,
                               hipStream_t s);
 hipError_t launch_loop_open(LoopCtrl *ctrl, unsigned long long *log, int n, unsigned long long *lred,
                             hipStream_t s);

@@ -109,3 +109,26 @@ def test_predicted_batch_matches_single(fp32_above, tmp_path):
     B.close()
     for r in single + batched:
         r.close()
+
+
+def test_guarded_batch_stays_on_the_plain_fields(tmp_path):
+    """Batched predicted starts with a guard (pauses per case, no restart):
+    the fields stay within 1e-6 relative L2 of the same cases stepped from
+    x^n, and every step is counted as predicted or from x^n."""
+    pkg = importlib.import_module("simple-ts_amd")
+    plain, batched = runs(tmp_path, "p", 32), runs(tmp_path, "b", 32)
+    B = pkg.Batch([r.ctx for r in batched])
+    loop = plain[0].loop
+    steps = 8
+    for _ in range(steps):
+        for r in plain:
+            r.ctx.time_step(*loop)
+        B.time_step(*loop, mixed=(200, (1e-7,) * 3, 6, 0.95, 2, 80, 16))
+    for a, b in zip(plain, batched):
+        st = b.ctx.predict_stats()
+        assert st[0] + st[4] == steps, st
+        for x, y in zip(state(a), state(b)):
+            assert float(np.linalg.norm(x - y) / np.linalg.norm(x)) <= 1e-6
+    B.close()
+    for r in plain + batched:
+        r.close()

@@ -137,3 +137,33 @@ def test_fp32_gate_skips_the_fp32_phase_after_short_steps(tmp_path):
     print("gate 1:", n32, flush=True)
     assert all(n > 0 for n in n32)
     r.close()
+
+
+def test_guarded_predicted_start(tmp_path):
+    """guard: every step either runs predicted (and is kept only if it met the
+    test within the budget, else restarted from x^n) or from x^n; the fields
+    stay within REL_L2_MAX of the plain path and the counters add up."""
+    cases = importlib.import_module("simple-ts_amd.cases")
+    ws = importlib.import_module("simple-ts_amd.warmstart")
+    steps = 30
+    runs = []
+    for tag in ("plain", "guard"):
+        d = cases.confined_square(str(tmp_path / tag), 240, 48, 1415, 0.00283, steps, dt_over_tin=0.01)
+        runs.append(ws.Run(d, 32))
+    plain, guard = runs
+    n_plain = n_guard = 0
+    for _ in range(steps):
+        n_plain += plain.ctx.time_step_mixed(*plain.loop, 0, (1e-7,) * 3)[1]
+        a, b, _, _ = guard.ctx.time_step_mixed(*guard.loop, 200, (1e-7,) * 3, 6, 0.95, extrapolate=2,
+                                               fp32_above=80, guard=16)
+        n_guard += b
+    st = guard.ctx.predict_stats()
+    print(f"plain {n_plain} FP64 iterations, guarded {n_guard} (+{st[3]} abandoned); stats {st}", flush=True)
+    assert st[0] + st[4] == steps and st[1] <= st[0]
+    assert plain.ctx.predict_stats() == (0, 0, 0, 0, 0)
+    for f in ("u", "v", "p", "Temper"):
+        e = rel_l2(guard.ctx.get(f), plain.ctx.get(f))
+        print(f"{f}: rel L2 {e:.3e}", flush=True)
+        assert e <= REL_L2_MAX, f
+    for r in runs:
+        r.close()
