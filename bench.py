@@ -75,11 +75,15 @@ DEGREE = 2        # predicted start: polynomial degree in time
 GUARD = 64        # predicted start: restart from x^n past the budget, pauses up to this many steps
 FP32_ABOVE = 80   # warm_mixed: FP32 phase only after a step that needed more outer iterations than this
 N_I32_WARM = 200  # warm_mixed: cap on the FP32 phase
-PATHS = {  # name: (FP32 cap, extrapolation degree, FP32 gate, guard, network)
-    "predicted": (0, DEGREE, 0, GUARD, False),
-    "warm_mixed": (N_I32_WARM, DEGREE, FP32_ABOVE, GUARD, False),
-    "mixed": (MIXED[0], 0, 0, 0, False),
-    "surrogate": (MIXED[0], 0, 0, 0, True),
+AA_DEPTH = 3      # accelerated: Anderson history depth
+AA_EVERY = 1      # accelerated: outer iterations per Anderson step
+PATHS = {  # name: (FP32 cap, extrapolation degree, FP32 gate, guard, network, Anderson depth, period)
+    "predicted": (0, DEGREE, 0, GUARD, False, 0, 1),
+    "warm_mixed": (N_I32_WARM, DEGREE, FP32_ABOVE, GUARD, False, 0, 1),
+    "anderson": (0, 0, 0, 0, False, AA_DEPTH, AA_EVERY),
+    "accelerated": (0, DEGREE, 0, GUARD, False, AA_DEPTH, AA_EVERY),
+    "mixed": (MIXED[0], 0, 0, 0, False, 0, 1),
+    "surrogate": (MIXED[0], 0, 0, 0, True, 0, 1),
 }
 HEADLINE = "warm_mixed"
 
@@ -114,7 +118,7 @@ def run_driver(steps, warmup, kernel="sweep"):
 
 def run_python_path(steps, warmup, name, net=None):
     """One of PATHS through the Python binding of the C ABI."""
-    n_i32, degree, above, guard, _ = PATHS[name]
+    n_i32, degree, above, guard, _, aa, every = PATHS[name]
     ws = importlib.import_module("simple-ts_amd.warmstart")
     work = tempfile.mkdtemp(prefix="sts_bench_")
     try:
@@ -127,9 +131,10 @@ def run_python_path(steps, warmup, name, net=None):
                 t0 = time.perf_counter()
             if net is None:
                 a, b, _, _ = run.ctx.time_step_mixed(*run.loop, n_i32, *MIXED[1:], extrapolate=degree,
-                                                     fp32_above=above, guard=guard)
+                                                     fp32_above=above, guard=guard, anderson=aa,
+                                                     anderson_every=every)
             else:
-                u, a, b = run.step_ws(net, mixed=(n_i32, *MIXED[1:], degree, above, guard))
+                u, a, b = run.step_ws(net, mixed=(n_i32, *MIXED[1:], degree, above, guard, aa, every))
                 taken += u if s >= warmup else 0
             if s >= warmup:
                 n32, n64 = n32 + a, n64 + b

@@ -220,6 +220,14 @@ typedef struct sts_mixed_control {
                                          climbs back after 4 accepted predicted steps); an abandoned
                                          degree-1 attempt pauses prediction for 1, 2, 4, ... steps (at
                                          most guard).  0: every step predicted at the full degree */
+  int anderson;                       /* m = 1..8: Anderson acceleration of the FP64 phase (Walker and Ni
+                                         2011) over u, v, p, T of every cell: every anderson_every outer
+                                         iterations the next start becomes the affine combination of the
+                                         last m+1 stored iterates whose combined change is least in the
+                                         2-norm, and the derived fields are rebuilt from it; the test
+                                         still judges each outer iteration's own change, and the
+                                         iteration that meets it is the state the step keeps.  0: off */
+  int anderson_every;                 /* period of the Anderson step in outer iterations (0 or 1: every) */
 } sts_mixed_control;
 int sts_time_step_mixed(sts_ctx *ctx, const sts_loop_control *lc, const sts_mixed_control *mc, int *iters32,
                         int *iters64, double *res_log, double *p_probe);

@@ -82,7 +82,8 @@ class LoopControl(ctypes.Structure):
 class MixedControl(ctypes.Structure):
     _fields_ = [("N_I32", ctypes.c_int), ("tol32_uv", ctypes.c_double), ("tol32_p", ctypes.c_double),
                 ("tol32_T", ctypes.c_double), ("stall_w", ctypes.c_int), ("stall_ratio", ctypes.c_double),
-                ("extrapolate", ctypes.c_int), ("fp32_above", ctypes.c_int), ("guard", ctypes.c_int)]
+                ("extrapolate", ctypes.c_int), ("fp32_above", ctypes.c_int), ("guard", ctypes.c_int),
+                ("anderson", ctypes.c_int), ("anderson_every", ctypes.c_int)]
 
 
 class UNetNorm(ctypes.Structure):
@@ -306,13 +307,14 @@ class Context:
         return it.value, log[:6 * it.value].reshape(-1, 6), float(pr[0])
 
     def time_step_mixed(self, N_I, N_I_p_c, tol_uv, tol_p, tol_T, N_I32, tol32, stall_w=4, stall_ratio=0.7,
-                        batch=0, extrapolate=0, fp32_above=0, guard=0):
+                        batch=0, extrapolate=0, fp32_above=0, guard=0, anderson=0, anderson_every=1):
         """Mixed-precision time step (FP32 phase, then FP64 phase with the
         reference's test); needs precision=32.  tol32 = (uv, p, T) tolerances
         of the FP32 phase.  Returns (FP32 iterations, FP64 iterations,
         residual log [n32 + n64, 6], p probe)."""
         lc = LoopControl(N_I, N_I_p_c, tol_uv, tol_p, tol_T, batch)
-        mc = MixedControl(N_I32, tol32[0], tol32[1], tol32[2], stall_w, stall_ratio, extrapolate, fp32_above, guard)
+        mc = MixedControl(N_I32, tol32[0], tol32[1], tol32[2], stall_w, stall_ratio, extrapolate, fp32_above, guard,
+                          anderson, anderson_every)
         n32, n64 = ctypes.c_int(), ctypes.c_int()
         log, pr = np.zeros(6 * (N_I + N_I32)), np.zeros(1)
         _check(lib().sts_time_step_mixed(self.h, ctypes.byref(lc), ctypes.byref(mc), ctypes.byref(n32),
@@ -328,8 +330,8 @@ class Context:
         mc, n_i32 = None, 0
         if mixed is not None:
             n_i32, tol32, w, r = mixed[:4]
-            mc = MixedControl(n_i32, tol32[0], tol32[1], tol32[2], w, r, mixed[4] if len(mixed) > 4 else 0,
-                              mixed[5] if len(mixed) > 5 else 0, mixed[6] if len(mixed) > 6 else 0)
+            mc = MixedControl(n_i32, tol32[0], tol32[1], tol32[2], w, r,
+                              *(mixed[q] if len(mixed) > q else 0 for q in range(4, 9)))
         used, n32, n64 = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         log, pr = np.zeros(6 * (N_I + n_i32)), np.zeros(1)
         _check(lib().sts_time_step_ws(self.h, net.h, ctypes.byref(norm), ctypes.byref(lc),
@@ -376,15 +378,16 @@ class Batch:
 
     def time_step(self, N_I, N_I_p_c, tol_uv, tol_p, tol_T, mixed=None, batch=0):
         """One time step of every case; mixed = None or (N_I32, tol32,
-        stall_w, stall_ratio[, extrapolate[, fp32_above[, guard]]]).
+        stall_w, stall_ratio[, extrapolate[, fp32_above[, guard[, anderson[,
+        anderson_every]]]]]).
         Returns (FP32 counts, FP64 counts)."""
         n = len(self.contexts)
         lc = LoopControl(N_I, N_I_p_c, tol_uv, tol_p, tol_T, batch)
         mc = None
         if mixed is not None:
             n_i32, tol32, w, r = mixed[:4]
-            mc = MixedControl(n_i32, tol32[0], tol32[1], tol32[2], w, r, mixed[4] if len(mixed) > 4 else 0,
-                              mixed[5] if len(mixed) > 5 else 0, mixed[6] if len(mixed) > 6 else 0)
+            mc = MixedControl(n_i32, tol32[0], tol32[1], tol32[2], w, r,
+                              *(mixed[q] if len(mixed) > q else 0 for q in range(4, 9)))
         a, b = np.zeros(n, np.int32), np.zeros(n, np.int32)
         _check(lib().sts_batch_time_step(self.h, ctypes.byref(lc), ctypes.byref(mc) if mc is not None else None,
                                          a.ctypes.data_as(_IP), b.ctypes.data_as(_IP)), "sts_batch_time_step")

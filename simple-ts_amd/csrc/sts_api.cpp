@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "simple_ts.h"
@@ -86,6 +87,11 @@ struct sts_ctx {
   double *p_ws = nullptr, *T_ws = nullptr;  // predicted starting guess (warm start)
   ExtrapHist hist{};  // u, v, p, T of the last converged steps (extrapolated starts)
   int n_hist = 0;     // stored levels
+  // Anderson acceleration of the FP64 loop 2: ring of FP32 differences
+  // (aa_cap slots) and the per-context vectors
+  float *aa_dG[MAX_AA] = {}, *aa_dF[MAX_AA] = {};
+  int aa_cap = 0;
+  AABuf aa{};
   bool fuse_sweeps = true;  // pairs of loop-3 sweeps in one k_sweep2 pass
   // kernel timing: event pairs around launches of kernel `timed`
   int timed = -1;
@@ -213,6 +219,16 @@ extern "C" int sts_destroy(sts_ctx *c) {
   (void)hipFree(c->p_ws);
   for (auto &lv : c->hist.f)
     for (double *q : lv) (void)hipFree(q);
+  for (int j = 0; j < MAX_AA; j++) {
+    (void)hipFree(c->aa_dG[j]);
+    (void)hipFree(c->aa_dF[j]);
+  }
+  (void)hipFree(c->aa.gp);
+  (void)hipFree(c->aa.D);
+  (void)hipFree(c->aa.fp);
+  (void)hipFree(c->aa.part);
+  (void)hipFree(c->aa.gram);
+  (void)hipFree(c->aa.gam);
   (void)hipFree(c->wall_ops);
   for (unsigned *q : c->drag_cells) (void)hipFree(q);
   (void)hipFree(c->drag_terms);
@@ -710,7 +726,62 @@ struct Loop2State {
   KFieldsT<R> f;
   R *p_alt, *T_alt;
   int kid;  // timing id offset: 0 for FP64 kernels, STS_K_COUNT for FP32
+  int aa_m = 0, aa_every = 1;  // Anderson acceleration (FP64 only): history depth (0: off), period
 };
+
+// Anderson acceleration (history depth m, a step every `every` outer
+// iterations): the differences in use after iteration k's store.
+static AASel anderson_select(const sts_ctx *c, int k, int m, int every) {
+  const int K = (k + 1) / every - 1, n = std::min(K, m);
+  AASel sel{};
+  sel.n = n;
+  for (int j = 0; j < n; j++) {
+    const int sl = (K - n + j) % m;
+    sel.slot[j] = sl;
+    sel.dG[j] = c->aa_dG[sl];
+    sel.dF[j] = c->aa_dF[sl];
+  }
+  return sel;
+}
+
+// The Anderson step after iteration k's edge conditions, gated on iteration
+// k + 1 running (the iteration that meets the test is the state the step
+// keeps): the iteration is stored; with at least one difference in use the
+// mixed start replaces u, v, p, T, with sqrt_T, rho and the face densities,
+// and the wall and edge conditions are applied to it as an outer iteration
+// applies them.  The iteration's own flux and diffusion pass, which
+// follows, then works on the mixed state.
+static int enqueue_anderson(sts_ctx *c, const KFields &kf, int k, const AASel &sel) {
+  const Gate gn{c->ctrl, k + 1};
+  const AAFields af{{kf.f[STS_U], kf.f[STS_V], kf.f[STS_P], kf.f[STS_T]}};
+  HIP_OK(launch_aa_store(af, c->aa, sel, c->Na, c->stream, gn));
+  if (sel.n == 0) return STS_OK;
+  HIP_OK(launch_aa_mix(c->g, kf, af, c->aa, sel, c->stream, gn));
+  HIP_OK(launch_wall(c->g, kf, STS_BC_SLIP_U | STS_BC_SLIP_V | STS_BC_JUMP, c->stream, gn, c->wall_ops,
+                     c->n_wall_ops));
+  HIP_OK(launch_edges(c->g, kf, STS_BC_VELOCITIES | STS_BC_TEMPERATURE | STS_BC_PRESSURE, nullptr, c->stream, gn));
+  return STS_OK;
+}
+
+// Buffers of the Anderson step for history depth m.
+static int ensure_anderson(sts_ctx *c, int m) {
+  if (m < 1 || m > MAX_AA) return fail(STS_ERR_ARG, "Anderson history depth must be 0..8");
+  const size_t n4 = 4 * (size_t)c->Na;
+  if (!c->aa.gp) {
+    c->aa.nb = aa_store_blocks(c->Na);
+    HIP_OK(hipMalloc(&c->aa.gp, sizeof(double) * n4));
+    HIP_OK(hipMalloc(&c->aa.D, sizeof(float) * n4));
+    HIP_OK(hipMalloc(&c->aa.fp, sizeof(float) * n4));
+    HIP_OK(hipMalloc(&c->aa.part, sizeof(double) * 2 * MAX_AA * (size_t)c->aa.nb));
+    HIP_OK(hipMalloc(&c->aa.gram, sizeof(double) * MAX_AA * MAX_AA));
+    HIP_OK(hipMalloc(&c->aa.gam, sizeof(double) * MAX_AA));
+  }
+  for (; c->aa_cap < m; c->aa_cap++) {
+    HIP_OK(hipMalloc(&c->aa_dG[c->aa_cap], sizeof(float) * n4));
+    HIP_OK(hipMalloc(&c->aa_dF[c->aa_cap], sizeof(float) * n4));
+  }
+  return STS_OK;
+}
 
 // Enqueues outer iteration k of the current loop 2; the p/T buffer parity at
 // its start follows the passes of the iterations before it.  The kernels
@@ -751,6 +822,13 @@ static int enqueue_outer(sts_ctx *c, const Loop2State<R> &st, int k, int n, Loop
     par ^= 1;
   }
   const KFieldsT<R> kf = fields_at(par);
+  // Anderson acceleration (FP64): a step after the edge conditions
+  bool aa = false;
+  AASel sel{};
+  if constexpr (std::is_same<R, double>::value) {
+    aa = st.aa_m > 0 && (k + 1) % st.aa_every == 0;
+    if (aa) sel = anderson_select(c, k, st.aa_m, st.aa_every);
+  }
   t = timed_begin(c, st.kid + STS_K_CORRECT);
   HIP_OK(launch_correct(g, kf, lr + 0, lr + 1, 1, c->stream, gate));
   timed_end(c, t);
@@ -764,6 +842,11 @@ static int enqueue_outer(sts_ctx *c, const Loop2State<R> &st, int k, int n, Loop
   HIP_OK(launch_edges(g, kf, STS_BC_VELOCITIES | STS_BC_TEMPERATURE | STS_BC_PRESSURE, c->probe, c->stream, gate,
                       lt));
   timed_end(c, t);
+  if constexpr (std::is_same<R, double>::value)
+    if (aa) {
+      int r;
+      if ((r = enqueue_anderson(c, kf, k, sel))) return r;
+    }
   t = timed_begin(c, st.kid + STS_K_FLUX_DIFF);
   HIP_OK(launch_flux_diff(g, kf, c->stream, gate));
   timed_end(c, t);
@@ -779,6 +862,12 @@ template <typename R>
 static int run_loop2(sts_ctx *c, Loop2State<R> &st, R *&p, R *&T, R *&p_alt, R *&T_alt, int n_pc, LoopTest lt,
                      int batch, int *iters) {
   HIP_OK(launch_loop_open(c->ctrl, lt.log_k, 6 * lt.N_I, c->lred, c->stream));
+  if constexpr (std::is_same<R, double>::value)
+    if (st.aa_m > 0) {
+      int r;
+      if ((r = ensure_anderson(c, st.aa_m))) return r;
+      HIP_OK(launch_aa_begin(AAFields{{st.f.f[STS_U], st.f.f[STS_V], p, T}}, c->aa, c->Na, c->stream));
+    }
   int k = 0, stop = -1, rc;
   while (stop < 0) {
     const int end = std::min(lt.N_I, k + batch);
@@ -921,6 +1010,10 @@ static int run_phases(sts_ctx *c, const sts_loop_control *lc, const sts_mixed_co
     HIP_OK(launch_convert(c->fields32(), c->fields(), WIDEN_MASK, c->Na, c->stream));
   }
   Loop2State<double> s64{&c->g, c->fields(), c->p_alt, c->T_alt, 0};
+  if (mc && mc->anderson > 0) {
+    s64.aa_m = mc->anderson;
+    s64.aa_every = std::max(1, mc->anderson_every);
+  }
   int &last = mc ? c->last_iters64m : c->last_iters;
   const int b64 = lc->batch > 0 ? lc->batch : (last > 2 ? last - 1 : 1);
   rc = run_loop2(c, s64, c->f[STS_P], c->f[STS_T], c->p_alt, c->T_alt, lc->N_I_p_c,
@@ -934,7 +1027,9 @@ static int run_phases(sts_ctx *c, const sts_loop_control *lc, const sts_mixed_co
 
 static int check_mixed(const sts_ctx *c, const sts_loop_control *lc, const sts_mixed_control *mc) {
   if (!lc || lc->N_I < 1 || lc->N_I_p_c < 1 || lc->N_I_p_c > MAX_SWEEPS) return fail(STS_ERR_ARG, "bad loop control");
-  if (mc && (mc->N_I32 < 0 || mc->stall_w < 0)) return fail(STS_ERR_ARG, "bad mixed control");
+  if (mc && (mc->N_I32 < 0 || mc->stall_w < 0 || mc->anderson < 0 || mc->anderson > MAX_AA ||
+             mc->anderson_every < 0))
+    return fail(STS_ERR_ARG, "bad mixed control");
   if (mc && mc->N_I32 > 0 && !c->mixed) return fail(STS_ERR_UNSUPPORTED, "the context was not created with precision 32");
   return STS_OK;
 }
@@ -1317,6 +1412,7 @@ static int run_batch_loop2(sts_batch *b, BatchCase<R> *C, int n_pc, int N_I, int
 extern "C" int sts_batch_time_step(sts_batch *b, const sts_loop_control *lc, const sts_mixed_control *mc,
                                    int *iters32, int *iters64) {
   if (!b) return fail(STS_ERR_ARG, "null batch");
+  if (mc && mc->anderson > 0) return fail(STS_ERR_UNSUPPORTED, "Anderson acceleration runs per context, not in a batch");
   int rc;
   const int n = (int)b->cs.size();
   for (sts_ctx *c : b->cs) {

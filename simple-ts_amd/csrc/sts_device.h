@@ -153,6 +153,38 @@ struct ExtrapHist {
   double *f[MAX_EXTRAP][4];  // [h][u, v, p, T]
 };
 
+// Anderson acceleration of loop 2 (Walker and Ni, SIAM J. Numer. Anal. 49,
+// 2011), in the difference form: the state is (u, v, p, T) of every cell, an
+// outer iteration the map x -> g(x), f = g - x.  After an iteration the
+// differences dF = f - f_prev, dG = g - g_prev of the last MAX_AA or fewer
+// iterations give gamma = argmin |f - dF gamma|_2 and the next start
+// x = g - dG gamma.  Every vector holds a cell's four values side by side
+// (entry 4 k + q); the differences are stored in FP32.
+constexpr int MAX_AA = 8;
+// The differences in use, oldest first, newest last: ring slots (rows of the
+// Gram matrix) and buffers.
+struct AASel {
+  int n;
+  int slot[MAX_AA];
+  float *dG[MAX_AA];
+  float *dF[MAX_AA];
+};
+// Per-context vectors: gp = the newest iterate g (FP64), D = the current
+// start minus gp, fp = the newest f (both FP32); part: partial sums of the
+// store pass (2 MAX_AA rows of nb workgroups); gram: dF_a . dF_b by ring slot;
+// gam: the weights.
+struct AABuf {
+  double *gp;
+  float *D, *fp;
+  double *part, *gram, *gam;
+  int nb;
+};
+// The four state arrays as the outer iteration leaves them (p and T in
+// whichever ping-pong buffer holds them).
+struct AAFields {
+  double *x[4];  // u, v, p, T
+};
+
 // One case of a batch run in shared launches (device-resident): grid and
 // fields with p and T as they stand at the start of the step, the p/T
 // ping-pong partners, loop control, residual words and log, loop-2 test.

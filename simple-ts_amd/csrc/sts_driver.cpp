@@ -590,7 +590,7 @@ struct Driver {
   double loop_seconds = 0;
   // mixed-precision time steps (sts_time_step_mixed) when set
   bool mixed = false;
-  sts_mixed_control mc{0, 1e-6, 1e-6, 1e-6, 4, 0.7, 0, 0, 0};
+  sts_mixed_control mc{0, 1e-6, 1e-6, 1e-6, 4, 0.7, 0, 0, 0, 0, 0};
   long long outer_total32 = 0, outer32_at_start = 0;
   // timing of the steps after the warm-up ones (benchmarking)
   unsigned warmup_steps = 0;
@@ -616,7 +616,7 @@ int main(int argc, char **argv) {
   const char *knames[] = {"assemble",   "sweep",   "correct",   "wall",   "edges",   "flux_diff",
                           "assemble32", "sweep32", "correct32", "wall32", "edges32", "flux_diff32"};
   bool mixed = false;
-  sts_mixed_control mc{0, 1e-6, 1e-6, 1e-6, 4, 0.7, 0, 0, 0};
+  sts_mixed_control mc{0, 1e-6, 1e-6, 1e-6, 4, 0.7, 0, 0, 0, 0, 0};
   int n32 = -1;
   for (int a = 1; a < argc; a++) {
     std::string s = argv[a];
@@ -637,6 +637,10 @@ int main(int argc, char **argv) {
       mc.fp32_above = std::atoi(argv[++a]);
     } else if (s == "--guard" && a + 1 < argc) {
       mc.guard = std::atoi(argv[++a]);
+    } else if (s == "--anderson" && a + 1 < argc) {
+      mc.anderson = std::atoi(argv[++a]);
+    } else if (s == "--anderson-every" && a + 1 < argc) {
+      mc.anderson_every = std::atoi(argv[++a]);
     } else if (s == "--tol32" && a + 1 < argc) {
       mc.tol32_uv = mc.tol32_p = mc.tol32_T = std::atof(argv[++a]);
     } else if (s == "--stall" && a + 2 < argc) {
@@ -645,7 +649,8 @@ int main(int argc, char **argv) {
     } else if (s == "-h" || s == "--help") {
       std::cout << "usage: sts_gpu [case_dir] [--timing file.json] [--warmup steps] [--time-kernel "
                    "assemble|sweep|correct|wall|edges|flux_diff[32]] [--extrapolate degree] [--mixed [--n32 cap] "
-                   "[--tol32 tol] [--stall window ratio] [--fp32-above iterations]] [--guard max_pause]"
+                   "[--tol32 tol] [--stall window ratio] [--fp32-above iterations]] [--guard max_pause] "
+                   "[--anderson depth [--anderson-every iterations]]"
                 << std::endl;
       return 0;
     } else dir = s;
@@ -656,7 +661,7 @@ int main(int argc, char **argv) {
   d.time_kernel = kernel;
   d.mixed = mixed;
   mc.N_I32 = mixed ? n32 : 0;
-  d.mixed = mixed || mc.extrapolate > 0;
+  d.mixed = mixed || mc.extrapolate > 0 || mc.anderson > 0;
   d.mc = mc;
   std::string err = sts::load_case(dir, d.c);
   if (!err.empty()) die(err);

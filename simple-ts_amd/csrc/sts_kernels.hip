@@ -1324,6 +1324,312 @@ __global__ void __launch_bounds__(BX *BY) k_extrapolate(KGrid g, KFields fl, Ext
   }
 }
 
+// ---------------------------------------------------------------------------
+// Anderson acceleration of loop 2 (FP64), difference form.  The vectors of
+// the acceleration hold the four state values of a cell side by side
+// (v[4 k + q], q = u, v, p, T), so a thread moves a cell's entry of each
+// vector with one or two 16-byte accesses.
+// ---------------------------------------------------------------------------
+constexpr int AA_THREADS = 256;
+constexpr int AA_SOLVE_THREADS = 1024;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Sums acc[0..NV) over a workgroup of NT threads (fixed order: lanes, then
+// waves in turn); thread t < NV gets sum t in out[t].
+template <int NV, int NT>
+__device__ __forceinline__ void block_sums(const double (&acc)[NV], double *out) {
+  __shared__ double sm[NV][NT / 64];
+  const int lane = (int)threadIdx.x & 63, wid = (int)threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j < NV; j++) {
+    const double w = wave_sum(acc[j]);
+    if (lane == 0) sm[j][wid] = w;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < NV) {
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < NT / 64; q++) s += sm[threadIdx.x][q];
+    out[threadIdx.x] = s;
+  }
+}
+
+__device__ __forceinline__ void load4(const double *p, double (&v)[4]) {
+  const double2 a = reinterpret_cast<const double2 *>(p)[0], b = reinterpret_cast<const double2 *>(p)[1];
+  v[0] = a.x;
+  v[1] = a.y;
+  v[2] = b.x;
+  v[3] = b.y;
+}
+__device__ __forceinline__ void store4(double *p, const double (&v)[4]) {
+  reinterpret_cast<double2 *>(p)[0] = make_double2(v[0], v[1]);
+  reinterpret_cast<double2 *>(p)[1] = make_double2(v[2], v[3]);
+}
+__device__ __forceinline__ void load4(const float *p, double (&v)[4]) {
+  const float4 a = *reinterpret_cast<const float4 *>(p);
+  v[0] = (double)a.x;
+  v[1] = (double)a.y;
+  v[2] = (double)a.z;
+  v[3] = (double)a.w;
+}
+__device__ __forceinline__ void store4(float *p, const double (&v)[4]) {
+  *reinterpret_cast<float4 *>(p) = make_float4((float)v[0], (float)v[1], (float)v[2], (float)v[3]);
+}
+
+// Loop start: the start x = gp + D with gp = the state, D = 0.
+__global__ void __launch_bounds__(AA_THREADS) k_aa_begin(AAFields s, AABuf ab, int Na) {
+  const int k = (int)(blockIdx.x * AA_THREADS + threadIdx.x);
+  if (k >= Na) return;
+  const double g[4] = {s.x[0][k], s.x[1][k], s.x[2][k], s.x[3][k]}, z[4] = {0.0, 0.0, 0.0, 0.0};
+  store4(ab.gp + 4 * (size_t)k, g);
+  store4(ab.D + 4 * (size_t)k, z);
+}
+
+// After an outer iteration: g = the state, x = gp + D its start, f = g - x.
+// First iteration of the loop (sel.n = 0): the next start is g (gp = g,
+// D = 0), fp = f.  Later: the newest differences dF = f - fp, dG = g - gp
+// go to sel.dF/dG[n-1], fp = f, gp = g, and the workgroup's partial sums
+// a_j = dF_j . dF_new and b_j = dF_j . f (every difference in use) go to
+// part[j nb + block] and part[(MAX_AA + j) nb + block].  NM >= sel.n bounds
+// the differences at compile time.
+template <int NM>
+__global__ void __launch_bounds__(AA_THREADS) k_aa_store(AAFields s, AABuf ab, AASel sel, int Na, Gate gate) {
+  if (gated_out(gate)) return;
+  const int n = sel.n;
+  const int k = (int)(blockIdx.x * AA_THREADS + threadIdx.x);
+  double acc[2 * NM];
+#pragma unroll
+  for (int j = 0; j < 2 * NM; j++) acc[j] = 0.0;
+  if (k < Na) {
+    const size_t e = 4 * (size_t)k;
+    double g[4], gp[4], D[4], fp[4], o[NM][4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) g[q] = s.x[q][k];
+    load4(ab.gp + e, gp);
+    load4(ab.D + e, D);
+    if (n > 0) load4(ab.fp + e, fp);
+#pragma unroll
+    for (int j = 0; j < NM; j++) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) o[j][q] = 0.0;
+      if (j < n - 1) load4(sel.dF[j] + e, o[j]);
+    }
+    double f[4], dF[4], dG[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) f[q] = g[q] - (gp[q] + D[q]);
+    if (n == 0) {
+      const double z[4] = {0.0, 0.0, 0.0, 0.0};
+      store4(ab.D + e, z);
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        dF[q] = (double)(float)(f[q] - fp[q]);
+        dG[q] = g[q] - gp[q];
+      }
+#pragma unroll
+      for (int j = 0; j < NM; j++)
+        if (j == n - 1) {
+          store4(sel.dF[j] + e, dF);
+          store4(sel.dG[j] + e, dG);
+#pragma unroll
+          for (int q = 0; q < 4; q++) o[j][q] = dF[q];
+        }
+#pragma unroll
+      for (int j = 0; j < NM; j++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          acc[j] += o[j][q] * dF[q];
+          acc[NM + j] += o[j][q] * f[q];
+        }
+    }
+    store4(ab.gp + e, g);
+    store4(ab.fp + e, f);
+  }
+  if (n == 0) return;
+  __shared__ double sums[2 * NM];
+  block_sums<2 * NM, AA_THREADS>(acc, sums);
+  __syncthreads();
+  if ((int)threadIdx.x < 2 * NM) {
+    const int j = (int)threadIdx.x;
+    ab.part[(size_t)(j < NM ? j : MAX_AA + j - NM) * ab.nb + blockIdx.x] = sums[j];
+  }
+}
+
+// One workgroup: folds the store pass's partial sums, completes the Gram
+// matrix of the differences in use (the newest row from the sums, stored
+// for the next steps; the rest as stored), and solves
+// (Gram + 1e-12 trace I) gamma = b by Gaussian elimination with partial
+// pivoting into ab.gam.  A singular or non-finite solve gives gamma = 0
+// (the next start is g).
+template <int NM>
+__global__ void __launch_bounds__(AA_SOLVE_THREADS) k_aa_solve(AABuf ab, AASel sel, Gate gate) {
+  if (gated_out(gate)) return;
+  const int n = sel.n;
+  double acc[2 * NM];
+#pragma unroll
+  for (int j = 0; j < 2 * NM; j++) acc[j] = 0.0;
+  for (int b = threadIdx.x; b < ab.nb; b += AA_SOLVE_THREADS)
+#pragma unroll
+    for (int j = 0; j < NM; j++)
+      if (j < n) {
+        acc[j] += ab.part[(size_t)j * ab.nb + b];
+        acc[NM + j] += ab.part[(size_t)(MAX_AA + j) * ab.nb + b];
+      }
+  __shared__ double dots[2 * NM];
+  __shared__ double A[NM][NM + 1];
+  __shared__ double gam[NM];
+  if ((int)threadIdx.x < NM * NM) {  // the stored part of the Gram matrix
+    const int a = (int)threadIdx.x / NM, b = (int)threadIdx.x % NM;
+    if (a < n - 1 && b < n - 1) A[a][b] = ab.gram[sel.slot[a] * MAX_AA + sel.slot[b]];
+  }
+  block_sums<2 * NM, AA_SOLVE_THREADS>(acc, dots);
+  __syncthreads();
+  if ((int)threadIdx.x < n) {  // the newest row, kept for the next steps
+    const int kn = sel.slot[n - 1], sa = sel.slot[threadIdx.x];
+    ab.gram[sa * MAX_AA + kn] = dots[threadIdx.x];
+    ab.gram[kn * MAX_AA + sa] = dots[threadIdx.x];
+  }
+  if (threadIdx.x != 0) return;
+  double tr = 0.0;
+  for (int a = 0; a < n; a++) {
+    A[a][n - 1] = dots[a];
+    A[n - 1][a] = dots[a];
+  }
+  for (int a = 0; a < n; a++) {
+    A[a][n] = dots[NM + a];
+    tr += A[a][a];
+  }
+  for (int a = 0; a < n; a++) A[a][a] += 1e-12 * tr;
+  bool ok = true;
+  for (int col = 0; col < n && ok; col++) {
+    int piv = col;
+    for (int r = col + 1; r < n; r++)
+      if (fabs(A[r][col]) > fabs(A[piv][col])) piv = r;
+    if (!(fabs(A[piv][col]) > 0.0)) {
+      ok = false;
+      break;
+    }
+    if (piv != col)
+      for (int q = col; q <= n; q++) {
+        const double t = A[col][q];
+        A[col][q] = A[piv][q];
+        A[piv][q] = t;
+      }
+    for (int r = col + 1; r < n; r++) {
+      const double l = A[r][col] / A[col][col];
+      for (int q = col; q <= n; q++) A[r][q] -= l * A[col][q];
+    }
+  }
+  for (int a = n - 1; a >= 0 && ok; a--) {
+    double t = A[a][n];
+    for (int q = a + 1; q < n; q++) t -= A[a][q] * gam[q];
+    gam[a] = t / A[a][a];
+    ok = isfinite(gam[a]);
+  }
+  for (int a = 0; a < MAX_AA; a++) ab.gam[a] = (ok && a < n) ? gam[a] : 0.0;
+}
+
+// Mixed state of cell k: x = gp - sum_j gamma_j dG_j (gp = the newest
+// iterate); a value that is not finite, or a p or T that is not positive
+// where gp's is, falls back to gp.
+template <int NM>
+__device__ __forceinline__ void aa_mixed(const AABuf &ab, const AASel &sel, const double (&gam)[NM], unsigned k,
+                                         double (&x)[4]) {
+  const size_t e = 4 * (size_t)k;
+  double gp[4], dG[NM][4];
+  load4(ab.gp + e, gp);
+#pragma unroll
+  for (int j = 0; j < NM; j++)
+    if (j < sel.n) load4(sel.dG[j] + e, dG[j]);
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    double v = gp[q];
+#pragma unroll
+    for (int j = 0; j < NM; j++)
+      if (j < sel.n) v -= gam[j] * dG[j][q];
+    if (!isfinite(v) || (q >= 2 && !(v > 0.0) && gp[q] > 0.0)) v = gp[q];
+    x[q] = v;
+  }
+}
+
+// The next start x into the state arrays (and D = x - gp), and, from it,
+// what the correction pass derives without correcting: sqrt_T, rho and the
+// face densities (CorrectIn::apply on the mixed p, T of the cell and of its
+// west and south neighbours).  BX x BY tiles; the mixed p, T of the tile and
+// of its west column and south row (formed by the first threads) meet in LDS.
+template <int NM>
+__global__ void __launch_bounds__(BX *BY) k_aa_mix(KGrid g, KFields fl, AAFields s, AABuf ab, AASel sel,
+                                                   Gate gate) {
+  if (gated_out(gate)) return;
+  __shared__ double sp[BY + 1][BX + 1], sT[BY + 1][BX + 1];  // [1 + row][1 + column]; row 0, column 0: halo
+  double gam[NM];
+#pragma unroll
+  for (int j = 0; j < NM; j++) gam[j] = ab.gam[j];
+  int i0, j0;
+  tile_origin<BX, BY>(g.Nx, i0, j0);
+  const int Nx = g.Nx, Ny = g.Ny, tx = (int)threadIdx.x, ty = (int)threadIdx.y;
+  const int i = i0 + tx, j = j0 + ty;
+  const bool in = i < Nx && j < Ny;
+  const Nbr nb((unsigned)(in ? i : 0), (unsigned)(in ? j : 0), (unsigned)Nx, (unsigned)Ny, g.px);
+  double x[4] = {0.0, 0.0, 0.0, 0.0};
+  if (in) {
+    aa_mixed<NM>(ab, sel, gam, nb.ij, x);
+    sp[ty + 1][tx + 1] = x[2];
+    sT[ty + 1][tx + 1] = x[3];
+  }
+  // halo: thread (t, 0) the cell south of column t, thread (t < BY, 1) the
+  // cell west of row t (the clamped or wrapped neighbour of the reference)
+  double h[4];
+  if (ty == 0 && in) {
+    aa_mixed<NM>(ab, sel, gam, nb.ij_1, h);
+    sp[0][tx + 1] = h[2];
+    sT[0][tx + 1] = h[3];
+  }
+  if (ty == 1 && tx < BY && i0 < Nx && j0 + tx < Ny) {
+    const Nbr w((unsigned)i0, (unsigned)(j0 + tx), (unsigned)Nx, (unsigned)Ny, g.px);
+    aa_mixed<NM>(ab, sel, gam, w.i_1j, h);
+    sp[tx + 1][0] = h[2];
+    sT[tx + 1][0] = h[3];
+  }
+  __syncthreads();
+  if (!in) return;
+  CorrectIn<double> ci;
+  ci.in = true;
+  ci.cu = ci.cv = false;
+  ci.ij = nb.ij;
+  ci.i_1j = nb.i_1j;
+  ci.ij_1 = nb.ij_1;
+  ci.mc = g.m[nb.ij];
+  ci.mw = g.m[nb.i_1j];
+  ci.ms = g.m[nb.ij_1];
+  ci.uf = x[0];
+  ci.vf = x[1];
+  ci.pc = x[2];
+  ci.Tc = x[3];
+  ci.pw = sp[ty + 1][tx];
+  ci.Tw = sT[ty + 1][tx];
+  ci.ps = sp[ty][tx + 1];
+  ci.Ts = sT[ty][tx + 1];
+  double d[4], gp[4];
+  load4(ab.gp + 4 * (size_t)nb.ij, gp);
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    s.x[q][nb.ij] = x[q];
+    d[q] = x[q] - gp[q];
+  }
+  store4(ab.D + 4 * (size_t)nb.ij, d);
+  KFields kf = fl;
+  kf.f[STS_P] = s.x[2];
+  kf.f[STS_T] = s.x[3];
+  unsigned long long rr[2] = {0ull, 0ull};
+  ci.apply(kf, rr);
+}
+
 __global__ void __launch_bounds__(BX *BY) k_set_inflow(KGrid g, KFields fl, double u_in) {
   const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
   if (i >= g.Nx_u0 || j >= g.Ny) return;
@@ -1725,6 +2031,30 @@ hipError_t launch_copy5(const double *const *src, double *const *dst, int n, hip
 hipError_t launch_extrapolate(const KGrid &g, const KFields &f, const ExtrapHist &hist, int order, int slot,
                               hipStream_t s) {
   hipLaunchKernelGGL(k_extrapolate, grid2d(g), dim3(BX, BY), 0, s, g, f, hist, order, slot);
+  return hipGetLastError();
+}
+int aa_store_blocks(int Na) { return (Na + AA_THREADS - 1) / AA_THREADS; }
+hipError_t launch_aa_begin(const AAFields &f, const AABuf &ab, int Na, hipStream_t s) {
+  hipLaunchKernelGGL(k_aa_begin, dim3(aa_store_blocks(Na)), dim3(AA_THREADS), 0, s, f, ab, Na);
+  return hipGetLastError();
+}
+hipError_t launch_aa_store(const AAFields &f, const AABuf &ab, const AASel &sel, int Na, hipStream_t s, Gate gate) {
+  if (sel.n < 0 || sel.n > MAX_AA || ab.nb != aa_store_blocks(Na)) return hipErrorInvalidValue;
+  if (sel.n <= 4) hipLaunchKernelGGL(k_aa_store<4>, dim3(ab.nb), dim3(AA_THREADS), 0, s, f, ab, sel, Na, gate);
+  else hipLaunchKernelGGL(k_aa_store<MAX_AA>, dim3(ab.nb), dim3(AA_THREADS), 0, s, f, ab, sel, Na, gate);
+  return hipGetLastError();
+}
+hipError_t launch_aa_mix(const KGrid &g, const KFields &fl, const AAFields &f, const AABuf &ab, const AASel &sel,
+                         hipStream_t s, Gate gate) {
+  const int Na = g.Nx * g.Ny;
+  if (sel.n < 1 || sel.n > MAX_AA || ab.nb != aa_store_blocks(Na)) return hipErrorInvalidValue;
+  if (sel.n <= 4) {
+    hipLaunchKernelGGL(k_aa_solve<4>, dim3(1), dim3(AA_SOLVE_THREADS), 0, s, ab, sel, gate);
+    hipLaunchKernelGGL(k_aa_mix<4>, (grid_tiles<BX, BY>(g)), dim3(BX, BY), 0, s, g, fl, f, ab, sel, gate);
+  } else {
+    hipLaunchKernelGGL(k_aa_solve<MAX_AA>, dim3(1), dim3(AA_SOLVE_THREADS), 0, s, ab, sel, gate);
+    hipLaunchKernelGGL(k_aa_mix<MAX_AA>, (grid_tiles<BX, BY>(g)), dim3(BX, BY), 0, s, g, fl, f, ab, sel, gate);
+  }
   return hipGetLastError();
 }
 hipError_t launch_set_inflow(const KGrid &g, const KFields &f, double u_in, hipStream_t s) {

@@ -43,6 +43,19 @@ hipError_t launch_set_inflow(const KGrid &g, const KFields &f, double u_in, hipS
 // (h < order); x^n is stored into hist[slot]; order 0 only stores.
 hipError_t launch_extrapolate(const KGrid &g, const KFields &f, const ExtrapHist &hist, int order, int slot,
                               hipStream_t s);
+// Anderson acceleration of loop 2.  launch_aa_begin at the loop start;
+// after an outer iteration's edge conditions, launch_aa_store (its
+// differences and their partial dot products), then, when sel.n > 0,
+// launch_aa_mix (solve for the weights, mix into the state arrays, with
+// sqrt_T, rho and the face densities of the mixed state).
+// ab.nb = aa_store_blocks(Na); ab.part: 2 MAX_AA * ab.nb doubles; ab.gram:
+// MAX_AA^2; ab.gam: MAX_AA; ab.gp: 4 Na doubles; ab.D, ab.fp and every
+// sel.dG/dF: 4 Na floats.
+int aa_store_blocks(int Na);
+hipError_t launch_aa_begin(const AAFields &f, const AABuf &ab, int Na, hipStream_t s);
+hipError_t launch_aa_store(const AAFields &f, const AABuf &ab, const AASel &sel, int Na, hipStream_t s, Gate gate);
+hipError_t launch_aa_mix(const KGrid &g, const KFields &fl, const AAFields &f, const AABuf &ab, const AASel &sel,
+                         hipStream_t s, Gate gate);
 hipError_t launch_loop_open(LoopCtrl *ctrl, unsigned long long *log, int n, unsigned long long *lred,
                             hipStream_t s);
 template <typename S, typename D>
