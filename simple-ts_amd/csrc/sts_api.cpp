@@ -747,19 +747,20 @@ static AASel anderson_select(const sts_ctx *c, int k, int m, int every) {
 // The Anderson step after iteration k's edge conditions, gated on iteration
 // k + 1 running (the iteration that meets the test is the state the step
 // keeps): the iteration is stored; with at least one difference in use the
-// mixed start replaces u, v, p, T, with sqrt_T, rho and the face densities,
-// and the wall and edge conditions are applied to it as an outer iteration
-// applies them.  The iteration's own flux and diffusion pass, which
-// follows, then works on the mixed state.
+// mixed start replaces u, v, p, T, with sqrt_T, rho and the face densities.
+// The wall and edge conditions are not applied again: every stored iterate
+// satisfies them, so the affine combination satisfies each condition that
+// is affine in the state (given values, zero-gradient copies, constant-Kn
+// slip and jump); the wall temperatures stay the newest iterate's, and the
+// rest (local-Kn slip and jump, the rho u copies) the next iteration
+// restores, whose own change the test judges.  The iteration's flux and
+// diffusion pass, which follows, then works on the mixed state.
 static int enqueue_anderson(sts_ctx *c, const KFields &kf, int k, const AASel &sel) {
   const Gate gn{c->ctrl, k + 1};
   const AAFields af{{kf.f[STS_U], kf.f[STS_V], kf.f[STS_P], kf.f[STS_T]}};
   HIP_OK(launch_aa_store(af, c->aa, sel, c->Na, c->stream, gn));
   if (sel.n == 0) return STS_OK;
   HIP_OK(launch_aa_mix(c->g, kf, af, c->aa, sel, c->stream, gn));
-  HIP_OK(launch_wall(c->g, kf, STS_BC_SLIP_U | STS_BC_SLIP_V | STS_BC_JUMP, c->stream, gn, c->wall_ops,
-                     c->n_wall_ops));
-  HIP_OK(launch_edges(c->g, kf, STS_BC_VELOCITIES | STS_BC_TEMPERATURE | STS_BC_PRESSURE, nullptr, c->stream, gn));
   return STS_OK;
 }
 

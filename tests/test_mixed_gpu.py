@@ -178,7 +178,7 @@ def test_guarded_predicted_start(tmp_path):
 REL_L2_AA = 1e-5
 
 
-@pytest.mark.parametrize("depth,every,degree", [(3, 1, 0), (3, 2, 0), (8, 1, 0), (3, 1, 2)])
+@pytest.mark.parametrize("depth,every,degree", [(3, 1, 0), (3, 2, 0), (8, 1, 0), (3, 1, 2), (2, 1, 3)])
 def test_anderson_matches_fp64_path(depth, every, degree, tmp_path):
     """Anderson-accelerated loop 2 (history `depth`, a step every `every`
     outer iterations, optionally from the predicted start) still ends on the
