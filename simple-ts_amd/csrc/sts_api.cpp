@@ -85,6 +85,13 @@ struct sts_ctx {
   double *drag_terms = nullptr, *drag_out = nullptr;
   size_t drag_cap = 0;
   double *p_ws = nullptr, *T_ws = nullptr;  // predicted starting guess (warm start)
+  // Gamma_* and D_*: whether forming them from sqrt_T, the wall temperatures
+  // and the mask gives the values the reference holds (true after a flux and
+  // diffusion pass of the library, until those inputs or the arrays are set
+  // another way), and whether the arrays lag behind (the loops' passes write
+  // only the fluxes and the thermal coefficients, and the assembly forms the
+  // rest itself)
+  bool diff_inline_ok = false, diff_stale = false;
   ExtrapHist hist{};  // u, v, p, T of the last converged steps (extrapolated starts)
   int n_hist = 0;     // stored levels
   // Anderson acceleration of the FP64 loop 2: ring of FP32 differences
@@ -461,8 +468,33 @@ extern "C" int sts_create(const sts_grid *gi, int precision, sts_ctx **out) {
   return STS_OK;
 }
 
+// Writes Gamma_* and D_* where the loops left them behind (from the inputs
+// as the loops left them); after this the arrays hold the reference's values.
+static int refresh_diff(sts_ctx *c, hipStream_t s = nullptr) {
+  if (!c->diff_stale) return STS_OK;
+  HIP_OK(launch_flux_diff(c->g, c->fields(), s ? s : c->stream, Gate{nullptr, 0}, DIFF_PART));
+  c->diff_stale = false;
+  return STS_OK;
+}
+
+// Before a stage that changes sqrt_T or the wall temperatures: the arrays
+// are brought up to date first, and no longer follow from the inputs.
+static int diff_inputs_change(sts_ctx *c) {
+  int rc = refresh_diff(c);
+  c->diff_inline_ok = false;
+  return rc;
+}
+
+static bool diff_field(int f) {
+  return f == STS_GAMMA_XF || f == STS_GAMMA_YF || f == STS_D_UX || f == STS_D_UY || f == STS_D_VX || f == STS_D_VY;
+}
+
 extern "C" int sts_set_field(sts_ctx *c, int field, const double *host) {
   if (!c || !host || field < 0 || field >= STS_NFIELDS) return fail(STS_ERR_ARG, "bad field or pointer");
+  if (diff_field(field) || field == STS_SQRT_T || field == STS_TW_U || field == STS_TW_V) {
+    int rc = diff_inputs_change(c);
+    if (rc) return rc;
+  }
   c->n_hist = 0;  // the state no longer continues the previous steps
   c->last_total = -1;
   c->plain_mean = -1.0;
@@ -477,6 +509,10 @@ extern "C" int sts_set_field(sts_ctx *c, int field, const double *host) {
 
 extern "C" int sts_get_field(sts_ctx *c, int field, double *host) {
   if (!c || !host || field < 0 || field >= STS_NFIELDS) return fail(STS_ERR_ARG, "bad field or pointer");
+  if (diff_field(field)) {
+    int rc = refresh_diff(c);
+    if (rc) return rc;
+  }
   HIP_OK(hipMemcpyAsync(host, c->f[field], sizeof(double) * (size_t)c->Na, hipMemcpyDeviceToHost, c->stream));
   HIP_OK(hipStreamSynchronize(c->stream));
   return STS_OK;
@@ -484,6 +520,8 @@ extern "C" int sts_get_field(sts_ctx *c, int field, double *host) {
 
 extern "C" int sts_set_scalars(sts_ctx *c, double cT, double cPch) {
   if (!c) return fail(STS_ERR_ARG, "null context");
+  int rc = diff_inputs_change(c);  // cT enters D_*
+  if (rc) return rc;
   c->g.cT = cT;
   c->g.cPch = cPch;
   if (c->mixed) narrow_grid(c);
@@ -506,6 +544,8 @@ extern "C" int sts_save_previous(sts_ctx *c) {
 
 extern "C" int sts_assemble(sts_ctx *c) {
   if (!c) return fail(STS_ERR_ARG, "null context");
+  int rc = refresh_diff(c);
+  if (rc) return rc;
   const int t = timed_begin(c, STS_K_ASSEMBLE);
   HIP_OK(launch_assemble(c->g, c->fields(), c->stream));
   timed_end(c, t);
@@ -571,6 +611,7 @@ extern "C" int sts_pt_jacobi(sts_ctx *c, int n, double *res) {
 
 extern "C" int sts_correct(sts_ctx *c, double *res) {
   if (!c) return fail(STS_ERR_ARG, "null context");
+  if (int e = diff_inputs_change(c)) return e;
   HIP_OK(hipMemsetAsync(c->red, 0, sizeof(unsigned long long) * NSLOT * SLOT_STRIDE, c->stream));
   const int t = timed_begin(c, STS_K_CORRECT);
   HIP_OK(launch_correct(c->g, c->fields(), c->red, c->red + 1, 1, c->stream));
@@ -586,6 +627,7 @@ extern "C" int sts_correct(sts_ctx *c, double *res) {
 
 extern "C" int sts_properties(sts_ctx *c) {
   if (!c) return fail(STS_ERR_ARG, "null context");
+  if (int e = diff_inputs_change(c)) return e;
   HIP_OK(launch_properties(c->g, c->fields(), c->stream));
   return STS_OK;
 }
@@ -617,6 +659,8 @@ static int enqueue_bcs(sts_ctx *c, int parts, bool probe) {
 
 extern "C" int sts_boundary_conditions(sts_ctx *c, int parts, double *p_probe) {
   if (!c || (parts & ~STS_BC_ALL)) return fail(STS_ERR_ARG, "bad boundary-condition parts");
+  if (parts & STS_BC_JUMP)
+    if (int e = diff_inputs_change(c)) return e;
   int rc = enqueue_bcs(c, parts, p_probe != nullptr);
   if (rc) return rc;
   if (p_probe) {
@@ -631,6 +675,8 @@ extern "C" int sts_fluxes_diffusion(sts_ctx *c) {
   const int t = timed_begin(c, STS_K_FLUX_DIFF);
   HIP_OK(launch_flux_diff(c->g, c->fields(), c->stream));
   timed_end(c, t);
+  c->diff_inline_ok = true;
+  c->diff_stale = false;
   return STS_OK;
 }
 
@@ -642,6 +688,7 @@ extern "C" int sts_apply_bodies(sts_ctx *c) {
 
 extern "C" int sts_jump_overwrite(sts_ctx *c) {
   if (!c) return fail(STS_ERR_ARG, "null context");
+  if (int e = diff_inputs_change(c)) return e;  // the jump pass rewrites the wall temperatures
   HIP_OK(launch_wall(c->g, c->fields(), STS_BC_JUMP, c->stream));
   HIP_OK(launch_jump_overwrite(c->g, c->fields(), c->stream));
   return STS_OK;
@@ -687,6 +734,7 @@ extern "C" int sts_continuity(sts_ctx *c, double *cont_stat, double *cont, doubl
 extern "C" int sts_outer_iteration(sts_ctx *c, int n_pc, double *res, double *res_p0T0, double *p_probe) {
   if (!c || n_pc < 1 || n_pc > MAX_SWEEPS) return fail(STS_ERR_ARG, "N_I_p_c must be 1..62");
   const int nslots = 2 + 2 * n_pc;
+  if (int e = refresh_diff(c)) return e;
   HIP_OK(hipMemsetAsync(c->red, 0, sizeof(unsigned long long) * NSLOT * SLOT_STRIDE, c->stream));
   int t = timed_begin(c, STS_K_ASSEMBLE);
   HIP_OK(launch_assemble(c->g, c->fields(), c->stream));
@@ -700,6 +748,8 @@ extern "C" int sts_outer_iteration(sts_ctx *c, int n_pc, double *res, double *re
   t = timed_begin(c, STS_K_FLUX_DIFF);
   HIP_OK(launch_flux_diff(c->g, c->fields(), c->stream));
   timed_end(c, t);
+  c->diff_inline_ok = true;
+  c->diff_stale = false;
   if ((rc = read_red(c, nslots, true))) return rc;
   if (res) {
     res[0] = bits_to_double(c->h_red[0]);
@@ -727,6 +777,7 @@ struct Loop2State {
   R *p_alt, *T_alt;
   int kid;  // timing id offset: 0 for FP64 kernels, STS_K_COUNT for FP32
   int aa_m = 0, aa_every = 1;  // Anderson acceleration (FP64 only): history depth (0: off), period
+  bool inline0 = false;        // the first iteration may form the diffusion coefficients itself
 };
 
 // Anderson acceleration (history depth m, a step every `every` outer
@@ -801,8 +852,11 @@ static int enqueue_outer(sts_ctx *c, const Loop2State<R> &st, int k, int n, Loop
     return kf;
   };
   int par = (int)(((long long)k * sweep_passes(c, n)) & 1);
+  // after the loop's first flux pass the assembly forms the diffusion
+  // coefficients from the inputs that pass saw; the pass itself writes only
+  // the fluxes and thermal coefficients
   int t = timed_begin(c, st.kid + STS_K_ASSEMBLE);
-  HIP_OK(launch_assemble(g, fields_at(par), c->stream, gate));
+  HIP_OK(launch_assemble(g, fields_at(par), c->stream, gate, k > 0 || st.inline0));
   timed_end(c, t);
   for (int s = 0; s < n;) {
     KFieldsT<R> kf = fields_at(par);
@@ -849,7 +903,7 @@ static int enqueue_outer(sts_ctx *c, const Loop2State<R> &st, int k, int n, Loop
       if ((r = enqueue_anderson(c, kf, k, sel))) return r;
     }
   t = timed_begin(c, st.kid + STS_K_FLUX_DIFF);
-  HIP_OK(launch_flux_diff(g, kf, c->stream, gate));
+  HIP_OK(launch_flux_diff(g, kf, c->stream, gate, FLUX_PART));
   timed_end(c, t);
   return STS_OK;
 }
@@ -930,10 +984,12 @@ extern "C" int sts_time_step(sts_ctx *c, const sts_loop_control *lc, int *iterat
   if ((rc = ensure_loop_buffers(c, lc->N_I))) return rc;
   if ((rc = sts_save_previous(c))) return rc;
   Loop2State<double> st{&c->g, c->fields(), c->p_alt, c->T_alt, 0};
+  st.inline0 = c->diff_inline_ok;
   const int batch = lc->batch > 0 ? lc->batch : (c->last_iters > 2 ? c->last_iters - 1 : 1);
   int iters = 0;
   rc = run_loop2(c, st, c->f[STS_P], c->f[STS_T], c->p_alt, c->T_alt, lc->N_I_p_c,
                  loop_test(c, c->log, lc->N_I, lc->tol_uv, lc->tol_p, lc->tol_T, 0, 0.0), batch, &iters);
+  c->diff_inline_ok = c->diff_stale = true;
   c->last_iters = iters;
   if (iterations) *iterations = iters;
   int rc2 = finish_step(c, iters, res_log, p_probe);
@@ -941,12 +997,11 @@ extern "C" int sts_time_step(sts_ctx *c, const sts_loop_control *lc, int *iterat
 }
 
 // Fields an outer iteration reads before it writes them: the FP32 phase hands
-// these to the FP64 phase.
+// these to the FP64 phase (whose assembly then forms Gamma and D itself).
 static constexpr unsigned long long bit(int f) { return 1ull << f; }
 static constexpr unsigned long long WIDEN_MASK =
     bit(STS_U) | bit(STS_V) | bit(STS_P) | bit(STS_T) | bit(STS_RHO) | bit(STS_SQRT_T) | bit(STS_RHO_U) |
-    bit(STS_RHO_V) | bit(STS_FX) | bit(STS_FY) | bit(STS_GAMMA_XF) | bit(STS_GAMMA_YF) | bit(STS_D_UX) |
-    bit(STS_D_UY) | bit(STS_D_VX) | bit(STS_D_VY) | bit(STS_DTX) | bit(STS_DTY) | bit(STS_TW_U) | bit(STS_TW_V);
+    bit(STS_RHO_V) | bit(STS_FX) | bit(STS_FY) | bit(STS_DTX) | bit(STS_DTY) | bit(STS_TW_U) | bit(STS_TW_V);
 
 // After the prologue: loop 2 starts from the polynomial of degree `order`
 // (capped by the stored steps) through x^n and the previous converged steps
@@ -981,7 +1036,8 @@ static int extrapolate_start(sts_ctx *c, int order, hipStream_t s = nullptr, boo
   HIP_OK(launch_wall(c->g, kf, STS_BC_SLIP_U | STS_BC_SLIP_V | STS_BC_JUMP, s, Gate{nullptr, 0}, c->wall_ops,
                      c->n_wall_ops));
   HIP_OK(launch_edges(c->g, kf, STS_BC_VELOCITIES | STS_BC_TEMPERATURE | STS_BC_PRESSURE, nullptr, s));
-  HIP_OK(launch_flux_diff(c->g, kf, s));
+  HIP_OK(launch_flux_diff(c->g, kf, s, Gate{nullptr, 0}, FLUX_PART));
+  c->diff_inline_ok = c->diff_stale = true;
   return STS_OK;
 }
 
@@ -997,6 +1053,9 @@ static int run_phases(sts_ctx *c, const sts_loop_control *lc, const sts_mixed_co
   int rc, n32 = 0, n64 = 0;
   *n32p = *n64p = 0;
   if (fp32_phase(c, mc)) {
+    // the FP32 phase starts from the narrowed FP64 arrays, Gamma and D
+    // included (as the batched steps do)
+    if ((rc = refresh_diff(c))) return rc;
     const unsigned long long all = (1ull << STS_NFIELDS) - 1ull;
     HIP_OK(launch_convert(c->fields(), c->fields32(), all, c->Na, c->stream));
     Loop2State<float> s32{&c->g32, c->fields32(), c->p_alt32, c->T_alt32, STS_K_COUNT};
@@ -1009,8 +1068,10 @@ static int run_phases(sts_ctx *c, const sts_loop_control *lc, const sts_mixed_co
     *n32p = n32;
     if (rc) return rc;
     HIP_OK(launch_convert(c->fields32(), c->fields(), WIDEN_MASK, c->Na, c->stream));
+    c->diff_inline_ok = c->diff_stale = true;
   }
   Loop2State<double> s64{&c->g, c->fields(), c->p_alt, c->T_alt, 0};
+  s64.inline0 = c->diff_inline_ok;
   if (mc && mc->anderson > 0) {
     s64.aa_m = mc->anderson;
     s64.aa_every = std::max(1, mc->anderson_every);
@@ -1020,6 +1081,7 @@ static int run_phases(sts_ctx *c, const sts_loop_control *lc, const sts_mixed_co
   rc = run_loop2(c, s64, c->f[STS_P], c->f[STS_T], c->p_alt, c->T_alt, lc->N_I_p_c,
                  loop_test(c, c->log + 6 * (size_t)n32, lc->N_I, lc->tol_uv, lc->tol_p, lc->tol_T, 0, 0.0), b64,
                  &n64);
+  c->diff_inline_ok = c->diff_stale = true;
   last = n64;
   *n64p = n64;
   c->last_total = n32 + n64;
@@ -1048,7 +1110,8 @@ static int restart_from_previous(sts_ctx *c) {
   HIP_OK(launch_wall(c->g, kf, STS_BC_SLIP_U | STS_BC_SLIP_V | STS_BC_JUMP, c->stream, Gate{nullptr, 0},
                      c->wall_ops, c->n_wall_ops));
   HIP_OK(launch_edges(c->g, kf, STS_BC_VELOCITIES | STS_BC_TEMPERATURE | STS_BC_PRESSURE, nullptr, c->stream));
-  HIP_OK(launch_flux_diff(c->g, kf, c->stream));
+  HIP_OK(launch_flux_diff(c->g, kf, c->stream, Gate{nullptr, 0}, FLUX_PART));
+  c->diff_inline_ok = c->diff_stale = true;
   return STS_OK;
 }
 
@@ -1171,7 +1234,7 @@ extern "C" int sts_time_step_ws(sts_ctx *c, sts_unet *net, const sts_unet_norm *
   // First sweep of iteration 1 from both guesses; the coefficients do not
   // depend on the guess (assembly reads neither p nor T).
   HIP_OK(hipMemsetAsync(c->red, 0, sizeof(unsigned long long) * NSLOT * SLOT_STRIDE, c->stream));
-  HIP_OK(launch_assemble(c->g, c->fields(), c->stream));
+  HIP_OK(launch_assemble(c->g, c->fields(), c->stream, Gate{nullptr, 0}, c->diff_inline_ok));
   HIP_OK(launch_sweep(c->g, c->fields(), (const double *)c->f[STS_P], (const double *)c->f[STS_T], c->p_alt, c->T_alt,
                       c->red + 2, c->red + 3, c->stream));
   HIP_OK(launch_sweep(c->g, c->fields(), (const double *)c->p_ws, (const double *)c->T_ws, c->p_alt, c->T_alt,
@@ -1448,6 +1511,7 @@ extern "C" int sts_batch_time_step(sts_batch *b, const sts_loop_control *lc, con
     }
     predicted[i] = pred && c->n_hist > 0;
     if ((rc = extrapolate_start(c, q, s, !pred))) return rc;
+    if ((rc = refresh_diff(c, s))) return rc;  // the batched assembly reads Gamma and D
   }
   std::vector<int> n32(n, 0), n64(n, 0);
   // cases the FP32 gate sends straight to FP64 sit the FP32 phase out
@@ -1471,8 +1535,11 @@ extern "C" int sts_batch_time_step(sts_batch *b, const sts_loop_control *lc, con
       if (!skip32[i]) lo = std::min(lo, n32[i]);
     b->last_iters32 = lo;
     for (int i = 0; i < n; i++)
-      if (!skip32[i])
+      if (!skip32[i]) {
         HIP_OK(launch_convert(b->cs[i]->fields32(), b->cs[i]->fields(), WIDEN_MASK, b->cs[i]->Na, s));
+        // the batched assembly reads Gamma and D: formed from the widened state
+        HIP_OK(launch_flux_diff(b->cs[i]->g, b->cs[i]->fields(), s, Gate{nullptr, 0}, DIFF_PART));
+      }
   }
   // the FP64 logs follow the FP32 ones of each case (offset = the largest FP32 count)
   const int off = mc ? mc->N_I32 : 0;
@@ -1487,6 +1554,8 @@ extern "C" int sts_batch_time_step(sts_batch *b, const sts_loop_control *lc, con
     sts_ctx *c = b->cs[i];
     c->last_iters = n64[i];
     c->last_total = n32[i] + n64[i];
+    c->diff_inline_ok = true;  // the batched flux and diffusion pass writes every array
+    c->diff_stale = false;
     if (mc && mc->extrapolate > 0 && mc->guard > 0) {
       if (!predicted[i]) {
         c->plain_mean = c->plain_mean < 0.0 ? n64[i] : 0.5 * (c->plain_mean + n64[i]);
@@ -1544,6 +1613,10 @@ extern "C" int sts_batch_autotune(sts_batch *b, int reps, int *cell_variant, int
   for (int v = 1; v < NVARIANT; v++) {
     if (t_cell[v] < t_cell[bc]) bc = v;
     if (t_sweep[v] < t_sweep[bs]) bs = v;
+  }
+  for (sts_ctx *c : b->cs) {  // the timed flux and diffusion passes wrote every array
+    c->diff_inline_ok = true;
+    c->diff_stale = false;
   }
   b->cell_v = bc;
   b->sweep_v = bs;

@@ -147,6 +147,10 @@ struct WallOp {
 };
 constexpr int WALL_OPS_PER_TILE = 256;
 
+// Parts of the flux and diffusion pass (S:10440-10618): fluxes and thermal
+// diffusion coefficients; Gamma and the velocity diffusion coefficients.
+enum : int { FLUX_PART = 1, DIFF_PART = 2 };
+
 // Converged u, v, p, T of the last steps: f[h] = x^(n-1-h) (extrapolated starts).
 constexpr int MAX_EXTRAP = 5;
 struct ExtrapHist {

@@ -91,14 +91,160 @@ __device__ __forceinline__ void block_max_atomic(unsigned long long (&v)[NV], un
 }
 
 // ---------------------------------------------------------------------------
+// A10 + A11 per face: fluxes (S:10440-10454) and diffusion coefficients
+// (S:10457-10618).
+// ---------------------------------------------------------------------------
+// Inputs of one face/cell (load), then the reference's expressions on them
+// (one method per array); a thread issues the loads of all its cells before
+// computing any.  FLUX: the inputs of Fx, Fy; DIFF: those of Gamma and the
+// velocity diffusion coefficients (the thermal ones need neither set alone).
+template <typename R>
+struct FluxIn {
+  bool in, m00, m10, m01, m11;  // fluid: (i-1,j-1), (i,j-1), (i-1,j), (i,j)
+  unsigned ij;
+  R hxi, hxw, hyj, hys;  // hx[i], hx[i-1], hy[j], hy[j-1] (clamped or wrapped as the reference's)
+  R ru, u, rv, v;
+  R s00, s10, s01, s11;      // sqrt_T at the four cells
+  R twu10, twu11, twv01, twv11;  // Temper_of_fluid_on_the_wall_u[ij_1], [ij], _v[i_1j], [ij]
+  template <bool FLUX = true, bool DIFF = true>
+  __device__ __forceinline__ void load(const KGridT<R> &g, const KFieldsT<R> &fl, int i, int j) {
+    in = i < g.Nx && j < g.Ny;
+    if (!in) return;
+    const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny, g.px);
+    const uint8_t *m = g.m;
+    const R *sT = FLD(SQRT_T), *Twu = FLD(TW_U), *Twv = FLD(TW_V);
+    ij = n.ij;
+    const unsigned jm = (unsigned)j - 1u * (0 < j);
+    hxi = g.hx[i];
+    hxw = g.hx[n.i_1];
+    hyj = g.hy[j];
+    hys = g.hy[jm];
+    if (FLUX) {
+      ru = FLD(RHO_U)[ij];
+      u = FLD(U)[ij];
+      rv = FLD(RHO_V)[ij];
+      v = FLD(V)[ij];
+    }
+    m00 = DIFF ? m[n.i_1j_1] : false;
+    m10 = m[n.ij_1];
+    m01 = m[n.i_1j];
+    m11 = m[ij];
+    s00 = DIFF ? sT[n.i_1j_1] : R(0);
+    s10 = sT[n.ij_1];
+    s01 = sT[n.i_1j];
+    s11 = sT[ij];
+    twu10 = DIFF ? Twu[n.ij_1] : R(0);
+    twu11 = Twu[ij];
+    twv01 = DIFF ? Twv[n.i_1j] : R(0);
+    twv11 = Twv[ij];
+  }
+  __device__ __forceinline__ R fx() const { return hyj * ru * u; }
+  __device__ __forceinline__ R fy() const { return hxi * rv * v; }
+  // Gamma_yf, Gamma_xf; false where all four cells are body and the
+  // reference leaves them untouched (S:10477-10543), with Gy = Gx = 0.
+  __device__ __forceinline__ bool gamma(R &Gy, R &Gx) const {
+    Gy = R(0.0);
+    Gx = R(0.0);
+    if (m00 && m10 && m01 && m11) {
+      Gy = R(0.5) * (Li(s00, s01, hys, hyj) + Li(s10, s11, hys, hyj));
+      Gx = R(0.5) * (Li(s00, s10, hxw, hxi) + Li(s01, s11, hxw, hxi));
+      return true;
+    }
+    if (m00 || m10 || m01 || m11) {
+      Gy = R(0.5) * (R(m00 && m01) * Li(s00, s01, hys, hyj) + R(m00 != m01) * sqrt(twv01)
+                     + R(m10 && m11) * Li(s10, s11, hys, hyj) + R(m10 != m11) * sqrt(twv11)
+                     + R((!(m00 && m01)) || (!(m10 && m11))) * Li(sqrt(twu10), sqrt(twu11), hys, hyj));
+      Gx = R(0.5) * (R(m00 && m10) * Li(s00, s10, hxw, hxi) + R(m00 != m10) * sqrt(twu10)
+                     + R(m01 && m11) * Li(s01, s11, hxw, hxi) + R(m01 != m11) * sqrt(twu11)
+                     + R((!(m00 && m10)) || (!(m01 && m11))) * Li(sqrt(twv01), sqrt(twv11), hxw, hxi));
+      return true;
+    }
+    return false;
+  }
+  __device__ __forceinline__ R d_ux(const KGridT<R> &g) const { return g.cT * s01 * (hyj / hxw); }
+  __device__ __forceinline__ R d_uy(const KGridT<R> &g, R Gy) const {
+    return ((m11 && m01) || (m10 && m00)) ? g.cT * Gy * ((hxi + hxw) / (R(m11 * m01) * hyj + R(m10 * m00) * hys))
+                                          : R(0.0);
+  }
+  __device__ __forceinline__ R d_vx(const KGridT<R> &g, R Gx) const {
+    return ((m11 && m10) || (m01 && m00)) ? g.cT * Gx * ((hyj + hys) / (R(m11 * m10) * hxi + R(m01 * m00) * hxw))
+                                          : R(0.0);
+  }
+  __device__ __forceinline__ R d_vy(const KGridT<R> &g) const { return g.cT * s10 * (hxi / hys); }
+  __device__ __forceinline__ R dtx(const KGridT<R> &g) const {
+    return (m11 || m01) ? g.cT1 * (hyj / (R(0.5) * (R(m11) * hxi + R(m01) * hxw)))
+                              * (R((!m11) || (!m01)) * sqrt(twu11) + R(m11 * m01) * Hi(s01, s11, hxw, hxi))
+                        : R(0.0);
+  }
+  __device__ __forceinline__ R dty(const KGridT<R> &g) const {
+    return (m11 || m10) ? g.cT1 * (hxi / (R(0.5) * (R(m11) * hyj + R(m10) * hys)))
+                              * (R((!m11) || (!m10)) * sqrt(twv11) + R(m11 * m10) * Hi(s10, s11, hys, hyj))
+                        : R(0.0);
+  }
+  // Gamma and the velocity diffusion coefficients (needs DIFF)
+  __device__ __forceinline__ void apply_diff(const KGridT<R> &g, const KFieldsT<R> &fl) const {
+    if (!in) return;
+    R Gy, Gx;
+    if (gamma(Gy, Gx)) {
+      FLD(GAMMA_YF)[ij] = Gy;
+      FLD(GAMMA_XF)[ij] = Gx;
+    }
+    FLD(D_UX)[ij] = d_ux(g);
+    FLD(D_UY)[ij] = d_uy(g, Gy);
+    FLD(D_VX)[ij] = d_vx(g, Gx);
+    FLD(D_VY)[ij] = d_vy(g);
+  }
+  // fluxes and thermal diffusion coefficients (needs FLUX)
+  __device__ __forceinline__ void apply_flux(const KGridT<R> &g, const KFieldsT<R> &fl) const {
+    if (!in) return;
+    FLD(FX)[ij] = fx();
+    FLD(FY)[ij] = fy();
+    FLD(DTX)[ij] = dtx(g);
+    FLD(DTY)[ij] = dty(g);
+  }
+  __device__ __forceinline__ void apply(const KGridT<R> &g, const KFieldsT<R> &fl) const {
+    apply_flux(g, fl);
+    apply_diff(g, fl);
+  }
+};
+
+// Gamma and the velocity diffusion coefficients of face k as the flux and
+// diffusion pass writes them (the assembly's faces always have a fluid cell,
+// so the untouched case does not arise there).
+template <typename R>
+struct FaceDiff {
+  R Gy, Gx, Dux, Duy, Dvx, Dvy;
+};
+template <typename R>
+__device__ __forceinline__ FaceDiff<R> face_diff(const KGridT<R> &g, const KFieldsT<R> &fl, unsigned k) {
+  FluxIn<R> f;
+  f.template load<false, true>(g, fl, (int)(k % (unsigned)g.Nx), (int)(k / (unsigned)g.Nx));
+  FaceDiff<R> d;
+  f.gamma(d.Gy, d.Gx);
+  d.Dux = f.d_ux(g);
+  d.Duy = f.d_uy(g, d.Gy);
+  d.Dvx = f.d_vx(g, d.Gx);
+  d.Dvy = f.d_vy(g);
+  return d;
+}
+
+// ---------------------------------------------------------------------------
 // A2 + A3 + A4: coefficient assembly (S:3269-3756)
 // ---------------------------------------------------------------------------
+// The diffusion coefficients of the faces an assembled cell reads: Gamma_yf
+// and D_uy at ij and ij1, Gamma_xf and D_vx at ij and i1j, D_ux at ij and
+// i1j, D_vy at ij and ij1.
 template <typename R>
-__device__ __forceinline__ void cell_assemble_u(const KGridT<R> &g, const KFieldsT<R> &fl, const Nbr &n) {
+struct CellDiff {
+  R Gy0, Gy1, Gx0, Gx1, Dux0, Dux1, Duy0, Duy1, Dvx0, Dvx1, Dvy0, Dvy1;
+};
+
+template <typename R>
+__device__ __forceinline__ void cell_assemble_u(const KGridT<R> &g, const KFieldsT<R> &fl, const Nbr &n,
+                                                const CellDiff<R> &df) {
   const uint8_t *m = g.m;
   const R *hx = g.hx, *hy = g.hy;
   const R *Fx = FLD(FX), *Fy = FLD(FY), *u = FLD(U), *v = FLD(V), *sqrt_T = FLD(SQRT_T);
-  const R *Gamma_yf = FLD(GAMMA_YF), *D_ux = FLD(D_UX), *D_uy = FLD(D_UY);
   const R *rho_pr = FLD(RHO_PR), *u_pr = FLD(U_PR), *rho = FLD(RHO), *rho_u = FLD(RHO_U);
   const unsigned i = n.i, j = n.j, ij = n.ij, i_1j = n.i_1j, i1j = n.i1j, ij_1 = n.ij_1, ij1 = n.ij1;
   const unsigned i_1j_1 = n.i_1j_1, i_1j1 = n.i_1j1, i_2j = n.i_2j, i_1 = n.i_1;
@@ -107,15 +253,15 @@ __device__ __forceinline__ void cell_assemble_u(const KGridT<R> &g, const KField
   R F_xu2 = R(0.5) * (Fx[ij] + Fx[i1j]);
   R Scu = cT * (R(m[i1j]) * u[i1j] * sqrt_T[ij] * hy[j] / (hx[i] * R(3.0))
                      + R(m[i_2j]) * u[i_1j] * sqrt_T[i_1j] * hy[j] / (hx[i_1] * R(3.0))
-                     + Gamma_yf[ij1] * (v[ij1] * R(m[ij1]) - v[i_1j1] * R(m[i_1j1]))
-                     - Gamma_yf[ij] * (v[ij] * R(m[ij_1]) - v[i_1j] * R(m[i_1j_1]))
+                     + df.Gy1 * (v[ij1] * R(m[ij1]) - v[i_1j1] * R(m[i_1j1]))
+                     - df.Gy0 * (v[ij] * R(m[ij_1]) - v[i_1j] * R(m[i_1j_1]))
                      - sqrt_T[ij] * (v[ij1] * R(m[ij1]) - v[ij] * R(m[ij_1])) * R(2.0) / R(3.0)
                      + sqrt_T[i_1j] * (v[i_1j1] * R(m[i_1j1]) - v[i_1j] * R(m[i_1j_1])) * R(2.0) / R(3.0));
-  R Spu = -(D_ux[ij] + D_ux[i1j]) / R(3.0);
-  R au1 = maxi0(F_xu1) + D_ux[ij];
-  R au2 = maxi0(-F_xu2) + D_ux[i1j];
-  R au3 = R(0.5) * (maxi0(Fy[i_1j]) + maxi0(Fy[ij])) + D_uy[ij];
-  R au4 = R(0.5) * (maxi0(-Fy[i_1j1]) + maxi0(-Fy[ij1])) + D_uy[ij1];
+  R Spu = -(df.Dux0 + df.Dux1) / R(3.0);
+  R au1 = maxi0(F_xu1) + df.Dux0;
+  R au2 = maxi0(-F_xu2) + df.Dux1;
+  R au3 = R(0.5) * (maxi0(Fy[i_1j]) + maxi0(Fy[ij])) + df.Duy0;
+  R au4 = R(0.5) * (maxi0(-Fy[i_1j1]) + maxi0(-Fy[ij1])) + df.Duy1;
   R bu3 = (rho_pr[i_1j] * hx[i_1] + rho_pr[ij] * hx[i]) * u_pr[ij] * hy[j] * R(0.5) / ht + Scu;
   R au0 = au1 + au2 + au3 + au4 + R(0.5) * (-Fy[i_1j] - Fy[ij] + Fy[i_1j1] + Fy[ij1]) + F_xu2 - F_xu1 - Spu
                + (rho[i_1j] * hx[i_1] + rho[ij] * hx[i]) * hy[j] * R(0.5) / ht;
@@ -129,12 +275,12 @@ __device__ __forceinline__ void cell_assemble_u(const KGridT<R> &g, const KField
 }
 
 template <typename R>
-__device__ __forceinline__ void cell_assemble_v(const KGridT<R> &g, const KFieldsT<R> &fl, const Nbr &n) {
+__device__ __forceinline__ void cell_assemble_v(const KGridT<R> &g, const KFieldsT<R> &fl, const Nbr &n,
+                                                const CellDiff<R> &df) {
   const uint8_t *m = g.m;
   const unsigned Nx = (unsigned)g.Nx;
   const R *hx = g.hx, *hy = g.hy;
   const R *Fx = FLD(FX), *Fy = FLD(FY), *u = FLD(U), *v = FLD(V), *sqrt_T = FLD(SQRT_T);
-  const R *Gamma_xf = FLD(GAMMA_XF), *D_vx = FLD(D_VX), *D_vy = FLD(D_VY);
   const R *rho_pr = FLD(RHO_PR), *v_pr = FLD(V_PR), *rho = FLD(RHO), *rho_v = FLD(RHO_V);
   const unsigned i = n.i, j = n.j, ij = n.ij, i_1j = n.i_1j, i1j = n.i1j, ij_1 = n.ij_1, ij1 = n.ij1;
   const unsigned i_1j_1 = n.i_1j_1, i1j_1 = n.i1j_1;
@@ -146,16 +292,16 @@ __device__ __forceinline__ void cell_assemble_v(const KGridT<R> &g, const KField
   R F_yv4 = R(0.5) * (Fy[ij] + Fy[ij1]);
   R Scv = cT * (R(m[ij1]) * v[ij1] * sqrt_T[ij] * hx[i] / (hy[j] * R(3.0))
                      + R(m[jm2]) * v[ij_1] * sqrt_T[ij_1] * hx[i] / (hyjm * R(3.0))
-                     + Gamma_xf[i1j] * (u[i1j] * R(m[i1j]) - u[i1j_1] * R(m[i1j_1]))
-                     - Gamma_xf[ij] * (u[ij] * R(m[i_1j]) - u[ij_1] * R(m[i_1j_1]))
+                     + df.Gx1 * (u[i1j] * R(m[i1j]) - u[i1j_1] * R(m[i1j_1]))
+                     - df.Gx0 * (u[ij] * R(m[i_1j]) - u[ij_1] * R(m[i_1j_1]))
                      - sqrt_T[ij] * (u[i1j] * R(m[i1j]) - u[ij] * R(m[i_1j])) * R(2.0) / R(3.0)
                      + sqrt_T[ij_1] * (u[i1j_1] * R(m[i1j_1]) - u[ij_1] * R(m[i_1j_1])) * R(2.0) / R(3.0))
                - g.c_Fr_v * (rho[ij_1] * hx[i] * R(0.5) * hyjm + rho[ij] * hx[i] * R(0.5) * hy[j]);
-  R Spv = -(D_vy[ij] + D_vy[ij1]) / R(3.0);
-  R av1 = R(0.5) * (maxi0(Fx[ij]) + maxi0(Fx[ij_1])) + D_vx[ij];
-  R av2 = R(0.5) * (maxi0(-Fx[i1j]) + maxi0(-Fx[i1j_1])) + D_vx[i1j];
-  R av3 = maxi0(F_yv3) + D_vy[ij];
-  R av4 = maxi0(-F_yv4) + D_vy[ij1];
+  R Spv = -(df.Dvy0 + df.Dvy1) / R(3.0);
+  R av1 = R(0.5) * (maxi0(Fx[ij]) + maxi0(Fx[ij_1])) + df.Dvx0;
+  R av2 = R(0.5) * (maxi0(-Fx[i1j]) + maxi0(-Fx[i1j_1])) + df.Dvx1;
+  R av3 = maxi0(F_yv3) + df.Dvy0;
+  R av4 = maxi0(-F_yv4) + df.Dvy1;
   R bv3 = (rho_pr[ij_1] * hy[j - 1] + rho_pr[ij] * hy[j]) * v_pr[ij] * hx[i] * R(0.5) / ht + Scv;
   R av0 = av1 + av2 + av3 + av4 + R(0.5) * (Fx[i1j] - Fx[ij] + Fx[i1j_1] - Fx[ij_1]) + F_yv4 - F_yv3 - Spv
                + (rho[ij_1] * hy[j - 1] + rho[ij] * hy[j]) * hx[i] * R(0.5) / ht;
@@ -195,13 +341,31 @@ __device__ __forceinline__ void cell_energy_source(const KGridT<R> &g, const KFi
                   + p_pr[ij] * hx[i] * hy[j];
 }
 
-template <typename R>
+// INL: the diffusion coefficients are formed here from sqrt_T, the wall
+// temperatures and the mask, with the flux and diffusion pass's own code,
+// instead of read from the arrays that pass writes; the values are the
+// same as long as those inputs have not changed since that pass.
+template <typename R, bool INL = false>
 __device__ __forceinline__ void assemble_cell(const KGridT<R> &g, const KFieldsT<R> &fl, int i, int j) {
   if (i >= g.Nx || j >= g.Ny) return;
   const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny, g.px);
   const uint8_t *m = g.m;
-  if (in_box(i, j, g.Nx_u0, g.Nx_u1, g.Ny_u0, g.Ny_u1) && m[n.i_1j] && m[n.ij]) cell_assemble_u(g, fl, n);
-  if (in_box(i, j, g.Nx_v0, g.Nx_v1, g.Ny_v0, g.Ny_v1) && m[n.ij_1] && m[n.ij]) cell_assemble_v(g, fl, n);
+  const bool du = in_box(i, j, g.Nx_u0, g.Nx_u1, g.Ny_u0, g.Ny_u1) && m[n.i_1j] && m[n.ij];
+  const bool dv = in_box(i, j, g.Nx_v0, g.Nx_v1, g.Ny_v0, g.Ny_v1) && m[n.ij_1] && m[n.ij];
+  if (du || dv) {
+    CellDiff<R> df;
+    if (INL) {
+      const FaceDiff<R> f0 = face_diff(g, fl, n.ij), f1 = face_diff(g, fl, n.i1j), f2 = face_diff(g, fl, n.ij1);
+      df = CellDiff<R>{f0.Gy, f2.Gy, f0.Gx, f1.Gx, f0.Dux, f1.Dux, f0.Duy, f2.Duy, f0.Dvx, f1.Dvx, f0.Dvy, f2.Dvy};
+    } else {
+      const R *Gyf = FLD(GAMMA_YF), *Gxf = FLD(GAMMA_XF), *Dux = FLD(D_UX), *Duy = FLD(D_UY), *Dvx = FLD(D_VX),
+              *Dvy = FLD(D_VY);
+      df = CellDiff<R>{Gyf[n.ij], Gyf[n.ij1], Gxf[n.ij], Gxf[n.i1j], Dux[n.ij], Dux[n.i1j],
+                       Duy[n.ij], Duy[n.ij1], Dvx[n.ij], Dvx[n.i1j], Dvy[n.ij], Dvy[n.ij1]};
+    }
+    if (du) cell_assemble_u(g, fl, n, df);
+    if (dv) cell_assemble_v(g, fl, n, df);
+  }
   if (in_box(i, j, g.Nx_o0, g.Nx_o1, g.Ny_o0, g.Ny_o1) && m[n.ij]) cell_energy_source(g, fl, n);
 }
 
@@ -210,7 +374,56 @@ __global__ void __launch_bounds__(BX *BY) k_assemble(KGridT<R> g, KFieldsT<R> fl
   if (gated_out(gate)) return;
   int i0, j0;
   tile_origin<BX, BY>(g.Nx, i0, j0);
-  assemble_cell(g, fl, i0 + (int)threadIdx.x, j0 + (int)threadIdx.y);
+  assemble_cell<R, false>(g, fl, i0 + (int)threadIdx.x, j0 + (int)threadIdx.y);
+}
+
+// The assembly with the diffusion coefficients formed in the workgroup: each
+// face of the tile once, plus the faces its cells read beyond it (column BX:
+// the i1j face of each row's last cell, row BY: the ij1 face of each
+// column's last cell, as the clamped or wrapped neighbour indices give
+// them), kept in LDS; then every cell assembles from them.
+template <typename R>
+__global__ void __launch_bounds__(BX *BY) k_assemble_inl(KGridT<R> g, KFieldsT<R> fl, Gate gate) {
+  if (gated_out(gate)) return;
+  __shared__ R Lgy[BY + 1][BX + 1], Lgx[BY + 1][BX + 1], Lux[BY + 1][BX + 1], Luy[BY + 1][BX + 1],
+      Lvx[BY + 1][BX + 1], Lvy[BY + 1][BX + 1];
+  int i0, j0;
+  tile_origin<BX, BY>(g.Nx, i0, j0);
+  const int Nx = g.Nx, Ny = g.Ny, tx = (int)threadIdx.x, ty = (int)threadIdx.y;
+  const int i = i0 + tx, j = j0 + ty;
+  const int il = min(i0 + BX, Nx) - 1, jl = min(j0 + BY, Ny) - 1;  // last cell of the tile in x, y
+  auto put = [&](int y, int x, unsigned k) {
+    const FaceDiff<R> d = face_diff(g, fl, k);
+    Lgy[y][x] = d.Gy;
+    Lgx[y][x] = d.Gx;
+    Lux[y][x] = d.Dux;
+    Luy[y][x] = d.Duy;
+    Lvx[y][x] = d.Dvx;
+    Lvy[y][x] = d.Dvy;
+  };
+  const bool in = i < Nx && j < Ny;
+  if (in) put(ty, tx, (unsigned)i + (unsigned)j * (unsigned)Nx);
+  const int t = tx + ty * BX;
+  if (t < BX) {  // north: the ij1 face of column t's last cell
+    if (i0 + t < Nx) put(BY, t, Nbr((unsigned)(i0 + t), (unsigned)jl, (unsigned)Nx, (unsigned)Ny, g.px).ij1);
+  } else if (t < BX + BY) {  // east: the i1j face of row t - BX's last cell
+    const int r = t - BX;
+    if (j0 + r < Ny) put(r, BX, Nbr((unsigned)il, (unsigned)(j0 + r), (unsigned)Nx, (unsigned)Ny, g.px).i1j);
+  }
+  __syncthreads();
+  if (!in) return;
+  const Nbr n((unsigned)i, (unsigned)j, (unsigned)Nx, (unsigned)Ny, g.px);
+  const uint8_t *m = g.m;
+  const bool du = in_box(i, j, g.Nx_u0, g.Nx_u1, g.Ny_u0, g.Ny_u1) && m[n.i_1j] && m[n.ij];
+  const bool dv = in_box(i, j, g.Nx_v0, g.Nx_v1, g.Ny_v0, g.Ny_v1) && m[n.ij_1] && m[n.ij];
+  if (du || dv) {
+    const int ex = (i == il) ? BX : tx + 1, ny = (j == jl) ? BY : ty + 1;
+    const CellDiff<R> df{Lgy[ty][tx], Lgy[ny][tx], Lgx[ty][tx], Lgx[ty][ex], Lux[ty][tx], Lux[ty][ex],
+                         Luy[ty][tx], Luy[ny][tx], Lvx[ty][tx], Lvx[ty][ex], Lvy[ty][tx], Lvy[ny][tx]};
+    if (du) cell_assemble_u(g, fl, n, df);
+    if (dv) cell_assemble_v(g, fl, n, df);
+  }
+  if (in_box(i, j, g.Nx_o0, g.Nx_o1, g.Ny_o0, g.Ny_o1) && m[n.ij]) cell_energy_source(g, fl, n);
 }
 
 // ---------------------------------------------------------------------------
@@ -1116,84 +1329,6 @@ __global__ void __launch_bounds__(EDGE_THREADS) k_edges(KGridT<R> g, KFieldsT<R>
 // A10 + A11: fluxes (S:10440-10454) and diffusion coefficients
 // (S:10457-10618) over all cells.
 // ---------------------------------------------------------------------------
-// Inputs of one face/cell (load), then the reference's expressions on them
-// (apply); a thread issues the loads of all its cells before computing any.
-template <typename R>
-struct FluxIn {
-  bool in, m00, m10, m01, m11;  // fluid: (i-1,j-1), (i,j-1), (i-1,j), (i,j)
-  unsigned ij;
-  R hxi, hxw, hyj, hys;  // hx[i], hx[i-1], hy[j], hy[j-1] (clamped or wrapped as the reference's)
-  R ru, u, rv, v;
-  R s00, s10, s01, s11;      // sqrt_T at the four cells
-  R twu10, twu11, twv01, twv11;  // Temper_of_fluid_on_the_wall_u[ij_1], [ij], _v[i_1j], [ij]
-  __device__ __forceinline__ void load(const KGridT<R> &g, const KFieldsT<R> &fl, int i, int j) {
-    in = i < g.Nx && j < g.Ny;
-    if (!in) return;
-    const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny, g.px);
-    const uint8_t *m = g.m;
-    const R *sT = FLD(SQRT_T), *Twu = FLD(TW_U), *Twv = FLD(TW_V);
-    ij = n.ij;
-    const unsigned jm = (unsigned)j - 1u * (0 < j);
-    hxi = g.hx[i];
-    hxw = g.hx[n.i_1];
-    hyj = g.hy[j];
-    hys = g.hy[jm];
-    ru = FLD(RHO_U)[ij];
-    u = FLD(U)[ij];
-    rv = FLD(RHO_V)[ij];
-    v = FLD(V)[ij];
-    m00 = m[n.i_1j_1];
-    m10 = m[n.ij_1];
-    m01 = m[n.i_1j];
-    m11 = m[ij];
-    s00 = sT[n.i_1j_1];
-    s10 = sT[n.ij_1];
-    s01 = sT[n.i_1j];
-    s11 = sT[ij];
-    twu10 = Twu[n.ij_1];
-    twu11 = Twu[ij];
-    twv01 = Twv[n.i_1j];
-    twv11 = Twv[ij];
-  }
-  __device__ __forceinline__ void apply(const KGridT<R> &g, const KFieldsT<R> &fl) const {
-    if (!in) return;
-    const R cT = g.cT, cT1 = g.cT1;
-    FLD(FX)[ij] = hyj * ru * u;
-    FLD(FY)[ij] = hxi * rv * v;
-    // All four cells body: the reference leaves Gamma untouched (S:10477-10543).
-    R Gy = R(0.0), Gx = R(0.0);
-    if (m00 && m10 && m01 && m11) {
-      Gy = R(0.5) * (Li(s00, s01, hys, hyj) + Li(s10, s11, hys, hyj));
-      Gx = R(0.5) * (Li(s00, s10, hxw, hxi) + Li(s01, s11, hxw, hxi));
-      FLD(GAMMA_YF)[ij] = Gy;
-      FLD(GAMMA_XF)[ij] = Gx;
-    } else if (m00 || m10 || m01 || m11) {
-      Gy = R(0.5) * (R(m00 && m01) * Li(s00, s01, hys, hyj) + R(m00 != m01) * sqrt(twv01)
-                     + R(m10 && m11) * Li(s10, s11, hys, hyj) + R(m10 != m11) * sqrt(twv11)
-                     + R((!(m00 && m01)) || (!(m10 && m11))) * Li(sqrt(twu10), sqrt(twu11), hys, hyj));
-      Gx = R(0.5) * (R(m00 && m10) * Li(s00, s10, hxw, hxi) + R(m00 != m10) * sqrt(twu10)
-                     + R(m01 && m11) * Li(s01, s11, hxw, hxi) + R(m01 != m11) * sqrt(twu11)
-                     + R((!(m00 && m10)) || (!(m01 && m11))) * Li(sqrt(twv01), sqrt(twv11), hxw, hxi));
-      FLD(GAMMA_YF)[ij] = Gy;
-      FLD(GAMMA_XF)[ij] = Gx;
-    }
-    FLD(D_UX)[ij] = cT * s01 * (hyj / hxw);
-    FLD(D_UY)[ij] = ((m11 && m01) || (m10 && m00))
-                        ? cT * Gy * ((hxi + hxw) / (R(m11 * m01) * hyj + R(m10 * m00) * hys))
-                        : R(0.0);
-    FLD(D_VX)[ij] = ((m11 && m10) || (m01 && m00))
-                        ? cT * Gx * ((hyj + hys) / (R(m11 * m10) * hxi + R(m01 * m00) * hxw))
-                        : R(0.0);
-    FLD(D_VY)[ij] = cT * s10 * (hxi / hys);
-    FLD(DTX)[ij] = (m11 || m01) ? cT1 * (hyj / (R(0.5) * (R(m11) * hxi + R(m01) * hxw)))
-                                      * (R((!m11) || (!m01)) * sqrt(twu11) + R(m11 * m01) * Hi(s01, s11, hxw, hxi))
-                                : R(0.0);
-    FLD(DTY)[ij] = (m11 || m10) ? cT1 * (hxi / (R(0.5) * (R(m11) * hyj + R(m10) * hys)))
-                                      * (R((!m11) || (!m10)) * sqrt(twv11) + R(m11 * m10) * Hi(s10, s11, hys, hyj))
-                                : R(0.0);
-  }
-};
-
 template <typename R>
 __device__ __forceinline__ void flux_diff_cell(const KGridT<R> &g, const KFieldsT<R> &fl, int i, int j) {
   FluxIn<R> c;
@@ -1204,16 +1339,23 @@ __device__ __forceinline__ void flux_diff_cell(const KGridT<R> &g, const KFields
 // FLUX_ROWS cells per thread, rows BY apart, all loads issued first.
 constexpr int FLUX_ROWS = 2;
 
-template <typename R>
+// PARTS: FLUX_PART (Fx, Fy, DTx, DTy) and/or DIFF_PART (Gamma, D_ux, D_uy,
+// D_vx, D_vy).
+template <typename R, int PARTS>
 __global__ void __launch_bounds__(BX *BY) k_flux_diff(KGridT<R> g, KFieldsT<R> fl, Gate gate) {
   if (gated_out(gate)) return;
   int i0, j0;
   tile_origin<BX, BY * FLUX_ROWS>(g.Nx, i0, j0);
   FluxIn<R> c[FLUX_ROWS];
 #pragma unroll
-  for (int q = 0; q < FLUX_ROWS; q++) c[q].load(g, fl, i0 + (int)threadIdx.x, j0 + (int)threadIdx.y + q * BY);
+  for (int q = 0; q < FLUX_ROWS; q++)
+    c[q].template load<(PARTS & FLUX_PART) != 0, (PARTS & DIFF_PART) != 0>(g, fl, i0 + (int)threadIdx.x,
+                                                                          j0 + (int)threadIdx.y + q * BY);
 #pragma unroll
-  for (int q = 0; q < FLUX_ROWS; q++) c[q].apply(g, fl);
+  for (int q = 0; q < FLUX_ROWS; q++) {
+    if (PARTS & FLUX_PART) c[q].apply_flux(g, fl);
+    if (PARTS & DIFF_PART) c[q].apply_diff(g, fl);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1937,8 +2079,9 @@ static inline dim3 grid_tiles(const KGridT<R> &g) {
 }
 
 template <typename R>
-hipError_t launch_assemble(const KGridT<R> &g, const KFieldsT<R> &f, hipStream_t s, Gate gate) {
-  hipLaunchKernelGGL(k_assemble<R>, (grid_tiles<BX, BY>(g)), dim3(BX, BY), 0, s, g, f, gate);
+hipError_t launch_assemble(const KGridT<R> &g, const KFieldsT<R> &f, hipStream_t s, Gate gate, bool inline_diff) {
+  if (inline_diff) hipLaunchKernelGGL(k_assemble_inl<R>, (grid_tiles<BX, BY>(g)), dim3(BX, BY), 0, s, g, f, gate);
+  else hipLaunchKernelGGL(k_assemble<R>, (grid_tiles<BX, BY>(g)), dim3(BX, BY), 0, s, g, f, gate);
   return hipGetLastError();
 }
 template <typename R>
@@ -1981,13 +2124,21 @@ hipError_t launch_edges(const KGridT<R> &g, const KFieldsT<R> &f, int parts, dou
   return hipGetLastError();
 }
 template <typename R>
-hipError_t launch_flux_diff(const KGridT<R> &g, const KFieldsT<R> &f, hipStream_t s, Gate gate) {
-  hipLaunchKernelGGL(k_flux_diff<R>, (grid_tiles<BX, BY * FLUX_ROWS>(g)), dim3(BX, BY), 0, s, g, f, gate);
+hipError_t launch_flux_diff(const KGridT<R> &g, const KFieldsT<R> &f, hipStream_t s, Gate gate, int parts) {
+  if (parts == FLUX_PART)
+    hipLaunchKernelGGL((k_flux_diff<R, FLUX_PART>), (grid_tiles<BX, BY * FLUX_ROWS>(g)), dim3(BX, BY), 0, s, g, f,
+                       gate);
+  else if (parts == DIFF_PART)
+    hipLaunchKernelGGL((k_flux_diff<R, DIFF_PART>), (grid_tiles<BX, BY * FLUX_ROWS>(g)), dim3(BX, BY), 0, s, g, f,
+                       gate);
+  else
+    hipLaunchKernelGGL((k_flux_diff<R, FLUX_PART | DIFF_PART>), (grid_tiles<BX, BY * FLUX_ROWS>(g)), dim3(BX, BY), 0,
+                       s, g, f, gate);
   return hipGetLastError();
 }
 
 #define STS_INSTANTIATE(R)                                                                                     \
-  template hipError_t launch_assemble<R>(const KGridT<R> &, const KFieldsT<R> &, hipStream_t, Gate);           \
+  template hipError_t launch_assemble<R>(const KGridT<R> &, const KFieldsT<R> &, hipStream_t, Gate, bool);     \
   template hipError_t launch_sweep<R>(const KGridT<R> &, const KFieldsT<R> &, const R *, const R *, R *, R *,  \
                                       unsigned long long *, unsigned long long *, hipStream_t,                 \
                                       unsigned long long *, unsigned long long *, Gate);                       \
@@ -2000,7 +2151,7 @@ hipError_t launch_flux_diff(const KGridT<R> &g, const KFieldsT<R> &f, hipStream_
                                      const WallOp *, int);                                                    \
   template hipError_t launch_edges<R>(const KGridT<R> &, const KFieldsT<R> &, int, double *, hipStream_t, Gate, \
                                       LoopTest);                                                               \
-  template hipError_t launch_flux_diff<R>(const KGridT<R> &, const KFieldsT<R> &, hipStream_t, Gate);
+  template hipError_t launch_flux_diff<R>(const KGridT<R> &, const KFieldsT<R> &, hipStream_t, Gate, int);
 STS_INSTANTIATE(double)
 STS_INSTANTIATE(float)
 #undef STS_INSTANTIATE

@@ -5,8 +5,12 @@
 #include "sts_device.h"
 
 // Loop-2 kernels, instantiated for double and float.
+// inline_diff: form the diffusion coefficients from sqrt_T, the wall
+// temperatures and the mask instead of reading Gamma_* and D_* (equal while
+// those inputs are unchanged since the last flux and diffusion pass).
 template <typename R>
-hipError_t launch_assemble(const KGridT<R> &g, const KFieldsT<R> &f, hipStream_t s, Gate gate = Gate{nullptr, 0});
+hipError_t launch_assemble(const KGridT<R> &g, const KFieldsT<R> &f, hipStream_t s, Gate gate = Gate{nullptr, 0},
+                           bool inline_diff = false);
 template <typename R>
 hipError_t launch_sweep(const KGridT<R> &g, const KFieldsT<R> &f, const R *Po, const R *To, R *Pn, R *Tn,
                         unsigned long long *rp, unsigned long long *rT, hipStream_t s,
@@ -29,8 +33,10 @@ template <typename R>
 hipError_t launch_edges(const KGridT<R> &g, const KFieldsT<R> &f, int parts, double *probe, hipStream_t s,
                         Gate gate = Gate{nullptr, 0},
                         LoopTest lt = LoopTest{nullptr, nullptr, nullptr, 0, 0, 0, 0, 0, 0});
+// parts: FLUX_PART (Fx, Fy, DTx, DTy) and/or DIFF_PART (Gamma_*, D_*).
 template <typename R>
-hipError_t launch_flux_diff(const KGridT<R> &g, const KFieldsT<R> &f, hipStream_t s, Gate gate = Gate{nullptr, 0});
+hipError_t launch_flux_diff(const KGridT<R> &g, const KFieldsT<R> &f, hipStream_t s, Gate gate = Gate{nullptr, 0},
+                            int parts = FLUX_PART | DIFF_PART);
 
 // FP64-only kernels.
 hipError_t launch_properties(const KGrid &g, const KFields &f, hipStream_t s);
