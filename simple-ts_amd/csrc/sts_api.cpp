@@ -798,14 +798,15 @@ static AASel anderson_select(const sts_ctx *c, int k, int m, int every) {
 // The Anderson step after iteration k's edge conditions, gated on iteration
 // k + 1 running (the iteration that meets the test is the state the step
 // keeps): the iteration is stored; with at least one difference in use the
-// mixed start replaces u, v, p, T, with sqrt_T, rho and the face densities.
+// mixed start replaces u, v, p, T, with sqrt_T, rho, the face densities,
+// the fluxes and the thermal diffusion coefficients.
 // The wall and edge conditions are not applied again: every stored iterate
 // satisfies them, so the affine combination satisfies each condition that
 // is affine in the state (given values, zero-gradient copies, constant-Kn
 // slip and jump); the wall temperatures stay the newest iterate's, and the
 // rest (local-Kn slip and jump, the rho u copies) the next iteration
-// restores, whose own change the test judges.  The iteration's flux and
-// diffusion pass, which follows, then works on the mixed state.
+// restores, whose own change the test judges.  The assembly of the next
+// iteration forms Gamma and D from the mixed sqrt_T.
 static int enqueue_anderson(sts_ctx *c, const KFields &kf, int k, const AASel &sel) {
   const Gate gn{c->ctrl, k + 1};
   const AAFields af{{kf.f[STS_U], kf.f[STS_V], kf.f[STS_P], kf.f[STS_T]}};
@@ -902,8 +903,10 @@ static int enqueue_outer(sts_ctx *c, const Loop2State<R> &st, int k, int n, Loop
       int r;
       if ((r = enqueue_anderson(c, kf, k, sel))) return r;
     }
+  // an Anderson mix forms the fluxes itself; the pass then runs only when
+  // iteration k is the last (the mix, gated on k + 1, does not run)
   t = timed_begin(c, st.kid + STS_K_FLUX_DIFF);
-  HIP_OK(launch_flux_diff(g, kf, c->stream, gate, FLUX_PART));
+  HIP_OK(launch_flux_diff(g, kf, c->stream, aa && sel.n > 0 ? Gate{c->ctrl, k, 1} : gate, FLUX_PART));
   timed_end(c, t);
   return STS_OK;
 }

@@ -777,7 +777,8 @@ struct CorrectIn {
     Ts = T[ij_1];
   }
   // S:4170-4305 (u, v), S:4323-4330 (sqrt_T, rho), S:10382-10437 (face densities)
-  __device__ __forceinline__ void apply(const KFieldsT<R> &fl, unsigned long long (&r)[2]) const {
+  // rr (optional): the face densities rho_u, rho_v written
+  __device__ __forceinline__ void apply(const KFieldsT<R> &fl, unsigned long long (&r)[2], R *rr = nullptr) const {
     if (!in) return;
     R u = uf, v = vf;
     if (cu) {
@@ -807,6 +808,10 @@ struct CorrectIn {
     else rv = R(0.0);
     FLD(RHO_U)[ij] = ru;
     FLD(RHO_V)[ij] = rv;
+    if (rr) {
+      rr[0] = ru;
+      rr[1] = rv;
+    }
   }
 };
 
@@ -1702,8 +1707,10 @@ __device__ __forceinline__ void aa_mixed(const AABuf &ab, const AASel &sel, cons
 // The next start x into the state arrays (and D = x - gp), and, from it,
 // what the correction pass derives without correcting: sqrt_T, rho and the
 // face densities (CorrectIn::apply on the mixed p, T of the cell and of its
-// west and south neighbours).  BX x BY tiles; the mixed p, T of the tile and
-// of its west column and south row (formed by the first threads) meet in LDS.
+// west and south neighbours), then what the loop's flux pass writes (Fx, Fy,
+// DTx, DTy; FluxIn::apply_flux).  BX x BY tiles; the mixed p, T of the tile
+// and of its west column and south row (formed by the first threads) meet in
+// LDS.
 template <int NM>
 __global__ void __launch_bounds__(BX *BY) k_aa_mix(KGrid g, KFields fl, AAFields s, AABuf ab, AASel sel,
                                                    Gate gate) {
@@ -1769,7 +1776,30 @@ __global__ void __launch_bounds__(BX *BY) k_aa_mix(KGrid g, KFields fl, AAFields
   kf.f[STS_P] = s.x[2];
   kf.f[STS_T] = s.x[3];
   unsigned long long rr[2] = {0ull, 0ull};
-  ci.apply(kf, rr);
+  double rho_uv[2];
+  ci.apply(kf, rr, rho_uv);
+  // the loop's flux pass on the mixed state (fluxes, thermal coefficients):
+  // its inputs at this cell and the west and south sqrt_T are all here
+  FluxIn<double> fi;
+  fi.in = true;
+  fi.ij = nb.ij;
+  fi.hxi = g.hx[i];
+  fi.hxw = g.hx[nb.i_1];
+  fi.hyj = g.hy[j];
+  fi.hys = g.hy[j - 1 * (0 < j)];
+  fi.ru = rho_uv[0];
+  fi.u = x[0];
+  fi.rv = rho_uv[1];
+  fi.v = x[1];
+  fi.m10 = ci.ms;
+  fi.m01 = ci.mw;
+  fi.m11 = ci.mc;
+  fi.s10 = sqrt(ci.Ts);
+  fi.s01 = sqrt(ci.Tw);
+  fi.s11 = sqrt(ci.Tc);
+  fi.twu11 = FLD(TW_U)[nb.ij];
+  fi.twv11 = FLD(TW_V)[nb.ij];
+  fi.apply_flux(g, kf);
 }
 
 __global__ void __launch_bounds__(BX *BY) k_set_inflow(KGrid g, KFields fl, double u_in) {
