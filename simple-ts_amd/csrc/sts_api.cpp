@@ -78,6 +78,7 @@ struct sts_ctx {
   long long fluid_cells = 0;
   WallOp *wall_ops = nullptr;  // the wall pass's writes, planned once from the mask (k_wall_ops)
   int n_wall_ops = 0;
+  int n_wall_var = 0;  // the first n_wall_var of them depend on the state; the rest write constants
   // drag coefficients: host copy of vol_inf_fb, per-body cell lists, buffers
   std::vector<uint32_t> h_vol;
   std::vector<unsigned *> drag_cells;
@@ -340,8 +341,12 @@ static int alloc_fp32(sts_ctx *c, const sts_grid *gi) {
 
 // The wall pass's writes of the listed cells, planned on the device from the
 // mask and body table and kept in a compact list (at least one slot, so that
-// an empty list still selects the list kernel).
-static int plan_wall(sts_ctx *c, const std::vector<unsigned> &cells) {
+// an empty list still selects the list kernel): first the writes whose value
+// depends on the state (slip velocities, wall temperatures of jump or
+// zero-gradient bodies), then those that write a body's constant
+// temperature.  The writes have distinct destinations, so their order is
+// free; a loop applies the constant ones in its first outer iteration only.
+static int plan_wall(sts_ctx *c, const std::vector<unsigned> &cells, const sts_body *bodies) {
   const int n = (int)cells.size();
   std::vector<WallOp> all((size_t)4 * n), ops;
   if (n > 0) {
@@ -357,8 +362,16 @@ static int plan_wall(sts_ctx *c, const std::vector<unsigned> &cells) {
     (void)hipFree(d_cells);
     (void)hipFree(d_all);
     HIP_OK(e);
-    for (const WallOp &o : all)
-      if ((o.kb & 7u) != WOP_NONE) ops.push_back(o);
+    std::vector<WallOp> fixed;
+    for (const WallOp &o : all) {
+      const unsigned kind = o.kb & 7u;
+      if (kind == WOP_NONE) continue;
+      const sts_body &b = bodies[o.kb >> 3];
+      const bool constant = (kind == WOP_TWU || kind == WOP_TWV) && !b.is_jump && b.dTdn_on_wall_0 == 0.0;
+      (constant ? fixed : ops).push_back(o);
+    }
+    c->n_wall_var = (int)ops.size();
+    ops.insert(ops.end(), fixed.begin(), fixed.end());
   }
   c->n_wall_ops = (int)ops.size();
   HIP_OK(hipMalloc(&c->wall_ops, sizeof(WallOp) * (ops.empty() ? 1 : ops.size())));
@@ -453,7 +466,7 @@ extern "C" int sts_create(const sts_grid *gi, int precision, sts_ctx **out) {
   g.kn_local = gi->kn_local;
   g.bc = bc_bits(gi);
   g.px = px ? 1 : 0;
-  if ((arc = plan_wall(c, wl))) {
+  if ((arc = plan_wall(c, wl, gi->bodies))) {
     sts_destroy(c);
     return arc;
   }
@@ -890,7 +903,7 @@ static int enqueue_outer(sts_ctx *c, const Loop2State<R> &st, int k, int n, Loop
   timed_end(c, t);
   t = timed_begin(c, st.kid + STS_K_WALL);
   HIP_OK(launch_wall(g, kf, STS_BC_SLIP_U | STS_BC_SLIP_V | STS_BC_JUMP, c->stream, gate, c->wall_ops,
-                     c->n_wall_ops));
+                     k == 0 ? c->n_wall_ops : c->n_wall_var));
   timed_end(c, t);
   lt.lred = lr;
   lt.log_k += 6 * (size_t)k;
