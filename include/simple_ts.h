@@ -216,7 +216,8 @@ typedef struct sts_mixed_control {
   int guard;                          /* > 0 (with extrapolate): a predicted step whose FP64 phase has not
                                          met the reference's test within as many outer iterations as
                                          steps from x^n recently took (their running mean) is restarted
-                                         from x^n, FP64 only, and the degree in use drops by one (it
+                                         from x^n, FP64 only (accelerated if anderson asks for it, as
+                                         every step from x^n), and the degree in use drops by one (it
                                          climbs back after 4 accepted predicted steps); an abandoned
                                          degree-1 attempt pauses prediction for 1, 2, 4, ... steps (at
                                          most guard).  0: every step predicted at the full degree */

@@ -835,10 +835,12 @@ static int ensure_anderson(sts_ctx *c, int m) {
   const size_t n4 = 4 * (size_t)c->Na;
   if (!c->aa.gp) {
     c->aa.nb = aa_store_blocks(c->Na);
+    int slots = c->aa.nb;  // partial sums: enough for the batched steps' tiles too
+    for (int v = 0; v < NVARIANT; v++) slots = std::max(slots, aa_case_tiles(c->Nx, c->Ny, v));
     HIP_OK(hipMalloc(&c->aa.gp, sizeof(double) * n4));
     HIP_OK(hipMalloc(&c->aa.D, sizeof(float) * n4));
     HIP_OK(hipMalloc(&c->aa.fp, sizeof(float) * n4));
-    HIP_OK(hipMalloc(&c->aa.part, sizeof(double) * 2 * MAX_AA * (size_t)c->aa.nb));
+    HIP_OK(hipMalloc(&c->aa.part, sizeof(double) * 2 * MAX_AA * (size_t)slots));
     HIP_OK(hipMalloc(&c->aa.gram, sizeof(double) * MAX_AA * MAX_AA));
     HIP_OK(hipMalloc(&c->aa.gam, sizeof(double) * MAX_AA));
   }
@@ -1192,10 +1194,12 @@ extern "C" int sts_time_step_mixed(sts_ctx *c, const sts_loop_control *lc, const
     } else if (c->pred_pause > 0) {
       c->pred_pause--;
     }
-    if (plain) {
+    if (plain) {  // from x^n: FP64 only, with the acceleration if asked for
       c->pstat[4] += !predict;
       n32 = 0;
-      rc = run_phases(c, lc, nullptr, &n32, &n64);
+      sts_mixed_control m64 = *mc;
+      m64.N_I32 = 0;
+      rc = run_phases(c, lc, &m64, &n32, &n64);
       if (!rc) c->plain_mean = c->plain_mean < 0.0 ? n64 : 0.5 * (c->plain_mean + n64);
     }
   }
@@ -1423,6 +1427,12 @@ static int upload_cases(sts_batch *b, BatchCase<R> *dst, int N_I, double tol_uv,
     x.wops = c->wall_ops;
     x.nwops = c->n_wall_ops;
     x.skip = skip ? (*skip)[i] : 0;
+    x.aa = c->aa;
+    x.aa.nb = aa_case_tiles(c->Nx, c->Ny, b->cell_v);
+    for (int j = 0; j < MAX_AA; j++) {
+      x.aa_dG[j] = c->aa_dG[j];
+      x.aa_dF[j] = c->aa_dF[j];
+    }
   }
   HIP_OK(hipMemcpyAsync(dst, h.data(), sizeof(BatchCase<R>) * h.size(), hipMemcpyHostToDevice, b->stream));
   return STS_OK;
@@ -1430,13 +1440,16 @@ static int upload_cases(sts_batch *b, BatchCase<R> *dst, int N_I, double tol_uv,
 
 // Loop 2 of every case in shared launches; iters[i] gets case i's count.
 template <typename R>
-static int run_batch_loop2(sts_batch *b, BatchCase<R> *C, int n_pc, int N_I, int batch, std::vector<int> &iters) {
+static int run_batch_loop2(sts_batch *b, BatchCase<R> *C, int n_pc, int N_I, int batch, std::vector<int> &iters,
+                           int aa_m = 0, int aa_every = 1) {
   const int n = (int)b->cs.size(), passes = sweep_passes(b->cs[0], n_pc);
   hipStream_t s = b->stream;
   const int cv = b->cell_v, sv = b->sweep_v;
   BatchTile *ct = b->cell_t[cv], *st = b->sweep_t[sv];
   const int nct = b->n_cell[cv], nst = b->n_sweep[sv];
   HIP_OK(launch_batch<R>(BK_LOOP_OPEN, 0, C, n, nullptr, 0, 0, 0, 0, 0, s));
+  if constexpr (sizeof(R) == sizeof(double))
+    if (aa_m > 0) HIP_OK(launch_batch_aa(BKA_BEGIN, cv, C, n, ct, nct, 0, 0, AABatchSel{}, s));
   int k = 0;
   while (true) {
     const int end = std::min(N_I, k + batch);
@@ -1458,6 +1471,18 @@ static int run_batch_loop2(sts_batch *b, BatchCase<R> *C, int n_pc, int N_I, int
                              STS_BC_SLIP_U | STS_BC_SLIP_V | STS_BC_JUMP, 0, s));
       HIP_OK(launch_batch<R>(BK_EDGES, 0, C, n, nullptr, 0, k, par,
                              STS_BC_VELOCITIES | STS_BC_TEMPERATURE | STS_BC_PRESSURE, 0, s));
+      if constexpr (sizeof(R) == sizeof(double))
+        if (aa_m > 0 && (k + 1) % aa_every == 0) {  // as enqueue_anderson, for every case at once
+          const int K = (k + 1) / aa_every - 1;
+          AABatchSel bs{};
+          bs.n = std::min(K, aa_m);
+          for (int j = 0; j < bs.n; j++) bs.slot[j] = (K - bs.n + j) % aa_m;
+          HIP_OK(launch_batch_aa(BKA_STORE, cv, C, n, ct, nct, k, par, bs, s));
+          if (bs.n > 0) {
+            HIP_OK(launch_batch_aa(BKA_SOLVE, cv, C, n, ct, nct, k, par, bs, s));
+            HIP_OK(launch_batch_aa(BKA_MIX, cv, C, n, ct, nct, k, par, bs, s));
+          }
+        }
       HIP_OK(launch_batch<R>(BK_FLUX_DIFF, cv, C, n, ct, nct, k, par, 0, 0, s));
     }
     HIP_OK(hipMemcpyAsync(b->h_ctrl, b->ctrl, sizeof(LoopCtrl) * n, hipMemcpyDeviceToHost, s));
@@ -1492,12 +1517,12 @@ static int run_batch_loop2(sts_batch *b, BatchCase<R> *C, int n_pc, int N_I, int
 extern "C" int sts_batch_time_step(sts_batch *b, const sts_loop_control *lc, const sts_mixed_control *mc,
                                    int *iters32, int *iters64) {
   if (!b) return fail(STS_ERR_ARG, "null batch");
-  if (mc && mc->anderson > 0) return fail(STS_ERR_UNSUPPORTED, "Anderson acceleration runs per context, not in a batch");
   int rc;
   const int n = (int)b->cs.size();
   for (sts_ctx *c : b->cs) {
     if ((rc = check_mixed(c, lc, mc))) return rc;
     if (c->fuse_sweeps != b->cs[0]->fuse_sweeps) return fail(STS_ERR_ARG, "cases differ in sweep fusion");
+    if (mc && mc->anderson > 0 && (rc = ensure_anderson(c, mc->anderson))) return rc;
     HIP_OK(hipStreamSynchronize(c->stream));
   }
   const int cap = lc->N_I + (mc ? mc->N_I32 : 0);
@@ -1562,7 +1587,9 @@ extern "C" int sts_batch_time_step(sts_batch *b, const sts_loop_control *lc, con
   if ((rc = upload_cases<double>(b, b->C64, lc->N_I, lc->tol_uv, lc->tol_p, lc->tol_T, 0, 0.0, 6 * (size_t)off, true)))
     return rc;
   const int b64 = lc->batch > 0 ? lc->batch : (b->last_iters64 > 2 ? b->last_iters64 - 1 : 1);
-  if ((rc = run_batch_loop2<double>(b, b->C64, lc->N_I_p_c, lc->N_I, b64, n64))) return rc;
+  if ((rc = run_batch_loop2<double>(b, b->C64, lc->N_I_p_c, lc->N_I, b64, n64, mc ? mc->anderson : 0,
+                                    mc ? std::max(1, mc->anderson_every) : 1)))
+    return rc;
   b->last_iters64 = *std::min_element(n64.begin(), n64.end());
   for (int i = 0; i < n; i++) {
     if (iters32) iters32[i] = n32[i];

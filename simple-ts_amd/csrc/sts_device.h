@@ -209,6 +209,17 @@ struct BatchCase {
   const WallOp *wops;  // the case's planned wall writes
   int nwops;
   int skip;  // the case sits this loop out (its loop control opens closed)
+  // Anderson acceleration (FP64 loops): the case's vectors (aa.nb = its
+  // tiles of the cell variant in use) and its ring of differences
+  AABuf aa;
+  float *aa_dG[MAX_AA], *aa_dF[MAX_AA];
+};
+
+// The differences in use in every case of a batched Anderson step (the cases
+// step together, so their rings rotate alike): count and ring slots.
+struct AABatchSel {
+  int n;
+  int slot[MAX_AA];
 };
 
 // Work item of a batched launch: tile (x, y) of case c.

@@ -1537,90 +1537,96 @@ __global__ void __launch_bounds__(AA_THREADS) k_aa_begin(AAFields s, AABuf ab, i
   store4(ab.D + 4 * (size_t)k, z);
 }
 
-// After an outer iteration: g = the state, x = gp + D its start, f = g - x.
-// First iteration of the loop (sel.n = 0): the next start is g (gp = g,
-// D = 0), fp = f.  Later: the newest differences dF = f - fp, dG = g - gp
-// go to sel.dF/dG[n-1], fp = f, gp = g, and the workgroup's partial sums
-// a_j = dF_j . dF_new and b_j = dF_j . f (every difference in use) go to
-// part[j nb + block] and part[(MAX_AA + j) nb + block].  NM >= sel.n bounds
-// the differences at compile time.
+// After an outer iteration: g = the state of cell k, x = gp + D its start,
+// f = g - x.  First iteration of the loop (sel.n = 0): the next start is g
+// (gp = g, D = 0), fp = f.  Later: the newest differences dF = f - fp,
+// dG = g - gp go to sel.dF/dG[n-1], fp = f, gp = g, and acc[j] +=
+// dF_j . dF_new, acc[NM + j] += dF_j . f for every difference in use.
+// NM >= sel.n bounds the differences at compile time.
+template <int NM>
+__device__ __forceinline__ void aa_store_cell(const AAFields &s, const AABuf &ab, const AASel &sel, int k,
+                                              double (&acc)[2 * NM]) {
+  const int n = sel.n;
+  const size_t e = 4 * (size_t)k;
+  double g[4], gp[4], D[4], fp[4], o[NM][4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) g[q] = s.x[q][k];
+  load4(ab.gp + e, gp);
+  load4(ab.D + e, D);
+  if (n > 0) load4(ab.fp + e, fp);
+#pragma unroll
+  for (int j = 0; j < NM; j++) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) o[j][q] = 0.0;
+    if (j < n - 1) load4(sel.dF[j] + e, o[j]);
+  }
+  double f[4], dF[4], dG[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) f[q] = g[q] - (gp[q] + D[q]);
+  if (n == 0) {
+    const double z[4] = {0.0, 0.0, 0.0, 0.0};
+    store4(ab.D + e, z);
+  } else {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      dF[q] = (double)(float)(f[q] - fp[q]);
+      dG[q] = g[q] - gp[q];
+    }
+#pragma unroll
+    for (int j = 0; j < NM; j++)
+      if (j == n - 1) {
+        store4(sel.dF[j] + e, dF);
+        store4(sel.dG[j] + e, dG);
+#pragma unroll
+        for (int q = 0; q < 4; q++) o[j][q] = dF[q];
+      }
+#pragma unroll
+    for (int j = 0; j < NM; j++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        acc[j] += o[j][q] * dF[q];
+        acc[NM + j] += o[j][q] * f[q];
+      }
+  }
+  store4(ab.gp + e, g);
+  store4(ab.fp + e, f);
+}
+
+// The workgroup's (NT threads) sums of acc into partial slot idx:
+// part[j nb + idx] (a_j) and part[(MAX_AA + j) nb + idx] (b_j).
+template <int NM, int NT>
+__device__ __forceinline__ void aa_store_sums(const double (&acc)[2 * NM], const AABuf &ab, int idx) {
+  __shared__ double sums[2 * NM];
+  block_sums<2 * NM, NT>(acc, sums);
+  __syncthreads();
+  const int j = (int)threadIdx.x;
+  if (j < 2 * NM) ab.part[(size_t)(j < NM ? j : MAX_AA + j - NM) * ab.nb + idx] = sums[j];
+}
+
 template <int NM>
 __global__ void __launch_bounds__(AA_THREADS) k_aa_store(AAFields s, AABuf ab, AASel sel, int Na, Gate gate) {
   if (gated_out(gate)) return;
-  const int n = sel.n;
   const int k = (int)(blockIdx.x * AA_THREADS + threadIdx.x);
   double acc[2 * NM];
 #pragma unroll
   for (int j = 0; j < 2 * NM; j++) acc[j] = 0.0;
-  if (k < Na) {
-    const size_t e = 4 * (size_t)k;
-    double g[4], gp[4], D[4], fp[4], o[NM][4];
-#pragma unroll
-    for (int q = 0; q < 4; q++) g[q] = s.x[q][k];
-    load4(ab.gp + e, gp);
-    load4(ab.D + e, D);
-    if (n > 0) load4(ab.fp + e, fp);
-#pragma unroll
-    for (int j = 0; j < NM; j++) {
-#pragma unroll
-      for (int q = 0; q < 4; q++) o[j][q] = 0.0;
-      if (j < n - 1) load4(sel.dF[j] + e, o[j]);
-    }
-    double f[4], dF[4], dG[4];
-#pragma unroll
-    for (int q = 0; q < 4; q++) f[q] = g[q] - (gp[q] + D[q]);
-    if (n == 0) {
-      const double z[4] = {0.0, 0.0, 0.0, 0.0};
-      store4(ab.D + e, z);
-    } else {
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        dF[q] = (double)(float)(f[q] - fp[q]);
-        dG[q] = g[q] - gp[q];
-      }
-#pragma unroll
-      for (int j = 0; j < NM; j++)
-        if (j == n - 1) {
-          store4(sel.dF[j] + e, dF);
-          store4(sel.dG[j] + e, dG);
-#pragma unroll
-          for (int q = 0; q < 4; q++) o[j][q] = dF[q];
-        }
-#pragma unroll
-      for (int j = 0; j < NM; j++)
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-          acc[j] += o[j][q] * dF[q];
-          acc[NM + j] += o[j][q] * f[q];
-        }
-    }
-    store4(ab.gp + e, g);
-    store4(ab.fp + e, f);
-  }
-  if (n == 0) return;
-  __shared__ double sums[2 * NM];
-  block_sums<2 * NM, AA_THREADS>(acc, sums);
-  __syncthreads();
-  if ((int)threadIdx.x < 2 * NM) {
-    const int j = (int)threadIdx.x;
-    ab.part[(size_t)(j < NM ? j : MAX_AA + j - NM) * ab.nb + blockIdx.x] = sums[j];
-  }
+  if (k < Na) aa_store_cell<NM>(s, ab, sel, k, acc);
+  if (sel.n > 0) aa_store_sums<NM, AA_THREADS>(acc, ab, (int)blockIdx.x);
 }
 
-// One workgroup: folds the store pass's partial sums, completes the Gram
-// matrix of the differences in use (the newest row from the sums, stored
-// for the next steps; the rest as stored), and solves
+// One workgroup of NT threads: folds the store pass's ab.nb partial sums,
+// completes the Gram matrix of the differences in use (the newest row from
+// the sums, stored for the next steps; the rest as stored), and solves
 // (Gram + 1e-12 trace I) gamma = b by Gaussian elimination with partial
 // pivoting into ab.gam.  A singular or non-finite solve gives gamma = 0
 // (the next start is g).
-template <int NM>
-__global__ void __launch_bounds__(AA_SOLVE_THREADS) k_aa_solve(AABuf ab, AASel sel, Gate gate) {
-  if (gated_out(gate)) return;
+template <int NM, int NT>
+__device__ __forceinline__ void aa_solve_group(const AABuf &ab, const AASel &sel) {
   const int n = sel.n;
   double acc[2 * NM];
 #pragma unroll
   for (int j = 0; j < 2 * NM; j++) acc[j] = 0.0;
-  for (int b = threadIdx.x; b < ab.nb; b += AA_SOLVE_THREADS)
+  for (int b = threadIdx.x; b < ab.nb; b += NT)
 #pragma unroll
     for (int j = 0; j < NM; j++)
       if (j < n) {
@@ -1634,7 +1640,7 @@ __global__ void __launch_bounds__(AA_SOLVE_THREADS) k_aa_solve(AABuf ab, AASel s
     const int a = (int)threadIdx.x / NM, b = (int)threadIdx.x % NM;
     if (a < n - 1 && b < n - 1) A[a][b] = ab.gram[sel.slot[a] * MAX_AA + sel.slot[b]];
   }
-  block_sums<2 * NM, AA_SOLVE_THREADS>(acc, dots);
+  block_sums<2 * NM, NT>(acc, dots);
   __syncthreads();
   if ((int)threadIdx.x < n) {  // the newest row, kept for the next steps
     const int kn = sel.slot[n - 1], sa = sel.slot[threadIdx.x];
@@ -1681,6 +1687,12 @@ __global__ void __launch_bounds__(AA_SOLVE_THREADS) k_aa_solve(AABuf ab, AASel s
   for (int a = 0; a < MAX_AA; a++) ab.gam[a] = (ok && a < n) ? gam[a] : 0.0;
 }
 
+template <int NM>
+__global__ void __launch_bounds__(AA_SOLVE_THREADS) k_aa_solve(AABuf ab, AASel sel, Gate gate) {
+  if (gated_out(gate)) return;
+  aa_solve_group<NM, AA_SOLVE_THREADS>(ab, sel);
+}
+
 // Mixed state of cell k: x = gp - sum_j gamma_j dG_j (gp = the newest
 // iterate); a value that is not finite, or a p or T that is not positive
 // where gp's is, falls back to gp.
@@ -1704,24 +1716,22 @@ __device__ __forceinline__ void aa_mixed(const AABuf &ab, const AASel &sel, cons
   }
 }
 
-// The next start x into the state arrays (and D = x - gp), and, from it,
-// what the correction pass derives without correcting: sqrt_T, rho and the
-// face densities (CorrectIn::apply on the mixed p, T of the cell and of its
-// west and south neighbours), then what the loop's flux pass writes (Fx, Fy,
-// DTx, DTy; FluxIn::apply_flux).  BX x BY tiles; the mixed p, T of the tile
-// and of its west column and south row (formed by the first threads) meet in
-// LDS.
-template <int NM>
-__global__ void __launch_bounds__(BX *BY) k_aa_mix(KGrid g, KFields fl, AAFields s, AABuf ab, AASel sel,
-                                                   Gate gate) {
-  if (gated_out(gate)) return;
-  __shared__ double sp[BY + 1][BX + 1], sT[BY + 1][BX + 1];  // [1 + row][1 + column]; row 0, column 0: halo
+// The next start x of a TX x TY tile (origin i0, j0; this thread tx, ty)
+// into the state arrays (and D = x - gp), and, from it, what the correction
+// pass derives without correcting: sqrt_T, rho and the face densities
+// (CorrectIn::apply on the mixed p, T of the cell and of its west and south
+// neighbours); with FLUX also what the loop's flux pass writes (Fx, Fy, DTx,
+// DTy; FluxIn::apply_flux).  The mixed p, T of the tile and of its west
+// column and south row (formed by the first threads) meet in LDS (sp, sT:
+// [1 + row][1 + column], row 0 and column 0 the halo).
+template <int NM, int TX, int TY, bool FLUX>
+__device__ __forceinline__ void aa_mix_tile(const KGrid &g, const KFields &fl, const AAFields &s, const AABuf &ab,
+                                            const AASel &sel, int i0, int j0, int tx, int ty,
+                                            double (*sp)[TX + 1], double (*sT)[TX + 1]) {
   double gam[NM];
 #pragma unroll
   for (int j = 0; j < NM; j++) gam[j] = ab.gam[j];
-  int i0, j0;
-  tile_origin<BX, BY>(g.Nx, i0, j0);
-  const int Nx = g.Nx, Ny = g.Ny, tx = (int)threadIdx.x, ty = (int)threadIdx.y;
+  const int Nx = g.Nx, Ny = g.Ny;
   const int i = i0 + tx, j = j0 + ty;
   const bool in = i < Nx && j < Ny;
   const Nbr nb((unsigned)(in ? i : 0), (unsigned)(in ? j : 0), (unsigned)Nx, (unsigned)Ny, g.px);
@@ -1731,7 +1741,7 @@ __global__ void __launch_bounds__(BX *BY) k_aa_mix(KGrid g, KFields fl, AAFields
     sp[ty + 1][tx + 1] = x[2];
     sT[ty + 1][tx + 1] = x[3];
   }
-  // halo: thread (t, 0) the cell south of column t, thread (t < BY, 1) the
+  // halo: thread (t, 0) the cell south of column t, thread (t < TY, 1) the
   // cell west of row t (the clamped or wrapped neighbour of the reference)
   double h[4];
   if (ty == 0 && in) {
@@ -1739,7 +1749,7 @@ __global__ void __launch_bounds__(BX *BY) k_aa_mix(KGrid g, KFields fl, AAFields
     sp[0][tx + 1] = h[2];
     sT[0][tx + 1] = h[3];
   }
-  if (ty == 1 && tx < BY && i0 < Nx && j0 + tx < Ny) {
+  if (ty == 1 && tx < TY && i0 < Nx && j0 + tx < Ny) {
     const Nbr w((unsigned)i0, (unsigned)(j0 + tx), (unsigned)Nx, (unsigned)Ny, g.px);
     aa_mixed<NM>(ab, sel, gam, w.i_1j, h);
     sp[tx + 1][0] = h[2];
@@ -1778,6 +1788,7 @@ __global__ void __launch_bounds__(BX *BY) k_aa_mix(KGrid g, KFields fl, AAFields
   unsigned long long rr[2] = {0ull, 0ull};
   double rho_uv[2];
   ci.apply(kf, rr, rho_uv);
+  if (!FLUX) return;
   // the loop's flux pass on the mixed state (fluxes, thermal coefficients):
   // its inputs at this cell and the west and south sqrt_T are all here
   FluxIn<double> fi;
@@ -1800,6 +1811,16 @@ __global__ void __launch_bounds__(BX *BY) k_aa_mix(KGrid g, KFields fl, AAFields
   fi.twu11 = FLD(TW_U)[nb.ij];
   fi.twv11 = FLD(TW_V)[nb.ij];
   fi.apply_flux(g, kf);
+}
+
+template <int NM>
+__global__ void __launch_bounds__(BX *BY) k_aa_mix(KGrid g, KFields fl, AAFields s, AABuf ab, AASel sel,
+                                                   Gate gate) {
+  if (gated_out(gate)) return;
+  __shared__ double sp[BY + 1][BX + 1], sT[BY + 1][BX + 1];
+  int i0, j0;
+  tile_origin<BX, BY>(g.Nx, i0, j0);
+  aa_mix_tile<NM, BX, BY, true>(g, fl, s, ab, sel, i0, j0, (int)threadIdx.x, (int)threadIdx.y, sp, sT);
 }
 
 __global__ void __launch_bounds__(BX *BY) k_set_inflow(KGrid g, KFields fl, double u_in) {
@@ -2067,6 +2088,74 @@ __global__ void __launch_bounds__(TX *TY) kb_flux_diff(const BatchCase<R> *C, co
   flux_diff_cell(c.g, f, t.x * TX + (int)threadIdx.x % TX, t.y * TY + (int)threadIdx.x / TX);
 }
 
+// Batched Anderson steps (FP64): the single-context kernels' code per tile
+// of a case, with the case's vectors and the shared ring slots.
+__device__ __forceinline__ AASel case_sel(const BatchCase<double> &c, const AABatchSel &bs) {
+  AASel sel;
+  sel.n = bs.n;
+#pragma unroll
+  for (int j = 0; j < MAX_AA; j++) {
+    sel.slot[j] = bs.slot[j];
+    sel.dG[j] = j < bs.n ? c.aa_dG[bs.slot[j]] : nullptr;
+    sel.dF[j] = j < bs.n ? c.aa_dF[bs.slot[j]] : nullptr;
+  }
+  return sel;
+}
+
+__device__ __forceinline__ AAFields case_state(const BatchCase<double> &c, int par) {
+  const KFields f = fields_at(c, par);
+  return AAFields{{f.f[STS_U], f.f[STS_V], f.f[STS_P], f.f[STS_T]}};
+}
+
+template <int TX, int TY>
+__global__ void __launch_bounds__(TX *TY) kb_aa_begin(const BatchCase<double> *C, const BatchTile *T) {
+  const BatchTile t = T[blockIdx.x];
+  const BatchCase<double> &c = C[t.c];
+  const int i = t.x * TX + (int)threadIdx.x % TX, j = t.y * TY + (int)threadIdx.x / TX;
+  if (i >= c.g.Nx || j >= c.g.Ny) return;
+  const AAFields s = case_state(c, 0);
+  const int k = i + j * c.g.Nx;
+  const double g4[4] = {s.x[0][k], s.x[1][k], s.x[2][k], s.x[3][k]}, z[4] = {0.0, 0.0, 0.0, 0.0};
+  store4(c.aa.gp + 4 * (size_t)k, g4);
+  store4(c.aa.D + 4 * (size_t)k, z);
+}
+
+// Partial slot of a tile: its position in the case's row-major tile grid.
+template <int TX, int TY, int NM>
+__global__ void __launch_bounds__(TX *TY) kb_aa_store(const BatchCase<double> *C, const BatchTile *T, int k, int par,
+                                                      AABatchSel bs) {
+  const BatchTile t = T[blockIdx.x];
+  const BatchCase<double> &c = C[t.c];
+  if (gated_out(Gate{c.ctrl, k + 1})) return;
+  const AASel sel = case_sel(c, bs);
+  const int i = t.x * TX + (int)threadIdx.x % TX, j = t.y * TY + (int)threadIdx.x / TX;
+  double acc[2 * NM];
+#pragma unroll
+  for (int q = 0; q < 2 * NM; q++) acc[q] = 0.0;
+  if (i < c.g.Nx && j < c.g.Ny) aa_store_cell<NM>(case_state(c, par), c.aa, sel, i + j * c.g.Nx, acc);
+  if (bs.n > 0) aa_store_sums<NM, TX * TY>(acc, c.aa, t.y * ((c.g.Nx + TX - 1) / TX) + t.x);
+}
+
+// One workgroup per case.
+template <int NM>
+__global__ void __launch_bounds__(AA_SOLVE_THREADS) kb_aa_solve(const BatchCase<double> *C, int k, AABatchSel bs) {
+  const BatchCase<double> &c = C[blockIdx.x];
+  if (gated_out(Gate{c.ctrl, k + 1})) return;
+  aa_solve_group<NM, AA_SOLVE_THREADS>(c.aa, case_sel(c, bs));
+}
+
+// The mix without the flux part (the batched flux and diffusion pass follows).
+template <int TX, int TY, int NM>
+__global__ void __launch_bounds__(TX *TY) kb_aa_mix(const BatchCase<double> *C, const BatchTile *T, int k, int par,
+                                                    AABatchSel bs) {
+  const BatchTile t = T[blockIdx.x];
+  const BatchCase<double> &c = C[t.c];
+  if (gated_out(Gate{c.ctrl, k + 1})) return;
+  __shared__ double sp[TY + 1][TX + 1], sT[TY + 1][TX + 1];
+  aa_mix_tile<NM, TX, TY, false>(c.g, fields_at(c, par), case_state(c, par), c.aa, case_sel(c, bs), t.x * TX,
+                                 t.y * TY, (int)threadIdx.x % TX, (int)threadIdx.x / TX, sp, sT);
+}
+
 // Edge conditions and the loop-2 test: one workgroup per case.
 template <typename R>
 __global__ void __launch_bounds__(EDGE_THREADS) kb_edges(const BatchCase<R> *C, int k, int par, int parts) {
@@ -2297,6 +2386,47 @@ const int SWEEP_TILES[NVARIANT][2] = {{64, 8}, {64, 4}, {128, 4}, {64, 16}};
     case 2: hipLaunchKernelGGL((KERNEL<R, 128, 2>), dim3(ntiles), dim3(256), 0, s, __VA_ARGS__); break; \
     default: hipLaunchKernelGGL((KERNEL<R, 32, 8>), dim3(ntiles), dim3(256), 0, s, __VA_ARGS__); break; \
   }
+
+#define AA_SWITCH(v, NM, KERNEL, ...)                                                                 \
+  switch (v) {                                                                                        \
+    case 0: hipLaunchKernelGGL((KERNEL<64, 4, NM>), dim3(ntiles), dim3(256), 0, s, __VA_ARGS__); break;  \
+    case 1: hipLaunchKernelGGL((KERNEL<64, 8, NM>), dim3(ntiles), dim3(512), 0, s, __VA_ARGS__); break;  \
+    case 2: hipLaunchKernelGGL((KERNEL<128, 2, NM>), dim3(ntiles), dim3(256), 0, s, __VA_ARGS__); break; \
+    default: hipLaunchKernelGGL((KERNEL<32, 8, NM>), dim3(ntiles), dim3(256), 0, s, __VA_ARGS__); break; \
+  }
+
+int aa_case_tiles(int Nx, int Ny, int variant) {
+  const int TX = CELL_TILES[variant][0], TY = CELL_TILES[variant][1];
+  return ((Nx + TX - 1) / TX) * ((Ny + TY - 1) / TY);
+}
+
+hipError_t launch_batch_aa(int kernel, int variant, const BatchCase<double> *C, int ncases, const BatchTile *T,
+                           int ntiles, int k, int par, const AABatchSel &bs, hipStream_t s) {
+  if (bs.n < 0 || bs.n > MAX_AA || variant < 0 || variant >= NVARIANT) return hipErrorInvalidValue;
+  if (kernel == BKA_SOLVE) {
+    if (bs.n <= 4) hipLaunchKernelGGL(kb_aa_solve<4>, dim3(ncases), dim3(AA_SOLVE_THREADS), 0, s, C, k, bs);
+    else hipLaunchKernelGGL(kb_aa_solve<MAX_AA>, dim3(ncases), dim3(AA_SOLVE_THREADS), 0, s, C, k, bs);
+    return hipGetLastError();
+  }
+  if (ntiles == 0) return hipSuccess;
+  if (kernel == BKA_BEGIN) {
+    switch (variant) {
+      case 0: hipLaunchKernelGGL((kb_aa_begin<64, 4>), dim3(ntiles), dim3(256), 0, s, C, T); break;
+      case 1: hipLaunchKernelGGL((kb_aa_begin<64, 8>), dim3(ntiles), dim3(512), 0, s, C, T); break;
+      case 2: hipLaunchKernelGGL((kb_aa_begin<128, 2>), dim3(ntiles), dim3(256), 0, s, C, T); break;
+      default: hipLaunchKernelGGL((kb_aa_begin<32, 8>), dim3(ntiles), dim3(256), 0, s, C, T); break;
+    }
+  } else if (kernel == BKA_STORE) {
+    if (bs.n <= 4) { AA_SWITCH(variant, 4, kb_aa_store, C, T, k, par, bs); }
+    else { AA_SWITCH(variant, MAX_AA, kb_aa_store, C, T, k, par, bs); }
+  } else if (kernel == BKA_MIX) {
+    if (bs.n <= 4) { AA_SWITCH(variant, 4, kb_aa_mix, C, T, k, par, bs); }
+    else { AA_SWITCH(variant, MAX_AA, kb_aa_mix, C, T, k, par, bs); }
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
 
 template <typename R>
 hipError_t launch_batch(int kernel, int variant, const BatchCase<R> *C, int ncases, const BatchTile *T, int ntiles,

@@ -87,3 +87,14 @@ extern const int SWEEP_TILES[NVARIANT][2];  // (SX, SY)
 template <typename R>
 hipError_t launch_batch(int kernel, int variant, const BatchCase<R> *C, int ncases, const BatchTile *T, int ntiles,
                         int k, int par, int a, int b, hipStream_t s);
+
+// Batched Anderson steps (FP64) over the cell tiles T of variant `variant`:
+// BKA_BEGIN at the loop start (parity 0), then after outer iteration k's edge
+// conditions BKA_STORE and, when bs.n > 0, BKA_SOLVE (one workgroup per
+// case) and BKA_MIX, all gated per case on iteration k + 1.  Each case's
+// aa.nb must be aa_case_tiles(Nx, Ny, variant) and its aa.part that many
+// partial slots.
+enum { BKA_BEGIN, BKA_STORE, BKA_SOLVE, BKA_MIX };
+int aa_case_tiles(int Nx, int Ny, int variant);
+hipError_t launch_batch_aa(int kernel, int variant, const BatchCase<double> *C, int ncases, const BatchTile *T,
+                           int ntiles, int k, int par, const AABatchSel &bs, hipStream_t s);
