@@ -245,8 +245,12 @@ int sts_predict_stats(sts_ctx *ctx, long long out[5]);
    case, each workgroup following its own case's loop-2 control, so a case
    that has converged drops out while the others iterate.  Each case's
    arithmetic is that of sts_time_step / sts_time_step_mixed (mc may be
-   NULL), bit for bit.  All contexts need the same N_I_p_c; their own streams
-   must be idle.  iters32/iters64 (may be NULL) get n counts each. */
+   NULL), bit for bit, except that the dot products of an Anderson step
+   (mc->anderson > 0) are summed over the case's tiles in another order;
+   a guarded predicted step that exceeds its budget pauses prediction for
+   that case instead of restarting inside the shared launches.  All
+   contexts need the same N_I_p_c; their own streams must be idle.
+   iters32/iters64 (may be NULL) get n counts each. */
 typedef struct sts_batch sts_batch;
 int sts_batch_create(sts_ctx *const *ctxs, int n, sts_batch **out);
 int sts_batch_destroy(sts_batch *b);
