@@ -918,10 +918,12 @@ static int enqueue_outer(sts_ctx *c, const Loop2State<R> &st, int k, int n, Loop
       int r;
       if ((r = enqueue_anderson(c, kf, k, sel))) return r;
     }
-  // an Anderson mix forms the fluxes itself; the pass then runs only when
-  // iteration k is the last (the mix, gated on k + 1, does not run)
+  // an Anderson mix forms the fluxes itself, and for the loop's last
+  // iteration (where the mix, gated on k + 1, does not run) run_loop2 runs
+  // the pass once the loop has stopped
+  if (aa && sel.n > 0) return STS_OK;
   t = timed_begin(c, st.kid + STS_K_FLUX_DIFF);
-  HIP_OK(launch_flux_diff(g, kf, c->stream, aa && sel.n > 0 ? Gate{c->ctrl, k, 1} : gate, FLUX_PART));
+  HIP_OK(launch_flux_diff(g, kf, c->stream, gate, FLUX_PART));
   timed_end(c, t);
   return STS_OK;
 }
@@ -957,6 +959,14 @@ static int run_loop2(sts_ctx *c, Loop2State<R> &st, R *&p, R *&T, R *&p_alt, R *
     std::swap(p, p_alt);
     std::swap(T, T_alt);
   }
+  if constexpr (std::is_same<R, double>::value)
+    if (st.aa_m > 0 && *iters % st.aa_every == 0 && (*iters / st.aa_every - 1) > 0) {
+      // the last iteration's flux pass (its Anderson mix did not run)
+      KFieldsT<R> kf = st.f;
+      kf.f[STS_P] = p;
+      kf.f[STS_T] = T;
+      HIP_OK(launch_flux_diff(*st.g, kf, c->stream, Gate{nullptr, 0}, FLUX_PART));
+    }
   if (c->h_ctrl->nonfinite) return fail(STS_ERR_NONFINITE, "non-finite residual in outer iteration " + std::to_string(*iters));
   return STS_OK;
 }

@@ -60,12 +60,9 @@ struct LoopCtrl {
 struct Gate {
   const LoopCtrl *ctrl;  // nullptr: always run
   int k;                 // outer-iteration index of this launch
-  int last_only;         // run only if iteration k is the loop's last (k == stop_at)
 };
 
-__device__ __forceinline__ bool gated_out(const Gate &g) {
-  return g.ctrl && (g.last_only ? g.k != g.ctrl->stop_at : g.k > g.ctrl->stop_at);
-}
+__device__ __forceinline__ bool gated_out(const Gate &g) { return g.ctrl && g.k > g.ctrl->stop_at; }
 
 // Residual maxima are accumulated with one atomicMax per workgroup into one
 // of NSLOT copies of each destination (copy = workgroup index mod NSLOT,
