@@ -1626,13 +1626,24 @@ __device__ __forceinline__ void aa_solve_group(const AABuf &ab, const AASel &sel
   double acc[2 * NM];
 #pragma unroll
   for (int j = 0; j < 2 * NM; j++) acc[j] = 0.0;
-  for (int b = threadIdx.x; b < ab.nb; b += NT)
+  // four rows of partial sums per thread and pass, their loads issued together
+  for (int b0 = 0; b0 < ab.nb; b0 += 4 * NT) {
+    double v[4][2 * NM];
 #pragma unroll
-    for (int j = 0; j < NM; j++)
-      if (j < n) {
-        acc[j] += ab.part[(size_t)j * ab.nb + b];
-        acc[NM + j] += ab.part[(size_t)(MAX_AA + j) * ab.nb + b];
+    for (int r = 0; r < 4; r++) {
+      const int b = b0 + r * NT + (int)threadIdx.x;
+#pragma unroll
+      for (int j = 0; j < NM; j++) {
+        const bool use = j < n && b < ab.nb;
+        v[r][j] = use ? ab.part[(size_t)j * ab.nb + b] : 0.0;
+        v[r][NM + j] = use ? ab.part[(size_t)(MAX_AA + j) * ab.nb + b] : 0.0;
       }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+      for (int j = 0; j < 2 * NM; j++) acc[j] += v[r][j];
+  }
   __shared__ double dots[2 * NM];
   __shared__ double A[NM][NM + 1];
   __shared__ double gam[NM];
