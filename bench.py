@@ -96,9 +96,11 @@ PATHS = {  # name: (FP32 cap, extrapolation degree, FP32 gate, guard, network, A
 HEADLINE = "accelerated"
 
 
-def workload_case(dst, steps):
+def workload_case(dst, steps, nx=NX, ny=NY, cfl=1.0):
+    """The supersonic square on an nx x ny grid at cfl times the Courant
+    number of the 240 x 48 case (dt = 0.01 cfl (240/nx) H/u_in)."""
     cases = importlib.import_module("simple-ts_amd.cases")
-    return cases.confined_square(dst, NX, NY, RE, KN, steps, dt_over_tin=DT_OVER_TIN)
+    return cases.confined_square(dst, nx, ny, RE, KN, steps, dt_over_tin=0.01 * cfl * 240 / nx)
 
 
 def read_fields(d):
@@ -124,13 +126,14 @@ def run_driver(steps, warmup, kernel="sweep"):
         shutil.rmtree(work, ignore_errors=True)
 
 
-def run_python_path(steps, warmup, name, net=None):
+def run_python_path(steps, warmup, name, net=None, nx=NX, ny=NY, cfl=1.0):
     """One of PATHS through the Python binding of the C ABI."""
     n_i32, degree, above, guard, _, aa = PATHS[name] or (0, 0, 0, 0, False, 0)
     ws = importlib.import_module("simple-ts_amd.warmstart")
     work = tempfile.mkdtemp(prefix="sts_bench_")
     try:
-        run = ws.Run(workload_case(os.path.join(work, "c"), warmup + steps), precision=32 if n_i32 else 64)
+        run = ws.Run(workload_case(os.path.join(work, "c"), warmup + steps, nx, ny, cfl),
+                     precision=32 if n_i32 else 64)
         n32 = n64 = taken = 0
         t0 = None
         print(f"[bench] {name} path: {warmup} + {steps} steps", file=sys.stderr, flush=True)
@@ -310,8 +313,9 @@ def main():
     ap.add_argument("--fp64-only", action="store_true", help="run only the FP64 program")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes")
     ap.add_argument("--write-case", metavar="DIR", help="only write the workload case (warmup + steps steps) to DIR")
-    ap.add_argument("--workload", default="square", choices=["square", "batch"],
-                    help="square: the 1920x384 case (default); batch: uneven cases in shared launches")
+    ap.add_argument("--workload", default="square", choices=["square", "batch", "fine"],
+                    help="square: the 1920x384 case (default); batch: uneven cases in shared launches; fine: "
+                         "the same case on 3840x768, mixed and accelerated paths against FP64 on that grid")
     ap.add_argument("--cases", type=int, default=64)
     args = ap.parse_args()
     if args.write_case:
@@ -321,6 +325,43 @@ def main():
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the workload does not shard "
               f"across GPUs in this build)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "fine":
+        # at the full Courant number the plain FP64 solver diverges on this grid
+        # (a non-finite residual within the first 500 steps); half of it converges
+        nx, ny, cfl = 2 * NX, 2 * NY, 0.5
+        paths, ref = {}, None
+        for name in ("plain", "mixed", "warm_mixed", "accelerated"):
+            try:
+                r, f = run_python_path(args.steps, args.warmup, name, None, nx, ny, cfl)
+            except Exception as e:  # a path that fails is reported, the others still run
+                paths[name] = {"error": str(e)}
+                continue
+            if ref is None and name == "plain":
+                ref = f
+            if ref is None:
+                paths[name] = {"error": "no FP64 reference fields (the plain path failed)"}
+                continue
+            paths[name] = {"value": nx * ny * args.steps / r["seconds"] / 1e6,
+                           "ms_per_step": 1e3 * r["seconds"] / args.steps,
+                           "fp32_outer_iterations_per_step": r["fp32_it"] / args.steps,
+                           "fp64_outer_iterations_per_step": r["fp64_it"] / args.steps,
+                           "field_rel_l2_vs_fp64": {k: rel_l2(f[k], ref[k]) for k in f},
+                           "field_max_abs_vs_fp64": {k: float(np.max(np.abs(f[k] - ref[k]))) for k in f}}
+        for name in paths:
+            if "value" in paths[name] and "value" in paths.get("plain", {}):
+                paths[name]["speedup_vs_fp64"] = paths[name]["value"] / paths["plain"]["value"]
+        r = paths["mixed"] if "value" in paths.get("mixed", {}) else {"value": None, "ms_per_step": None}
+        print(json.dumps({
+            "metric": "Mcell·steps/s to 1e-8 residual, mixed precision on the finer grid", "value": r["value"],
+            "unit": "Mcell·steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32+f64", "data": "synthetic: confined-square channel in the reference's input format",
+            "config": {"workload": f"supersonic confined square, {nx}x{ny}, dt {0.01 * cfl * 240 / nx:g} H/u_in, FP32 "
+                                   f"loop-2 phase then FP64 phase on the reference's test; errors against the "
+                                   f"plain FP64 path on the same grid after warmup + steps steps",
+                       "Nx": nx, "Ny": ny},
+            "paths": paths}))
+        return
     if args.workload == "batch":
         out = {"fp64": run_batch_workload(args.cases, args.steps, args.warmup, None),
                "warm_mixed": run_batch_workload(args.cases, args.steps, args.warmup,
