@@ -308,9 +308,13 @@ class Context:
 
     def time_step_mixed(self, N_I, N_I_p_c, tol_uv, tol_p, tol_T, N_I32, tol32, stall_w=4, stall_ratio=0.7,
                         batch=0, extrapolate=0, fp32_above=0, guard=0, anderson=0, anderson_every=1):
-        """Mixed-precision time step (FP32 phase, then FP64 phase with the
-        reference's test); needs precision=32.  tol32 = (uv, p, T) tolerances
-        of the FP32 phase.  Returns (FP32 iterations, FP64 iterations,
+        """Time step through sts_time_step_mixed: an FP32 phase (N_I32 > 0;
+        needs precision=32), then the FP64 phase with the reference's test.
+        tol32 = (uv, p, T) tolerances of the FP32 phase; extrapolate = degree
+        of the predicted start, fp32_above = FP32 gate, guard = restart and
+        pause bound of predicted starts, anderson = history depth of the
+        Anderson acceleration of the FP64 outer iterations (every
+        anderson_every of them).  Returns (FP32 iterations, FP64 iterations,
         residual log [n32 + n64, 6], p probe)."""
         lc = LoopControl(N_I, N_I_p_c, tol_uv, tol_p, tol_T, batch)
         mc = MixedControl(N_I32, tol32[0], tol32[1], tol32[2], stall_w, stall_ratio, extrapolate, fp32_above, guard,
