@@ -249,7 +249,7 @@ def test_accelerated_steps_end_on_the_test(anderson, tmp_path):
     run_.close()
 
 
-ACCELERATED = ["--extrapolate", "3", "--guard", "64", "--anderson", "2"]
+ACCELERATED = ["--extrapolate", "3", "--guard", "64", "--anderson", "1"]
 
 
 def _other_case(kind, d):
