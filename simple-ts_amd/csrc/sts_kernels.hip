@@ -341,11 +341,9 @@ __device__ __forceinline__ void cell_energy_source(const KGridT<R> &g, const KFi
                   + p_pr[ij] * hx[i] * hy[j];
 }
 
-// INL: the diffusion coefficients are formed here from sqrt_T, the wall
-// temperatures and the mask, with the flux and diffusion pass's own code,
-// instead of read from the arrays that pass writes; the values are the
-// same as long as those inputs have not changed since that pass.
-template <typename R, bool INL = false>
+// The assembly of cell (i, j) with Gamma and D read from their arrays (stage
+// calls, batched steps; the loops use k_assemble_inl).
+template <typename R>
 __device__ __forceinline__ void assemble_cell(const KGridT<R> &g, const KFieldsT<R> &fl, int i, int j) {
   if (i >= g.Nx || j >= g.Ny) return;
   const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny, g.px);
@@ -353,16 +351,10 @@ __device__ __forceinline__ void assemble_cell(const KGridT<R> &g, const KFieldsT
   const bool du = in_box(i, j, g.Nx_u0, g.Nx_u1, g.Ny_u0, g.Ny_u1) && m[n.i_1j] && m[n.ij];
   const bool dv = in_box(i, j, g.Nx_v0, g.Nx_v1, g.Ny_v0, g.Ny_v1) && m[n.ij_1] && m[n.ij];
   if (du || dv) {
-    CellDiff<R> df;
-    if (INL) {
-      const FaceDiff<R> f0 = face_diff(g, fl, n.ij), f1 = face_diff(g, fl, n.i1j), f2 = face_diff(g, fl, n.ij1);
-      df = CellDiff<R>{f0.Gy, f2.Gy, f0.Gx, f1.Gx, f0.Dux, f1.Dux, f0.Duy, f2.Duy, f0.Dvx, f1.Dvx, f0.Dvy, f2.Dvy};
-    } else {
-      const R *Gyf = FLD(GAMMA_YF), *Gxf = FLD(GAMMA_XF), *Dux = FLD(D_UX), *Duy = FLD(D_UY), *Dvx = FLD(D_VX),
-              *Dvy = FLD(D_VY);
-      df = CellDiff<R>{Gyf[n.ij], Gyf[n.ij1], Gxf[n.ij], Gxf[n.i1j], Dux[n.ij], Dux[n.i1j],
-                       Duy[n.ij], Duy[n.ij1], Dvx[n.ij], Dvx[n.i1j], Dvy[n.ij], Dvy[n.ij1]};
-    }
+    const R *Gyf = FLD(GAMMA_YF), *Gxf = FLD(GAMMA_XF), *Dux = FLD(D_UX), *Duy = FLD(D_UY), *Dvx = FLD(D_VX),
+            *Dvy = FLD(D_VY);
+    const CellDiff<R> df{Gyf[n.ij], Gyf[n.ij1], Gxf[n.ij], Gxf[n.i1j], Dux[n.ij], Dux[n.i1j],
+                         Duy[n.ij], Duy[n.ij1], Dvx[n.ij], Dvx[n.i1j], Dvy[n.ij], Dvy[n.ij1]};
     if (du) cell_assemble_u(g, fl, n, df);
     if (dv) cell_assemble_v(g, fl, n, df);
   }
@@ -374,7 +366,7 @@ __global__ void __launch_bounds__(BX *BY) k_assemble(KGridT<R> g, KFieldsT<R> fl
   if (gated_out(gate)) return;
   int i0, j0;
   tile_origin<BX, BY>(g.Nx, i0, j0);
-  assemble_cell<R, false>(g, fl, i0 + (int)threadIdx.x, j0 + (int)threadIdx.y);
+  assemble_cell<R>(g, fl, i0 + (int)threadIdx.x, j0 + (int)threadIdx.y);
 }
 
 // The assembly with the diffusion coefficients formed in the workgroup: each
