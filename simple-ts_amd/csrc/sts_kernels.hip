@@ -1595,14 +1595,20 @@ __device__ __forceinline__ void aa_store_sums(const double (&acc)[2 * NM], const
   if (j < 2 * NM) ab.part[(size_t)(j < NM ? j : MAX_AA + j - NM) * ab.nb + idx] = sums[j];
 }
 
+// AA_STORE_CELLS cells per thread (AA_THREADS apart), so that the partial
+// sums the weights are folded from are few.
+constexpr int AA_STORE_CELLS = 4;
 template <int NM>
 __global__ void __launch_bounds__(AA_THREADS) k_aa_store(AAFields s, AABuf ab, AASel sel, int Na, Gate gate) {
   if (gated_out(gate)) return;
-  const int k = (int)(blockIdx.x * AA_THREADS + threadIdx.x);
   double acc[2 * NM];
 #pragma unroll
   for (int j = 0; j < 2 * NM; j++) acc[j] = 0.0;
-  if (k < Na) aa_store_cell<NM>(s, ab, sel, k, acc);
+#pragma unroll
+  for (int r = 0; r < AA_STORE_CELLS; r++) {
+    const int k = (int)((blockIdx.x * AA_STORE_CELLS + r) * AA_THREADS + threadIdx.x);
+    if (k < Na) aa_store_cell<NM>(s, ab, sel, k, acc);
+  }
   if (sel.n > 0) aa_store_sums<NM, AA_THREADS>(acc, ab, (int)blockIdx.x);
 }
 
@@ -1612,8 +1618,10 @@ __global__ void __launch_bounds__(AA_THREADS) k_aa_store(AAFields s, AABuf ab, A
 // (Gram + 1e-12 trace I) gamma = b by Gaussian elimination with partial
 // pivoting into ab.gam.  A singular or non-finite solve gives gamma = 0
 // (the next start is g).
+// The weights go to gam_out (shared memory of the workgroup, valid after a
+// barrier); only a leader workgroup stores the Gram row and ab.gam.
 template <int NM, int NT>
-__device__ __forceinline__ void aa_solve_group(const AABuf &ab, const AASel &sel) {
+__device__ __forceinline__ void aa_solve_group(const AABuf &ab, const AASel &sel, double *gam_out, bool leader) {
   const int n = sel.n;
   double acc[2 * NM];
 #pragma unroll
@@ -1638,14 +1646,14 @@ __device__ __forceinline__ void aa_solve_group(const AABuf &ab, const AASel &sel
   }
   __shared__ double dots[2 * NM];
   __shared__ double A[NM][NM + 1];
-  __shared__ double gam[NM];
+  double *gam = gam_out;
   if ((int)threadIdx.x < NM * NM) {  // the stored part of the Gram matrix
     const int a = (int)threadIdx.x / NM, b = (int)threadIdx.x % NM;
     if (a < n - 1 && b < n - 1) A[a][b] = ab.gram[sel.slot[a] * MAX_AA + sel.slot[b]];
   }
   block_sums<2 * NM, NT>(acc, dots);
   __syncthreads();
-  if ((int)threadIdx.x < n) {  // the newest row, kept for the next steps
+  if (leader && (int)threadIdx.x < n) {  // the newest row, kept for the next steps
     const int kn = sel.slot[n - 1], sa = sel.slot[threadIdx.x];
     ab.gram[sa * MAX_AA + kn] = dots[threadIdx.x];
     ab.gram[kn * MAX_AA + sa] = dots[threadIdx.x];
@@ -1687,13 +1695,17 @@ __device__ __forceinline__ void aa_solve_group(const AABuf &ab, const AASel &sel
     gam[a] = t / A[a][a];
     ok = isfinite(gam[a]);
   }
-  for (int a = 0; a < MAX_AA; a++) ab.gam[a] = (ok && a < n) ? gam[a] : 0.0;
+  for (int a = 0; a < n; a++)
+    if (!ok) gam[a] = 0.0;
+  if (leader)
+    for (int a = 0; a < MAX_AA; a++) ab.gam[a] = a < n ? gam[a] : 0.0;
 }
 
 template <int NM>
 __global__ void __launch_bounds__(AA_SOLVE_THREADS) k_aa_solve(AABuf ab, AASel sel, Gate gate) {
   if (gated_out(gate)) return;
-  aa_solve_group<NM, AA_SOLVE_THREADS>(ab, sel);
+  __shared__ double gam[NM];
+  aa_solve_group<NM, AA_SOLVE_THREADS>(ab, sel, gam, true);
 }
 
 // Mixed state of cell k: x = gp - sum_j gamma_j dG_j (gp = the newest
@@ -1729,11 +1741,11 @@ __device__ __forceinline__ void aa_mixed(const AABuf &ab, const AASel &sel, cons
 // [1 + row][1 + column], row 0 and column 0 the halo).
 template <int NM, int TX, int TY, bool FLUX>
 __device__ __forceinline__ void aa_mix_tile(const KGrid &g, const KFields &fl, const AAFields &s, const AABuf &ab,
-                                            const AASel &sel, int i0, int j0, int tx, int ty,
+                                            const AASel &sel, const double *gsrc, int i0, int j0, int tx, int ty,
                                             double (*sp)[TX + 1], double (*sT)[TX + 1]) {
   double gam[NM];
 #pragma unroll
-  for (int j = 0; j < NM; j++) gam[j] = ab.gam[j];
+  for (int j = 0; j < NM; j++) gam[j] = gsrc[j];
   const int Nx = g.Nx, Ny = g.Ny;
   const int i = i0 + tx, j = j0 + ty;
   const bool in = i < Nx && j < Ny;
@@ -1816,14 +1828,26 @@ __device__ __forceinline__ void aa_mix_tile(const KGrid &g, const KFields &fl, c
   fi.apply_flux(g, kf);
 }
 
+// The weights and the mix in one launch: every workgroup folds the store
+// pass's partial sums itself (the same sums in the same order, so the same
+// weights; workgroup 0 stores the Gram row), then mixes AA_MIX_TILES tiles
+// of AA_MIX_TX x AA_MIX_TY cells in turn.
+constexpr int AA_MIX_TX = 64, AA_MIX_TY = 8, AA_MIX_TILES = 3;
 template <int NM>
-__global__ void __launch_bounds__(BX *BY) k_aa_mix(KGrid g, KFields fl, AAFields s, AABuf ab, AASel sel,
-                                                   Gate gate) {
+__global__ void __launch_bounds__(AA_MIX_TX *AA_MIX_TY) k_aa_mix(KGrid g, KFields fl, AAFields s, AABuf ab,
+                                                                 AASel sel, Gate gate) {
   if (gated_out(gate)) return;
-  __shared__ double sp[BY + 1][BX + 1], sT[BY + 1][BX + 1];
-  int i0, j0;
-  tile_origin<BX, BY>(g.Nx, i0, j0);
-  aa_mix_tile<NM, BX, BY, true>(g, fl, s, ab, sel, i0, j0, (int)threadIdx.x, (int)threadIdx.y, sp, sT);
+  __shared__ double gam[NM];
+  aa_solve_group<NM, AA_MIX_TX * AA_MIX_TY>(ab, sel, gam, blockIdx.x == 0);
+  __syncthreads();
+  __shared__ double sp[AA_MIX_TY + 1][AA_MIX_TX + 1], sT[AA_MIX_TY + 1][AA_MIX_TX + 1];
+  const int ntx = (g.Nx + AA_MIX_TX - 1) / AA_MIX_TX, ntiles = ntx * ((g.Ny + AA_MIX_TY - 1) / AA_MIX_TY);
+  const int tx = (int)threadIdx.x % AA_MIX_TX, ty = (int)threadIdx.x / AA_MIX_TX;
+  for (int t = (int)blockIdx.x; t < ntiles; t += (int)gridDim.x) {
+    aa_mix_tile<NM, AA_MIX_TX, AA_MIX_TY, true>(g, fl, s, ab, sel, gam, (t % ntx) * AA_MIX_TX,
+                                               (t / ntx) * AA_MIX_TY, tx, ty, sp, sT);
+    __syncthreads();  // the next tile reuses sp, sT
+  }
 }
 
 __global__ void __launch_bounds__(BX *BY) k_set_inflow(KGrid g, KFields fl, double u_in) {
@@ -2144,7 +2168,8 @@ template <int NM>
 __global__ void __launch_bounds__(AA_SOLVE_THREADS) kb_aa_solve(const BatchCase<double> *C, int k, AABatchSel bs) {
   const BatchCase<double> &c = C[blockIdx.x];
   if (gated_out(Gate{c.ctrl, k + 1})) return;
-  aa_solve_group<NM, AA_SOLVE_THREADS>(c.aa, case_sel(c, bs));
+  __shared__ double gam[NM];
+  aa_solve_group<NM, AA_SOLVE_THREADS>(c.aa, case_sel(c, bs), gam, true);
 }
 
 // The mix without the flux part (the batched flux and diffusion pass follows).
@@ -2155,8 +2180,8 @@ __global__ void __launch_bounds__(TX *TY) kb_aa_mix(const BatchCase<double> *C, 
   const BatchCase<double> &c = C[t.c];
   if (gated_out(Gate{c.ctrl, k + 1})) return;
   __shared__ double sp[TY + 1][TX + 1], sT[TY + 1][TX + 1];
-  aa_mix_tile<NM, TX, TY, false>(c.g, fields_at(c, par), case_state(c, par), c.aa, case_sel(c, bs), t.x * TX,
-                                 t.y * TY, (int)threadIdx.x % TX, (int)threadIdx.x / TX, sp, sT);
+  aa_mix_tile<NM, TX, TY, false>(c.g, fields_at(c, par), case_state(c, par), c.aa, case_sel(c, bs), c.aa.gam,
+                                 t.x * TX, t.y * TY, (int)threadIdx.x % TX, (int)threadIdx.x / TX, sp, sT);
 }
 
 // Edge conditions and the loop-2 test: one workgroup per case.
@@ -2306,9 +2331,9 @@ hipError_t launch_extrapolate(const KGrid &g, const KFields &f, const ExtrapHist
   hipLaunchKernelGGL(k_extrapolate, grid2d(g), dim3(BX, BY), 0, s, g, f, hist, order, slot);
   return hipGetLastError();
 }
-int aa_store_blocks(int Na) { return (Na + AA_THREADS - 1) / AA_THREADS; }
+int aa_store_blocks(int Na) { return (Na + AA_THREADS * AA_STORE_CELLS - 1) / (AA_THREADS * AA_STORE_CELLS); }
 hipError_t launch_aa_begin(const AAFields &f, const AABuf &ab, int Na, hipStream_t s) {
-  hipLaunchKernelGGL(k_aa_begin, dim3(aa_store_blocks(Na)), dim3(AA_THREADS), 0, s, f, ab, Na);
+  hipLaunchKernelGGL(k_aa_begin, dim3((Na + AA_THREADS - 1) / AA_THREADS), dim3(AA_THREADS), 0, s, f, ab, Na);
   return hipGetLastError();
 }
 hipError_t launch_aa_store(const AAFields &f, const AABuf &ab, const AASel &sel, int Na, hipStream_t s, Gate gate) {
@@ -2321,13 +2346,10 @@ hipError_t launch_aa_mix(const KGrid &g, const KFields &fl, const AAFields &f, c
                          hipStream_t s, Gate gate) {
   const int Na = g.Nx * g.Ny;
   if (sel.n < 1 || sel.n > MAX_AA || ab.nb != aa_store_blocks(Na)) return hipErrorInvalidValue;
-  if (sel.n <= 4) {
-    hipLaunchKernelGGL(k_aa_solve<4>, dim3(1), dim3(AA_SOLVE_THREADS), 0, s, ab, sel, gate);
-    hipLaunchKernelGGL(k_aa_mix<4>, (grid_tiles<BX, BY>(g)), dim3(BX, BY), 0, s, g, fl, f, ab, sel, gate);
-  } else {
-    hipLaunchKernelGGL(k_aa_solve<MAX_AA>, dim3(1), dim3(AA_SOLVE_THREADS), 0, s, ab, sel, gate);
-    hipLaunchKernelGGL(k_aa_mix<MAX_AA>, (grid_tiles<BX, BY>(g)), dim3(BX, BY), 0, s, g, fl, f, ab, sel, gate);
-  }
+  const int ntiles = ((g.Nx + AA_MIX_TX - 1) / AA_MIX_TX) * ((g.Ny + AA_MIX_TY - 1) / AA_MIX_TY);
+  const dim3 gm((ntiles + AA_MIX_TILES - 1) / AA_MIX_TILES);
+  if (sel.n <= 4) hipLaunchKernelGGL(k_aa_mix<4>, gm, dim3(AA_MIX_TX * AA_MIX_TY), 0, s, g, fl, f, ab, sel, gate);
+  else hipLaunchKernelGGL(k_aa_mix<MAX_AA>, gm, dim3(AA_MIX_TX * AA_MIX_TY), 0, s, g, fl, f, ab, sel, gate);
   return hipGetLastError();
 }
 hipError_t launch_set_inflow(const KGrid &g, const KFields &f, double u_in, hipStream_t s) {
