@@ -58,7 +58,9 @@ struct sts_ctx {
   // device-side loop 2
   LoopCtrl *ctrl = nullptr;
   LoopCtrl *h_ctrl = nullptr;          // pinned mirror
+  hipEvent_t ev_ctrl = nullptr;        // after the copy of ctrl to h_ctrl
   unsigned long long *log = nullptr;   // 6 residual maxima per outer iteration
+  unsigned long long *h_log = nullptr; // pinned mirror of log
   int log_cap = 0;
   int last_iters = 0;
   // FP32 copy of the state for the mixed-precision time step (precision 32)
@@ -260,6 +262,8 @@ extern "C" int sts_destroy(sts_ctx *c) {
   (void)hipFree(c->cont);
   if (c->h_red) (void)hipHostFree(c->h_red);
   if (c->h_ctrl) (void)hipHostFree(c->h_ctrl);
+  if (c->h_log) (void)hipHostFree(c->h_log);
+  if (c->ev_ctrl) (void)hipEventDestroy(c->ev_ctrl);
   (void)hipFree(c->ctrl);
   (void)hipFree(c->log);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -943,15 +947,25 @@ static int run_loop2(sts_ctx *c, Loop2State<R> &st, R *&p, R *&T, R *&p_alt, R *
       if ((r = ensure_anderson(c, st.aa_m))) return r;
       HIP_OK(launch_aa_begin(AAFields{{st.f.f[STS_U], st.f.f[STS_V], p, T}}, c->aa, c->Na, c->stream));
     }
+  // The loop control is read after iteration end - 1 - LOOKAHEAD of each
+  // batch (after the last one of a batch that ends at N_I), so that the host
+  // decides while the GPU still runs the batch's tail; iterations enqueued
+  // past the stop are gated out on the device.
+  constexpr int LOOKAHEAD = 1;
   int k = 0, stop = -1, rc;
   while (stop < 0) {
     const int end = std::min(lt.N_I, k + batch);
-    for (; k < end; k++)
+    const int check = end == lt.N_I ? end - 1 : std::max(k, end - 1 - LOOKAHEAD);
+    for (; k < end; k++) {
       if ((rc = enqueue_outer(c, st, k, n_pc, lt))) return rc;
-    HIP_OK(hipMemcpyAsync(c->h_ctrl, c->ctrl, sizeof(LoopCtrl), hipMemcpyDeviceToHost, c->stream));
-    HIP_OK(hipStreamSynchronize(c->stream));
+      if (k == check) {
+        HIP_OK(hipMemcpyAsync(c->h_ctrl, c->ctrl, sizeof(LoopCtrl), hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipEventRecord(c->ev_ctrl, c->stream));
+      }
+    }
+    HIP_OK(hipEventSynchronize(c->ev_ctrl));
     if (c->h_ctrl->stop_at != 0x7fffffff) stop = c->h_ctrl->stop_at;
-    else if (k >= lt.N_I) return fail(STS_ERR_ARG, "loop 2 did not stop at N_I");  // the test always stops there
+    else if (check == lt.N_I - 1) return fail(STS_ERR_ARG, "loop 2 did not stop at N_I");  // the test always stops there
     batch = 2;
   }
   *iters = stop + 1;
@@ -976,25 +990,28 @@ static int ensure_loop_buffers(sts_ctx *c, int n_iter) {
     (void)hipFree(c->log);
     c->log = nullptr;
     HIP_OK(hipMalloc(&c->log, sizeof(unsigned long long) * 6 * (size_t)n_iter));
+    if (c->h_log) (void)hipHostFree(c->h_log);
+    c->h_log = nullptr;
+    HIP_OK(hipHostMalloc(&c->h_log, sizeof(unsigned long long) * 6 * (size_t)n_iter, hipHostMallocDefault));
     c->log_cap = n_iter;
   }
   if (!c->ctrl) {
     HIP_OK(hipMalloc(&c->ctrl, sizeof(LoopCtrl)));
     HIP_OK(hipHostMalloc(&c->h_ctrl, sizeof(LoopCtrl), hipHostMallocDefault));
+    HIP_OK(hipEventCreateWithFlags(&c->ev_ctrl, hipEventDisableTiming));
   }
   return STS_OK;
 }
 
 // Copies n_iter iterations of the residual log and the probe to the host.
 static int finish_step(sts_ctx *c, int n_iter, double *res_log, double *p_probe) {
-  std::vector<unsigned long long> lg(6 * (size_t)n_iter);
-  if (n_iter)
-    HIP_OK(hipMemcpyAsync(lg.data(), c->log, sizeof(unsigned long long) * lg.size(), hipMemcpyDeviceToHost,
-                          c->stream));
+  const size_t nw = 6 * (size_t)n_iter;  // n_iter <= log_cap
+  if (n_iter && res_log)
+    HIP_OK(hipMemcpyAsync(c->h_log, c->log, sizeof(unsigned long long) * nw, hipMemcpyDeviceToHost, c->stream));
   HIP_OK(hipMemcpyAsync(&c->h_red[NRED], c->probe, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_OK(hipStreamSynchronize(c->stream));
   if (res_log)
-    for (size_t q = 0; q < lg.size(); q++) res_log[q] = bits_to_double(lg[q]);
+    for (size_t q = 0; q < nw; q++) res_log[q] = bits_to_double(c->h_log[q]);
   if (p_probe) std::memcpy(p_probe, &c->h_red[NRED], sizeof(double));
   return STS_OK;
 }
