@@ -1322,7 +1322,8 @@ struct sts_batch {
   int n_wall = 0;
   int cell_v = 0, sweep_v = 0;
   int last_iters32 = 0, last_iters64 = 0;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;  // autotuner timing
+  hipEvent_t ev_ctrl = nullptr;             // after the copy of ctrl to h_ctrl
 };
 
 extern "C" int sts_batch_destroy(sts_batch *b) {
@@ -1337,6 +1338,7 @@ extern "C" int sts_batch_destroy(sts_batch *b) {
   (void)hipFree(b->wall_t);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->ev_ctrl) (void)hipEventDestroy(b->ev_ctrl);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return STS_OK;
@@ -1394,6 +1396,7 @@ extern "C" int sts_batch_create(sts_ctx *const *ctxs, int n, sts_batch **out) {
   B_OK(hipMalloc(&b->lred, sizeof(unsigned long long) * NSLOT * SLOT_STRIDE * (size_t)n));
   B_OK(hipEventCreateWithFlags(&b->ev0, hipEventDisableSystemFence));
   B_OK(hipEventCreateWithFlags(&b->ev1, hipEventDisableSystemFence));
+  B_OK(hipEventCreateWithFlags(&b->ev_ctrl, hipEventDisableTiming));
 #undef B_OK
   int rc;
   if ((rc = build_tiles(b, CELL_TILES, b->cell_t, b->n_cell)) || (rc = build_tiles(b, SWEEP_TILES, b->sweep_t, b->n_sweep)))
@@ -1477,9 +1480,13 @@ static int run_batch_loop2(sts_batch *b, BatchCase<R> *C, int n_pc, int N_I, int
   HIP_OK(launch_batch<R>(BK_LOOP_OPEN, 0, C, n, nullptr, 0, 0, 0, 0, 0, s));
   if constexpr (sizeof(R) == sizeof(double))
     if (aa_m > 0) HIP_OK(launch_batch_aa(BKA_BEGIN, cv, C, n, ct, nct, 0, 0, AABatchSel{}, s));
+  // the loop control is read one iteration before a batch ends (as in the
+  // single-context loop), after the last one of a batch that ends at N_I
+  constexpr int LOOKAHEAD = 1;
   int k = 0;
   while (true) {
     const int end = std::min(N_I, k + batch);
+    const int check = end == N_I ? end - 1 : std::max(k, end - 1 - LOOKAHEAD);
     for (; k < end; k++) {
       int par = (int)(((long long)k * passes) & 1);
       HIP_OK(launch_batch<R>(BK_ASSEMBLE, cv, C, n, ct, nct, k, par, 0, 0, s));
@@ -1511,13 +1518,16 @@ static int run_batch_loop2(sts_batch *b, BatchCase<R> *C, int n_pc, int N_I, int
           }
         }
       HIP_OK(launch_batch<R>(BK_FLUX_DIFF, cv, C, n, ct, nct, k, par, 0, 0, s));
+      if (k == check) {
+        HIP_OK(hipMemcpyAsync(b->h_ctrl, b->ctrl, sizeof(LoopCtrl) * n, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipEventRecord(b->ev_ctrl, s));
+      }
     }
-    HIP_OK(hipMemcpyAsync(b->h_ctrl, b->ctrl, sizeof(LoopCtrl) * n, hipMemcpyDeviceToHost, s));
-    HIP_OK(hipStreamSynchronize(s));
+    HIP_OK(hipEventSynchronize(b->ev_ctrl));
     bool open = false;
     for (int i = 0; i < n; i++) open = open || b->h_ctrl[i].stop_at == 0x7fffffff;
     if (!open) break;
-    if (k >= N_I) return fail(STS_ERR_ARG, "loop 2 did not stop at N_I");
+    if (check == N_I - 1) return fail(STS_ERR_ARG, "loop 2 did not stop at N_I");
     batch = 2;
   }
   iters.assign(n, 0);
