@@ -1614,12 +1614,13 @@ __global__ void __launch_bounds__(AA_THREADS) k_aa_store(AAFields s, AABuf ab, A
 
 // One workgroup of NT threads: folds the store pass's ab.nb partial sums,
 // completes the Gram matrix of the differences in use (the newest row from
-// the sums, stored for the next steps; the rest as stored), and solves
-// (Gram + 1e-12 trace I) gamma = b by Gaussian elimination with partial
-// pivoting into ab.gam.  A singular or non-finite solve gives gamma = 0
-// (the next start is g).
-// The weights go to gam_out (shared memory of the workgroup, valid after a
-// barrier); only a leader workgroup stores the Gram row and ab.gam.
+// the sums; the rest as stored), and solves (Gram + 1e-12 trace I) gamma = b
+// by Gaussian elimination with partial pivoting.  A singular or non-finite
+// solve gives gamma = 0 (the next start is g).  The weights go to gam_out
+// (shared memory of the workgroup, valid after a barrier); a leader
+// workgroup also stores the newest Gram row, for the next steps, and the
+// weights in ab.gam.  Workgroups that solve the same system concurrently
+// read only rows other than the newest, which the leader alone writes.
 template <int NM, int NT>
 __device__ __forceinline__ void aa_solve_group(const AABuf &ab, const AASel &sel, double *gam_out, bool leader) {
   const int n = sel.n;
@@ -1699,13 +1700,6 @@ __device__ __forceinline__ void aa_solve_group(const AABuf &ab, const AASel &sel
     if (!ok) gam[a] = 0.0;
   if (leader)
     for (int a = 0; a < MAX_AA; a++) ab.gam[a] = a < n ? gam[a] : 0.0;
-}
-
-template <int NM>
-__global__ void __launch_bounds__(AA_SOLVE_THREADS) k_aa_solve(AABuf ab, AASel sel, Gate gate) {
-  if (gated_out(gate)) return;
-  __shared__ double gam[NM];
-  aa_solve_group<NM, AA_SOLVE_THREADS>(ab, sel, gam, true);
 }
 
 // Mixed state of cell k: x = gp - sum_j gamma_j dG_j (gp = the newest
