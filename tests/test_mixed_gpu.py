@@ -285,3 +285,44 @@ def test_accelerated_path_on_other_cases(kind, tmp_path):
     n64, nac = _iterations(d64), _iterations(dac)
     print(f"{kind}: iterations plain {sum(n64)} {n64}, accelerated {sum(nac)} {nac}", flush=True)
     assert sum(nac) < sum(n64)
+
+
+@pytest.mark.parametrize("cap", [0, 5])
+def test_batch_size_does_not_change_the_step(cap, tmp_path):
+    """Loop 2 is enqueued in batches whose loop control the host reads one
+    iteration before the batch ends (after the last iteration when the batch
+    ends at N_I).  Any batch size must give the same steps bit for bit:
+    iteration counts, residual logs, probe and fields, on the plain and the
+    accelerated entry points, both when the reference's test stops the loop
+    (cap = 0) and when N_I does (cap = 5, far fewer than the steps from rest
+    need)."""
+    cases = importlib.import_module("simple-ts_amd.cases")
+    ws = importlib.import_module("simple-ts_amd.warmstart")
+    d = cases.confined_square(str(tmp_path / "c"), 120, 24, 1415, 0.00283, 6, dt_over_tin=0.01)
+    ref = None
+    for batch in (0, 1, 2, 3, 5, 1000):
+        run_ = ws.Run(d, 64)
+        N_I, npc, tuv, tp, tT = run_.loop
+        N_I = cap or N_I
+        out = []
+        for step in range(3):
+            it, log, pr = run_.ctx.time_step(N_I, npc, tuv, tp, tT, batch=batch)
+            out.append((it, log.copy(), pr))
+        for step in range(3):
+            _, it, log, pr = run_.ctx.time_step_mixed(N_I, npc, tuv, tp, tT, 0, (1e-7,) * 3, batch=batch,
+                                                      extrapolate=2, anderson=1)
+            out.append((it, log.copy(), pr))
+        fields = [run_.ctx.get(f) for f in ("u", "v", "p", "Temper")]
+        run_.close()
+        print(f"cap {cap} batch {batch}: iterations {[o[0] for o in out]}", flush=True)
+        if cap:
+            assert all(o[0] == cap for o in out), [o[0] for o in out]
+        else:
+            assert all(0 < o[0] < N_I for o in out), [o[0] for o in out]
+        if ref is None:
+            ref = (out, fields)
+            continue
+        for (a, b) in zip(out, ref[0]):
+            assert a[0] == b[0] and np.array_equal(a[1], b[1]) and a[2] == b[2], (batch, a[0], b[0])
+        for a, b in zip(fields, ref[1]):
+            assert np.array_equal(a, b), batch
