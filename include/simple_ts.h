@@ -173,7 +173,8 @@ typedef struct sts_loop_control {
   double tol_uv;      /* MaxError_Velocities */
   double tol_p;       /* MaxError_p_c */
   double tol_T;       /* MaxError_T */
-  int batch;          /* outer iterations enqueued before the first host check; 0 = from the last step */
+  int batch;          /* outer iterations in the first batch enqueued (the host reads the loop control one
+                         iteration before a batch ends); 0 = from the last step's count */
 } sts_loop_control;
 
 /* One time step of loop 1 (S:3230-4608): the prologue S:3233-3239, then
