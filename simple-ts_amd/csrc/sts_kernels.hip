@@ -1597,7 +1597,7 @@ __device__ __forceinline__ void aa_store_sums(const double (&acc)[2 * NM], const
 
 // AA_STORE_CELLS cells per thread (AA_THREADS apart), so that the partial
 // sums the weights are folded from are few.
-constexpr int AA_STORE_CELLS = 4;
+constexpr int AA_STORE_CELLS = 2;
 template <int NM>
 __global__ void __launch_bounds__(AA_THREADS) k_aa_store(AAFields s, AABuf ab, AASel sel, int Na, Gate gate) {
   if (gated_out(gate)) return;
