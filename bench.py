@@ -41,19 +41,32 @@ path's figure, its speed-up over the faster of fp64 and plain, and every
 field's relative L2 error against the fp64 path after the W + K steps are
 reported beside it.
 
-roofline: the FP64 fused p/T Jacobi pass (k_sweep2, two sweeps per launch),
-2 x 144 algorithmic bytes per cell per launch (16 FP64 arrays read and 2
-written per sweep) over all Nx*Ny cells, divided by its average launch
-duration from HIP events recorded around every launch in the timed region,
-against the 8 TB/s HBM peak.  traffic: the memory bytes one launch really
-moves, from two rocprofv3 counter passes over a 3-step run of the same case
-(FETCH_SIZE, then WRITE_SIZE, median KiB per launch), each scaled by what the
-counter reports for the 5-array copy of the time-step prologue (k_copy5,
-5 x 8 x Nx*Ny bytes read and written).
+roofline (plain FP64 program, timed steps only): the kernel with the largest
+share of the outer iteration's time, the coefficient assembly of the loops
+(k_assemble_inl: u and v pseudo-velocities and the energy source), at the
+reference-formula bytes of A2 + A3 + A4, 128 + 128 + 40 = 296 B per cell
+(12 + 12 + 4 arrays read, 4 + 4 + 1 written, 8 B each, shared arrays counted
+once per equation) over all Nx*Ny cells, divided by its average launch
+duration from HIP events around every launch in the timed region that did
+work (launches that the device-side loop gates out past its stop are not
+counted), against the 8 TB/s HBM peak.  Beside it, frac_measured: the bytes
+one launch really moves (traffic) over the same duration.  traffic comes from
+two rocprofv3 counter passes over a 3-step run of the same case (FETCH_SIZE,
+then WRITE_SIZE, median KiB per working launch), each scaled by what the
+counter reports for the 5-array copy of the time-step prologue (k_copy5, 5 x
+8 x Nx*Ny bytes read and written).  The same figures are given for the fused
+p/T sweep pass (k_sweep2: two sweeps that read their 16 arrays once and write
+p and T once, 144 B per cell per launch) and for the Jacobi sweeps together
+(assembly + loop 3 of one outer iteration).  At 1920 x 384 the state (about
+212 MB) fits the 256 MB last-level cache, so the measured bytes are memory
+traffic that may be served by that cache rather than by HBM.
 
 The CPU baseline is the reference solver itself (oracle/_ref/sts_ref, serial,
-one pinned core) on the first steps of the same case: the time of step 2 is
-the difference of a 2-step and a 1-step run (setup excluded).
+one pinned core) inside the GPU's timed window: the GPU program, bit-identical
+to the reference, writes the state after the W warm-up steps; the reference
+restarts from those files (ToReadSolvedDataFromFile) for 1 and for 1 + n
+steps, and the difference of the two runs times steps W+2 .. W+1+n with setup
+excluded.  Outer iterations per step are reported for both sides.
 """
 import argparse
 import importlib
@@ -75,7 +88,12 @@ NX, NY = 1920, 384
 RE, KN = 1415.0, 0.00283
 DT_OVER_TIN = 0.01 * 240 / NX
 HBM_PEAK_GBS = 8000.0
-SWEEP_BYTES_PER_CELL = 144  # FP64, per sweep: Fx Fy DTx DTy T_old p_old u v ScT1 rho apx apy bpx bpy p_pr T_pr + T p
+# FP64 reference-formula bytes per cell: assembly A2 + A3 + A4; the fused
+# pass of two p/T sweeps reads Fx Fy DTx DTy T_old p_old u v ScT1 rho apx apy
+# bpx bpy p_pr T_pr once and writes T p once
+ASSEMBLE_BYTES_PER_CELL = 128 + 128 + 40
+SWEEP2_BYTES_PER_CELL = 144
+CPU_STEPS = 2  # steps of the CPU sample (after one restart step)
 WEIGHTS = os.path.join(ROOT, "simple-ts_amd", "unet_b32.npz")
 MIXED = (5000, (1e-7, 1e-7, 1e-7), 6, 0.95)  # N_I32, FP32 tolerances, stagnation window and ratio
 DEGREE = 2        # predicted start: polynomial degree in time
@@ -108,7 +126,7 @@ def read_fields(d):
     return {f: np.fromfile(os.path.join(d, f + ".bin"), dtype=np.float64) for f in ("u", "v", "p", "Temper")}
 
 
-def run_driver(steps, warmup, kernel="sweep"):
+def run_driver(steps, warmup, kernel="assemble_inl,sweep"):
     """The FP64 program; returns its timing record and final fields."""
     pkg = importlib.import_module("simple-ts_amd")
     if not os.access(pkg.DRIVER, os.X_OK):
@@ -161,8 +179,11 @@ def run_python_path(steps, warmup, name, net=None, nx=NX, ny=NY, cfl=1.0):
         shutil.rmtree(work, ignore_errors=True)
 
 
+KERNELS = {"assemble_inl": "k_assemble_inl<double", "sweep": "k_sweep2<double", "copy": "k_copy5"}
+
+
 def counter_pass(counter, case_dir, out_dir):
-    """Median KiB per launch of k_sweep2<double> and k_copy5 for one counter."""
+    """Median KiB per working launch of each kernel of KERNELS for one counter."""
     import csv
     import statistics
     pkg = importlib.import_module("simple-ts_amd")
@@ -170,43 +191,47 @@ def counter_pass(counter, case_dir, out_dir):
     if not prof:
         return None
     d = os.path.join(out_dir, counter)
-    cmd = ["timeout", "-s", "KILL", "120", prof, "--pmc", counter, "--kernel-include-regex", "k_sweep2|k_copy5",
-           "--output-format", "csv", "-d", d, "-o", "c", "--", pkg.DRIVER, case_dir]
+    cmd = ["timeout", "-s", "KILL", "120", prof, "--pmc", counter, "--kernel-include-regex",
+           "k_assemble_inl|k_sweep2|k_copy5", "--output-format", "csv", "-d", d, "-o", "c", "--", pkg.DRIVER,
+           case_dir]
     print(f"[bench] counter pass {counter}", file=sys.stderr, flush=True)
     env = dict(os.environ, TMPDIR=out_dir)
     r = subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, cwd=out_dir, env=env)
     path = os.path.join(d, "c_counter_collection.csv")
     if r.returncode != 0 or not os.path.exists(path):
         return None
-    by = {"sweep": [], "copy": []}
+    by = {k: [] for k in KERNELS}
     for row in csv.DictReader(open(path)):
-        name = row["Kernel_Name"]
-        key = "sweep" if "k_sweep2<double>" in name else ("copy" if "k_copy5" in name else None)
-        if key:
-            by[key].append(float(row["Counter_Value"]))
-    if not by["sweep"] or not by["copy"]:
+        for k, pat in KERNELS.items():
+            if pat in row["Kernel_Name"]:
+                by[k].append(float(row["Counter_Value"]))
+    if not all(by.values()):
         return None
-    top = max(by["sweep"])
-    live = [v for v in by["sweep"] if v > 0.05 * top]  # launches past the loop-2 stop return at once
-    return statistics.median(live), statistics.median(by["copy"])
+    out = {}
+    for k, vals in by.items():
+        top = max(vals)
+        live = [v for v in vals if v > 0.05 * top]  # launches past the loop-2 stop return at once
+        out[k] = statistics.median(live)
+    return out
 
 
 def measure_traffic():
-    """HBM bytes per k_sweep2 launch: FETCH_SIZE + WRITE_SIZE, each scaled by
-    the same counter's reading for k_copy5 (known bytes)."""
+    """Memory bytes per launch of k_assemble_inl and k_sweep2: FETCH_SIZE +
+    WRITE_SIZE, each scaled by the same counter's reading for k_copy5 (known
+    bytes)."""
     work = tempfile.mkdtemp(prefix="sts_pmc_")
     try:
         d = workload_case(os.path.join(work, "c"), 3)
         copy_bytes = 5 * 8 * NX * NY
-        total, factors = 0.0, {}
+        total, factors = {"assemble_inl": 0.0, "sweep": 0.0}, {}
         for counter in ("FETCH_SIZE", "WRITE_SIZE"):
             r = counter_pass(counter, d, work)
             if r is None:
                 return None, None
-            sweep_kib, copy_kib = r
-            f = copy_bytes / (copy_kib * 1024.0)
+            f = copy_bytes / (r["copy"] * 1024.0)
             factors[counter] = f
-            total += sweep_kib * 1024.0 * f
+            for k in total:
+                total[k] += r[k] * 1024.0 * f
         return total, factors
     finally:
         shutil.rmtree(work, ignore_errors=True)
@@ -223,28 +248,64 @@ def load_net():
                  "holdout_loss": float(z["holdout_loss"][-1])}
 
 
-def run_cpu_baseline():
-    """The reference solver on one pinned core: step 2 of the same case."""
+def _set_entries(case_dir, **values):
+    """Rewrites entries of a case's EnteredData.txt in place."""
+    cases = importlib.import_module("simple-ts_amd.cases")
+    names = cases.entry_names()
+    path = os.path.join(case_dir, "EnteredData.txt")
+    lines = open(path).read().split("\n")
+    for k, name in enumerate(names):
+        if name in values:
+            lines[k] = cases.fmt(values[name]) + "\t" + lines[k].split("\t", 1)[1]
+    open(path, "w").write("\n".join(lines))
+
+
+def run_cpu_baseline(warmup, n=CPU_STEPS):
+    """The reference solver on one pinned core inside the GPU's timed window:
+    restarted from the GPU program's state after `warmup` steps, the
+    difference of a (1 + n)-step and a 1-step run times steps warmup+2 ..
+    warmup+1+n."""
+    pkg = importlib.import_module("simple-ts_amd")
     ref = os.path.join(ROOT, "oracle", "_ref", "sts_ref")
-    if not os.access(ref, os.X_OK):
+    if not os.access(ref, os.X_OK) or not os.access(pkg.DRIVER, os.X_OK):
         return None
     work = tempfile.mkdtemp(prefix="sts_cpu_")
     try:
-        times = {}
-        for n in (1, 2):
-            d = workload_case(os.path.join(work, f"n{n}"), n)
+        base = workload_case(os.path.join(work, "w"), warmup)
+        _set_entries(base, Nt_save_solved_data=warmup)
+        subprocess.run([pkg.DRIVER, base], check=True, stdout=subprocess.DEVNULL)
+        times, iters = {}, {}
+        for steps in (1, 1 + n):
+            d = os.path.join(work, f"r{steps}")
+            shutil.copytree(base, d)
+            # the program's saves rewrite max_time_for_solve with its elapsed hours,
+            # which would end the restarted run's time loop at once
+            _set_entries(d, ToReadSolvedDataFromFile=1, Nt=steps, Nt_save_temp_data=1, max_time_for_solve=1000.0)
+            for f in ("Tmp.txt", "Iterations_in_loops.txt"):
+                if os.path.exists(os.path.join(d, f)):
+                    os.remove(os.path.join(d, f))
             cmd = [ref]
             if shutil.which("taskset"):
                 cmd = ["taskset", "-c", "0"] + cmd
-            print(f"[bench] cpu reference: {n} step(s)", file=sys.stderr, flush=True)
+            print(f"[bench] cpu reference: restart after step {warmup}, {steps} step(s)", file=sys.stderr,
+                  flush=True)
             t0 = time.perf_counter()
             subprocess.run(cmd, cwd=d, check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
-            times[n] = time.perf_counter() - t0
-        dt = times[2] - times[1]
-        return {"value": NX * NY * 1 / dt / 1e6, "unit": "Mcell·steps/s", "cores": 1, "kind": "reference",
-                "sample": f"reference serial build, step 2 of the same {NX}x{NY} case from a uniform start "
-                          f"(2-step run {times[2]:.2f} s minus 1-step run {times[1]:.2f} s, setup excluded), "
-                          f"1 of {os.cpu_count()} host cores"}
+            times[steps] = time.perf_counter() - t0
+            # a restarted run appends its rows without the header line
+            rows = open(os.path.join(d, "Iterations_in_loops.txt")).read().split("\n")
+            iters[steps] = [int(r.split()[2]) for r in rows if r.strip() and not r.startswith("it")]
+        dt = times[1 + n] - times[1]
+        its = iters[1 + n][1:]
+        if len(its) != n:
+            raise RuntimeError(f"the restarted reference ran {len(iters[1 + n])} steps, not {1 + n}")
+        return {"value": NX * NY * n / dt / 1e6, "unit": "Mcell·steps/s", "cores": 1, "kind": "reference",
+                "outer_iterations_per_step": sum(its) / len(its), "steps": [warmup + 2, warmup + 1 + n],
+                "sample": f"reference serial build restarted from the GPU program's (bit-identical) state after step "
+                          f"{warmup} of the same {NX}x{NY} case: steps {warmup + 2}-{warmup + 1 + n} "
+                          f"({'+'.join(map(str, its))} outer iterations), the {1 + n}-step run {times[1 + n]:.2f} s "
+                          f"minus the 1-step run {times[1]:.2f} s (setup excluded), 1 of {os.cpu_count()} host "
+                          f"cores"}
     finally:
         shutil.rmtree(work, ignore_errors=True)
 
@@ -303,6 +364,47 @@ def run_batch_workload(n, steps, warmup, mc):
 
 def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def roofline(t, na, measured):
+    """Roofline figures of the plain FP64 program's timed steps from its
+    per-kernel HIP-event totals; measured = (traffic per launch, counter
+    scales) or (None, None)."""
+    ks = t.get("kernels", {})
+    if not ks.get("assemble_inl", {}).get("launches") or not ks.get("sweep", {}).get("launches"):
+        return None
+    traffic, factors = measured if measured else (None, None)
+    traffic = traffic or {}
+
+    def entry(name, bytes_per_cell, seconds, moved):
+        achieved = bytes_per_cell * na / seconds / 1e9
+        e = {"achieved": achieved, "frac": achieved / HBM_PEAK_GBS, "bytes_per_launch": bytes_per_cell * na,
+             "avg_launch_us": seconds * 1e6, "traffic": moved}
+        if moved:
+            e["frac_measured"] = moved / seconds / 1e9 / HBM_PEAK_GBS
+        return e
+
+    a = ks["assemble_inl"]
+    w = ks["sweep"]
+    ta = a["ms"] / a["launches"] / 1e3
+    tw = w["ms"] / w["launches"] / 1e3
+    asm = entry("assemble_inl", ASSEMBLE_BYTES_PER_CELL, ta, traffic.get("assemble_inl"))
+    swp = entry("sweep", SWEEP2_BYTES_PER_CELL, tw, traffic.get("sweep"))
+    both = None if not traffic else traffic["assemble_inl"] + traffic["sweep"]
+    agg = entry("jacobi", ASSEMBLE_BYTES_PER_CELL + SWEEP2_BYTES_PER_CELL, ta + tw, both)
+    roof = {"bound": "hbm", "achieved": asm["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": asm["frac"], "traffic": asm["traffic"], "frac_measured": asm.get("frac_measured"),
+            "kernel": "k_assemble_inl (FP64 coefficient assembly of the loops: u, v pseudo-velocities and energy "
+                      "source, the largest share of the outer iteration's time)",
+            "bytes_per_launch": asm["bytes_per_launch"], "avg_launch_us": asm["avg_launch_us"],
+            "launches": a["launches"], "counter_scale": factors,
+            "k_sweep2": dict(swp, kernel="fused pass of two p/T Jacobi sweeps, distinct arrays read and written "
+                                         "once", launches=w["launches"]),
+            "jacobi_sweeps": dict(agg, kernel="k_assemble_inl + k_sweep2 per outer iteration (A2 + A3 + A4 + "
+                                              "loop 3 at N_I_p_c = 2)"),
+            "cache_note": "the 1920x384 FP64 state (about 212 MB) fits the 256 MB last-level cache: traffic is "
+                          "counted memory traffic, part of which that cache may serve instead of HBM"}
+    return roof
 
 
 def main():
@@ -417,17 +519,14 @@ def main():
     base = max(fp64_value, paths["plain"]["value"]) if "plain" in paths else fp64_value
     for name in paths:
         paths[name]["speedup_vs_fp64"] = paths[name]["value"] / base
-    roof = None
-    if t.get("kernel_launches"):
-        avg_s = t["kernel_ms"] / t["kernel_launches"] / 1e3
-        achieved = 2 * SWEEP_BYTES_PER_CELL * na / avg_s / 1e9
-        traffic, factors = (None, None) if args.no_traffic else measure_traffic()
-        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "counter_scale": factors,
-                "kernel": "k_sweep2 (FP64 fused pass of two p/T Jacobi sweeps)",
-                "bytes_per_launch": 2 * SWEEP_BYTES_PER_CELL * na, "avg_launch_us": avg_s * 1e6,
-                "launches": t["kernel_launches"]}
-    cpu = None if args.no_cpu else run_cpu_baseline()
+    roof = roofline(t, na, None if args.no_traffic else measure_traffic())
+    cpu = None if args.no_cpu else run_cpu_baseline(t["warmup_steps"])
+    if cpu is not None:
+        # the GPU's FP64 program over its whole timed window, per outer iteration, beside the CPU's
+        gpu_it = t["timed_outer_iterations"] / t["timed_steps"]
+        cpu["gpu_fp64_outer_iterations_per_step"] = gpu_it
+        cpu["gpu_fp64_vs_cpu_per_outer_iteration"] = (
+            (fp64_value / gpu_it) / (cpu["value"] / cpu["outer_iterations_per_step"]))
     path_desc = {"fp64": "plain FP64",
                  "accelerated": f"start predicted by degree-{AA_DEGREE} extrapolation in time (every cell, derived "
                                 f"fields rebuilt), guarded, + Anderson acceleration of the FP64 outer iterations "

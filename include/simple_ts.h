@@ -276,15 +276,20 @@ int sts_drag_sums(sts_ctx *ctx, int body, double *sums);
 /* The HIP stream all work of the context is issued on (diagnostics). */
 int sts_stream_info(sts_ctx *ctx, void **hip_stream);
 
-/* Kernel timing (diagnostics, no counterpart in the reference): while
-   enabled, every launch of the chosen kernel is bracketed by HIP events on
-   the context's stream; sts_kernel_time reports the launches and their summed
-   duration since it was enabled.  kernel: STS_K_* for the FP64 kernel,
-   STS_K_* + STS_K_COUNT for its FP32 instance, or -1 to disable. */
+/* Kernel timing (diagnostics, no counterpart in the reference): every launch
+   of a kernel in the timed set is bracketed by HIP events on the context's
+   stream; launches that a device-side loop gates out (outer iterations
+   enqueued past the iteration where the loop stopped, which return at once)
+   are not counted.  sts_time_kernel adds `kernel` to the set and clears its
+   totals (-1: empties the set); sts_kernel_time reports the counted launches
+   of `kernel` and their summed duration.  kernel: STS_K_* for the FP64
+   kernel, STS_K_* + STS_K_COUNT for its FP32 instance.  STS_K_ASSEMBLE is
+   the assembly that reads Gamma_* and D_*, STS_K_ASSEMBLE_INL the one inside
+   the loops that forms them per face; STS_K_AA_* are the Anderson kernels. */
 enum sts_kernel { STS_K_ASSEMBLE = 0, STS_K_SWEEP, STS_K_CORRECT, STS_K_WALL, STS_K_EDGES, STS_K_FLUX_DIFF,
-                  STS_K_COUNT };
+                  STS_K_ASSEMBLE_INL, STS_K_AA_STORE, STS_K_AA_MIX, STS_K_COUNT };
 int sts_time_kernel(sts_ctx *ctx, int kernel);
-int sts_kernel_time(sts_ctx *ctx, long long *launches, double *total_ms);
+int sts_kernel_time(sts_ctx *ctx, int kernel, long long *launches, double *total_ms);
 
 #ifdef __cplusplus
 }

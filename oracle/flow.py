@@ -31,7 +31,7 @@ class Run:
         self.fields = {}
 
 
-def run_case(case_dir, max_steps=None, on_step=None):
+def run_case(case_dir, max_steps=None, on_step=None, on_iteration=None):
     c = sts.Case(case_dir)
     prob = c.problem()
     o = ora.Oracle(prob)
@@ -138,6 +138,8 @@ def run_case(case_dir, max_steps=None, on_step=None):
             o.boundary_conditions(8 | 16 | 32)
             o.fluxes_diffusion()
             Iter += 1
+            if on_iteration:
+                on_iteration(it_counter, Iter, o)
             rp, rT = rpt[-1]
             step_res.append((ruv[0], ruv[1], rp, rT, rpt[0][0], rpt[0][1]))
             if V("ToSolveContinuityEquation"):  # S:4586-4589 (fluxes recomputed first)

@@ -190,7 +190,7 @@ def lib():
             ("sts_time_step_ws", [vp, vp, ctypes.POINTER(UNetNorm), ctypes.POINTER(LoopControl),
                                   ctypes.POINTER(MixedControl), ctypes.POINTER(c_int), ctypes.POINTER(c_int),
                                   ctypes.POINTER(c_int), _DP, _DP]),
-            ("sts_kernel_time", [vp, ctypes.POINTER(ctypes.c_longlong), _DP]),
+            ("sts_kernel_time", [vp, c_int, ctypes.POINTER(ctypes.c_longlong), _DP]),
         ]:
             fn = getattr(L, name)
             fn.argtypes = args

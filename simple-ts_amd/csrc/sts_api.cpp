@@ -40,6 +40,7 @@ constexpr int NRED = 2 + 2 * MAX_SWEEPS;
 
 struct sts_ctx {
   int Nx = 0, Ny = 0, Na = 0;
+  int device = -1;  // the HIP device current at sts_create
   hipStream_t stream = nullptr;
   KGrid g{};
   double *hx = nullptr, *hy = nullptr;
@@ -103,12 +104,21 @@ struct sts_ctx {
   int aa_cap = 0;
   AABuf aa{};
   bool fuse_sweeps = true;  // pairs of loop-3 sweeps in one k_sweep2 pass
-  // kernel timing: event pairs around launches of kernel `timed`
-  int timed = -1;
+  // kernel timing: event pairs around the launches of the kernels in the set
+  // `timed` (bit per kernel id); a pair launched as outer iteration k of a
+  // device-side loop is dropped once that loop has stopped before k (such a
+  // launch returns at once)
+  unsigned timed = 0;
+  struct TimedPair {
+    int kernel, loop, k;
+  };
   std::vector<hipEvent_t> ev;  // 2 per pending launch
+  std::vector<TimedPair> ev_tag;
   int ev_used = 0;
-  long long timed_launches = 0;
-  double timed_ms = 0;
+  int loop_seq = 0;
+  bool loop_open = false;
+  long long timed_launches[2 * STS_K_COUNT] = {};
+  double timed_ms[2 * STS_K_COUNT] = {};
 
   KFields fields() const {
     KFields k;
@@ -127,55 +137,94 @@ extern "C" const char *sts_last_error(void) { return g_err.c_str(); }
 namespace {
 constexpr int EV_POOL = 4096;
 
-// Folds the durations of completed event pairs into the totals.
+// Folds the durations of completed event pairs into the per-kernel totals.
 int drain_events(sts_ctx *c) {
   if (c->ev_used == 0) return STS_OK;
   HIP_OK(hipEventSynchronize(c->ev[2 * (c->ev_used - 1) + 1]));
   for (int k = 0; k < c->ev_used; k++) {
+    const int id = c->ev_tag[k].kernel;
+    if (id < 0) continue;
     float ms = 0;
     HIP_OK(hipEventElapsedTime(&ms, c->ev[2 * k], c->ev[2 * k + 1]));
-    c->timed_ms += ms;
-    c->timed_launches++;
+    c->timed_ms[id] += ms;
+    c->timed_launches[id]++;
   }
   c->ev_used = 0;
   return STS_OK;
 }
 
-// Records the start event when kernel `id` is timed; returns the pair slot or -1.
-int timed_begin(sts_ctx *c, int id) {
-  if (c->timed != id) return -1;
-  if (c->ev_used == EV_POOL && drain_events(c)) return -1;
-  const int k = c->ev_used++;
-  (void)hipEventRecord(c->ev[2 * k], c->stream);
-  return k;
+int grow_events(sts_ctx *c, int pairs) {
+  const size_t old = c->ev.size();
+  c->ev.resize(old + 2 * (size_t)pairs);
+  c->ev_tag.resize(c->ev.size() / 2);
+  // timing only: no system-scope fence (cache writeback) at each record
+  for (size_t q = old; q < c->ev.size(); q++)
+    HIP_OK(hipEventCreateWithFlags(&c->ev[q], hipEventDisableSystemFence));
+  return STS_OK;
 }
 
-void timed_end(sts_ctx *c, int k) {
-  if (k >= 0) (void)hipEventRecord(c->ev[2 * k + 1], c->stream);
+// Records the start event when kernel `id` is timed; returns the pair slot or
+// -1.  k: the launch's outer iteration inside an open device-side loop, or
+// -1 for a launch that always runs.
+int timed_begin(sts_ctx *c, int id, int k = -1) {
+  if (!(c->timed >> id & 1u)) return -1;
+  if (2 * c->ev_used == (int)c->ev.size()) {
+    // while a loop is open its pairs are kept until its stop is known
+    if (c->loop_open ? grow_events(c, EV_POOL) : drain_events(c)) return -1;
+  }
+  const int q = c->ev_used++;
+  c->ev_tag[q] = sts_ctx::TimedPair{id, k >= 0 ? c->loop_seq : -1, k};
+  (void)hipEventRecord(c->ev[2 * q], c->stream);
+  return q;
+}
+
+void timed_end(sts_ctx *c, int q) {
+  if (q >= 0) (void)hipEventRecord(c->ev[2 * q + 1], c->stream);
+}
+
+// A device-side loop opens; when it closes at iteration `stop`, the pairs of
+// its launches past the stop are dropped.
+void timed_loop_open(sts_ctx *c) {
+  c->loop_seq++;
+  c->loop_open = true;
+}
+void timed_loop_close(sts_ctx *c, int stop) {
+  for (int q = 0; q < c->ev_used; q++) {
+    sts_ctx::TimedPair &t = c->ev_tag[q];
+    if (t.loop == c->loop_seq && t.k > stop) t.kernel = -1;
+  }
+  c->loop_open = false;
 }
 }  // namespace
 
 extern "C" int sts_time_kernel(sts_ctx *c, int kernel) {
   if (!c || kernel < -1 || kernel >= 2 * STS_K_COUNT) return fail(STS_ERR_ARG, "bad kernel id");
   if (c->ev.empty()) {
-    c->ev.resize(2 * EV_POOL);
-    // timing only: no system-scope fence (cache writeback) at each record
-    for (hipEvent_t &e : c->ev) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+    int rc = grow_events(c, EV_POOL);
+    if (rc) return rc;
   }
   int rc = drain_events(c);
   if (rc) return rc;
-  c->timed = kernel;
-  c->timed_launches = 0;
-  c->timed_ms = 0;
+  if (kernel < 0) {
+    c->timed = 0;
+    for (int q = 0; q < 2 * STS_K_COUNT; q++) {
+      c->timed_launches[q] = 0;
+      c->timed_ms[q] = 0;
+    }
+    return STS_OK;
+  }
+  c->timed |= 1u << kernel;
+  c->timed_launches[kernel] = 0;
+  c->timed_ms[kernel] = 0;
   return STS_OK;
 }
 
-extern "C" int sts_kernel_time(sts_ctx *c, long long *launches, double *total_ms) {
-  if (!c) return fail(STS_ERR_ARG, "null context");
+extern "C" int sts_kernel_time(sts_ctx *c, int kernel, long long *launches, double *total_ms) {
+  if (!c || kernel < 0 || kernel >= 2 * STS_K_COUNT) return fail(STS_ERR_ARG, "bad kernel id");
   int rc = drain_events(c);
   if (rc) return rc;
-  if (launches) *launches = c->timed_launches;
-  if (total_ms) *total_ms = c->timed_ms;
+  if (launches) *launches = c->timed_launches[kernel];
+  if (total_ms) *total_ms = c->timed_ms[kernel];
   return STS_OK;
 }
 
@@ -294,6 +343,8 @@ static int alloc_state(sts_ctx *c, const sts_grid *gi, const std::vector<uint8_t
   HIP_OK(hipMemsetAsync(c->lred, 0, sizeof(unsigned long long) * NSLOT * SLOT_STRIDE, c->stream));
   HIP_OK(hipMalloc(&c->probe, sizeof(double)));
   HIP_OK(hipHostMalloc(&c->h_red, sizeof(unsigned long long) * (NRED + 1), hipHostMallocDefault));
+  HIP_OK(hipHostMalloc(&c->h_ctrl, sizeof(LoopCtrl), hipHostMallocDefault));
+  HIP_OK(hipEventCreateWithFlags(&c->ev_ctrl, hipEventDisableTiming));
   HIP_OK(hipMemcpyAsync(c->hx, gi->hx, sizeof(double) * gi->Nx, hipMemcpyHostToDevice, c->stream));
   HIP_OK(hipMemcpyAsync(c->hy, gi->hy, sizeof(double) * gi->Ny, hipMemcpyHostToDevice, c->stream));
   HIP_OK(hipMemcpyAsync(c->m, mask.data(), Na, hipMemcpyHostToDevice, c->stream));
@@ -414,6 +465,10 @@ extern "C" int sts_create(const sts_grid *gi, int precision, sts_ctx **out) {
       (gi->drhodt_xe && mask[top + (size_t)(gi->Nx_u[1] - 1)]))
     return fail(STS_ERR_UNSUPPORTED, "drhodt_durhodx_dvrhody edge condition on a fluid cell of the top row");
   sts_ctx *c = new sts_ctx();
+  if (hipGetDevice(&c->device) != hipSuccess) {
+    delete c;
+    return fail(STS_ERR_HIP, "hipGetDevice failed");
+  }
   for (uint8_t b : mask) c->fluid_cells += b;
   c->h_vol.assign(gi->vol_inf_fb, gi->vol_inf_fb + Na);
   c->Nx = gi->Nx;
@@ -827,9 +882,13 @@ static AASel anderson_select(const sts_ctx *c, int k, int m, int every) {
 static int enqueue_anderson(sts_ctx *c, const KFields &kf, int k, const AASel &sel) {
   const Gate gn{c->ctrl, k + 1};
   const AAFields af{{kf.f[STS_U], kf.f[STS_V], kf.f[STS_P], kf.f[STS_T]}};
+  int t = timed_begin(c, STS_K_AA_STORE, k + 1);
   HIP_OK(launch_aa_store(af, c->aa, sel, c->Na, c->stream, gn));
+  timed_end(c, t);
   if (sel.n == 0) return STS_OK;
+  t = timed_begin(c, STS_K_AA_MIX, k + 1);
   HIP_OK(launch_aa_mix(c->g, kf, af, c->aa, sel, c->stream, gn));
+  timed_end(c, t);
   return STS_OK;
 }
 
@@ -875,13 +934,14 @@ static int enqueue_outer(sts_ctx *c, const Loop2State<R> &st, int k, int n, Loop
   // after the loop's first flux pass the assembly forms the diffusion
   // coefficients from the inputs that pass saw; the pass itself writes only
   // the fluxes and thermal coefficients
-  int t = timed_begin(c, st.kid + STS_K_ASSEMBLE);
-  HIP_OK(launch_assemble(g, fields_at(par), c->stream, gate, k > 0 || st.inline0));
+  const bool inl = k > 0 || st.inline0;
+  int t = timed_begin(c, st.kid + (inl ? STS_K_ASSEMBLE_INL : STS_K_ASSEMBLE), k);
+  HIP_OK(launch_assemble(g, fields_at(par), c->stream, gate, inl));
   timed_end(c, t);
   for (int s = 0; s < n;) {
     KFieldsT<R> kf = fields_at(par);
     R *Pi = par ? p1 : p0, *Ti = par ? T1 : T0, *Po = par ? p0 : p1, *To = par ? T0 : T1;
-    t = timed_begin(c, st.kid + STS_K_SWEEP);
+    t = timed_begin(c, st.kid + STS_K_SWEEP, k);
     if (c->fuse_sweeps && s + 1 < n) {
       // sweep s (first?) and s+1 (last?)
       HIP_OK(launch_sweep2(g, kf, Pi, Ti, Po, To, s == 0 ? lr + 4 : nullptr, s == 0 ? lr + 5 : nullptr,
@@ -904,16 +964,16 @@ static int enqueue_outer(sts_ctx *c, const Loop2State<R> &st, int k, int n, Loop
     aa = st.aa_m > 0 && (k + 1) % st.aa_every == 0;
     if (aa) sel = anderson_select(c, k, st.aa_m, st.aa_every);
   }
-  t = timed_begin(c, st.kid + STS_K_CORRECT);
+  t = timed_begin(c, st.kid + STS_K_CORRECT, k);
   HIP_OK(launch_correct(g, kf, lr + 0, lr + 1, 1, c->stream, gate));
   timed_end(c, t);
-  t = timed_begin(c, st.kid + STS_K_WALL);
+  t = timed_begin(c, st.kid + STS_K_WALL, k);
   HIP_OK(launch_wall(g, kf, STS_BC_SLIP_U | STS_BC_SLIP_V | STS_BC_JUMP, c->stream, gate, c->wall_ops,
                      k == 0 ? c->n_wall_ops : c->n_wall_var));
   timed_end(c, t);
   lt.lred = lr;
   lt.log_k += 6 * (size_t)k;
-  t = timed_begin(c, st.kid + STS_K_EDGES);
+  t = timed_begin(c, st.kid + STS_K_EDGES, k);
   HIP_OK(launch_edges(g, kf, STS_BC_VELOCITIES | STS_BC_TEMPERATURE | STS_BC_PRESSURE, c->probe, c->stream, gate,
                       lt));
   timed_end(c, t);
@@ -926,7 +986,7 @@ static int enqueue_outer(sts_ctx *c, const Loop2State<R> &st, int k, int n, Loop
   // iteration (where the mix, gated on k + 1, does not run) run_loop2 runs
   // the pass once the loop has stopped
   if (aa && sel.n > 0) return STS_OK;
-  t = timed_begin(c, st.kid + STS_K_FLUX_DIFF);
+  t = timed_begin(c, st.kid + STS_K_FLUX_DIFF, k);
   HIP_OK(launch_flux_diff(g, kf, c->stream, gate, FLUX_PART));
   timed_end(c, t);
   return STS_OK;
@@ -941,6 +1001,7 @@ template <typename R>
 static int run_loop2(sts_ctx *c, Loop2State<R> &st, R *&p, R *&T, R *&p_alt, R *&T_alt, int n_pc, LoopTest lt,
                      int batch, int *iters) {
   HIP_OK(launch_loop_open(c->ctrl, lt.log_k, 6 * lt.N_I, c->lred, c->stream));
+  timed_loop_open(c);
   if constexpr (std::is_same<R, double>::value)
     if (st.aa_m > 0) {
       int r;
@@ -968,6 +1029,7 @@ static int run_loop2(sts_ctx *c, Loop2State<R> &st, R *&p, R *&T, R *&p_alt, R *
     else if (check == lt.N_I - 1) return fail(STS_ERR_ARG, "loop 2 did not stop at N_I");  // the test always stops there
     batch = 2;
   }
+  timed_loop_close(c, stop);
   *iters = stop + 1;
   if (((long long)*iters * sweep_passes(c, n_pc)) & 1) {  // odd number of passes: result in the partners
     std::swap(p, p_alt);
@@ -995,11 +1057,7 @@ static int ensure_loop_buffers(sts_ctx *c, int n_iter) {
     HIP_OK(hipHostMalloc(&c->h_log, sizeof(unsigned long long) * 6 * (size_t)n_iter, hipHostMallocDefault));
     c->log_cap = n_iter;
   }
-  if (!c->ctrl) {
-    HIP_OK(hipMalloc(&c->ctrl, sizeof(LoopCtrl)));
-    HIP_OK(hipHostMalloc(&c->h_ctrl, sizeof(LoopCtrl), hipHostMallocDefault));
-    HIP_OK(hipEventCreateWithFlags(&c->ev_ctrl, hipEventDisableTiming));
-  }
+  if (!c->ctrl) HIP_OK(hipMalloc(&c->ctrl, sizeof(LoopCtrl)));
   return STS_OK;
 }
 
@@ -1021,21 +1079,38 @@ static LoopTest loop_test(sts_ctx *c, unsigned long long *log, int N_I, double t
   return LoopTest{c->ctrl, c->lred, log, N_I, tol_uv, tol_p, tol_T, stall_w, stall_ratio};
 }
 
-extern "C" int sts_time_step(sts_ctx *c, const sts_loop_control *lc, int *iterations, double *res_log,
-                             double *p_probe) {
-  if (!c || !lc || lc->N_I < 1 || lc->N_I_p_c < 1 || lc->N_I_p_c > MAX_SWEEPS)
-    return fail(STS_ERR_ARG, "bad loop control");
+// Loop 1 body of the plain path: the prologue, then loop 2 on the device
+// until the reference's test stops it.
+static int plain_step(sts_ctx *c, const sts_loop_control *lc, int *iters) {
   int rc;
-  if ((rc = ensure_loop_buffers(c, lc->N_I))) return rc;
+  *iters = 0;
   if ((rc = sts_save_previous(c))) return rc;
   Loop2State<double> st{&c->g, c->fields(), c->p_alt, c->T_alt, 0};
   st.inline0 = c->diff_inline_ok;
   const int batch = lc->batch > 0 ? lc->batch : (c->last_iters > 2 ? c->last_iters - 1 : 1);
-  int iters = 0;
   rc = run_loop2(c, st, c->f[STS_P], c->f[STS_T], c->p_alt, c->T_alt, lc->N_I_p_c,
-                 loop_test(c, c->log, lc->N_I, lc->tol_uv, lc->tol_p, lc->tol_T, 0, 0.0), batch, &iters);
+                 loop_test(c, c->log, lc->N_I, lc->tol_uv, lc->tol_p, lc->tol_T, 0, 0.0), batch, iters);
   c->diff_inline_ok = c->diff_stale = true;
-  c->last_iters = iters;
+  c->last_iters = *iters;
+  // the step is not stored in the history of predicted starts, so the stored
+  // steps no longer precede the state evenly spaced in time
+  c->n_hist = 0;
+  return rc;
+}
+
+static int check_loop_control(const sts_loop_control *lc) {
+  if (!lc || lc->N_I < 1 || lc->N_I_p_c < 1 || lc->N_I_p_c > MAX_SWEEPS) return fail(STS_ERR_ARG, "bad loop control");
+  return STS_OK;
+}
+
+extern "C" int sts_time_step(sts_ctx *c, const sts_loop_control *lc, int *iterations, double *res_log,
+                             double *p_probe) {
+  if (!c) return fail(STS_ERR_ARG, "null context");
+  int rc;
+  if ((rc = check_loop_control(lc))) return rc;
+  if ((rc = ensure_loop_buffers(c, lc->N_I))) return rc;
+  int iters = 0;
+  rc = plain_step(c, lc, &iters);
   if (iterations) *iterations = iters;
   int rc2 = finish_step(c, iters, res_log, p_probe);
   return rc ? rc : rc2;

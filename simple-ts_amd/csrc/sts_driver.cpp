@@ -10,7 +10,7 @@
 // host only evaluates the loop-control scalars, as the reference's rank 0
 // does.
 //
-// Usage: sts_gpu [case_dir] [--timing file.json]
+// Usage: sts_gpu [case_dir] [--timing file.json] (--help lists the other options)
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -75,6 +75,7 @@ std::string num(double v) {  // toString (stringConv.h): default stream format
 struct Driver {
   sts::Case c;
   std::string dir;
+  std::string norms_file;  // --continuity-norms: norms of every continuity evaluation
   sts_ctx *ctx = nullptr;
   sts_grid grid{};
   int Nx = 0, Ny = 0;
@@ -198,6 +199,10 @@ struct Driver {
     std::vector<double> cs(Na), cf(Na);
     double norms[3];
     ck(sts_continuity(ctx, cs.data(), cf.data(), norms), "sts_continuity");
+    if (!norms_file.empty()) {  // not a reference output: one line per evaluation
+      std::ofstream o(norms_file, std::ios::app);
+      o << std::setprecision(17) << it_counter << ' ' << norms[0] << ' ' << norms[1] << ' ' << norms[2] << '\n';
+    }
     if (c.flag("ToWriteSolvedDataToBinaryFile")) {
       write_bin(path("ContinuityEquation_stationary.bin"), cs.data(), Na);
       write_bin(path("ContinuityEquation.bin"), cf.data(), Na);
@@ -574,7 +579,8 @@ struct Driver {
     timed_seconds = std::chrono::duration<double>(t_end - t_timed).count();
     timed_steps = it_counter - warmup_steps;
     timed_outer = outer_total - outer_at_start;
-    if (time_kernel >= 0) ck(sts_kernel_time(ctx, &kernel_launches, &kernel_ms), "sts_kernel_time");
+    for (size_t q = 0; q < time_kernels.size(); q++)
+      ck(sts_kernel_time(ctx, time_kernels[q], &kernel_launches[q], &kernel_ms[q]), "sts_kernel_time");
     solve_is_finished = Iter <= 1 && Iter_p_c <= 1 && max_res_u < c.get("MaxError_to_stop_program") &&
                         max_res_v < c.get("MaxError_to_stop_program") &&
                         max_res_p[Iter - 1] < c.get("MaxError_to_stop_program") &&
@@ -594,13 +600,18 @@ struct Driver {
   long long outer_total32 = 0, outer32_at_start = 0;
   // timing of the steps after the warm-up ones (benchmarking)
   unsigned warmup_steps = 0;
-  int time_kernel = -1;
-  double timed_seconds = 0, kernel_ms = 0;
+  std::vector<int> time_kernels;  // kernel ids timed with HIP events
+  std::vector<double> kernel_ms;
+  std::vector<long long> kernel_launches;
+  double timed_seconds = 0;
   unsigned timed_steps = 0;
-  long long timed_outer = 0, kernel_launches = 0;
+  long long timed_outer = 0;
 
   void start_timing(std::chrono::steady_clock::time_point &t, long long &outer0) {
-    if (time_kernel >= 0) ck(sts_time_kernel(ctx, time_kernel), "sts_time_kernel");
+    ck(sts_time_kernel(ctx, -1), "sts_time_kernel");
+    for (int k : time_kernels) ck(sts_time_kernel(ctx, k), "sts_time_kernel");
+    kernel_ms.assign(time_kernels.size(), 0.0);
+    kernel_launches.assign(time_kernels.size(), 0);
     t = std::chrono::steady_clock::now();
     outer0 = outer_total;
     outer32_at_start = outer_total32;
@@ -610,11 +621,13 @@ struct Driver {
 }  // namespace
 
 int main(int argc, char **argv) {
-  std::string dir = ".", timing;
+  std::string dir = ".", timing, norms_file;
   unsigned warmup = 0;
-  int kernel = -1;
-  const char *knames[] = {"assemble",   "sweep",   "correct",   "wall",   "edges",   "flux_diff",
-                          "assemble32", "sweep32", "correct32", "wall32", "edges32", "flux_diff32"};
+  std::vector<int> kernels;
+  const char *knames[2 * STS_K_COUNT] = {
+      "assemble",   "sweep",      "correct", "wall",      "edges",  "flux_diff", "assemble_inl",   "aa_store",
+      "aa_mix",     "assemble32", "sweep32", "correct32", "wall32", "edges32",   "flux_diff32", "assemble_inl32",
+      "aa_store32", "aa_mix32"};
   bool mixed = false;
   sts_mixed_control mc{0, 1e-6, 1e-6, 1e-6, 4, 0.7, 0, 0, 0, 0, 0};
   int n32 = -1;
@@ -623,10 +636,15 @@ int main(int argc, char **argv) {
     if (s == "--timing" && a + 1 < argc) timing = argv[++a];
     else if (s == "--warmup" && a + 1 < argc) warmup = (unsigned)std::atoi(argv[++a]);
     else if (s == "--time-kernel" && a + 1 < argc) {
-      std::string k = argv[++a];
-      for (int q = 0; q < 2 * STS_K_COUNT; q++)
-        if (k == knames[q]) kernel = q;
-      if (kernel < 0) die("unknown kernel " + k);
+      std::stringstream list(argv[++a]);
+      std::string k;
+      while (std::getline(list, k, ',')) {
+        int id = -1;
+        for (int q = 0; q < 2 * STS_K_COUNT; q++)
+          if (k == knames[q]) id = q;
+        if (id < 0) die("unknown kernel " + k);
+        kernels.push_back(id);
+      }
     } else if (s == "--mixed") {
       mixed = true;
     } else if (s == "--n32" && a + 1 < argc) {
@@ -641,6 +659,8 @@ int main(int argc, char **argv) {
       mc.anderson = std::atoi(argv[++a]);
     } else if (s == "--anderson-every" && a + 1 < argc) {
       mc.anderson_every = std::atoi(argv[++a]);
+    } else if (s == "--continuity-norms" && a + 1 < argc) {
+      norms_file = argv[++a];
     } else if (s == "--tol32" && a + 1 < argc) {
       mc.tol32_uv = mc.tol32_p = mc.tol32_T = std::atof(argv[++a]);
     } else if (s == "--stall" && a + 2 < argc) {
@@ -648,17 +668,21 @@ int main(int argc, char **argv) {
       mc.stall_ratio = std::atof(argv[++a]);
     } else if (s == "-h" || s == "--help") {
       std::cout << "usage: sts_gpu [case_dir] [--timing file.json] [--warmup steps] [--time-kernel "
-                   "assemble|sweep|correct|wall|edges|flux_diff[32]] [--extrapolate degree] [--mixed [--n32 cap] "
+                   "name[,name...]] (assemble, assemble_inl, sweep, correct, wall, edges, flux_diff, aa_store, "
+                   "aa_mix; FP32 instances with suffix 32) [--extrapolate degree] [--mixed [--n32 cap] "
                    "[--tol32 tol] [--stall window ratio] [--fp32-above iterations]] [--guard max_pause] "
-                   "[--anderson depth [--anderson-every iterations]]"
+                   "[--anderson depth [--anderson-every iterations]] [--continuity-norms file]"
                 << std::endl;
       return 0;
     } else dir = s;
   }
   Driver d;
   d.dir = dir;
+  d.norms_file = norms_file;
   d.warmup_steps = warmup;
-  d.time_kernel = kernel;
+  d.time_kernels = kernels;
+  d.kernel_ms.assign(kernels.size(), 0.0);
+  d.kernel_launches.assign(kernels.size(), 0);
   d.mixed = mixed;
   mc.N_I32 = mixed ? n32 : 0;
   d.mixed = mixed || mc.extrapolate > 0 || mc.anderson > 0;
@@ -672,8 +696,11 @@ int main(int argc, char **argv) {
       << ", \"outer_iterations\": " << d.outer_total << ", \"loop_seconds\": " << d.loop_seconds
       << ", \"warmup_steps\": " << d.warmup_steps << ", \"timed_steps\": " << d.timed_steps
       << ", \"timed_outer_iterations\": " << d.timed_outer << ", \"timed_seconds\": " << d.timed_seconds
-      << ", \"kernel\": \"" << (kernel >= 0 ? knames[kernel] : "") << "\", \"kernel_launches\": "
-      << d.kernel_launches << ", \"kernel_ms\": " << d.kernel_ms << ", \"mixed\": " << (mixed ? 1 : 0)
+      << ", \"kernels\": {";
+    for (size_t q = 0; q < kernels.size(); q++)
+      o << (q ? ", " : "") << "\"" << knames[kernels[q]] << "\": {\"launches\": " << d.kernel_launches[q]
+        << ", \"ms\": " << d.kernel_ms[q] << "}";
+    o << "}, \"mixed\": " << (mixed ? 1 : 0)
       << ", \"fp32_iterations\": " << d.outer_total32
       << ", \"timed_fp32_iterations\": " << d.outer_total32 - d.outer32_at_start << "}" << std::endl;
   }

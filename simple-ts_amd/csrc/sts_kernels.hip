@@ -1446,13 +1446,17 @@ __global__ void __launch_bounds__(BX *BY) k_extrapolate(KGrid g, KFields fl, Ext
   if (i >= g.Nx || j >= g.Ny) return;
   const unsigned k = (unsigned)i + (unsigned)j * (unsigned)g.Nx;
   const int src[4] = {STS_U_PR, STS_V_PR, STS_P_PR, STS_T_PR}, dst[4] = {STS_U, STS_V, STS_P, STS_T};
+  // the inflow columns i < Nx_u0 hold the given velocity, which no kernel of
+  // loop 2 rewrites and which the driver advances in time itself
+  // (BoundaryConditions_function_of_time, S:8102-8144): u keeps x^n there
+  const bool given_u = i < g.Nx_u0;
 #pragma unroll
   for (int q = 0; q < 4; q++) {
     double x[MAX_EXTRAP + 1];
     x[0] = fl.f[src[q]][k];
 #pragma unroll
     for (int h = 0; h < MAX_EXTRAP; h++) x[h + 1] = (h < order) ? hist.f[h][q][k] : 0.0;
-    if (order > 0) {
+    if (order > 0 && !(q == 0 && given_u)) {
       // a guess that is not finite, or a pressure or temperature that is not
       // positive, keeps the previous step's value in that cell
       const double e = extrap(order, x);
@@ -1968,25 +1972,35 @@ __global__ void __launch_bounds__(BX *BY) k_continuity(KGrid g, KFields fl, doub
       s1 = fabs(c);
     }
   }
-  __shared__ double sm[3][BX * BY];
-  const int t = threadIdx.x + threadIdx.y * BX;
-  sm[0][t] = amax;
-  sm[1][t] = s2;
-  sm[2][t] = s1;
-  __syncthreads();
-  for (int o = (BX * BY) / 2; o > 0; o >>= 1) {
-    if (t < o) {
-      sm[0][t] = fmax(sm[0][t], sm[0][t + o]);
-      sm[1][t] += sm[1][t + o];
-      sm[2][t] += sm[2][t + o];
-    }
-    __syncthreads();
+  // wavefront reductions by lane exchange, then one value per wave in LDS
+  // folded by the first wave
+  constexpr int NW = (BX * BY) / 64;
+  __shared__ double sm[3][NW];
+  const int t = threadIdx.x + threadIdx.y * BX, lane = t & 63, w = t >> 6;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    amax = fmax(amax, __shfl_xor(amax, o, 64));
+    s2 += __shfl_xor(s2, o, 64);
+    s1 += __shfl_xor(s1, o, 64);
   }
+  if (lane == 0) {
+    sm[0][w] = amax;
+    sm[1][w] = s2;
+    sm[2][w] = s1;
+  }
+  __syncthreads();
   if (t == 0) {
+    double a = sm[0][0], b2 = sm[1][0], b1 = sm[2][0];
+#pragma unroll
+    for (int q = 1; q < NW; q++) {
+      a = fmax(a, sm[0][q]);
+      b2 += sm[1][q];
+      b1 += sm[2][q];
+    }
     const int b = blockIdx.x + blockIdx.y * gridDim.x;
-    partial[3 * b] = sm[0][0];
-    partial[3 * b + 1] = sm[1][0];
-    partial[3 * b + 2] = sm[2][0];
+    partial[3 * b] = a;
+    partial[3 * b + 1] = b2;
+    partial[3 * b + 2] = b1;
   }
 }
 
