@@ -188,6 +188,22 @@ typedef struct sts_loop_control {
    outer iteration.  Returns STS_ERR_NONFINITE if a residual is not finite. */
 int sts_time_step(sts_ctx *ctx, const sts_loop_control *lc, int *iterations, double *res_log, double *p_probe);
 
+/* No counterpart in the reference: sts_time_step for callers that cannot
+   allocate memory (code under HIP graph capture, tight serving loops,
+   harnesses that forbid allocation).  Same arguments, arithmetic and results,
+   and no call of any allocation routine: the loop control, the residual
+   words, the probe and the residual log live in one fixed device array of the
+   library per GPU (about 260 KB, held until the process ends), which every
+   context on that GPU shares and which keeps the last call's contents (each
+   call clears what it reads before writing it).  Limits: N_I <=
+   STS_NOALLOC_MAX_N_I (else STS_ERR_ARG); the context's device must be the
+   current device.  Calls that overlap on one GPU (from several threads, or on
+   several contexts) race on that array.  sts_time_kernel does not time these
+   calls. */
+#define STS_NOALLOC_MAX_N_I 4096
+int sts_time_step_noalloc(sts_ctx *ctx, const sts_loop_control *lc, int *iterations, double *res_log,
+                          double *p_probe);
+
 /* Mixed-precision time step (no counterpart in the reference; needs a context
    created with precision 32).  After the prologue, loop 2 first runs on an
    FP32 copy of the state -- assembly, sweeps, correction, boundary conditions,

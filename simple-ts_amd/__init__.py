@@ -170,6 +170,7 @@ def lib():
             ("sts_set_inflow_u", [vp, c_double]), ("sts_continuity", [vp, _DP, _DP, _DP]),
             ("sts_outer_iteration", [vp, c_int, _DP, _DP, _DP]), ("sts_stream_info", [vp, ctypes.POINTER(vp)]),
             ("sts_time_step", [vp, ctypes.POINTER(LoopControl), ctypes.POINTER(c_int), _DP, _DP]),
+            ("sts_time_step_noalloc", [vp, ctypes.POINTER(LoopControl), ctypes.POINTER(c_int), _DP, _DP]),
             ("sts_time_step_mixed", [vp, ctypes.POINTER(LoopControl), ctypes.POINTER(MixedControl),
                                      ctypes.POINTER(c_int), ctypes.POINTER(c_int), _DP, _DP]),
             ("sts_time_kernel", [vp, c_int]),
@@ -304,6 +305,16 @@ class Context:
         it = ctypes.c_int()
         log, pr = np.zeros(6 * N_I), np.zeros(1)
         _check(lib().sts_time_step(self.h, ctypes.byref(lc), ctypes.byref(it), _dp(log), _dp(pr)), "sts_time_step")
+        return it.value, log[:6 * it.value].reshape(-1, 6), float(pr[0])
+
+    def time_step_noalloc(self, N_I, N_I_p_c, tol_uv, tol_p, tol_T, batch=0):
+        """time_step through sts_time_step_noalloc (no allocation; loop
+        buffers in the library's fixed per-GPU array, N_I <= 4096)."""
+        lc = LoopControl(N_I, N_I_p_c, tol_uv, tol_p, tol_T, batch)
+        it = ctypes.c_int()
+        log, pr = np.zeros(6 * N_I), np.zeros(1)
+        _check(lib().sts_time_step_noalloc(self.h, ctypes.byref(lc), ctypes.byref(it), _dp(log), _dp(pr)),
+               "sts_time_step_noalloc")
         return it.value, log[:6 * it.value].reshape(-1, 6), float(pr[0])
 
     def time_step_mixed(self, N_I, N_I_p_c, tol_uv, tol_p, tol_T, N_I32, tol32, stall_w=4, stall_ratio=0.7,

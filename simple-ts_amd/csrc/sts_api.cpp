@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -1114,6 +1115,80 @@ extern "C" int sts_time_step(sts_ctx *c, const sts_loop_control *lc, int *iterat
   if (iterations) *iterations = iters;
   int rc2 = finish_step(c, iters, res_log, p_probe);
   return rc ? rc : rc2;
+}
+
+// ---------------------------------------------------------------------------
+// The plain time step without allocation (sts_time_step_noalloc): its loop
+// control, residual words, probe and residual log live in `scratch`, one
+// fixed array per GPU that every context on that GPU shares.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr size_t SCR_CTRL = 0, SCR_PROBE = 256, SCR_LRED = 512;
+constexpr size_t SCR_LOG = SCR_LRED + sizeof(unsigned long long) * NSLOT * SLOT_STRIDE;
+constexpr size_t SCRATCH_BYTES = SCR_LOG + sizeof(unsigned long long) * 6 * (size_t)STS_NOALLOC_MAX_N_I;
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch[MAX_DEVICES] = {};  // device address of each GPU's copy
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+
+// The address of device dev's copy of scratch, looked up once with dev
+// current (the array's name is not a device address in host code).
+static int scratch_of(int dev, unsigned char **out) {
+  if (dev < 0 || dev >= MAX_DEVICES) return fail(STS_ERR_UNSUPPORTED, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void *p = nullptr;
+    HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(STS_ERR_HIP, "hipGetSymbolAddress gave a null address for scratch");
+    g_scratch[dev] = static_cast<unsigned char *>(p);
+  }
+  *out = g_scratch[dev];
+  return STS_OK;
+}
+
+extern "C" int sts_time_step_noalloc(sts_ctx *c, const sts_loop_control *lc, int *iterations, double *res_log,
+                                     double *p_probe) {
+  if (!c) return fail(STS_ERR_ARG, "null context");
+  int rc;
+  if ((rc = check_loop_control(lc))) return rc;
+  if (lc->N_I > STS_NOALLOC_MAX_N_I)
+    return fail(STS_ERR_ARG, "N_I = " + std::to_string(lc->N_I) + " exceeds STS_NOALLOC_MAX_N_I = " +
+                                 std::to_string(STS_NOALLOC_MAX_N_I) + " of the allocation-free time step");
+  int dev = -1;
+  HIP_OK(hipGetDevice(&dev));
+  if (dev != c->device) return fail(STS_ERR_ARG, "the context's device must be the current device");
+  unsigned char *scr = nullptr;
+  if ((rc = scratch_of(dev, &scr))) return rc;
+  // the context's own loop buffers are set aside for the call; timing is off
+  // (its event pool may grow)
+  LoopCtrl *ctrl0 = c->ctrl;
+  double *probe0 = c->probe;
+  unsigned long long *lred0 = c->lred, *log0 = c->log;
+  const unsigned timed0 = c->timed;
+  c->ctrl = reinterpret_cast<LoopCtrl *>(scr + SCR_CTRL);
+  c->probe = reinterpret_cast<double *>(scr + SCR_PROBE);
+  c->lred = reinterpret_cast<unsigned long long *>(scr + SCR_LRED);
+  c->log = reinterpret_cast<unsigned long long *>(scr + SCR_LOG);
+  c->timed = 0;
+  int iters = 0;
+  rc = plain_step(c, lc, &iters);
+  // the log words are the residuals' IEEE bit patterns, copied as doubles
+  hipError_t e = hipSuccess;
+  if (iters > 0 && res_log)
+    e = hipMemcpyAsync(res_log, c->log, sizeof(double) * 6 * (size_t)iters, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess && p_probe) e = hipMemcpyAsync(p_probe, c->probe, sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  c->ctrl = ctrl0;
+  c->probe = probe0;
+  c->lred = lred0;
+  c->log = log0;
+  c->timed = timed0;
+  if (iterations) *iterations = iters;
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(STS_ERR_HIP, std::string("sts_time_step_noalloc copies: ") + hipGetErrorString(e));
+  return STS_OK;
 }
 
 // Fields an outer iteration reads before it writes them: the FP32 phase hands
