@@ -246,6 +246,10 @@ typedef struct sts_mixed_control {
                                          still judges each outer iteration's own change, and the
                                          iteration that meets it is the state the step keeps.  0: off */
   int anderson_every;                 /* period of the Anderson step in outer iterations (0 or 1: every) */
+  int anderson32;                     /* m = 1..8: the same acceleration of the FP32 phase (its iterates
+                                         and mixes in FP32, the acceleration's sums in FP64); the FP64
+                                         phase starts its own history.  0: off.  Batched steps run the
+                                         FP32 phase without it. */
 } sts_mixed_control;
 int sts_time_step_mixed(sts_ctx *ctx, const sts_loop_control *lc, const sts_mixed_control *mc, int *iters32,
                         int *iters64, double *res_log, double *p_probe);

@@ -83,7 +83,7 @@ class MixedControl(ctypes.Structure):
     _fields_ = [("N_I32", ctypes.c_int), ("tol32_uv", ctypes.c_double), ("tol32_p", ctypes.c_double),
                 ("tol32_T", ctypes.c_double), ("stall_w", ctypes.c_int), ("stall_ratio", ctypes.c_double),
                 ("extrapolate", ctypes.c_int), ("fp32_above", ctypes.c_int), ("guard", ctypes.c_int),
-                ("anderson", ctypes.c_int), ("anderson_every", ctypes.c_int)]
+                ("anderson", ctypes.c_int), ("anderson_every", ctypes.c_int), ("anderson32", ctypes.c_int)]
 
 
 class UNetNorm(ctypes.Structure):
@@ -318,18 +318,18 @@ class Context:
         return it.value, log[:6 * it.value].reshape(-1, 6), float(pr[0])
 
     def time_step_mixed(self, N_I, N_I_p_c, tol_uv, tol_p, tol_T, N_I32, tol32, stall_w=4, stall_ratio=0.7,
-                        batch=0, extrapolate=0, fp32_above=0, guard=0, anderson=0, anderson_every=1):
+                        batch=0, extrapolate=0, fp32_above=0, guard=0, anderson=0, anderson_every=1, anderson32=0):
         """Time step through sts_time_step_mixed: an FP32 phase (N_I32 > 0;
         needs precision=32), then the FP64 phase with the reference's test.
         tol32 = (uv, p, T) tolerances of the FP32 phase; extrapolate = degree
         of the predicted start, fp32_above = FP32 gate, guard = restart and
         pause bound of predicted starts, anderson = history depth of the
         Anderson acceleration of the FP64 outer iterations (every
-        anderson_every of them).  Returns (FP32 iterations, FP64 iterations,
+        anderson_every of them), anderson32 = the same for the FP32 phase.  Returns (FP32 iterations, FP64 iterations,
         residual log [n32 + n64, 6], p probe)."""
         lc = LoopControl(N_I, N_I_p_c, tol_uv, tol_p, tol_T, batch)
         mc = MixedControl(N_I32, tol32[0], tol32[1], tol32[2], stall_w, stall_ratio, extrapolate, fp32_above, guard,
-                          anderson, anderson_every)
+                          anderson, anderson_every, anderson32)
         n32, n64 = ctypes.c_int(), ctypes.c_int()
         log, pr = np.zeros(6 * (N_I + N_I32)), np.zeros(1)
         _check(lib().sts_time_step_mixed(self.h, ctypes.byref(lc), ctypes.byref(mc), ctypes.byref(n32),
@@ -346,7 +346,7 @@ class Context:
         if mixed is not None:
             n_i32, tol32, w, r = mixed[:4]
             mc = MixedControl(n_i32, tol32[0], tol32[1], tol32[2], w, r,
-                              *(mixed[q] if len(mixed) > q else 0 for q in range(4, 9)))
+                              *(mixed[q] if len(mixed) > q else 0 for q in range(4, 10)))
         used, n32, n64 = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         log, pr = np.zeros(6 * (N_I + n_i32)), np.zeros(1)
         _check(lib().sts_time_step_ws(self.h, net.h, ctypes.byref(norm), ctypes.byref(lc),
@@ -402,7 +402,7 @@ class Batch:
         if mixed is not None:
             n_i32, tol32, w, r = mixed[:4]
             mc = MixedControl(n_i32, tol32[0], tol32[1], tol32[2], w, r,
-                              *(mixed[q] if len(mixed) > q else 0 for q in range(4, 9)))
+                              *(mixed[q] if len(mixed) > q else 0 for q in range(4, 10)))
         a, b = np.zeros(n, np.int32), np.zeros(n, np.int32)
         _check(lib().sts_batch_time_step(self.h, ctypes.byref(lc), ctypes.byref(mc) if mc is not None else None,
                                          a.ctypes.data_as(_IP), b.ctypes.data_as(_IP)), "sts_batch_time_step")

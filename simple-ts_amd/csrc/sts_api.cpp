@@ -46,6 +46,7 @@ struct sts_ctx {
   KGrid g{};
   double *hx = nullptr, *hy = nullptr;
   uint8_t *m = nullptr;
+  uint16_t *nb = nullptr;  // neighbour fluid bits (NB_*), from the mask
   uint32_t *vol = nullptr;
   DevBody *bodies = nullptr;
   double *f[STS_NFIELDS] = {};
@@ -301,6 +302,7 @@ extern "C" int sts_destroy(sts_ctx *c) {
   (void)hipFree(c->hx);
   (void)hipFree(c->hy);
   (void)hipFree(c->m);
+  (void)hipFree(c->nb);
   (void)hipFree(c->vol);
   (void)hipFree(c->bodies);
   (void)hipFree(c->red);
@@ -319,6 +321,43 @@ extern "C" int sts_destroy(sts_ctx *c) {
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return STS_OK;
+}
+
+// The neighbour fluid bits of every cell (KGridT::nb): the mask at the
+// neighbour indices of Nbr -- clamped at the grid edge, x wrapping when
+// periodic -- and at the v equation's second south row.
+static std::vector<uint16_t> neighbour_bits(const std::vector<uint8_t> &mask, int Nx, int Ny, bool px) {
+  std::vector<uint16_t> out((size_t)Nx * (size_t)Ny);
+  for (int j = 0; j < Ny; j++) {
+    const int js = j - (0 < j), jn = j + (j < Ny - 1), jss = j >= 2 ? j - 2 : js;
+    for (int i = 0; i < Nx; i++) {
+      int iw, iww, ie;
+      if (px) {
+        iw = i == 0 ? Nx - 1 : i - 1;
+        iww = i == 1 ? Nx - 1 : (i == 0 ? Nx - 2 : i - 2);
+        ie = i == Nx - 1 ? 0 : i + 1;
+      } else {
+        iw = i - (0 < i);
+        iww = i - (0 < i) - (1 < i);
+        ie = i + (i < Nx - 1);
+      }
+      auto f = [&](int x, int y) { return mask[(size_t)x + (size_t)y * Nx] != 0; };
+      unsigned b = 0;
+      b |= f(i, j) ? NB_C : 0u;
+      b |= f(iw, j) ? NB_W : 0u;
+      b |= f(ie, j) ? NB_E : 0u;
+      b |= f(iww, j) ? NB_WW : 0u;
+      b |= f(i, js) ? NB_S : 0u;
+      b |= f(i, jn) ? NB_N : 0u;
+      b |= f(iw, js) ? NB_SW : 0u;
+      b |= f(ie, js) ? NB_SE : 0u;
+      b |= f(iw, jn) ? NB_NW : 0u;
+      b |= f(ie, jn) ? NB_NE : 0u;
+      b |= f(i, jss) ? NB_SS : 0u;
+      out[(size_t)i + (size_t)j * Nx] = (uint16_t)b;
+    }
+  }
+  return out;
 }
 
 static int alloc_state(sts_ctx *c, const sts_grid *gi, const std::vector<uint8_t> &mask) {
@@ -349,6 +388,9 @@ static int alloc_state(sts_ctx *c, const sts_grid *gi, const std::vector<uint8_t
   HIP_OK(hipMemcpyAsync(c->hx, gi->hx, sizeof(double) * gi->Nx, hipMemcpyHostToDevice, c->stream));
   HIP_OK(hipMemcpyAsync(c->hy, gi->hy, sizeof(double) * gi->Ny, hipMemcpyHostToDevice, c->stream));
   HIP_OK(hipMemcpyAsync(c->m, mask.data(), Na, hipMemcpyHostToDevice, c->stream));
+  const std::vector<uint16_t> nbits = neighbour_bits(mask, gi->Nx, gi->Ny, gi->periodic_x != 0);
+  HIP_OK(hipMalloc(&c->nb, sizeof(uint16_t) * Na));
+  HIP_OK(hipMemcpyAsync(c->nb, nbits.data(), sizeof(uint16_t) * Na, hipMemcpyHostToDevice, c->stream));
   HIP_OK(hipMemcpyAsync(c->vol, gi->vol_inf_fb, sizeof(uint32_t) * Na, hipMemcpyHostToDevice, c->stream));
   std::vector<DevBody> db(nb);
   for (int k = 0; k < gi->nbodies; k++) {
@@ -366,7 +408,7 @@ static void narrow_grid(sts_ctx *c) {
   KGridT<float> &h = c->g32;
   h.Nx = g.Nx; h.Ny = g.Ny;
   h.hx = c->hx32; h.hy = c->hy32;
-  h.m = g.m; h.vol = g.vol; h.bodies = g.bodies; h.nbodies = g.nbodies;
+  h.m = g.m; h.nb = g.nb; h.vol = g.vol; h.bodies = g.bodies; h.nbodies = g.nbodies;
   h.Nx_u0 = g.Nx_u0; h.Nx_u1 = g.Nx_u1; h.Ny_u0 = g.Ny_u0; h.Ny_u1 = g.Ny_u1;
   h.Nx_v0 = g.Nx_v0; h.Nx_v1 = g.Nx_v1; h.Ny_v0 = g.Ny_v0; h.Ny_v1 = g.Ny_v1;
   h.Nx_o0 = g.Nx_o0; h.Nx_o1 = g.Nx_o1; h.Ny_o0 = g.Ny_o0; h.Ny_o1 = g.Ny_o1;
@@ -507,6 +549,7 @@ extern "C" int sts_create(const sts_grid *gi, int precision, sts_ctx **out) {
   g.hx = c->hx;
   g.hy = c->hy;
   g.m = c->m;
+  g.nb = c->nb;
   g.vol = c->vol;
   g.bodies = c->bodies;
   g.nbodies = gi->nbodies;
@@ -849,7 +892,7 @@ struct Loop2State {
   KFieldsT<R> f;
   R *p_alt, *T_alt;
   int kid;  // timing id offset: 0 for FP64 kernels, STS_K_COUNT for FP32
-  int aa_m = 0, aa_every = 1;  // Anderson acceleration (FP64 only): history depth (0: off), period
+  int aa_m = 0, aa_every = 1;  // Anderson acceleration: history depth (0: off), period
   bool inline0 = false;        // the first iteration may form the diffusion coefficients itself
 };
 
@@ -880,15 +923,16 @@ static AASel anderson_select(const sts_ctx *c, int k, int m, int every) {
 // rest (local-Kn slip and jump, the rho u copies) the next iteration
 // restores, whose own change the test judges.  The assembly of the next
 // iteration forms Gamma and D from the mixed sqrt_T.
-static int enqueue_anderson(sts_ctx *c, const KFields &kf, int k, const AASel &sel) {
+template <typename R>
+static int enqueue_anderson(sts_ctx *c, const Loop2State<R> &st, const KFieldsT<R> &kf, int k, const AASel &sel) {
   const Gate gn{c->ctrl, k + 1};
-  const AAFields af{{kf.f[STS_U], kf.f[STS_V], kf.f[STS_P], kf.f[STS_T]}};
-  int t = timed_begin(c, STS_K_AA_STORE, k + 1);
+  const AAFieldsT<R> af{{kf.f[STS_U], kf.f[STS_V], kf.f[STS_P], kf.f[STS_T]}};
+  int t = timed_begin(c, st.kid + STS_K_AA_STORE, k + 1);
   HIP_OK(launch_aa_store(af, c->aa, sel, c->Na, c->stream, gn));
   timed_end(c, t);
   if (sel.n == 0) return STS_OK;
-  t = timed_begin(c, STS_K_AA_MIX, k + 1);
-  HIP_OK(launch_aa_mix(c->g, kf, af, c->aa, sel, c->stream, gn));
+  t = timed_begin(c, st.kid + STS_K_AA_MIX, k + 1);
+  HIP_OK(launch_aa_mix(*st.g, kf, af, c->aa, sel, c->stream, gn));
   timed_end(c, t);
   return STS_OK;
 }
@@ -958,13 +1002,10 @@ static int enqueue_outer(sts_ctx *c, const Loop2State<R> &st, int k, int n, Loop
     par ^= 1;
   }
   const KFieldsT<R> kf = fields_at(par);
-  // Anderson acceleration (FP64): a step after the edge conditions
-  bool aa = false;
+  // Anderson acceleration: a step after the edge conditions
+  const bool aa = st.aa_m > 0 && (k + 1) % st.aa_every == 0;
   AASel sel{};
-  if constexpr (std::is_same<R, double>::value) {
-    aa = st.aa_m > 0 && (k + 1) % st.aa_every == 0;
-    if (aa) sel = anderson_select(c, k, st.aa_m, st.aa_every);
-  }
+  if (aa) sel = anderson_select(c, k, st.aa_m, st.aa_every);
   t = timed_begin(c, st.kid + STS_K_CORRECT, k);
   HIP_OK(launch_correct(g, kf, lr + 0, lr + 1, 1, c->stream, gate));
   timed_end(c, t);
@@ -978,11 +1019,10 @@ static int enqueue_outer(sts_ctx *c, const Loop2State<R> &st, int k, int n, Loop
   HIP_OK(launch_edges(g, kf, STS_BC_VELOCITIES | STS_BC_TEMPERATURE | STS_BC_PRESSURE, c->probe, c->stream, gate,
                       lt));
   timed_end(c, t);
-  if constexpr (std::is_same<R, double>::value)
-    if (aa) {
-      int r;
-      if ((r = enqueue_anderson(c, kf, k, sel))) return r;
-    }
+  if (aa) {
+    int r;
+    if ((r = enqueue_anderson(c, st, kf, k, sel))) return r;
+  }
   // an Anderson mix forms the fluxes itself, and for the loop's last
   // iteration (where the mix, gated on k + 1, does not run) run_loop2 runs
   // the pass once the loop has stopped
@@ -1003,12 +1043,11 @@ static int run_loop2(sts_ctx *c, Loop2State<R> &st, R *&p, R *&T, R *&p_alt, R *
                      int batch, int *iters) {
   HIP_OK(launch_loop_open(c->ctrl, lt.log_k, 6 * lt.N_I, c->lred, c->stream));
   timed_loop_open(c);
-  if constexpr (std::is_same<R, double>::value)
-    if (st.aa_m > 0) {
-      int r;
-      if ((r = ensure_anderson(c, st.aa_m))) return r;
-      HIP_OK(launch_aa_begin(AAFields{{st.f.f[STS_U], st.f.f[STS_V], p, T}}, c->aa, c->Na, c->stream));
-    }
+  if (st.aa_m > 0) {
+    int r;
+    if ((r = ensure_anderson(c, st.aa_m))) return r;
+    HIP_OK(launch_aa_begin(AAFieldsT<R>{{st.f.f[STS_U], st.f.f[STS_V], p, T}}, c->aa, c->Na, c->stream));
+  }
   // The loop control is read after iteration end - 1 - LOOKAHEAD of each
   // batch (after the last one of a batch that ends at N_I), so that the host
   // decides while the GPU still runs the batch's tail; iterations enqueued
@@ -1036,14 +1075,13 @@ static int run_loop2(sts_ctx *c, Loop2State<R> &st, R *&p, R *&T, R *&p_alt, R *
     std::swap(p, p_alt);
     std::swap(T, T_alt);
   }
-  if constexpr (std::is_same<R, double>::value)
-    if (st.aa_m > 0 && *iters % st.aa_every == 0 && (*iters / st.aa_every - 1) > 0) {
-      // the last iteration's flux pass (its Anderson mix did not run)
-      KFieldsT<R> kf = st.f;
-      kf.f[STS_P] = p;
-      kf.f[STS_T] = T;
-      HIP_OK(launch_flux_diff(*st.g, kf, c->stream, Gate{nullptr, 0}, FLUX_PART));
-    }
+  if (st.aa_m > 0 && *iters % st.aa_every == 0 && (*iters / st.aa_every - 1) > 0) {
+    // the last iteration's flux pass (its Anderson mix did not run)
+    KFieldsT<R> kf = st.f;
+    kf.f[STS_P] = p;
+    kf.f[STS_T] = T;
+    HIP_OK(launch_flux_diff(*st.g, kf, c->stream, Gate{nullptr, 0}, FLUX_PART));
+  }
   if (c->h_ctrl->nonfinite) return fail(STS_ERR_NONFINITE, "non-finite residual in outer iteration " + std::to_string(*iters));
   return STS_OK;
 }
@@ -1254,6 +1292,10 @@ static int run_phases(sts_ctx *c, const sts_loop_control *lc, const sts_mixed_co
     const unsigned long long all = (1ull << STS_NFIELDS) - 1ull;
     HIP_OK(launch_convert(c->fields(), c->fields32(), all, c->Na, c->stream));
     Loop2State<float> s32{&c->g32, c->fields32(), c->p_alt32, c->T_alt32, STS_K_COUNT};
+    if (mc->anderson32 > 0) {
+      s32.aa_m = mc->anderson32;
+      s32.aa_every = std::max(1, mc->anderson_every);
+    }
     const int b32 = lc->batch > 0 ? lc->batch : (c->last_iters32 > 2 ? c->last_iters32 - 1 : 1);
     rc = run_loop2(c, s32, c->f32[STS_P], c->f32[STS_T], c->p_alt32, c->T_alt32, lc->N_I_p_c,
                    loop_test(c, c->log, mc->N_I32, mc->tol32_uv, mc->tol32_p, mc->tol32_T, mc->stall_w,
@@ -1286,7 +1328,7 @@ static int run_phases(sts_ctx *c, const sts_loop_control *lc, const sts_mixed_co
 static int check_mixed(const sts_ctx *c, const sts_loop_control *lc, const sts_mixed_control *mc) {
   if (!lc || lc->N_I < 1 || lc->N_I_p_c < 1 || lc->N_I_p_c > MAX_SWEEPS) return fail(STS_ERR_ARG, "bad loop control");
   if (mc && (mc->N_I32 < 0 || mc->stall_w < 0 || mc->anderson < 0 || mc->anderson > MAX_AA ||
-             mc->anderson_every < 0))
+             mc->anderson_every < 0 || mc->anderson32 < 0 || mc->anderson32 > MAX_AA))
     return fail(STS_ERR_ARG, "bad mixed control");
   if (mc && mc->N_I32 > 0 && !c->mixed) return fail(STS_ERR_UNSUPPORTED, "the context was not created with precision 32");
   return STS_OK;

@@ -26,11 +26,23 @@ enum : unsigned {
 // Grid, scalars and switches of one case; R is the arithmetic type of the
 // kernels that take it (double, or float for the FP32 phase of the mixed
 // precision time step).
+// Fluid bits of the cells the stencils read around a cell, one 16-bit word
+// per cell formed once from the (static) mask with Nbr's neighbour indices
+// (clamped at the grid edge, x wrapping when periodic): the cell itself, west,
+// east, west-west (i_2), south, north, the four diagonals, and the v
+// equation's second south row (ij_1 - Nx*(0 <= j-2), S:3399).  One load
+// replaces the byte loads of every neighbour's mask.
+enum : unsigned {
+  NB_C = 1u << 0, NB_W = 1u << 1, NB_E = 1u << 2, NB_WW = 1u << 3, NB_S = 1u << 4, NB_N = 1u << 5,
+  NB_SW = 1u << 6, NB_SE = 1u << 7, NB_NW = 1u << 8, NB_NE = 1u << 9, NB_SS = 1u << 10,
+};
+
 template <typename R>
 struct KGridT {
   int Nx, Ny;
   const R *hx, *hy;
   const uint8_t *m;       // vol_inf_fb_bool
+  const uint16_t *nb;     // neighbour fluid bits (NB_*)
   const uint32_t *vol;    // vol_inf_fb
   const DevBody *bodies;
   int nbodies;
@@ -184,10 +196,13 @@ struct AABuf {
   int nb;
 };
 // The four state arrays as the outer iteration leaves them (p and T in
-// whichever ping-pong buffer holds them).
-struct AAFields {
-  double *x[4];  // u, v, p, T
+// whichever ping-pong buffer holds them), of the loop's precision R (the
+// acceleration's own vectors stay as above for both).
+template <typename R>
+struct AAFieldsT {
+  R *x[4];  // u, v, p, T
 };
+using AAFields = AAFieldsT<double>;
 
 // One case of a batch run in shared launches (device-resident): grid and
 // fields with p and T as they stand at the start of the step, the p/T

@@ -596,7 +596,7 @@ struct Driver {
   double loop_seconds = 0;
   // mixed-precision time steps (sts_time_step_mixed) when set
   bool mixed = false;
-  sts_mixed_control mc{0, 1e-6, 1e-6, 1e-6, 4, 0.7, 0, 0, 0, 0, 0};
+  sts_mixed_control mc{0, 1e-6, 1e-6, 1e-6, 4, 0.7, 0, 0, 0, 0, 0, 0};
   long long outer_total32 = 0, outer32_at_start = 0;
   // timing of the steps after the warm-up ones (benchmarking)
   unsigned warmup_steps = 0;
@@ -629,7 +629,7 @@ int main(int argc, char **argv) {
       "aa_mix",     "assemble32", "sweep32", "correct32", "wall32", "edges32",   "flux_diff32", "assemble_inl32",
       "aa_store32", "aa_mix32"};
   bool mixed = false;
-  sts_mixed_control mc{0, 1e-6, 1e-6, 1e-6, 4, 0.7, 0, 0, 0, 0, 0};
+  sts_mixed_control mc{0, 1e-6, 1e-6, 1e-6, 4, 0.7, 0, 0, 0, 0, 0, 0};
   int n32 = -1;
   for (int a = 1; a < argc; a++) {
     std::string s = argv[a];
@@ -657,6 +657,8 @@ int main(int argc, char **argv) {
       mc.guard = std::atoi(argv[++a]);
     } else if (s == "--anderson" && a + 1 < argc) {
       mc.anderson = std::atoi(argv[++a]);
+    } else if (s == "--anderson32" && a + 1 < argc) {
+      mc.anderson32 = std::atoi(argv[++a]);
     } else if (s == "--anderson-every" && a + 1 < argc) {
       mc.anderson_every = std::atoi(argv[++a]);
     } else if (s == "--continuity-norms" && a + 1 < argc) {
@@ -671,7 +673,7 @@ int main(int argc, char **argv) {
                    "name[,name...]] (assemble, assemble_inl, sweep, correct, wall, edges, flux_diff, aa_store, "
                    "aa_mix; FP32 instances with suffix 32) [--extrapolate degree] [--mixed [--n32 cap] "
                    "[--tol32 tol] [--stall window ratio] [--fp32-above iterations]] [--guard max_pause] "
-                   "[--anderson depth [--anderson-every iterations]] [--continuity-norms file]"
+                   "[--anderson depth [--anderson-every iterations]] [--anderson32 depth] [--continuity-norms file]"
                 << std::endl;
       return 0;
     } else dir = s;

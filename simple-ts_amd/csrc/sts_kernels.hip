@@ -111,7 +111,6 @@ struct FluxIn {
     in = i < g.Nx && j < g.Ny;
     if (!in) return;
     const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny, g.px);
-    const uint8_t *m = g.m;
     const R *sT = FLD(SQRT_T), *Twu = FLD(TW_U), *Twv = FLD(TW_V);
     ij = n.ij;
     const unsigned jm = (unsigned)j - 1u * (0 < j);
@@ -125,10 +124,11 @@ struct FluxIn {
       rv = FLD(RHO_V)[ij];
       v = FLD(V)[ij];
     }
-    m00 = DIFF ? m[n.i_1j_1] : false;
-    m10 = m[n.ij_1];
-    m01 = m[n.i_1j];
-    m11 = m[ij];
+    const unsigned b = g.nb[ij];
+    m00 = DIFF ? (b & NB_SW) != 0 : false;
+    m10 = (b & NB_S) != 0;
+    m01 = (b & NB_W) != 0;
+    m11 = (b & NB_C) != 0;
     s00 = DIFF ? sT[n.i_1j_1] : R(0);
     s10 = sT[n.ij_1];
     s01 = sT[n.i_1j];
@@ -241,22 +241,21 @@ struct CellDiff {
 
 template <typename R>
 __device__ __forceinline__ void cell_assemble_u(const KGridT<R> &g, const KFieldsT<R> &fl, const Nbr &n,
-                                                const CellDiff<R> &df) {
-  const uint8_t *m = g.m;
+                                                const CellDiff<R> &df, unsigned nb) {
   const R *hx = g.hx, *hy = g.hy;
   const R *Fx = FLD(FX), *Fy = FLD(FY), *u = FLD(U), *v = FLD(V), *sqrt_T = FLD(SQRT_T);
   const R *rho_pr = FLD(RHO_PR), *u_pr = FLD(U_PR), *rho = FLD(RHO), *rho_u = FLD(RHO_U);
   const unsigned i = n.i, j = n.j, ij = n.ij, i_1j = n.i_1j, i1j = n.i1j, ij_1 = n.ij_1, ij1 = n.ij1;
-  const unsigned i_1j_1 = n.i_1j_1, i_1j1 = n.i_1j1, i_2j = n.i_2j, i_1 = n.i_1;
+  const unsigned i_1j1 = n.i_1j1, i_1 = n.i_1;
   const R cT = g.cT, ht = g.ht;
   R F_xu1 = R(0.5) * (Fx[i_1j] + Fx[ij]);
   R F_xu2 = R(0.5) * (Fx[ij] + Fx[i1j]);
-  R Scu = cT * (R(m[i1j]) * u[i1j] * sqrt_T[ij] * hy[j] / (hx[i] * R(3.0))
-                     + R(m[i_2j]) * u[i_1j] * sqrt_T[i_1j] * hy[j] / (hx[i_1] * R(3.0))
-                     + df.Gy1 * (v[ij1] * R(m[ij1]) - v[i_1j1] * R(m[i_1j1]))
-                     - df.Gy0 * (v[ij] * R(m[ij_1]) - v[i_1j] * R(m[i_1j_1]))
-                     - sqrt_T[ij] * (v[ij1] * R(m[ij1]) - v[ij] * R(m[ij_1])) * R(2.0) / R(3.0)
-                     + sqrt_T[i_1j] * (v[i_1j1] * R(m[i_1j1]) - v[i_1j] * R(m[i_1j_1])) * R(2.0) / R(3.0));
+  R Scu = cT * (R((nb & NB_E) != 0) * u[i1j] * sqrt_T[ij] * hy[j] / (hx[i] * R(3.0))
+                     + R((nb & NB_WW) != 0) * u[i_1j] * sqrt_T[i_1j] * hy[j] / (hx[i_1] * R(3.0))
+                     + df.Gy1 * (v[ij1] * R((nb & NB_N) != 0) - v[i_1j1] * R((nb & NB_NW) != 0))
+                     - df.Gy0 * (v[ij] * R((nb & NB_S) != 0) - v[i_1j] * R((nb & NB_SW) != 0))
+                     - sqrt_T[ij] * (v[ij1] * R((nb & NB_N) != 0) - v[ij] * R((nb & NB_S) != 0)) * R(2.0) / R(3.0)
+                     + sqrt_T[i_1j] * (v[i_1j1] * R((nb & NB_NW) != 0) - v[i_1j] * R((nb & NB_SW) != 0)) * R(2.0) / R(3.0));
   R Spu = -(df.Dux0 + df.Dux1) / R(3.0);
   R au1 = maxi0(F_xu1) + df.Dux0;
   R au2 = maxi0(-F_xu2) + df.Dux1;
@@ -265,37 +264,34 @@ __device__ __forceinline__ void cell_assemble_u(const KGridT<R> &g, const KField
   R bu3 = (rho_pr[i_1j] * hx[i_1] + rho_pr[ij] * hx[i]) * u_pr[ij] * hy[j] * R(0.5) / ht + Scu;
   R au0 = au1 + au2 + au3 + au4 + R(0.5) * (-Fy[i_1j] - Fy[ij] + Fy[i_1j1] + Fy[ij1]) + F_xu2 - F_xu1 - Spu
                + (rho[i_1j] * hx[i_1] + rho[ij] * hx[i]) * hy[j] * R(0.5) / ht;
-  R ups = (au1 * R(m[i1j]) * u[i_1j] + au2 * R(m[i_2j]) * u[i1j] + au3 * u[ij_1] + au4 * u[ij1] + bu3) / au0;
+  R ups = (au1 * R((nb & NB_E) != 0) * u[i_1j] + au2 * R((nb & NB_WW) != 0) * u[i1j] + au3 * u[ij_1] + au4 * u[ij1] + bu3) / au0;
   R dpc = g.cPch * hy[j] / au0;
   R h = hy[j] * rho_u[ij];
   FLD(U_PSEUDO)[ij] = ups;
   FLD(D_P_C_12)[ij] = dpc;
   FLD(APX)[ij] = dpc * h;
-  FLD(BPX)[ij] = R(m[i_1j] * m[ij]) * h * ups;
+  FLD(BPX)[ij] = R((nb & NB_W) != 0 && (nb & NB_C) != 0) * h * ups;
 }
 
 template <typename R>
 __device__ __forceinline__ void cell_assemble_v(const KGridT<R> &g, const KFieldsT<R> &fl, const Nbr &n,
-                                                const CellDiff<R> &df) {
-  const uint8_t *m = g.m;
-  const unsigned Nx = (unsigned)g.Nx;
+                                                const CellDiff<R> &df, unsigned nb) {
   const R *hx = g.hx, *hy = g.hy;
   const R *Fx = FLD(FX), *Fy = FLD(FY), *u = FLD(U), *v = FLD(V), *sqrt_T = FLD(SQRT_T);
   const R *rho_pr = FLD(RHO_PR), *v_pr = FLD(V_PR), *rho = FLD(RHO), *rho_v = FLD(RHO_V);
   const unsigned i = n.i, j = n.j, ij = n.ij, i_1j = n.i_1j, i1j = n.i1j, ij_1 = n.ij_1, ij1 = n.ij1;
-  const unsigned i_1j_1 = n.i_1j_1, i1j_1 = n.i1j_1;
+  const unsigned i1j_1 = n.i1j_1;
   const R cT = g.cT, ht = g.ht;
-  // S:3399 ij_1 - Nx*(0 <= (int)(j-2)); S:3400 hy[j - 1*(0 <= (int)(j-1))]
-  const unsigned jm2 = ij_1 - Nx * (0 <= (int)(j - 2));
+  // S:3399 the mask at ij_1 - Nx*(0 <= (int)(j-2)) (NB_SS); S:3400 hy[j - 1*(0 <= (int)(j-1))]
   const R hyjm = hy[j - 1 * (0 <= (int)(j - 1))];
   R F_yv3 = R(0.5) * (Fy[ij_1] + Fy[ij]);
   R F_yv4 = R(0.5) * (Fy[ij] + Fy[ij1]);
-  R Scv = cT * (R(m[ij1]) * v[ij1] * sqrt_T[ij] * hx[i] / (hy[j] * R(3.0))
-                     + R(m[jm2]) * v[ij_1] * sqrt_T[ij_1] * hx[i] / (hyjm * R(3.0))
-                     + df.Gx1 * (u[i1j] * R(m[i1j]) - u[i1j_1] * R(m[i1j_1]))
-                     - df.Gx0 * (u[ij] * R(m[i_1j]) - u[ij_1] * R(m[i_1j_1]))
-                     - sqrt_T[ij] * (u[i1j] * R(m[i1j]) - u[ij] * R(m[i_1j])) * R(2.0) / R(3.0)
-                     + sqrt_T[ij_1] * (u[i1j_1] * R(m[i1j_1]) - u[ij_1] * R(m[i_1j_1])) * R(2.0) / R(3.0))
+  R Scv = cT * (R((nb & NB_N) != 0) * v[ij1] * sqrt_T[ij] * hx[i] / (hy[j] * R(3.0))
+                     + R((nb & NB_SS) != 0) * v[ij_1] * sqrt_T[ij_1] * hx[i] / (hyjm * R(3.0))
+                     + df.Gx1 * (u[i1j] * R((nb & NB_E) != 0) - u[i1j_1] * R((nb & NB_SE) != 0))
+                     - df.Gx0 * (u[ij] * R((nb & NB_W) != 0) - u[ij_1] * R((nb & NB_SW) != 0))
+                     - sqrt_T[ij] * (u[i1j] * R((nb & NB_E) != 0) - u[ij] * R((nb & NB_W) != 0)) * R(2.0) / R(3.0)
+                     + sqrt_T[ij_1] * (u[i1j_1] * R((nb & NB_SE) != 0) - u[ij_1] * R((nb & NB_SW) != 0)) * R(2.0) / R(3.0))
                - g.c_Fr_v * (rho[ij_1] * hx[i] * R(0.5) * hyjm + rho[ij] * hx[i] * R(0.5) * hy[j]);
   R Spv = -(df.Dvy0 + df.Dvy1) / R(3.0);
   R av1 = R(0.5) * (maxi0(Fx[ij]) + maxi0(Fx[ij_1])) + df.Dvx0;
@@ -305,35 +301,35 @@ __device__ __forceinline__ void cell_assemble_v(const KGridT<R> &g, const KField
   R bv3 = (rho_pr[ij_1] * hy[j - 1] + rho_pr[ij] * hy[j]) * v_pr[ij] * hx[i] * R(0.5) / ht + Scv;
   R av0 = av1 + av2 + av3 + av4 + R(0.5) * (Fx[i1j] - Fx[ij] + Fx[i1j_1] - Fx[ij_1]) + F_yv4 - F_yv3 - Spv
                + (rho[ij_1] * hy[j - 1] + rho[ij] * hy[j]) * hx[i] * R(0.5) / ht;
-  R vps = (av1 * v[i_1j] + av2 * v[i1j] + av3 * R(m[jm2]) * v[ij_1] + av4 * R(m[ij1]) * v[ij1] + bv3) / av0;
+  R vps = (av1 * v[i_1j] + av2 * v[i1j] + av3 * R((nb & NB_SS) != 0) * v[ij_1] + av4 * R((nb & NB_N) != 0) * v[ij1] + bv3) / av0;
   R dpc = g.cPch * hx[i] / av0;
   R h = hx[i] * rho_v[ij];
   FLD(V_PSEUDO)[ij] = vps;
   FLD(D_P_C_34)[ij] = dpc;
   FLD(APY)[ij] = dpc * h;
-  FLD(BPY)[ij] = R(m[ij - Nx] * m[ij]) * h * vps;
+  FLD(BPY)[ij] = R((nb & NB_S) != 0 && (nb & NB_C) != 0) * h * vps;
 }
 
 template <typename R>
-__device__ __forceinline__ void cell_energy_source(const KGridT<R> &g, const KFieldsT<R> &fl, const Nbr &n) {
-  const uint8_t *m = g.m;
+__device__ __forceinline__ void cell_energy_source(const KGridT<R> &g, const KFieldsT<R> &fl, const Nbr &n,
+                                                   unsigned nb) {
   const unsigned Nx = (unsigned)g.Nx;
   const R *hx = g.hx, *hy = g.hy;
   const R *u = FLD(U), *v = FLD(V), *sqrt_T = FLD(SQRT_T), *p_pr = FLD(P_PR);
   const unsigned i = n.i, j = n.j, ij = n.ij, i_1j = n.i_1j, i1j = n.i1j, ij_1 = n.ij_1, ij1 = n.ij1;
-  const unsigned i_1j_1 = n.i_1j_1, i_1j1 = n.i_1j1, i1j_1 = n.i1j_1, i1j1 = n.i1j1, i_1 = n.i_1, i1 = n.i1;
+  const unsigned i_1j1 = n.i_1j1, i1j_1 = n.i1j_1, i1j1 = n.i1j1, i_1 = n.i_1, i1 = n.i1;
   // S:3729-3740: the upper neighbour row is guarded by j < Nx-1
   const unsigned jm = j - 1 * (0 < j), jp = j + 1 * (j < (Nx - 1));
-  R dudx_T = (R(m[i1j]) * u[i1j] - R(m[i_1j]) * u[ij]) / hx[i];
-  R dvdy_T = (R(m[ij1]) * v[ij1] - R(m[ij_1]) * v[ij]) / hy[j];
-  R s0 = ((u[ij] - u[ij_1]) / (R(m[i_1j_1] && m[ij_1]) * hy[jm] + hy[j]))
-              + ((v[ij] - v[i_1j]) / (R(m[i_1j_1] && m[i_1j]) * hx[i_1] + hx[i]));
-  R s1 = ((u[i1j] - u[i1j_1]) / (R(m[ij_1] && m[i1j_1]) * hy[jm] + hy[j]))
-              + ((v[i1j] - v[ij]) / (hx[i] + R(m[i1j_1] && m[i1j]) * hx[i1]));
-  R s2 = ((u[ij1] - u[ij]) / (hy[j] + R(m[i_1j1] && m[ij1]) * hy[jp]))
-              + ((v[ij1] - v[i_1j1]) / (R(m[i_1j] && m[i_1j1]) * hx[i_1] + hx[i]));
-  R s3 = ((u[i1j1] - u[i1j]) / (hy[j] + R(m[ij1] && m[i1j1]) * hy[jp]))
-              + ((v[i1j1] - v[ij1]) / (hx[i] + R(m[i1j] && m[i1j1]) * hx[i1]));
+  R dudx_T = (R(((nb & NB_E) != 0)) * u[i1j] - R(((nb & NB_W) != 0)) * u[ij]) / hx[i];
+  R dvdy_T = (R(((nb & NB_N) != 0)) * v[ij1] - R(((nb & NB_S) != 0)) * v[ij]) / hy[j];
+  R s0 = ((u[ij] - u[ij_1]) / (R(((nb & NB_SW) != 0) && ((nb & NB_S) != 0)) * hy[jm] + hy[j]))
+              + ((v[ij] - v[i_1j]) / (R(((nb & NB_SW) != 0) && ((nb & NB_W) != 0)) * hx[i_1] + hx[i]));
+  R s1 = ((u[i1j] - u[i1j_1]) / (R(((nb & NB_S) != 0) && ((nb & NB_SE) != 0)) * hy[jm] + hy[j]))
+              + ((v[i1j] - v[ij]) / (hx[i] + R(((nb & NB_SE) != 0) && ((nb & NB_E) != 0)) * hx[i1]));
+  R s2 = ((u[ij1] - u[ij]) / (hy[j] + R(((nb & NB_NW) != 0) && ((nb & NB_N) != 0)) * hy[jp]))
+              + ((v[ij1] - v[i_1j1]) / (R(((nb & NB_W) != 0) && ((nb & NB_NW) != 0)) * hx[i_1] + hx[i]));
+  R s3 = ((u[i1j1] - u[i1j]) / (hy[j] + R(((nb & NB_N) != 0) && ((nb & NB_NE) != 0)) * hy[jp]))
+              + ((v[i1j1] - v[ij1]) / (hx[i] + R(((nb & NB_E) != 0) && ((nb & NB_NE) != 0)) * hx[i1]));
   R dsum = dudx_T + dvdy_T;
   FLD(SCT1)[ij] = g.ht * sqrt_T[ij] * g.cT2 * hx[i] * hy[j]
                       * (R(2.0) * (dudx_T * dudx_T + dvdy_T * dvdy_T) + s0 * s0 + s1 * s1 + s2 * s2 + s3 * s3
@@ -347,18 +343,19 @@ template <typename R>
 __device__ __forceinline__ void assemble_cell(const KGridT<R> &g, const KFieldsT<R> &fl, int i, int j) {
   if (i >= g.Nx || j >= g.Ny) return;
   const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny, g.px);
-  const uint8_t *m = g.m;
-  const bool du = in_box(i, j, g.Nx_u0, g.Nx_u1, g.Ny_u0, g.Ny_u1) && m[n.i_1j] && m[n.ij];
-  const bool dv = in_box(i, j, g.Nx_v0, g.Nx_v1, g.Ny_v0, g.Ny_v1) && m[n.ij_1] && m[n.ij];
+  const unsigned nb = g.nb[n.ij];
+  const bool c = (nb & NB_C) != 0;
+  const bool du = in_box(i, j, g.Nx_u0, g.Nx_u1, g.Ny_u0, g.Ny_u1) && (nb & NB_W) && c;
+  const bool dv = in_box(i, j, g.Nx_v0, g.Nx_v1, g.Ny_v0, g.Ny_v1) && (nb & NB_S) && c;
   if (du || dv) {
     const R *Gyf = FLD(GAMMA_YF), *Gxf = FLD(GAMMA_XF), *Dux = FLD(D_UX), *Duy = FLD(D_UY), *Dvx = FLD(D_VX),
             *Dvy = FLD(D_VY);
     const CellDiff<R> df{Gyf[n.ij], Gyf[n.ij1], Gxf[n.ij], Gxf[n.i1j], Dux[n.ij], Dux[n.i1j],
                          Duy[n.ij], Duy[n.ij1], Dvx[n.ij], Dvx[n.i1j], Dvy[n.ij], Dvy[n.ij1]};
-    if (du) cell_assemble_u(g, fl, n, df);
-    if (dv) cell_assemble_v(g, fl, n, df);
+    if (du) cell_assemble_u(g, fl, n, df, nb);
+    if (dv) cell_assemble_v(g, fl, n, df, nb);
   }
-  if (in_box(i, j, g.Nx_o0, g.Nx_o1, g.Ny_o0, g.Ny_o1) && m[n.ij]) cell_energy_source(g, fl, n);
+  if (in_box(i, j, g.Nx_o0, g.Nx_o1, g.Ny_o0, g.Ny_o1) && c) cell_energy_source(g, fl, n, nb);
 }
 
 template <typename R>
@@ -405,17 +402,18 @@ __global__ void __launch_bounds__(BX *BY) k_assemble_inl(KGridT<R> g, KFieldsT<R
   __syncthreads();
   if (!in) return;
   const Nbr n((unsigned)i, (unsigned)j, (unsigned)Nx, (unsigned)Ny, g.px);
-  const uint8_t *m = g.m;
-  const bool du = in_box(i, j, g.Nx_u0, g.Nx_u1, g.Ny_u0, g.Ny_u1) && m[n.i_1j] && m[n.ij];
-  const bool dv = in_box(i, j, g.Nx_v0, g.Nx_v1, g.Ny_v0, g.Ny_v1) && m[n.ij_1] && m[n.ij];
+  const unsigned nb = g.nb[n.ij];
+  const bool c = (nb & NB_C) != 0;
+  const bool du = in_box(i, j, g.Nx_u0, g.Nx_u1, g.Ny_u0, g.Ny_u1) && (nb & NB_W) && c;
+  const bool dv = in_box(i, j, g.Nx_v0, g.Nx_v1, g.Ny_v0, g.Ny_v1) && (nb & NB_S) && c;
   if (du || dv) {
     const int ex = (i == il) ? BX : tx + 1, ny = (j == jl) ? BY : ty + 1;
     const CellDiff<R> df{Lgy[ty][tx], Lgy[ny][tx], Lgx[ty][tx], Lgx[ty][ex], Lux[ty][tx], Lux[ty][ex],
                          Luy[ty][tx], Luy[ny][tx], Lvx[ty][tx], Lvx[ty][ex], Lvy[ty][tx], Lvy[ny][tx]};
-    if (du) cell_assemble_u(g, fl, n, df);
-    if (dv) cell_assemble_v(g, fl, n, df);
+    if (du) cell_assemble_u(g, fl, n, df, nb);
+    if (dv) cell_assemble_v(g, fl, n, df, nb);
   }
-  if (in_box(i, j, g.Nx_o0, g.Nx_o1, g.Ny_o0, g.Ny_o1) && m[n.ij]) cell_energy_source(g, fl, n);
+  if (in_box(i, j, g.Nx_o0, g.Nx_o1, g.Ny_o0, g.Ny_o1) && c) cell_energy_source(g, fl, n, nb);
 }
 
 // ---------------------------------------------------------------------------
@@ -450,7 +448,6 @@ struct PTCoef {
 
   __device__ __forceinline__ void load(const KGridT<R> &g, const KFieldsT<R> &fl, const Nbr &n) {
     const unsigned ij = n.ij, i1j = n.i1j, ij1 = n.ij1;
-    const uint8_t *m = g.m;
     const R *Fx = FLD(FX), *Fy = FLD(FY), *DTx = FLD(DTX), *DTy = FLD(DTY), *u = FLD(U), *v = FLD(V);
     const R *apx = FLD(APX), *apy = FLD(APY), *bpx = FLD(BPX), *bpy = FLD(BPY);
     ht = g.ht;
@@ -460,7 +457,8 @@ struct PTCoef {
     aT2 = maxi0(-Fx[i1j]) + DTx[i1j];
     aT3 = maxi0(Fy[ij]) + DTy[ij];
     aT4 = maxi0(-Fy[ij1]) + DTy[ij1];
-    fluid = (m[n.i_1j] ? 1u : 0u) | (m[i1j] ? 2u : 0u) | (m[n.ij_1] ? 4u : 0u) | (m[ij1] ? 8u : 0u);
+    const unsigned nb = g.nb[ij];
+    fluid = ((nb & NB_W) ? 1u : 0u) | ((nb & NB_E) ? 2u : 0u) | ((nb & NB_S) ? 4u : 0u) | ((nb & NB_N) ? 8u : 0u);
     X1 = (fluid & 1u) ? R(0) : FLD(TW_U)[ij];
     X2 = (fluid & 2u) ? R(0) : FLD(TW_U)[i1j];
     X3 = (fluid & 4u) ? R(0) : FLD(TW_V)[ij];
@@ -745,13 +743,13 @@ struct CorrectIn {
     in = i < g.Nx && j < g.Ny;
     if (!in) return;
     const Nbr n((unsigned)i, (unsigned)j, (unsigned)g.Nx, (unsigned)g.Ny, g.px);
-    const uint8_t *m = g.m;
     ij = n.ij;
     i_1j = n.i_1j;
     ij_1 = n.ij_1;
-    mc = m[ij];
-    mw = m[i_1j];
-    ms = m[ij_1];
+    const unsigned nb = g.nb[ij];
+    mc = (nb & NB_C) != 0;
+    mw = (nb & NB_W) != 0;
+    ms = (nb & NB_S) != 0;
     cu = do_correct && in_box(i, j, g.Nx_u0, g.Nx_u1, g.Ny_u0, g.Ny_u1) && mw && mc;
     cv = do_correct && in_box(i, j, g.Nx_v0, g.Nx_v1, g.Ny_v0, g.Ny_v1) && ms && mc;
     const R *p = FLD(P), *T = FLD(T);
@@ -1524,11 +1522,14 @@ __device__ __forceinline__ void store4(float *p, const double (&v)[4]) {
   *reinterpret_cast<float4 *>(p) = make_float4((float)v[0], (float)v[1], (float)v[2], (float)v[3]);
 }
 
-// Loop start: the start x = gp + D with gp = the state, D = 0.
-__global__ void __launch_bounds__(AA_THREADS) k_aa_begin(AAFields s, AABuf ab, int Na) {
+// Loop start: the start x = gp + D with gp = the state, D = 0.  The state
+// may be FP32 (the FP32 phase); gp holds it widened, exactly.
+template <typename R>
+__global__ void __launch_bounds__(AA_THREADS) k_aa_begin(AAFieldsT<R> s, AABuf ab, int Na) {
   const int k = (int)(blockIdx.x * AA_THREADS + threadIdx.x);
   if (k >= Na) return;
-  const double g[4] = {s.x[0][k], s.x[1][k], s.x[2][k], s.x[3][k]}, z[4] = {0.0, 0.0, 0.0, 0.0};
+  const double g[4] = {(double)s.x[0][k], (double)s.x[1][k], (double)s.x[2][k], (double)s.x[3][k]},
+               z[4] = {0.0, 0.0, 0.0, 0.0};
   store4(ab.gp + 4 * (size_t)k, g);
   store4(ab.D + 4 * (size_t)k, z);
 }
@@ -1539,14 +1540,14 @@ __global__ void __launch_bounds__(AA_THREADS) k_aa_begin(AAFields s, AABuf ab, i
 // dG = g - gp go to sel.dF/dG[n-1], fp = f, gp = g, and acc[j] +=
 // dF_j . dF_new, acc[NM + j] += dF_j . f for every difference in use.
 // NM >= sel.n bounds the differences at compile time.
-template <int NM>
-__device__ __forceinline__ void aa_store_cell(const AAFields &s, const AABuf &ab, const AASel &sel, int k,
+template <int NM, typename R>
+__device__ __forceinline__ void aa_store_cell(const AAFieldsT<R> &s, const AABuf &ab, const AASel &sel, int k,
                                               double (&acc)[2 * NM]) {
   const int n = sel.n;
   const size_t e = 4 * (size_t)k;
   double g[4], gp[4], D[4], fp[4], o[NM][4];
 #pragma unroll
-  for (int q = 0; q < 4; q++) g[q] = s.x[q][k];
+  for (int q = 0; q < 4; q++) g[q] = (double)s.x[q][k];
   load4(ab.gp + e, gp);
   load4(ab.D + e, D);
   if (n > 0) load4(ab.fp + e, fp);
@@ -1602,8 +1603,8 @@ __device__ __forceinline__ void aa_store_sums(const double (&acc)[2 * NM], const
 // AA_STORE_CELLS cells per thread (AA_THREADS apart), so that the partial
 // sums the weights are folded from are few.
 constexpr int AA_STORE_CELLS = 2;
-template <int NM>
-__global__ void __launch_bounds__(AA_THREADS) k_aa_store(AAFields s, AABuf ab, AASel sel, int Na, Gate gate) {
+template <int NM, typename R>
+__global__ void __launch_bounds__(AA_THREADS) k_aa_store(AAFieldsT<R> s, AABuf ab, AASel sel, int Na, Gate gate) {
   if (gated_out(gate)) return;
   double acc[2 * NM];
 #pragma unroll
@@ -1611,7 +1612,7 @@ __global__ void __launch_bounds__(AA_THREADS) k_aa_store(AAFields s, AABuf ab, A
 #pragma unroll
   for (int r = 0; r < AA_STORE_CELLS; r++) {
     const int k = (int)((blockIdx.x * AA_STORE_CELLS + r) * AA_THREADS + threadIdx.x);
-    if (k < Na) aa_store_cell<NM>(s, ab, sel, k, acc);
+    if (k < Na) aa_store_cell<NM, R>(s, ab, sel, k, acc);
   }
   if (sel.n > 0) aa_store_sums<NM, AA_THREADS>(acc, ab, (int)blockIdx.x);
 }
@@ -1737,10 +1738,10 @@ __device__ __forceinline__ void aa_mixed(const AABuf &ab, const AASel &sel, cons
 // DTy; FluxIn::apply_flux).  The mixed p, T of the tile and of its west
 // column and south row (formed by the first threads) meet in LDS (sp, sT:
 // [1 + row][1 + column], row 0 and column 0 the halo).
-template <int NM, int TX, int TY, bool FLUX>
-__device__ __forceinline__ void aa_mix_tile(const KGrid &g, const KFields &fl, const AAFields &s, const AABuf &ab,
-                                            const AASel &sel, const double *gsrc, int i0, int j0, int tx, int ty,
-                                            double (*sp)[TX + 1], double (*sT)[TX + 1]) {
+template <int NM, int TX, int TY, bool FLUX, typename R>
+__device__ __forceinline__ void aa_mix_tile(const KGridT<R> &g, const KFieldsT<R> &fl, const AAFieldsT<R> &s,
+                                            const AABuf &ab, const AASel &sel, const double *gsrc, int i0, int j0,
+                                            int tx, int ty, double (*sp)[TX + 1], double (*sT)[TX + 1]) {
   double gam[NM];
 #pragma unroll
   for (int j = 0; j < NM; j++) gam[j] = gsrc[j];
@@ -1748,9 +1749,12 @@ __device__ __forceinline__ void aa_mix_tile(const KGrid &g, const KFields &fl, c
   const int i = i0 + tx, j = j0 + ty;
   const bool in = i < Nx && j < Ny;
   const Nbr nb((unsigned)(in ? i : 0), (unsigned)(in ? j : 0), (unsigned)Nx, (unsigned)Ny, g.px);
+  // the mixed values as the state holds them (rounded to R), here and in LDS
   double x[4] = {0.0, 0.0, 0.0, 0.0};
   if (in) {
     aa_mixed<NM>(ab, sel, gam, nb.ij, x);
+#pragma unroll
+    for (int q = 0; q < 4; q++) x[q] = (double)(R)x[q];
     sp[ty + 1][tx + 1] = x[2];
     sT[ty + 1][tx + 1] = x[3];
   }
@@ -1759,52 +1763,53 @@ __device__ __forceinline__ void aa_mix_tile(const KGrid &g, const KFields &fl, c
   double h[4];
   if (ty == 0 && in) {
     aa_mixed<NM>(ab, sel, gam, nb.ij_1, h);
-    sp[0][tx + 1] = h[2];
-    sT[0][tx + 1] = h[3];
+    sp[0][tx + 1] = (double)(R)h[2];
+    sT[0][tx + 1] = (double)(R)h[3];
   }
   if (ty == 1 && tx < TY && i0 < Nx && j0 + tx < Ny) {
     const Nbr w((unsigned)i0, (unsigned)(j0 + tx), (unsigned)Nx, (unsigned)Ny, g.px);
     aa_mixed<NM>(ab, sel, gam, w.i_1j, h);
-    sp[tx + 1][0] = h[2];
-    sT[tx + 1][0] = h[3];
+    sp[tx + 1][0] = (double)(R)h[2];
+    sT[tx + 1][0] = (double)(R)h[3];
   }
   __syncthreads();
   if (!in) return;
-  CorrectIn<double> ci;
+  CorrectIn<R> ci;
   ci.in = true;
   ci.cu = ci.cv = false;
   ci.ij = nb.ij;
   ci.i_1j = nb.i_1j;
   ci.ij_1 = nb.ij_1;
-  ci.mc = g.m[nb.ij];
-  ci.mw = g.m[nb.i_1j];
-  ci.ms = g.m[nb.ij_1];
-  ci.uf = x[0];
-  ci.vf = x[1];
-  ci.pc = x[2];
-  ci.Tc = x[3];
-  ci.pw = sp[ty + 1][tx];
-  ci.Tw = sT[ty + 1][tx];
-  ci.ps = sp[ty][tx + 1];
-  ci.Ts = sT[ty][tx + 1];
+  const unsigned bits = g.nb[nb.ij];
+  ci.mc = (bits & NB_C) != 0;
+  ci.mw = (bits & NB_W) != 0;
+  ci.ms = (bits & NB_S) != 0;
+  ci.uf = (R)x[0];
+  ci.vf = (R)x[1];
+  ci.pc = (R)x[2];
+  ci.Tc = (R)x[3];
+  ci.pw = (R)sp[ty + 1][tx];
+  ci.Tw = (R)sT[ty + 1][tx];
+  ci.ps = (R)sp[ty][tx + 1];
+  ci.Ts = (R)sT[ty][tx + 1];
   double d[4], gp[4];
   load4(ab.gp + 4 * (size_t)nb.ij, gp);
 #pragma unroll
   for (int q = 0; q < 4; q++) {
-    s.x[q][nb.ij] = x[q];
+    s.x[q][nb.ij] = (R)x[q];
     d[q] = x[q] - gp[q];
   }
   store4(ab.D + 4 * (size_t)nb.ij, d);
-  KFields kf = fl;
+  KFieldsT<R> kf = fl;
   kf.f[STS_P] = s.x[2];
   kf.f[STS_T] = s.x[3];
   unsigned long long rr[2] = {0ull, 0ull};
-  double rho_uv[2];
+  R rho_uv[2];
   ci.apply(kf, rr, rho_uv);
   if (!FLUX) return;
   // the loop's flux pass on the mixed state (fluxes, thermal coefficients):
   // its inputs at this cell and the west and south sqrt_T are all here
-  FluxIn<double> fi;
+  FluxIn<R> fi;
   fi.in = true;
   fi.ij = nb.ij;
   fi.hxi = g.hx[i];
@@ -1812,9 +1817,9 @@ __device__ __forceinline__ void aa_mix_tile(const KGrid &g, const KFields &fl, c
   fi.hyj = g.hy[j];
   fi.hys = g.hy[j - 1 * (0 < j)];
   fi.ru = rho_uv[0];
-  fi.u = x[0];
+  fi.u = (R)x[0];
   fi.rv = rho_uv[1];
-  fi.v = x[1];
+  fi.v = (R)x[1];
   fi.m10 = ci.ms;
   fi.m01 = ci.mw;
   fi.m11 = ci.mc;
@@ -1831,9 +1836,9 @@ __device__ __forceinline__ void aa_mix_tile(const KGrid &g, const KFields &fl, c
 // weights; workgroup 0 stores the Gram row), then mixes AA_MIX_TILES tiles
 // of AA_MIX_TX x AA_MIX_TY cells in turn.
 constexpr int AA_MIX_TX = 64, AA_MIX_TY = 8, AA_MIX_TILES = 3;
-template <int NM>
-__global__ void __launch_bounds__(AA_MIX_TX *AA_MIX_TY) k_aa_mix(KGrid g, KFields fl, AAFields s, AABuf ab,
-                                                                 AASel sel, Gate gate) {
+template <int NM, typename R>
+__global__ void __launch_bounds__(AA_MIX_TX *AA_MIX_TY) k_aa_mix(KGridT<R> g, KFieldsT<R> fl, AAFieldsT<R> s,
+                                                                 AABuf ab, AASel sel, Gate gate) {
   if (gated_out(gate)) return;
   __shared__ double gam[NM];
   aa_solve_group<NM, AA_MIX_TX * AA_MIX_TY>(ab, sel, gam, blockIdx.x == 0);
@@ -1842,8 +1847,8 @@ __global__ void __launch_bounds__(AA_MIX_TX *AA_MIX_TY) k_aa_mix(KGrid g, KField
   const int ntx = (g.Nx + AA_MIX_TX - 1) / AA_MIX_TX, ntiles = ntx * ((g.Ny + AA_MIX_TY - 1) / AA_MIX_TY);
   const int tx = (int)threadIdx.x % AA_MIX_TX, ty = (int)threadIdx.x / AA_MIX_TX;
   for (int t = (int)blockIdx.x; t < ntiles; t += (int)gridDim.x) {
-    aa_mix_tile<NM, AA_MIX_TX, AA_MIX_TY, true>(g, fl, s, ab, sel, gam, (t % ntx) * AA_MIX_TX,
-                                               (t / ntx) * AA_MIX_TY, tx, ty, sp, sT);
+    aa_mix_tile<NM, AA_MIX_TX, AA_MIX_TY, true, R>(g, fl, s, ab, sel, gam, (t % ntx) * AA_MIX_TX,
+                                                  (t / ntx) * AA_MIX_TY, tx, ty, sp, sT);
     __syncthreads();  // the next tile reuses sp, sT
   }
 }
@@ -2340,26 +2345,40 @@ hipError_t launch_extrapolate(const KGrid &g, const KFields &f, const ExtrapHist
   return hipGetLastError();
 }
 int aa_store_blocks(int Na) { return (Na + AA_THREADS * AA_STORE_CELLS - 1) / (AA_THREADS * AA_STORE_CELLS); }
-hipError_t launch_aa_begin(const AAFields &f, const AABuf &ab, int Na, hipStream_t s) {
-  hipLaunchKernelGGL(k_aa_begin, dim3((Na + AA_THREADS - 1) / AA_THREADS), dim3(AA_THREADS), 0, s, f, ab, Na);
+template <typename R>
+hipError_t launch_aa_begin(const AAFieldsT<R> &f, const AABuf &ab, int Na, hipStream_t s) {
+  hipLaunchKernelGGL(k_aa_begin<R>, dim3((Na + AA_THREADS - 1) / AA_THREADS), dim3(AA_THREADS), 0, s, f, ab, Na);
   return hipGetLastError();
 }
-hipError_t launch_aa_store(const AAFields &f, const AABuf &ab, const AASel &sel, int Na, hipStream_t s, Gate gate) {
+template <typename R>
+hipError_t launch_aa_store(const AAFieldsT<R> &f, const AABuf &ab, const AASel &sel, int Na, hipStream_t s,
+                           Gate gate) {
   if (sel.n < 0 || sel.n > MAX_AA || ab.nb != aa_store_blocks(Na)) return hipErrorInvalidValue;
-  if (sel.n <= 4) hipLaunchKernelGGL(k_aa_store<4>, dim3(ab.nb), dim3(AA_THREADS), 0, s, f, ab, sel, Na, gate);
-  else hipLaunchKernelGGL(k_aa_store<MAX_AA>, dim3(ab.nb), dim3(AA_THREADS), 0, s, f, ab, sel, Na, gate);
+  if (sel.n <= 4) hipLaunchKernelGGL((k_aa_store<4, R>), dim3(ab.nb), dim3(AA_THREADS), 0, s, f, ab, sel, Na, gate);
+  else hipLaunchKernelGGL((k_aa_store<MAX_AA, R>), dim3(ab.nb), dim3(AA_THREADS), 0, s, f, ab, sel, Na, gate);
   return hipGetLastError();
 }
-hipError_t launch_aa_mix(const KGrid &g, const KFields &fl, const AAFields &f, const AABuf &ab, const AASel &sel,
-                         hipStream_t s, Gate gate) {
+template <typename R>
+hipError_t launch_aa_mix(const KGridT<R> &g, const KFieldsT<R> &fl, const AAFieldsT<R> &f, const AABuf &ab,
+                         const AASel &sel, hipStream_t s, Gate gate) {
   const int Na = g.Nx * g.Ny;
   if (sel.n < 1 || sel.n > MAX_AA || ab.nb != aa_store_blocks(Na)) return hipErrorInvalidValue;
   const int ntiles = ((g.Nx + AA_MIX_TX - 1) / AA_MIX_TX) * ((g.Ny + AA_MIX_TY - 1) / AA_MIX_TY);
   const dim3 gm((ntiles + AA_MIX_TILES - 1) / AA_MIX_TILES);
-  if (sel.n <= 4) hipLaunchKernelGGL(k_aa_mix<4>, gm, dim3(AA_MIX_TX * AA_MIX_TY), 0, s, g, fl, f, ab, sel, gate);
-  else hipLaunchKernelGGL(k_aa_mix<MAX_AA>, gm, dim3(AA_MIX_TX * AA_MIX_TY), 0, s, g, fl, f, ab, sel, gate);
+  if (sel.n <= 4)
+    hipLaunchKernelGGL((k_aa_mix<4, R>), gm, dim3(AA_MIX_TX * AA_MIX_TY), 0, s, g, fl, f, ab, sel, gate);
+  else hipLaunchKernelGGL((k_aa_mix<MAX_AA, R>), gm, dim3(AA_MIX_TX * AA_MIX_TY), 0, s, g, fl, f, ab, sel, gate);
   return hipGetLastError();
 }
+template hipError_t launch_aa_begin<double>(const AAFields &, const AABuf &, int, hipStream_t);
+template hipError_t launch_aa_begin<float>(const AAFieldsT<float> &, const AABuf &, int, hipStream_t);
+template hipError_t launch_aa_store<double>(const AAFields &, const AABuf &, const AASel &, int, hipStream_t, Gate);
+template hipError_t launch_aa_store<float>(const AAFieldsT<float> &, const AABuf &, const AASel &, int, hipStream_t,
+                                           Gate);
+template hipError_t launch_aa_mix<double>(const KGrid &, const KFields &, const AAFields &, const AABuf &,
+                                          const AASel &, hipStream_t, Gate);
+template hipError_t launch_aa_mix<float>(const KGridT<float> &, const KFieldsT<float> &, const AAFieldsT<float> &,
+                                         const AABuf &, const AASel &, hipStream_t, Gate);
 hipError_t launch_set_inflow(const KGrid &g, const KFields &f, double u_in, hipStream_t s) {
   dim3 gr((g.Nx_u0 + BX - 1) / BX > 0 ? (g.Nx_u0 + BX - 1) / BX : 1, (g.Ny + BY - 1) / BY);
   hipLaunchKernelGGL(k_set_inflow, gr, dim3(BX, BY), 0, s, g, f, u_in);

@@ -58,10 +58,16 @@ hipError_t launch_extrapolate(const KGrid &g, const KFields &f, const ExtrapHist
 // MAX_AA^2; ab.gam: MAX_AA; ab.gp: 4 Na doubles; ab.D, ab.fp and every
 // sel.dG/dF: 4 Na floats.
 int aa_store_blocks(int Na);
-hipError_t launch_aa_begin(const AAFields &f, const AABuf &ab, int Na, hipStream_t s);
-hipError_t launch_aa_store(const AAFields &f, const AABuf &ab, const AASel &sel, int Na, hipStream_t s, Gate gate);
-hipError_t launch_aa_mix(const KGrid &g, const KFields &fl, const AAFields &f, const AABuf &ab, const AASel &sel,
-                         hipStream_t s, Gate gate);
+// R: the loop's precision (double, or float in the FP32 phase of the mixed
+// time step); the acceleration's vectors are the same for both.
+template <typename R>
+hipError_t launch_aa_begin(const AAFieldsT<R> &f, const AABuf &ab, int Na, hipStream_t s);
+template <typename R>
+hipError_t launch_aa_store(const AAFieldsT<R> &f, const AABuf &ab, const AASel &sel, int Na, hipStream_t s,
+                           Gate gate);
+template <typename R>
+hipError_t launch_aa_mix(const KGridT<R> &g, const KFieldsT<R> &fl, const AAFieldsT<R> &f, const AABuf &ab,
+                         const AASel &sel, hipStream_t s, Gate gate);
 hipError_t launch_loop_open(LoopCtrl *ctrl, unsigned long long *log, int n, unsigned long long *lred,
                             hipStream_t s);
 template <typename S, typename D>
