@@ -23,19 +23,22 @@ step ends on the reference's loop-2 test (1e-8):
                    met the test within the recent iteration count of steps
                    from the previous state is restarted from that state
                    (sts_time_step_mixed, extrapolate = 2, guard = 64);
-  warm_mixed       the same guarded predicted start, plus an FP32 loop-2
-                   phase (at most 200 outer iterations) on steps whose
-                   predecessor needed more than 80;
   anderson         FP64 loop 2 from the previous step's state with Anderson
                    acceleration of the outer iterations (history depth 2);
   accelerated      the headline path: the guarded predicted start of degree
-                   3 with Anderson acceleration (depth 1);
-  mixed            FP32 phase then FP64 phase from the previous step's state
-                   (no predicted start);
-  surrogate        the mixed path warm-started by the trained BF16 U-Net with
-                   its residual-based fallback (sts_time_step_ws), when
-                   trained parameters are present (simple-ts_amd/unet_b32.npz,
-                   written by train_unet.py).
+                   3 with Anderson acceleration (depth 1), all FP64;
+  two_phase        an FP32 loop-2 phase until it stagnates, then the FP64
+                   phase, from the previous step's state;
+  mixed            the guarded degree-3 predicted start, 6 FP32 outer
+                   iterations with Anderson acceleration, then the FP64
+                   accelerated phase whose FP64 residuals, equation of state
+                   and test decide the step;
+  surrogate        the mixed path with the trained corrector network refining
+                   each predicted start (sts_set_corrector, kept only when it
+                   lowers the first sweep's residual), when its parameters
+                   are present (simple-ts_amd/unet_corr_b32.npz, written by
+                   train_unet.py --corrector);
+  surrogate_accelerated  the accelerated path with the same corrector.
 value = Nx*Ny*K / timed seconds / 1e6 (Mcell.steps/s) of accelerated; every
 path's figure, its speed-up over the faster of fp64 and plain, and every
 field's relative L2 error against the fp64 path after the W + K steps are
@@ -94,23 +97,25 @@ HBM_PEAK_GBS = 8000.0
 ASSEMBLE_BYTES_PER_CELL = 128 + 128 + 40
 SWEEP2_BYTES_PER_CELL = 144
 CPU_STEPS = 2  # steps of the CPU sample (after one restart step)
-WEIGHTS = os.path.join(ROOT, "simple-ts_amd", "unet_b32.npz")
-MIXED = (5000, (1e-7, 1e-7, 1e-7), 6, 0.95)  # N_I32, FP32 tolerances, stagnation window and ratio
+WEIGHTS = os.path.join(ROOT, "simple-ts_amd", "unet_corr_b32.npz")
+TOL32 = (1e-7, 1e-7, 1e-7)  # FP32-phase tolerances
 DEGREE = 2        # predicted start: polynomial degree in time
 GUARD = 64        # predicted start: restart from x^n past the budget, pauses up to this many steps
-FP32_ABOVE = 80   # warm_mixed: FP32 phase only after a step that needed more outer iterations than this
-N_I32_WARM = 200  # warm_mixed: cap on the FP32 phase
+N_I32_MIXED = 6   # mixed: FP32 outer iterations (accelerated) before the FP64 phase
 AA_DEPTH = 1      # accelerated: Anderson history depth (with the predicted start)
 AA_DEPTH_FULL = 2  # Anderson depth without a predicted start and in the batch (from-rest transients)
 AA_DEGREE = 3     # accelerated: predicted-start degree
-PATHS = {  # name: (FP32 cap, extrapolation degree, FP32 gate, guard, network, Anderson depth)
+_ACC = dict(extrapolate=AA_DEGREE, guard=GUARD, anderson=AA_DEPTH)
+_MIX = dict(_ACC, n32=N_I32_MIXED, anderson32=AA_DEPTH)
+PATHS = {  # sts_time_step_mixed settings of each path (n32: FP32 cap; stall: FP32 stagnation window, ratio)
     "plain": None,  # sts_time_step through the same binding as the other paths
-    "predicted": (0, DEGREE, 0, GUARD, False, 0),
-    "warm_mixed": (N_I32_WARM, DEGREE, FP32_ABOVE, GUARD, False, 0),
-    "anderson": (0, 0, 0, 0, False, AA_DEPTH_FULL),
-    "accelerated": (0, AA_DEGREE, 0, GUARD, False, AA_DEPTH),
-    "mixed": (MIXED[0], 0, 0, 0, False, 0),
-    "surrogate": (MIXED[0], 0, 0, 0, True, 0),
+    "predicted": dict(extrapolate=DEGREE, guard=GUARD),
+    "anderson": dict(anderson=AA_DEPTH_FULL),
+    "accelerated": _ACC,
+    "two_phase": dict(n32=5000, stall=(6, 0.95)),
+    "mixed": _MIX,
+    "surrogate": dict(_MIX, net=True),
+    "surrogate_accelerated": dict(_ACC, net=True),
 }
 HEADLINE = "accelerated"
 
@@ -146,14 +151,20 @@ def run_driver(steps, warmup, kernel="assemble_inl,sweep"):
 
 
 def run_python_path(steps, warmup, name, net=None, nx=NX, ny=NY, cfl=1.0):
-    """One of PATHS through the Python binding of the C ABI."""
-    n_i32, degree, above, guard, _, aa = PATHS[name] or (0, 0, 0, 0, False, 0)
+    """One of PATHS through the Python binding of the C ABI; net: the
+    corrector for the paths that use one."""
+    spec = dict(PATHS[name] or {})
+    n_i32 = spec.pop("n32", 0)
+    stall_w, stall_r = spec.pop("stall", (0, 0.95))
+    spec.pop("net", None)
     ws = importlib.import_module("simple-ts_amd.warmstart")
     work = tempfile.mkdtemp(prefix="sts_bench_")
     try:
         run = ws.Run(workload_case(os.path.join(work, "c"), warmup + steps, nx, ny, cfl),
                      precision=32 if n_i32 else 64)
-        n32 = n64 = taken = 0
+        if net is not None:
+            run.ctx.set_corrector(net, ws.corr_norm(run.case))
+        n32 = n64 = 0
         t0 = None
         print(f"[bench] {name} path: {warmup} + {steps} steps", file=sys.stderr, flush=True)
         for s in range(warmup + steps):
@@ -161,20 +172,19 @@ def run_python_path(steps, warmup, name, net=None, nx=NX, ny=NY, cfl=1.0):
                 t0 = time.perf_counter()
             if PATHS[name] is None:
                 a, b = 0, run.ctx.time_step(*run.loop)[0]
-            elif net is None:
-                a, b, _, _ = run.ctx.time_step_mixed(*run.loop, n_i32, *MIXED[1:], extrapolate=degree,
-                                                     fp32_above=above, guard=guard, anderson=aa)
             else:
-                u, a, b = run.step_ws(net, mixed=(n_i32, *MIXED[1:], degree, above, guard, aa))
-                taken += u if s >= warmup else 0
+                a, b, _, _ = run.ctx.time_step_mixed(*run.loop, n_i32, TOL32, stall_w, stall_r, **spec)
             if s >= warmup:
                 n32, n64 = n32 + a, n64 + b
         dt = time.perf_counter() - t0
         stats = run.ctx.predict_stats()
+        corr = run.ctx.corrector_stats() if net is not None else None
+        # the FP64 mass-imbalance check of the final state (S:10621-10646 over the fluid cells)
+        mass = run.ctx.continuity()[2]
         run.ctx.jump_overwrite()  # as the program does before it writes the fields
         f = {k: run.ctx.get(k) for k in ("u", "v", "p", "Temper")}
         run.close()
-        return dict(seconds=dt, fp32_it=n32, fp64_it=n64, taken=taken, stats=stats), f
+        return dict(seconds=dt, fp32_it=n32, fp64_it=n64, stats=stats, corr=corr, mass=mass.tolist()), f
     finally:
         shutil.rmtree(work, ignore_errors=True)
 
@@ -237,12 +247,13 @@ def measure_traffic():
         shutil.rmtree(work, ignore_errors=True)
 
 
-def load_net():
+def load_net(nx=NX, ny=NY):
+    """The trained corrector for an nx x ny grid, or (None, None)."""
     if not os.path.exists(WEIGHTS):
         return None, None
     pkg = importlib.import_module("simple-ts_amd")
     z = np.load(WEIGHTS, allow_pickle=False)
-    net = pkg.UNet(int(z["cin"]), int(z["cout"]), int(z["base"]), 1, NY, NX)
+    net = pkg.UNet(int(z["cin"]), int(z["cout"]), int(z["base"]), 1, ny, nx)
     net.set_params(z["params"])
     return net, {"samples": int(z["samples"]), "train_grid": z["grid"].tolist(),
                  "holdout_loss": float(z["holdout_loss"][-1])}
@@ -433,7 +444,7 @@ def main():
         # (a non-finite residual within the first 500 steps); half of it converges
         nx, ny, cfl = 2 * NX, 2 * NY, 0.5
         paths, ref = {}, None
-        for name in ("plain", "mixed", "warm_mixed", "accelerated"):
+        for name in ("plain", "two_phase", "mixed", "accelerated"):
             try:
                 r, f = run_python_path(args.steps, args.warmup, name, None, nx, ny, cfl)
             except Exception as e:  # a path that fails is reported, the others still run
@@ -449,7 +460,8 @@ def main():
                            "fp32_outer_iterations_per_step": r["fp32_it"] / args.steps,
                            "fp64_outer_iterations_per_step": r["fp64_it"] / args.steps,
                            "field_rel_l2_vs_fp64": {k: rel_l2(f[k], ref[k]) for k in f},
-                           "field_max_abs_vs_fp64": {k: float(np.max(np.abs(f[k] - ref[k]))) for k in f}}
+                           "field_max_abs_vs_fp64": {k: float(np.max(np.abs(f[k] - ref[k]))) for k in f},
+                           "mass_imbalance_max_l2_l1": r["mass"]}
         for name in paths:
             if "value" in paths[name] and "value" in paths.get("plain", {}):
                 paths[name]["speedup_vs_fp64"] = paths[name]["value"] / paths["plain"]["value"]
@@ -459,18 +471,19 @@ def main():
             "unit": "Mcell·steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "f32+f64", "data": "synthetic: confined-square channel in the reference's input format",
-            "config": {"workload": f"supersonic confined square, {nx}x{ny}, dt {0.01 * cfl * 240 / nx:g} H/u_in, FP32 "
-                                   f"loop-2 phase then FP64 phase on the reference's test; errors against the "
-                                   f"plain FP64 path on the same grid after warmup + steps steps",
+            "config": {"workload": f"supersonic confined square, {nx}x{ny}, dt {0.01 * cfl * 240 / nx:g} H/u_in; mixed: "
+                                   f"predicted start, {N_I32_MIXED} accelerated FP32 outer iterations, then the "
+                                   f"accelerated FP64 phase on the reference's test; errors against the plain FP64 "
+                                   f"path on the same grid after warmup + steps steps",
                        "Nx": nx, "Ny": ny},
             "paths": paths}))
         return
     if args.workload == "batch":
         out = {"fp64": run_batch_workload(args.cases, args.steps, args.warmup, None),
-               "warm_mixed": run_batch_workload(args.cases, args.steps, args.warmup,
-                                                (N_I32_WARM, *MIXED[1:], DEGREE, FP32_ABOVE, GUARD)),
+               "mixed": run_batch_workload(args.cases, args.steps, args.warmup,
+                                           (N_I32_MIXED, TOL32, 0, 0.95, AA_DEGREE, 0, GUARD, AA_DEPTH_FULL)),
                "accelerated": run_batch_workload(args.cases, args.steps, args.warmup,
-                                                 (0, *MIXED[1:], AA_DEGREE, 0, GUARD, AA_DEPTH_FULL))}
+                                                 (0, TOL32, 0, 0.95, AA_DEGREE, 0, GUARD, AA_DEPTH_FULL))}
         r = out["accelerated"]
         print(json.dumps({
             "metric": "Mcell·steps/s to 1e-8 residual, batched uneven cases", "value": r["batched"],
@@ -493,7 +506,7 @@ def main():
     if not args.fp64_only:
         for name, spec in PATHS.items():
             net = info = None
-            if spec and spec[4]:
+            if spec and spec.get("net"):
                 net, info = load_net()
                 if net is None:
                     continue
@@ -502,14 +515,15 @@ def main():
             paths[name] = {"value": v, "ms_per_step": 1e3 * r["seconds"] / args.steps,
                            "fp32_outer_iterations_per_step": r["fp32_it"] / args.steps,
                            "fp64_outer_iterations_per_step": r["fp64_it"] / args.steps,
-                           "field_rel_l2_vs_fp64": {k: rel_l2(f[k], f64[k]) for k in f}}
-            if spec and spec[3]:
+                           "field_rel_l2_vs_fp64": {k: rel_l2(f[k], f64[k]) for k in f},
+                           "mass_imbalance_max_l2_l1": r["mass"]}
+            if spec and spec.get("guard"):
                 s = r["stats"]
                 paths[name]["guard"] = {"predicted_steps": s[0], "restarted": s[1],
                                         "abandoned_fp32_iterations": s[2], "abandoned_fp64_iterations": s[3],
                                         "steps_from_previous_state": s[4]}
             if net is not None:
-                paths[name]["prediction_taken_steps"] = r["taken"]
+                paths[name]["corrector"] = {"kept": r["corr"][0], "rejected": r["corr"][1]}
                 paths[name]["network"] = info
                 net.close()
             if name == HEADLINE:
@@ -532,11 +546,10 @@ def main():
                                 f"fields rebuilt), guarded, + Anderson acceleration of the FP64 outer iterations "
                                 f"(history depth {AA_DEPTH}, FP32 differences), every step ending on the "
                                 f"reference's test",
-                 "warm_mixed": f"start predicted by degree-{DEGREE} extrapolation in time (every cell, derived "
-                               f"fields rebuilt), guarded (restart from the previous state when loop 2 exceeds "
-                               f"the recent iteration count of such starts) + FP32 loop-2 phase (at most "
-                               f"{N_I32_WARM}) after steps of more than {FP32_ABOVE} outer iterations + FP64 "
-                               f"phase on the reference's test"}
+                 "mixed": f"start predicted by degree-{AA_DEGREE} extrapolation in time, guarded, "
+                          f"{N_I32_MIXED} FP32 outer iterations with Anderson acceleration, then FP64 outer "
+                          f"iterations with Anderson acceleration whose FP64 test ends the step",
+                 "surrogate": "the mixed path with the trained corrector network refining each predicted start"}
     line = {
         "metric": METRIC, "value": best, "unit": "Mcell·steps/s", "n_gpus": 1, "steps": t["timed_steps"],
         "warmup": t["warmup_steps"], "ms_per_step": paths[best_name]["ms_per_step"], "higher_is_better": True,

@@ -67,6 +67,36 @@ int sts_unet_train_step(sts_unet *net, const int *slots, int n, float lr, double
 /* Loss of the current parameters on recorded samples (no update). */
 int sts_unet_eval(sts_unet *net, const int *slots, int n, double *loss);
 
+/* The corrector: the same network with cin = 9, cout = 4, refining the
+   start that the polynomial extrapolation in time predicts
+   (sts_mixed_control.extrapolate).  Input: the state x^n the step starts
+   from (u/u_ref, v/u_ref, p/p_ref - 1, T - 1), the predicted start's change
+   x_e - x^n in the same units divided by `step`, and the fluid mask.
+   Output: (x^(n+1) - x_e) in the same units divided by `corr` -- the part of
+   the converged step that the extrapolation misses. */
+typedef struct sts_unet_corr_norm {
+  double u_ref, p_ref, step, corr;
+} sts_unet_corr_norm;
+
+/* The context keeps a copy of each step's start of loop 2 (the predicted
+   start, or x^n when the step is not predicted), for recording; on != 0
+   allocates it (4 fields), 0 frees it. */
+int sts_keep_predicted_start(sts_ctx *ctx, int on);
+/* Records corrector sample `slot` right after a time step of a context that
+   keeps its predicted start: input from x^n and the kept start, target from
+   the converged state. */
+int sts_unet_record_correction(sts_unet *net, int slot, sts_ctx *ctx, const sts_unet_corr_norm *nrm);
+/* Attaches a corrector (net == NULL detaches): every predicted step of
+   sts_time_step_mixed then refines the extrapolated start with the network
+   on the loop's update sets, and keeps the refinement only if it lowers the
+   first sweep's residual max(max|dp|/tol_p, max|dT|/tol_T) (each start with
+   its own assembly); otherwise the step runs from the extrapolated start.
+   The network must be of the context's grid (H = Ny, W = Nx). */
+int sts_set_corrector(sts_ctx *ctx, sts_unet *net, const sts_unet_corr_norm *nrm);
+/* out[0]: predicted steps the corrector refined and kept, out[1]: refinements
+   rejected by the residual test. */
+int sts_corrector_stats(sts_ctx *ctx, long long out[2]);
+
 /* Time step with the network's warm start: prologue, prediction, choice of
    the starting p, T by the first sweep's residual (the prediction is used
    only if it lowers max(max|dp|/tol_p, max|dT|/tol_T) of that sweep), then

@@ -91,6 +91,12 @@ class UNetNorm(ctypes.Structure):
                 ("dT_scale", ctypes.c_double)]
 
 
+class CorrNorm(ctypes.Structure):
+    """sts_unet_corr_norm: the corrector's normalisation."""
+    _fields_ = [("u_ref", ctypes.c_double), ("p_ref", ctypes.c_double), ("step", ctypes.c_double),
+                ("corr", ctypes.c_double)]
+
+
 class Problem:
     """A SIMPLE-TS case as the time step sees it: mesh, body mask and table,
     solve boxes, scalars and boundary-condition switches.  Keeps the numpy
@@ -192,6 +198,10 @@ def lib():
                                   ctypes.POINTER(MixedControl), ctypes.POINTER(c_int), ctypes.POINTER(c_int),
                                   ctypes.POINTER(c_int), _DP, _DP]),
             ("sts_kernel_time", [vp, c_int, ctypes.POINTER(ctypes.c_longlong), _DP]),
+            ("sts_keep_predicted_start", [vp, c_int]),
+            ("sts_unet_record_correction", [vp, c_int, vp, ctypes.POINTER(CorrNorm)]),
+            ("sts_set_corrector", [vp, vp, ctypes.POINTER(CorrNorm)]),
+            ("sts_corrector_stats", [vp, ctypes.POINTER(ctypes.c_longlong)]),
         ]:
             fn = getattr(L, name)
             fn.argtypes = args
@@ -355,6 +365,21 @@ class Context:
         n = n32.value + n64.value
         return used.value, n32.value, n64.value, log[:6 * n].reshape(-1, 6), float(pr[0])
 
+    def keep_predicted_start(self, on=True):
+        _check(lib().sts_keep_predicted_start(self.h, 1 if on else 0), "sts_keep_predicted_start")
+
+    def set_corrector(self, net, norm=None):
+        """Attach the corrector network (UNet with cin 9, cout 4; None
+        detaches) with its CorrNorm."""
+        _check(lib().sts_set_corrector(self.h, net.h if net is not None else None,
+                                       ctypes.byref(norm) if norm is not None else None), "sts_set_corrector")
+
+    def corrector_stats(self):
+        """(refinements kept, refinements rejected)."""
+        out = (ctypes.c_longlong * 2)()
+        _check(lib().sts_corrector_stats(self.h, out), "sts_corrector_stats")
+        return tuple(out)
+
     def predict_stats(self):
         """(predicted steps, restarted ones, FP32 and FP64 iterations of the
         abandoned attempts, steps run from x^n by the guard)."""
@@ -496,6 +521,10 @@ class UNet:
 
     def record(self, slot, ctx, norm):
         _check(lib().sts_unet_record(self.h, slot, ctx.h, ctypes.byref(norm)), "sts_unet_record")
+
+    def record_correction(self, slot, ctx, norm):
+        _check(lib().sts_unet_record_correction(self.h, slot, ctx.h, ctypes.byref(norm)),
+               "sts_unet_record_correction")
 
     def train_step(self, slots, lr):
         s = np.ascontiguousarray(slots, dtype=np.int32)

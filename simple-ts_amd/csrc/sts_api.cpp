@@ -91,6 +91,12 @@ struct sts_ctx {
   double *drag_terms = nullptr, *drag_out = nullptr;
   size_t drag_cap = 0;
   double *p_ws = nullptr, *T_ws = nullptr;  // predicted starting guess (warm start)
+  // the corrector (sts_set_corrector): the kept start of loop 2 (u, v, p,
+  // T), the network, its normalisation, kept / rejected refinements
+  double *xs[4] = {};
+  sts_unet *corr_net = nullptr;
+  sts_unet_corr_norm corr_norm{};
+  long long cstat[2] = {0, 0};
   // Gamma_* and D_*: whether forming them from sqrt_T, the wall temperatures
   // and the mask gives the values the reference holds (true after a flux and
   // diffusion pass of the library, until those inputs or the arrays are set
@@ -278,6 +284,7 @@ extern "C" int sts_destroy(sts_ctx *c) {
   (void)hipFree(c->T_alt);
   for (int q = 0; q < STS_NFIELDS; q++) (void)hipFree(c->f32[q]);
   (void)hipFree(c->p_ws);
+  for (double *q : c->xs) (void)hipFree(q);
   for (auto &lv : c->hist.f)
     for (double *q : lv) (void)hipFree(q);
   for (int j = 0; j < MAX_AA; j++) {
@@ -1236,16 +1243,42 @@ static constexpr unsigned long long WIDEN_MASK =
     bit(STS_U) | bit(STS_V) | bit(STS_P) | bit(STS_T) | bit(STS_RHO) | bit(STS_SQRT_T) | bit(STS_RHO_U) |
     bit(STS_RHO_V) | bit(STS_FX) | bit(STS_FY) | bit(STS_DTX) | bit(STS_DTY) | bit(STS_TW_U) | bit(STS_TW_V);
 
+// Every field an outer iteration derives from u, v, p, T, rebuilt as the end
+// of an outer iteration builds it (after a new start of loop 2).
+static int rebuild_derived(sts_ctx *c, hipStream_t s) {
+  const KFields kf = c->fields();
+  HIP_OK(launch_correct(c->g, kf, nullptr, nullptr, 0, s));
+  HIP_OK(launch_wall(c->g, kf, STS_BC_SLIP_U | STS_BC_SLIP_V | STS_BC_JUMP, s, Gate{nullptr, 0}, c->wall_ops,
+                     c->n_wall_ops));
+  HIP_OK(launch_edges(c->g, kf, STS_BC_VELOCITIES | STS_BC_TEMPERATURE | STS_BC_PRESSURE, nullptr, s));
+  HIP_OK(launch_flux_diff(c->g, kf, s, Gate{nullptr, 0}, FLUX_PART));
+  c->diff_inline_ok = c->diff_stale = true;
+  return STS_OK;
+}
+
+// Copies u, v, p, T into the kept start (when the context keeps one), or
+// back from it (back = true).
+static int copy_start(sts_ctx *c, hipStream_t s, bool back = false) {
+  if (!c->xs[0]) return STS_OK;
+  const int f[4] = {STS_U, STS_V, STS_P, STS_T};
+  const size_t bytes = sizeof(double) * (size_t)c->Na;
+  for (int q = 0; q < 4; q++)
+    HIP_OK(hipMemcpyAsync(back ? c->f[f[q]] : c->xs[q], back ? c->xs[q] : c->f[f[q]], bytes,
+                          hipMemcpyDeviceToDevice, s));
+  return STS_OK;
+}
+
 // After the prologue: loop 2 starts from the polynomial of degree `order`
 // (capped by the stored steps) through x^n and the previous converged steps
 // for u, v, p, T, and every field derived from them is rebuilt the way the
 // end of an outer iteration builds it.  The history then advances a step.
 static int extrapolate_start(sts_ctx *c, int order, hipStream_t s = nullptr, bool store_only = false,
-                             int degree_cap = MAX_EXTRAP) {
+                             int degree_cap = MAX_EXTRAP, int *used = nullptr) {
   if (!s) s = c->stream;
+  if (used) *used = 0;
   if (order <= 0) {
     c->n_hist = 0;
-    return STS_OK;
+    return copy_start(c, s);
   }
   if (order > MAX_EXTRAP) return fail(STS_ERR_ARG, "extrapolation degree must be 0..5");
   const size_t bytes = sizeof(double) * (size_t)c->Na;
@@ -1264,14 +1297,49 @@ static int extrapolate_start(sts_ctx *c, int order, hipStream_t s = nullptr, boo
     for (int q = 0; q < 4; q++) c->hist.f[h][q] = c->hist.f[h - 1][q];
   for (int q = 0; q < 4; q++) c->hist.f[0][q] = top[q];
   c->n_hist = std::min(keep + 1, order);
-  if (use == 0) return STS_OK;
-  HIP_OK(launch_correct(c->g, kf, nullptr, nullptr, 0, s));
-  HIP_OK(launch_wall(c->g, kf, STS_BC_SLIP_U | STS_BC_SLIP_V | STS_BC_JUMP, s, Gate{nullptr, 0}, c->wall_ops,
-                     c->n_wall_ops));
-  HIP_OK(launch_edges(c->g, kf, STS_BC_VELOCITIES | STS_BC_TEMPERATURE | STS_BC_PRESSURE, nullptr, s));
-  HIP_OK(launch_flux_diff(c->g, kf, s, Gate{nullptr, 0}, FLUX_PART));
-  c->diff_inline_ok = c->diff_stale = true;
+  if (used) *used = use;
+  int rc;
+  if (use > 0 && (rc = rebuild_derived(c, s))) return rc;
+  return copy_start(c, s);
+}
+
+// The first sweep's residual max(max|dp|/tol_p, max|dT|/tol_T) from the
+// current start (its own assembly; both overwritten by the loop's first
+// iteration), through residual slots q, q + 1; *r = +inf if not finite.
+static int first_sweep_residual(sts_ctx *c, const sts_loop_control *lc, int q, double *r) {
+  HIP_OK(hipMemsetAsync(c->red, 0, sizeof(unsigned long long) * NSLOT * SLOT_STRIDE, c->stream));
+  HIP_OK(launch_assemble(c->g, c->fields(), c->stream, Gate{nullptr, 0}, c->diff_inline_ok));
+  HIP_OK(launch_sweep(c->g, c->fields(), (const double *)c->f[STS_P], (const double *)c->f[STS_T], c->p_alt,
+                      c->T_alt, c->red + q, c->red + q + 1, c->stream));
+  HIP_OK(launch_fold(c->red, q + 2, c->red_out, c->stream));
+  unsigned long long w[2];
+  HIP_OK(hipMemcpyAsync(w, c->red_out + q, sizeof w, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  double a, b;
+  std::memcpy(&a, &w[0], sizeof a);
+  std::memcpy(&b, &w[1], sizeof b);
+  *r = (std::isfinite(a) && std::isfinite(b)) ? std::max(a / lc->tol_p, b / lc->tol_T) : HUGE_VAL;
   return STS_OK;
+}
+
+// The corrector on a predicted start (see sts_set_corrector): the network's
+// refinement is kept only if it lowers the first sweep's residual.
+static int corrector_refine(sts_ctx *c, const sts_loop_control *lc) {
+  int rc;
+  double rA, rB;
+  if ((rc = first_sweep_residual(c, lc, 2, &rA))) return rc;
+  StsView v;
+  sts_internal_view(c, &v);
+  double *const out[4] = {c->f[STS_U], c->f[STS_V], c->f[STS_P], c->f[STS_T]};
+  if ((rc = sts_internal_unet_correct(c->corr_net, v, &c->corr_norm, out))) return rc;
+  if ((rc = rebuild_derived(c, c->stream)) || (rc = first_sweep_residual(c, lc, 4, &rB))) return rc;
+  if (rB < rA) {
+    c->cstat[0]++;
+    return STS_OK;
+  }
+  c->cstat[1]++;
+  if ((rc = copy_start(c, c->stream, true))) return rc;
+  return rebuild_derived(c, c->stream);
 }
 
 // Whether this step of context c runs an FP32 phase (mc asks for one and the
@@ -1342,14 +1410,7 @@ static int restart_from_previous(sts_ctx *c) {
   const int src[4] = {STS_U_PR, STS_V_PR, STS_P_PR, STS_T_PR}, dst[4] = {STS_U, STS_V, STS_P, STS_T};
   for (int q = 0; q < 4; q++)
     HIP_OK(hipMemcpyAsync(c->f[dst[q]], c->f[src[q]], bytes, hipMemcpyDeviceToDevice, c->stream));
-  const KFields kf = c->fields();
-  HIP_OK(launch_correct(c->g, kf, nullptr, nullptr, 0, c->stream));
-  HIP_OK(launch_wall(c->g, kf, STS_BC_SLIP_U | STS_BC_SLIP_V | STS_BC_JUMP, c->stream, Gate{nullptr, 0},
-                     c->wall_ops, c->n_wall_ops));
-  HIP_OK(launch_edges(c->g, kf, STS_BC_VELOCITIES | STS_BC_TEMPERATURE | STS_BC_PRESSURE, nullptr, c->stream));
-  HIP_OK(launch_flux_diff(c->g, kf, c->stream, Gate{nullptr, 0}, FLUX_PART));
-  c->diff_inline_ok = c->diff_stale = true;
-  return STS_OK;
+  return rebuild_derived(c, c->stream);
 }
 
 // Whether the last FP64 log entry of a loop 2 of n outer iterations (log
@@ -1372,7 +1433,9 @@ extern "C" int sts_time_step_mixed(sts_ctx *c, const sts_loop_control *lc, const
   if ((rc = sts_save_previous(c))) return rc;
   int n32 = 0, n64 = 0;
   if (mc->guard <= 0 || mc->extrapolate <= 0) {
-    if ((rc = extrapolate_start(c, mc->extrapolate))) return rc;
+    int used = 0;
+    if ((rc = extrapolate_start(c, mc->extrapolate, nullptr, false, MAX_EXTRAP, &used))) return rc;
+    if (used > 0 && c->corr_net && (rc = corrector_refine(c, lc))) return rc;
     rc = run_phases(c, lc, mc, &n32, &n64);
   } else {
     // guarded predicted start: the degree drops by one after an abandoned
@@ -1380,7 +1443,9 @@ extern "C" int sts_time_step_mixed(sts_ctx *c, const sts_loop_control *lc, const
     // attempt pauses prediction
     if (c->pred_degree < 0 || c->pred_degree > mc->extrapolate) c->pred_degree = mc->extrapolate;
     const bool predict = c->n_hist > 0 && c->plain_mean > 0.0 && c->pred_pause == 0;
-    if ((rc = extrapolate_start(c, mc->extrapolate, nullptr, !predict, c->pred_degree))) return rc;
+    int used = 0;
+    if ((rc = extrapolate_start(c, mc->extrapolate, nullptr, !predict, c->pred_degree, &used))) return rc;
+    if (used > 0 && c->corr_net && (rc = corrector_refine(c, lc))) return rc;
     bool plain = !predict;
     if (predict) {
       c->pstat[0]++;
@@ -1448,7 +1513,51 @@ int sts_internal_view(sts_ctx *c, StsView *v) {
   v->box_o[1] = c->g.Nx_o1;
   v->box_o[2] = c->g.Ny_o0;
   v->box_o[3] = c->g.Ny_o1;
+  const int bu[4] = {c->g.Nx_u0, c->g.Nx_u1, c->g.Ny_u0, c->g.Ny_u1}, bv[4] = {c->g.Nx_v0, c->g.Nx_v1, c->g.Ny_v0,
+                                                                              c->g.Ny_v1};
+  for (int q = 0; q < 4; q++) {
+    v->box_u[q] = bu[q];
+    v->box_v[q] = bv[q];
+    v->xe[q] = c->xs[q];
+  }
+  v->px = c->g.px;
   v->stream = c->stream;
+  return STS_OK;
+}
+
+extern "C" int sts_keep_predicted_start(sts_ctx *c, int on) {
+  if (!c) return fail(STS_ERR_ARG, "null context");
+  HIP_OK(hipStreamSynchronize(c->stream));
+  if (!on) {
+    if (c->corr_net) return fail(STS_ERR_ARG, "a corrector is attached (it needs the kept start)");
+    for (double *&q : c->xs) {
+      (void)hipFree(q);
+      q = nullptr;
+    }
+    return STS_OK;
+  }
+  for (double *&q : c->xs)
+    if (!q) HIP_OK(hipMalloc(&q, sizeof(double) * (size_t)c->Na));
+  return copy_start(c, c->stream);
+}
+
+extern "C" int sts_set_corrector(sts_ctx *c, sts_unet *net, const sts_unet_corr_norm *nrm) {
+  if (!c || (net && !nrm)) return fail(STS_ERR_ARG, "null argument");
+  if (net && !(nrm->u_ref > 0.0 && nrm->p_ref > 0.0 && nrm->step > 0.0 && nrm->corr > 0.0))
+    return fail(STS_ERR_ARG, "corrector normalisation must be positive");
+  c->corr_net = nullptr;
+  if (!net) return STS_OK;
+  int rc = sts_keep_predicted_start(c, 1);
+  if (rc) return rc;
+  c->corr_net = net;
+  c->corr_norm = *nrm;
+  return STS_OK;
+}
+
+extern "C" int sts_corrector_stats(sts_ctx *c, long long out[2]) {
+  if (!c || !out) return fail(STS_ERR_ARG, "null argument");
+  out[0] = c->cstat[0];
+  out[1] = c->cstat[1];
   return STS_OK;
 }
 

@@ -461,6 +461,38 @@ extern "C" int sts_unet_record(sts_unet *u, int slot, sts_ctx *ctx, const sts_un
   return STS_OK;
 }
 
+static int check_corrector(sts_unet *u, const StsView &v) {
+  if (v.Nx != u->W || v.Ny != u->H) return ufail(STS_ERR_ARG, "solver grid differs from the network's H x W");
+  if (u->cin != 9 || u->cout != 4) return ufail(STS_ERR_ARG, "the corrector needs cin = 9, cout = 4");
+  if (!v.xe[0]) return ufail(STS_ERR_ARG, "the context does not keep its predicted start (sts_keep_predicted_start)");
+  return STS_OK;
+}
+
+static CorrFields state_fields(const StsView &v, bool previous) {
+  const double *const *f = v.f;
+  return previous ? CorrFields{{f[STS_U_PR], f[STS_V_PR], f[STS_P_PR], f[STS_T_PR]}}
+                  : CorrFields{{f[STS_U], f[STS_V], f[STS_P], f[STS_T]}};
+}
+
+extern "C" int sts_unet_record_correction(sts_unet *u, int slot, sts_ctx *ctx, const sts_unet_corr_norm *nrm) {
+  if (!u || !ctx || !nrm || slot < 0 || slot >= u->ds_n) return ufail(STS_ERR_ARG, "bad record arguments");
+  StsView v;
+  int rc = sts_internal_view(ctx, &v);
+  if (rc) return rc;
+  if ((rc = check_corrector(u, v))) return rc;
+  U_OK(hipStreamSynchronize(v.stream));
+  const int np = v.Nx * v.Ny;
+  const CorrFields xe{{v.xe[0], v.xe[1], v.xe[2], v.xe[3]}};
+  U_OK(unet_pack_corr_in(state_fields(v, true), xe, v.mask, np, u->Cx0, 1.0 / nrm->u_ref, 1.0 / nrm->p_ref,
+                         1.0 / nrm->step, u->ds_x + (size_t)slot * np * u->Cx0, u->stream));
+  U_OK(unet_pack_corr_target(state_fields(v, false), xe, v.mask, np, 1.0 / nrm->u_ref, 1.0 / nrm->p_ref,
+                             1.0 / nrm->corr, u->ds_t + (size_t)slot * np * u->cout, u->ds_m + (size_t)slot * np,
+                             u->stream));
+  U_OK(hipStreamSynchronize(u->stream));
+  u->ds_count[slot] = v.fluid_cells;
+  return STS_OK;
+}
+
 static int batch_loss(sts_unet *u, const int *slots, int n, double *loss, bool train, float lr) {
   if (!u || !slots) return ufail(STS_ERR_ARG, "null argument");
   int rc = check_n(u, n);
@@ -485,6 +517,28 @@ extern "C" int sts_unet_train_step(sts_unet *u, const int *slots, int n, float l
 
 extern "C" int sts_unet_eval(sts_unet *u, const int *slots, int n, double *loss) {
   return batch_loss(u, slots, n, loss, false, 0.f);
+}
+
+// Corrected start for the solver (see sts_set_corrector).
+int sts_internal_unet_correct(sts_unet *u, const StsView &v, const sts_unet_corr_norm *nrm, double *const out[4]) {
+  int rc = check_corrector(u, v);
+  if (rc) return rc;
+  U_OK(hipStreamSynchronize(v.stream));
+  const int np = v.Nx * v.Ny;
+  const CorrFields xe{{v.xe[0], v.xe[1], v.xe[2], v.xe[3]}};
+  U_OK(unet_pack_corr_in(state_fields(v, true), xe, v.mask, np, u->Cx0, 1.0 / nrm->u_ref, 1.0 / nrm->p_ref,
+                         1.0 / nrm->step, u->x, u->stream));
+  if ((rc = forward_dev(u, 1, false))) return rc;
+  CorrBoxes bx{v.Nx, v.px, {}, {}, {}};
+  for (int q = 0; q < 4; q++) {
+    bx.u[q] = v.box_u[q];
+    bx.v[q] = v.box_v[q];
+    bx.o[q] = v.box_o[q];
+  }
+  U_OK(unet_apply_corr(u->y, xe, v.mask, bx, np, nrm->u_ref * nrm->corr, nrm->p_ref * nrm->corr, nrm->corr,
+                       CorrOut{{out[0], out[1], out[2], out[3]}}, u->stream));
+  U_OK(hipStreamSynchronize(u->stream));
+  return STS_OK;
 }
 
 // Prediction for the solver: candidate p, T into pw, Tw (device, FP64).

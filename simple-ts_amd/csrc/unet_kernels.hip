@@ -414,6 +414,62 @@ __global__ void k_apply_pred(const float *Y, const double *p, const double *T, c
   Tw[q] = upd ? T[q] + (double)Y[2 * (size_t)q + 1] * sT : T[q];
 }
 
+// Corrector input [H][W][Cx]: the state x^n of the step (u/u_ref, v/u_ref,
+// p/p_ref - 1, T - 1), the predicted start's change x_e - x^n in the same
+// units divided by `step`, the fluid mask; zero padding channels.
+__global__ void k_pack_corr_in(CorrFields xn, CorrFields xe, const uint8_t *m, int npix, int Cx, double inv_u,
+                               double inv_p, double inv_step, u16 *X) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= npix) return;
+  u16 *o = X + (size_t)q * Cx;
+  const double sc[4] = {inv_u, inv_u, inv_p, 1.0};
+  o[0] = f2bf((float)(xn.f[0][q] * inv_u));
+  o[1] = f2bf((float)(xn.f[1][q] * inv_u));
+  o[2] = f2bf((float)(xn.f[2][q] * inv_p - 1.0));
+  o[3] = f2bf((float)(xn.f[3][q] - 1.0));
+#pragma unroll
+  for (int c = 0; c < 4; c++) o[4 + c] = f2bf((float)((xe.f[c][q] - xn.f[c][q]) * sc[c] * inv_step));
+  o[8] = m[q] ? f2bf(1.f) : (u16)0;
+  for (int c = 9; c < Cx; c++) o[c] = 0;
+}
+
+// Corrector target: the converged step x^(n+1) minus the predicted start
+// x_e, normalised as the input and divided by `corr`; mask = fluid cells.
+__global__ void k_pack_corr_target(CorrFields x1, CorrFields xe, const uint8_t *m, int npix, double inv_u,
+                                   double inv_p, double inv_corr, float *Tg, uint8_t *M) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= npix) return;
+  const double sc[4] = {inv_u, inv_u, inv_p, 1.0};
+#pragma unroll
+  for (int c = 0; c < 4; c++) Tg[4 * (size_t)q + c] = (float)((x1.f[c][q] - xe.f[c][q]) * sc[c] * inv_corr);
+  M[q] = m[q];
+}
+
+// Corrected start x = x_e + y * scale on the cells loop 2 updates (u: fluid
+// faces of the u box, v: of the v box, p and T: fluid cells of the p/T
+// box); every other cell keeps x_e.  A value that is not finite, or a p or T
+// that is not positive, keeps x_e.
+__global__ void k_apply_corr(const float *Y, CorrFields xe, const uint8_t *m, CorrBoxes bx, int npix, double su,
+                             double sp, double sT, CorrOut out) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= npix) return;
+  const int Nx = bx.Nx, i = q % Nx, j = q / Nx;
+  const int iw = i > 0 ? q - 1 : (bx.px ? q + Nx - 1 : q), js = j > 0 ? q - Nx : q;
+  const bool c = m[q];
+  const bool upd[4] = {c && m[iw] && i >= bx.u[0] && i < bx.u[1] && j >= bx.u[2] && j < bx.u[3],
+                       c && m[js] && i >= bx.v[0] && i < bx.v[1] && j >= bx.v[2] && j < bx.v[3],
+                       c && i >= bx.o[0] && i < bx.o[1] && j >= bx.o[2] && j < bx.o[3],
+                       c && i >= bx.o[0] && i < bx.o[1] && j >= bx.o[2] && j < bx.o[3]};
+  const double sc[4] = {su, su, sp, sT};
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const double e = xe.f[k][q];
+    double x = upd[k] ? e + (double)Y[4 * (size_t)q + k] * sc[k] : e;
+    if (!isfinite(x) || (k >= 2 && !(x > 0.0))) x = e;
+    out.f[k][q] = x;
+  }
+}
+
 // Gathers dataset samples into a batch.
 __global__ void k_gather(const u16 *Xs, const float *Ts, const uint8_t *Ms, const int *slots, int n, int npix, int Cx,
                          int Co, u16 *X, float *Tg, uint8_t *M) {
@@ -528,6 +584,23 @@ hipError_t unet_apply_pred(const float *Y, const double *p, const double *T, con
                            const int box[4], double sp, double sT, double *pw, double *Tw, hipStream_t s) {
   hipLaunchKernelGGL(k_apply_pred, dim3(nblk(npix, 256)), dim3(256), 0, s, Y, p, T, m, Nx, npix, box[0], box[1],
                      box[2], box[3], sp, sT, pw, Tw);
+  return hipGetLastError();
+}
+hipError_t unet_pack_corr_in(const CorrFields &xn, const CorrFields &xe, const uint8_t *m, int npix, int Cx,
+                            double inv_u, double inv_p, double inv_step, uint16_t *X, hipStream_t s) {
+  hipLaunchKernelGGL(k_pack_corr_in, dim3(nblk(npix, 256)), dim3(256), 0, s, xn, xe, m, npix, Cx, inv_u, inv_p,
+                     inv_step, X);
+  return hipGetLastError();
+}
+hipError_t unet_pack_corr_target(const CorrFields &x1, const CorrFields &xe, const uint8_t *m, int npix, double inv_u,
+                                 double inv_p, double inv_corr, float *Tg, uint8_t *M, hipStream_t s) {
+  hipLaunchKernelGGL(k_pack_corr_target, dim3(nblk(npix, 256)), dim3(256), 0, s, x1, xe, m, npix, inv_u, inv_p,
+                     inv_corr, Tg, M);
+  return hipGetLastError();
+}
+hipError_t unet_apply_corr(const float *Y, const CorrFields &xe, const uint8_t *m, const CorrBoxes &bx, int npix,
+                           double su, double sp, double sT, const CorrOut &out, hipStream_t s) {
+  hipLaunchKernelGGL(k_apply_corr, dim3(nblk(npix, 256)), dim3(256), 0, s, Y, xe, m, bx, npix, su, sp, sT, out);
   return hipGetLastError();
 }
 hipError_t unet_gather(const uint16_t *Xs, const float *Ts, const uint8_t *Ms, const int *slots, int n, int npix,

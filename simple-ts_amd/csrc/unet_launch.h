@@ -63,5 +63,23 @@ hipError_t unet_pack_target(const double *p, const double *p_pr, const double *T
                             const uint8_t *m, int npix, double sp, double sT, float *Tg, uint8_t *M, hipStream_t s);
 hipError_t unet_apply_pred(const float *Y, const double *p, const double *T, const uint8_t *m, int Nx, int npix,
                            const int box[4], double sp, double sT, double *pw, double *Tw, hipStream_t s);
+// The corrector's coupling (sts_set_corrector): u, v, p, T of one state
+// (device, FP64), and the grid with its solve boxes.
+struct CorrFields {
+  const double *f[4];
+};
+struct CorrOut {
+  double *f[4];
+};
+struct CorrBoxes {
+  int Nx, px;
+  int u[4], v[4], o[4];  // x0, x1, y0, y1 of the u, v and p/T boxes
+};
+hipError_t unet_pack_corr_in(const CorrFields &xn, const CorrFields &xe, const uint8_t *m, int npix, int Cx,
+                            double inv_u, double inv_p, double inv_step, uint16_t *X, hipStream_t s);
+hipError_t unet_pack_corr_target(const CorrFields &x1, const CorrFields &xe, const uint8_t *m, int npix, double inv_u,
+                                 double inv_p, double inv_corr, float *Tg, uint8_t *M, hipStream_t s);
+hipError_t unet_apply_corr(const float *Y, const CorrFields &xe, const uint8_t *m, const CorrBoxes &bx, int npix,
+                           double su, double sp, double sT, const CorrOut &out, hipStream_t s);
 hipError_t unet_gather(const uint16_t *Xs, const float *Ts, const uint8_t *Ms, const int *slots, int n, int npix,
                        int Cx, int Co, uint16_t *X, float *Tg, uint8_t *M, hipStream_t s);

@@ -1,11 +1,15 @@
 """Trains the warm-start U-Net on the GPU and writes its parameters.
 
     python simple-ts_amd/train_unet.py --out simple-ts_amd/unet_b32.npz
+    python simple-ts_amd/train_unet.py --corrector --out simple-ts_amd/unet_corr_b32.npz
 
-Generates the training set with the plain FP64 solver (random confined-
-square cases on a 512 x 128 grid, one sample per time step), trains with the
-network's own HIP forward/backward kernels and Adam, and prints one JSON line
-with the loss history."""
+Generates the training set with the solver on random confined-square cases
+on a 512 x 128 grid and trains with the network's own HIP forward/backward
+kernels and Adam; prints one JSON line with the loss history.  Default: the
+p/T predictor (cin 5, cout 2; one sample per plain FP64 time step).
+--corrector: the corrector of the extrapolated start (cin 9, cout 4; one
+sample per predicted step of the accelerated FP64 method, 40 cases x 100
+samples = 4000, a diverging case replaced by the next seeded one)."""
 import argparse
 import importlib
 import json
@@ -31,18 +35,26 @@ def main():
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", required=True)
+    ap.add_argument("--corrector", action="store_true")
     a = ap.parse_args()
     pkg = importlib.import_module("simple-ts_amd")
     ws = importlib.import_module("simple-ts_amd.warmstart")
-    net = pkg.UNet(5, 2, a.base, a.batch, a.ny, a.nx)
-    net.set_params(ws.he_init(5, 2, a.base, a.seed))
+    cin, cout = (ws.CORR_CIN, ws.CORR_COUT) if a.corrector else (5, 2)
+    net = pkg.UNet(cin, cout, a.base, a.batch, a.ny, a.nx)
+    net.set_params(ws.he_init(cin, cout, a.base, a.seed))
     t0 = time.time()
-    n, used = ws.generate(net, a.cases, a.steps, seed=a.seed, log=lambda m: print(m, flush=True))
+    if a.corrector:
+        used = ws.generate_corrections(net, a.cases, a.steps, seed=a.seed, log=lambda m: print(m, flush=True))
+        n = a.cases * a.steps
+    else:
+        n, used = ws.generate(net, a.cases, a.steps, seed=a.seed, log=lambda m: print(m, flush=True))
     t1 = time.time()
     hist, hold = ws.train(net, n, a.epochs, a.batch, a.lr, seed=a.seed + 1, log=lambda m: print(m, flush=True))
     t2 = time.time()
-    np.savez(a.out, params=net.params(), base=a.base, cin=5, cout=2, train_loss=np.array(hist),
-             holdout_loss=np.array(hold), samples=n, grid=np.array([a.nx, a.ny]))
+    np.savez(a.out, params=net.params(), base=a.base, cin=cin, cout=cout, train_loss=np.array(hist),
+             holdout_loss=np.array(hold), samples=n, grid=np.array([a.nx, a.ny]),
+             corr_fraction=ws.CORR_FRACTION if a.corrector else 0.0,
+             cases=np.array([[p["Ma"], p["Kn"], p["side"], p["xc"]] for p in used]))
     print(json.dumps({"samples": n, "cases_used": len(used), "generate_s": t1 - t0, "train_s": t2 - t1,
                       "train_loss": hist, "holdout_loss": hold, "out": a.out}), flush=True)
 

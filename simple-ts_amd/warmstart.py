@@ -152,3 +152,73 @@ def train(net, n_samples, epochs, batch, lr, seed, holdout=0.1, log=print):
         hist_hold.append(float(np.mean(hl)))
         log(f"epoch {ep}: train loss {hist[-1]:.4e}, held-out {hist_hold[-1]:.4e}")
     return hist, hist_hold
+
+
+# ---------------------------------------------------------------------------
+# The corrector (sts_set_corrector): the same network with cin = 9, cout = 4,
+# refining the start predicted by the polynomial extrapolation in time.
+# ---------------------------------------------------------------------------
+CORR_CIN, CORR_COUT = 9, 4
+CORR_FRACTION = 0.05  # the correction's scale as a fraction of the step's change scale
+ACCEL = dict(extrapolate=3, guard=64, anderson=1)  # the predicted, accelerated FP64 step the corrector serves
+
+
+def corr_norm(case):
+    """Normalisation of a case: the step's change scale ht u_in (in the
+    normalised units of u/u_in, p/p_ref, T) and the correction's scale."""
+    u_in, ht = case.value("u_gas_nd"), case.value("ht")
+    return PKG.CorrNorm(u_in, case.value("p_gas_nd"), ht * u_in, CORR_FRACTION * ht * u_in)
+
+
+def accelerated_step(run):
+    """One step of the predicted, accelerated FP64 method; returns (FP64
+    outer iterations, whether the step ran from a predicted start)."""
+    before = run.ctx.predict_stats()[0]
+    _, n64, _, _ = run.ctx.time_step_mixed(*run.loop, 0, (1e-7,) * 3, **ACCEL)
+    return n64, run.ctx.predict_stats()[0] > before
+
+
+def generate_corrections(net, n_cases, samples_per_case, seed, max_steps=None, log=print):
+    """Records n_cases * samples_per_case corrector samples into the network's
+    training set (allocated here): seeded cases of the training distribution
+    are stepped with the predicted, accelerated FP64 method, and every step
+    that ran from a predicted start gives one sample (x^n, the predicted
+    start, the converged x^(n+1)).  A case whose solve fails is dropped and
+    the next seeded case takes its place, so the set is always full.
+    Returns the parameters of the cases used."""
+    rng = np.random.default_rng(seed)
+    max_steps = max_steps or samples_per_case + 20
+    net.dataset(n_cases * samples_per_case)
+    work = tempfile.mkdtemp(prefix="sts_corr_")
+    slot, used, k = 0, [], 0
+    try:
+        while len(used) < n_cases:
+            prm = case_params(rng)
+            d = write_case(os.path.join(work, f"c{k}"), net.W, net.H, prm, max_steps)
+            run = Run(d)
+            run.ctx.keep_predicted_start(True)
+            nrm = corr_norm(run.case)
+            start, ok = slot, False
+            try:
+                for _ in range(max_steps):
+                    _, predicted = accelerated_step(run)
+                    if predicted:
+                        net.record_correction(slot, run.ctx, nrm)
+                        slot += 1
+                        if slot - start == samples_per_case:
+                            ok = True
+                            break
+            except PKG.StsError as e:
+                log(f"case {k} {prm}: {e}; dropped")
+            finally:
+                run.close()
+            if not ok:
+                slot = start
+            else:
+                used.append(prm)
+                log(f"case {k}: Ma {prm['Ma']:.2f} Kn {prm['Kn']:.4f} side {prm['side']:.2f} xc {prm['xc']:.2f}: "
+                    f"{slot} samples")
+            k += 1
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+    return used

@@ -268,7 +268,7 @@ def _other_case(kind, d):
                                   "supersonic_square_slip_local_kn"])
 def test_accelerated_path_on_other_cases(kind, tmp_path):
     """The headline configuration (degree-3 predicted start, guard, Anderson
-    depth 2) on cases whose wall conditions are not affine in the state
+    depth 1) on cases whose wall conditions are not affine in the state
     (slip and temperature jump with the local Kn, which the Anderson mix does
     not re-apply), periodic seams and a heated cylinder: 16 steps within
     REL_L2_AA of the plain path, with fewer outer iterations."""
@@ -326,3 +326,47 @@ def test_batch_size_does_not_change_the_step(cap, tmp_path):
             assert a[0] == b[0] and np.array_equal(a[1], b[1]) and a[2] == b[2], (batch, a[0], b[0])
         for a, b in zip(fields, ref[1]):
             assert np.array_equal(a, b), batch
+
+
+def test_mixed_defect_correction_on_the_fine_grid(tmp_path):
+    """The mixed-precision configuration on its own grid: the supersonic
+    square on 3840 x 768 (half the benchmark's Courant number, where the
+    plain FP64 solver converges).  Mixed: the guarded degree-3 predicted
+    start, 6 FP32 outer iterations with Anderson acceleration (FP32
+    assembly, sweeps and correction), then FP64 outer iterations with
+    Anderson acceleration whose FP64 residuals, equation of state and test
+    end each step.  After 6 steps its fields are within 1e-5 relative L2 of
+    the plain FP64 path, it runs fewer FP64 outer iterations than the
+    same start and acceleration without the FP32 phase, and the FP64
+    mass-imbalance norm of its final state (S:10621-10646) is of the plain
+    path's size."""
+    cases = importlib.import_module("simple-ts_amd.cases")
+    ws = importlib.import_module("simple-ts_amd.warmstart")
+    Nx, Ny, steps = 3840, 768, 6
+    dt = 0.01 * 0.5 * 240 / Nx
+    acc = dict(extrapolate=3, guard=64, anderson=1)
+    runs, its, mass = {}, {}, {}
+    for tag in ("plain", "accelerated", "mixed"):
+        d = cases.confined_square(str(tmp_path / tag), Nx, Ny, 1415, 0.00283, steps, dt_over_tin=dt)
+        r = ws.Run(d, 32 if tag == "mixed" else 64)
+        n64 = 0
+        for s in range(steps):
+            print(f"{tag} step {s}", flush=True)
+            if tag == "plain":
+                n64 += r.ctx.time_step(*r.loop)[0]
+            elif tag == "accelerated":
+                n64 += r.ctx.time_step_mixed(*r.loop, 0, (1e-7,) * 3, **acc)[1]
+            else:
+                n64 += r.ctx.time_step_mixed(*r.loop, 6, (1e-7,) * 3, 0, 0.95, anderson32=1, **acc)[1]
+        runs[tag], its[tag], mass[tag] = r, n64, r.ctx.continuity()[2]
+    print(f"FP64 outer iterations: {its}; mass imbalance (max, L2, L1): {mass}", flush=True)
+    assert its["mixed"] < its["accelerated"] < its["plain"]
+    assert mass["mixed"][1] <= 2.0 * mass["plain"][1]
+    ref = {k: runs["plain"].ctx.get(k) for k in ("u", "v", "p", "Temper")}
+    for k, b in ref.items():
+        a = runs["mixed"].ctx.get(k)
+        e = rel_l2(a, b)
+        print(f"{k}: rel L2 {e:.3e}, max abs {np.max(np.abs(a - b)):.3e}", flush=True)
+        assert np.all(np.isfinite(a)) and e <= REL_L2_AA, k
+    for r in runs.values():
+        r.close()
