@@ -239,32 +239,68 @@ struct CellDiff {
   R Gy0, Gy1, Gx0, Gx1, Dux0, Dux1, Duy0, Duy1, Dvx0, Dvx1, Dvy0, Dvy1;
 };
 
+// The values of the fields a cell's assembly reads around it -- Fx, Fy, u,
+// v, sqrt_T at offsets (di, dj) in {-1, 0, 1}^2, the clamped (x: wrapped when
+// periodic) neighbours of Nbr -- either from global memory through Nbr's
+// indices or from a workgroup's LDS tile of them.  The offsets are
+// compile-time constants wherever the accessors are used.
 template <typename R>
+struct GlobalStencil {
+  const R *Fx_, *Fy_, *u_, *v_, *sT_;
+  const Nbr &n;
+  __device__ __forceinline__ GlobalStencil(const KFieldsT<R> &fl, const Nbr &nb)
+      : Fx_(FLD(FX)), Fy_(FLD(FY)), u_(FLD(U)), v_(FLD(V)), sT_(FLD(SQRT_T)), n(nb) {}
+  __device__ __forceinline__ unsigned at(int di, int dj) const {
+    if (dj == 0) return di < 0 ? n.i_1j : (di > 0 ? n.i1j : n.ij);
+    if (dj < 0) return di < 0 ? n.i_1j_1 : (di > 0 ? n.i1j_1 : n.ij_1);
+    return di < 0 ? n.i_1j1 : (di > 0 ? n.i1j1 : n.ij1);
+  }
+  __device__ __forceinline__ R Fx(int di, int dj) const { return Fx_[at(di, dj)]; }
+  __device__ __forceinline__ R Fy(int di, int dj) const { return Fy_[at(di, dj)]; }
+  __device__ __forceinline__ R u(int di, int dj) const { return u_[at(di, dj)]; }
+  __device__ __forceinline__ R v(int di, int dj) const { return v_[at(di, dj)]; }
+  __device__ __forceinline__ R sT(int di, int dj) const { return sT_[at(di, dj)]; }
+};
+
+// LDS tile [TY + 2][TX + 2] of each field: entry [y][x] holds the field at
+// cell (i0 + x - 1, j0 + y - 1) with the coordinates clamped (x wrapped when
+// periodic) as Nbr clamps a neighbour, so that the neighbour (di, dj) of the
+// tile's cell (tx, ty) is entry [ty + 1 + dj][tx + 1 + di].
+template <typename R, int TX, int TY>
+struct TileStencil {
+  const R (*Fx_)[TX + 2], (*Fy_)[TX + 2], (*u_)[TX + 2], (*v_)[TX + 2], (*sT_)[TX + 2];
+  int tx, ty;
+  __device__ __forceinline__ R Fx(int di, int dj) const { return Fx_[ty + 1 + dj][tx + 1 + di]; }
+  __device__ __forceinline__ R Fy(int di, int dj) const { return Fy_[ty + 1 + dj][tx + 1 + di]; }
+  __device__ __forceinline__ R u(int di, int dj) const { return u_[ty + 1 + dj][tx + 1 + di]; }
+  __device__ __forceinline__ R v(int di, int dj) const { return v_[ty + 1 + dj][tx + 1 + di]; }
+  __device__ __forceinline__ R sT(int di, int dj) const { return sT_[ty + 1 + dj][tx + 1 + di]; }
+};
+
+template <typename R, typename St>
 __device__ __forceinline__ void cell_assemble_u(const KGridT<R> &g, const KFieldsT<R> &fl, const Nbr &n,
-                                                const CellDiff<R> &df, unsigned nb) {
+                                                const CellDiff<R> &df, unsigned nb, const St &S) {
   const R *hx = g.hx, *hy = g.hy;
-  const R *Fx = FLD(FX), *Fy = FLD(FY), *u = FLD(U), *v = FLD(V), *sqrt_T = FLD(SQRT_T);
   const R *rho_pr = FLD(RHO_PR), *u_pr = FLD(U_PR), *rho = FLD(RHO), *rho_u = FLD(RHO_U);
-  const unsigned i = n.i, j = n.j, ij = n.ij, i_1j = n.i_1j, i1j = n.i1j, ij_1 = n.ij_1, ij1 = n.ij1;
-  const unsigned i_1j1 = n.i_1j1, i_1 = n.i_1;
+  const unsigned i = n.i, j = n.j, ij = n.ij, i_1j = n.i_1j, i_1 = n.i_1;
   const R cT = g.cT, ht = g.ht;
-  R F_xu1 = R(0.5) * (Fx[i_1j] + Fx[ij]);
-  R F_xu2 = R(0.5) * (Fx[ij] + Fx[i1j]);
-  R Scu = cT * (R((nb & NB_E) != 0) * u[i1j] * sqrt_T[ij] * hy[j] / (hx[i] * R(3.0))
-                     + R((nb & NB_WW) != 0) * u[i_1j] * sqrt_T[i_1j] * hy[j] / (hx[i_1] * R(3.0))
-                     + df.Gy1 * (v[ij1] * R((nb & NB_N) != 0) - v[i_1j1] * R((nb & NB_NW) != 0))
-                     - df.Gy0 * (v[ij] * R((nb & NB_S) != 0) - v[i_1j] * R((nb & NB_SW) != 0))
-                     - sqrt_T[ij] * (v[ij1] * R((nb & NB_N) != 0) - v[ij] * R((nb & NB_S) != 0)) * R(2.0) / R(3.0)
-                     + sqrt_T[i_1j] * (v[i_1j1] * R((nb & NB_NW) != 0) - v[i_1j] * R((nb & NB_SW) != 0)) * R(2.0) / R(3.0));
+  R F_xu1 = R(0.5) * (S.Fx(-1, 0) + S.Fx(0, 0));
+  R F_xu2 = R(0.5) * (S.Fx(0, 0) + S.Fx(1, 0));
+  R Scu = cT * (R((nb & NB_E) != 0) * S.u(1, 0) * S.sT(0, 0) * hy[j] / (hx[i] * R(3.0))
+                     + R((nb & NB_WW) != 0) * S.u(-1, 0) * S.sT(-1, 0) * hy[j] / (hx[i_1] * R(3.0))
+                     + df.Gy1 * (S.v(0, 1) * R((nb & NB_N) != 0) - S.v(-1, 1) * R((nb & NB_NW) != 0))
+                     - df.Gy0 * (S.v(0, 0) * R((nb & NB_S) != 0) - S.v(-1, 0) * R((nb & NB_SW) != 0))
+                     - S.sT(0, 0) * (S.v(0, 1) * R((nb & NB_N) != 0) - S.v(0, 0) * R((nb & NB_S) != 0)) * R(2.0) / R(3.0)
+                     + S.sT(-1, 0) * (S.v(-1, 1) * R((nb & NB_NW) != 0) - S.v(-1, 0) * R((nb & NB_SW) != 0)) * R(2.0) / R(3.0));
   R Spu = -(df.Dux0 + df.Dux1) / R(3.0);
   R au1 = maxi0(F_xu1) + df.Dux0;
   R au2 = maxi0(-F_xu2) + df.Dux1;
-  R au3 = R(0.5) * (maxi0(Fy[i_1j]) + maxi0(Fy[ij])) + df.Duy0;
-  R au4 = R(0.5) * (maxi0(-Fy[i_1j1]) + maxi0(-Fy[ij1])) + df.Duy1;
+  R au3 = R(0.5) * (maxi0(S.Fy(-1, 0)) + maxi0(S.Fy(0, 0))) + df.Duy0;
+  R au4 = R(0.5) * (maxi0(-S.Fy(-1, 1)) + maxi0(-S.Fy(0, 1))) + df.Duy1;
   R bu3 = (rho_pr[i_1j] * hx[i_1] + rho_pr[ij] * hx[i]) * u_pr[ij] * hy[j] * R(0.5) / ht + Scu;
-  R au0 = au1 + au2 + au3 + au4 + R(0.5) * (-Fy[i_1j] - Fy[ij] + Fy[i_1j1] + Fy[ij1]) + F_xu2 - F_xu1 - Spu
+  R au0 = au1 + au2 + au3 + au4 + R(0.5) * (-S.Fy(-1, 0) - S.Fy(0, 0) + S.Fy(-1, 1) + S.Fy(0, 1)) + F_xu2 - F_xu1 - Spu
                + (rho[i_1j] * hx[i_1] + rho[ij] * hx[i]) * hy[j] * R(0.5) / ht;
-  R ups = (au1 * R((nb & NB_E) != 0) * u[i_1j] + au2 * R((nb & NB_WW) != 0) * u[i1j] + au3 * u[ij_1] + au4 * u[ij1] + bu3) / au0;
+  R ups = (au1 * R((nb & NB_E) != 0) * S.u(-1, 0) + au2 * R((nb & NB_WW) != 0) * S.u(1, 0) + au3 * S.u(0, -1) + au4 * S.u(0, 1) + bu3) / au0;
   R dpc = g.cPch * hy[j] / au0;
   R h = hy[j] * rho_u[ij];
   FLD(U_PSEUDO)[ij] = ups;
@@ -273,35 +309,33 @@ __device__ __forceinline__ void cell_assemble_u(const KGridT<R> &g, const KField
   FLD(BPX)[ij] = R((nb & NB_W) != 0 && (nb & NB_C) != 0) * h * ups;
 }
 
-template <typename R>
+template <typename R, typename St>
 __device__ __forceinline__ void cell_assemble_v(const KGridT<R> &g, const KFieldsT<R> &fl, const Nbr &n,
-                                                const CellDiff<R> &df, unsigned nb) {
+                                                const CellDiff<R> &df, unsigned nb, const St &S) {
   const R *hx = g.hx, *hy = g.hy;
-  const R *Fx = FLD(FX), *Fy = FLD(FY), *u = FLD(U), *v = FLD(V), *sqrt_T = FLD(SQRT_T);
   const R *rho_pr = FLD(RHO_PR), *v_pr = FLD(V_PR), *rho = FLD(RHO), *rho_v = FLD(RHO_V);
-  const unsigned i = n.i, j = n.j, ij = n.ij, i_1j = n.i_1j, i1j = n.i1j, ij_1 = n.ij_1, ij1 = n.ij1;
-  const unsigned i1j_1 = n.i1j_1;
+  const unsigned i = n.i, j = n.j, ij = n.ij, ij_1 = n.ij_1;
   const R cT = g.cT, ht = g.ht;
   // S:3399 the mask at ij_1 - Nx*(0 <= (int)(j-2)) (NB_SS); S:3400 hy[j - 1*(0 <= (int)(j-1))]
   const R hyjm = hy[j - 1 * (0 <= (int)(j - 1))];
-  R F_yv3 = R(0.5) * (Fy[ij_1] + Fy[ij]);
-  R F_yv4 = R(0.5) * (Fy[ij] + Fy[ij1]);
-  R Scv = cT * (R((nb & NB_N) != 0) * v[ij1] * sqrt_T[ij] * hx[i] / (hy[j] * R(3.0))
-                     + R((nb & NB_SS) != 0) * v[ij_1] * sqrt_T[ij_1] * hx[i] / (hyjm * R(3.0))
-                     + df.Gx1 * (u[i1j] * R((nb & NB_E) != 0) - u[i1j_1] * R((nb & NB_SE) != 0))
-                     - df.Gx0 * (u[ij] * R((nb & NB_W) != 0) - u[ij_1] * R((nb & NB_SW) != 0))
-                     - sqrt_T[ij] * (u[i1j] * R((nb & NB_E) != 0) - u[ij] * R((nb & NB_W) != 0)) * R(2.0) / R(3.0)
-                     + sqrt_T[ij_1] * (u[i1j_1] * R((nb & NB_SE) != 0) - u[ij_1] * R((nb & NB_SW) != 0)) * R(2.0) / R(3.0))
+  R F_yv3 = R(0.5) * (S.Fy(0, -1) + S.Fy(0, 0));
+  R F_yv4 = R(0.5) * (S.Fy(0, 0) + S.Fy(0, 1));
+  R Scv = cT * (R((nb & NB_N) != 0) * S.v(0, 1) * S.sT(0, 0) * hx[i] / (hy[j] * R(3.0))
+                     + R((nb & NB_SS) != 0) * S.v(0, -1) * S.sT(0, -1) * hx[i] / (hyjm * R(3.0))
+                     + df.Gx1 * (S.u(1, 0) * R((nb & NB_E) != 0) - S.u(1, -1) * R((nb & NB_SE) != 0))
+                     - df.Gx0 * (S.u(0, 0) * R((nb & NB_W) != 0) - S.u(0, -1) * R((nb & NB_SW) != 0))
+                     - S.sT(0, 0) * (S.u(1, 0) * R((nb & NB_E) != 0) - S.u(0, 0) * R((nb & NB_W) != 0)) * R(2.0) / R(3.0)
+                     + S.sT(0, -1) * (S.u(1, -1) * R((nb & NB_SE) != 0) - S.u(0, -1) * R((nb & NB_SW) != 0)) * R(2.0) / R(3.0))
                - g.c_Fr_v * (rho[ij_1] * hx[i] * R(0.5) * hyjm + rho[ij] * hx[i] * R(0.5) * hy[j]);
   R Spv = -(df.Dvy0 + df.Dvy1) / R(3.0);
-  R av1 = R(0.5) * (maxi0(Fx[ij]) + maxi0(Fx[ij_1])) + df.Dvx0;
-  R av2 = R(0.5) * (maxi0(-Fx[i1j]) + maxi0(-Fx[i1j_1])) + df.Dvx1;
+  R av1 = R(0.5) * (maxi0(S.Fx(0, 0)) + maxi0(S.Fx(0, -1))) + df.Dvx0;
+  R av2 = R(0.5) * (maxi0(-S.Fx(1, 0)) + maxi0(-S.Fx(1, -1))) + df.Dvx1;
   R av3 = maxi0(F_yv3) + df.Dvy0;
   R av4 = maxi0(-F_yv4) + df.Dvy1;
   R bv3 = (rho_pr[ij_1] * hy[j - 1] + rho_pr[ij] * hy[j]) * v_pr[ij] * hx[i] * R(0.5) / ht + Scv;
-  R av0 = av1 + av2 + av3 + av4 + R(0.5) * (Fx[i1j] - Fx[ij] + Fx[i1j_1] - Fx[ij_1]) + F_yv4 - F_yv3 - Spv
+  R av0 = av1 + av2 + av3 + av4 + R(0.5) * (S.Fx(1, 0) - S.Fx(0, 0) + S.Fx(1, -1) - S.Fx(0, -1)) + F_yv4 - F_yv3 - Spv
                + (rho[ij_1] * hy[j - 1] + rho[ij] * hy[j]) * hx[i] * R(0.5) / ht;
-  R vps = (av1 * v[i_1j] + av2 * v[i1j] + av3 * R((nb & NB_SS) != 0) * v[ij_1] + av4 * R((nb & NB_N) != 0) * v[ij1] + bv3) / av0;
+  R vps = (av1 * S.v(-1, 0) + av2 * S.v(1, 0) + av3 * R((nb & NB_SS) != 0) * S.v(0, -1) + av4 * R((nb & NB_N) != 0) * S.v(0, 1) + bv3) / av0;
   R dpc = g.cPch * hx[i] / av0;
   R h = hx[i] * rho_v[ij];
   FLD(V_PSEUDO)[ij] = vps;
@@ -310,28 +344,27 @@ __device__ __forceinline__ void cell_assemble_v(const KGridT<R> &g, const KField
   FLD(BPY)[ij] = R((nb & NB_S) != 0 && (nb & NB_C) != 0) * h * vps;
 }
 
-template <typename R>
+template <typename R, typename St>
 __device__ __forceinline__ void cell_energy_source(const KGridT<R> &g, const KFieldsT<R> &fl, const Nbr &n,
-                                                   unsigned nb) {
+                                                   unsigned nb, const St &S) {
   const unsigned Nx = (unsigned)g.Nx;
   const R *hx = g.hx, *hy = g.hy;
-  const R *u = FLD(U), *v = FLD(V), *sqrt_T = FLD(SQRT_T), *p_pr = FLD(P_PR);
-  const unsigned i = n.i, j = n.j, ij = n.ij, i_1j = n.i_1j, i1j = n.i1j, ij_1 = n.ij_1, ij1 = n.ij1;
-  const unsigned i_1j1 = n.i_1j1, i1j_1 = n.i1j_1, i1j1 = n.i1j1, i_1 = n.i_1, i1 = n.i1;
+  const R *p_pr = FLD(P_PR);
+  const unsigned i = n.i, j = n.j, ij = n.ij, i_1 = n.i_1, i1 = n.i1;
   // S:3729-3740: the upper neighbour row is guarded by j < Nx-1
   const unsigned jm = j - 1 * (0 < j), jp = j + 1 * (j < (Nx - 1));
-  R dudx_T = (R(((nb & NB_E) != 0)) * u[i1j] - R(((nb & NB_W) != 0)) * u[ij]) / hx[i];
-  R dvdy_T = (R(((nb & NB_N) != 0)) * v[ij1] - R(((nb & NB_S) != 0)) * v[ij]) / hy[j];
-  R s0 = ((u[ij] - u[ij_1]) / (R(((nb & NB_SW) != 0) && ((nb & NB_S) != 0)) * hy[jm] + hy[j]))
-              + ((v[ij] - v[i_1j]) / (R(((nb & NB_SW) != 0) && ((nb & NB_W) != 0)) * hx[i_1] + hx[i]));
-  R s1 = ((u[i1j] - u[i1j_1]) / (R(((nb & NB_S) != 0) && ((nb & NB_SE) != 0)) * hy[jm] + hy[j]))
-              + ((v[i1j] - v[ij]) / (hx[i] + R(((nb & NB_SE) != 0) && ((nb & NB_E) != 0)) * hx[i1]));
-  R s2 = ((u[ij1] - u[ij]) / (hy[j] + R(((nb & NB_NW) != 0) && ((nb & NB_N) != 0)) * hy[jp]))
-              + ((v[ij1] - v[i_1j1]) / (R(((nb & NB_W) != 0) && ((nb & NB_NW) != 0)) * hx[i_1] + hx[i]));
-  R s3 = ((u[i1j1] - u[i1j]) / (hy[j] + R(((nb & NB_N) != 0) && ((nb & NB_NE) != 0)) * hy[jp]))
-              + ((v[i1j1] - v[ij1]) / (hx[i] + R(((nb & NB_E) != 0) && ((nb & NB_NE) != 0)) * hx[i1]));
+  R dudx_T = (R(((nb & NB_E) != 0)) * S.u(1, 0) - R(((nb & NB_W) != 0)) * S.u(0, 0)) / hx[i];
+  R dvdy_T = (R(((nb & NB_N) != 0)) * S.v(0, 1) - R(((nb & NB_S) != 0)) * S.v(0, 0)) / hy[j];
+  R s0 = ((S.u(0, 0) - S.u(0, -1)) / (R(((nb & NB_SW) != 0) && ((nb & NB_S) != 0)) * hy[jm] + hy[j]))
+              + ((S.v(0, 0) - S.v(-1, 0)) / (R(((nb & NB_SW) != 0) && ((nb & NB_W) != 0)) * hx[i_1] + hx[i]));
+  R s1 = ((S.u(1, 0) - S.u(1, -1)) / (R(((nb & NB_S) != 0) && ((nb & NB_SE) != 0)) * hy[jm] + hy[j]))
+              + ((S.v(1, 0) - S.v(0, 0)) / (hx[i] + R(((nb & NB_SE) != 0) && ((nb & NB_E) != 0)) * hx[i1]));
+  R s2 = ((S.u(0, 1) - S.u(0, 0)) / (hy[j] + R(((nb & NB_NW) != 0) && ((nb & NB_N) != 0)) * hy[jp]))
+              + ((S.v(0, 1) - S.v(-1, 1)) / (R(((nb & NB_W) != 0) && ((nb & NB_NW) != 0)) * hx[i_1] + hx[i]));
+  R s3 = ((S.u(1, 1) - S.u(1, 0)) / (hy[j] + R(((nb & NB_N) != 0) && ((nb & NB_NE) != 0)) * hy[jp]))
+              + ((S.v(1, 1) - S.v(0, 1)) / (hx[i] + R(((nb & NB_E) != 0) && ((nb & NB_NE) != 0)) * hx[i1]));
   R dsum = dudx_T + dvdy_T;
-  FLD(SCT1)[ij] = g.ht * sqrt_T[ij] * g.cT2 * hx[i] * hy[j]
+  FLD(SCT1)[ij] = g.ht * S.sT(0, 0) * g.cT2 * hx[i] * hy[j]
                       * (R(2.0) * (dudx_T * dudx_T + dvdy_T * dvdy_T) + s0 * s0 + s1 * s1 + s2 * s2 + s3 * s3
                          - dsum * dsum * R(2.0) / R(3.0))
                   + p_pr[ij] * hx[i] * hy[j];
@@ -352,10 +385,12 @@ __device__ __forceinline__ void assemble_cell(const KGridT<R> &g, const KFieldsT
             *Dvy = FLD(D_VY);
     const CellDiff<R> df{Gyf[n.ij], Gyf[n.ij1], Gxf[n.ij], Gxf[n.i1j], Dux[n.ij], Dux[n.i1j],
                          Duy[n.ij], Duy[n.ij1], Dvx[n.ij], Dvx[n.i1j], Dvy[n.ij], Dvy[n.ij1]};
-    if (du) cell_assemble_u(g, fl, n, df, nb);
-    if (dv) cell_assemble_v(g, fl, n, df, nb);
+    const GlobalStencil<R> S(fl, n);
+    if (du) cell_assemble_u(g, fl, n, df, nb, S);
+    if (dv) cell_assemble_v(g, fl, n, df, nb, S);
   }
-  if (in_box(i, j, g.Nx_o0, g.Nx_o1, g.Ny_o0, g.Ny_o1) && c) cell_energy_source(g, fl, n, nb);
+  if (in_box(i, j, g.Nx_o0, g.Nx_o1, g.Ny_o0, g.Ny_o1) && c)
+    cell_energy_source(g, fl, n, nb, GlobalStencil<R>(fl, n));
 }
 
 template <typename R>
@@ -376,9 +411,25 @@ __global__ void __launch_bounds__(BX *BY) k_assemble_inl(KGridT<R> g, KFieldsT<R
   if (gated_out(gate)) return;
   __shared__ R Lgy[BY + 1][BX + 1], Lgx[BY + 1][BX + 1], Lux[BY + 1][BX + 1], Luy[BY + 1][BX + 1],
       Lvx[BY + 1][BX + 1], Lvy[BY + 1][BX + 1];
+  __shared__ R SFx[BY + 2][BX + 2], SFy[BY + 2][BX + 2], Su[BY + 2][BX + 2], Sv[BY + 2][BX + 2],
+      SsT[BY + 2][BX + 2];
   int i0, j0;
   tile_origin<BX, BY>(g.Nx, i0, j0);
   const int Nx = g.Nx, Ny = g.Ny, tx = (int)threadIdx.x, ty = (int)threadIdx.y;
+  // the tile's neighbourhood of the fields the cells read (TileStencil); the
+  // entries past the grid's last tile column are never read, only kept in it
+  for (int e = tx + ty * BX; e < (BX + 2) * (BY + 2); e += BX * BY) {
+    const int x = e % (BX + 2), y = e / (BX + 2);
+    int gi = i0 + x - 1;
+    gi = g.px ? ((gi % Nx) + Nx) % Nx : min(max(gi, 0), Nx - 1);
+    const int gj = min(max(j0 + y - 1, 0), Ny - 1);
+    const unsigned k = (unsigned)gi + (unsigned)gj * (unsigned)Nx;
+    SFx[y][x] = FLD(FX)[k];
+    SFy[y][x] = FLD(FY)[k];
+    Su[y][x] = FLD(U)[k];
+    Sv[y][x] = FLD(V)[k];
+    SsT[y][x] = FLD(SQRT_T)[k];
+  }
   const int i = i0 + tx, j = j0 + ty;
   const int il = min(i0 + BX, Nx) - 1, jl = min(j0 + BY, Ny) - 1;  // last cell of the tile in x, y
   auto put = [&](int y, int x, unsigned k) {
@@ -406,14 +457,15 @@ __global__ void __launch_bounds__(BX *BY) k_assemble_inl(KGridT<R> g, KFieldsT<R
   const bool c = (nb & NB_C) != 0;
   const bool du = in_box(i, j, g.Nx_u0, g.Nx_u1, g.Ny_u0, g.Ny_u1) && (nb & NB_W) && c;
   const bool dv = in_box(i, j, g.Nx_v0, g.Nx_v1, g.Ny_v0, g.Ny_v1) && (nb & NB_S) && c;
+  const TileStencil<R, BX, BY> S{SFx, SFy, Su, Sv, SsT, tx, ty};
   if (du || dv) {
     const int ex = (i == il) ? BX : tx + 1, ny = (j == jl) ? BY : ty + 1;
     const CellDiff<R> df{Lgy[ty][tx], Lgy[ny][tx], Lgx[ty][tx], Lgx[ty][ex], Lux[ty][tx], Lux[ty][ex],
                          Luy[ty][tx], Luy[ny][tx], Lvx[ty][tx], Lvx[ty][ex], Lvy[ty][tx], Lvy[ny][tx]};
-    if (du) cell_assemble_u(g, fl, n, df, nb);
-    if (dv) cell_assemble_v(g, fl, n, df, nb);
+    if (du) cell_assemble_u(g, fl, n, df, nb, S);
+    if (dv) cell_assemble_v(g, fl, n, df, nb, S);
   }
-  if (in_box(i, j, g.Nx_o0, g.Nx_o1, g.Ny_o0, g.Ny_o1) && c) cell_energy_source(g, fl, n, nb);
+  if (in_box(i, j, g.Nx_o0, g.Nx_o1, g.Ny_o0, g.Ny_o1) && c) cell_energy_source(g, fl, n, nb, S);
 }
 
 // ---------------------------------------------------------------------------

@@ -5,8 +5,9 @@ inputs, and a short training run that must lower the loss.
 Tolerances: the GPU sums in FP32 in another order than numpy, so a BF16
 rounding of an intermediate activation or gradient occasionally lands on
 the other side (one BF16 ulp, 2^-8 relative); through 15 layers this leaves
-relative L2 differences of order 1e-3.  Forward outputs must agree to 1e-2,
-each gradient tensor to 3e-2 relative L2 (cosine > 0.999), Adam to 1e-6."""
+relative L2 differences of order 1e-3 (3.7e-3 measured at base 32, 0 at
+base 8).  Forward outputs must agree to 5e-3, each gradient tensor to 3e-2
+relative L2 (cosine > 0.999), Adam to 1e-6."""
 import importlib
 
 import numpy as np
@@ -70,7 +71,7 @@ def test_forward_matches_oracle(base):
     yr = U.UNet(P, CIN, COUT, base).forward(x)
     e = rel_l2(y, yr)
     print(f"forward rel L2 {e:.2e}", flush=True)
-    assert np.all(np.isfinite(y)) and e <= 1e-2
+    assert np.all(np.isfinite(y)) and e <= 5e-3
 
 
 @pytest.mark.parametrize("base", [8, 32])
