@@ -1621,6 +1621,11 @@ struct sts_batch {
   int n_cell[NVARIANT] = {}, n_sweep[NVARIANT] = {};
   BatchTile *wall_t = nullptr;  // chunks of the cases' planned wall writes
   int n_wall = 0;
+  // tiles of the cases whose loop is still open (rebuilt as cases stop):
+  // device lists for the cell, sweep and wall launches, and their pinned
+  // host staging (rewritten only after the stream has passed the last copy)
+  BatchTile *act_t[3] = {}, *h_act = nullptr;
+  int act_cap[3] = {};
   int cell_v = 0, sweep_v = 0;
   int last_iters32 = 0, last_iters64 = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;  // autotuner timing
@@ -1637,6 +1642,8 @@ extern "C" int sts_batch_destroy(sts_batch *b) {
     (void)hipFree(b->sweep_t[v]);
   }
   (void)hipFree(b->wall_t);
+  for (BatchTile *p : b->act_t) (void)hipFree(p);
+  if (b->h_act) (void)hipHostFree(b->h_act);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
   if (b->ev_ctrl) (void)hipEventDestroy(b->ev_ctrl);
@@ -1645,27 +1652,80 @@ extern "C" int sts_batch_destroy(sts_batch *b) {
   return STS_OK;
 }
 
+// Tile list of the cases with open[c] (all when open is null) for tiles of
+// TX x TY cells, into t.  Workgroup b runs on chiplet b % 8 (round-robin
+// dispatch); every chiplet gets one contiguous run of the list (same order
+// as xcd_tile).
+static void tile_list(const sts_batch *b, int TX, int TY, const char *open, std::vector<BatchTile> &t) {
+  std::vector<BatchTile> lt;
+  for (int c = 0; c < (int)b->cs.size(); c++) {
+    if (open && !open[c]) continue;
+    const int nx = (b->cs[c]->Nx + TX - 1) / TX, ny = (b->cs[c]->Ny + TY - 1) / TY;
+    for (int y = 0; y < ny; y++)
+      for (int x = 0; x < nx; x++) lt.push_back(BatchTile{c, x, y});
+  }
+  const int nt = (int)lt.size(), per = nt / 8, rem = nt % 8;
+  t.resize(lt.size());
+  for (int bid = 0; bid < nt; bid++) {
+    const int x = bid % 8;
+    t[bid] = lt[x * per + std::min(x, rem) + bid / 8];
+  }
+}
+
+// Chunks of WALL_OPS_PER_TILE planned wall writes of the cases with open[c].
+static void wall_list(const sts_batch *b, const char *open, std::vector<BatchTile> &t) {
+  t.clear();
+  for (int c = 0; c < (int)b->cs.size(); c++)
+    if (!open || open[c])
+      for (int x = 0; x * WALL_OPS_PER_TILE < b->cs[c]->n_wall_ops; x++) t.push_back(BatchTile{c, x, 0});
+}
+
 static int build_tiles(sts_batch *b, const int (*shape)[2], BatchTile **out, int *count) {
   for (int v = 0; v < NVARIANT; v++) {
-    std::vector<BatchTile> lt, t;
-    const int TX = shape[v][0], TY = shape[v][1];
-    for (int c = 0; c < (int)b->cs.size(); c++) {
-      const int nx = (b->cs[c]->Nx + TX - 1) / TX, ny = (b->cs[c]->Ny + TY - 1) / TY;
-      for (int y = 0; y < ny; y++)
-        for (int x = 0; x < nx; x++) lt.push_back(BatchTile{c, x, y});
-    }
-    // Workgroup b runs on chiplet b % 8 (round-robin dispatch); give every
-    // chiplet one contiguous run of the tile list (same order as xcd_tile).
-    const int nt = (int)lt.size(), per = nt / 8, rem = nt % 8;
-    t.resize(lt.size());
-    for (int bid = 0; bid < nt; bid++) {
-      const int x = bid % 8;
-      t[bid] = lt[x * per + std::min(x, rem) + bid / 8];
-    }
+    std::vector<BatchTile> t;
+    tile_list(b, shape[v][0], shape[v][1], nullptr, t);
     HIP_OK(hipMalloc(&out[v], sizeof(BatchTile) * (t.empty() ? 1 : t.size())));
     HIP_OK(hipMemcpyAsync(out[v], t.data(), sizeof(BatchTile) * t.size(), hipMemcpyHostToDevice, b->stream));
+    HIP_OK(hipStreamSynchronize(b->stream));
     count[v] = (int)t.size();
   }
+  return STS_OK;
+}
+
+// Launch lists of one loop 2: the full lists, or those of the open cases.
+struct BatchLists {
+  const BatchTile *cell, *sweep, *wall;
+  int n_cell, n_sweep, n_wall, n_open;
+};
+
+// Rebuilds the active lists for the cases with open[c] and enqueues their
+// upload on the batch stream (kernels enqueued before it still read the
+// previous lists, which held a superset of these tiles).  Called only right
+// after the host waited for an event that follows the previous upload, so
+// the pinned staging is free.
+static int compact_lists(sts_batch *b, const std::vector<char> &open, BatchLists &L) {
+  std::vector<BatchTile> t[3];
+  tile_list(b, CELL_TILES[b->cell_v][0], CELL_TILES[b->cell_v][1], open.data(), t[0]);
+  tile_list(b, SWEEP_TILES[b->sweep_v][0], SWEEP_TILES[b->sweep_v][1], open.data(), t[1]);
+  wall_list(b, open.data(), t[2]);
+  size_t off = 0;
+  for (int q = 0; q < 3; q++) {
+    if ((int)t[q].size() > b->act_cap[q]) return fail(STS_ERR_ARG, "active tile list exceeds the full list");
+    if (!t[q].empty()) {
+      std::memcpy(b->h_act + off, t[q].data(), sizeof(BatchTile) * t[q].size());
+      HIP_OK(hipMemcpyAsync(b->act_t[q], b->h_act + off, sizeof(BatchTile) * t[q].size(), hipMemcpyHostToDevice,
+                            b->stream));
+    }
+    off += (size_t)b->act_cap[q];
+  }
+  L.cell = b->act_t[0];
+  L.sweep = b->act_t[1];
+  L.wall = b->act_t[2];
+  L.n_cell = (int)t[0].size();
+  L.n_sweep = (int)t[1].size();
+  L.n_wall = (int)t[2].size();
+  L.n_open = 0;
+  for (char o : open) L.n_open += o ? 1 : 0;
   return STS_OK;
 }
 
@@ -1704,14 +1764,23 @@ extern "C" int sts_batch_create(sts_ctx *const *ctxs, int n, sts_batch **out) {
     return bail(rc);
   {
     std::vector<BatchTile> wt;
-    for (int c = 0; c < n; c++)
-      for (int x = 0; x * WALL_OPS_PER_TILE < b->cs[c]->n_wall_ops; x++) wt.push_back(BatchTile{c, x, 0});
+    wall_list(b, nullptr, wt);
     b->n_wall = (int)wt.size();
     if (hipMalloc(&b->wall_t, sizeof(BatchTile) * (wt.empty() ? 1 : wt.size())) != hipSuccess ||
         (!wt.empty() && hipMemcpyAsync(b->wall_t, wt.data(), sizeof(BatchTile) * wt.size(), hipMemcpyHostToDevice,
-                                       b->stream) != hipSuccess))
+                                       b->stream) != hipSuccess) ||
+        hipStreamSynchronize(b->stream) != hipSuccess)
       return bail(fail(STS_ERR_HIP, "wall tiles"));
   }
+  b->act_cap[0] = *std::max_element(b->n_cell, b->n_cell + NVARIANT);
+  b->act_cap[1] = *std::max_element(b->n_sweep, b->n_sweep + NVARIANT);
+  b->act_cap[2] = b->n_wall;
+  for (int q = 0; q < 3; q++)
+    if (hipMalloc(&b->act_t[q], sizeof(BatchTile) * std::max(1, b->act_cap[q])) != hipSuccess)
+      return bail(fail(STS_ERR_HIP, "active tile lists"));
+  if (hipHostMalloc(&b->h_act, sizeof(BatchTile) * std::max(1, b->act_cap[0] + b->act_cap[1] + b->act_cap[2]),
+                    hipHostMallocDefault) != hipSuccess)
+    return bail(fail(STS_ERR_HIP, "active tile staging"));
   if (hipStreamSynchronize(b->stream) != hipSuccess) return bail(fail(STS_ERR_HIP, "stream sync"));
   *out = b;
   return STS_OK;
@@ -1776,11 +1845,14 @@ static int run_batch_loop2(sts_batch *b, BatchCase<R> *C, int n_pc, int N_I, int
   const int n = (int)b->cs.size(), passes = sweep_passes(b->cs[0], n_pc);
   hipStream_t s = b->stream;
   const int cv = b->cell_v, sv = b->sweep_v;
-  BatchTile *ct = b->cell_t[cv], *st = b->sweep_t[sv];
-  const int nct = b->n_cell[cv], nst = b->n_sweep[sv];
+  // launches cover the tiles of every case until the host has seen cases
+  // stop; from then on only those of the open ones (a stopped case's tiles
+  // would only read its control word and return)
+  BatchLists L{b->cell_t[cv], b->sweep_t[sv], b->wall_t, b->n_cell[cv], b->n_sweep[sv], b->n_wall, n};
+  std::vector<char> open_c(n, 1);
   HIP_OK(launch_batch<R>(BK_LOOP_OPEN, 0, C, n, nullptr, 0, 0, 0, 0, 0, s));
   if constexpr (sizeof(R) == sizeof(double))
-    if (aa_m > 0) HIP_OK(launch_batch_aa(BKA_BEGIN, cv, C, n, ct, nct, 0, 0, AABatchSel{}, s));
+    if (aa_m > 0) HIP_OK(launch_batch_aa(BKA_BEGIN, cv, C, n, L.cell, L.n_cell, 0, 0, AABatchSel{}, s));
   // the loop control is read one iteration before a batch ends (as in the
   // single-context loop), after the last one of a batch that ends at N_I
   constexpr int LOOKAHEAD = 1;
@@ -1790,19 +1862,19 @@ static int run_batch_loop2(sts_batch *b, BatchCase<R> *C, int n_pc, int N_I, int
     const int check = end == N_I ? end - 1 : std::max(k, end - 1 - LOOKAHEAD);
     for (; k < end; k++) {
       int par = (int)(((long long)k * passes) & 1);
-      HIP_OK(launch_batch<R>(BK_ASSEMBLE, cv, C, n, ct, nct, k, par, 0, 0, s));
+      HIP_OK(launch_batch<R>(BK_ASSEMBLE, cv, C, n, L.cell, L.n_cell, k, par, 0, 0, s));
       for (int q = 0; q < n_pc;) {
         if (b->cs[0]->fuse_sweeps && q + 1 < n_pc) {
-          HIP_OK(launch_batch<R>(BK_SWEEP2, sv, C, n, st, nst, k, par, q == 0, q + 1 == n_pc - 1, s));
+          HIP_OK(launch_batch<R>(BK_SWEEP2, sv, C, n, L.sweep, L.n_sweep, k, par, q == 0, q + 1 == n_pc - 1, s));
           q += 2;
         } else {
-          HIP_OK(launch_batch<R>(BK_SWEEP, cv, C, n, ct, nct, k, par, q == 0, q == n_pc - 1, s));
+          HIP_OK(launch_batch<R>(BK_SWEEP, cv, C, n, L.cell, L.n_cell, k, par, q == 0, q == n_pc - 1, s));
           q += 1;
         }
         par ^= 1;
       }
-      HIP_OK(launch_batch<R>(BK_CORRECT, cv, C, n, ct, nct, k, par, 0, 0, s));
-      HIP_OK(launch_batch<R>(BK_WALL_OPS, 0, C, n, b->wall_t, b->n_wall, k, par,
+      HIP_OK(launch_batch<R>(BK_CORRECT, cv, C, n, L.cell, L.n_cell, k, par, 0, 0, s));
+      HIP_OK(launch_batch<R>(BK_WALL_OPS, 0, C, n, L.wall, L.n_wall, k, par,
                              STS_BC_SLIP_U | STS_BC_SLIP_V | STS_BC_JUMP, 0, s));
       HIP_OK(launch_batch<R>(BK_EDGES, 0, C, n, nullptr, 0, k, par,
                              STS_BC_VELOCITIES | STS_BC_TEMPERATURE | STS_BC_PRESSURE, 0, s));
@@ -1812,23 +1884,30 @@ static int run_batch_loop2(sts_batch *b, BatchCase<R> *C, int n_pc, int N_I, int
           AABatchSel bs{};
           bs.n = std::min(K, aa_m);
           for (int j = 0; j < bs.n; j++) bs.slot[j] = (K - bs.n + j) % aa_m;
-          HIP_OK(launch_batch_aa(BKA_STORE, cv, C, n, ct, nct, k, par, bs, s));
+          HIP_OK(launch_batch_aa(BKA_STORE, cv, C, n, L.cell, L.n_cell, k, par, bs, s));
           if (bs.n > 0) {
-            HIP_OK(launch_batch_aa(BKA_SOLVE, cv, C, n, ct, nct, k, par, bs, s));
-            HIP_OK(launch_batch_aa(BKA_MIX, cv, C, n, ct, nct, k, par, bs, s));
+            HIP_OK(launch_batch_aa(BKA_SOLVE, cv, C, n, L.cell, L.n_cell, k, par, bs, s));
+            HIP_OK(launch_batch_aa(BKA_MIX, cv, C, n, L.cell, L.n_cell, k, par, bs, s));
           }
         }
-      HIP_OK(launch_batch<R>(BK_FLUX_DIFF, cv, C, n, ct, nct, k, par, 0, 0, s));
+      HIP_OK(launch_batch<R>(BK_FLUX_DIFF, cv, C, n, L.cell, L.n_cell, k, par, 0, 0, s));
       if (k == check) {
         HIP_OK(hipMemcpyAsync(b->h_ctrl, b->ctrl, sizeof(LoopCtrl) * n, hipMemcpyDeviceToHost, s));
         HIP_OK(hipEventRecord(b->ev_ctrl, s));
       }
     }
     HIP_OK(hipEventSynchronize(b->ev_ctrl));
-    bool open = false;
-    for (int i = 0; i < n; i++) open = open || b->h_ctrl[i].stop_at == 0x7fffffff;
-    if (!open) break;
+    int n_open = 0;
+    for (int i = 0; i < n; i++) {
+      open_c[i] = b->h_ctrl[i].stop_at == 0x7fffffff;
+      n_open += open_c[i];
+    }
+    if (n_open == 0) break;
     if (check == N_I - 1) return fail(STS_ERR_ARG, "loop 2 did not stop at N_I");
+    if (n_open < L.n_open) {
+      int rc;
+      if ((rc = compact_lists(b, open_c, L))) return rc;
+    }
     batch = 2;
   }
   iters.assign(n, 0);

@@ -1,7 +1,10 @@
 """Batched time steps: cases of different grid sizes stepped together in
 shared launches give, case by case, exactly the fields and iteration counts
 of the same cases stepped one by one -- for every tile shape, after the
-autotuner has run, and for the mixed-precision step."""
+autotuner has run, and for the mixed-precision step.  The set holds
+confined squares, a channel periodic about OX and two walled
+Rayleigh-Benard boxes (gravity term c_Fr_v in the v equation,
+SIMPLE_TS_MPI.cpp:3404, :3585, c_Fr_v set at :2066)."""
 import importlib
 
 import numpy as np
@@ -11,6 +14,11 @@ pytestmark = pytest.mark.gpu
 
 # (Nx, Ny, Ma, Kn, side, xc)
 CASES = [(60, 14, 0.1, 0.00194, 0.25, 4.0), (96, 20, 1.2, 0.01, 0.3, 3.0), (128, 32, 2.0, 0.003, 0.2, 5.0)]
+# (Nx, Ny, Ra) of the Rayleigh-Benard boxes, stepped at dt = 2e-3 (the oracle
+# needs 14 and 28-34 outer iterations per step over the first 10 steps; at
+# dt = 5e-3 the 100 x 51 box goes non-finite in its third step, on the oracle
+# as on the GPU)
+RB = [(64, 32, 3000.0), (100, 51, 20000.0)]
 
 
 def runs(tmp_path, tag, precision=64):
@@ -24,6 +32,11 @@ def runs(tmp_path, tag, precision=64):
     cs = importlib.import_module("simple-ts_amd.cases")
     d = cs.periodic_channel(str(tmp_path / f"{tag}p"), 100, 22, 0.3, 0.02, 10, dt=0.005)
     out.append(ws.Run(d, precision))
+    # walled Rayleigh-Benard boxes (2:1, hot bottom, cold top) at Ra 3000 and 20000
+    for k, (Nx, Ny, Ra) in enumerate(RB):
+        d = cs.rayleigh_benard(str(tmp_path / f"{tag}rb{k}"), Nx, Ny, Kn=0.05, Fr=cs.rayleigh_fr(Ra, 0.05), steps=10,
+                               dt=2e-3)
+        out.append(ws.Run(d, precision))
     return out
 
 
