@@ -1798,7 +1798,7 @@ extern "C" int sts_batch_set_tiles(sts_batch *b, int cell_variant, int sweep_var
 template <typename R>
 static int upload_cases(sts_batch *b, BatchCase<R> *dst, int N_I, double tol_uv, double tol_p, double tol_T,
                         int stall_w, double stall_ratio, size_t log_off, bool gated,
-                        const std::vector<char> *skip = nullptr) {
+                        const std::vector<char> *skip = nullptr, const std::vector<char> *inline0 = nullptr) {
   std::vector<BatchCase<R>> h(b->cs.size());
   for (size_t i = 0; i < b->cs.size(); i++) {
     sts_ctx *c = b->cs[i];
@@ -1827,6 +1827,7 @@ static int upload_cases(sts_batch *b, BatchCase<R> *dst, int N_I, double tol_uv,
     x.wops = c->wall_ops;
     x.nwops = c->n_wall_ops;
     x.skip = skip ? (*skip)[i] : 0;
+    x.inline0 = inline0 ? (*inline0)[i] : 0;
     x.aa = c->aa;
     x.aa.nb = aa_case_tiles(c->Nx, c->Ny, b->cell_v);
     for (int j = 0; j < MAX_AA; j++) {
@@ -1862,6 +1863,7 @@ static int run_batch_loop2(sts_batch *b, BatchCase<R> *C, int n_pc, int N_I, int
     const int check = end == N_I ? end - 1 : std::max(k, end - 1 - LOOKAHEAD);
     for (; k < end; k++) {
       int par = (int)(((long long)k * passes) & 1);
+      if (k == 0) HIP_OK(launch_batch<R>(BK_ASSEMBLE_ARR, cv, C, n, L.cell, L.n_cell, k, par, 0, 0, s));
       HIP_OK(launch_batch<R>(BK_ASSEMBLE, cv, C, n, L.cell, L.n_cell, k, par, 0, 0, s));
       for (int q = 0; q < n_pc;) {
         if (b->cs[0]->fuse_sweeps && q + 1 < n_pc) {
@@ -1969,7 +1971,6 @@ extern "C" int sts_batch_time_step(sts_batch *b, const sts_loop_control *lc, con
     }
     predicted[i] = pred && c->n_hist > 0;
     if ((rc = extrapolate_start(c, q, s, !pred))) return rc;
-    if ((rc = refresh_diff(c, s))) return rc;  // the batched assembly reads Gamma and D
   }
   std::vector<int> n32(n, 0), n64(n, 0);
   // cases the FP32 gate sends straight to FP64 sit the FP32 phase out
@@ -1979,12 +1980,21 @@ extern "C" int sts_batch_time_step(sts_batch *b, const sts_loop_control *lc, con
     skip32[i] = fp32_phase(b->cs[i], mc) ? 0 : 1;
     any32 = any32 || !skip32[i];
   }
+  // the first FP64 outer iteration forms Gamma and D itself where the
+  // single-case step would (after an FP32 phase, or when forming them gives
+  // the values the arrays hold); the FP32 phase starts from the narrowed
+  // arrays, brought up to date first
+  std::vector<char> inl64(n, 0), inl32(n, 0);
+  for (int i = 0; i < n; i++) {
+    inl64[i] = (!skip32[i] || b->cs[i]->diff_inline_ok) ? 1 : 0;
+    if (!skip32[i] && (rc = refresh_diff(b->cs[i], s))) return rc;
+  }
   if (any32) {
     const unsigned long long all = (1ull << STS_NFIELDS) - 1ull;
     for (int i = 0; i < n; i++)
       if (!skip32[i]) HIP_OK(launch_convert(b->cs[i]->fields(), b->cs[i]->fields32(), all, b->cs[i]->Na, s));
     if ((rc = upload_cases<float>(b, b->C32, mc->N_I32, mc->tol32_uv, mc->tol32_p, mc->tol32_T, mc->stall_w,
-                                  mc->stall_ratio, 0, true, &skip32)))
+                                  mc->stall_ratio, 0, true, &skip32, &inl32)))
       return rc;
     const int b32 = lc->batch > 0 ? lc->batch : (b->last_iters32 > 2 ? b->last_iters32 - 1 : 1);
     if ((rc = run_batch_loop2<float>(b, b->C32, lc->N_I_p_c, mc->N_I32, b32, n32))) return rc;
@@ -1993,15 +2003,12 @@ extern "C" int sts_batch_time_step(sts_batch *b, const sts_loop_control *lc, con
       if (!skip32[i]) lo = std::min(lo, n32[i]);
     b->last_iters32 = lo;
     for (int i = 0; i < n; i++)
-      if (!skip32[i]) {
-        HIP_OK(launch_convert(b->cs[i]->fields32(), b->cs[i]->fields(), WIDEN_MASK, b->cs[i]->Na, s));
-        // the batched assembly reads Gamma and D: formed from the widened state
-        HIP_OK(launch_flux_diff(b->cs[i]->g, b->cs[i]->fields(), s, Gate{nullptr, 0}, DIFF_PART));
-      }
+      if (!skip32[i]) HIP_OK(launch_convert(b->cs[i]->fields32(), b->cs[i]->fields(), WIDEN_MASK, b->cs[i]->Na, s));
   }
   // the FP64 logs follow the FP32 ones of each case (offset = the largest FP32 count)
   const int off = mc ? mc->N_I32 : 0;
-  if ((rc = upload_cases<double>(b, b->C64, lc->N_I, lc->tol_uv, lc->tol_p, lc->tol_T, 0, 0.0, 6 * (size_t)off, true)))
+  if ((rc = upload_cases<double>(b, b->C64, lc->N_I, lc->tol_uv, lc->tol_p, lc->tol_T, 0, 0.0, 6 * (size_t)off, true,
+                                 nullptr, &inl64)))
     return rc;
   const int b64 = lc->batch > 0 ? lc->batch : (b->last_iters64 > 2 ? b->last_iters64 - 1 : 1);
   if ((rc = run_batch_loop2<double>(b, b->C64, lc->N_I_p_c, lc->N_I, b64, n64, mc ? mc->anderson : 0,
@@ -2014,8 +2021,7 @@ extern "C" int sts_batch_time_step(sts_batch *b, const sts_loop_control *lc, con
     sts_ctx *c = b->cs[i];
     c->last_iters = n64[i];
     c->last_total = n32[i] + n64[i];
-    c->diff_inline_ok = true;  // the batched flux and diffusion pass writes every array
-    c->diff_stale = false;
+    c->diff_inline_ok = c->diff_stale = true;  // the batched passes write only the fluxes
     if (mc && mc->extrapolate > 0 && mc->guard > 0) {
       if (!predicted[i]) {
         c->plain_mean = c->plain_mean < 0.0 ? n64[i] : 0.5 * (c->plain_mean + n64[i]);
@@ -2038,14 +2044,16 @@ extern "C" int sts_batch_time_step(sts_batch *b, const sts_loop_control *lc, con
 
 // Times the bulk kernels of each tile variant on the current state (the
 // timed launches only rewrite what the next outer iteration recomputes before
-// reading it: assembly outputs, fluxes and diffusion coefficients from
-// unchanged inputs, and the p/T partners), then keeps the fastest.
+// reading it: assembly outputs, fluxes from unchanged inputs, and the p/T
+// partners), then keeps the fastest.  The assembly is timed in the form the
+// loops use after their first iteration (Gamma and D formed in the tile).
 extern "C" int sts_batch_autotune(sts_batch *b, int reps, int *cell_variant, int *sweep_variant, double *ms) {
   if (!b || reps < 1) return fail(STS_ERR_ARG, "bad autotune arguments");
   const int n = (int)b->cs.size();
   for (sts_ctx *c : b->cs) HIP_OK(hipStreamSynchronize(c->stream));
   int rc;
-  if ((rc = upload_cases<double>(b, b->C64, 1, 0, 0, 0, 0, 0.0, 0, false))) return rc;
+  const std::vector<char> inl(n, 1);
+  if ((rc = upload_cases<double>(b, b->C64, 1, 0, 0, 0, 0, 0.0, 0, false, nullptr, &inl))) return rc;
   hipStream_t s = b->stream;
   double t_cell[NVARIANT], t_sweep[NVARIANT];
   for (int v = 0; v < NVARIANT; v++) {
@@ -2073,10 +2081,6 @@ extern "C" int sts_batch_autotune(sts_batch *b, int reps, int *cell_variant, int
   for (int v = 1; v < NVARIANT; v++) {
     if (t_cell[v] < t_cell[bc]) bc = v;
     if (t_sweep[v] < t_sweep[bs]) bs = v;
-  }
-  for (sts_ctx *c : b->cs) {  // the timed flux and diffusion passes wrote every array
-    c->diff_inline_ok = true;
-    c->diff_stale = false;
   }
   b->cell_v = bc;
   b->sweep_v = bs;

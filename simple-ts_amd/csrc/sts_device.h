@@ -220,7 +220,8 @@ struct BatchCase {
   double tol_uv, tol_p, tol_T, stall_ratio;
   const WallOp *wops;  // the case's planned wall writes
   int nwops;
-  int skip;  // the case sits this loop out (its loop control opens closed)
+  int skip;     // the case sits this loop out (its loop control opens closed)
+  int inline0;  // the first outer iteration forms Gamma and D itself (else reads the arrays)
   // Anderson acceleration (FP64 loops): the case's vectors (aa.nb = its
   // tiles of the cell variant in use) and its ring of differences
   AABuf aa;

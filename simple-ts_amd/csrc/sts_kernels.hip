@@ -401,25 +401,26 @@ __global__ void __launch_bounds__(BX *BY) k_assemble(KGridT<R> g, KFieldsT<R> fl
   assemble_cell<R>(g, fl, i0 + (int)threadIdx.x, j0 + (int)threadIdx.y);
 }
 
-// The assembly with the diffusion coefficients formed in the workgroup: each
-// face of the tile once, plus the faces its cells read beyond it (column BX:
-// the i1j face of each row's last cell, row BY: the ij1 face of each
-// column's last cell, as the clamped or wrapped neighbour indices give
-// them), kept in LDS; then every cell assembles from them.
-template <typename R>
-__global__ void __launch_bounds__(BX *BY) k_assemble_inl(KGridT<R> g, KFieldsT<R> fl, Gate gate) {
-  if (gated_out(gate)) return;
-  __shared__ R Lgy[BY + 1][BX + 1], Lgx[BY + 1][BX + 1], Lux[BY + 1][BX + 1], Luy[BY + 1][BX + 1],
-      Lvx[BY + 1][BX + 1], Lvy[BY + 1][BX + 1];
-  __shared__ R SFx[BY + 2][BX + 2], SFy[BY + 2][BX + 2], Su[BY + 2][BX + 2], Sv[BY + 2][BX + 2],
-      SsT[BY + 2][BX + 2];
-  int i0, j0;
-  tile_origin<BX, BY>(g.Nx, i0, j0);
-  const int Nx = g.Nx, Ny = g.Ny, tx = (int)threadIdx.x, ty = (int)threadIdx.y;
+// The assembly of the TX x TY tile at (i0, j0) with the diffusion
+// coefficients formed in the workgroup: each face of the tile once, plus the
+// faces its cells read beyond it (column TX: the i1j face of each row's last
+// cell, row TY: the ij1 face of each column's last cell, as the clamped or
+// wrapped neighbour indices give them), kept in LDS; then every cell
+// assembles from them.  (tx, ty): the thread's cell in the tile; the
+// workgroup has TX * TY >= TX + TY threads.
+template <typename R, int TX, int TY>
+__device__ __forceinline__ void assemble_inl_tile(const KGridT<R> &g, const KFieldsT<R> &fl, int i0, int j0, int tx,
+                                                  int ty) {
+  static_assert(TX * TY >= TX + TY, "the face ring needs TX + TY threads");
+  __shared__ R Lgy[TY + 1][TX + 1], Lgx[TY + 1][TX + 1], Lux[TY + 1][TX + 1], Luy[TY + 1][TX + 1],
+      Lvx[TY + 1][TX + 1], Lvy[TY + 1][TX + 1];
+  __shared__ R SFx[TY + 2][TX + 2], SFy[TY + 2][TX + 2], Su[TY + 2][TX + 2], Sv[TY + 2][TX + 2],
+      SsT[TY + 2][TX + 2];
+  const int Nx = g.Nx, Ny = g.Ny;
   // the tile's neighbourhood of the fields the cells read (TileStencil); the
   // entries past the grid's last tile column are never read, only kept in it
-  for (int e = tx + ty * BX; e < (BX + 2) * (BY + 2); e += BX * BY) {
-    const int x = e % (BX + 2), y = e / (BX + 2);
+  for (int e = tx + ty * TX; e < (TX + 2) * (TY + 2); e += TX * TY) {
+    const int x = e % (TX + 2), y = e / (TX + 2);
     int gi = i0 + x - 1;
     gi = g.px ? ((gi % Nx) + Nx) % Nx : min(max(gi, 0), Nx - 1);
     const int gj = min(max(j0 + y - 1, 0), Ny - 1);
@@ -431,7 +432,7 @@ __global__ void __launch_bounds__(BX *BY) k_assemble_inl(KGridT<R> g, KFieldsT<R
     SsT[y][x] = FLD(SQRT_T)[k];
   }
   const int i = i0 + tx, j = j0 + ty;
-  const int il = min(i0 + BX, Nx) - 1, jl = min(j0 + BY, Ny) - 1;  // last cell of the tile in x, y
+  const int il = min(i0 + TX, Nx) - 1, jl = min(j0 + TY, Ny) - 1;  // last cell of the tile in x, y
   auto put = [&](int y, int x, unsigned k) {
     const FaceDiff<R> d = face_diff(g, fl, k);
     Lgy[y][x] = d.Gy;
@@ -443,12 +444,12 @@ __global__ void __launch_bounds__(BX *BY) k_assemble_inl(KGridT<R> g, KFieldsT<R
   };
   const bool in = i < Nx && j < Ny;
   if (in) put(ty, tx, (unsigned)i + (unsigned)j * (unsigned)Nx);
-  const int t = tx + ty * BX;
-  if (t < BX) {  // north: the ij1 face of column t's last cell
-    if (i0 + t < Nx) put(BY, t, Nbr((unsigned)(i0 + t), (unsigned)jl, (unsigned)Nx, (unsigned)Ny, g.px).ij1);
-  } else if (t < BX + BY) {  // east: the i1j face of row t - BX's last cell
-    const int r = t - BX;
-    if (j0 + r < Ny) put(r, BX, Nbr((unsigned)il, (unsigned)(j0 + r), (unsigned)Nx, (unsigned)Ny, g.px).i1j);
+  const int t = tx + ty * TX;
+  if (t < TX) {  // north: the ij1 face of column t's last cell
+    if (i0 + t < Nx) put(TY, t, Nbr((unsigned)(i0 + t), (unsigned)jl, (unsigned)Nx, (unsigned)Ny, g.px).ij1);
+  } else if (t < TX + TY) {  // east: the i1j face of row t - TX's last cell
+    const int r = t - TX;
+    if (j0 + r < Ny) put(r, TX, Nbr((unsigned)il, (unsigned)(j0 + r), (unsigned)Nx, (unsigned)Ny, g.px).i1j);
   }
   __syncthreads();
   if (!in) return;
@@ -457,15 +458,23 @@ __global__ void __launch_bounds__(BX *BY) k_assemble_inl(KGridT<R> g, KFieldsT<R
   const bool c = (nb & NB_C) != 0;
   const bool du = in_box(i, j, g.Nx_u0, g.Nx_u1, g.Ny_u0, g.Ny_u1) && (nb & NB_W) && c;
   const bool dv = in_box(i, j, g.Nx_v0, g.Nx_v1, g.Ny_v0, g.Ny_v1) && (nb & NB_S) && c;
-  const TileStencil<R, BX, BY> S{SFx, SFy, Su, Sv, SsT, tx, ty};
+  const TileStencil<R, TX, TY> S{SFx, SFy, Su, Sv, SsT, tx, ty};
   if (du || dv) {
-    const int ex = (i == il) ? BX : tx + 1, ny = (j == jl) ? BY : ty + 1;
+    const int ex = (i == il) ? TX : tx + 1, ny = (j == jl) ? TY : ty + 1;
     const CellDiff<R> df{Lgy[ty][tx], Lgy[ny][tx], Lgx[ty][tx], Lgx[ty][ex], Lux[ty][tx], Lux[ty][ex],
                          Luy[ty][tx], Luy[ny][tx], Lvx[ty][tx], Lvx[ty][ex], Lvy[ty][tx], Lvy[ny][tx]};
     if (du) cell_assemble_u(g, fl, n, df, nb, S);
     if (dv) cell_assemble_v(g, fl, n, df, nb, S);
   }
   if (in_box(i, j, g.Nx_o0, g.Nx_o1, g.Ny_o0, g.Ny_o1) && c) cell_energy_source(g, fl, n, nb, S);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(BX *BY) k_assemble_inl(KGridT<R> g, KFieldsT<R> fl, Gate gate) {
+  if (gated_out(gate)) return;
+  int i0, j0;
+  tile_origin<BX, BY>(g.Nx, i0, j0);
+  assemble_inl_tile<R, BX, BY>(g, fl, i0, j0, (int)threadIdx.x, (int)threadIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -1376,13 +1385,6 @@ __global__ void __launch_bounds__(EDGE_THREADS) k_edges(KGridT<R> g, KFieldsT<R>
 // A10 + A11: fluxes (S:10440-10454) and diffusion coefficients
 // (S:10457-10618) over all cells.
 // ---------------------------------------------------------------------------
-template <typename R>
-__device__ __forceinline__ void flux_diff_cell(const KGridT<R> &g, const KFieldsT<R> &fl, int i, int j) {
-  FluxIn<R> c;
-  c.load(g, fl, i, j);
-  c.apply(g, fl);
-}
-
 // FLUX_ROWS cells per thread, rows BY apart, all loads issued first.
 constexpr int FLUX_ROWS = 2;
 
@@ -2078,13 +2080,25 @@ __device__ __forceinline__ KFieldsT<R> fields_at(const BatchCase<R> &c, int par)
   return f;
 }
 
+// The assembly forms Gamma and D itself (assemble_inl_tile), except in the
+// first outer iteration of a case whose arrays are to be read (inline0 = 0),
+// as the single-case loops do: kb_assemble_arr does those, in a kernel of its
+// own so that the two forms do not share one register budget.
 template <typename R, int TX, int TY>
 __global__ void __launch_bounds__(TX *TY) kb_assemble(const BatchCase<R> *C, const BatchTile *T, int k, int par) {
   const BatchTile t = T[blockIdx.x];
   const BatchCase<R> &c = C[t.c];
-  if (gated_out(Gate{c.ctrl, k})) return;
-  const KFieldsT<R> f = fields_at(c, par);
-  assemble_cell(c.g, f, t.x * TX + (int)threadIdx.x % TX, t.y * TY + (int)threadIdx.x / TX);
+  if (gated_out(Gate{c.ctrl, k}) || (k == 0 && !c.inline0)) return;
+  assemble_inl_tile<R, TX, TY>(c.g, fields_at(c, par), t.x * TX, t.y * TY, (int)threadIdx.x % TX,
+                               (int)threadIdx.x / TX);
+}
+
+template <typename R, int TX, int TY>
+__global__ void __launch_bounds__(TX *TY) kb_assemble_arr(const BatchCase<R> *C, const BatchTile *T, int k, int par) {
+  const BatchTile t = T[blockIdx.x];
+  const BatchCase<R> &c = C[t.c];
+  if (gated_out(Gate{c.ctrl, k}) || k != 0 || c.inline0) return;
+  assemble_cell(c.g, fields_at(c, par), t.x * TX + (int)threadIdx.x % TX, t.y * TY + (int)threadIdx.x / TX);
 }
 
 // Pass of two fused sweeps from parity par; first/last: whether it holds
@@ -2171,13 +2185,17 @@ __global__ void __launch_bounds__(WALL_OPS_PER_TILE) kb_wall_ops(const BatchCase
   if (parts & need) apply_wall_op(c.g, fields_at(c, par), o);
 }
 
+// The fluxes and thermal coefficients (FLUX_PART); the assembly forms the
+// diffusion coefficients itself.
 template <typename R, int TX, int TY>
 __global__ void __launch_bounds__(TX *TY) kb_flux_diff(const BatchCase<R> *C, const BatchTile *T, int k, int par) {
   const BatchTile t = T[blockIdx.x];
   const BatchCase<R> &c = C[t.c];
   if (gated_out(Gate{c.ctrl, k})) return;
   const KFieldsT<R> f = fields_at(c, par);
-  flux_diff_cell(c.g, f, t.x * TX + (int)threadIdx.x % TX, t.y * TY + (int)threadIdx.x / TX);
+  FluxIn<R> x;
+  x.template load<true, false>(c.g, f, t.x * TX + (int)threadIdx.x % TX, t.y * TY + (int)threadIdx.x / TX);
+  x.apply_flux(c.g, f);
 }
 
 // Batched Anderson steps (FP64): the single-context kernels' code per tile
@@ -2538,6 +2556,7 @@ hipError_t launch_batch(int kernel, int variant, const BatchCase<R> *C, int ncas
   if (kernel != BK_EDGES && kernel != BK_LOOP_OPEN && ntiles == 0) return hipSuccess;
   switch (kernel) {
     case BK_ASSEMBLE: CELL_SWITCH(variant, kb_assemble, C, T, k, par); break;
+    case BK_ASSEMBLE_ARR: CELL_SWITCH(variant, kb_assemble_arr, C, T, k, par); break;
     case BK_SWEEP: CELL_SWITCH(variant, kb_sweep, C, T, k, par, a, b); break;
     case BK_CORRECT: CELL_SWITCH(variant, kb_correct, C, T, k, par); break;
     case BK_WALL: CELL_SWITCH(variant, kb_wall, C, T, k, par, a); break;

@@ -84,7 +84,20 @@ hipError_t launch_continuity(const KGrid &g, const KFields &f, double *cs, doubl
 // C[0..ncases).  kernel: BK_*; variant: index into CELL_TILES (cell
 // kernels) or SWEEP_TILES (BK_SWEEP2).  (a, b) = (first, last) for the
 // sweeps, a = parts for BK_WALL and BK_EDGES.
-enum { BK_ASSEMBLE, BK_SWEEP2, BK_SWEEP, BK_CORRECT, BK_WALL, BK_EDGES, BK_FLUX_DIFF, BK_LOOP_OPEN, BK_WALL_OPS };
+// BK_ASSEMBLE forms Gamma and D in the tile; BK_ASSEMBLE_ARR (first outer
+// iteration only) assembles the cases whose arrays are to be read instead.
+enum {
+  BK_ASSEMBLE,
+  BK_SWEEP2,
+  BK_SWEEP,
+  BK_CORRECT,
+  BK_WALL,
+  BK_EDGES,
+  BK_FLUX_DIFF,
+  BK_LOOP_OPEN,
+  BK_WALL_OPS,
+  BK_ASSEMBLE_ARR
+};
 // BK_WALL_OPS: tile (c, x) = planned wall writes [x W, x W + W) of case c,
 // W = WALL_OPS_PER_TILE.
 constexpr int NVARIANT = 4;
