@@ -208,26 +208,6 @@ struct FluxIn {
   }
 };
 
-// Gamma and the velocity diffusion coefficients of face k as the flux and
-// diffusion pass writes them (the assembly's faces always have a fluid cell,
-// so the untouched case does not arise there).
-template <typename R>
-struct FaceDiff {
-  R Gy, Gx, Dux, Duy, Dvx, Dvy;
-};
-template <typename R>
-__device__ __forceinline__ FaceDiff<R> face_diff(const KGridT<R> &g, const KFieldsT<R> &fl, unsigned k) {
-  FluxIn<R> f;
-  f.template load<false, true>(g, fl, (int)(k % (unsigned)g.Nx), (int)(k / (unsigned)g.Nx));
-  FaceDiff<R> d;
-  f.gamma(d.Gy, d.Gx);
-  d.Dux = f.d_ux(g);
-  d.Duy = f.d_uy(g, d.Gy);
-  d.Dvx = f.d_vx(g, d.Gx);
-  d.Dvy = f.d_vy(g);
-  return d;
-}
-
 // ---------------------------------------------------------------------------
 // A2 + A3 + A4: coefficient assembly (S:3269-3756)
 // ---------------------------------------------------------------------------
@@ -401,40 +381,106 @@ __global__ void __launch_bounds__(BX *BY) k_assemble(KGridT<R> g, KFieldsT<R> fl
   assemble_cell<R>(g, fl, i0 + (int)threadIdx.x, j0 + (int)threadIdx.y);
 }
 
+// Gamma and the velocity diffusion coefficients of one face as the flux and
+// diffusion pass writes them (the assembly's faces always have a fluid cell,
+// so the untouched case does not arise there).
+template <typename R>
+struct FaceDiff {
+  R Gy, Gx, Dux, Duy, Dvx, Dvy;
+};
+
+// Gamma and the velocity diffusion coefficients of the face of cell (fi, fj)
+// as face_diff forms them, with sqrt_T and the mesh steps taken from a
+// workgroup's LDS tile: (cx, cy) is the face cell's position in the tile
+// (entry [cy + 1][cx + 1] of sT holds its sqrt_T, [cy][cx] that of its
+// clamped or wrapped south-west neighbour; Shx[cx + 1], Shx[cx] its hx and
+// the west one, Shy likewise in y), b its neighbour fluid bits.  The wall
+// temperatures are read only where some but not all of the four cells are
+// fluid, the one case gamma() uses them.
+template <typename R, int TX>
+__device__ __forceinline__ FaceDiff<R> face_diff_tile(const KGridT<R> &g, const KFieldsT<R> &fl, unsigned fi,
+                                                      unsigned fj, unsigned b, const R (*sT)[TX + 2], const R *Shx,
+                                                      const R *Shy, int cx, int cy) {
+  FluxIn<R> f;
+  f.in = true;
+  f.hxi = Shx[cx + 1];
+  f.hxw = Shx[cx];
+  f.hyj = Shy[cy + 1];
+  f.hys = Shy[cy];
+  f.m00 = (b & NB_SW) != 0;
+  f.m10 = (b & NB_S) != 0;
+  f.m01 = (b & NB_W) != 0;
+  f.m11 = (b & NB_C) != 0;
+  f.s00 = sT[cy][cx];
+  f.s10 = sT[cy][cx + 1];
+  f.s01 = sT[cy + 1][cx];
+  f.s11 = sT[cy + 1][cx + 1];
+  f.twu10 = f.twu11 = f.twv01 = f.twv11 = R(0);
+  if ((f.m00 || f.m10 || f.m01 || f.m11) && !(f.m00 && f.m10 && f.m01 && f.m11)) {
+    const Nbr n(fi, fj, (unsigned)g.Nx, (unsigned)g.Ny, g.px);
+    f.twu10 = FLD(TW_U)[n.ij_1];
+    f.twu11 = FLD(TW_U)[n.ij];
+    f.twv01 = FLD(TW_V)[n.i_1j];
+    f.twv11 = FLD(TW_V)[n.ij];
+  }
+  FaceDiff<R> d;
+  f.gamma(d.Gy, d.Gx);
+  d.Dux = f.d_ux(g);
+  d.Duy = f.d_uy(g, d.Gy);
+  d.Dvx = f.d_vx(g, d.Gx);
+  d.Dvy = f.d_vy(g);
+  return d;
+}
+
 // The assembly of the TX x TY tile at (i0, j0) with the diffusion
 // coefficients formed in the workgroup: each face of the tile once, plus the
 // faces its cells read beyond it (column TX: the i1j face of each row's last
 // cell, row TY: the ij1 face of each column's last cell, as the clamped or
 // wrapped neighbour indices give them), kept in LDS; then every cell
 // assembles from them.  (tx, ty): the thread's cell in the tile; the
-// workgroup has TX * TY >= TX + TY threads.
+// workgroup has TX * TY >= TX + TY + 4 threads.
 template <typename R, int TX, int TY>
 __device__ __forceinline__ void assemble_inl_tile(const KGridT<R> &g, const KFieldsT<R> &fl, int i0, int j0, int tx,
                                                   int ty) {
-  static_assert(TX * TY >= TX + TY, "the face ring needs TX + TY threads");
+  static_assert(TX * TY >= TX + TY + 4, "the face ring and the mesh steps need TX + TY + 4 threads");
   __shared__ R Lgy[TY + 1][TX + 1], Lgx[TY + 1][TX + 1], Lux[TY + 1][TX + 1], Luy[TY + 1][TX + 1],
       Lvx[TY + 1][TX + 1], Lvy[TY + 1][TX + 1];
   __shared__ R SFx[TY + 2][TX + 2], SFy[TY + 2][TX + 2], Su[TY + 2][TX + 2], Sv[TY + 2][TX + 2],
-      SsT[TY + 2][TX + 2];
-  const int Nx = g.Nx, Ny = g.Ny;
-  // the tile's neighbourhood of the fields the cells read (TileStencil); the
-  // entries past the grid's last tile column are never read, only kept in it
-  for (int e = tx + ty * TX; e < (TX + 2) * (TY + 2); e += TX * TY) {
-    const int x = e % (TX + 2), y = e / (TX + 2);
+      SsT[TY + 2][TX + 2], Shx[TX + 2], Shy[TY + 2];
+  const int Nx = g.Nx, Ny = g.Ny, t = tx + ty * TX;
+  // column x - 1 and row y - 1 of the tile, clamped (x wrapped when periodic)
+  // as Nbr takes a neighbour; x - 1 >= -1, and x - 1 < Nx + TX
+  auto col = [&](int x) {
     int gi = i0 + x - 1;
-    gi = g.px ? ((gi % Nx) + Nx) % Nx : min(max(gi, 0), Nx - 1);
-    const int gj = min(max(j0 + y - 1, 0), Ny - 1);
-    const unsigned k = (unsigned)gi + (unsigned)gj * (unsigned)Nx;
+    if (g.px) {
+      if (gi < 0) gi += Nx;
+      while (gi >= Nx) gi -= Nx;
+      return gi;
+    }
+    return min(max(gi, 0), Nx - 1);
+  };
+  auto row = [&](int y) { return min(max(j0 + y - 1, 0), Ny - 1); };
+  // the tile's neighbourhood of the fields the cells and faces read
+  // (TileStencil, face_diff_tile); the entries past the grid's last tile
+  // column are never read, only kept in it
+  for (int e = t; e < (TX + 2) * (TY + 2); e += TX * TY) {
+    const int x = e % (TX + 2), y = e / (TX + 2);
+    const unsigned k = (unsigned)col(x) + (unsigned)row(y) * (unsigned)Nx;
     SFx[y][x] = FLD(FX)[k];
     SFy[y][x] = FLD(FY)[k];
     Su[y][x] = FLD(U)[k];
     Sv[y][x] = FLD(V)[k];
     SsT[y][x] = FLD(SQRT_T)[k];
   }
+  if (t < TX + 2) Shx[t] = g.hx[col(t)];
+  else if (t < TX + TY + 4) Shy[t - (TX + 2)] = g.hy[row(t - (TX + 2))];
   const int i = i0 + tx, j = j0 + ty;
   const int il = min(i0 + TX, Nx) - 1, jl = min(j0 + TY, Ny) - 1;  // last cell of the tile in x, y
-  auto put = [&](int y, int x, unsigned k) {
-    const FaceDiff<R> d = face_diff(g, fl, k);
+  const bool in = i < Nx && j < Ny;
+  const unsigned ij = (unsigned)i + (unsigned)j * (unsigned)Nx, nb = in ? g.nb[ij] : 0u;
+  __syncthreads();
+  auto put = [&](int y, int x, unsigned fi, unsigned fj, unsigned b, int cx, int cy) {
+    const FaceDiff<R> d = face_diff_tile<R, TX>(g, fl, fi, fj, b, SsT, Shx, Shy, cx, cy);
     Lgy[y][x] = d.Gy;
     Lgx[y][x] = d.Gx;
     Lux[y][x] = d.Dux;
@@ -442,19 +488,20 @@ __device__ __forceinline__ void assemble_inl_tile(const KGridT<R> &g, const KFie
     Lvx[y][x] = d.Dvx;
     Lvy[y][x] = d.Dvy;
   };
-  const bool in = i < Nx && j < Ny;
-  if (in) put(ty, tx, (unsigned)i + (unsigned)j * (unsigned)Nx);
-  const int t = tx + ty * TX;
-  if (t < TX) {  // north: the ij1 face of column t's last cell
-    if (i0 + t < Nx) put(TY, t, Nbr((unsigned)(i0 + t), (unsigned)jl, (unsigned)Nx, (unsigned)Ny, g.px).ij1);
-  } else if (t < TX + TY) {  // east: the i1j face of row t - TX's last cell
+  if (in) put(ty, tx, (unsigned)i, (unsigned)j, nb, tx, ty);
+  if (t < TX) {  // north: the ij1 face of column t's last cell (jl itself on the grid's top row)
+    const int fj = jl + (jl < Ny - 1 ? 1 : 0);
+    if (i0 + t < Nx) put(TY, t, (unsigned)(i0 + t), (unsigned)fj, g.nb[(unsigned)(i0 + t) + (unsigned)fj * (unsigned)Nx],
+                         t, fj - j0);
+  } else if (t < TX + TY) {  // east: the i1j face of row t - TX's last cell (Nbr's i1 of il)
     const int r = t - TX;
-    if (j0 + r < Ny) put(r, TX, Nbr((unsigned)il, (unsigned)(j0 + r), (unsigned)Nx, (unsigned)Ny, g.px).i1j);
+    const int fi = g.px ? (il == Nx - 1 ? 0 : il + 1) : il + (il < Nx - 1 ? 1 : 0);
+    if (j0 + r < Ny) put(r, TX, (unsigned)fi, (unsigned)(j0 + r), g.nb[(unsigned)fi + (unsigned)(j0 + r) * (unsigned)Nx],
+                         il - i0 + (fi != il ? 1 : 0), r);
   }
   __syncthreads();
   if (!in) return;
   const Nbr n((unsigned)i, (unsigned)j, (unsigned)Nx, (unsigned)Ny, g.px);
-  const unsigned nb = g.nb[n.ij];
   const bool c = (nb & NB_C) != 0;
   const bool du = in_box(i, j, g.Nx_u0, g.Nx_u1, g.Ny_u0, g.Ny_u1) && (nb & NB_W) && c;
   const bool dv = in_box(i, j, g.Nx_v0, g.Nx_v1, g.Ny_v0, g.Ny_v1) && (nb & NB_S) && c;
@@ -469,12 +516,15 @@ __device__ __forceinline__ void assemble_inl_tile(const KGridT<R> &g, const KFie
   if (in_box(i, j, g.Nx_o0, g.Nx_o1, g.Ny_o0, g.Ny_o1) && c) cell_energy_source(g, fl, n, nb, S);
 }
 
+// Tile of the loops' assembly.
+constexpr int AX = 64, AY = 4;
+
 template <typename R>
-__global__ void __launch_bounds__(BX *BY) k_assemble_inl(KGridT<R> g, KFieldsT<R> fl, Gate gate) {
+__global__ void __launch_bounds__(AX *AY) k_assemble_inl(KGridT<R> g, KFieldsT<R> fl, Gate gate) {
   if (gated_out(gate)) return;
   int i0, j0;
-  tile_origin<BX, BY>(g.Nx, i0, j0);
-  assemble_inl_tile<R, BX, BY>(g, fl, i0, j0, (int)threadIdx.x, (int)threadIdx.y);
+  tile_origin<AX, AY>(g.Nx, i0, j0);
+  assemble_inl_tile<R, AX, AY>(g, fl, i0, j0, (int)threadIdx.x, (int)threadIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -2310,7 +2360,7 @@ static inline dim3 grid_tiles(const KGridT<R> &g) {
 
 template <typename R>
 hipError_t launch_assemble(const KGridT<R> &g, const KFieldsT<R> &f, hipStream_t s, Gate gate, bool inline_diff) {
-  if (inline_diff) hipLaunchKernelGGL(k_assemble_inl<R>, (grid_tiles<BX, BY>(g)), dim3(BX, BY), 0, s, g, f, gate);
+  if (inline_diff) hipLaunchKernelGGL(k_assemble_inl<R>, (grid_tiles<AX, AY>(g)), dim3(AX, AY), 0, s, g, f, gate);
   else hipLaunchKernelGGL(k_assemble<R>, (grid_tiles<BX, BY>(g)), dim3(BX, BY), 0, s, g, f, gate);
   return hipGetLastError();
 }
