@@ -421,6 +421,8 @@ static void narrow_grid(sts_ctx *c) {
   h.Nx_o0 = g.Nx_o0; h.Nx_o1 = g.Nx_o1; h.Ny_o0 = g.Ny_o0; h.Ny_o1 = g.Ny_o1;
   h.ht = (float)g.ht; h.gamma1 = (float)g.gamma1; h.Kn = (float)g.Kn; h.cT = (float)g.cT;
   h.cT1 = (float)g.cT1; h.cT2 = (float)g.cT2; h.cPch = (float)g.cPch; h.c_Fr_v = (float)g.c_Fr_v;
+  h.rht = 1.0f / h.ht;  // IEEE single division on the host: RN(1/ht) in FP32
+  h.rht_one = std::fabs(std::fma(-h.rht, h.ht, 1.0f)) <= 0x1p-25f;
   h.given_xb = g.given_xb; h.given_xe = g.given_xe; h.kn_local = g.kn_local;
   h.bc = g.bc;
   h.px = g.px;
@@ -564,6 +566,8 @@ extern "C" int sts_create(const sts_grid *gi, int precision, sts_ctx **out) {
   g.Nx_v0 = gi->Nx_v[0]; g.Nx_v1 = gi->Nx_v[1]; g.Ny_v0 = gi->Ny_v[0]; g.Ny_v1 = gi->Ny_v[1];
   g.Nx_o0 = gi->Nx_o[0]; g.Nx_o1 = gi->Nx_o[1]; g.Ny_o0 = gi->Ny_o[0]; g.Ny_o1 = gi->Ny_o[1];
   g.ht = gi->ht;
+  g.rht = 1.0 / g.ht;  // RN(1/ht); 1 - rht ht is exact in an fma
+  g.rht_one = std::fabs(std::fma(-g.rht, g.ht, 1.0)) <= 0x1p-54;
   g.gamma1 = gi->gamma1;
   g.Kn = gi->Kn;
   g.cT = gi->cT;

@@ -39,6 +39,25 @@ __device__ __forceinline__ bool in_box(int i, int j, int x0, int x1, int y0, int
   return i >= x0 && i < x1 && j >= y0 && j < y1;
 }
 
+// a / b, correctly rounded, from y = RN(1/b) when RN(a y) is within one ulp
+// of a/b -- Markstein's theorem: then r = a - b q is exact in an fma and
+// RN(q + r y) = RN(a/b), barring underflow and overflow.  RN(a y) is that
+// close for every a when |1 - b y| <= 2^-(p+1) (half the unit roundoff).
+// r = 0 (an exact quotient, a = +-0) or NaN (a infinite or NaN) keeps q, so
+// signed zeros and non-finite quotients are those of a / b.
+template <typename R>
+__device__ __forceinline__ R quot1(R a, R b, R y) {
+  const R q = a * y;
+  const R r = fma(-q, b, a);
+  return (fabs(r) > R(0)) ? fma(r, y, q) : q;
+}
+
+// a / 3: RN(1/3) = (1/3)(1 - 2^-(p+1)) meets quot1's condition.
+template <typename R>
+__device__ __forceinline__ R third(R a) {
+  return quot1(a, R(3.0), R(1.0) / R(3.0));
+}
+
 // Non-negative doubles order like their bit patterns; a NaN (0x7ff8...)
 // compares above +inf and so survives the max and is reported.  FP32
 // differences are widened to double (exactly) first.
@@ -240,13 +259,15 @@ struct GlobalStencil {
   __device__ __forceinline__ R u(int di, int dj) const { return u_[at(di, dj)]; }
   __device__ __forceinline__ R v(int di, int dj) const { return v_[at(di, dj)]; }
   __device__ __forceinline__ R sT(int di, int dj) const { return sT_[at(di, dj)]; }
+  __device__ __forceinline__ R div_ht(R a, const KGridT<R> &g) const { return a / g.ht; }
 };
 
 // LDS tile [TY + 2][TX + 2] of each field: entry [y][x] holds the field at
 // cell (i0 + x - 1, j0 + y - 1) with the coordinates clamped (x wrapped when
 // periodic) as Nbr clamps a neighbour, so that the neighbour (di, dj) of the
-// tile's cell (tx, ty) is entry [ty + 1 + dj][tx + 1 + di].
-template <typename R, int TX, int TY>
+// tile's cell (tx, ty) is entry [ty + 1 + dj][tx + 1 + di].  HT1: the grid's
+// 1/ht meets quot1's condition (rht_one), quotients by ht are formed from it.
+template <typename R, int TX, int TY, bool HT1>
 struct TileStencil {
   const R (*Fx_)[TX + 2], (*Fy_)[TX + 2], (*u_)[TX + 2], (*v_)[TX + 2], (*sT_)[TX + 2];
   int tx, ty;
@@ -255,6 +276,10 @@ struct TileStencil {
   __device__ __forceinline__ R u(int di, int dj) const { return u_[ty + 1 + dj][tx + 1 + di]; }
   __device__ __forceinline__ R v(int di, int dj) const { return v_[ty + 1 + dj][tx + 1 + di]; }
   __device__ __forceinline__ R sT(int di, int dj) const { return sT_[ty + 1 + dj][tx + 1 + di]; }
+  __device__ __forceinline__ R div_ht(R a, const KGridT<R> &g) const {
+    if constexpr (HT1) return quot1(a, g.ht, g.rht);
+    else return a / g.ht;
+  }
 };
 
 template <typename R, typename St>
@@ -263,23 +288,23 @@ __device__ __forceinline__ void cell_assemble_u(const KGridT<R> &g, const KField
   const R *hx = g.hx, *hy = g.hy;
   const R *rho_pr = FLD(RHO_PR), *u_pr = FLD(U_PR), *rho = FLD(RHO), *rho_u = FLD(RHO_U);
   const unsigned i = n.i, j = n.j, ij = n.ij, i_1j = n.i_1j, i_1 = n.i_1;
-  const R cT = g.cT, ht = g.ht;
+  const R cT = g.cT;
   R F_xu1 = R(0.5) * (S.Fx(-1, 0) + S.Fx(0, 0));
   R F_xu2 = R(0.5) * (S.Fx(0, 0) + S.Fx(1, 0));
   R Scu = cT * (R((nb & NB_E) != 0) * S.u(1, 0) * S.sT(0, 0) * hy[j] / (hx[i] * R(3.0))
                      + R((nb & NB_WW) != 0) * S.u(-1, 0) * S.sT(-1, 0) * hy[j] / (hx[i_1] * R(3.0))
                      + df.Gy1 * (S.v(0, 1) * R((nb & NB_N) != 0) - S.v(-1, 1) * R((nb & NB_NW) != 0))
                      - df.Gy0 * (S.v(0, 0) * R((nb & NB_S) != 0) - S.v(-1, 0) * R((nb & NB_SW) != 0))
-                     - S.sT(0, 0) * (S.v(0, 1) * R((nb & NB_N) != 0) - S.v(0, 0) * R((nb & NB_S) != 0)) * R(2.0) / R(3.0)
-                     + S.sT(-1, 0) * (S.v(-1, 1) * R((nb & NB_NW) != 0) - S.v(-1, 0) * R((nb & NB_SW) != 0)) * R(2.0) / R(3.0));
-  R Spu = -(df.Dux0 + df.Dux1) / R(3.0);
+                     - third(S.sT(0, 0) * (S.v(0, 1) * R((nb & NB_N) != 0) - S.v(0, 0) * R((nb & NB_S) != 0)) * R(2.0))
+                     + third(S.sT(-1, 0) * (S.v(-1, 1) * R((nb & NB_NW) != 0) - S.v(-1, 0) * R((nb & NB_SW) != 0)) * R(2.0)));
+  R Spu = third(-(df.Dux0 + df.Dux1));
   R au1 = maxi0(F_xu1) + df.Dux0;
   R au2 = maxi0(-F_xu2) + df.Dux1;
   R au3 = R(0.5) * (maxi0(S.Fy(-1, 0)) + maxi0(S.Fy(0, 0))) + df.Duy0;
   R au4 = R(0.5) * (maxi0(-S.Fy(-1, 1)) + maxi0(-S.Fy(0, 1))) + df.Duy1;
-  R bu3 = (rho_pr[i_1j] * hx[i_1] + rho_pr[ij] * hx[i]) * u_pr[ij] * hy[j] * R(0.5) / ht + Scu;
+  R bu3 = S.div_ht((rho_pr[i_1j] * hx[i_1] + rho_pr[ij] * hx[i]) * u_pr[ij] * hy[j] * R(0.5), g) + Scu;
   R au0 = au1 + au2 + au3 + au4 + R(0.5) * (-S.Fy(-1, 0) - S.Fy(0, 0) + S.Fy(-1, 1) + S.Fy(0, 1)) + F_xu2 - F_xu1 - Spu
-               + (rho[i_1j] * hx[i_1] + rho[ij] * hx[i]) * hy[j] * R(0.5) / ht;
+               + S.div_ht((rho[i_1j] * hx[i_1] + rho[ij] * hx[i]) * hy[j] * R(0.5), g);
   R ups = (au1 * R((nb & NB_E) != 0) * S.u(-1, 0) + au2 * R((nb & NB_WW) != 0) * S.u(1, 0) + au3 * S.u(0, -1) + au4 * S.u(0, 1) + bu3) / au0;
   R dpc = g.cPch * hy[j] / au0;
   R h = hy[j] * rho_u[ij];
@@ -295,7 +320,7 @@ __device__ __forceinline__ void cell_assemble_v(const KGridT<R> &g, const KField
   const R *hx = g.hx, *hy = g.hy;
   const R *rho_pr = FLD(RHO_PR), *v_pr = FLD(V_PR), *rho = FLD(RHO), *rho_v = FLD(RHO_V);
   const unsigned i = n.i, j = n.j, ij = n.ij, ij_1 = n.ij_1;
-  const R cT = g.cT, ht = g.ht;
+  const R cT = g.cT;
   // S:3399 the mask at ij_1 - Nx*(0 <= (int)(j-2)) (NB_SS); S:3400 hy[j - 1*(0 <= (int)(j-1))]
   const R hyjm = hy[j - 1 * (0 <= (int)(j - 1))];
   R F_yv3 = R(0.5) * (S.Fy(0, -1) + S.Fy(0, 0));
@@ -304,17 +329,17 @@ __device__ __forceinline__ void cell_assemble_v(const KGridT<R> &g, const KField
                      + R((nb & NB_SS) != 0) * S.v(0, -1) * S.sT(0, -1) * hx[i] / (hyjm * R(3.0))
                      + df.Gx1 * (S.u(1, 0) * R((nb & NB_E) != 0) - S.u(1, -1) * R((nb & NB_SE) != 0))
                      - df.Gx0 * (S.u(0, 0) * R((nb & NB_W) != 0) - S.u(0, -1) * R((nb & NB_SW) != 0))
-                     - S.sT(0, 0) * (S.u(1, 0) * R((nb & NB_E) != 0) - S.u(0, 0) * R((nb & NB_W) != 0)) * R(2.0) / R(3.0)
-                     + S.sT(0, -1) * (S.u(1, -1) * R((nb & NB_SE) != 0) - S.u(0, -1) * R((nb & NB_SW) != 0)) * R(2.0) / R(3.0))
+                     - third(S.sT(0, 0) * (S.u(1, 0) * R((nb & NB_E) != 0) - S.u(0, 0) * R((nb & NB_W) != 0)) * R(2.0))
+                     + third(S.sT(0, -1) * (S.u(1, -1) * R((nb & NB_SE) != 0) - S.u(0, -1) * R((nb & NB_SW) != 0)) * R(2.0)))
                - g.c_Fr_v * (rho[ij_1] * hx[i] * R(0.5) * hyjm + rho[ij] * hx[i] * R(0.5) * hy[j]);
-  R Spv = -(df.Dvy0 + df.Dvy1) / R(3.0);
+  R Spv = third(-(df.Dvy0 + df.Dvy1));
   R av1 = R(0.5) * (maxi0(S.Fx(0, 0)) + maxi0(S.Fx(0, -1))) + df.Dvx0;
   R av2 = R(0.5) * (maxi0(-S.Fx(1, 0)) + maxi0(-S.Fx(1, -1))) + df.Dvx1;
   R av3 = maxi0(F_yv3) + df.Dvy0;
   R av4 = maxi0(-F_yv4) + df.Dvy1;
-  R bv3 = (rho_pr[ij_1] * hy[j - 1] + rho_pr[ij] * hy[j]) * v_pr[ij] * hx[i] * R(0.5) / ht + Scv;
+  R bv3 = S.div_ht((rho_pr[ij_1] * hy[j - 1] + rho_pr[ij] * hy[j]) * v_pr[ij] * hx[i] * R(0.5), g) + Scv;
   R av0 = av1 + av2 + av3 + av4 + R(0.5) * (S.Fx(1, 0) - S.Fx(0, 0) + S.Fx(1, -1) - S.Fx(0, -1)) + F_yv4 - F_yv3 - Spv
-               + (rho[ij_1] * hy[j - 1] + rho[ij] * hy[j]) * hx[i] * R(0.5) / ht;
+               + S.div_ht((rho[ij_1] * hy[j - 1] + rho[ij] * hy[j]) * hx[i] * R(0.5), g);
   R vps = (av1 * S.v(-1, 0) + av2 * S.v(1, 0) + av3 * R((nb & NB_SS) != 0) * S.v(0, -1) + av4 * R((nb & NB_N) != 0) * S.v(0, 1) + bv3) / av0;
   R dpc = g.cPch * hx[i] / av0;
   R h = hx[i] * rho_v[ij];
@@ -346,7 +371,7 @@ __device__ __forceinline__ void cell_energy_source(const KGridT<R> &g, const KFi
   R dsum = dudx_T + dvdy_T;
   FLD(SCT1)[ij] = g.ht * S.sT(0, 0) * g.cT2 * hx[i] * hy[j]
                       * (R(2.0) * (dudx_T * dudx_T + dvdy_T * dvdy_T) + s0 * s0 + s1 * s1 + s2 * s2 + s3 * s3
-                         - dsum * dsum * R(2.0) / R(3.0))
+                         - third(dsum * dsum * R(2.0)))
                   + p_pr[ij] * hx[i] * hy[j];
 }
 
@@ -439,7 +464,7 @@ __device__ __forceinline__ FaceDiff<R> face_diff_tile(const KGridT<R> &g, const 
 // wrapped neighbour indices give them), kept in LDS; then every cell
 // assembles from them.  (tx, ty): the thread's cell in the tile; the
 // workgroup has TX * TY >= TX + TY + 4 threads.
-template <typename R, int TX, int TY>
+template <typename R, int TX, int TY, bool HT1 = false>
 __device__ __forceinline__ void assemble_inl_tile(const KGridT<R> &g, const KFieldsT<R> &fl, int i0, int j0, int tx,
                                                   int ty) {
   static_assert(TX * TY >= TX + TY + 4, "the face ring and the mesh steps need TX + TY + 4 threads");
@@ -505,7 +530,7 @@ __device__ __forceinline__ void assemble_inl_tile(const KGridT<R> &g, const KFie
   const bool c = (nb & NB_C) != 0;
   const bool du = in_box(i, j, g.Nx_u0, g.Nx_u1, g.Ny_u0, g.Ny_u1) && (nb & NB_W) && c;
   const bool dv = in_box(i, j, g.Nx_v0, g.Nx_v1, g.Ny_v0, g.Ny_v1) && (nb & NB_S) && c;
-  const TileStencil<R, TX, TY> S{SFx, SFy, Su, Sv, SsT, tx, ty};
+  const TileStencil<R, TX, TY, HT1> S{SFx, SFy, Su, Sv, SsT, tx, ty};
   if (du || dv) {
     const int ex = (i == il) ? TX : tx + 1, ny = (j == jl) ? TY : ty + 1;
     const CellDiff<R> df{Lgy[ty][tx], Lgy[ny][tx], Lgx[ty][tx], Lgx[ty][ex], Lux[ty][tx], Lux[ty][ex],
@@ -519,12 +544,13 @@ __device__ __forceinline__ void assemble_inl_tile(const KGridT<R> &g, const KFie
 // Tile of the loops' assembly.
 constexpr int AX = 64, AY = 4;
 
-template <typename R>
+// HT1: launched for grids with rht_one.
+template <typename R, bool HT1>
 __global__ void __launch_bounds__(AX *AY) k_assemble_inl(KGridT<R> g, KFieldsT<R> fl, Gate gate) {
   if (gated_out(gate)) return;
   int i0, j0;
   tile_origin<AX, AY>(g.Nx, i0, j0);
-  assemble_inl_tile<R, AX, AY>(g, fl, i0, j0, (int)threadIdx.x, (int)threadIdx.y);
+  assemble_inl_tile<R, AX, AY, HT1>(g, fl, i0, j0, (int)threadIdx.x, (int)threadIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -2360,7 +2386,10 @@ static inline dim3 grid_tiles(const KGridT<R> &g) {
 
 template <typename R>
 hipError_t launch_assemble(const KGridT<R> &g, const KFieldsT<R> &f, hipStream_t s, Gate gate, bool inline_diff) {
-  if (inline_diff) hipLaunchKernelGGL(k_assemble_inl<R>, (grid_tiles<AX, AY>(g)), dim3(AX, AY), 0, s, g, f, gate);
+  if (inline_diff && g.rht_one)
+    hipLaunchKernelGGL((k_assemble_inl<R, true>), (grid_tiles<AX, AY>(g)), dim3(AX, AY), 0, s, g, f, gate);
+  else if (inline_diff)
+    hipLaunchKernelGGL((k_assemble_inl<R, false>), (grid_tiles<AX, AY>(g)), dim3(AX, AY), 0, s, g, f, gate);
   else hipLaunchKernelGGL(k_assemble<R>, (grid_tiles<BX, BY>(g)), dim3(BX, BY), 0, s, g, f, gate);
   return hipGetLastError();
 }
