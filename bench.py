@@ -102,11 +102,12 @@ TOL32 = (1e-7, 1e-7, 1e-7)  # FP32-phase tolerances
 DEGREE = 2        # predicted start: polynomial degree in time
 GUARD = 64        # predicted start: restart from x^n past the budget, pauses up to this many steps
 N_I32_MIXED = 6   # mixed: FP32 outer iterations (accelerated) before the FP64 phase
+MIX_FP32_ABOVE = 14  # mixed: the FP32 phase runs after steps of more than this many outer iterations
 AA_DEPTH = 1      # accelerated: Anderson history depth (with the predicted start)
 AA_DEPTH_FULL = 2  # Anderson depth without a predicted start and in the batch (from-rest transients)
 AA_DEGREE = 3     # accelerated: predicted-start degree
 _ACC = dict(extrapolate=AA_DEGREE, guard=GUARD, anderson=AA_DEPTH)
-_MIX = dict(_ACC, n32=N_I32_MIXED, anderson32=AA_DEPTH)
+_MIX = dict(_ACC, n32=N_I32_MIXED, anderson32=AA_DEPTH, fp32_above=MIX_FP32_ABOVE)
 PATHS = {  # sts_time_step_mixed settings of each path (n32: FP32 cap; stall: FP32 stagnation window, ratio)
     "plain": None,  # sts_time_step through the same binding as the other paths
     "predicted": dict(extrapolate=DEGREE, guard=GUARD),
@@ -472,8 +473,9 @@ def main():
             "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "f32+f64", "data": "synthetic: confined-square channel in the reference's input format",
             "config": {"workload": f"supersonic confined square, {nx}x{ny}, dt {0.01 * cfl * 240 / nx:g} H/u_in; mixed: "
-                                   f"predicted start, {N_I32_MIXED} accelerated FP32 outer iterations, then the "
-                                   f"accelerated FP64 phase on the reference's test; errors against the plain FP64 "
+                                   f"predicted start, {N_I32_MIXED} accelerated FP32 outer iterations after steps "
+                                   f"of more than {MIX_FP32_ABOVE} outer iterations, then the accelerated FP64 "
+                                   f"phase on the reference's test; errors against the plain FP64 "
                                    f"path on the same grid after warmup + steps steps",
                        "Nx": nx, "Ny": ny},
             "paths": paths}))
@@ -547,8 +549,9 @@ def main():
                                 f"(history depth {AA_DEPTH}, FP32 differences), every step ending on the "
                                 f"reference's test",
                  "mixed": f"start predicted by degree-{AA_DEGREE} extrapolation in time, guarded, "
-                          f"{N_I32_MIXED} FP32 outer iterations with Anderson acceleration, then FP64 outer "
-                          f"iterations with Anderson acceleration whose FP64 test ends the step",
+                          f"{N_I32_MIXED} FP32 outer iterations with Anderson acceleration after steps of more "
+                          f"than {MIX_FP32_ABOVE} outer iterations, then FP64 outer iterations with Anderson "
+                          f"acceleration whose FP64 test ends the step",
                  "surrogate": "the mixed path with the trained corrector network refining each predicted start"}
     line = {
         "metric": METRIC, "value": best, "unit": "Mcell·steps/s", "n_gpus": 1, "steps": t["timed_steps"],
