@@ -251,6 +251,7 @@ struct Driver {
     const size_t nc = c.circles.size(), nb = nc + c.polys.size();
     const unsigned type = (unsigned)c.get("TypeOfNonDimensiolization");
     const bool restart = c.flag("ToReadSolvedDataFromFile");
+    double CD_fr_x_last = 0, CD_p_x_last = 0;  // the last body written (S:8149-8150, S:8337-8338)
     for (size_t k = 0; k < nb; k++) {
       if (!(k < nc ? c.circles[k].drag : c.polys[k - nc].drag)) continue;
       double s[12];
@@ -273,6 +274,8 @@ struct Driver {
       const double S_bottom = s[2], S_front = s[5], S_top = s[8], S_behind = s[11];
       const double CD_fr_x = (fr_top / S_top) + (fr_bottom / S_bottom);
       const double CD_p_x = (p_front / S_front) - (p_behind / S_behind);
+      CD_fr_x_last = CD_fr_x;
+      CD_p_x_last = CD_p_x;
       const std::string name = path("CD." + std::to_string(k) + ".txt");
       std::ofstream o;
       if (cd_first && !restart) o.open(name);
@@ -299,6 +302,65 @@ struct Driver {
       o << std::endl;
       cd_first = false;
     }
+    if ((int)c.get("WriteDataInShortFormat_kind") == 1) write_short(CD_fr_x_last, CD_p_x_last);
+    if (c.flag("Pressure_BC")) write_p_u_xb();
+  }
+
+  // DataInShortFormat.txt, S:8438-8500: Re, Kn, CD, CD_p, u_gas_nd, the mass
+  // flows through column 0 and column Nx_u[1] (sum over j of rho u hy on fluid
+  // cells), p_BC_xb, p_BC_xe; rewritten at every drag-coefficient write.  A
+  // node past the arrays (Nx_u[1] = Nx in a periodic case, last row) counts
+  // as body.
+  void write_short(double CD_fr_x, double CD_p_x) {
+    std::vector<double> rho(Na), u(Na);
+    ck(sts_get_field(ctx, STS_RHO, rho.data()), "sts_get_field");
+    ck(sts_get_field(ctx, STS_U, u.data()), "sts_get_field");
+    auto flow = [&](unsigned i) {
+      double q = 0;
+      for (int j = 0; j < Ny; j++) {
+        const size_t node = (size_t)i + (size_t)j * (size_t)Nx;
+        if (node < Na && mask[node]) q += rho[node] * u[node] * c.hy[j];
+      }
+      return q;
+    };
+    const double q_xb = flow(0), q_xe = flow((unsigned)c.Nx_u[1]);
+    std::ofstream o(path("DataInShortFormat.txt"));
+    if (!o) {
+      std::cout << "The program can not open file DataInShortFormat.txt to write data." << std::endl;
+      return;
+    }
+    const int w = 30;
+    o << std::setiosflags(std::ios::scientific) << std::setprecision(15);
+    for (double x : {c.get("Re"), c.get("Kn"), CD_p_x + CD_fr_x, CD_p_x, c.get("u_gas_nd"), q_xb, q_xe, p_BC_xb,
+                     c.get("p_BC_xe")})
+      o << std::setw(w) << x;
+    o << std::endl;
+  }
+
+  // p_and_u_xb_it.txt, S:8504-8565: the inflow pressure and velocities at every
+  // drag-coefficient write; the first write of a run (not a restart) truncates
+  // the file and writes the header.
+  bool pu_first = true;
+  void write_p_u_xb() {
+    const bool restart = c.flag("ToReadSolvedDataFromFile");
+    std::ofstream o;
+    if (pu_first && !restart) o.open(path("p_and_u_xb_it.txt"));
+    else o.open(path("p_and_u_xb_it.txt"), std::ios::out | std::ios::app);
+    if (!o) {
+      std::cout << "The program can not open file p_and_u_xb_it.txt to write data." << std::endl;
+      return;
+    }
+    const int w = 30;
+    if (!restart && pu_first)
+      for (const char *h : {"//it (number of step by time)", "ht (step by time)", "p_BC_xb", "u_given_xb_in_it",
+                            "u_given_xb_in_it_max", "u_given_xb_in_it_mean", "p_correction"})
+        o << std::setw(w) << h;
+    if (!restart && pu_first) o << std::endl;
+    o << std::setw(w) << it;
+    o << std::setiosflags(std::ios::scientific) << std::setprecision(15);
+    for (double x : {ht, p_BC_xb, u_given_xb_in_it, u_xb_max, u_xb_mean, c.get("p_correction")}) o << std::setw(w) << x;
+    o << std::endl;
+    pu_first = false;
   }
 
   // WriteIterationsInLoopsToFile, S:5945-6006
@@ -337,8 +399,27 @@ struct Driver {
 
   // BoundaryConditions_Pressure_begin_iteration_for_it, S:7430-7647, for
   // p_BC_xb_correction_method == 0 and no pressure-ratio correction.
+  // The largest u and the hy-weighted mean u over the fluid cells of column
+  // Nx_u[0] (S:7452-7486) are kept for p_and_u_xb_it.txt, so they are formed
+  // only when that file is written (Pressure_BC).
+  double u_xb_max = 0, u_xb_mean = 0;
   void pressure_begin(double probe) {
     if (c.flag("Periodic_boundary_conditions_about_OX")) return;  // S:7436
+    if (c.flag("Pressure_BC")) {
+      std::vector<double> u(Na);
+      ck(sts_get_field(ctx, STS_U, u.data()), "sts_get_field");
+      const unsigned i = (unsigned)c.Nx_u[0];
+      u_xb_max = u[0];
+      double mean = 0, h = 0;
+      for (int j = 0; j < Ny; j++) {
+        const size_t node = (size_t)i + (size_t)j * (size_t)Nx;
+        if (!mask[node]) continue;
+        u_xb_max = (u_xb_max > u[node]) ? u_xb_max : u[node];
+        mean += u[node] * c.hy[j];
+        h += c.hy[j];
+      }
+      u_xb_mean = mean / h;
+    }
     if (c.flag("GivenReKn")) c.at("u_given_xb") = c.get("Re") * c.get("Kn") * sqrt(15.0 * M_PI / 128.0);
     u_given_xb_in_it = c.get("u_given_xb");
     if (c.flag("Given_velocity_on_xb")) p_BC_xb = probe;

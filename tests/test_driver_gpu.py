@@ -140,3 +140,33 @@ def test_driver_restart_matches_live_reference(tmp_path):
     run_driver(mine)
     bad = compare_dirs(mine, ref)
     assert not bad, f"files differing from the reference's after the restart: {bad}"
+
+
+@pytest.mark.skipif(not have_reference(), reason="oracle/_ref/sts_ref not built")
+@pytest.mark.parametrize("name", ["shipped_coarse", "square_cd"])
+def test_short_format_and_inflow_files_match_live_reference(name, tmp_path):
+    """DataInShortFormat.txt (Re, Kn, CD, CD_p, u_gas_nd, the mass flows at
+    the inlet and at column Nx_u[1], the box-end pressures; S:8438-8500) and
+    p_and_u_xb_it.txt (inflow pressure and velocities when Pressure_BC is
+    set; S:8504-8565), written at every drag-coefficient write: the
+    pressure-driven microchannel (Pressure_BC, no drag bodies) and the
+    confined square with drag bodies, 3 steps, byte for byte against the
+    reference run in the same job; then 2 more steps continued from the saved
+    files (p_and_u_xb_it.txt appended to)."""
+    mine, ref = fresh_input(name, tmp_path / "gpu"), fresh_input(name, tmp_path / "ref")
+    for d in (mine, ref):
+        _set_entries(d, WriteDataInShortFormat_kind=1, Nt_save_DragCoefficient=1, Nt=3, Nt_save_solved_data=3)
+    run_reference(ref)
+    run_driver(mine)
+    files = ["DataInShortFormat.txt"] + (["p_and_u_xb_it.txt"] if name == "shipped_coarse" else [])
+    for f in files:
+        assert os.path.exists(os.path.join(ref, f)), f
+        assert open(os.path.join(mine, f), "rb").read() == open(os.path.join(ref, f), "rb").read(), f
+    assert not compare_dirs(mine, ref)
+    for d in (mine, ref):
+        _set_entries(d, ToReadSolvedDataFromFile=1, Nt=2, max_time_for_solve=1000.0)
+    run_reference(ref)
+    run_driver(mine)
+    for f in files:
+        assert open(os.path.join(mine, f), "rb").read() == open(os.path.join(ref, f), "rb").read(), f + " (restart)"
+    assert not compare_dirs(mine, ref)
