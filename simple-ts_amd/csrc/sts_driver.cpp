@@ -397,15 +397,17 @@ struct Driver {
     ck(sts_set_scalars(ctx, cT, cPch), "sts_set_scalars");
   }
 
-  // BoundaryConditions_Pressure_begin_iteration_for_it, S:7430-7647, for
-  // p_BC_xb_correction_method == 0 and no pressure-ratio correction.
-  // The largest u and the hy-weighted mean u over the fluid cells of column
-  // Nx_u[0] (S:7452-7486) are kept for p_and_u_xb_it.txt, so they are formed
-  // only when that file is written (Pressure_BC).
+  // BoundaryConditions_Pressure_begin_iteration_for_it, S:7430-7647, without
+  // the pressure-ratio correction.  The largest u and the hy-weighted mean u
+  // over the fluid cells of column Nx_u[0] (S:7452-7486) are formed when
+  // something reads them: p_and_u_xb_it.txt (Pressure_BC), or
+  // p_BC_xb_correction_method 1 / 2, which take them as the inflow velocity
+  // of the step's bookkeeping (cT of types 1 and 2, the drag coefficients).
   double u_xb_max = 0, u_xb_mean = 0;
   void pressure_begin(double probe) {
     if (c.flag("Periodic_boundary_conditions_about_OX")) return;  // S:7436
-    if (c.flag("Pressure_BC")) {
+    const unsigned method = (unsigned)c.get("p_BC_xb_correction_method");
+    if (c.flag("Pressure_BC") || method != 0) {
       std::vector<double> u(Na);
       ck(sts_get_field(ctx, STS_U, u.data()), "sts_get_field");
       const unsigned i = (unsigned)c.Nx_u[0];
@@ -421,8 +423,14 @@ struct Driver {
       u_xb_mean = mean / h;
     }
     if (c.flag("GivenReKn")) c.at("u_given_xb") = c.get("Re") * c.get("Kn") * sqrt(15.0 * M_PI / 128.0);
-    u_given_xb_in_it = c.get("u_given_xb");
-    if (c.flag("Given_velocity_on_xb")) p_BC_xb = probe;
+    if (method == 0) {
+      u_given_xb_in_it = c.get("u_given_xb");
+      if (c.flag("Given_velocity_on_xb")) p_BC_xb = probe;
+    } else if (method == 1) {
+      u_given_xb_in_it = u_xb_max;
+    } else if (method == 2) {
+      u_given_xb_in_it = u_xb_mean;
+    }
   }
 
   void run() {
@@ -435,7 +443,6 @@ struct Driver {
     for (const char *f : {"ToContinueFromInterpolation", "ToStartFromExactSolutionForChannelFlowWithPressureBC",
                           "Pressure_ratio_correct"})
       if (c.flag(f)) die(std::string(f) + " = 1 is not provided");
-    if ((unsigned)c.get("p_BC_xb_correction_method") != 0) die("p_BC_xb_correction_method != 0 is not provided");
     ht = c.get("ht");
     N_I = (unsigned)c.get("N_I");
     N_I_p_c = (unsigned)c.get("N_I_p_c");

@@ -170,3 +170,24 @@ def test_short_format_and_inflow_files_match_live_reference(name, tmp_path):
     for f in files:
         assert open(os.path.join(mine, f), "rb").read() == open(os.path.join(ref, f), "rb").read(), f + " (restart)"
     assert not compare_dirs(mine, ref)
+
+
+@pytest.mark.skipif(not have_reference(), reason="oracle/_ref/sts_ref not built")
+@pytest.mark.parametrize("method", [1, 2])
+@pytest.mark.parametrize("nd_type", [3, 1])
+def test_inflow_velocity_correction_methods_match_live_reference(method, nd_type, tmp_path):
+    """p_BC_xb_correction_method 1 / 2 (S:7496-7521): the step's inflow
+    velocity is the largest / the hy-weighted mean u of the inflow column,
+    which enters the inflow trace and, with the nondimensionalisation of type
+    1, cT of every outer iteration (S:4347).  The pressure-driven
+    microchannel, 3 steps, every file byte for byte against the reference."""
+    mine, ref = fresh_input("shipped_coarse", tmp_path / "gpu"), fresh_input("shipped_coarse", tmp_path / "ref")
+    for d in (mine, ref):
+        _set_entries(d, p_BC_xb_correction_method=method, TypeOfNonDimensiolization=nd_type, Nt=3,
+                     Nt_save_DragCoefficient=1)
+    run_reference(ref)
+    run_driver(mine)
+    f = "p_and_u_xb_it.txt"
+    assert open(os.path.join(mine, f), "rb").read() == open(os.path.join(ref, f), "rb").read()
+    bad = compare_dirs(mine, ref)
+    assert not bad, f"files differing from the reference's: {bad}"
