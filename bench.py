@@ -428,9 +428,10 @@ def main():
     ap.add_argument("--fp64-only", action="store_true", help="run only the FP64 program")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes")
     ap.add_argument("--write-case", metavar="DIR", help="only write the workload case (warmup + steps steps) to DIR")
-    ap.add_argument("--workload", default="square", choices=["square", "batch", "fine"],
+    ap.add_argument("--workload", default="square", choices=["square", "batch", "fine", "unet"],
                     help="square: the 1920x384 case (default); batch: uneven cases in shared launches; fine: "
-                         "the same case on 3840x768, mixed and accelerated paths against FP64 on that grid")
+                         "the same case on 3840x768, mixed and accelerated paths against FP64 on that grid; "
+                         "unet: inference of the corrector network at 1920x384")
     ap.add_argument("--cases", type=int, default=64)
     args = ap.parse_args()
     if args.write_case:
@@ -479,6 +480,29 @@ def main():
                                    f"path on the same grid after warmup + steps steps",
                        "Nx": nx, "Ny": ny},
             "paths": paths}))
+        return
+    if args.workload == "unet":
+        # the corrector's network (9 inputs, 4 outputs, base 32) at the benchmark grid; the
+        # time includes the host transfers of sts_unet_infer (a profiler's kernel trace
+        # gives the device part)
+        pkg = importlib.import_module("simple-ts_amd")
+        ws = importlib.import_module("simple-ts_amd.warmstart")
+        net = pkg.UNet(ws.CORR_CIN, ws.CORR_COUT, 32, 1, NY, NX)
+        net.set_params(ws.he_init(ws.CORR_CIN, ws.CORR_COUT, 32, 0))
+        x = np.random.default_rng(0).standard_normal((1, NY, NX, ws.CORR_CIN)).astype(np.float32)
+        for _ in range(args.warmup):
+            net.infer(x)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            y = net.infer(x)
+        dt = (time.perf_counter() - t0) / args.steps
+        net.close()
+        print(json.dumps({
+            "metric": "U-Net inference at 1920x384 (host transfers included)", "value": 1e3 * dt, "unit": "ms",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": False,
+            "dtype": "bf16", "data": "synthetic input, He-initialised weights",
+            "config": {"workload": "corrector U-Net, 9 inputs, 4 outputs, base 32, depth 4", "Nx": NX, "Ny": NY},
+            "finite": bool(np.all(np.isfinite(y)))}))
         return
     if args.workload == "batch":
         out = {"fp64": run_batch_workload(args.cases, args.steps, args.warmup, None),
