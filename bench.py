@@ -106,6 +106,7 @@ MIX_FP32_ABOVE = 14  # mixed: the FP32 phase runs after steps of more than this 
 AA_DEPTH = 1      # accelerated: Anderson history depth (with the predicted start)
 AA_DEPTH_FULL = 2  # Anderson depth without a predicted start and in the batch (from-rest transients)
 AA_DEGREE = 3     # accelerated: predicted-start degree
+CORR_PAUSE = 64   # corrector back-off: at most this many predicted steps sat out after a rejection
 _ACC = dict(extrapolate=AA_DEGREE, guard=GUARD, anderson=AA_DEPTH)
 _MIX = dict(_ACC, n32=N_I32_MIXED, anderson32=AA_DEPTH, fp32_above=MIX_FP32_ABOVE)
 PATHS = {  # sts_time_step_mixed settings of each path (n32: FP32 cap; stall: FP32 stagnation window, ratio)
@@ -115,8 +116,8 @@ PATHS = {  # sts_time_step_mixed settings of each path (n32: FP32 cap; stall: FP
     "accelerated": _ACC,
     "two_phase": dict(n32=5000, stall=(6, 0.95)),
     "mixed": _MIX,
-    "surrogate": dict(_MIX, net=True),
-    "surrogate_accelerated": dict(_ACC, net=True),
+    "surrogate": dict(_MIX, net=True, corr_pause=CORR_PAUSE),
+    "surrogate_accelerated": dict(_ACC, net=True, corr_pause=CORR_PAUSE),
 }
 HEADLINE = "accelerated"
 
@@ -158,6 +159,7 @@ def run_python_path(steps, warmup, name, net=None, nx=NX, ny=NY, cfl=1.0):
     n_i32 = spec.pop("n32", 0)
     stall_w, stall_r = spec.pop("stall", (0, 0.95))
     spec.pop("net", None)
+    pause = spec.pop("corr_pause", 0)
     ws = importlib.import_module("simple-ts_amd.warmstart")
     work = tempfile.mkdtemp(prefix="sts_bench_")
     try:
@@ -165,6 +167,7 @@ def run_python_path(steps, warmup, name, net=None, nx=NX, ny=NY, cfl=1.0):
                      precision=32 if n_i32 else 64)
         if net is not None:
             run.ctx.set_corrector(net, ws.corr_norm(run.case))
+            run.ctx.set_corrector_pause(pause)
         n32 = n64 = 0
         t0 = None
         print(f"[bench] {name} path: {warmup} + {steps} steps", file=sys.stderr, flush=True)
@@ -179,7 +182,7 @@ def run_python_path(steps, warmup, name, net=None, nx=NX, ny=NY, cfl=1.0):
                 n32, n64 = n32 + a, n64 + b
         dt = time.perf_counter() - t0
         stats = run.ctx.predict_stats()
-        corr = run.ctx.corrector_stats() if net is not None else None
+        corr = (*run.ctx.corrector_stats(), run.ctx.corrector_skipped()) if net is not None else None
         # the FP64 mass-imbalance check of the final state (S:10621-10646 over the fluid cells)
         mass = run.ctx.continuity()[2]
         run.ctx.jump_overwrite()  # as the program does before it writes the fields
@@ -549,7 +552,8 @@ def main():
                                         "abandoned_fp32_iterations": s[2], "abandoned_fp64_iterations": s[3],
                                         "steps_from_previous_state": s[4]}
             if net is not None:
-                paths[name]["corrector"] = {"kept": r["corr"][0], "rejected": r["corr"][1]}
+                paths[name]["corrector"] = {"kept": r["corr"][0], "rejected": r["corr"][1], "skipped": r["corr"][2],
+                                            "max_pause": CORR_PAUSE}
                 paths[name]["network"] = info
                 net.close()
             if name == HEADLINE:
@@ -576,7 +580,9 @@ def main():
                           f"{N_I32_MIXED} FP32 outer iterations with Anderson acceleration after steps of more "
                           f"than {MIX_FP32_ABOVE} outer iterations, then FP64 outer iterations with Anderson "
                           f"acceleration whose FP64 test ends the step",
-                 "surrogate": "the mixed path with the trained corrector network refining each predicted start"}
+                 "surrogate": "the mixed path with the trained corrector network refining the predicted starts "
+                              f"(after a rejected refinement it sits out 1, 2, 4, ... predicted steps, at most "
+                              f"{CORR_PAUSE})"}
     line = {
         "metric": METRIC, "value": best, "unit": "Mcell·steps/s", "n_gpus": 1, "steps": t["timed_steps"],
         "warmup": t["warmup_steps"], "ms_per_step": paths[best_name]["ms_per_step"], "higher_is_better": True,

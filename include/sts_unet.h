@@ -96,6 +96,15 @@ int sts_set_corrector(sts_ctx *ctx, sts_unet *net, const sts_unet_corr_norm *nrm
 /* out[0]: predicted steps the corrector refined and kept, out[1]: refinements
    rejected by the residual test. */
 int sts_corrector_stats(sts_ctx *ctx, long long out[2]);
+/* Back-off of the corrector (no counterpart in the reference): with
+   max_pause > 0, a rejected refinement makes the corrector sit out the next
+   1, 2, 4, ... predicted steps (at most max_pause; back to 1 after a kept
+   refinement), so a network that does not help costs little; those steps
+   run from the extrapolated start as without a corrector.  0 (the default
+   of sts_set_corrector): every predicted step is refined and tested. */
+int sts_set_corrector_pause(sts_ctx *ctx, int max_pause);
+/* Predicted steps the corrector sat out under sts_set_corrector_pause. */
+int sts_corrector_skipped(sts_ctx *ctx, long long *skipped);
 
 /* Time step with the network's warm start: prologue, prediction, choice of
    the starting p, T by the first sweep's residual (the prediction is used

@@ -202,6 +202,8 @@ def lib():
             ("sts_unet_record_correction", [vp, c_int, vp, ctypes.POINTER(CorrNorm)]),
             ("sts_set_corrector", [vp, vp, ctypes.POINTER(CorrNorm)]),
             ("sts_corrector_stats", [vp, ctypes.POINTER(ctypes.c_longlong)]),
+            ("sts_set_corrector_pause", [vp, c_int]),
+            ("sts_corrector_skipped", [vp, ctypes.POINTER(ctypes.c_longlong)]),
         ]:
             fn = getattr(L, name)
             fn.argtypes = args
@@ -379,6 +381,17 @@ class Context:
         out = (ctypes.c_longlong * 2)()
         _check(lib().sts_corrector_stats(self.h, out), "sts_corrector_stats")
         return tuple(out)
+
+    def set_corrector_pause(self, max_pause):
+        """After a rejected refinement the corrector sits out 1, 2, 4, ...
+        predicted steps, at most max_pause (0: every predicted step)."""
+        _check(lib().sts_set_corrector_pause(self.h, int(max_pause)), "sts_set_corrector_pause")
+
+    def corrector_skipped(self):
+        """Predicted steps the corrector sat out."""
+        out = ctypes.c_longlong()
+        _check(lib().sts_corrector_skipped(self.h, ctypes.byref(out)), "sts_corrector_skipped")
+        return out.value
 
     def predict_stats(self):
         """(predicted steps, restarted ones, FP32 and FP64 iterations of the

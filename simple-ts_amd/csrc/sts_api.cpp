@@ -96,7 +96,11 @@ struct sts_ctx {
   double *xs[4] = {};
   sts_unet *corr_net = nullptr;
   sts_unet_corr_norm corr_norm{};
-  long long cstat[2] = {0, 0};
+  long long cstat[3] = {0, 0, 0};  // kept, rejected, skipped (paused)
+  // pause after a rejected refinement (sts_set_corrector_pause): the next
+  // corr_pause predicted steps skip the corrector; corr_backoff doubles up to
+  // corr_max_pause on each further rejection and returns to 1 when one is kept
+  int corr_max_pause = 0, corr_pause = 0, corr_backoff = 1;
   // Gamma_* and D_*: whether forming them from sqrt_T, the wall temperatures
   // and the mask gives the values the reference holds (true after a flux and
   // diffusion pass of the library, until those inputs or the arrays are set
@@ -1331,6 +1335,11 @@ static int first_sweep_residual(sts_ctx *c, const sts_loop_control *lc, int q, d
 static int corrector_refine(sts_ctx *c, const sts_loop_control *lc) {
   int rc;
   double rA, rB;
+  if (c->corr_pause > 0) {
+    c->corr_pause--;
+    c->cstat[2]++;
+    return STS_OK;
+  }
   if ((rc = first_sweep_residual(c, lc, 2, &rA))) return rc;
   StsView v;
   sts_internal_view(c, &v);
@@ -1339,9 +1348,14 @@ static int corrector_refine(sts_ctx *c, const sts_loop_control *lc) {
   if ((rc = rebuild_derived(c, c->stream)) || (rc = first_sweep_residual(c, lc, 4, &rB))) return rc;
   if (rB < rA) {
     c->cstat[0]++;
+    c->corr_backoff = 1;
     return STS_OK;
   }
   c->cstat[1]++;
+  if (c->corr_max_pause > 0) {
+    c->corr_pause = c->corr_backoff;
+    c->corr_backoff = std::min(2 * c->corr_backoff, c->corr_max_pause);
+  }
   if ((rc = copy_start(c, c->stream, true))) return rc;
   return rebuild_derived(c, c->stream);
 }
@@ -1550,6 +1564,8 @@ extern "C" int sts_set_corrector(sts_ctx *c, sts_unet *net, const sts_unet_corr_
   if (net && !(nrm->u_ref > 0.0 && nrm->p_ref > 0.0 && nrm->step > 0.0 && nrm->corr > 0.0))
     return fail(STS_ERR_ARG, "corrector normalisation must be positive");
   c->corr_net = nullptr;
+  c->corr_pause = 0;
+  c->corr_backoff = 1;
   if (!net) return STS_OK;
   int rc = sts_keep_predicted_start(c, 1);
   if (rc) return rc;
@@ -1562,6 +1578,20 @@ extern "C" int sts_corrector_stats(sts_ctx *c, long long out[2]) {
   if (!c || !out) return fail(STS_ERR_ARG, "null argument");
   out[0] = c->cstat[0];
   out[1] = c->cstat[1];
+  return STS_OK;
+}
+
+extern "C" int sts_set_corrector_pause(sts_ctx *c, int max_pause) {
+  if (!c || max_pause < 0) return fail(STS_ERR_ARG, "bad corrector pause");
+  c->corr_max_pause = max_pause;
+  c->corr_pause = 0;
+  c->corr_backoff = 1;
+  return STS_OK;
+}
+
+extern "C" int sts_corrector_skipped(sts_ctx *c, long long *skipped) {
+  if (!c || !skipped) return fail(STS_ERR_ARG, "null argument");
+  *skipped = c->cstat[2];
   return STS_OK;
 }
 

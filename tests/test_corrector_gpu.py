@@ -33,26 +33,39 @@ def fields(ctx):
     return {k: ctx.get(k) for k in ("u", "v", "p", "Temper")}
 
 
+@pytest.mark.parametrize("pause", [0, 8])
 @pytest.mark.parametrize("kind", ["zero", "garbage"])
-def test_rejected_corrector_keeps_the_method(kind, tmp_path):
+def test_rejected_corrector_keeps_the_method(kind, pause, tmp_path):
+    """With a back-off (pause 8) the corrector sits out 1, 2, 4, 8, 8, ...
+    predicted steps after each rejection (it tries predicted steps 1, 3, 6,
+    11, 20, ...); the steps stay bit-identical either way."""
     pkg, ws = mods()
     net = pkg.UNet(ws.CORR_CIN, ws.CORR_COUT, 8, 1, NY, NX)
     p = np.zeros(net.n_params, np.float32)
     if kind == "garbage":
         p[-ws.CORR_COUT:] = 1e3  # head bias
     net.set_params(p)
-    steps = 8
+    steps = 12
     a = ws.Run(ws.write_case(str(tmp_path / "a"), NX, NY, PRM, steps))
     b = ws.Run(ws.write_case(str(tmp_path / "b"), NX, NY, PRM, steps))
     a.ctx.set_corrector(net, ws.corr_norm(a.case))
+    a.ctx.set_corrector_pause(pause)
     for s in range(steps):
         print(f"{kind} step {s}", flush=True)
         ia, _ = ws.accelerated_step(a)
         ib, _ = ws.accelerated_step(b)
         assert ia == ib
     kept, rejected = a.ctx.corrector_stats()
-    print(f"kept {kept}, rejected {rejected}", flush=True)
-    assert kept == 0 and rejected == a.ctx.predict_stats()[0] > 0
+    skipped = a.ctx.corrector_skipped()
+    predicted = a.ctx.predict_stats()[0]
+    print(f"kept {kept}, rejected {rejected}, skipped {skipped} of {predicted} predicted steps", flush=True)
+    assert kept == 0 and rejected + skipped == predicted > 0
+    tries, nxt, gap = 0, 1, 1
+    while nxt <= predicted:
+        tries += 1
+        nxt += gap + 1
+        gap = min(2 * gap, pause)
+    assert (rejected, skipped) == ((predicted, 0) if pause == 0 else (tries, predicted - tries))
     fa, fb = fields(a.ctx), fields(b.ctx)
     a.close()
     b.close()
