@@ -75,36 +75,70 @@ __device__ __forceinline__ unsigned long long wave_max(unsigned long long v) {
   return v;
 }
 
-// Block-wide max of two values, one atomicMax per block and destination, into
-// the block's slot copy of each destination; destinations may be null, and a
-// second pair (dst2, dst3) receives the same maxima (the first and the last
-// sweep of a single-sweep loop 3 coincide).
+// One step of wave_max_last: v and its DPP-moved copy (both 32-bit halves
+// moved alike); rows outside ROW_MASK keep v.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ unsigned long long dpp_max_step(unsigned long long v) {
+  const int lo = (int)(unsigned)v, hi = (int)(unsigned)(v >> 32);
+  const unsigned long long w =
+      ((unsigned long long)(unsigned)__builtin_amdgcn_update_dpp(hi, hi, CTRL, ROW_MASK, 0xf, false) << 32) |
+      (unsigned long long)(unsigned)__builtin_amdgcn_update_dpp(lo, lo, CTRL, ROW_MASK, 0xf, false);
+  return (w > v) ? w : v;
+}
+
+// Max over a full wave of a 64-bit word, complete in lane 63: within quads
+// (quad permutations), half rows and rows (mirrors), then row 1 and 3 take
+// the maxima of rows 0 and 2 (broadcast of lane 15) and rows 2 and 3 that of
+// rows 0-1 (broadcast of lane 31).  DPP moves are VALU operations, with no
+// LDS round trip per step.  Every lane of the wave must be active.
+__device__ __forceinline__ unsigned long long wave_max_last(unsigned long long v) {
+  v = dpp_max_step<0xb1, 0xf>(v);   // quad_perm(1, 0, 3, 2)
+  v = dpp_max_step<0x4e, 0xf>(v);   // quad_perm(2, 3, 0, 1)
+  v = dpp_max_step<0x141, 0xf>(v);  // row_half_mirror
+  v = dpp_max_step<0x140, 0xf>(v);  // row_mirror
+  v = dpp_max_step<0x142, 0xa>(v);  // row_bcast:15
+  v = dpp_max_step<0x143, 0xc>(v);  // row_bcast:31
+  return v;
+}
+
+// Destinations of block_max_atomic's NV maxima: value k goes to d[k] and,
+// when set, also to e[k] (the first and the last sweep of a single-sweep
+// loop 3 coincide); null destinations are skipped.
 template <int NV>
-__device__ __forceinline__ void block_max_atomic(unsigned long long (&v)[NV], unsigned long long *dst0,
-                                                 unsigned long long *dst1, unsigned long long *dst2 = nullptr,
-                                                 unsigned long long *dst3 = nullptr) {
-  __shared__ unsigned long long sm[NV][16];
+struct MaxDst {
+  unsigned long long *d[NV];
+  unsigned long long *e[NV];
+};
+
+// Block-wide max of NV values over a block of NT threads (whole waves), one
+// atomicMax per block and destination, into the block's slot copy of each
+// destination.  Every thread leaves its values in LDS, one barrier, then the
+// first wave alone folds the NT / 64 entries of each lane and reduces across
+// its lanes; the other waves are done at the barrier.  Called once per
+// kernel by every thread of the block (its LDS words are not reused).
+template <int NV, int NT>
+__device__ __forceinline__ void block_max_atomic(const unsigned long long (&v)[NV], const MaxDst<NV> &dst) {
+  static_assert(NT % 64 == 0, "whole waves");
+  __shared__ unsigned long long sm[NV][NT];
   const int tid = threadIdx.x + threadIdx.y * blockDim.x;
-  const int lane = tid & 63, wid = tid >> 6;
-  const int nw = (blockDim.x * blockDim.y + 63) >> 6;
-  __syncthreads();  // a previous call's reader may still be reading sm
+#pragma unroll
+  for (int k = 0; k < NV; k++) sm[k][tid] = v[k];
+  __syncthreads();
+  if (tid >= 64) return;
+  const unsigned slot = ((unsigned)blockIdx.x + (unsigned)blockIdx.y * gridDim.x) % (unsigned)NSLOT;
+  const unsigned off = slot * (unsigned)SLOT_STRIDE;
 #pragma unroll
   for (int k = 0; k < NV; k++) {
-    unsigned long long w = wave_max(v[k]);
-    if (lane == 0) sm[k][wid] = w;
-  }
-  __syncthreads();
-  if (tid == 0) {
-    const unsigned slot = ((unsigned)blockIdx.x + (unsigned)blockIdx.y * gridDim.x) % (unsigned)NSLOT;
-    const unsigned off = slot * (unsigned)SLOT_STRIDE;
+    unsigned long long b = sm[k][tid];
 #pragma unroll
-    for (int k = 0; k < NV; k++) {
-      unsigned long long b = 0;
-      for (int q = 0; q < nw; q++) b = (sm[k][q] > b) ? sm[k][q] : b;
-      unsigned long long *d = (k == 0) ? dst0 : dst1;
-      unsigned long long *e = (k == 0) ? dst2 : dst3;
-      if (b && d) atomicMax(d + off, b);
-      if (b && e) atomicMax(e + off, b);
+    for (int q = 1; q < NT / 64; q++) {
+      const unsigned long long w = sm[k][tid + 64 * q];
+      b = (w > b) ? w : b;
+    }
+    b = wave_max_last(b);
+    if (tid == 63 && b) {
+      if (dst.d[k]) atomicMax(dst.d[k] + off, b);
+      if (dst.e[k]) atomicMax(dst.e[k] + off, b);
     }
   }
 }
@@ -695,7 +729,7 @@ __global__ void __launch_bounds__(BX *BY) k_sweep(KGridT<R> g, KFieldsT<R> fl, c
       Tn[ij] = To[ij];
     }
   }
-  if (red_p || red_T || red_p2 || red_T2) block_max_atomic<2>(r, red_p, red_T, red_p2, red_T2);
+  if (red_p || red_T || red_p2 || red_T2) block_max_atomic<2, BX * BY>(r, MaxDst<2>{{red_p, red_T}, {red_p2, red_T2}});
 }
 
 // Two sweeps in one pass (temporal blocking): each block first sweeps its
@@ -705,12 +739,12 @@ __global__ void __launch_bounds__(BX *BY) k_sweep(KGridT<R> g, KFieldsT<R> fl, c
 // the tile cells go to (r1p, r1T) and (r2p, r2T).
 constexpr int S2X = 64, S2Y = 8;
 
+// r: the thread's residuals of sweep 1 (p, T) and sweep 2 (p, T).
 template <typename R>
 __device__ __forceinline__ void sweep2_tile_body(const KGridT<R> &g, const KFieldsT<R> &fl, const R *__restrict__ Po,
                                                  const R *__restrict__ To, R *__restrict__ Pn, R *__restrict__ Tn,
                                                  int i0, int j0, int tx, int ty, int SX, int SY, R *sP, R *sT,
-                                                 unsigned long long *r1p, unsigned long long *r1T,
-                                                 unsigned long long *r2p, unsigned long long *r2T) {
+                                                 unsigned long long (&r)[4]) {
   const int Nx = g.Nx, Ny = g.Ny;
   const int RX = SX + 2, RY = SY + 2;
   const int tid = tx + ty * SX;
@@ -792,8 +826,10 @@ __device__ __forceinline__ void sweep2_tile_body(const KGridT<R> &g, const KFiel
     Pn[ij] = sP[cy * RX + cx];
     Tn[ij] = sT[cy * RX + cx];
   }
-  if (r1p || r1T) block_max_atomic<2>(r1, r1p, r1T);
-  if (r2p || r2T) block_max_atomic<2>(r2, r2p, r2T);
+  r[0] = r1[0];
+  r[1] = r1[1];
+  r[2] = r2[0];
+  r[3] = r2[1];
 }
 
 // Tile (i0, j0) of SX x SY cells with its one-cell halo in LDS.
@@ -803,7 +839,9 @@ __device__ __forceinline__ void sweep2_tile(const KGridT<R> &g, const KFieldsT<R
                                             int j0, int tx, int ty, unsigned long long *r1p, unsigned long long *r1T,
                                             unsigned long long *r2p, unsigned long long *r2T) {
   __shared__ R sP[(SY + 2) * (SX + 2)], sT[(SY + 2) * (SX + 2)];
-  sweep2_tile_body(g, fl, Po, To, Pn, Tn, i0, j0, tx, ty, SX, SY, sP, sT, r1p, r1T, r2p, r2T);
+  unsigned long long r[4];
+  sweep2_tile_body(g, fl, Po, To, Pn, Tn, i0, j0, tx, ty, SX, SY, sP, sT, r);
+  if (r1p || r1T || r2p || r2T) block_max_atomic<4, SX * SY>(r, MaxDst<4>{{r1p, r1T, r2p, r2T}, {}});
 }
 
 template <typename R>
@@ -958,7 +996,7 @@ __global__ void __launch_bounds__(BX *BY) k_correct(KGridT<R> g, KFieldsT<R> fl,
     c[q].load(g, fl, i0 + (int)threadIdx.x, j0 + (int)threadIdx.y + q * BY, do_correct);
 #pragma unroll
   for (int q = 0; q < CORRECT_ROWS; q++) c[q].apply(fl, r);
-  if (do_correct) block_max_atomic<2>(r, red_u, red_v);
+  if (do_correct) block_max_atomic<2, BX * BY>(r, MaxDst<2>{{red_u, red_v}, {}});
 }
 
 // A7 alone (S:4323-4330)
@@ -2222,8 +2260,8 @@ __global__ void __launch_bounds__(TX *TY) kb_sweep(const BatchCase<R> *C, const 
       Tn[ij] = To[ij];
     }
   }
-  block_max_atomic<2>(r, last ? c.lred + 2 : nullptr, last ? c.lred + 3 : nullptr, first ? c.lred + 4 : nullptr,
-                      first ? c.lred + 5 : nullptr);
+  block_max_atomic<2, TX * TY>(r, MaxDst<2>{{last ? c.lred + 2 : nullptr, last ? c.lred + 3 : nullptr},
+                                         {first ? c.lred + 4 : nullptr, first ? c.lred + 5 : nullptr}});
 }
 
 template <typename R, int TX, int TY>
@@ -2234,7 +2272,7 @@ __global__ void __launch_bounds__(TX *TY) kb_correct(const BatchCase<R> *C, cons
   const KFieldsT<R> f = fields_at(c, par);
   unsigned long long r[2] = {0ull, 0ull};
   correct_cell(c.g, f, t.x * TX + (int)threadIdx.x % TX, t.y * TY + (int)threadIdx.x / TX, 1, r);
-  block_max_atomic<2>(r, c.lred + 0, c.lred + 1);
+  block_max_atomic<2, TX * TY>(r, MaxDst<2>{{c.lred + 0, c.lred + 1}, {}});
 }
 
 template <typename R, int TX, int TY>
