@@ -844,8 +844,12 @@ __device__ __forceinline__ void sweep2_tile(const KGridT<R> &g, const KFieldsT<R
   if (r1p || r1T || r2p || r2T) block_max_atomic<4, SX * SY>(r, MaxDst<4>{{r1p, r1T, r2p, r2T}, {}});
 }
 
+// Six waves per SIMD (80 registers, three workgroups per CU): the pass is
+// bound by the latency of its two load phases, and the third resident
+// workgroup hides more of it than the registers it gives up cost (one 8-byte
+// spill across the barrier).
 template <typename R>
-__global__ void __launch_bounds__(S2X *S2Y) k_sweep2(KGridT<R> g, KFieldsT<R> fl, const R *__restrict__ Po,
+__global__ void __launch_bounds__(S2X *S2Y) __attribute__((amdgpu_waves_per_eu(6, 8))) k_sweep2(KGridT<R> g, KFieldsT<R> fl, const R *__restrict__ Po,
                                                      const R *__restrict__ To, R *__restrict__ Pn,
                                                      R *__restrict__ Tn, unsigned long long *r1p,
                                                      unsigned long long *r1T, unsigned long long *r2p,
