@@ -2220,9 +2220,10 @@ __global__ void __launch_bounds__(TX *TY) kb_assemble_arr(const BatchCase<R> *C,
 }
 
 // Pass of two fused sweeps from parity par; first/last: whether it holds
-// loop 3's first / last sweep (their residuals go to the loop-2 test).
+// loop 3's first / last sweep (their residuals go to the loop-2 test).  Six
+// waves per SIMD, as k_sweep2.
 template <typename R, int SX, int SY>
-__global__ void __launch_bounds__(SX *SY) kb_sweep2(const BatchCase<R> *C, const BatchTile *T, int k, int par,
+__global__ void __launch_bounds__(SX *SY) __attribute__((amdgpu_waves_per_eu(6, 8))) kb_sweep2(const BatchCase<R> *C, const BatchTile *T, int k, int par,
                                                     int first, int last) {
   const BatchTile t = T[blockIdx.x];
   const BatchCase<R> &c = C[t.c];
