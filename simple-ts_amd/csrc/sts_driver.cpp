@@ -144,6 +144,125 @@ struct Driver {
     if (with_rho) set(STS_RHO, rho);
   }
 
+  // The channel's fluid rows at column 1: jb the last body row below it,
+  // je the first body row above it (S:2569-2580, S:2719-2730; rows keep their
+  // previous values when the column has no such transition).
+  unsigned j_channel_b = 0, j_channel_e = 0;
+  void channel_rows() {
+    for (int j = 1; j < Ny; j++) {
+      const size_t node = 1 + (size_t)j * Nx;
+      if (!mask[node - Nx] && mask[node]) j_channel_b = (unsigned)j - 1;
+      if (mask[node - Nx] && !mask[node]) j_channel_e = (unsigned)j;
+    }
+  }
+
+  // Slip coefficient of the exact solutions: polyhedron 0's (S:2586-2592).
+  const sts::Polyhedron &slip_body() const {
+    if (c.polys.empty()) die("ToStartFromExactSolutionForChannelFlowWithPressureBC needs a polyhedron body 0");
+    return c.polys[0];
+  }
+
+  // ToStartFromExactSolutionForChannelFlowWithPressureBC with Pressure_BC
+  // (S:2566-2689): p on the solve columns and u on every fluid cell from the
+  // exact solution of slip flow in a channel between the box-end pressures.
+  void exact_start_pressure_bc() {
+    std::vector<double> u, p;
+    get(STS_U, u);
+    get(STS_P, p);
+    channel_rows();
+    const double *x_v = c.x_v.data(), *y_v = c.y_v.data();
+    const double h_channel = y_v[j_channel_e] - y_v[j_channel_b];
+    const double L_channel = x_v[c.Nx_o[1]] - x_v[c.Nx_o[0] - 1];
+    const double Kn = c.get("Kn"), pb = c.get("p_BC_xb"), pe = c.get("p_BC_xe");
+    const sts::Polyhedron &b0 = slip_body();
+    const double sigma = (double)b0.is_slip * b0.F_slip, Px = pb / pe;
+    std::vector<double> xa(Nx), ya(Ny);
+    for (int i = 0; i < Nx; i++) xa[i] = (x_v[i] - x_v[0]) / L_channel;
+    for (int j = 0; j < Ny; j++) {
+      if ((unsigned)j <= j_channel_b) ya[j] = -0.5 * h_channel;
+      else if (j_channel_e <= (unsigned)j) ya[j] = 0.5 * h_channel;
+      else ya[j] = y_v[(unsigned)j + j_channel_b] - (y_v[j_channel_b] + 0.5 * h_channel);
+    }
+    // the square root of the pressure profile, (p/pe + 6 sigma Kn)
+    auto root = [&](int i) {
+      return sqrt(pow((6.0 * sigma * Kn), 2) + (1.0 + 12.0 * sigma * Kn) * xa[i] +
+                  (Px + 12.0 * sigma * Kn) * Px * (1.0 - xa[i]));
+    };
+    for (int j = 0; j < Ny; j++)
+      for (int i = c.Nx_o[0]; i < c.Nx_o[1]; i++) {
+        const size_t node = (size_t)i + (size_t)j * Nx;
+        if (mask[node]) p[node] = pe * (-6.0 * sigma * Kn + root(i));
+      }
+    const unsigned type = (unsigned)c.get("TypeOfNonDimensiolization");
+    double nd = 0;
+    if (type == 1 || type == 2) nd = c.get("Re") * h_channel * h_channel / (4.0 * c.get("u_given_xb"));
+    else if (type == 3) nd = h_channel / (5.0 * sqrt(M_PI * c.get("T_BC_xb")) * Kn * L_channel);
+    for (int j = 0; j < Ny; j++)
+      for (int i = 0; i < Nx; i++) {
+        const size_t node = (size_t)i + (size_t)j * Nx;
+        if (!mask[node]) continue;
+        u[node] = -nd * ((pe * (1.0 - Px * Px + 12.0 * sigma * Kn * (1.0 - Px))) / (2.0 * root(i))) *
+                  (1.0 - 4.0 * ya[j] * ya[j] / (h_channel * h_channel) + 4.0 * sigma * Kn / p[node]);
+      }
+    set(STS_U, u);
+    set(STS_P, p);
+  }
+
+  // ToStartFromExactSolutionForChannelFlowWithPressureBC with a given inflow
+  // velocity (S:2694-2806): the inflow columns take p_BC_xb, one box-end
+  // pressure is set so that the channel's centre moves at u_given_xb, and
+  // the inflow columns i < Nx_u[0] take the parabolic slip profile.
+  void exact_start_given_velocity() {
+    std::vector<double> u, p;
+    get(STS_U, u);
+    get(STS_P, p);
+    for (int j = 0; j < Ny; j++)
+      for (int i = 0; i < c.Nx_o[0] + 1; i++) {
+        const size_t node = (size_t)i + (size_t)j * Nx;
+        if (mask[node]) p[node] = p_BC_xb;
+      }
+    channel_rows();
+    const double *x_v = c.x_v.data(), *y_v = c.y_v.data();
+    const double h_channel = c.y_f[j_channel_e + 1] - c.y_f[j_channel_b];  // j = Ny > 0 after the scan
+    const double L_channel = x_v[c.Nx_o[1]] - x_v[c.Nx_o[0] - 1];
+    const double yc = y_v[j_channel_b] + 0.5 * h_channel;
+    const double ug = c.get("u_given_xb"), Re = c.get("Re"), Kn = c.get("Kn");
+    // the reference's centre-line factor, its operator precedence kept
+    const double q = yc - y_v[j_channel_b] / h_channel;
+    const double dp = 4.0 * L_channel * ug * ug / (Re * h_channel * h_channel * (q - q * q));
+    if (c.flag("correct_p_BC_xb")) p_BC_xb = c.get("p_BC_xe") + dp;
+    else c.at("p_BC_xe") = p_BC_xb - dp;
+    c.at("p_BC_xb") = p_BC_xb;
+    const double dpdx = (c.get("p_BC_xe") - p_BC_xb) / L_channel;
+    const unsigned type = (unsigned)c.get("TypeOfNonDimensiolization");
+    double nd = 0;
+    if (type == 1 || type == 2) nd = Re * h_channel * h_channel / (4.0 * ug);
+    else if (type == 3) nd = 4.0 * h_channel * h_channel / (5.0 * sqrt(M_PI * c.get("T_BC_xb")) * Kn);
+    const sts::Polyhedron &b0 = slip_body();
+    for (int j = 0; j < Ny; j++)
+      for (int i = 0; i < c.Nx_u[0]; i++) {
+        const size_t node = (size_t)i + (size_t)j * Nx;
+        if (!mask[node]) continue;
+        const double eta = (y_v[j] - y_v[j_channel_b]) / h_channel;
+        u[node] = -nd * dpdx * (eta - eta * eta + Kn * b0.F_slip * (double)b0.is_slip / h_channel);
+      }
+    set(STS_U, u);
+    set(STS_P, p);
+  }
+
+  // ... with a given outflow velocity (S:2808-2830): the outflow columns
+  // take p_BC_xe.
+  void exact_start_given_outflow() {
+    std::vector<double> p;
+    get(STS_P, p);
+    for (int j = 0; j < Ny; j++)
+      for (int i = c.Nx_o[1]; i < Nx; i++) {
+        const size_t node = (size_t)i + (size_t)j * Nx;
+        if (mask[node]) p[node] = c.get("p_BC_xe");
+      }
+    set(STS_P, p);
+  }
+
   // WriteSolvedDataToFile, S:5564-5731 (serial build).
   void write_solved() {
     ck(sts_jump_overwrite(ctx), "sts_jump_overwrite");  // S:5566-5567
@@ -397,12 +516,14 @@ struct Driver {
     ck(sts_set_scalars(ctx, cT, cPch), "sts_set_scalars");
   }
 
-  // BoundaryConditions_Pressure_begin_iteration_for_it, S:7430-7647, without
-  // the pressure-ratio correction.  The largest u and the hy-weighted mean u
-  // over the fluid cells of column Nx_u[0] (S:7452-7486) are formed when
-  // something reads them: p_and_u_xb_it.txt (Pressure_BC), or
-  // p_BC_xb_correction_method 1 / 2, which take them as the inflow velocity
-  // of the step's bookkeeping (cT of types 1 and 2, the drag coefficients).
+  // BoundaryConditions_Pressure_begin_iteration_for_it, S:7430-7647.  The
+  // largest u and the hy-weighted mean u over the fluid cells of column
+  // Nx_u[0] (S:7452-7486) are formed when something reads them:
+  // p_and_u_xb_it.txt (Pressure_BC), or p_BC_xb_correction_method 1 / 2,
+  // which take them as the inflow velocity of the step's bookkeeping (cT of
+  // types 1 and 2, the drag coefficients).  With Pressure_BC and
+  // Pressure_ratio_correct, the pressure at one box end follows that
+  // velocity (pressure_ratio_correct).
   double u_xb_max = 0, u_xb_mean = 0;
   void pressure_begin(double probe) {
     if (c.flag("Periodic_boundary_conditions_about_OX")) return;  // S:7436
@@ -431,6 +552,59 @@ struct Driver {
     } else if (method == 2) {
       u_given_xb_in_it = u_xb_mean;
     }
+    if (c.flag("Pressure_BC") && c.flag("Pressure_ratio_correct")) pressure_ratio_correct();
+  }
+
+  // S:7525-7640: p_correction adapts to the last correction's velocity
+  // residuals (p_correction_auto), the inflow (correct_p_BC_xb) or outflow
+  // pressure moves by p_correction times the squared miss of u_given_xb when
+  // the inflow velocity is outside u_given_xb (1 +- u_given_xb_error), and
+  // that end's fluid columns take the pressure and its density.
+  void pressure_ratio_correct() {
+    double &pc = c.at("p_correction");
+    if (c.flag("p_correction_auto")) {
+      const double r = sqrt(max_res_u * max_res_u + max_res_v * max_res_v);
+      double t;
+      if (Iter_pr1 <= Iter_pr2 && Iter_pr1 < (N_I - 1)) t = pc * (1.0 + r);
+      else t = pc * (1.0 - r);
+      if (t < c.get("p_correction_min")) t = c.get("p_correction_min");
+      else if (c.get("p_correction_max") < t) t = c.get("p_correction_max");
+      pc = t;
+    }
+    const double ug = c.get("u_given_xb"), err = c.get("u_given_xb_error"), ui = u_given_xb_in_it;
+    const bool at_xb = c.flag("correct_p_BC_xb");
+    double &pe = c.at("p_BC_xe");
+    if (ui < (ug * (1.0 - err))) {
+      if (at_xb) {
+        p_BC_xb = p_BC_xb * (1.0 + pc * (ug - ui) * (ug - ui));
+      } else {
+        const double t = pe * (1.0 - pc * (ug - ui) * (ug - ui));
+        pe = (t < 0.0) ? 0.0 : t;
+      }
+    } else if ((ug * (1.0 + err)) < ui) {
+      if (at_xb) {
+        const double t = p_BC_xb * (1.0 - pc * (ug - ui) * (ug - ui));
+        p_BC_xb = (t > pe) ? t : pe;
+      } else {
+        const double t = p_BC_xb * (1.0 + pc * (ug - ui) * (ug - ui));
+        pe = (t > p_BC_xb) ? p_BC_xb : t;
+      }
+    }
+    c.at("p_BC_xb") = p_BC_xb;
+    std::vector<double> p, rho;
+    get(STS_P, p);
+    get(STS_RHO, rho);
+    const int i0 = at_xb ? 0 : c.Nx_o[1], i1 = at_xb ? c.Nx_o[0] : Nx;
+    const double pv = at_xb ? p_BC_xb : pe, rv = pv / c.get(at_xb ? "T_BC_xb" : "T_BC_xe");
+    for (int j = 0; j < Ny; j++)
+      for (int i = i0; i < i1; i++) {
+        const size_t node = (size_t)i + (size_t)j * Nx;
+        if (!mask[node]) continue;
+        p[node] = pv;
+        rho[node] = rv;
+      }
+    set(STS_P, p);
+    set(STS_RHO, rho);
   }
 
   void run() {
@@ -440,9 +614,6 @@ struct Driver {
     Na = (size_t)Nx * (size_t)Ny;
     mask.resize(Na);
     for (size_t k = 0; k < Na; k++) mask[k] = c.vol[k] == 0;
-    for (const char *f : {"ToContinueFromInterpolation", "ToStartFromExactSolutionForChannelFlowWithPressureBC",
-                          "Pressure_ratio_correct"})
-      if (c.flag(f)) die(std::string(f) + " = 1 is not provided");
     ht = c.get("ht");
     N_I = (unsigned)c.get("N_I");
     N_I_p_c = (unsigned)c.get("N_I_p_c");
@@ -498,6 +669,10 @@ struct Driver {
       set(STS_RHO, rho);
       ck(sts_apply_bodies(ctx), "apply_bodies");  // S:2507
       if (c.flag("Pressure_BC")) pressure_columns(true);
+      const bool exact = c.flag("ToStartFromExactSolutionForChannelFlowWithPressureBC");
+      if (exact && c.flag("Pressure_BC")) exact_start_pressure_bc();             // S:2566-2689
+      if (exact && c.flag("Given_velocity_on_xb")) exact_start_given_velocity();  // S:2694-2806
+      if (exact && c.flag("Given_velocity_on_xe")) exact_start_given_outflow();   // S:2808-2830
       ck(sts_save_previous(ctx), "save_previous");  // S:2838-2845
       ck(sts_rho_in_middle_points(ctx), "rho_in_middle_points");
       ck(sts_apply_bodies(ctx), "apply_bodies");
@@ -518,6 +693,11 @@ struct Driver {
     Iter_pr1 = Iter;
     Iter_pr2 = Iter_pr1;
     if (c.flag("Pressure_BC") && !c.flag("ToReadSolvedDataFromFile")) pressure_columns(false);
+    if (c.flag("ToContinueFromInterpolation")) {  // S:2982-2991: the wall conditions on the state read in
+      ck(sts_apply_bodies(ctx), "apply_bodies");
+      ck(sts_boundary_conditions(ctx, STS_BC_SLIP_U | STS_BC_SLIP_V | STS_BC_JUMP, nullptr), "bc");
+      ck(sts_jump_overwrite(ctx), "jump_overwrite");
+    }
     ck(sts_properties(ctx), "properties");  // S:2948-2953
     ck(sts_rho_in_middle_points(ctx), "rho_in_middle_points");
     it = it_begin;
@@ -532,7 +712,12 @@ struct Driver {
     ck(sts_boundary_conditions(ctx, STS_BC_JUMP | STS_BC_TEMPERATURE | STS_BC_PRESSURE, nullptr), "bc");
     ck(sts_fluxes_diffusion(ctx), "fluxes_diffusion");
 
-    const bool fused = (unsigned)c.get("TypeOfNonDimensiolization") == 3;
+    // loop 2 on the device when nothing the host computes changes inside it:
+    // cT is constant (type 3) and no box-end pressure follows the inflow
+    // velocity from one outer iteration to the next
+    const bool ratio = c.flag("Pressure_BC") && c.flag("Pressure_ratio_correct") &&
+                       !c.flag("Periodic_boundary_conditions_about_OX");
+    const bool fused = (unsigned)c.get("TypeOfNonDimensiolization") == 3 && !ratio;
     auto t_timed = std::chrono::steady_clock::now();
     long long outer_at_start = 0;
     if (warmup_steps == 0) start_timing(t_timed, outer_at_start);
@@ -600,8 +785,11 @@ struct Driver {
           ck(sts_pt_jacobi(ctx, (int)N_I_p_c, rs.data()), "pt_jacobi");
           double ruv[2];
           ck(sts_correct(ctx, ruv), "correct");
+          max_res_u = ruv[0];
+          max_res_v = ruv[1];
           scalars();
           ck(sts_boundary_conditions(ctx, STS_BC_SLIP_U | STS_BC_SLIP_V | STS_BC_VELOCITIES, &probe), "bc");
+          pressure_begin(probe);  // S:4509, between the velocity and the temperature-jump conditions
           ck(sts_boundary_conditions(ctx, STS_BC_JUMP | STS_BC_TEMPERATURE | STS_BC_PRESSURE, nullptr), "bc");
           ck(sts_fluxes_diffusion(ctx), "fluxes_diffusion");
           r[0] = ruv[0];
@@ -630,7 +818,6 @@ struct Driver {
               (it_counter < Nt && (double)time(nullptr) < time_solve_stop &&
                ((!(Iter <= 1 && Iter_p_c <= 2)) || es < max_res_u || es < max_res_v ||
                 es < max_res_p0[Iter - 1] || es < max_res_T0[Iter - 1]));
-          pressure_begin(probe);  // S:4480, after the velocity conditions
           if (c.flag("ToSolveContinuityEquation")) continuity();
         } while (continue_iter);
       }

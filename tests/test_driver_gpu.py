@@ -191,3 +191,75 @@ def test_inflow_velocity_correction_methods_match_live_reference(method, nd_type
     assert open(os.path.join(mine, f), "rb").read() == open(os.path.join(ref, f), "rb").read()
     bad = compare_dirs(mine, ref)
     assert not bad, f"files differing from the reference's: {bad}"
+
+
+@pytest.mark.skipif(not have_reference(), reason="oracle/_ref/sts_ref not built")
+@pytest.mark.parametrize("name", ["shipped_coarse", "square"])
+def test_exact_solution_start_matches_live_reference(name, tmp_path):
+    """ToStartFromExactSolutionForChannelFlowWithPressureBC (S:2566-2830).
+    The pressure-driven microchannel starts from the exact slip-flow solution
+    between its box-end pressures (p on the solve columns, u on every fluid
+    cell; polyhedron 0's slip coefficient); the confined square, whose inflow
+    velocity is given, takes the parabolic profile on its inflow columns and
+    the outflow pressure that gives u_given_xb at the channel's centre.
+    3 steps, every file byte for byte against the reference run in the same
+    job (EnteredData.txt carries the box-end pressures it rewrote)."""
+    mine, ref = fresh_input(name, tmp_path / "gpu"), fresh_input(name, tmp_path / "ref")
+    for d in (mine, ref):
+        _set_entries(d, ToStartFromExactSolutionForChannelFlowWithPressureBC=1, Nt=3, Nt_save_solved_data=3,
+                     Nt_save_DragCoefficient=1, WriteDataInShortFormat_kind=1)
+    run_reference(ref)
+    run_driver(mine)
+    for f in ("DataInShortFormat.txt",):
+        assert open(os.path.join(mine, f), "rb").read() == open(os.path.join(ref, f), "rb").read(), f
+    bad = compare_dirs(mine, ref)
+    assert not bad, f"files differing from the reference's: {bad}"
+
+
+@pytest.mark.skipif(not have_reference(), reason="oracle/_ref/sts_ref not built")
+def test_continue_from_interpolation_matches_live_reference(tmp_path):
+    """ToContinueFromInterpolation (S:2982-2991): a run continued from saved
+    fields first applies the body conditions, the velocity-slip and
+    temperature-jump wall conditions and the body-cell temperature overwrite
+    to them.  The microchannel (slip walls), 3 steps, then 2 more continued
+    that way; every file byte for byte against the reference."""
+    mine, ref = fresh_input("shipped_coarse", tmp_path / "gpu"), fresh_input("shipped_coarse", tmp_path / "ref")
+    for d in (mine, ref):
+        _set_entries(d, Nt=3, Nt_save_solved_data=3)
+    run_reference(ref)
+    run_driver(mine)
+    assert not compare_dirs(mine, ref)
+    for d in (mine, ref):
+        _set_entries(d, ToReadSolvedDataFromFile=1, ToContinueFromInterpolation=1, Nt=2, max_time_for_solve=1000.0)
+    run_reference(ref)
+    run_driver(mine)
+    bad = compare_dirs(mine, ref)
+    assert not bad, f"files differing from the reference's after the continued run: {bad}"
+
+
+@pytest.mark.skipif(not have_reference(), reason="oracle/_ref/sts_ref not built")
+@pytest.mark.parametrize("method", [0, 2])
+@pytest.mark.parametrize("at_xb", [1, 0])
+def test_pressure_ratio_correction_matches_live_reference(method, at_xb, tmp_path):
+    """Pressure_ratio_correct (S:7525-7640), inside every outer iteration:
+    p_correction adapts to the correction's velocity residuals, and the
+    inflow (correct_p_BC_xb = 1) or outflow pressure moves by p_correction
+    times the squared miss of u_given_xb when the step's inflow velocity (the
+    given one, or with p_BC_xb_correction_method 2 the measured mean) lies
+    outside its band; that end's columns take the pressure and its density.
+    The microchannel from rest with a target velocity near its developing
+    flow (u_given_xb 0.05; the shipped 2.2147 drives the pressure up
+    tenfold per iteration and the reference itself overflows), p_correction
+    at most 0.1, 3 steps, every file byte for byte against the reference
+    (p_and_u_xb_it.txt traces p_BC_xb and p_correction)."""
+    mine, ref = fresh_input("shipped_coarse", tmp_path / "gpu"), fresh_input("shipped_coarse", tmp_path / "ref")
+    for d in (mine, ref):
+        _set_entries(d, Pressure_ratio_correct=1, correct_p_BC_xb=at_xb, p_BC_xb_correction_method=method,
+                     p_correction_auto=1, p_correction_max=0.1, u_given_xb=0.05, Nt=3, Nt_save_DragCoefficient=1,
+                     WriteDataInShortFormat_kind=1)
+    run_reference(ref)
+    run_driver(mine)
+    for f in ("p_and_u_xb_it.txt", "DataInShortFormat.txt"):
+        assert open(os.path.join(mine, f), "rb").read() == open(os.path.join(ref, f), "rb").read(), f
+    bad = compare_dirs(mine, ref)
+    assert not bad, f"files differing from the reference's: {bad}"
