@@ -426,6 +426,7 @@ static void narrow_grid(sts_ctx *c) {
   h.ht = (float)g.ht; h.gamma1 = (float)g.gamma1; h.Kn = (float)g.Kn; h.cT = (float)g.cT;
   h.cT1 = (float)g.cT1; h.cT2 = (float)g.cT2; h.cPch = (float)g.cPch; h.c_Fr_v = (float)g.c_Fr_v;
   h.rht = 1.0f / h.ht;  // IEEE single division on the host: RN(1/ht) in FP32
+  h.c1 = 1.0f - h.gamma1;  // single subtraction in FP32, as the kernels formed it
   h.rht_one = std::fabs(std::fma(-h.rht, h.ht, 1.0f)) <= 0x1p-25f;
   h.given_xb = g.given_xb; h.given_xe = g.given_xe; h.kn_local = g.kn_local;
   h.bc = g.bc;
@@ -573,6 +574,7 @@ extern "C" int sts_create(const sts_grid *gi, int precision, sts_ctx **out) {
   g.rht = 1.0 / g.ht;  // RN(1/ht); 1 - rht ht is exact in an fma
   g.rht_one = std::fabs(std::fma(-g.rht, g.ht, 1.0)) <= 0x1p-54;
   g.gamma1 = gi->gamma1;
+  g.c1 = 1.0 - g.gamma1;
   g.Kn = gi->Kn;
   g.cT = gi->cT;
   g.cT1 = gi->cT1;

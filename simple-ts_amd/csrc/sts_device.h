@@ -49,6 +49,7 @@ struct KGridT {
   int Nx_u0, Nx_u1, Ny_u0, Ny_u1, Nx_v0, Nx_v1, Ny_v0, Ny_v1, Nx_o0, Nx_o1, Ny_o0, Ny_o1;
   R ht, gamma1, Kn, cT, cT1, cT2, cPch, c_Fr_v;
   R rht;      // 1/ht rounded to nearest in R
+  R c1;       // 1 - gamma1 in R, formed on the host (a uniform kernel argument, no vector register)
   int rht_one;  // |1 - rht ht| <= 2^-(p+1): a rht is within an ulp of a/ht for every a (quot1)
   int given_xb, given_xe, kn_local;
   unsigned bc;

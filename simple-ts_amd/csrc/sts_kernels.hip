@@ -622,7 +622,7 @@ struct PTCoef {
     const R *Fx = FLD(FX), *Fy = FLD(FY), *DTx = FLD(DTX), *DTy = FLD(DTY), *u = FLD(U), *v = FLD(V);
     const R *apx = FLD(APX), *apy = FLD(APY), *bpx = FLD(BPX), *bpy = FLD(BPY);
     ht = g.ht;
-    c1 = R(1.0) - g.gamma1;
+    c1 = g.c1;
     const R hxi = g.hx[n.i], hyj = g.hy[n.j];
     aT1 = maxi0(Fx[ij]) + DTx[ij];
     aT2 = maxi0(-Fx[i1j]) + DTx[i1j];
@@ -846,8 +846,8 @@ __device__ __forceinline__ void sweep2_tile(const KGridT<R> &g, const KFieldsT<R
 
 // Six waves per SIMD (80 registers, three workgroups per CU): the pass is
 // bound by the latency of its two load phases, and the third resident
-// workgroup hides more of it than the registers it gives up cost (one 8-byte
-// spill across the barrier).
+// workgroup hides more of it.  Uniform factors come from the grid constants
+// (scalar registers), so nothing spills.
 template <typename R>
 __global__ void __launch_bounds__(S2X *S2Y) __attribute__((amdgpu_waves_per_eu(6, 8))) k_sweep2(KGridT<R> g, KFieldsT<R> fl, const R *__restrict__ Po,
                                                      const R *__restrict__ To, R *__restrict__ Pn,
