@@ -274,15 +274,19 @@ struct CellDiff {
 
 // The values of the fields a cell's assembly reads around it -- Fx, Fy, u,
 // v, sqrt_T at offsets (di, dj) in {-1, 0, 1}^2, the clamped (x: wrapped when
-// periodic) neighbours of Nbr -- either from global memory through Nbr's
-// indices or from a workgroup's LDS tile of them.  The offsets are
-// compile-time constants wherever the accessors are used.
+// periodic) neighbours of Nbr; rho and rho_pr at the cell and its west and
+// south neighbours; the mesh steps hx of the cell's column and its clamped
+// (wrapped) neighbours, hy of its row and the clamped row below -- either
+// from global memory through Nbr's indices or from a workgroup's LDS tile
+// of them.  The offsets are compile-time constants wherever the accessors
+// are used.
 template <typename R>
 struct GlobalStencil {
-  const R *Fx_, *Fy_, *u_, *v_, *sT_;
+  const R *Fx_, *Fy_, *u_, *v_, *sT_, *rho_, *rho_pr_, *hx_, *hy_;
   const Nbr &n;
-  __device__ __forceinline__ GlobalStencil(const KFieldsT<R> &fl, const Nbr &nb)
-      : Fx_(FLD(FX)), Fy_(FLD(FY)), u_(FLD(U)), v_(FLD(V)), sT_(FLD(SQRT_T)), n(nb) {}
+  __device__ __forceinline__ GlobalStencil(const KGridT<R> &g, const KFieldsT<R> &fl, const Nbr &nb)
+      : Fx_(FLD(FX)), Fy_(FLD(FY)), u_(FLD(U)), v_(FLD(V)), sT_(FLD(SQRT_T)), rho_(FLD(RHO)), rho_pr_(FLD(RHO_PR)),
+        hx_(g.hx), hy_(g.hy), n(nb) {}
   __device__ __forceinline__ unsigned at(int di, int dj) const {
     if (dj == 0) return di < 0 ? n.i_1j : (di > 0 ? n.i1j : n.ij);
     if (dj < 0) return di < 0 ? n.i_1j_1 : (di > 0 ? n.i1j_1 : n.ij_1);
@@ -293,6 +297,10 @@ struct GlobalStencil {
   __device__ __forceinline__ R u(int di, int dj) const { return u_[at(di, dj)]; }
   __device__ __forceinline__ R v(int di, int dj) const { return v_[at(di, dj)]; }
   __device__ __forceinline__ R sT(int di, int dj) const { return sT_[at(di, dj)]; }
+  __device__ __forceinline__ R rho(int di, int dj) const { return rho_[at(di, dj)]; }
+  __device__ __forceinline__ R rho_pr(int di, int dj) const { return rho_pr_[at(di, dj)]; }
+  __device__ __forceinline__ R hx(int di) const { return hx_[di < 0 ? n.i_1 : (di > 0 ? n.i1 : n.i)]; }
+  __device__ __forceinline__ R hy(int dj) const { return hy_[dj < 0 ? n.j - (0u < n.j) : n.j]; }  // dj = -1, 0
   __device__ __forceinline__ R div_ht(R a, const KGridT<R> &g) const { return a / g.ht; }
 };
 
@@ -303,13 +311,19 @@ struct GlobalStencil {
 // 1/ht meets quot1's condition (rht_one), quotients by ht are formed from it.
 template <typename R, int TX, int TY, bool HT1>
 struct TileStencil {
-  const R (*Fx_)[TX + 2], (*Fy_)[TX + 2], (*u_)[TX + 2], (*v_)[TX + 2], (*sT_)[TX + 2];
+  const R (*Fx_)[TX + 2], (*Fy_)[TX + 2], (*u_)[TX + 2], (*v_)[TX + 2], (*sT_)[TX + 2], (*rho_)[TX + 2],
+      (*rho_pr_)[TX + 2];
+  const R *hx_, *hy_;  // [TX + 2], [TY + 2]: the tile's columns and rows with their one-cell ring
   int tx, ty;
   __device__ __forceinline__ R Fx(int di, int dj) const { return Fx_[ty + 1 + dj][tx + 1 + di]; }
   __device__ __forceinline__ R Fy(int di, int dj) const { return Fy_[ty + 1 + dj][tx + 1 + di]; }
   __device__ __forceinline__ R u(int di, int dj) const { return u_[ty + 1 + dj][tx + 1 + di]; }
   __device__ __forceinline__ R v(int di, int dj) const { return v_[ty + 1 + dj][tx + 1 + di]; }
   __device__ __forceinline__ R sT(int di, int dj) const { return sT_[ty + 1 + dj][tx + 1 + di]; }
+  __device__ __forceinline__ R rho(int di, int dj) const { return rho_[ty + 1 + dj][tx + 1 + di]; }
+  __device__ __forceinline__ R rho_pr(int di, int dj) const { return rho_pr_[ty + 1 + dj][tx + 1 + di]; }
+  __device__ __forceinline__ R hx(int di) const { return hx_[tx + 1 + di]; }
+  __device__ __forceinline__ R hy(int dj) const { return hy_[ty + 1 + dj]; }
   __device__ __forceinline__ R div_ht(R a, const KGridT<R> &g) const {
     if constexpr (HT1) return quot1(a, g.ht, g.rht);
     else return a / g.ht;
@@ -319,14 +333,13 @@ struct TileStencil {
 template <typename R, typename St>
 __device__ __forceinline__ void cell_assemble_u(const KGridT<R> &g, const KFieldsT<R> &fl, const Nbr &n,
                                                 const CellDiff<R> &df, unsigned nb, const St &S) {
-  const R *hx = g.hx, *hy = g.hy;
-  const R *rho_pr = FLD(RHO_PR), *u_pr = FLD(U_PR), *rho = FLD(RHO), *rho_u = FLD(RHO_U);
-  const unsigned i = n.i, j = n.j, ij = n.ij, i_1j = n.i_1j, i_1 = n.i_1;
+  const R *u_pr = FLD(U_PR), *rho_u = FLD(RHO_U);
+  const unsigned ij = n.ij;
   const R cT = g.cT;
   R F_xu1 = R(0.5) * (S.Fx(-1, 0) + S.Fx(0, 0));
   R F_xu2 = R(0.5) * (S.Fx(0, 0) + S.Fx(1, 0));
-  R Scu = cT * (R((nb & NB_E) != 0) * S.u(1, 0) * S.sT(0, 0) * hy[j] / (hx[i] * R(3.0))
-                     + R((nb & NB_WW) != 0) * S.u(-1, 0) * S.sT(-1, 0) * hy[j] / (hx[i_1] * R(3.0))
+  R Scu = cT * (R((nb & NB_E) != 0) * S.u(1, 0) * S.sT(0, 0) * S.hy(0) / (S.hx(0) * R(3.0))
+                     + R((nb & NB_WW) != 0) * S.u(-1, 0) * S.sT(-1, 0) * S.hy(0) / (S.hx(-1) * R(3.0))
                      + df.Gy1 * (S.v(0, 1) * R((nb & NB_N) != 0) - S.v(-1, 1) * R((nb & NB_NW) != 0))
                      - df.Gy0 * (S.v(0, 0) * R((nb & NB_S) != 0) - S.v(-1, 0) * R((nb & NB_SW) != 0))
                      - third(S.sT(0, 0) * (S.v(0, 1) * R((nb & NB_N) != 0) - S.v(0, 0) * R((nb & NB_S) != 0)) * R(2.0))
@@ -336,12 +349,12 @@ __device__ __forceinline__ void cell_assemble_u(const KGridT<R> &g, const KField
   R au2 = maxi0(-F_xu2) + df.Dux1;
   R au3 = R(0.5) * (maxi0(S.Fy(-1, 0)) + maxi0(S.Fy(0, 0))) + df.Duy0;
   R au4 = R(0.5) * (maxi0(-S.Fy(-1, 1)) + maxi0(-S.Fy(0, 1))) + df.Duy1;
-  R bu3 = S.div_ht((rho_pr[i_1j] * hx[i_1] + rho_pr[ij] * hx[i]) * u_pr[ij] * hy[j] * R(0.5), g) + Scu;
+  R bu3 = S.div_ht((S.rho_pr(-1, 0) * S.hx(-1) + S.rho_pr(0, 0) * S.hx(0)) * u_pr[ij] * S.hy(0) * R(0.5), g) + Scu;
   R au0 = au1 + au2 + au3 + au4 + R(0.5) * (-S.Fy(-1, 0) - S.Fy(0, 0) + S.Fy(-1, 1) + S.Fy(0, 1)) + F_xu2 - F_xu1 - Spu
-               + S.div_ht((rho[i_1j] * hx[i_1] + rho[ij] * hx[i]) * hy[j] * R(0.5), g);
+               + S.div_ht((S.rho(-1, 0) * S.hx(-1) + S.rho(0, 0) * S.hx(0)) * S.hy(0) * R(0.5), g);
   R ups = (au1 * R((nb & NB_E) != 0) * S.u(-1, 0) + au2 * R((nb & NB_WW) != 0) * S.u(1, 0) + au3 * S.u(0, -1) + au4 * S.u(0, 1) + bu3) / au0;
-  R dpc = g.cPch * hy[j] / au0;
-  R h = hy[j] * rho_u[ij];
+  R dpc = g.cPch * S.hy(0) / au0;
+  R h = S.hy(0) * rho_u[ij];
   FLD(U_PSEUDO)[ij] = ups;
   FLD(D_P_C_12)[ij] = dpc;
   FLD(APX)[ij] = dpc * h;
@@ -351,32 +364,32 @@ __device__ __forceinline__ void cell_assemble_u(const KGridT<R> &g, const KField
 template <typename R, typename St>
 __device__ __forceinline__ void cell_assemble_v(const KGridT<R> &g, const KFieldsT<R> &fl, const Nbr &n,
                                                 const CellDiff<R> &df, unsigned nb, const St &S) {
-  const R *hx = g.hx, *hy = g.hy;
-  const R *rho_pr = FLD(RHO_PR), *v_pr = FLD(V_PR), *rho = FLD(RHO), *rho_v = FLD(RHO_V);
-  const unsigned i = n.i, j = n.j, ij = n.ij, ij_1 = n.ij_1;
+  const R *v_pr = FLD(V_PR), *rho_v = FLD(RHO_V);
+  const unsigned ij = n.ij;
   const R cT = g.cT;
-  // S:3399 the mask at ij_1 - Nx*(0 <= (int)(j-2)) (NB_SS); S:3400 hy[j - 1*(0 <= (int)(j-1))]
-  const R hyjm = hy[j - 1 * (0 <= (int)(j - 1))];
+  // S:3399 the mask at ij_1 - Nx*(0 <= (int)(j-2)) (NB_SS); S:3399, S:3404 hy[j - 1*(0 <= (int)(j-1))],
+  // the clamped row below; S:3419, S:3426 read hy[j - 1] unclamped, the same row inside the v box (j >= 1)
+  const R hyjm = S.hy(-1);
   R F_yv3 = R(0.5) * (S.Fy(0, -1) + S.Fy(0, 0));
   R F_yv4 = R(0.5) * (S.Fy(0, 0) + S.Fy(0, 1));
-  R Scv = cT * (R((nb & NB_N) != 0) * S.v(0, 1) * S.sT(0, 0) * hx[i] / (hy[j] * R(3.0))
-                     + R((nb & NB_SS) != 0) * S.v(0, -1) * S.sT(0, -1) * hx[i] / (hyjm * R(3.0))
+  R Scv = cT * (R((nb & NB_N) != 0) * S.v(0, 1) * S.sT(0, 0) * S.hx(0) / (S.hy(0) * R(3.0))
+                     + R((nb & NB_SS) != 0) * S.v(0, -1) * S.sT(0, -1) * S.hx(0) / (hyjm * R(3.0))
                      + df.Gx1 * (S.u(1, 0) * R((nb & NB_E) != 0) - S.u(1, -1) * R((nb & NB_SE) != 0))
                      - df.Gx0 * (S.u(0, 0) * R((nb & NB_W) != 0) - S.u(0, -1) * R((nb & NB_SW) != 0))
                      - third(S.sT(0, 0) * (S.u(1, 0) * R((nb & NB_E) != 0) - S.u(0, 0) * R((nb & NB_W) != 0)) * R(2.0))
                      + third(S.sT(0, -1) * (S.u(1, -1) * R((nb & NB_SE) != 0) - S.u(0, -1) * R((nb & NB_SW) != 0)) * R(2.0)))
-               - g.c_Fr_v * (rho[ij_1] * hx[i] * R(0.5) * hyjm + rho[ij] * hx[i] * R(0.5) * hy[j]);
+               - g.c_Fr_v * (S.rho(0, -1) * S.hx(0) * R(0.5) * hyjm + S.rho(0, 0) * S.hx(0) * R(0.5) * S.hy(0));
   R Spv = third(-(df.Dvy0 + df.Dvy1));
   R av1 = R(0.5) * (maxi0(S.Fx(0, 0)) + maxi0(S.Fx(0, -1))) + df.Dvx0;
   R av2 = R(0.5) * (maxi0(-S.Fx(1, 0)) + maxi0(-S.Fx(1, -1))) + df.Dvx1;
   R av3 = maxi0(F_yv3) + df.Dvy0;
   R av4 = maxi0(-F_yv4) + df.Dvy1;
-  R bv3 = S.div_ht((rho_pr[ij_1] * hy[j - 1] + rho_pr[ij] * hy[j]) * v_pr[ij] * hx[i] * R(0.5), g) + Scv;
+  R bv3 = S.div_ht((S.rho_pr(0, -1) * S.hy(-1) + S.rho_pr(0, 0) * S.hy(0)) * v_pr[ij] * S.hx(0) * R(0.5), g) + Scv;
   R av0 = av1 + av2 + av3 + av4 + R(0.5) * (S.Fx(1, 0) - S.Fx(0, 0) + S.Fx(1, -1) - S.Fx(0, -1)) + F_yv4 - F_yv3 - Spv
-               + S.div_ht((rho[ij_1] * hy[j - 1] + rho[ij] * hy[j]) * hx[i] * R(0.5), g);
+               + S.div_ht((S.rho(0, -1) * S.hy(-1) + S.rho(0, 0) * S.hy(0)) * S.hx(0) * R(0.5), g);
   R vps = (av1 * S.v(-1, 0) + av2 * S.v(1, 0) + av3 * R((nb & NB_SS) != 0) * S.v(0, -1) + av4 * R((nb & NB_N) != 0) * S.v(0, 1) + bv3) / av0;
-  R dpc = g.cPch * hx[i] / av0;
-  R h = hx[i] * rho_v[ij];
+  R dpc = g.cPch * S.hx(0) / av0;
+  R h = S.hx(0) * rho_v[ij];
   FLD(V_PSEUDO)[ij] = vps;
   FLD(D_P_C_34)[ij] = dpc;
   FLD(APY)[ij] = dpc * h;
@@ -387,26 +400,27 @@ template <typename R, typename St>
 __device__ __forceinline__ void cell_energy_source(const KGridT<R> &g, const KFieldsT<R> &fl, const Nbr &n,
                                                    unsigned nb, const St &S) {
   const unsigned Nx = (unsigned)g.Nx;
-  const R *hx = g.hx, *hy = g.hy;
   const R *p_pr = FLD(P_PR);
-  const unsigned i = n.i, j = n.j, ij = n.ij, i_1 = n.i_1, i1 = n.i1;
-  // S:3729-3740: the upper neighbour row is guarded by j < Nx-1
-  const unsigned jm = j - 1 * (0 < j), jp = j + 1 * (j < (Nx - 1));
-  R dudx_T = (R(((nb & NB_E) != 0)) * S.u(1, 0) - R(((nb & NB_W) != 0)) * S.u(0, 0)) / hx[i];
-  R dvdy_T = (R(((nb & NB_N) != 0)) * S.v(0, 1) - R(((nb & NB_S) != 0)) * S.v(0, 0)) / hy[j];
-  R s0 = ((S.u(0, 0) - S.u(0, -1)) / (R(((nb & NB_SW) != 0) && ((nb & NB_S) != 0)) * hy[jm] + hy[j]))
-              + ((S.v(0, 0) - S.v(-1, 0)) / (R(((nb & NB_SW) != 0) && ((nb & NB_W) != 0)) * hx[i_1] + hx[i]));
-  R s1 = ((S.u(1, 0) - S.u(1, -1)) / (R(((nb & NB_S) != 0) && ((nb & NB_SE) != 0)) * hy[jm] + hy[j]))
-              + ((S.v(1, 0) - S.v(0, 0)) / (hx[i] + R(((nb & NB_SE) != 0) && ((nb & NB_E) != 0)) * hx[i1]));
-  R s2 = ((S.u(0, 1) - S.u(0, 0)) / (hy[j] + R(((nb & NB_NW) != 0) && ((nb & NB_N) != 0)) * hy[jp]))
-              + ((S.v(0, 1) - S.v(-1, 1)) / (R(((nb & NB_W) != 0) && ((nb & NB_NW) != 0)) * hx[i_1] + hx[i]));
-  R s3 = ((S.u(1, 1) - S.u(1, 0)) / (hy[j] + R(((nb & NB_N) != 0) && ((nb & NB_NE) != 0)) * hy[jp]))
-              + ((S.v(1, 1) - S.v(0, 1)) / (hx[i] + R(((nb & NB_E) != 0) && ((nb & NB_NE) != 0)) * hx[i1]));
+  const unsigned j = n.j, ij = n.ij;
+  // S:3729-3740: the upper neighbour row is guarded by j < Nx-1 (from global
+  // memory: that row may lie past the grid's last one); the lower is clamped
+  const unsigned jp = j + 1 * (j < (Nx - 1));
+  const R hyp = g.hy[jp];
+  R dudx_T = (R(((nb & NB_E) != 0)) * S.u(1, 0) - R(((nb & NB_W) != 0)) * S.u(0, 0)) / S.hx(0);
+  R dvdy_T = (R(((nb & NB_N) != 0)) * S.v(0, 1) - R(((nb & NB_S) != 0)) * S.v(0, 0)) / S.hy(0);
+  R s0 = ((S.u(0, 0) - S.u(0, -1)) / (R(((nb & NB_SW) != 0) && ((nb & NB_S) != 0)) * S.hy(-1) + S.hy(0)))
+              + ((S.v(0, 0) - S.v(-1, 0)) / (R(((nb & NB_SW) != 0) && ((nb & NB_W) != 0)) * S.hx(-1) + S.hx(0)));
+  R s1 = ((S.u(1, 0) - S.u(1, -1)) / (R(((nb & NB_S) != 0) && ((nb & NB_SE) != 0)) * S.hy(-1) + S.hy(0)))
+              + ((S.v(1, 0) - S.v(0, 0)) / (S.hx(0) + R(((nb & NB_SE) != 0) && ((nb & NB_E) != 0)) * S.hx(1)));
+  R s2 = ((S.u(0, 1) - S.u(0, 0)) / (S.hy(0) + R(((nb & NB_NW) != 0) && ((nb & NB_N) != 0)) * hyp))
+              + ((S.v(0, 1) - S.v(-1, 1)) / (R(((nb & NB_W) != 0) && ((nb & NB_NW) != 0)) * S.hx(-1) + S.hx(0)));
+  R s3 = ((S.u(1, 1) - S.u(1, 0)) / (S.hy(0) + R(((nb & NB_N) != 0) && ((nb & NB_NE) != 0)) * hyp))
+              + ((S.v(1, 1) - S.v(0, 1)) / (S.hx(0) + R(((nb & NB_E) != 0) && ((nb & NB_NE) != 0)) * S.hx(1)));
   R dsum = dudx_T + dvdy_T;
-  FLD(SCT1)[ij] = g.ht * S.sT(0, 0) * g.cT2 * hx[i] * hy[j]
+  FLD(SCT1)[ij] = g.ht * S.sT(0, 0) * g.cT2 * S.hx(0) * S.hy(0)
                       * (R(2.0) * (dudx_T * dudx_T + dvdy_T * dvdy_T) + s0 * s0 + s1 * s1 + s2 * s2 + s3 * s3
                          - third(dsum * dsum * R(2.0)))
-                  + p_pr[ij] * hx[i] * hy[j];
+                  + p_pr[ij] * S.hx(0) * S.hy(0);
 }
 
 // The assembly of cell (i, j) with Gamma and D read from their arrays (stage
@@ -424,12 +438,12 @@ __device__ __forceinline__ void assemble_cell(const KGridT<R> &g, const KFieldsT
             *Dvy = FLD(D_VY);
     const CellDiff<R> df{Gyf[n.ij], Gyf[n.ij1], Gxf[n.ij], Gxf[n.i1j], Dux[n.ij], Dux[n.i1j],
                          Duy[n.ij], Duy[n.ij1], Dvx[n.ij], Dvx[n.i1j], Dvy[n.ij], Dvy[n.ij1]};
-    const GlobalStencil<R> S(fl, n);
+    const GlobalStencil<R> S(g, fl, n);
     if (du) cell_assemble_u(g, fl, n, df, nb, S);
     if (dv) cell_assemble_v(g, fl, n, df, nb, S);
   }
   if (in_box(i, j, g.Nx_o0, g.Nx_o1, g.Ny_o0, g.Ny_o1) && c)
-    cell_energy_source(g, fl, n, nb, GlobalStencil<R>(fl, n));
+    cell_energy_source(g, fl, n, nb, GlobalStencil<R>(g, fl, n));
 }
 
 template <typename R>
@@ -505,7 +519,7 @@ __device__ __forceinline__ void assemble_inl_tile(const KGridT<R> &g, const KFie
   __shared__ R Lgy[TY + 1][TX + 1], Lgx[TY + 1][TX + 1], Lux[TY + 1][TX + 1], Luy[TY + 1][TX + 1],
       Lvx[TY + 1][TX + 1], Lvy[TY + 1][TX + 1];
   __shared__ R SFx[TY + 2][TX + 2], SFy[TY + 2][TX + 2], Su[TY + 2][TX + 2], Sv[TY + 2][TX + 2],
-      SsT[TY + 2][TX + 2], Shx[TX + 2], Shy[TY + 2];
+      SsT[TY + 2][TX + 2], Srho[TY + 2][TX + 2], Srho_pr[TY + 2][TX + 2], Shx[TX + 2], Shy[TY + 2];
   const int Nx = g.Nx, Ny = g.Ny, t = tx + ty * TX;
   // column x - 1 and row y - 1 of the tile, clamped (x wrapped when periodic)
   // as Nbr takes a neighbour; x - 1 >= -1, and x - 1 < Nx + TX
@@ -530,6 +544,8 @@ __device__ __forceinline__ void assemble_inl_tile(const KGridT<R> &g, const KFie
     Su[y][x] = FLD(U)[k];
     Sv[y][x] = FLD(V)[k];
     SsT[y][x] = FLD(SQRT_T)[k];
+    Srho[y][x] = FLD(RHO)[k];
+    Srho_pr[y][x] = FLD(RHO_PR)[k];
   }
   if (t < TX + 2) Shx[t] = g.hx[col(t)];
   else if (t < TX + TY + 4) Shy[t - (TX + 2)] = g.hy[row(t - (TX + 2))];
@@ -564,7 +580,7 @@ __device__ __forceinline__ void assemble_inl_tile(const KGridT<R> &g, const KFie
   const bool c = (nb & NB_C) != 0;
   const bool du = in_box(i, j, g.Nx_u0, g.Nx_u1, g.Ny_u0, g.Ny_u1) && (nb & NB_W) && c;
   const bool dv = in_box(i, j, g.Nx_v0, g.Nx_v1, g.Ny_v0, g.Ny_v1) && (nb & NB_S) && c;
-  const TileStencil<R, TX, TY, HT1> S{SFx, SFy, Su, Sv, SsT, tx, ty};
+  const TileStencil<R, TX, TY, HT1> S{SFx, SFy, Su, Sv, SsT, Srho, Srho_pr, Shx, Shy, tx, ty};
   if (du || dv) {
     const int ex = (i == il) ? TX : tx + 1, ny = (j == jl) ? TY : ty + 1;
     const CellDiff<R> df{Lgy[ty][tx], Lgy[ny][tx], Lgx[ty][tx], Lgx[ty][ex], Lux[ty][tx], Lux[ty][ex],
