@@ -1662,22 +1662,44 @@ __global__ void __launch_bounds__(BX *BY) k_extrapolate(KGrid g, KFields fl, Ext
 constexpr int AA_THREADS = 256;
 constexpr int AA_SOLVE_THREADS = 1024;
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+// One step of wave_sum_last: v plus its DPP-moved copy; rows outside
+// ROW_MASK add +0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_sum_step(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const int lo = (int)(unsigned)b, hi = (int)(unsigned)(b >> 32);
+  const unsigned long long m =
+      ((unsigned long long)(unsigned)__builtin_amdgcn_update_dpp(0, hi, CTRL, ROW_MASK, 0xf, false) << 32) |
+      (unsigned long long)(unsigned)__builtin_amdgcn_update_dpp(0, lo, CTRL, ROW_MASK, 0xf, false);
+  return v + __longlong_as_double((long long)m);
+}
+
+// Sum over a full wave, complete in lane 63, in a fixed order: quads, half
+// rows and rows by DPP permutations and mirrors, then rows 1 and 3 add rows 0
+// and 2 (broadcast of lane 15) and rows 2 and 3 add rows 0-1 (broadcast of
+// lane 31).  Every lane of the wave must be active.
+__device__ __forceinline__ double wave_sum_last(double v) {
+  v = dpp_sum_step<0xb1, 0xf>(v);   // quad_perm(1, 0, 3, 2)
+  v = dpp_sum_step<0x4e, 0xf>(v);   // quad_perm(2, 3, 0, 1)
+  v = dpp_sum_step<0x141, 0xf>(v);  // row_half_mirror
+  v = dpp_sum_step<0x140, 0xf>(v);  // row_mirror
+  v = dpp_sum_step<0x142, 0xa>(v);  // row_bcast:15
+  v = dpp_sum_step<0x143, 0xc>(v);  // row_bcast:31
   return v;
 }
 
 // Sums acc[0..NV) over a workgroup of NT threads (fixed order: lanes, then
-// waves in turn); thread t < NV gets sum t in out[t].
-template <int NV, int NT>
-__device__ __forceinline__ void block_sums(const double (&acc)[NV], double *out) {
+// waves in turn); thread t < NV gets sum t in out[t].  Only the sums j with
+// live(j) are formed, the others are 0 (their acc is unused).
+template <int NV, int NT, typename Live>
+__device__ __forceinline__ void block_sums(const double (&acc)[NV], double *out, Live live) {
+  static_assert(NT % 64 == 0, "whole waves");
   __shared__ double sm[NV][NT / 64];
   const int lane = (int)threadIdx.x & 63, wid = (int)threadIdx.x >> 6;
 #pragma unroll
   for (int j = 0; j < NV; j++) {
-    const double w = wave_sum(acc[j]);
-    if (lane == 0) sm[j][wid] = w;
+    const double w = live(j) ? wave_sum_last(acc[j]) : 0.0;
+    if (lane == 63) sm[j][wid] = w;
   }
   __syncthreads();
   if ((int)threadIdx.x < NV) {
@@ -1780,9 +1802,9 @@ __device__ __forceinline__ void aa_store_cell(const AAFieldsT<R> &s, const AABuf
 // The workgroup's (NT threads) sums of acc into partial slot idx:
 // part[j nb + idx] (a_j) and part[(MAX_AA + j) nb + idx] (b_j).
 template <int NM, int NT>
-__device__ __forceinline__ void aa_store_sums(const double (&acc)[2 * NM], const AABuf &ab, int idx) {
+__device__ __forceinline__ void aa_store_sums(const double (&acc)[2 * NM], const AABuf &ab, int idx, int n) {
   __shared__ double sums[2 * NM];
-  block_sums<2 * NM, NT>(acc, sums);
+  block_sums<2 * NM, NT>(acc, sums, [n](int j) { return j % NM < n; });
   __syncthreads();
   const int j = (int)threadIdx.x;
   if (j < 2 * NM) ab.part[(size_t)(j < NM ? j : MAX_AA + j - NM) * ab.nb + idx] = sums[j];
@@ -1802,7 +1824,7 @@ __global__ void __launch_bounds__(AA_THREADS) k_aa_store(AAFieldsT<R> s, AABuf a
     const int k = (int)((blockIdx.x * AA_STORE_CELLS + r) * AA_THREADS + threadIdx.x);
     if (k < Na) aa_store_cell<NM, R>(s, ab, sel, k, acc);
   }
-  if (sel.n > 0) aa_store_sums<NM, AA_THREADS>(acc, ab, (int)blockIdx.x);
+  if (sel.n > 0) aa_store_sums<NM, AA_THREADS>(acc, ab, (int)blockIdx.x, sel.n);
 }
 
 // One workgroup of NT threads: folds the store pass's ab.nb partial sums,
@@ -1845,7 +1867,7 @@ __device__ __forceinline__ void aa_solve_group(const AABuf &ab, const AASel &sel
     const int a = (int)threadIdx.x / NM, b = (int)threadIdx.x % NM;
     if (a < n - 1 && b < n - 1) A[a][b] = ab.gram[sel.slot[a] * MAX_AA + sel.slot[b]];
   }
-  block_sums<2 * NM, NT>(acc, dots);
+  block_sums<2 * NM, NT>(acc, dots, [n](int j) { return j % NM < n; });
   __syncthreads();
   if (leader && (int)threadIdx.x < n) {  // the newest row, kept for the next steps
     const int kn = sel.slot[n - 1], sa = sel.slot[threadIdx.x];
@@ -2378,7 +2400,7 @@ __global__ void __launch_bounds__(TX *TY) kb_aa_store(const BatchCase<double> *C
 #pragma unroll
   for (int q = 0; q < 2 * NM; q++) acc[q] = 0.0;
   if (i < c.g.Nx && j < c.g.Ny) aa_store_cell<NM>(case_state(c, par), c.aa, sel, i + j * c.g.Nx, acc);
-  if (bs.n > 0) aa_store_sums<NM, TX * TY>(acc, c.aa, t.y * ((c.g.Nx + TX - 1) / TX) + t.x);
+  if (bs.n > 0) aa_store_sums<NM, TX * TY>(acc, c.aa, t.y * ((c.g.Nx + TX - 1) / TX) + t.x, sel.n);
 }
 
 // One workgroup per case.
