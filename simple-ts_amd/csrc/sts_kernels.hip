@@ -1918,13 +1918,13 @@ __device__ __forceinline__ void aa_solve_group(const AABuf &ab, const AASel &sel
 }
 
 // Mixed state of cell k: x = gp - sum_j gamma_j dG_j (gp = the newest
-// iterate); a value that is not finite, or a p or T that is not positive
-// where gp's is, falls back to gp.
+// iterate, also returned); a value that is not finite, or a p or T that is
+// not positive where gp's is, falls back to gp.
 template <int NM>
 __device__ __forceinline__ void aa_mixed(const AABuf &ab, const AASel &sel, const double (&gam)[NM], unsigned k,
-                                         double (&x)[4]) {
+                                         double (&x)[4], double (&gp)[4]) {
   const size_t e = 4 * (size_t)k;
-  double gp[4], dG[NM][4];
+  double dG[NM][4];
   load4(ab.gp + e, gp);
 #pragma unroll
   for (int j = 0; j < NM; j++)
@@ -1947,11 +1947,13 @@ __device__ __forceinline__ void aa_mixed(const AABuf &ab, const AASel &sel, cons
 // neighbours); with FLUX also what the loop's flux pass writes (Fx, Fy, DTx,
 // DTy; FluxIn::apply_flux).  The mixed p, T of the tile and of its west
 // column and south row (formed by the first threads) meet in LDS (sp, sT:
-// [1 + row][1 + column], row 0 and column 0 the halo).
+// [1 + row][1 + column], row 0 and column 0 the halo); with FLUX their
+// sqrt(T) too (ss), each formed once.
 template <int NM, int TX, int TY, bool FLUX, typename R>
 __device__ __forceinline__ void aa_mix_tile(const KGridT<R> &g, const KFieldsT<R> &fl, const AAFieldsT<R> &s,
                                             const AABuf &ab, const AASel &sel, const double *gsrc, int i0, int j0,
-                                            int tx, int ty, double (*sp)[TX + 1], double (*sT)[TX + 1]) {
+                                            int tx, int ty, double (*sp)[TX + 1], double (*sT)[TX + 1],
+                                            double (*ss)[TX + 1]) {
   double gam[NM];
 #pragma unroll
   for (int j = 0; j < NM; j++) gam[j] = gsrc[j];
@@ -1960,27 +1962,30 @@ __device__ __forceinline__ void aa_mix_tile(const KGridT<R> &g, const KFieldsT<R
   const bool in = i < Nx && j < Ny;
   const Nbr nb((unsigned)(in ? i : 0), (unsigned)(in ? j : 0), (unsigned)Nx, (unsigned)Ny, g.px);
   // the mixed values as the state holds them (rounded to R), here and in LDS
-  double x[4] = {0.0, 0.0, 0.0, 0.0};
+  double x[4] = {0.0, 0.0, 0.0, 0.0}, gp[4] = {0.0, 0.0, 0.0, 0.0};
   if (in) {
-    aa_mixed<NM>(ab, sel, gam, nb.ij, x);
+    aa_mixed<NM>(ab, sel, gam, nb.ij, x, gp);
 #pragma unroll
     for (int q = 0; q < 4; q++) x[q] = (double)(R)x[q];
     sp[ty + 1][tx + 1] = x[2];
     sT[ty + 1][tx + 1] = x[3];
+    if constexpr (FLUX) ss[ty + 1][tx + 1] = (double)sqrt((R)x[3]);
   }
   // halo: thread (t, 0) the cell south of column t, thread (t < TY, 1) the
   // cell west of row t (the clamped or wrapped neighbour of the reference)
-  double h[4];
+  double h[4], hg[4];
   if (ty == 0 && in) {
-    aa_mixed<NM>(ab, sel, gam, nb.ij_1, h);
+    aa_mixed<NM>(ab, sel, gam, nb.ij_1, h, hg);
     sp[0][tx + 1] = (double)(R)h[2];
     sT[0][tx + 1] = (double)(R)h[3];
+    if constexpr (FLUX) ss[0][tx + 1] = (double)sqrt((R)h[3]);
   }
   if (ty == 1 && tx < TY && i0 < Nx && j0 + tx < Ny) {
     const Nbr w((unsigned)i0, (unsigned)(j0 + tx), (unsigned)Nx, (unsigned)Ny, g.px);
-    aa_mixed<NM>(ab, sel, gam, w.i_1j, h);
+    aa_mixed<NM>(ab, sel, gam, w.i_1j, h, hg);
     sp[tx + 1][0] = (double)(R)h[2];
     sT[tx + 1][0] = (double)(R)h[3];
+    if constexpr (FLUX) ss[tx + 1][0] = (double)sqrt((R)h[3]);
   }
   __syncthreads();
   if (!in) return;
@@ -2002,8 +2007,7 @@ __device__ __forceinline__ void aa_mix_tile(const KGridT<R> &g, const KFieldsT<R
   ci.Tw = (R)sT[ty + 1][tx];
   ci.ps = (R)sp[ty][tx + 1];
   ci.Ts = (R)sT[ty][tx + 1];
-  double d[4], gp[4];
-  load4(ab.gp + 4 * (size_t)nb.ij, gp);
+  double d[4];
 #pragma unroll
   for (int q = 0; q < 4; q++) {
     s.x[q][nb.ij] = (R)x[q];
@@ -2033,9 +2037,9 @@ __device__ __forceinline__ void aa_mix_tile(const KGridT<R> &g, const KFieldsT<R
   fi.m10 = ci.ms;
   fi.m01 = ci.mw;
   fi.m11 = ci.mc;
-  fi.s10 = sqrt(ci.Ts);
-  fi.s01 = sqrt(ci.Tw);
-  fi.s11 = sqrt(ci.Tc);
+  fi.s10 = (R)ss[ty][tx + 1];
+  fi.s01 = (R)ss[ty + 1][tx];
+  fi.s11 = (R)ss[ty + 1][tx + 1];
   fi.twu11 = FLD(TW_U)[nb.ij];
   fi.twv11 = FLD(TW_V)[nb.ij];
   fi.apply_flux(g, kf);
@@ -2053,13 +2057,14 @@ __global__ void __launch_bounds__(AA_MIX_TX *AA_MIX_TY) k_aa_mix(KGridT<R> g, KF
   __shared__ double gam[NM];
   aa_solve_group<NM, AA_MIX_TX * AA_MIX_TY>(ab, sel, gam, blockIdx.x == 0);
   __syncthreads();
-  __shared__ double sp[AA_MIX_TY + 1][AA_MIX_TX + 1], sT[AA_MIX_TY + 1][AA_MIX_TX + 1];
+  __shared__ double sp[AA_MIX_TY + 1][AA_MIX_TX + 1], sT[AA_MIX_TY + 1][AA_MIX_TX + 1],
+      ss[AA_MIX_TY + 1][AA_MIX_TX + 1];
   const int ntx = (g.Nx + AA_MIX_TX - 1) / AA_MIX_TX, ntiles = ntx * ((g.Ny + AA_MIX_TY - 1) / AA_MIX_TY);
   const int tx = (int)threadIdx.x % AA_MIX_TX, ty = (int)threadIdx.x / AA_MIX_TX;
   for (int t = (int)blockIdx.x; t < ntiles; t += (int)gridDim.x) {
     aa_mix_tile<NM, AA_MIX_TX, AA_MIX_TY, true, R>(g, fl, s, ab, sel, gam, (t % ntx) * AA_MIX_TX,
-                                                  (t / ntx) * AA_MIX_TY, tx, ty, sp, sT);
-    __syncthreads();  // the next tile reuses sp, sT
+                                                  (t / ntx) * AA_MIX_TY, tx, ty, sp, sT, ss);
+    __syncthreads();  // the next tile reuses sp, sT, ss
   }
 }
 
@@ -2421,7 +2426,7 @@ __global__ void __launch_bounds__(TX *TY) kb_aa_mix(const BatchCase<double> *C, 
   if (gated_out(Gate{c.ctrl, k + 1})) return;
   __shared__ double sp[TY + 1][TX + 1], sT[TY + 1][TX + 1];
   aa_mix_tile<NM, TX, TY, false>(c.g, fields_at(c, par), case_state(c, par), c.aa, case_sel(c, bs), c.aa.gam,
-                                 t.x * TX, t.y * TY, (int)threadIdx.x % TX, (int)threadIdx.x / TX, sp, sT);
+                                 t.x * TX, t.y * TY, (int)threadIdx.x % TX, (int)threadIdx.x / TX, sp, sT, nullptr);
 }
 
 // Edge conditions and the loop-2 test: one workgroup per case.
