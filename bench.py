@@ -25,24 +25,27 @@ step ends on the reference's loop-2 test (1e-8):
                    (sts_time_step_mixed, extrapolate = 2, guard = 64);
   anderson         FP64 loop 2 from the previous step's state with Anderson
                    acceleration of the outer iterations (history depth 2);
-  accelerated      the headline path: the guarded predicted start of degree
-                   3 with Anderson acceleration (depth 1), all FP64;
+  accelerated      the guarded predicted start of degree 3 with Anderson
+                   acceleration (depth 1), all FP64;
   two_phase        an FP32 loop-2 phase until it stagnates, then the FP64
                    phase, from the previous step's state;
   mixed            the guarded degree-3 predicted start, 6 FP32 outer
                    iterations with Anderson acceleration, then the FP64
                    accelerated phase whose FP64 residuals, equation of state
                    and test decide the step;
-  surrogate        the mixed path with the trained corrector network refining
-                   each predicted start (sts_set_corrector, kept only when it
-                   lowers the first sweep's residual), when its parameters
-                   are present (simple-ts_amd/unet_corr_b32.npz, written by
-                   train_unet.py --corrector);
+  surrogate        the method the metric names (surrogate warm start + FP32
+                   with FP64 defect correction): the mixed path with the
+                   trained corrector network refining the predicted starts
+                   (sts_set_corrector, kept only when it lowers the first
+                   sweep's residual; after a rejection the network sits out
+                   1, 2, 4, ... predicted steps, at most 64), when its
+                   parameters are present (simple-ts_amd/unet_corr_b32.npz,
+                   written by train_unet.py --corrector);
   surrogate_accelerated  the accelerated path with the same corrector.
-value = Nx*Ny*K / timed seconds / 1e6 (Mcell.steps/s) of accelerated; every
-path's figure, its speed-up over the faster of fp64 and plain, and every
-field's relative L2 error against the fp64 path after the W + K steps are
-reported beside it.
+value = Nx*Ny*K / timed seconds / 1e6 (Mcell.steps/s) of surrogate (of
+accelerated when the network's parameters are absent); every path's figure,
+its speed-up over the faster of fp64 and plain, and every field's relative
+L2 error against the fp64 path after the W + K steps are reported beside it.
 
 roofline (plain FP64 program, timed steps only): the kernel with the largest
 share of the outer iteration's time, the coefficient assembly of the loops
@@ -119,7 +122,8 @@ PATHS = {  # sts_time_step_mixed settings of each path (n32: FP32 cap; stall: FP
     "surrogate": dict(_MIX, net=True, corr_pause=CORR_PAUSE),
     "surrogate_accelerated": dict(_ACC, net=True, corr_pause=CORR_PAUSE),
 }
-HEADLINE = "accelerated"
+HEADLINE = "surrogate"  # the metric's method; "accelerated" when the network is absent
+FALLBACK = "accelerated"
 
 
 def workload_case(dst, steps, nx=NX, ny=NY, cfl=1.0):
@@ -556,7 +560,7 @@ def main():
                                             "max_pause": CORR_PAUSE}
                 paths[name]["network"] = info
                 net.close()
-            if name == HEADLINE:
+            if name == HEADLINE or (name == FALLBACK and best_name != HEADLINE):
                 best, best_name = v, name
     # speed-ups against the faster of the two plain FP64 measurements (the
     # program, and sts_time_step through the binding the other paths use)
@@ -587,7 +591,7 @@ def main():
         "metric": METRIC, "value": best, "unit": "Mcell·steps/s", "n_gpus": 1, "steps": t["timed_steps"],
         "warmup": t["warmup_steps"], "ms_per_step": paths[best_name]["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if best_name in ("fp64", "accelerated") else "f32+f64",
+        "dtype": {"fp64": "f64", "accelerated": "f64", "surrogate": "bf16+f32+f64"}.get(best_name, "f32+f64"),
         "data": "synthetic: confined-square channel generated in the reference's input format, uniform start",
         "config": {"workload": f"supersonic confined square, Ma 2.43 Kn {KN} Re {RE:g}, {NX}x{NY}, "
                                f"dt {DT_OVER_TIN:g} H/u_in, each step converged to 1e-8", "Nx": NX, "Ny": NY,
