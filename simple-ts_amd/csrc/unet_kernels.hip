@@ -133,6 +133,13 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
 // ---------------------------------------------------------------------------
 constexpr int CONV_PW = 64, CONV_CI = 32;
 
+// LDS slot of 16-byte vector g (of 4) of the pixel or output channel at
+// position c: a pixel's 4 vectors are rotated by c / 2, so that the 8 lanes
+// reading vector g of 8 consecutive positions in one LDS access hit 8
+// different groups of 4 banks (unrotated, the 64-byte stride between
+// positions puts them on 2 groups: 4-way conflicts).
+__device__ __forceinline__ int conv_slot(int c, int g) { return (g + (c >> 1)) & 3; }
+
 template <int TN, int RW>
 __global__ void __launch_bounds__(256) k_conv_lds(ConvArgs a) {
   constexpr int CONV_PH = 4 * RW, TM = 4 * RW;
@@ -188,12 +195,18 @@ __global__ void __launch_bounds__(256) k_conv_lds(ConvArgs a) {
 #pragma unroll
     for (int k = 0; k < RA; k++) {
       const int e = t + 256 * k;
-      if (e < NA) As[(e / NV) / (CONV_PW + 2)][(e / NV) % (CONV_PW + 2)][e % NV] = qa[k];
+      if (e < NA) {
+        const int col = (e / NV) % (CONV_PW + 2);
+        As[(e / NV) / (CONV_PW + 2)][col][conv_slot(col, e % NV)] = qa[k];
+      }
     }
 #pragma unroll
     for (int k = 0; k < RB; k++) {
       const int e = t + 256 * k;
-      if (e < NB) Bs[e / (NV * 16 * TN)][(e / NV) % (16 * TN)][e % NV] = qb[k];
+      if (e < NB) {
+        const int co = (e / NV) % (16 * TN);
+        Bs[e / (NV * 16 * TN)][co][conv_slot(co, e % NV)] = qb[k];
+      }
     }
     __syncthreads();
     if (ci0 + CONV_CI < a.Cx) fetch(ci0 + CONV_CI);  // in flight while this chunk's MFMAs run
@@ -202,10 +215,12 @@ __global__ void __launch_bounds__(256) k_conv_lds(ConvArgs a) {
       const int dy = tap / 3 - 1, dx = tap % 3 - 1;
       bf16x8 A[TM], B[TN];
 #pragma unroll
-      for (int s = 0; s < TM; s++)
-        A[s] = __builtin_bit_cast(bf16x8, As[RW * wave + s / 4 + 1 + dy][16 * (s % 4) + r + 1 + dx][g]);
+      for (int s = 0; s < TM; s++) {
+        const int col = 16 * (s % 4) + r + 1 + dx;
+        A[s] = __builtin_bit_cast(bf16x8, As[RW * wave + s / 4 + 1 + dy][col][conv_slot(col, g)]);
+      }
 #pragma unroll
-      for (int s = 0; s < TN; s++) B[s] = __builtin_bit_cast(bf16x8, Bs[tap][16 * s + r][g]);
+      for (int s = 0; s < TN; s++) B[s] = __builtin_bit_cast(bf16x8, Bs[tap][16 * s + r][conv_slot(16 * s + r, g)]);
 #pragma unroll
       for (int x = 0; x < TM; x++)
 #pragma unroll
