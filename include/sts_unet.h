@@ -97,11 +97,15 @@ int sts_set_corrector(sts_ctx *ctx, sts_unet *net, const sts_unet_corr_norm *nrm
    rejected by the residual test. */
 int sts_corrector_stats(sts_ctx *ctx, long long out[2]);
 /* Back-off of the corrector (no counterpart in the reference): with
-   max_pause > 0, a rejected refinement makes the corrector sit out the next
-   1, 2, 4, ... predicted steps (at most max_pause; back to 1 after a kept
-   refinement), so a network that does not help costs little; those steps
-   run from the extrapolated start as without a corrector.  0 (the default
-   of sts_set_corrector): every predicted step is refined and tested. */
+   max_pause > 0, a refinement that does not pay makes the corrector sit out
+   the next 1, 2, 4, ... predicted steps (at most max_pause; back to 1 after
+   one that pays), so a network that does not help costs little; those steps
+   run from the extrapolated start as without a corrector.  A refinement
+   pays when it is kept and its step then needs fewer outer iterations than
+   the running mean (weights 1/2, 1/4, ...) of the predicted steps that ran
+   without one; a rejected one, or one whose predicted attempt is abandoned,
+   never pays.  0 (the default of sts_set_corrector): every predicted step
+   is refined and tested. */
 int sts_set_corrector_pause(sts_ctx *ctx, int max_pause);
 /* Predicted steps the corrector sat out under sts_set_corrector_pause. */
 int sts_corrector_skipped(sts_ctx *ctx, long long *skipped);

@@ -97,10 +97,15 @@ struct sts_ctx {
   sts_unet *corr_net = nullptr;
   sts_unet_corr_norm corr_norm{};
   long long cstat[3] = {0, 0, 0};  // kept, rejected, skipped (paused)
-  // pause after a rejected refinement (sts_set_corrector_pause): the next
-  // corr_pause predicted steps skip the corrector; corr_backoff doubles up to
-  // corr_max_pause on each further rejection and returns to 1 when one is kept
+  // pause after a refinement that did not pay (sts_set_corrector_pause): the
+  // next corr_pause predicted steps skip the corrector; corr_backoff doubles
+  // up to corr_max_pause on each further one and returns to 1 when one pays.
+  // A refinement pays when it is kept and its step then needs fewer outer
+  // iterations than corr_ref_mean, the running mean of the predicted steps
+  // run without one (< 0: none yet).
   int corr_max_pause = 0, corr_pause = 0, corr_backoff = 1;
+  bool corr_kept_now = false;  // this step's start is a kept refinement
+  double corr_ref_mean = -1.0;
   // Gamma_* and D_*: whether forming them from sqrt_T, the wall temperatures
   // and the mask gives the values the reference holds (true after a flux and
   // diffusion pass of the library, until those inputs or the arrays are set
@@ -1337,6 +1342,7 @@ static int first_sweep_residual(sts_ctx *c, const sts_loop_control *lc, int q, d
 static int corrector_refine(sts_ctx *c, const sts_loop_control *lc) {
   int rc;
   double rA, rB;
+  c->corr_kept_now = false;
   if (c->corr_pause > 0) {
     c->corr_pause--;
     c->cstat[2]++;
@@ -1348,9 +1354,9 @@ static int corrector_refine(sts_ctx *c, const sts_loop_control *lc) {
   double *const out[4] = {c->f[STS_U], c->f[STS_V], c->f[STS_P], c->f[STS_T]};
   if ((rc = sts_internal_unet_correct(c->corr_net, v, &c->corr_norm, out))) return rc;
   if ((rc = rebuild_derived(c, c->stream)) || (rc = first_sweep_residual(c, lc, 4, &rB))) return rc;
-  if (rB < rA) {
+  if (rB < rA) {  // kept; whether it pays is known when the step has run (corrector_outcome)
     c->cstat[0]++;
-    c->corr_backoff = 1;
+    c->corr_kept_now = true;
     return STS_OK;
   }
   c->cstat[1]++;
@@ -1360,6 +1366,26 @@ static int corrector_refine(sts_ctx *c, const sts_loop_control *lc) {
   }
   if ((rc = copy_start(c, c->stream, true))) return rc;
   return rebuild_derived(c, c->stream);
+}
+
+// After a predicted step of a context with a corrector and a back-off: a
+// kept refinement pays when the step needed fewer outer iterations (all
+// phases) than the running mean of the predicted steps run without one;
+// one that does not pay backs off like a rejected one.  restarted: the
+// predicted attempt was abandoned (the step ran from x^n).
+static void corrector_outcome(sts_ctx *c, int total, bool restarted) {
+  if (!c->corr_net || c->corr_max_pause <= 0) return;
+  if (c->corr_kept_now) {
+    c->corr_kept_now = false;
+    if (!restarted && (c->corr_ref_mean < 0.0 || total < c->corr_ref_mean)) {
+      c->corr_backoff = 1;
+    } else {
+      c->corr_pause = c->corr_backoff;
+      c->corr_backoff = std::min(2 * c->corr_backoff, c->corr_max_pause);
+    }
+  } else if (!restarted) {
+    c->corr_ref_mean = c->corr_ref_mean < 0.0 ? total : 0.5 * (c->corr_ref_mean + total);
+  }
 }
 
 // Whether this step of context c runs an FP32 phase (mc asks for one and the
@@ -1457,6 +1483,7 @@ extern "C" int sts_time_step_mixed(sts_ctx *c, const sts_loop_control *lc, const
     if ((rc = extrapolate_start(c, mc->extrapolate, nullptr, false, MAX_EXTRAP, &used))) return rc;
     if (used > 0 && c->corr_net && (rc = corrector_refine(c, lc))) return rc;
     rc = run_phases(c, lc, mc, &n32, &n64);
+    if (!rc && used > 0) corrector_outcome(c, n32 + n64, false);
   } else {
     // guarded predicted start: the degree drops by one after an abandoned
     // attempt and climbs back after 4 accepted ones; an abandoned degree-1
@@ -1486,9 +1513,11 @@ extern "C" int sts_time_step_mixed(sts_ctx *c, const sts_loop_control *lc, const
           c->pred_pause = c->pred_backoff;
           c->pred_backoff = std::min(2 * c->pred_backoff, mc->guard);
         }
+        corrector_outcome(c, 0, true);
         if ((rc = restart_from_previous(c))) return rc;
         plain = true;
       } else {
+        corrector_outcome(c, n32 + n64, false);
         c->pred_backoff = 1;
         if (++c->pred_wins >= 4 && c->pred_degree < mc->extrapolate) {
           c->pred_degree++;
@@ -1588,6 +1617,8 @@ extern "C" int sts_set_corrector_pause(sts_ctx *c, int max_pause) {
   c->corr_max_pause = max_pause;
   c->corr_pause = 0;
   c->corr_backoff = 1;
+  c->corr_kept_now = false;
+  c->corr_ref_mean = -1.0;
   return STS_OK;
 }
 
