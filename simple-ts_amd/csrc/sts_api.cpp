@@ -924,6 +924,7 @@ static AASel anderson_select(const sts_ctx *c, int k, int m, int every) {
   const int K = (k + 1) / every - 1, n = std::min(K, m);
   AASel sel{};
   sel.n = n;
+  sel.keep_dF = m > 1;
   for (int j = 0; j < n; j++) {
     const int sl = (K - n + j) % m;
     sel.slot[j] = sl;
@@ -1952,6 +1953,7 @@ static int run_batch_loop2(sts_batch *b, BatchCase<R> *C, int n_pc, int N_I, int
           const int K = (k + 1) / aa_every - 1;
           AABatchSel bs{};
           bs.n = std::min(K, aa_m);
+          bs.keep_dF = aa_m > 1;
           for (int j = 0; j < bs.n; j++) bs.slot[j] = (K - bs.n + j) % aa_m;
           HIP_OK(launch_batch_aa(BKA_STORE, cv, C, n, L.cell, L.n_cell, k, par, bs, s));
           if (bs.n > 0) {

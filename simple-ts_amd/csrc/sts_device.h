@@ -184,6 +184,7 @@ constexpr int MAX_AA = 8;
 // Gram matrix) and buffers.
 struct AASel {
   int n;
+  bool keep_dF;  // the newest dF is read by later stores (history depth > 1)
   int slot[MAX_AA];
   float *dG[MAX_AA];
   float *dF[MAX_AA];
@@ -235,6 +236,7 @@ struct BatchCase {
 // step together, so their rings rotate alike): count and ring slots.
 struct AABatchSel {
   int n;
+  bool keep_dF;  // as AASel::keep_dF
   int slot[MAX_AA];
 };
 

@@ -1747,8 +1747,9 @@ __global__ void __launch_bounds__(AA_THREADS) k_aa_begin(AAFieldsT<R> s, AABuf a
 // After an outer iteration: g = the state of cell k, x = gp + D its start,
 // f = g - x.  First iteration of the loop (sel.n = 0): the next start is g
 // (gp = g, D = 0), fp = f.  Later: the newest differences dF = f - fp,
-// dG = g - gp go to sel.dF/dG[n-1], fp = f, gp = g, and acc[j] +=
-// dF_j . dF_new, acc[NM + j] += dF_j . f for every difference in use.
+// dG = g - gp go to sel.dF/dG[n-1] (dF only if a later store reads it,
+// sel.keep_dF), fp = f, gp = g, and acc[j] += dF_j . dF_new,
+// acc[NM + j] += dF_j . f for every difference in use.
 // NM >= sel.n bounds the differences at compile time.
 template <int NM, typename R>
 __device__ __forceinline__ void aa_store_cell(const AAFieldsT<R> &s, const AABuf &ab, const AASel &sel, int k,
@@ -1782,7 +1783,7 @@ __device__ __forceinline__ void aa_store_cell(const AAFieldsT<R> &s, const AABuf
 #pragma unroll
     for (int j = 0; j < NM; j++)
       if (j == n - 1) {
-        store4(sel.dF[j] + e, dF);
+        if (sel.keep_dF) store4(sel.dF[j] + e, dF);
         store4(sel.dG[j] + e, dG);
 #pragma unroll
         for (int q = 0; q < 4; q++) o[j][q] = dF[q];
@@ -2040,8 +2041,12 @@ __device__ __forceinline__ void aa_mix_tile(const KGridT<R> &g, const KFieldsT<R
   fi.s10 = (R)ss[ty][tx + 1];
   fi.s01 = (R)ss[ty + 1][tx];
   fi.s11 = (R)ss[ty + 1][tx + 1];
-  fi.twu11 = FLD(TW_U)[nb.ij];
-  fi.twv11 = FLD(TW_V)[nb.ij];
+  // a wall temperature enters DTx (DTy) only where exactly one of the cell
+  // and its west (south) neighbour is fluid: with both fluid its term is
+  // 0 * sqrt(Tw), +0 for every wall temperature the state holds (finite,
+  // >= 0) as for 0, and with neither DTx is 0; those loads are left out
+  fi.twu11 = ci.mc == ci.mw ? R(0) : FLD(TW_U)[nb.ij];
+  fi.twv11 = ci.mc == ci.ms ? R(0) : FLD(TW_V)[nb.ij];
   fi.apply_flux(g, kf);
 }
 
@@ -2365,6 +2370,7 @@ __global__ void __launch_bounds__(TX *TY) kb_flux_diff(const BatchCase<R> *C, co
 __device__ __forceinline__ AASel case_sel(const BatchCase<double> &c, const AABatchSel &bs) {
   AASel sel;
   sel.n = bs.n;
+  sel.keep_dF = bs.keep_dF;
 #pragma unroll
   for (int j = 0; j < MAX_AA; j++) {
     sel.slot[j] = bs.slot[j];
