@@ -178,10 +178,14 @@ def run_python_path(steps, warmup, name, net=None, nx=NX, ny=NY, cfl=1.0):
         for s in range(warmup + steps):
             if s == warmup:
                 t0 = time.perf_counter()
+            # steps hand back no residual log, so they return without waiting
+            # for their last launches; the last warm-up step and the last timed
+            # step wait, so the timed region holds exactly the timed steps
+            wait = s in (warmup - 1, warmup + steps - 1)
             if PATHS[name] is None:
-                a, b = 0, run.ctx.time_step(*run.loop)[0]
+                a, b = 0, run.ctx.time_step(*run.loop, wait=wait)[0]
             else:
-                a, b, _, _ = run.ctx.time_step_mixed(*run.loop, n_i32, TOL32, stall_w, stall_r, **spec)
+                a, b, _, _ = run.ctx.time_step_mixed(*run.loop, n_i32, TOL32, stall_w, stall_r, **spec, wait=wait)
             if s >= warmup:
                 n32, n64 = n32 + a, n64 + b
         dt = time.perf_counter() - t0

@@ -185,7 +185,12 @@ typedef struct sts_loop_control {
    step.  *iterations gets Iter; res_log (may be NULL) gets 6 doubles per outer
    iteration: max|du|, max|dv|, max|dp|, max|dT| (last sweep), max|dp|,
    max|dT| (first sweep).  p_probe as in sts_boundary_conditions, for the last
-   outer iteration.  Returns STS_ERR_NONFINITE if a residual is not finite. */
+   outer iteration.  With res_log and p_probe both NULL the call returns as
+   soon as the iteration count is known, without waiting for the step's last
+   launches (the flux pass of its last iteration, iterations gated out past
+   the stop); later calls on the context queue behind them, and every call
+   that hands device data back waits for them.  Returns STS_ERR_NONFINITE if
+   a residual is not finite. */
 int sts_time_step(sts_ctx *ctx, const sts_loop_control *lc, int *iterations, double *res_log, double *p_probe);
 
 /* No counterpart in the reference: sts_time_step for callers that cannot
@@ -214,7 +219,9 @@ int sts_time_step_noalloc(sts_ctx *ctx, const sts_loop_control *lc, int *iterati
    2 continues in FP64 exactly as in sts_time_step: the reference's test on
    FP64 successive differences alone decides convergence, so the step ends at
    the same tolerance as the FP64 path.  res_log (room for 6*(N_I32+N_I)
-   doubles) gets 6 doubles per outer iteration, FP32 iterations first.
+   doubles) gets 6 doubles per outer iteration, FP32 iterations first;
+   res_log and p_probe may both be NULL, with the meaning they have in
+   sts_time_step.
    N_I32 = 0 gives sts_time_step. */
 typedef struct sts_mixed_control {
   int N_I32;                          /* cap on FP32 outer iterations */

@@ -310,11 +310,16 @@ class Context:
         _check(lib().sts_continuity(self.h, _dp(cs), _dp(cf), _dp(nr)), "sts_continuity")
         return cs, cf, nr
 
-    def time_step(self, N_I, N_I_p_c, tol_uv, tol_p, tol_T, batch=0):  # S:3230-4608
+    def time_step(self, N_I, N_I_p_c, tol_uv, tol_p, tol_T, batch=0, wait=True):  # S:3230-4608
         """One time step with loop 2 controlled on the GPU; returns (Iter,
-        residual log [Iter, 6], p probe)."""
+        residual log [Iter, 6], p probe).  wait=False: no log and no probe
+        (None, None), and the call returns without waiting for the step's
+        last launches (calls that read device data wait for them)."""
         lc = LoopControl(N_I, N_I_p_c, tol_uv, tol_p, tol_T, batch)
         it = ctypes.c_int()
+        if not wait:
+            _check(lib().sts_time_step(self.h, ctypes.byref(lc), ctypes.byref(it), None, None), "sts_time_step")
+            return it.value, None, None
         log, pr = np.zeros(6 * N_I), np.zeros(1)
         _check(lib().sts_time_step(self.h, ctypes.byref(lc), ctypes.byref(it), _dp(log), _dp(pr)), "sts_time_step")
         return it.value, log[:6 * it.value].reshape(-1, 6), float(pr[0])
@@ -330,7 +335,8 @@ class Context:
         return it.value, log[:6 * it.value].reshape(-1, 6), float(pr[0])
 
     def time_step_mixed(self, N_I, N_I_p_c, tol_uv, tol_p, tol_T, N_I32, tol32, stall_w=4, stall_ratio=0.7,
-                        batch=0, extrapolate=0, fp32_above=0, guard=0, anderson=0, anderson_every=1, anderson32=0):
+                        batch=0, extrapolate=0, fp32_above=0, guard=0, anderson=0, anderson_every=1, anderson32=0,
+                        wait=True):
         """Time step through sts_time_step_mixed: an FP32 phase (N_I32 > 0;
         needs precision=32), then the FP64 phase with the reference's test.
         tol32 = (uv, p, T) tolerances of the FP32 phase; extrapolate = degree
@@ -338,11 +344,15 @@ class Context:
         pause bound of predicted starts, anderson = history depth of the
         Anderson acceleration of the FP64 outer iterations (every
         anderson_every of them), anderson32 = the same for the FP32 phase.  Returns (FP32 iterations, FP64 iterations,
-        residual log [n32 + n64, 6], p probe)."""
+        residual log [n32 + n64, 6], p probe); wait=False as in time_step."""
         lc = LoopControl(N_I, N_I_p_c, tol_uv, tol_p, tol_T, batch)
         mc = MixedControl(N_I32, tol32[0], tol32[1], tol32[2], stall_w, stall_ratio, extrapolate, fp32_above, guard,
                           anderson, anderson_every, anderson32)
         n32, n64 = ctypes.c_int(), ctypes.c_int()
+        if not wait:
+            _check(lib().sts_time_step_mixed(self.h, ctypes.byref(lc), ctypes.byref(mc), ctypes.byref(n32),
+                                             ctypes.byref(n64), None, None), "sts_time_step_mixed")
+            return n32.value, n64.value, None, None
         log, pr = np.zeros(6 * (N_I + N_I32)), np.zeros(1)
         _check(lib().sts_time_step_mixed(self.h, ctypes.byref(lc), ctypes.byref(mc), ctypes.byref(n32),
                                          ctypes.byref(n64), _dp(log), _dp(pr)), "sts_time_step_mixed")

@@ -1124,7 +1124,12 @@ static int ensure_loop_buffers(sts_ctx *c, int n_iter) {
 }
 
 // Copies n_iter iterations of the residual log and the probe to the host.
+// With neither asked for the step returns without waiting for its last
+// launches (the loop's flux pass, iterations gated out past the stop): the
+// next step's launches queue behind them on the context's stream, and every
+// call that hands device data back waits for the stream.
 static int finish_step(sts_ctx *c, int n_iter, double *res_log, double *p_probe) {
+  if (!res_log && !p_probe) return STS_OK;
   const size_t nw = 6 * (size_t)n_iter;  // n_iter <= log_cap
   if (n_iter && res_log)
     HIP_OK(hipMemcpyAsync(c->h_log, c->log, sizeof(unsigned long long) * nw, hipMemcpyDeviceToHost, c->stream));

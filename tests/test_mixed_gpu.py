@@ -328,6 +328,35 @@ def test_batch_size_does_not_change_the_step(cap, tmp_path):
             assert np.array_equal(a, b), batch
 
 
+def test_steps_without_log_return_early_and_match(tmp_path):
+    """A step asked for neither the residual log nor the probe returns
+    without waiting for its last launches (wait=False).  Steps run that way
+    -- the plain and the accelerated entry points, and the mixed one with a
+    guarded predicted start -- give the same iteration counts and, read
+    after them, the same fields bit for bit as steps that wait."""
+    cases = importlib.import_module("simple-ts_amd.cases")
+    ws = importlib.import_module("simple-ts_amd.warmstart")
+    d = cases.confined_square(str(tmp_path / "c"), 120, 24, 1415, 0.00283, 8, dt_over_tin=0.01)
+    res = []
+    for wait in (True, False):
+        r = ws.Run(d, 32)
+        its = []
+        for _ in range(2):
+            its.append(r.ctx.time_step(*r.loop[:5], wait=wait)[0])
+        for _ in range(3):
+            its.append(r.ctx.time_step_mixed(*r.loop[:5], 0, (1e-7,) * 3, extrapolate=2, anderson=1, wait=wait)[1])
+        for _ in range(3):
+            n32, n64, _, _ = r.ctx.time_step_mixed(*r.loop[:5], 6, (1e-5,) * 3, 0, 0.95, extrapolate=3, guard=64,
+                                                   anderson=1, anderson32=1, wait=wait)
+            its.append((n32, n64))
+        res.append((its, [r.ctx.get(f) for f in ("u", "v", "p", "Temper")]))
+        print(f"wait {wait}: iterations {its}", flush=True)
+        r.close()
+    assert res[0][0] == res[1][0]
+    for a, b in zip(res[0][1], res[1][1]):
+        assert np.array_equal(a, b)
+
+
 def test_mixed_defect_correction_on_the_fine_grid(tmp_path):
     """The mixed-precision configuration on its own grid: the supersonic
     square on 3840 x 768 (half the benchmark's Courant number, where the
