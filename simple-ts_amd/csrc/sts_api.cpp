@@ -918,6 +918,12 @@ struct Loop2State {
   bool inline0 = false;        // the first iteration may form the diffusion coefficients itself
 };
 
+// How the store after the K-th block of an Anderson loop of history depth m
+// finds the block's start: from D (m > 1), as gp (no mix since the last
+// store: K < 2), or as the last mix formed again (m = 1), whose D the mix
+// then does not write.
+static int aa_start(int K, int m) { return m > 1 ? AA_START_D : K < 2 ? AA_START_GP : AA_START_MIX; }
+
 // Anderson acceleration (history depth m, a step every `every` outer
 // iterations): the differences in use after iteration k's store.
 static AASel anderson_select(const sts_ctx *c, int k, int m, int every) {
@@ -925,6 +931,8 @@ static AASel anderson_select(const sts_ctx *c, int k, int m, int every) {
   AASel sel{};
   sel.n = n;
   sel.keep_dF = m > 1;
+  sel.start = aa_start(K, m);
+  sel.write_D = m > 1;
   for (int j = 0; j < n; j++) {
     const int sl = (K - n + j) % m;
     sel.slot[j] = sl;
@@ -1959,6 +1967,8 @@ static int run_batch_loop2(sts_batch *b, BatchCase<R> *C, int n_pc, int N_I, int
           AABatchSel bs{};
           bs.n = std::min(K, aa_m);
           bs.keep_dF = aa_m > 1;
+          bs.start = aa_start(K, aa_m);
+          bs.write_D = aa_m > 1;
           for (int j = 0; j < bs.n; j++) bs.slot[j] = (K - bs.n + j) % aa_m;
           HIP_OK(launch_batch_aa(BKA_STORE, cv, C, n, L.cell, L.n_cell, k, par, bs, s));
           if (bs.n > 0) {

@@ -182,9 +182,16 @@ struct ExtrapHist {
 constexpr int MAX_AA = 8;
 // The differences in use, oldest first, newest last: ring slots (rows of the
 // Gram matrix) and buffers.
+enum { AA_START_D = 0, AA_START_GP = 1, AA_START_MIX = 2 };
 struct AASel {
   int n;
   bool keep_dF;  // the newest dF is read by later stores (history depth > 1)
+  // where the store finds the start x of the iterations it follows:
+  // AA_START_D x = gp + D; AA_START_GP x = gp (no mix since the last store);
+  // AA_START_MIX (history depth 1) x = the last mix of gp, formed again from
+  // gp, gam[0] and dG[0] before the store overwrites dG[0]
+  int start;
+  bool write_D;  // the mix writes D (AA_START_D)
   int slot[MAX_AA];
   float *dG[MAX_AA];
   float *dF[MAX_AA];
@@ -236,7 +243,9 @@ struct BatchCase {
 // step together, so their rings rotate alike): count and ring slots.
 struct AABatchSel {
   int n;
-  bool keep_dF;  // as AASel::keep_dF
+  bool keep_dF;  // as AASel::keep_dF, start and write_D
+  int start;
+  bool write_D;
   int slot[MAX_AA];
 };
 

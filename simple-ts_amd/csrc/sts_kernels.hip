@@ -1732,6 +1732,13 @@ __device__ __forceinline__ void store4(float *p, const double (&v)[4]) {
   *reinterpret_cast<float4 *>(p) = make_float4((float)v[0], (float)v[1], (float)v[2], (float)v[3]);
 }
 
+// A cell's mixed value v of component q (0..3: u, v, p, T) from the newest
+// iterate gp: v itself, or gp if v is not finite or is a p or T that is not
+// positive where gp's is.
+__device__ __forceinline__ double aa_keep(double v, double gp, int q) {
+  return (!isfinite(v) || (q >= 2 && !(v > 0.0) && gp > 0.0)) ? gp : v;
+}
+
 // Loop start: the start x = gp + D with gp = the state, D = 0.  The state
 // may be FP32 (the FP32 phase); gp holds it widened, exactly.
 template <typename R>
@@ -1744,8 +1751,9 @@ __global__ void __launch_bounds__(AA_THREADS) k_aa_begin(AAFieldsT<R> s, AABuf a
   store4(ab.D + 4 * (size_t)k, z);
 }
 
-// After an outer iteration: g = the state of cell k, x = gp + D its start,
-// f = g - x.  First iteration of the loop (sel.n = 0): the next start is g
+// After an outer iteration: g = the state of cell k, x its start (sel.start:
+// gp + D, gp, or the last mix formed again exactly as aa_mix_tile formed it,
+// rounded to R), f = g - x.  First iteration of the loop (sel.n = 0): the next start is g
 // (gp = g, D = 0), fp = f.  Later: the newest differences dF = f - fp,
 // dG = g - gp go to sel.dF/dG[n-1] (dF only if a later store reads it,
 // sel.keep_dF), fp = f, gp = g, and acc[j] += dF_j . dF_new,
@@ -1756,11 +1764,12 @@ __device__ __forceinline__ void aa_store_cell(const AAFieldsT<R> &s, const AABuf
                                               double (&acc)[2 * NM]) {
   const int n = sel.n;
   const size_t e = 4 * (size_t)k;
-  double g[4], gp[4], D[4], fp[4], o[NM][4];
+  double g[4], gp[4], D[4] = {0.0, 0.0, 0.0, 0.0}, fp[4], o[NM][4];
 #pragma unroll
   for (int q = 0; q < 4; q++) g[q] = (double)s.x[q][k];
   load4(ab.gp + e, gp);
-  load4(ab.D + e, D);
+  if (sel.start == AA_START_D) load4(ab.D + e, D);
+  else if (sel.start == AA_START_MIX) load4(sel.dG[0] + e, D);  // the last mix's difference (depth 1)
   if (n > 0) load4(ab.fp + e, fp);
 #pragma unroll
   for (int j = 0; j < NM; j++) {
@@ -1769,11 +1778,21 @@ __device__ __forceinline__ void aa_store_cell(const AAFieldsT<R> &s, const AABuf
     if (j < n - 1) load4(sel.dF[j] + e, o[j]);
   }
   double f[4], dF[4], dG[4];
+  if (sel.start == AA_START_MIX) {
+    const double gam0 = ab.gam[0];
 #pragma unroll
-  for (int q = 0; q < 4; q++) f[q] = g[q] - (gp[q] + D[q]);
+    for (int q = 0; q < 4; q++) {
+      double v = gp[q];
+      v -= gam0 * D[q];
+      f[q] = g[q] - (double)(R)aa_keep(v, gp[q], q);
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 4; q++) f[q] = g[q] - (sel.start == AA_START_D ? gp[q] + D[q] : gp[q]);
+  }
   if (n == 0) {
     const double z[4] = {0.0, 0.0, 0.0, 0.0};
-    store4(ab.D + e, z);
+    if (sel.write_D) store4(ab.D + e, z);
   } else {
 #pragma unroll
     for (int q = 0; q < 4; q++) {
@@ -1936,8 +1955,7 @@ __device__ __forceinline__ void aa_mixed(const AABuf &ab, const AASel &sel, cons
 #pragma unroll
     for (int j = 0; j < NM; j++)
       if (j < sel.n) v -= gam[j] * dG[j][q];
-    if (!isfinite(v) || (q >= 2 && !(v > 0.0) && gp[q] > 0.0)) v = gp[q];
-    x[q] = v;
+    x[q] = aa_keep(v, gp[q], q);
   }
 }
 
@@ -2014,7 +2032,7 @@ __device__ __forceinline__ void aa_mix_tile(const KGridT<R> &g, const KFieldsT<R
     s.x[q][nb.ij] = (R)x[q];
     d[q] = x[q] - gp[q];
   }
-  store4(ab.D + 4 * (size_t)nb.ij, d);
+  if (sel.write_D) store4(ab.D + 4 * (size_t)nb.ij, d);
   KFieldsT<R> kf = fl;
   kf.f[STS_P] = s.x[2];
   kf.f[STS_T] = s.x[3];
@@ -2371,6 +2389,8 @@ __device__ __forceinline__ AASel case_sel(const BatchCase<double> &c, const AABa
   AASel sel;
   sel.n = bs.n;
   sel.keep_dF = bs.keep_dF;
+  sel.start = bs.start;
+  sel.write_D = bs.write_D;
 #pragma unroll
   for (int j = 0; j < MAX_AA; j++) {
     sel.slot[j] = bs.slot[j];
