@@ -2615,7 +2615,10 @@ template <typename R>
 hipError_t launch_aa_store(const AAFieldsT<R> &f, const AABuf &ab, const AASel &sel, int Na, hipStream_t s,
                            Gate gate) {
   if (sel.n < 0 || sel.n > MAX_AA || ab.nb != aa_store_blocks(Na)) return hipErrorInvalidValue;
-  if (sel.n <= 4) hipLaunchKernelGGL((k_aa_store<4, R>), dim3(ab.nb), dim3(AA_THREADS), 0, s, f, ab, sel, Na, gate);
+  // one difference (history depth 1, or the loop's first steps): the
+  // one-difference instantiation holds far fewer registers, same sums
+  if (sel.n <= 1) hipLaunchKernelGGL((k_aa_store<1, R>), dim3(ab.nb), dim3(AA_THREADS), 0, s, f, ab, sel, Na, gate);
+  else if (sel.n <= 4) hipLaunchKernelGGL((k_aa_store<4, R>), dim3(ab.nb), dim3(AA_THREADS), 0, s, f, ab, sel, Na, gate);
   else hipLaunchKernelGGL((k_aa_store<MAX_AA, R>), dim3(ab.nb), dim3(AA_THREADS), 0, s, f, ab, sel, Na, gate);
   return hipGetLastError();
 }
@@ -2626,7 +2629,9 @@ hipError_t launch_aa_mix(const KGridT<R> &g, const KFieldsT<R> &fl, const AAFiel
   if (sel.n < 1 || sel.n > MAX_AA || ab.nb != aa_store_blocks(Na)) return hipErrorInvalidValue;
   const int ntiles = ((g.Nx + AA_MIX_TX - 1) / AA_MIX_TX) * ((g.Ny + AA_MIX_TY - 1) / AA_MIX_TY);
   const dim3 gm((ntiles + AA_MIX_TILES - 1) / AA_MIX_TILES);
-  if (sel.n <= 4)
+  if (sel.n <= 1)
+    hipLaunchKernelGGL((k_aa_mix<1, R>), gm, dim3(AA_MIX_TX * AA_MIX_TY), 0, s, g, fl, f, ab, sel, gate);
+  else if (sel.n <= 4)
     hipLaunchKernelGGL((k_aa_mix<4, R>), gm, dim3(AA_MIX_TX * AA_MIX_TY), 0, s, g, fl, f, ab, sel, gate);
   else hipLaunchKernelGGL((k_aa_mix<MAX_AA, R>), gm, dim3(AA_MIX_TX * AA_MIX_TY), 0, s, g, fl, f, ab, sel, gate);
   return hipGetLastError();
