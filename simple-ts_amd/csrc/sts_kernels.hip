@@ -509,15 +509,23 @@ __device__ __forceinline__ FaceDiff<R> face_diff_tile(const KGridT<R> &g, const 
 // coefficients formed in the workgroup: each face of the tile once, plus the
 // faces its cells read beyond it (column TX: the i1j face of each row's last
 // cell, row TY: the ij1 face of each column's last cell, as the clamped or
-// wrapped neighbour indices give them), kept in LDS; then every cell
-// assembles from them.  (tx, ty): the thread's cell in the tile; the
-// workgroup has TX * TY >= TX + TY + 4 threads.
+// wrapped neighbour indices give them); then every cell assembles from them.
+// A cell reads the y-face values (Gamma_y, D_uy, D_vy) of its own face and the
+// north one, kept in LDS, and the x-face values (Gamma_x, D_ux, D_vx) of its
+// own face and the east one: when a tile row lies within one wave (64 % TX
+// == 0) the east cell's come from the next lane and only the ring column's
+// from LDS, which leaves room in LDS for a fifth workgroup per CU at 64 x 4;
+// otherwise they are all kept in LDS.  (tx, ty): the thread's cell in the
+// tile; the workgroup has TX * TY >= TX + TY + 4 threads, the linear thread
+// index tx + TX ty.
 template <typename R, int TX, int TY, bool HT1 = false>
 __device__ __forceinline__ void assemble_inl_tile(const KGridT<R> &g, const KFieldsT<R> &fl, int i0, int j0, int tx,
                                                   int ty) {
   static_assert(TX * TY >= TX + TY + 4, "the face ring and the mesh steps need TX + TY + 4 threads");
-  __shared__ R Lgy[TY + 1][TX + 1], Lgx[TY + 1][TX + 1], Lux[TY + 1][TX + 1], Luy[TY + 1][TX + 1],
-      Lvx[TY + 1][TX + 1], Lvy[TY + 1][TX + 1];
+  constexpr bool XL = 64 % TX == 0;      // x-face values of the east cell by lane exchange
+  constexpr int LXW = XL ? 1 : TX + 1;  // x-face arrays: the ring column only, or every face
+  constexpr int LXR = XL ? 0 : TX;      // their ring column
+  __shared__ R Lgy[TY + 1][TX], Luy[TY + 1][TX], Lvy[TY + 1][TX], Lgx[TY][LXW], Lux[TY][LXW], Lvx[TY][LXW];
   __shared__ R SFx[TY + 2][TX + 2], SFy[TY + 2][TX + 2], Su[TY + 2][TX + 2], Sv[TY + 2][TX + 2],
       SsT[TY + 2][TX + 2], Srho[TY + 2][TX + 2], Srho_pr[TY + 2][TX + 2], Shx[TX + 2], Shy[TY + 2];
   const int Nx = g.Nx, Ny = g.Ny, t = tx + ty * TX;
@@ -554,25 +562,50 @@ __device__ __forceinline__ void assemble_inl_tile(const KGridT<R> &g, const KFie
   const bool in = i < Nx && j < Ny;
   const unsigned ij = (unsigned)i + (unsigned)j * (unsigned)Nx, nb = in ? g.nb[ij] : 0u;
   __syncthreads();
-  auto put = [&](int y, int x, unsigned fi, unsigned fj, unsigned b, int cx, int cy) {
-    const FaceDiff<R> d = face_diff_tile<R, TX>(g, fl, fi, fj, b, SsT, Shx, Shy, cx, cy);
-    Lgy[y][x] = d.Gy;
-    Lgx[y][x] = d.Gx;
-    Lux[y][x] = d.Dux;
-    Luy[y][x] = d.Duy;
-    Lvx[y][x] = d.Dvx;
-    Lvy[y][x] = d.Dvy;
-  };
-  if (in) put(ty, tx, (unsigned)i, (unsigned)j, nb, tx, ty);
-  if (t < TX) {  // north: the ij1 face of column t's last cell (jl itself on the grid's top row)
+  // the cell's own face
+  R gx = R(0), ux = R(0), vx = R(0);
+  if (in) {
+    const FaceDiff<R> d = face_diff_tile<R, TX>(g, fl, (unsigned)i, (unsigned)j, nb, SsT, Shx, Shy, tx, ty);
+    Lgy[ty][tx] = d.Gy;
+    Luy[ty][tx] = d.Duy;
+    Lvy[ty][tx] = d.Dvy;
+    gx = d.Gx;
+    ux = d.Dux;
+    vx = d.Dvx;
+    if constexpr (!XL) {
+      Lgx[ty][tx] = gx;
+      Lux[ty][tx] = ux;
+      Lvx[ty][tx] = vx;
+    }
+  }
+  if (t < TX) {  // north: the ij1 face of column t's last cell (jl itself on the grid's top row); its y-face values
     const int fj = jl + (jl < Ny - 1 ? 1 : 0);
-    if (i0 + t < Nx) put(TY, t, (unsigned)(i0 + t), (unsigned)fj, g.nb[(unsigned)(i0 + t) + (unsigned)fj * (unsigned)Nx],
-                         t, fj - j0);
-  } else if (t < TX + TY) {  // east: the i1j face of row t - TX's last cell (Nbr's i1 of il)
+    if (i0 + t < Nx) {
+      const FaceDiff<R> d = face_diff_tile<R, TX>(g, fl, (unsigned)(i0 + t), (unsigned)fj,
+                                                  g.nb[(unsigned)(i0 + t) + (unsigned)fj * (unsigned)Nx], SsT, Shx,
+                                                  Shy, t, fj - j0);
+      Lgy[TY][t] = d.Gy;
+      Luy[TY][t] = d.Duy;
+      Lvy[TY][t] = d.Dvy;
+    }
+  } else if (t < TX + TY) {  // east: the i1j face of row t - TX's last cell (Nbr's i1 of il); its x-face values
     const int r = t - TX;
     const int fi = g.px ? (il == Nx - 1 ? 0 : il + 1) : il + (il < Nx - 1 ? 1 : 0);
-    if (j0 + r < Ny) put(r, TX, (unsigned)fi, (unsigned)(j0 + r), g.nb[(unsigned)fi + (unsigned)(j0 + r) * (unsigned)Nx],
-                         il - i0 + (fi != il ? 1 : 0), r);
+    if (j0 + r < Ny) {
+      const FaceDiff<R> d = face_diff_tile<R, TX>(g, fl, (unsigned)fi, (unsigned)(j0 + r),
+                                                  g.nb[(unsigned)fi + (unsigned)(j0 + r) * (unsigned)Nx], SsT, Shx,
+                                                  Shy, il - i0 + (fi != il ? 1 : 0), r);
+      Lgx[r][LXR] = d.Gx;
+      Lux[r][LXR] = d.Dux;
+      Lvx[r][LXR] = d.Dvx;
+    }
+  }
+  // the next lane's own x-face values (lanes of one wave, every lane active)
+  R egx = R(0), eux = R(0), evx = R(0);
+  if constexpr (XL) {
+    egx = __shfl_down(gx, 1);
+    eux = __shfl_down(ux, 1);
+    evx = __shfl_down(vx, 1);
   }
   __syncthreads();
   if (!in) return;
@@ -582,9 +615,13 @@ __device__ __forceinline__ void assemble_inl_tile(const KGridT<R> &g, const KFie
   const bool dv = in_box(i, j, g.Nx_v0, g.Nx_v1, g.Ny_v0, g.Ny_v1) && (nb & NB_S) && c;
   const TileStencil<R, TX, TY, HT1> S{SFx, SFy, Su, Sv, SsT, Srho, Srho_pr, Shx, Shy, tx, ty};
   if (du || dv) {
-    const int ex = (i == il) ? TX : tx + 1, ny = (j == jl) ? TY : ty + 1;
-    const CellDiff<R> df{Lgy[ty][tx], Lgy[ny][tx], Lgx[ty][tx], Lgx[ty][ex], Lux[ty][tx], Lux[ty][ex],
-                         Luy[ty][tx], Luy[ny][tx], Lvx[ty][tx], Lvx[ty][ex], Lvy[ty][tx], Lvy[ny][tx]};
+    const int ny = (j == jl) ? TY : ty + 1;
+    const bool last = i == il;  // the east face is the ring column's
+    const R gx1 = last ? Lgx[ty][LXR] : XL ? egx : Lgx[ty][tx + 1];
+    const R ux1 = last ? Lux[ty][LXR] : XL ? eux : Lux[ty][tx + 1];
+    const R vx1 = last ? Lvx[ty][LXR] : XL ? evx : Lvx[ty][tx + 1];
+    const CellDiff<R> df{Lgy[ty][tx], Lgy[ny][tx], gx, gx1, ux, ux1,
+                         Luy[ty][tx], Luy[ny][tx], vx, vx1, Lvy[ty][tx], Lvy[ny][tx]};
     if (du) cell_assemble_u(g, fl, n, df, nb, S);
     if (dv) cell_assemble_v(g, fl, n, df, nb, S);
   }
