@@ -514,7 +514,7 @@ __device__ __forceinline__ FaceDiff<R> face_diff_tile(const KGridT<R> &g, const 
 // north one, kept in LDS, and the x-face values (Gamma_x, D_ux, D_vx) of its
 // own face and the east one: when a tile row lies within one wave (64 % TX
 // == 0) the east cell's come from the next lane and only the ring column's
-// from LDS, which leaves room in LDS for a fifth workgroup per CU at 64 x 4;
+// from LDS (a quarter less LDS per workgroup, so more workgroups per CU);
 // otherwise they are all kept in LDS.  (tx, ty): the thread's cell in the
 // tile; the workgroup has TX * TY >= TX + TY + 4 threads, the linear thread
 // index tx + TX ty.
@@ -628,12 +628,13 @@ __device__ __forceinline__ void assemble_inl_tile(const KGridT<R> &g, const KFie
   if (in_box(i, j, g.Nx_o0, g.Nx_o1, g.Ny_o0, g.Ny_o1) && c) cell_energy_source(g, fl, n, nb, S);
 }
 
-// Tile of the loops' assembly.
-constexpr int AX = 64, AY = 4;
+// Tile of the loops' assembly: 64 x 8 cells at six waves per SIMD (78
+// registers; 51.6 KB of LDS, three workgroups per CU).
+constexpr int AX = 64, AY = 8;
 
 // HT1: launched for grids with rht_one.
 template <typename R, bool HT1>
-__global__ void __launch_bounds__(AX *AY) k_assemble_inl(KGridT<R> g, KFieldsT<R> fl, Gate gate) {
+__global__ void __launch_bounds__(AX *AY) __attribute__((amdgpu_waves_per_eu(6, 8))) k_assemble_inl(KGridT<R> g, KFieldsT<R> fl, Gate gate) {
   if (gated_out(gate)) return;
   int i0, j0;
   tile_origin<AX, AY>(g.Nx, i0, j0);
