@@ -105,7 +105,7 @@ TOL32 = (1e-7, 1e-7, 1e-7)  # FP32-phase tolerances
 DEGREE = 2        # predicted start: polynomial degree in time
 GUARD = 64        # predicted start: restart from x^n past the budget, pauses up to this many steps
 N_I32_MIXED = 6   # mixed: FP32 outer iterations (accelerated) before the FP64 phase
-MIX_FP32_ABOVE = 14  # mixed: the FP32 phase runs after steps of more than this many outer iterations
+MIX_FP32_ABOVE = 17  # mixed: the FP32 phase runs after steps of more than this many outer iterations
 AA_DEPTH = 1      # accelerated: Anderson history depth (with the predicted start)
 AA_DEPTH_FULL = 2  # Anderson depth without a predicted start and in the batch (from-rest transients)
 AA_DEGREE = 3     # accelerated: predicted-start degree
