@@ -153,3 +153,43 @@ def test_trained_corrector_on_held_out_case(tmp_path):
     for k in fa:
         e = np.linalg.norm(fa[k] - fb[k]) / np.linalg.norm(fb[k])
         assert e < 1e-5, (k, e)
+
+
+@pytest.mark.skipif(not os.path.exists(WEIGHTS), reason="no trained corrector")
+def test_trained_corrector_with_back_off(tmp_path):
+    """The committed corrector with a back-off of at most 8 predicted steps on
+    the held-out case: a refinement that is rejected, or kept while its step
+    needs no fewer outer iterations than the running mean of the predicted
+    steps without one, makes the corrector sit out the next 1, 2, 4, 8
+    predicted steps.  Every predicted step is tried, skipped or neither
+    exactly once (kept + rejected + skipped = predicted), the back-off skips
+    some, and every step still ends on the reference's test (fields within
+    1e-5 relative L2 of the method without the network)."""
+    pkg, ws = mods()
+    z = np.load(WEIGHTS, allow_pickle=False)
+    prm = ws.case_params(np.random.default_rng(12345))
+    steps = 40
+    runs = []
+    for with_net in (False, True):
+        r = ws.Run(ws.write_case(str(tmp_path / f"n{int(with_net)}"), 512, 128, prm, steps))
+        if with_net:
+            net = pkg.UNet(int(z["cin"]), int(z["cout"]), int(z["base"]), 1, 128, 512)
+            net.set_params(z["params"])
+            r.ctx.set_corrector(net, ws.corr_norm(r.case))
+            r.ctx.set_corrector_pause(8)
+        for _ in range(steps):
+            ws.accelerated_step(r)
+        runs.append(r)
+    kept, rejected = runs[1].ctx.corrector_stats()
+    skipped = runs[1].ctx.corrector_skipped()
+    predicted = runs[1].ctx.predict_stats()[0]
+    print(f"kept {kept}, rejected {rejected}, skipped {skipped} of {predicted} predicted steps", flush=True)
+    assert kept + rejected + skipped == predicted > 0
+    assert skipped > 0
+    fa, fb = fields(runs[1].ctx), fields(runs[0].ctx)
+    for r in runs:
+        r.close()
+    net.close()
+    for k in fa:
+        e = np.linalg.norm(fa[k] - fb[k]) / np.linalg.norm(fb[k])
+        assert e < 1e-5, (k, e)
