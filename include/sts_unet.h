@@ -101,9 +101,9 @@ int sts_corrector_stats(sts_ctx *ctx, long long out[2]);
    the next 1, 2, 4, ... predicted steps (at most max_pause; back to 1 after
    one that pays), so a network that does not help costs little; those steps
    run from the extrapolated start as without a corrector.  A refinement
-   pays when it is kept and its step then needs fewer outer iterations than
-   the running mean (weights 1/2, 1/4, ...) of the predicted steps that ran
-   without one; a rejected one, or one whose predicted attempt is abandoned,
+   pays when it is kept and its step then needs at least one outer
+   iteration fewer than the running mean (weights 1/2, 1/4, ...) of the
+   predicted steps that ran without one; a rejected one, or one whose predicted attempt is abandoned,
    never pays.  0 (the default of sts_set_corrector): every predicted step
    is refined and tested. */
 int sts_set_corrector_pause(sts_ctx *ctx, int max_pause);

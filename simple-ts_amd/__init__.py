@@ -394,8 +394,8 @@ class Context:
 
     def set_corrector_pause(self, max_pause):
         """After a refinement that does not pay (rejected, or kept and its
-        step still needs at least the running mean of the outer iterations
-        of predicted steps without one) the corrector sits out 1, 2, 4, ...
+        step does not need at least one outer iteration fewer than the
+        running mean of predicted steps without one) the corrector sits out 1, 2, 4, ...
         predicted steps, at most max_pause (0: every predicted step)."""
         _check(lib().sts_set_corrector_pause(self.h, int(max_pause)), "sts_set_corrector_pause")
 

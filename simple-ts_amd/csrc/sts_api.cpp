@@ -100,9 +100,9 @@ struct sts_ctx {
   // pause after a refinement that did not pay (sts_set_corrector_pause): the
   // next corr_pause predicted steps skip the corrector; corr_backoff doubles
   // up to corr_max_pause on each further one and returns to 1 when one pays.
-  // A refinement pays when it is kept and its step then needs fewer outer
-  // iterations than corr_ref_mean, the running mean of the predicted steps
-  // run without one (< 0: none yet).
+  // A refinement pays when it is kept and its step then needs at least one
+  // outer iteration fewer than corr_ref_mean, the running mean of the
+  // predicted steps run without one (< 0: none yet).
   int corr_max_pause = 0, corr_pause = 0, corr_backoff = 1;
   bool corr_kept_now = false;  // this step's start is a kept refinement
   double corr_ref_mean = -1.0;
@@ -1383,15 +1383,17 @@ static int corrector_refine(sts_ctx *c, const sts_loop_control *lc) {
 }
 
 // After a predicted step of a context with a corrector and a back-off: a
-// kept refinement pays when the step needed fewer outer iterations (all
-// phases) than the running mean of the predicted steps run without one;
-// one that does not pay backs off like a rejected one.  restarted: the
+// kept refinement pays when the step needed at least one outer iteration
+// (all phases) fewer than the running mean of the predicted steps run
+// without one -- a smaller difference is within the step-to-step variation
+// of the counts, which fall through a transient anyway; one that does not
+// pay backs off like a rejected one.  restarted: the
 // predicted attempt was abandoned (the step ran from x^n).
 static void corrector_outcome(sts_ctx *c, int total, bool restarted) {
   if (!c->corr_net || c->corr_max_pause <= 0) return;
   if (c->corr_kept_now) {
     c->corr_kept_now = false;
-    if (!restarted && (c->corr_ref_mean < 0.0 || total < c->corr_ref_mean)) {
+    if (!restarted && (c->corr_ref_mean < 0.0 || total <= c->corr_ref_mean - 1.0)) {
       c->corr_backoff = 1;
     } else {
       c->corr_pause = c->corr_backoff;

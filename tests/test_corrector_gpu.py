@@ -159,8 +159,8 @@ def test_trained_corrector_on_held_out_case(tmp_path):
 def test_trained_corrector_with_back_off(tmp_path):
     """The committed corrector with a back-off of at most 8 predicted steps on
     the held-out case: a refinement that is rejected, or kept while its step
-    needs no fewer outer iterations than the running mean of the predicted
-    steps without one, makes the corrector sit out the next 1, 2, 4, 8
+    does not need at least one outer iteration fewer than the running mean
+    of the predicted steps without one, makes the corrector sit out the next 1, 2, 4, 8
     predicted steps.  Every predicted step is tried, skipped or neither
     exactly once (kept + rejected + skipped = predicted), the back-off skips
     some, and every step still ends on the reference's test (fields within
