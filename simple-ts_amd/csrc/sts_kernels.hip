@@ -1998,8 +1998,9 @@ __device__ __forceinline__ void aa_mixed(const AABuf &ab, const AASel &sel, cons
 }
 
 // The next start x of a TX x TY tile (origin i0, j0; this thread tx, ty)
-// into the state arrays (and D = x - gp), and, from it, what the correction
-// pass derives without correcting: sqrt_T, rho and the face densities
+// into the state arrays (and D = x - gp when sel.write_D), and, from it,
+// what the correction pass derives without correcting: sqrt_T, rho and the
+// face densities
 // (CorrectIn::apply on the mixed p, T of the cell and of its west and south
 // neighbours); with FLUX also what the loop's flux pass writes (Fx, Fy, DTx,
 // DTy; FluxIn::apply_flux).  The mixed p, T of the tile and of its west
