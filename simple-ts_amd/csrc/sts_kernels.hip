@@ -2657,6 +2657,7 @@ hipError_t launch_aa_store(const AAFieldsT<R> &f, const AABuf &ab, const AASel &
   // one difference (history depth 1, or the loop's first steps): the
   // one-difference instantiation holds far fewer registers, same sums
   if (sel.n <= 1) hipLaunchKernelGGL((k_aa_store<1, R>), dim3(ab.nb), dim3(AA_THREADS), 0, s, f, ab, sel, Na, gate);
+  else if (sel.n <= 2) hipLaunchKernelGGL((k_aa_store<2, R>), dim3(ab.nb), dim3(AA_THREADS), 0, s, f, ab, sel, Na, gate);
   else if (sel.n <= 4) hipLaunchKernelGGL((k_aa_store<4, R>), dim3(ab.nb), dim3(AA_THREADS), 0, s, f, ab, sel, Na, gate);
   else hipLaunchKernelGGL((k_aa_store<MAX_AA, R>), dim3(ab.nb), dim3(AA_THREADS), 0, s, f, ab, sel, Na, gate);
   return hipGetLastError();
@@ -2670,6 +2671,8 @@ hipError_t launch_aa_mix(const KGridT<R> &g, const KFieldsT<R> &fl, const AAFiel
   const dim3 gm((ntiles + AA_MIX_TILES - 1) / AA_MIX_TILES);
   if (sel.n <= 1)
     hipLaunchKernelGGL((k_aa_mix<1, R>), gm, dim3(AA_MIX_TX * AA_MIX_TY), 0, s, g, fl, f, ab, sel, gate);
+  else if (sel.n <= 2)
+    hipLaunchKernelGGL((k_aa_mix<2, R>), gm, dim3(AA_MIX_TX * AA_MIX_TY), 0, s, g, fl, f, ab, sel, gate);
   else if (sel.n <= 4)
     hipLaunchKernelGGL((k_aa_mix<4, R>), gm, dim3(AA_MIX_TX * AA_MIX_TY), 0, s, g, fl, f, ab, sel, gate);
   else hipLaunchKernelGGL((k_aa_mix<MAX_AA, R>), gm, dim3(AA_MIX_TX * AA_MIX_TY), 0, s, g, fl, f, ab, sel, gate);
@@ -2761,7 +2764,9 @@ hipError_t launch_batch_aa(int kernel, int variant, const BatchCase<double> *C, 
                            int ntiles, int k, int par, const AABatchSel &bs, hipStream_t s) {
   if (bs.n < 0 || bs.n > MAX_AA || variant < 0 || variant >= NVARIANT) return hipErrorInvalidValue;
   if (kernel == BKA_SOLVE) {
-    if (bs.n <= 4) hipLaunchKernelGGL(kb_aa_solve<4>, dim3(ncases), dim3(AA_SOLVE_THREADS), 0, s, C, k, bs);
+    // the instantiation for the differences in use: the same sums, fewer registers
+    if (bs.n <= 2) hipLaunchKernelGGL(kb_aa_solve<2>, dim3(ncases), dim3(AA_SOLVE_THREADS), 0, s, C, k, bs);
+    else if (bs.n <= 4) hipLaunchKernelGGL(kb_aa_solve<4>, dim3(ncases), dim3(AA_SOLVE_THREADS), 0, s, C, k, bs);
     else hipLaunchKernelGGL(kb_aa_solve<MAX_AA>, dim3(ncases), dim3(AA_SOLVE_THREADS), 0, s, C, k, bs);
     return hipGetLastError();
   }
@@ -2774,10 +2779,12 @@ hipError_t launch_batch_aa(int kernel, int variant, const BatchCase<double> *C, 
       default: hipLaunchKernelGGL((kb_aa_begin<32, 8>), dim3(ntiles), dim3(256), 0, s, C, T); break;
     }
   } else if (kernel == BKA_STORE) {
-    if (bs.n <= 4) { AA_SWITCH(variant, 4, kb_aa_store, C, T, k, par, bs); }
+    if (bs.n <= 2) { AA_SWITCH(variant, 2, kb_aa_store, C, T, k, par, bs); }
+    else if (bs.n <= 4) { AA_SWITCH(variant, 4, kb_aa_store, C, T, k, par, bs); }
     else { AA_SWITCH(variant, MAX_AA, kb_aa_store, C, T, k, par, bs); }
   } else if (kernel == BKA_MIX) {
-    if (bs.n <= 4) { AA_SWITCH(variant, 4, kb_aa_mix, C, T, k, par, bs); }
+    if (bs.n <= 2) { AA_SWITCH(variant, 2, kb_aa_mix, C, T, k, par, bs); }
+    else if (bs.n <= 4) { AA_SWITCH(variant, 4, kb_aa_mix, C, T, k, par, bs); }
     else { AA_SWITCH(variant, MAX_AA, kb_aa_mix, C, T, k, par, bs); }
   } else {
     return hipErrorInvalidValue;
